@@ -1,0 +1,230 @@
+"""ctypes binding of libdcopasi.so (the C ABI declared in include/dcopasi.h).
+
+The library is the product: grid/compartment/BCSR set-up and the hipRTC
+assembly compiler on the host, the HIP kernels of assembly, SpMV, BiCGSTAB,
+block-Jacobi and ILU0 on the GPU.  This module only moves numpy arrays across
+the boundary; it has no numerical code of its own and no fallback: if the
+library is missing, loading fails with an error naming the build step.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "lib" / "libdcopasi.so"
+HEADER = _HERE.parent / "include" / "dcopasi.h"
+
+_lib = None
+
+
+class DcgError(RuntimeError):
+    """A call through the C ABI returned a negative status."""
+
+
+class StepStats(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int),
+        ("stages", ctypes.c_int),
+        ("newton_iterations", ctypes.c_int),
+        ("linear_iterations", ctypes.c_int),
+        ("refinement_steps", ctypes.c_int),
+        ("residual_norm", ctypes.c_double),
+    ]
+
+
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_I = ctypes.c_int
+_S = ctypes.c_size_t
+_SIGS = {
+    "dcg_model_create": (_I, [ctypes.c_char_p, ctypes.c_char_p, _I, ctypes.POINTER(_P)]),
+    "dcg_model_destroy": (None, [_P]),
+    "dcg_state_size": (_I, [_P, ctypes.POINTER(_S), ctypes.POINTER(_S), ctypes.POINTER(_S)]),
+    "dcg_set_state": (_I, [_P, _P, _S, _D]),
+    "dcg_get_state": (_I, [_P, _P, _S, ctypes.POINTER(_D)]),
+    "dcg_initial_state": (_I, [_P, _P, _S]),
+    "dcg_step": (_I, [_P, _D, _D, ctypes.POINTER(StepStats)]),
+    "dcg_mass": (_I, [_P, _P]),
+    "dcg_last_error": (ctypes.c_char_p, []),
+    "dcg_species_info": (_I, [_P, _S, ctypes.c_char_p, _S, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "dcg_compartment_info": (_I, [_P, _S, ctypes.c_char_p, _S, ctypes.POINTER(_S), ctypes.POINTER(_S),
+                                  ctypes.POINTER(_I)]),
+    "dcg_compartment_nodes": (_I, [_P, _S, _P]),
+    "dcg_pattern_info": (_I, [_P, ctypes.POINTER(_S), ctypes.POINTER(_S), ctypes.POINTER(_I)]),
+    "dcg_pattern": (_I, [_P, _P, _P]),
+    "dcg_generated_source": (_I, [_P, ctypes.c_char_p, _S, ctypes.POINTER(_S)]),
+    "dcg_assemble": (_I, [_P, _P, _D, _D, _D, _P, _P]),
+    "dcg_spmv": (_I, [_P, _P, _P, _P]),
+    "dcg_linear_solve": (_I, [_P, _P, _P, _P, _D, _I, ctypes.POINTER(_I)]),
+}
+
+
+def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
+    """Load libdcopasi.so (built by __graft_entry__.build()); raise if absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise FileNotFoundError(
+            f"{p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(make -C dune-copasi_amd/csrc)")
+    lib = ctypes.CDLL(str(p))
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _check(rc: int) -> int:
+    if rc < 0:
+        raise DcgError(load_library().dcg_last_error().decode())
+    return rc
+
+
+class Model:
+    """One model handle (dcg_model_t)."""
+
+    def __init__(self, ini: str, overrides: str = "", device: int = 0):
+        lib = load_library()
+        h = ctypes.c_void_p()
+        _check(lib.dcg_model_create(ini.encode(), overrides.encode(), device, ctypes.byref(h)))
+        self._h = h
+        self._lib = lib
+        self.device = device
+        nd, ns, nc = _S(), _S(), _S()
+        _check(lib.dcg_state_size(h, ctypes.byref(nd), ctypes.byref(ns), ctypes.byref(nc)))
+        self.n_dof, self.n_species, self.n_compartments = nd.value, ns.value, nc.value
+        sl, nb, bs = _S(), _S(), _I()
+        _check(lib.dcg_pattern_info(h, ctypes.byref(sl), ctypes.byref(nb), ctypes.byref(bs)))
+        self.n_slots, self.n_blocks, self.block_size = sl.value, nb.value, bs.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.dcg_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- layout
+    def species(self):
+        out = []
+        for s in range(self.n_species):
+            buf = ctypes.create_string_buffer(256)
+            c, l = _I(), _I()
+            _check(self._lib.dcg_species_info(self._h, s, buf, 256, ctypes.byref(c), ctypes.byref(l)))
+            out.append((buf.value.decode(), c.value, l.value))
+        return out
+
+    def compartments(self):
+        out = []
+        for c in range(self.n_compartments):
+            buf = ctypes.create_string_buffer(256)
+            nn, off, nsp = _S(), _S(), _I()
+            _check(self._lib.dcg_compartment_info(self._h, c, buf, 256, ctypes.byref(nn), ctypes.byref(off),
+                                                  ctypes.byref(nsp)))
+            out.append({"name": buf.value.decode(), "n_nodes": nn.value, "dof_offset": off.value,
+                        "n_species": nsp.value})
+        return out
+
+    def compartment_nodes(self, c: int) -> np.ndarray:
+        n = self.compartments()[c]["n_nodes"]
+        ids = np.zeros(n, dtype=np.int64)
+        _check(self._lib.dcg_compartment_nodes(self._h, c, _ptr(ids)))
+        return ids
+
+    def pattern(self):
+        rp = np.zeros(self.n_slots + 1, dtype=np.int32)
+        col = np.zeros(self.n_blocks, dtype=np.int32)
+        _check(self._lib.dcg_pattern(self._h, _ptr(rp), _ptr(col)))
+        return rp, col
+
+    def generated_source(self) -> str:
+        need = _S()
+        _check(self._lib.dcg_generated_source(self._h, None, 0, ctypes.byref(need)))
+        buf = ctypes.create_string_buffer(need.value)
+        _check(self._lib.dcg_generated_source(self._h, buf, need.value, None))
+        return buf.value.decode()
+
+    # ---- state and stepping
+    def initial_state(self) -> np.ndarray:
+        u = np.zeros(self.n_dof)
+        _check(self._lib.dcg_initial_state(self._h, _ptr(u), self.n_dof))
+        return u
+
+    def set_state(self, u: np.ndarray, time: float = 0.0):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert u.size == self.n_dof
+        _check(self._lib.dcg_set_state(self._h, _ptr(u), self.n_dof, time))
+
+    def get_state(self):
+        u = np.zeros(self.n_dof)
+        t = _D()
+        _check(self._lib.dcg_get_state(self._h, _ptr(u), self.n_dof, ctypes.byref(t)))
+        return u, t.value
+
+    def step(self, t: float, dt: float) -> StepStats:
+        st = StepStats()
+        _check(self._lib.dcg_step(self._h, t, dt, ctypes.byref(st)))
+        return st
+
+    def mass(self) -> np.ndarray:
+        m = np.zeros(self.n_species)
+        _check(self._lib.dcg_mass(self._h, _ptr(m)))
+        return m
+
+    # ---- kernel-level entry points
+    def assemble(self, x: np.ndarray, t: float, mass_weight: float, stiffness_weight: float, jacobian: bool = True):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        r = np.zeros(self.n_dof)
+        B = self.block_size
+        J = np.zeros(self.n_blocks * B * B) if jacobian else None
+        _check(self._lib.dcg_assemble(self._h, _ptr(x), t, mass_weight, stiffness_weight, _ptr(r),
+                                      _ptr(J) if J is not None else None))
+        return r, (J.reshape(self.n_blocks, B, B) if J is not None else None)
+
+    def spmv(self, J: np.ndarray, x: np.ndarray) -> np.ndarray:
+        J = np.ascontiguousarray(J, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.n_dof)
+        _check(self._lib.dcg_spmv(self._h, _ptr(J), _ptr(x), _ptr(y)))
+        return y
+
+    def linear_solve(self, J: np.ndarray, b: np.ndarray, rtol: float = 1e-10, maxit: int = 1000):
+        J = np.ascontiguousarray(J, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.n_dof)
+        its = _I()
+        rc = _check(self._lib.dcg_linear_solve(self._h, _ptr(J), _ptr(b), _ptr(x), rtol, maxit, ctypes.byref(its)))
+        return x, rc, its.value
+
+    # ---- helpers on the layout
+    def slot_dofs(self):
+        """For every slot: (first packed dof, species count)."""
+        first = np.zeros(self.n_slots, dtype=np.int64)
+        cnt = np.zeros(self.n_slots, dtype=np.int64)
+        s = 0
+        for c in self.compartments():
+            n, off, b = c["n_nodes"], c["dof_offset"], c["n_species"]
+            first[s:s + n] = off + np.arange(n) * b
+            cnt[s:s + n] = b
+            s += n
+        return first, cnt

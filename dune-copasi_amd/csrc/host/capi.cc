@@ -1,0 +1,210 @@
+// extern "C" boundary (include/dcopasi.h) over dcopasi::Model.
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../../include/dcopasi.h"
+#include "model.hh"
+
+struct dcg_model {
+  std::unique_ptr<dcopasi::Model> m;
+};
+
+namespace {
+thread_local std::string g_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -1;
+  } catch (...) {
+    g_error = "unknown error";
+    return -1;
+  }
+}
+
+int check(dcg_model_t h) {
+  if (!h || !h->m) {
+    g_error = "null model handle";
+    return -1;
+  }
+  return 0;
+}
+
+void copy_name(const std::string& s, char* buf, size_t len) {
+  if (!buf || !len) return;
+  std::strncpy(buf, s.c_str(), len - 1);
+  buf[len - 1] = '\0';
+}
+}  // namespace
+
+extern "C" {
+
+const char* dcg_last_error(void) { return g_error.c_str(); }
+
+int dcg_model_create(const char* ini_text, const char* overrides, int device, dcg_model_t* out) {
+  return guarded([&] {
+    if (!out) throw dcopasi::Error("null output handle");
+    *out = nullptr;
+    auto h = std::make_unique<dcg_model>();
+    h->m = std::make_unique<dcopasi::Model>(ini_text ? ini_text : "", overrides ? overrides : "", device);
+    *out = h.release();
+    return 0;
+  });
+}
+
+void dcg_model_destroy(dcg_model_t model) { delete model; }
+
+int dcg_state_size(dcg_model_t h, size_t* n_dof, size_t* n_species, size_t* n_compartments) {
+  if (check(h)) return -1;
+  if (n_dof) *n_dof = h->m->n_dof();
+  if (n_species) *n_species = h->m->n_species();
+  if (n_compartments) *n_compartments = h->m->n_compartments();
+  return 0;
+}
+
+int dcg_set_state(dcg_model_t h, const double* state, size_t n, double time) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    if (n != h->m->n_dof()) throw dcopasi::Error("state size mismatch");
+    h->m->set_state(state, time);
+    return 0;
+  });
+}
+
+int dcg_get_state(dcg_model_t h, double* state, size_t n, double* time) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    if (n != h->m->n_dof()) throw dcopasi::Error("state size mismatch");
+    h->m->get_state(state, time);
+    return 0;
+  });
+}
+
+int dcg_initial_state(dcg_model_t h, double* state, size_t n) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    if (n != h->m->n_dof()) throw dcopasi::Error("state size mismatch");
+    h->m->initial_state(state);
+    return 0;
+  });
+}
+
+int dcg_step(dcg_model_t h, double t, double dt, dcg_step_stats* stats) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    dcopasi::StepStats st;
+    int rc = h->m->step(t, dt, &st);
+    if (stats) {
+      stats->status = st.status;
+      stats->stages = st.stages;
+      stats->newton_iterations = st.newton_iterations;
+      stats->linear_iterations = st.linear_iterations;
+      stats->refinement_steps = st.refinement_steps;
+      stats->residual_norm = st.residual_norm;
+    }
+    return rc;
+  });
+}
+
+int dcg_mass(dcg_model_t h, double* per_species) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    h->m->mass(per_species);
+    return 0;
+  });
+}
+
+int dcg_species_info(dcg_model_t h, size_t s, char* name, size_t name_len, int* compartment, int* local_index) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    const auto& sp = h->m->spec().species;
+    if (s >= sp.size()) throw dcopasi::Error("species index out of range");
+    copy_name(sp[s].name, name, name_len);
+    if (compartment) *compartment = sp[s].comp;
+    if (local_index) *local_index = sp[s].local;
+    return 0;
+  });
+}
+
+int dcg_compartment_info(dcg_model_t h, size_t c, char* name, size_t name_len, size_t* n_nodes, size_t* dof_offset,
+                         int* n_species) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    const auto& comps = h->m->layout().comps;
+    if (c >= comps.size()) throw dcopasi::Error("compartment index out of range");
+    copy_name(comps[c].name, name, name_len);
+    if (n_nodes) *n_nodes = static_cast<size_t>(comps[c].nslot);
+    if (dof_offset) *dof_offset = static_cast<size_t>(comps[c].dof_off);
+    if (n_species) *n_species = comps[c].b;
+    return 0;
+  });
+}
+
+int dcg_compartment_nodes(dcg_model_t h, size_t c, long* vertex_ids) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    const auto& comps = h->m->layout().comps;
+    if (c >= comps.size()) throw dcopasi::Error("compartment index out of range");
+    std::copy(comps[c].nodes.begin(), comps[c].nodes.end(), vertex_ids);
+    return 0;
+  });
+}
+
+int dcg_pattern_info(dcg_model_t h, size_t* n_slots, size_t* n_blocks, int* block_size) {
+  if (check(h)) return -1;
+  const auto& L = h->m->layout();
+  if (n_slots) *n_slots = static_cast<size_t>(L.nslot);
+  if (n_blocks) *n_blocks = static_cast<size_t>(L.nnzb());
+  if (block_size) *block_size = L.B;
+  return 0;
+}
+
+int dcg_pattern(dcg_model_t h, int* row_ptr, int* col) {
+  if (check(h)) return -1;
+  const auto& L = h->m->layout();
+  if (row_ptr) std::copy(L.row_ptr.begin(), L.row_ptr.end(), row_ptr);
+  if (col) std::copy(L.col.begin(), L.col.end(), col);
+  return 0;
+}
+
+int dcg_generated_source(dcg_model_t h, char* buf, size_t len, size_t* needed) {
+  if (check(h)) return -1;
+  const std::string& s = h->m->source();
+  if (needed) *needed = s.size() + 1;
+  if (buf && len) copy_name(s, buf, len);
+  return 0;
+}
+
+int dcg_assemble(dcg_model_t h, const double* x, double t, double mass_weight, double stiffness_weight,
+                 double* residual, double* jacobian) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    h->m->assemble(x, t, mass_weight, stiffness_weight, residual, jacobian);
+    return 0;
+  });
+}
+
+int dcg_spmv(dcg_model_t h, const double* jacobian, const double* x, double* y) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    h->m->spmv(jacobian, x, y);
+    return 0;
+  });
+}
+
+int dcg_linear_solve(dcg_model_t h, const double* jacobian, const double* b, double* x, double rtol, int maxit,
+                     int* iterations) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    int its = 0;
+    int rc = h->m->linear_solve(jacobian, b, x, rtol, maxit, &its);
+    if (iterations) *iterations = its;
+    return rc;
+  });
+}
+
+}  // extern "C"

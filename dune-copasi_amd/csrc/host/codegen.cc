@@ -1,0 +1,866 @@
+#include "codegen.hh"
+
+#include <cmath>
+#include <cstdio>
+#include <sstream>
+
+#include "error.hh"
+
+namespace dcopasi {
+
+namespace {
+
+std::string D(double v) {
+  char b[40];
+  std::snprintf(b, sizeof b, "%.17g", v);
+  std::string s = b;
+  if (s.find_first_of(".eEn") == std::string::npos) s += ".0";
+  return s;
+}
+
+const char* kAxis = "xyz";
+
+// reference shape functions
+void shape(int dim, bool cube, const double* xi, double* phi, double (*dphi)[3]) {
+  if (cube) {
+    int nv = 1 << dim;
+    for (int v = 0; v < nv; ++v) {
+      double p = 1.0, g[3] = {1, 1, 1};
+      for (int a = 0; a < dim; ++a) {
+        bool b = (v >> a) & 1;
+        double f = b ? xi[a] : 1.0 - xi[a];
+        for (int c = 0; c < dim; ++c) g[c] *= (c == a) ? (b ? 1.0 : -1.0) : f;
+        p *= f;
+      }
+      phi[v] = p;
+      for (int c = 0; c < 3; ++c) dphi[v][c] = c < dim ? g[c] : 0.0;
+    }
+  } else {
+    double s = 0;
+    for (int a = 0; a < dim; ++a) s += xi[a];
+    phi[0] = 1.0 - s;
+    for (int c = 0; c < 3; ++c) dphi[0][c] = c < dim ? -1.0 : 0.0;
+    for (int v = 1; v <= dim; ++v) {
+      phi[v] = xi[v - 1];
+      for (int c = 0; c < 3; ++c) dphi[v][c] = (c == v - 1) ? 1.0 : 0.0;
+    }
+  }
+}
+
+void invert(int d, double A[3][3], double B[3][3], double& det) {
+  if (d == 1) {
+    det = A[0][0];
+    B[0][0] = 1.0 / det;
+  } else if (d == 2) {
+    det = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+    B[0][0] = A[1][1] / det;
+    B[0][1] = -A[0][1] / det;
+    B[1][0] = -A[1][0] / det;
+    B[1][1] = A[0][0] / det;
+  } else {
+    det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+          A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+    B[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) / det;
+    B[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
+    B[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
+    B[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) / det;
+    B[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det;
+    B[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / det;
+    B[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / det;
+    B[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) / det;
+    B[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det;
+  }
+  det = std::fabs(det);
+}
+
+struct ElementTables {
+  int nq, nv;
+  std::vector<double> phi;   // [q][a]
+  std::vector<double> gphi;  // [p][q][a][k] physical gradients
+  std::vector<double> fac;   // [q]
+  std::vector<double> xq;    // [p][q][3] offset of the point from the cell origin
+  std::vector<int> off;      // [p][a][c] stencil index
+  double evol;
+};
+
+ElementTables element_tables(const Grid& g) {
+  ElementTables T;
+  Rule r = element_rule(g.dim, g.cube);
+  T.nq = static_cast<int>(r.w.size());
+  T.nv = g.nvpe;
+  const int d = g.dim;
+  double phi[8], dref[8][3];
+  for (int q = 0; q < T.nq; ++q) {
+    shape(d, g.cube, r.x[q].data(), phi, dref);
+    for (int a = 0; a < T.nv; ++a) T.phi.push_back(phi[a]);
+  }
+  double det = 0;
+  for (int p = 0; p < g.nclass; ++p) {
+    // affine map of class p: x = cell_origin + J xi, J columns from the corners
+    double J[3][3] = {{0}}, Ji[3][3] = {{0}};
+    auto cpos = [&](int corner, int k) { return ((corner >> k) & 1) ? g.h[k] : 0.0; };
+    for (int col = 0; col < d; ++col) {
+      int cv = g.cube ? (1 << col) : g.corner[p][col + 1];
+      int c0 = g.corner[p][0];
+      for (int k = 0; k < d; ++k) J[k][col] = cpos(cv, k) - cpos(c0, k);
+    }
+    invert(d, J, Ji, det);
+    for (int q = 0; q < T.nq; ++q) {
+      shape(d, g.cube, r.x[q].data(), phi, dref);
+      for (int a = 0; a < T.nv; ++a)
+        for (int k = 0; k < d; ++k) {
+          double s = 0;
+          for (int l = 0; l < d; ++l) s += dref[a][l] * Ji[l][k];
+          T.gphi.push_back(s);
+        }
+      for (int k = 0; k < 3; ++k) {
+        double s = k < d ? cpos(g.corner[p][0], k) : 0.0;
+        for (int l = 0; l < d && k < d; ++l) s += J[k][l] * r.x[q][l];
+        T.xq.push_back(s);
+      }
+    }
+    for (int a = 0; a < T.nv; ++a)
+      for (int c = 0; c < T.nv; ++c) {
+        int ca = g.corner[p][a], cc = g.corner[p][c];
+        int o = 0, m = 1;
+        for (int k = 0; k < d; ++k) {
+          o += (((cc >> k) & 1) - ((ca >> k) & 1) + 1) * m;
+          m *= 3;
+        }
+        T.off.push_back(o);
+      }
+  }
+  T.evol = 0;
+  for (int q = 0; q < T.nq; ++q) {
+    T.fac.push_back(r.w[q] * det);
+    T.evol += r.w[q] * det;
+  }
+  return T;
+}
+
+template <class V>
+std::string array_init(const V& v) {
+  std::ostringstream s;
+  s << "{";
+  for (size_t i = 0; i < v.size(); ++i) s << (i ? "," : "") << D(static_cast<double>(v[i]));
+  s << "}";
+  return s.str();
+}
+std::string int_init(const std::vector<int>& v) {
+  std::ostringstream s;
+  s << "{";
+  for (size_t i = 0; i < v.size(); ++i) s << (i ? "," : "") << v[i];
+  s << "}";
+  return s.str();
+}
+
+bool tensor_const(const TensorTerm& t) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      if (t.d[i][j] && !expr::is_const(t.d[i][j])) return false;
+  return true;
+}
+
+const char* kPrelude = R"(
+#define DCG_B %B%
+__device__ __forceinline__ double dcg_sign(double v) { return (double)((v > 0.0) - (v < 0.0)); }
+__device__ __forceinline__ int dcg_pos(const int* own_start, const unsigned* own_mask, int s, int o) {
+  return own_start[s] + __popc(own_mask[s] & ((1u << o) - 1u));
+}
+)";
+
+// Volume kernel body; %C% is the compartment namespace.
+const char* kVolume = R"(
+namespace %C% {
+template <int P>
+__device__ __forceinline__ void residual(const double (&xl)[NV][BC], double cx, double cy, double cz, double t,
+                                         double cm, double ca, double (&rl)[NV][BC], double (*mc)[BC][BC],
+                                         double (*gc)[BC][BC][DIM], double (*dc)[BC][BC][DIM][DIM]) {
+#pragma unroll
+  for (int a = 0; a < NV; ++a)
+#pragma unroll
+    for (int i = 0; i < BC; ++i) rl[a][i] = 0.0;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double u[BC], g[BC][3];
+#pragma unroll
+    for (int i = 0; i < BC; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int a = 0; a < NV; ++a) s += PHI[q][a] * xl[a][i];
+      u[i] = s;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        double gs = 0.0;
+        if (k < DIM) {
+#pragma unroll
+          for (int a = 0; a < NV; ++a) gs += GPHI[P][q][a][k < DIM ? k : 0] * xl[a][i];
+        }
+        g[i][k] = gs;
+      }
+    }
+    const double px = cx + XQ[P][q][0], py = cy + XQ[P][q][1], pz = cz + XQ[P][q][2];
+    (void)px; (void)py; (void)pz;
+    double rs[BC], rf[BC][DIM];
+    qp(q, u, g, px, py, pz, t, cm, ca, rs, rf, mc ? mc[q] : nullptr, gc ? gc[q] : nullptr, dc ? dc[q] : nullptr);
+#pragma unroll
+    for (int a = 0; a < NV; ++a)
+#pragma unroll
+      for (int i = 0; i < BC; ++i) {
+        double s = rs[i] * PHI[q][a];
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) s += rf[i][k] * GPHI[P][q][a][k];
+        rl[a][i] += s;
+      }
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void element(const double (&xc)[NCORNER][BC], const int (&sl)[NCORNER], double cx,
+                                        double cy, double cz, double t, double cm, double ca, double* r,
+                                        double* val, const int* own_start, const unsigned* own_mask, int mode) {
+  double xl[NV][BC];
+#pragma unroll
+  for (int a = 0; a < NV; ++a)
+#pragma unroll
+    for (int i = 0; i < BC; ++i) xl[a][i] = xc[CORNER[P][a]][i];
+  double rl[NV][BC];
+  const bool want_j = (mode & 2) != 0;
+#if NUMERICAL_JAC
+  residual<P>(xl, cx, cy, cz, t, cm, ca, rl, nullptr, nullptr, nullptr);
+  if (want_j) {
+#pragma unroll 1
+    for (int c = 0; c < NV; ++c) {
+#pragma unroll 1
+      for (int j = 0; j < BC; ++j) {
+        double xp[NV][BC];
+#pragma unroll
+        for (int a = 0; a < NV; ++a)
+#pragma unroll
+          for (int i = 0; i < BC; ++i) xp[a][i] = xl[a][i];
+        const double delta = FD_EPS * (1.0 + fabs(xl[c][j]));
+        xp[c][j] += delta;
+        double rp[NV][BC];
+        residual<P>(xp, cx, cy, cz, t, cm, ca, rp, nullptr, nullptr, nullptr);
+#pragma unroll
+        for (int a = 0; a < NV; ++a) {
+          const int sa = sl[CORNER[P][a]];
+          const int pos = dcg_pos(own_start, own_mask, sa, OFF[P][a][c]);
+#pragma unroll
+          for (int i = 0; i < BC; ++i)
+            val[(size_t)pos * (DCG_B * DCG_B) + i * DCG_B + j] += (rp[a][i] - rl[a][i]) / delta;
+        }
+      }
+    }
+  }
+#else
+  double mc[NQ][BC][BC];
+#if HAS_GC
+  double gc[NQ][BC][BC][DIM];
+#endif
+#if HAS_DC
+  double dc[NQ][BC][BC][DIM][DIM];
+#endif
+  residual<P>(xl, cx, cy, cz, t, cm, ca, rl, want_j ? mc : nullptr,
+#if HAS_GC
+              want_j ? gc : nullptr,
+#else
+              nullptr,
+#endif
+#if HAS_DC
+              want_j ? dc : nullptr
+#else
+              nullptr
+#endif
+  );
+  if (want_j) {
+#pragma unroll
+    for (int a = 0; a < NV; ++a) {
+      const int sa = sl[CORNER[P][a]];
+      const int st = own_start[sa];
+      const unsigned mk = own_mask[sa];
+#pragma unroll
+      for (int c = 0; c < NV; ++c) {
+        const int pos = st + __popc(mk & ((1u << OFF[P][a][c]) - 1u));
+        double* blk = val + (size_t)pos * (DCG_B * DCG_B);
+#pragma unroll
+        for (int i = 0; i < BC; ++i)
+#pragma unroll
+          for (int j = 0; j < BC; ++j) {
+            double s = 0.0;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) s += PHI[q][a] * PHI[q][c] * mc[q][i][j];
+#if HAS_GC
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+#pragma unroll
+              for (int k = 0; k < DIM; ++k) s += PHI[q][c] * GPHI[P][q][a][k] * gc[q][i][j][k];
+#endif
+#if HAS_DC
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+#pragma unroll
+              for (int k = 0; k < DIM; ++k)
+#pragma unroll
+                for (int l = 0; l < DIM; ++l) s += GPHI[P][q][a][k] * dc[q][i][j][k][l] * GPHI[P][q][c][l];
+#endif
+            s += ca * const_diffusion<P>(a, c, i, j);
+            blk[i * DCG_B + j] += s;
+          }
+      }
+    }
+  }
+#endif
+  if (mode & 1) {
+#pragma unroll
+    for (int a = 0; a < NV; ++a) {
+      const int sa = sl[CORNER[P][a]];
+#pragma unroll
+      for (int i = 0; i < BC; ++i) r[(size_t)sa * DCG_B + i] += rl[a][i];
+    }
+  }
+}
+}  // namespace %C%
+
+extern "C" __global__ void __launch_bounds__(128)
+%KERNEL%(const int* __restrict__ cells, const unsigned char* __restrict__ cmask, const int* __restrict__ cslots,
+         int first, int count, const double* __restrict__ x, double t, double cm, double ca,
+         double* __restrict__ r, double* __restrict__ val, const int* __restrict__ own_start,
+         const unsigned* __restrict__ own_mask, int n0, int n1, double ox, double oy, double oz, double hx,
+         double hy, double hz, int mode) {
+  using namespace %C%;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= count) return;
+  const int idx = first + tid;
+  const int cell = cells[idx];
+  const unsigned msk = cmask[idx];
+  int sl[NCORNER];
+  double xc[NCORNER][BC];
+#pragma unroll
+  for (int c = 0; c < NCORNER; ++c) {
+    sl[c] = cslots[(size_t)idx * NCORNER + c];
+#pragma unroll
+    for (int i = 0; i < BC; ++i) xc[c][i] = sl[c] >= 0 ? x[(size_t)sl[c] * DCG_B + i] : 0.0;
+  }
+  const int ci = cell % n0, cj = (cell / n0) % n1, ck = cell / (n0 * n1);
+  const double cx = ox + ci * hx, cy = oy + cj * hy, cz = oz + ck * hz;
+%DISPATCH%
+}
+)";
+
+// Facet kernel body
+const char* kFacet = R"(
+namespace %F% {
+__device__ __forceinline__ void facet_residual(const double (&xi)[NVF][BI > 0 ? BI : 1],
+                                               const double (&xo)[NVF][BO > 0 ? BO : 1], const double* fd,
+                                               double t, double ca, double (&ri)[NVF][BI > 0 ? BI : 1],
+                                               double (&ro)[NVF][BO > 0 ? BO : 1], double* Jii, double* Jio,
+                                               double* Joi, double* Joo) {
+#pragma unroll
+  for (int f = 0; f < NVF; ++f) {
+#pragma unroll
+    for (int i = 0; i < BI; ++i) ri[f][i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < BO; ++i) ro[f][i] = 0.0;
+  }
+  const double nx = fd[NQF * 4 + 0], ny = fd[NQF * 4 + 1], nz = fd[NQF * 4 + 2], fvol = fd[NQF * 4 + 3];
+  (void)nx; (void)ny; (void)nz; (void)fvol;
+#pragma unroll
+  for (int q = 0; q < NQF; ++q) {
+    const double w = fd[q] * ca;
+    const double px = fd[NQF + 3 * q], py = fd[NQF + 3 * q + 1], pz = fd[NQF + 3 * q + 2];
+    (void)px; (void)py; (void)pz;
+    double ui[BI > 0 ? BI : 1], uo[BO > 0 ? BO : 1];
+#pragma unroll
+    for (int i = 0; i < BI; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int f = 0; f < NVF; ++f) s += PHIF[q][f] * xi[f][i];
+      ui[i] = s;
+    }
+#pragma unroll
+    for (int i = 0; i < BO; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int f = 0; f < NVF; ++f) s += PHIF[q][f] * xo[f][i];
+      uo[i] = s;
+    }
+    (void)ui; (void)uo;
+%QPBODY%
+  }
+}
+}  // namespace %F%
+
+extern "C" __global__ void __launch_bounds__(64)
+%KERNEL%(const int* __restrict__ fint, const double* __restrict__ fdbl, int first, int count,
+         const double* __restrict__ x, double t, double ca, double* __restrict__ r, double* __restrict__ val,
+         int mode) {
+  using namespace %F%;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= count) return;
+  const int id = first + tid;
+  const int* fi = fint + (size_t)id * FINT;
+  const double* fd = fdbl + (size_t)id * FDBL;
+  double xi[NVF][BI > 0 ? BI : 1], xo[NVF][BO > 0 ? BO : 1];
+#pragma unroll
+  for (int f = 0; f < NVF; ++f) {
+#pragma unroll
+    for (int i = 0; i < BI; ++i) xi[f][i] = x[(size_t)fi[f] * DCG_B + i];
+#pragma unroll
+    for (int i = 0; i < BO; ++i) xo[f][i] = x[(size_t)fi[NVF + f] * DCG_B + i];
+  }
+  double ri[NVF][BI > 0 ? BI : 1], ro[NVF][BO > 0 ? BO : 1];
+  const bool want_j = (mode & 2) != 0;
+#if NUMERICAL_JAC
+  facet_residual(xi, xo, fd, t, ca, ri, ro, nullptr, nullptr, nullptr, nullptr);
+  if (want_j) {
+    const int* pos = fi + 2 * NVF;
+#pragma unroll 1
+    for (int side = 0; side < 2; ++side) {
+      const int bs = side == 0 ? BI : BO;
+#pragma unroll 1
+      for (int g = 0; g < NVF; ++g)
+#pragma unroll 1
+        for (int j = 0; j < bs; ++j) {
+          double xi2[NVF][BI > 0 ? BI : 1], xo2[NVF][BO > 0 ? BO : 1];
+#pragma unroll
+          for (int f = 0; f < NVF; ++f) {
+#pragma unroll
+            for (int i = 0; i < BI; ++i) xi2[f][i] = xi[f][i];
+#pragma unroll
+            for (int i = 0; i < BO; ++i) xo2[f][i] = xo[f][i];
+          }
+          double& xv = side == 0 ? xi2[g][j] : xo2[g][j];
+          const double delta = FD_EPS * (1.0 + fabs(xv));
+          xv += delta;
+          double ri2[NVF][BI > 0 ? BI : 1], ro2[NVF][BO > 0 ? BO : 1];
+          facet_residual(xi2, xo2, fd, t, ca, ri2, ro2, nullptr, nullptr, nullptr, nullptr);
+#pragma unroll
+          for (int f = 0; f < NVF; ++f) {
+#pragma unroll
+            for (int i = 0; i < BI; ++i) {
+              const int p = pos[(side == 0 ? 0 : 1) * NVF * NVF + f * NVF + g];
+              val[(size_t)p * (DCG_B * DCG_B) + i * DCG_B + j] += (ri2[f][i] - ri[f][i]) / delta;
+            }
+#pragma unroll
+            for (int i = 0; i < BO; ++i) {
+              const int p = pos[(side == 0 ? 2 : 3) * NVF * NVF + f * NVF + g];
+              val[(size_t)p * (DCG_B * DCG_B) + i * DCG_B + j] += (ro2[f][i] - ro[f][i]) / delta;
+            }
+          }
+        }
+    }
+  }
+#else
+  double Jii[NVF * NVF * (BI > 0 ? BI * BI : 1)], Jio[NVF * NVF * (BI * BO > 0 ? BI * BO : 1)],
+      Joi[NVF * NVF * (BI * BO > 0 ? BI * BO : 1)], Joo[NVF * NVF * (BO > 0 ? BO * BO : 1)];
+#pragma unroll
+  for (int k = 0; k < NVF * NVF * BI * BI; ++k) Jii[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NVF * NVF * BI * BO; ++k) { Jio[k] = 0.0; Joi[k] = 0.0; }
+#pragma unroll
+  for (int k = 0; k < NVF * NVF * BO * BO; ++k) Joo[k] = 0.0;
+  facet_residual(xi, xo, fd, t, ca, ri, ro, Jii, Jio, Joi, Joo);
+  if (want_j) {
+    const int* pos = fi + 2 * NVF;
+#pragma unroll
+    for (int f = 0; f < NVF; ++f)
+#pragma unroll
+      for (int g = 0; g < NVF; ++g) {
+        double* b0 = val + (size_t)pos[0 * NVF * NVF + f * NVF + g] * (DCG_B * DCG_B);
+#pragma unroll
+        for (int i = 0; i < BI; ++i)
+#pragma unroll
+          for (int j = 0; j < BI; ++j) b0[i * DCG_B + j] += Jii[((f * NVF + g) * BI + i) * BI + j];
+        if (BO > 0) {
+          double* b1 = val + (size_t)pos[1 * NVF * NVF + f * NVF + g] * (DCG_B * DCG_B);
+          double* b2 = val + (size_t)pos[2 * NVF * NVF + f * NVF + g] * (DCG_B * DCG_B);
+          double* b3 = val + (size_t)pos[3 * NVF * NVF + f * NVF + g] * (DCG_B * DCG_B);
+#pragma unroll
+          for (int i = 0; i < BI; ++i)
+#pragma unroll
+            for (int j = 0; j < BO; ++j) b1[i * DCG_B + j] += Jio[((f * NVF + g) * BI + i) * (BO > 0 ? BO : 1) + j];
+#pragma unroll
+          for (int i = 0; i < BO; ++i)
+#pragma unroll
+            for (int j = 0; j < BI; ++j) b2[i * DCG_B + j] += Joi[((f * NVF + g) * (BO > 0 ? BO : 1) + i) * BI + j];
+#pragma unroll
+          for (int i = 0; i < BO; ++i)
+#pragma unroll
+            for (int j = 0; j < BO; ++j) b3[i * DCG_B + j] += Joo[((f * NVF + g) * (BO > 0 ? BO : 1) + i) * (BO > 0 ? BO : 1) + j];
+        }
+      }
+  }
+#endif
+  if (mode & 1) {
+#pragma unroll
+    for (int f = 0; f < NVF; ++f) {
+#pragma unroll
+      for (int i = 0; i < BI; ++i) r[(size_t)fi[f] * DCG_B + i] += ri[f][i];
+#pragma unroll
+      for (int i = 0; i < BO; ++i) r[(size_t)fi[NVF + f] * DCG_B + i] += ro[f][i];
+    }
+  }
+}
+)";
+
+void replace_all(std::string& s, const std::string& from, const std::string& to) {
+  size_t p = 0;
+  while ((p = s.find(from, p)) != std::string::npos) {
+    s.replace(p, from.size(), to);
+    p += to.size();
+  }
+}
+
+// variable names -> C expressions for one compartment's volume expressions
+std::map<std::string, std::string> volume_vars(const ModelSpec& m, int comp, double evol) {
+  std::map<std::string, std::string> v;
+  v["time"] = "t";
+  v["entity_volume"] = D(evol);
+  v["in_volume"] = "1.0";
+  v["in_boundary"] = "0.0";
+  v["in_skeleton"] = "0.0";
+  v["integration_factor"] = "0.0";
+  const char* pos[3] = {"px", "py", "pz"};
+  for (int k = 0; k < 3; ++k) v[std::string("position_") + kAxis[k]] = k < m.dim ? pos[k] : "0.0";
+  for (size_t s = 0; s < m.species.size(); ++s) {
+    const auto& sp = m.species[s];
+    bool own = sp.comp == comp;
+    v[sp.name] = own ? "u[" + std::to_string(sp.local) + "]" : "0.0";
+    for (int k = 0; k < 3; ++k)
+      v["grad_" + sp.name + "_" + kAxis[k]] =
+          (own && k < m.dim) ? "g[" + std::to_string(sp.local) + "][" + std::to_string(k) + "]" : "0.0";
+  }
+  return v;
+}
+
+std::string gen_volume(const ModelSpec& m, const Grid& g, int comp, const std::string& ns, const std::string& kname,
+                       bool numerical) {
+  const auto T = element_tables(g);
+  const auto& species = m.comp_species[comp];
+  const int bc = static_cast<int>(species.size());
+  const int d = m.dim;
+  auto vars = volume_vars(m, comp, T.evol);
+  auto E = [&](const Ex& e) { return expr::emit(e, vars); };
+
+  bool has_gc = false, has_dc = false;
+  for (int s : species) {
+    const auto& sp = m.species[s];
+    for (int k = 0; k < d; ++k) has_gc = has_gc || sp.velocity[k];
+    for (const auto& dt : sp.diffusion) {
+      if (!dt.jac.empty()) has_gc = true;
+      if (!tensor_const(dt.D)) has_dc = true;
+    }
+  }
+  std::ostringstream o;
+  o << "namespace " << ns << " {\n";
+  o << "constexpr int BC = " << bc << ", NV = " << T.nv << ", NQ = " << T.nq << ", NCORNER = " << g.ncorner
+    << ", NCLASS = " << g.nclass << ", DIM = " << d << ";\n";
+  o << "constexpr double FD_EPS = " << D(m.fd_epsilon) << ";\n";
+  std::vector<int> corner;
+  for (int p = 0; p < g.nclass; ++p)
+    for (int a = 0; a < T.nv; ++a) corner.push_back(g.corner[p][a]);
+  o << "__device__ constexpr int CORNER[NCLASS][NV] = " << int_init(corner) << ";\n";
+  o << "__device__ constexpr int OFF[NCLASS][NV][NV] = " << int_init(T.off) << ";\n";
+  o << "__device__ constexpr double PHI[NQ][NV] = " << array_init(T.phi) << ";\n";
+  o << "__device__ constexpr double GPHI[NCLASS][NQ][NV][DIM] = " << array_init(T.gphi) << ";\n";
+  o << "__device__ constexpr double FAC[NQ] = " << array_init(T.fac) << ";\n";
+  o << "__device__ constexpr double XQ[NCLASS][NQ][3] = " << array_init(T.xq) << ";\n";
+  // constant diffusion: sum_q FAC grad phi_a . D grad phi_c folded per species pair
+  {
+    // KD[p][a][c][i][j]
+    std::vector<double> kd(static_cast<size_t>(g.nclass) * T.nv * T.nv * std::max(1, bc * bc), 0.0);
+    bool any = false;
+    for (int i = 0; i < bc; ++i) {
+      const auto& sp = m.species[species[i]];
+      for (const auto& dt : sp.diffusion) {
+        if (!tensor_const(dt.D)) continue;
+        int j = m.species[dt.wrt].local;
+        double Dm[3][3] = {{0}};
+        for (int k = 0; k < d; ++k)
+          for (int l = 0; l < d; ++l) {
+            double v = 0;
+            if (dt.D.tensor) expr::is_const(dt.D.d[k][l], &v);
+            else if (k == l) expr::is_const(dt.D.d[0][0], &v);
+            Dm[k][l] = v;
+          }
+        for (int p = 0; p < g.nclass; ++p)
+          for (int a = 0; a < T.nv; ++a)
+            for (int c = 0; c < T.nv; ++c) {
+              double s = 0;
+              for (int q = 0; q < T.nq; ++q)
+                for (int k = 0; k < d; ++k)
+                  for (int l = 0; l < d; ++l) {
+                    double ga = T.gphi[((static_cast<size_t>(p) * T.nq + q) * T.nv + a) * d + k];
+                    double gc = T.gphi[((static_cast<size_t>(p) * T.nq + q) * T.nv + c) * d + l];
+                    s += T.fac[q] * ga * Dm[k][l] * gc;
+                  }
+              kd[((static_cast<size_t>(p) * T.nv + a) * T.nv + c) * bc * bc + i * bc + j] += s;
+              any = any || s != 0.0;
+            }
+      }
+    }
+    if (any) {
+      o << "__device__ constexpr double KD[NCLASS][NV][NV][BC][BC] = " << array_init(kd) << ";\n";
+      o << "template <int P> __device__ __forceinline__ double const_diffusion(int a, int c, int i, int j) { return KD[P][a][c][i][j]; }\n";
+    } else {
+      o << "template <int P> __device__ __forceinline__ double const_diffusion(int, int, int, int) { return 0.0; }\n";
+    }
+  }
+  // quadrature point kernel
+  o << "__device__ __forceinline__ void qp(int q, const double (&u)[BC], const double (&g)[BC][3], double px, double py, "
+       "double pz, double t, double cm, double ca, double (&rs)[BC], double (&rf)[BC][DIM], double (*mc)[BC], "
+       "double (*gc)[BC][DIM], double (*dc)[BC][DIM][DIM]) {\n";
+  o << "  (void)px; (void)py; (void)pz; (void)t; (void)g; (void)gc; (void)dc;\n";
+  o << "  const double fac = FAC[q];\n";
+  o << "#pragma unroll\n  for (int i = 0; i < BC; ++i) { rs[i] = 0.0;\n#pragma unroll\n for (int k = 0; k < DIM; ++k) rf[i][k] = 0.0; }\n";
+  o << "  if (mc) {\n#pragma unroll\n  for (int i = 0; i < BC; ++i)\n#pragma unroll\n for (int j = 0; j < BC; ++j) mc[i][j] = 0.0;}\n";
+  if (has_gc)
+    o << "  if (gc) {\n#pragma unroll\n for (int i = 0; i < BC; ++i)\n#pragma unroll\n for (int j = 0; j < BC; ++j)\n#pragma unroll\n for (int k = 0; k < DIM; ++k) gc[i][j][k] = 0.0;}\n";
+  if (has_dc)
+    o << "  if (dc) {\n#pragma unroll\n for (int i = 0; i < BC; ++i)\n#pragma unroll\n for (int j = 0; j < BC; ++j)\n#pragma unroll\n for (int k = 0; k < DIM; ++k)\n#pragma unroll\n for (int l = 0; l < DIM; ++l) dc[i][j][k][l] = 0.0;}\n";
+  for (int i = 0; i < bc; ++i) {
+    const auto& sp = m.species[species[i]];
+    const std::string I = std::to_string(i);
+    o << "  { // species " << sp.name << "\n";
+    if (sp.storage) {
+      o << "    const double S = " << E(sp.storage) << ";\n";
+      o << "    rs[" << I << "] += fac * cm * S * u[" << I << "];\n";
+      o << "    if (mc) { mc[" << I << "][" << I << "] += fac * cm * S;\n";
+      for (const auto& pj : sp.storage_jac) {
+        if (m.species[pj.wrt].comp != comp) continue;
+        o << "      mc[" << I << "][" << m.species[pj.wrt].local << "] += fac * cm * (" << E(pj.e) << ") * u[" << I << "];\n";
+      }
+      o << "    }\n";
+    }
+    if (sp.reaction) {
+      o << "    rs[" << I << "] -= fac * ca * (" << E(sp.reaction) << ");\n";
+      o << "    if (mc) {\n";
+      for (const auto& pj : sp.reaction_jac) {
+        if (m.species[pj.wrt].comp != comp) continue;
+        o << "      mc[" << I << "][" << m.species[pj.wrt].local << "] -= fac * ca * (" << E(pj.e) << ");\n";
+      }
+      o << "    }\n";
+    }
+    bool hv = false;
+    for (int k = 0; k < d; ++k) hv = hv || sp.velocity[k];
+    if (hv) {
+      for (int k = 0; k < d; ++k) {
+        if (!sp.velocity[k]) continue;
+        o << "    { const double vk = " << E(sp.velocity[k]) << ";\n";
+        o << "      rf[" << I << "][" << k << "] -= fac * ca * vk * u[" << I << "];\n";
+        o << "      if (gc) gc[" << I << "][" << I << "][" << k << "] -= fac * ca * vk; }\n";
+      }
+      for (const auto& vj : sp.velocity_jac) {
+        int j = m.species[vj.first].local;
+        for (int k = 0; k < d; ++k)
+          if (vj.second[k])
+            o << "    if (gc) gc[" << I << "][" << j << "][" << k << "] -= fac * ca * (" << E(vj.second[k]) << ") * u[" << I
+              << "];\n";
+      }
+    }
+    for (const auto& dt : sp.diffusion) {
+      int j = m.species[dt.wrt].local;
+      const std::string J = std::to_string(j);
+      // flux D grad u_j for the residual
+      o << "    {\n";
+      for (int k = 0; k < d; ++k)
+        for (int l = 0; l < d; ++l) {
+          Ex e = dt.D.tensor ? dt.D.d[k][l] : (k == l ? dt.D.d[0][0] : nullptr);
+          if (!e) continue;
+          o << "      { const double Dv = " << E(e) << ";\n";
+          o << "        rf[" << I << "][" << k << "] += fac * ca * Dv * g[" << J << "][" << l << "];\n";
+          if (!tensor_const(dt.D)) o << "        if (dc) dc[" << I << "][" << J << "][" << k << "][" << l << "] += fac * ca * Dv;\n";
+          o << "      }\n";
+        }
+      for (const auto& dj : dt.jac) {
+        int v = m.species[dj.first].local;
+        for (int k = 0; k < d; ++k)
+          for (int l = 0; l < d; ++l) {
+            Ex e = dj.second.tensor ? dj.second.d[k][l] : (k == l ? dj.second.d[0][0] : nullptr);
+            if (!e) continue;
+            o << "      if (gc) gc[" << I << "][" << v << "][" << k << "] += fac * ca * (" << E(e) << ") * g[" << J << "][" << l
+              << "];\n";
+          }
+      }
+      o << "    }\n";
+    }
+    o << "  }\n";
+  }
+  o << "}\n}  // namespace " << ns << "\n";
+  std::string body = kVolume;
+  replace_all(body, "%C%", ns);
+  replace_all(body, "%KERNEL%", kname);
+  std::ostringstream disp;
+  for (int p = 0; p < g.nclass; ++p)
+    disp << "  if ((msk >> " << p << ") & 1u) " << ns << "::element<" << p
+         << ">(xc, sl, cx, cy, cz, t, cm, ca, r, val, own_start, own_mask, mode);\n";
+  replace_all(body, "%DISPATCH%", disp.str());
+  std::ostringstream full;
+  full << "#define NUMERICAL_JAC " << (numerical ? 1 : 0) << "\n#define HAS_GC " << (has_gc ? 1 : 0) << "\n#define HAS_DC "
+       << (has_dc ? 1 : 0) << "\n";
+  full << o.str() << body << "#undef NUMERICAL_JAC\n#undef HAS_GC\n#undef HAS_DC\n";
+  return full.str();
+}
+
+std::string gen_facet(const ModelSpec& m, const Grid& g, int cin, int cout, bool boundary, const std::string& ns,
+                      const std::string& kname, bool numerical, FacetKernel& fk) {
+  Rule r = facet_rule(g.dim, g.cube);
+  const int nqf = static_cast<int>(r.w.size());
+  const int nvf = g.cube ? (1 << (g.dim - 1)) : g.dim;
+  const int bi = static_cast<int>(m.comp_species[cin].size());
+  const int bo = boundary ? 0 : static_cast<int>(m.comp_species[cout].size());
+  fk.nvf = nvf;
+  fk.nqf = nqf;
+  fk.fint = 2 * nvf + 4 * nvf * nvf;
+  fk.fdbl = 4 * nqf + 4;
+  // facet basis values (same vertex-order convention as the layout)
+  std::vector<double> phif;
+  for (int q = 0; q < nqf; ++q)
+    for (int f = 0; f < nvf; ++f) {
+      double e0 = r.x[q][0], e1 = r.x[q][1], phi;
+      if (g.dim == 1) phi = 1.0;
+      else if (g.cube) {
+        phi = (f & 1) ? e0 : 1.0 - e0;
+        if (g.dim == 3) phi *= ((f >> 1) & 1) ? e1 : 1.0 - e1;
+      } else {
+        phi = f == 0 ? 1.0 - e0 - (g.dim == 3 ? e1 : 0.0) : (f == 1 ? e0 : e1);
+      }
+      phif.push_back(phi);
+    }
+  auto vars_for = [&](bool inside) {
+    std::map<std::string, std::string> v;
+    v["time"] = "t";
+    v["entity_volume"] = "fvol";
+    v["in_volume"] = "0.0";
+    v["in_boundary"] = boundary ? "1.0" : "0.0";
+    v["in_skeleton"] = boundary ? "0.0" : "1.0";
+    v["integration_factor"] = "0.0";
+    const char* pos[3] = {"px", "py", "pz"};
+    const char* nrm[3] = {"nx", "ny", "nz"};
+    for (int k = 0; k < 3; ++k) {
+      v[std::string("position_") + kAxis[k]] = k < m.dim ? pos[k] : "0.0";
+      v[std::string("normal_") + kAxis[k]] = k < m.dim ? (inside ? std::string(nrm[k]) : "(-" + std::string(nrm[k]) + ")") : "0.0";
+    }
+    for (size_t s = 0; s < m.species.size(); ++s) {
+      const auto& sp = m.species[s];
+      std::string val = "0.0";
+      if (sp.comp == cin) val = "ui[" + std::to_string(sp.local) + "]";
+      else if (!boundary && sp.comp == cout) val = "uo[" + std::to_string(sp.local) + "]";
+      v[sp.name] = val;
+      // gradients on facets are not provided by this build
+    }
+    return v;
+  };
+  std::ostringstream qb;
+  auto side = [&](bool inside) {
+    int own = inside ? cin : cout;
+    int target = boundary ? cin : (inside ? cout : cin);
+    auto vars = vars_for(inside);
+    const std::string R = inside ? "ri" : "ro";
+    const int bown = inside ? bi : bo;
+    const std::string Jsame = inside ? "Jii" : "Joo";
+    const std::string Jother = inside ? "Jio" : "Joi";
+    const int bother = inside ? bo : bi;
+    for (int s : m.comp_species[own]) {
+      const auto& sp = m.species[s];
+      for (const auto& of : sp.outflow) {
+        if (of.to_comp != target) continue;
+        std::vector<std::string> used;
+        expr::collect_vars(of.e, used);
+        for (auto& p : of.jac) expr::collect_vars(p.e, used);
+        for (auto& u : used)
+          if (u.rfind("grad_", 0) == 0) throw NotImplemented("species gradients in outflow expressions");
+        const int i = sp.local;
+        qb << "    { const double T = " << expr::emit(of.e, vars) << ";\n";
+        qb << "#pragma unroll\n      for (int f = 0; f < NVF; ++f) " << R << "[f][" << i << "] += w * T * PHIF[q][f];\n";
+        qb << "      if (Jii) {\n";
+        for (const auto& pj : of.jac) {
+          const auto& wsp = m.species[pj.wrt];
+          std::string Jn;
+          int j = wsp.local, bcol;
+          if (wsp.comp == own) {
+            Jn = Jsame;
+            bcol = bown;
+          } else if (!boundary && wsp.comp == (inside ? cout : cin)) {
+            Jn = Jother;
+            bcol = bother;
+          } else {
+            continue;
+          }
+          qb << "        { const double dT = w * (" << expr::emit(pj.e, vars) << ");\n";
+          qb << "#pragma unroll\n          for (int f = 0; f < NVF; ++f)\n#pragma unroll\n            for (int gg = 0; gg < NVF; ++gg) "
+             << Jn << "[((f * NVF + gg) * " << std::max(bown, 1) << " + " << i << ") * " << std::max(bcol, 1) << " + " << j
+             << "] += dT * PHIF[q][f] * PHIF[q][gg]; }\n";
+        }
+        qb << "      }\n    }\n";
+      }
+    }
+    if (!inside) return;
+  };
+  qb << "    {\n";
+  side(true);
+  qb << "    }\n";
+  if (!boundary) {
+    qb << "    {\n";
+    side(false);
+    qb << "    }\n";
+  }
+  std::ostringstream o;
+  o << "namespace " << ns << " {\n";
+  o << "constexpr int NVF = " << nvf << ", NQF = " << nqf << ", BI = " << bi << ", BO = " << bo << ", FINT = " << fk.fint
+    << ", FDBL = " << fk.fdbl << ";\n";
+  o << "constexpr double FD_EPS = " << D(m.fd_epsilon) << ";\n";
+  o << "__device__ constexpr double PHIF[NQF][NVF] = " << array_init(phif) << ";\n";
+  o << "}\n";
+  std::string body = kFacet;
+  replace_all(body, "%F%", ns);
+  replace_all(body, "%KERNEL%", kname);
+  replace_all(body, "%QPBODY%", qb.str());
+  std::ostringstream full;
+  full << "#define NUMERICAL_JAC " << (numerical ? 1 : 0) << "\n" << o.str() << body << "#undef NUMERICAL_JAC\n";
+  return full.str();
+}
+
+}  // namespace
+
+GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L) {
+  GeneratedSource out;
+  std::string pre = kPrelude;
+  replace_all(pre, "%B%", std::to_string(L.B));
+  std::ostringstream src;
+  src << pre;
+  const bool numerical = spec.jacobian == JacobianType::Numerical && !spec.is_linear;
+  for (size_t c = 0; c < spec.comp_names.size(); ++c) {
+    if (spec.comp_species[c].empty()) {
+      out.volume_kernel.push_back("");
+      continue;
+    }
+    std::string ns = "vol" + std::to_string(c);
+    std::string kn = "dcg_volume_" + std::to_string(c);
+    src << gen_volume(spec, L.grid, static_cast<int>(c), ns, kn, numerical);
+    out.volume_kernel.push_back(kn);
+  }
+  // facet kernels for every (in, out, boundary) combination present
+  std::vector<std::tuple<int, int, bool>> combos;
+  for (const auto& F : L.facets) {
+    auto key = std::make_tuple(F.comp_in, F.comp_out, F.boundary);
+    bool seen = false;
+    for (auto& k : combos) seen = seen || k == key;
+    if (!seen) combos.push_back(key);
+  }
+  for (auto& [ci, co, bnd] : combos) {
+    FacetKernel fk;
+    fk.comp_in = ci;
+    fk.comp_out = co;
+    fk.boundary = bnd;
+    fk.name = "dcg_facet_" + std::to_string(ci) + "_" + std::to_string(co) + (bnd ? "_b" : "_i");
+    std::string ns = "fac" + std::to_string(ci) + "_" + std::to_string(co) + (bnd ? "b" : "i");
+    src << gen_facet(spec, L.grid, ci, co, bnd, ns, fk.name, numerical, fk);
+    out.facet_kernels.push_back(fk);
+  }
+  out.source = src.str();
+  return out;
+}
+
+}  // namespace dcopasi

@@ -1,0 +1,33 @@
+// Generates the HIP source of a model's assembly kernels (compiled once per
+// model with hipRTC).  One volume kernel per compartment restates
+// LocalOperator::localAssembleVolume / localAssembleAnalyticalJacobianVolume /
+// localAssembleNumericalJacobianVolume (local_operator.hh:417-491,541-765)
+// with the model's expressions inlined; one facet kernel per compartment pair
+// (and per compartment for domain-boundary outflow) restates
+// localAssembleSkeleton / localAssembleBoundary and their Jacobians
+// (local_operator.hh:777-1452) with exact derivatives in all four blocks.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "layout.hh"
+#include "spec.hh"
+
+namespace dcopasi {
+
+struct FacetKernel {
+  int comp_in, comp_out;
+  bool boundary;
+  std::string name;
+  int nvf, nqf, fint, fdbl;  // vertices, quadrature points, ints and doubles per facet record
+};
+
+struct GeneratedSource {
+  std::string source;
+  std::vector<std::string> volume_kernel;  // per compartment ("" when it has no species)
+  std::vector<FacetKernel> facet_kernels;
+};
+
+GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& layout);
+
+}  // namespace dcopasi

@@ -1,0 +1,122 @@
+// Structured grid, compartments, DOF layout and BCSR pattern.
+//
+// Mirrors what the reference sets up once per run:
+//   * the structured grid of `grid.cells` / `grid.extensions` / `grid.origin`,
+//     refined `grid.refinement_level` times; Kuhn simplices in 2D/3D as
+//     createSimplexGrid makes them (grid/make_multi_domain_grid.hh:76-91), or
+//     Q1 boxes with `grid.cell_type = cube`;
+//   * compartments by element centre (make_multi_domain_grid.hh:118-194);
+//   * per compartment, the vertices its elements touch, numbered in increasing
+//     vertex order, with the compartment's species interleaved per vertex
+//     ("EntityGrouping", model_single_compartment_traits.hh:26-27), and
+//     compartments in INI order (model_multi_compartment.impl.hh:63-83);
+//   * the Jacobian sparsity (local_operator.hh:276-399,
+//     model/make_step_operator.hh:295-401) as one BCSR matrix whose blocks
+//     are B x B with B the largest species count of any compartment.
+//
+// Device layout ("slot" = one vertex of one compartment):
+//   row_ptr[nslot+1], col[nnzb]      int32 BCSR structure, columns sorted
+//   val[nnzb][B][B]                  row-major blocks
+//   own_start[nslot], own_mask[nslot] position of the first same-compartment
+//                                    column and a 3^d-bit stencil mask, so an
+//                                    element finds its block (a,c) as
+//                                    own_start[a] + popcount(mask & ((1<<o)-1))
+//                                    with o the stencil offset of c from a.
+#pragma once
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "expr.hh"
+
+namespace dcopasi {
+
+struct Grid {
+  int dim = 2;
+  bool cube = false;  // Q1 boxes, else Kuhn simplices (lines in 1D)
+  std::array<long, 3> n{1, 1, 1};  // cells per axis (1 beyond dim)
+  std::array<double, 3> origin{0, 0, 0}, h{1, 1, 1};
+  long nvert = 0, ncell = 0;
+  int nclass = 1;   // simplices per cell (d!) or 1
+  int nvpe = 2;     // vertices per element
+  int ncorner = 2;  // vertices per cell (2^d)
+  // corner[cls][a]: cell-corner index (bit k = +1 along axis k) of local vertex a
+  std::vector<std::array<int, 8>> corner;
+
+  long vertex(long i, long j, long k) const { return i + (n[0] + 1) * (j + (n[1] + 1) * k); }
+  long cell_vertex(long cell, int c) const;  // global vertex of a cell corner
+  void cell_ijk(long cell, long ijk[3]) const;
+  void vertex_pos(long v, double x[3]) const;
+};
+
+struct Facet {
+  // one side: compartment, slots of the facet's vertices (nvf), and for
+  // boundary facets out == in.
+  int comp_in = -1, comp_out = -1;
+  bool boundary = false;
+  std::vector<int> slot_in, slot_out;
+  // per quadrature point: weight*|J_F|, position, unit normal (outward of in)
+  std::vector<double> qw, qx, qn;
+  std::vector<double> phi;  // [q][f] facet basis values
+  double volume = 0.0;
+};
+
+struct Compartment {
+  std::string name;
+  std::vector<int> species;  // global species indices, INI order
+  int b = 0;                 // species count
+  long nslot = 0, slot_off = 0, dof_off = 0;
+  std::vector<int> g2l;       // vertex -> local node or -1
+  std::vector<long> nodes;    // local node -> vertex
+  // assembly work list: cells touched by this compartment, sorted by colour
+  std::vector<int> cells;
+  std::vector<uint8_t> cell_mask;       // which Kuhn classes of the cell belong here
+  std::vector<int> cell_slots;          // [cell][corner] slot or -1
+  std::vector<int> color_ptr;           // colour ranges into cells
+};
+
+struct Layout {
+  Grid grid;
+  std::vector<Compartment> comps;
+  int B = 1;                 // block size
+  long nslot = 0, ndof = 0;  // ndof: packed DOF count (no padding)
+  std::vector<int> slot_comp;
+  std::vector<int> row_ptr, col;
+  std::vector<int> own_start;
+  std::vector<uint32_t> own_mask;
+  std::vector<int> diag_pos;  // position of the diagonal block per row
+  std::vector<Facet> facets;
+  std::vector<int> facet_color_ptr;  // facets sorted by colour
+  long nnzb() const { return static_cast<long>(col.size()); }
+};
+
+struct SpeciesDecl {
+  std::string name;
+  int comp;
+};
+
+// Build the grid from `grid.*` keys.
+Grid make_grid(int dim, const std::vector<double>& cells, const std::vector<double>& ext,
+               const std::vector<double>& origin, int refine, bool cube);
+
+// Assign elements to compartments by evaluating each compartment's
+// expression at element centres; returns per element (cell*nclass+cls) the
+// compartment or -1.  Overlapping compartments are rejected.
+std::vector<int8_t> mark_elements(const Grid& g, const std::vector<expr::Ex>& comp_expr);
+
+// Everything after marking: slots, work lists, colours, pattern, facets.
+// `need_facets[c]` tells whether species of compartment c have outflow terms
+// (boundary facets of c) and `interface` whether any interface term exists.
+Layout make_layout(const Grid& g, const std::vector<int8_t>& elem_comp, std::vector<Compartment> comps,
+                   bool boundary_facets, bool interface_facets);
+
+// quadrature on reference elements (order 2, as local_operator.hh:128-139)
+struct Rule {
+  std::vector<std::array<double, 3>> x;
+  std::vector<double> w;
+};
+Rule element_rule(int dim, bool cube);
+Rule facet_rule(int dim, bool cube);
+
+}  // namespace dcopasi

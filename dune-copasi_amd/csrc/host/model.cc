@@ -1,0 +1,714 @@
+#include "model.hh"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace dcopasi {
+
+// ------------------------------------------------------------------ RK tables
+// Stage r solves sum_{j<=r} a_rj M(u_j) + dt b_rj A(u_j, t + d_j dt) = 0, the
+// PDELab one-step form the reference configures (model/make_step_operator.hh:
+// 408-443); coefficients are the textbook ones of each scheme.
+RkTable rk_table(const std::string& type) {
+  RkTable t;
+  if (type == "ImplicitEuler") {
+    t.stages = 1;
+    t.a[0][0] = -1.0;
+    t.a[0][1] = 1.0;
+    t.b[0][1] = 1.0;
+    t.d[1] = 1.0;
+  } else if (type == "Alexander2") {
+    // Alexander (1977): 2-stage, L-stable SDIRK, alpha = 1 - sqrt(2)/2
+    const double al = 1.0 - std::sqrt(2.0) / 2.0;
+    t.stages = 2;
+    t.a[0][0] = -1.0;
+    t.a[0][1] = 1.0;
+    t.b[0][1] = al;
+    t.a[1][0] = -1.0;
+    t.a[1][2] = 1.0;
+    t.b[1][1] = 1.0 - al;
+    t.b[1][2] = al;
+    t.d[1] = al;
+    t.d[2] = 1.0;
+  } else {
+    throw NotImplemented("time stepping type '" + type + "' is not available on the device path");
+  }
+  return t;
+}
+
+// ------------------------------------------------------------------ set-up
+Model::Model(const std::string& ini, const std::string& overrides, int device) : device_(device) {
+  build(ini, overrides);
+  if (device_ >= 0) {
+    setup_device();
+    compile_and_load();
+  }
+}
+
+Model::~Model() {
+  if (stream_) {
+    (void)hipStreamSynchronize(stream_);
+    (void)hipStreamDestroy(stream_);
+  }
+}
+
+void Model::build(const std::string& ini, const std::string& overrides) {
+  cfg_tree_ = ParameterTree::from_ini(ini, overrides);
+  const auto& c = cfg_tree_;
+  const int dim = c.get("grid.dimension", 2);
+  if (c.has_key("grid.path")) throw NotImplemented("grid files (grid.path) are not read by this build");
+  spec_ = read_model(c, dim);
+  const std::string cell = c.get("grid.cell_type", std::string("simplex"));
+  if (cell != "simplex" && cell != "cube") throw IOError("grid.cell_type must be 'simplex' or 'cube'");
+  Grid g = make_grid(dim, c.get_list("grid.cells"), c.get_list("grid.extensions"), c.get_list("grid.origin"),
+                     c.get("grid.refinement_level", 0), cell == "cube");
+  auto elem_comp = mark_elements(g, spec_.comp_expr);
+  std::vector<Compartment> comps(spec_.comp_names.size());
+  for (size_t i = 0; i < comps.size(); ++i) {
+    comps[i].name = spec_.comp_names[i];
+    comps[i].species = spec_.comp_species[i];
+    comps[i].b = static_cast<int>(comps[i].species.size());
+  }
+  L_ = make_layout(g, elem_comp, std::move(comps), spec_.has_outflow_boundary, spec_.has_outflow_interface);
+  gen_ = generate_assembly(spec_, L_);
+
+  // solver configuration, keys of model/make_step_operator.hh:183-288
+  const std::string p = "model.time_step_operator.";
+  cfg_.rk = c.get(p + "type", std::string("Alexander2"));
+  rk_ = rk_table(cfg_.rk);
+  const std::string lin = c.get(p + "linear_solver.type", std::string("BiCGSTAB"));
+  if (lin != "BiCGSTAB")
+    cfg_.warnings.push_back("linear_solver.type '" + lin + "' runs as BiCGSTAB on the device");
+  std::string pre = c.get(p + "linear_solver.preconditioner.type", std::string("Jacobi"));
+  if (pre == "BlockJacobi") {
+    // block_jacobi.hh:46-81 with an ILU diagonal solver is ILU0 of the one block
+    const std::string inner = c.get(p + "linear_solver.preconditioner.diagonal_solver.preconditioner.type", std::string("ILU"));
+    pre = inner == "ILU" ? "ILU" : "Jacobi";
+  }
+  if (pre == "ILU" || pre == "ILU0") cfg_.precond = Preconditioner::ILU0;
+  else if (pre == "Jacobi") cfg_.precond = Preconditioner::Jacobi;
+  else if (pre == "None" || pre == "Richardson") cfg_.precond = Preconditioner::None;
+  else if (pre == "SSOR" || pre == "SOR" || pre == "GaussSeidel") {
+    cfg_.precond = Preconditioner::ILU0;
+    cfg_.warnings.push_back("preconditioner '" + pre + "' runs as block ILU0 on the device");
+  } else {
+    throw NotImplemented("preconditioner '" + pre + "'");
+  }
+  const std::string prec = c.get(p + "linear_solver.precision", std::string("fp64"));
+  if (prec == "fp64") cfg_.precision = Precision::FP64;
+  else if (prec == "mixed") cfg_.precision = Precision::Mixed;
+  else throw IOError("linear_solver.precision must be 'fp64' or 'mixed'");
+  cfg_.lin_rtol = c.get(p + "linear_solver.convergence_condition.relative_tolerance", 1e-4);
+  auto lr = c.get_list(p + "linear_solver.convergence_condition.iteration_range");
+  if (lr.size() == 2) cfg_.lin_maxit = static_cast<int>(lr[1]);
+  cfg_.ir_inner_rtol = c.get(p + "linear_solver.refinement.inner_relative_tolerance", 1e-3);
+  cfg_.ir_max = c.get(p + "linear_solver.refinement.max_steps", 50);
+  cfg_.nl_rtol = c.get(p + "nonlinear_solver.convergence_condition.relative_tolerance", 1e-4);
+  cfg_.nl_atol = c.get(p + "nonlinear_solver.convergence_condition.absolute_tolerance", 0.0);
+  auto nr = c.get_list(p + "nonlinear_solver.convergence_condition.iteration_range");
+  if (nr.size() == 2) {
+    cfg_.nl_min = static_cast<int>(nr[0]);
+    cfg_.nl_max = static_cast<int>(nr[1]);
+  }
+  const std::string norm = c.get(p + "nonlinear_solver.norm", std::string("l_2"));
+  if (norm == "l_2") cfg_.norm = 0;
+  else if (norm == "l_inf") cfg_.norm = 1;
+  else if (norm == "l_1") cfg_.norm = 2;
+  else throw IOError("Not known nonlinear norm solver of type '" + norm + "'");
+  time_ = c.get(p + "time_begin", 0.0);
+  t_end_ = c.get(p + "time_end", 1.0);
+  dt_initial_ = c.get(p + "time_step_initial", c.get(p + "time_step_max", 0.1));
+}
+
+void Model::require_device() const {
+  if (device_ < 0) throw DeviceError("model was created without a device");
+}
+
+void Model::synchronize() const {
+  if (stream_) DCG_HIP(hipStreamSynchronize(stream_));
+}
+
+void Model::setup_device() {
+  DCG_HIP(hipSetDevice(device_));
+  DCG_HIP(hipStreamCreate(&stream_));
+  const long ns = L_.nslot;
+  const int B = L_.B;
+  row_ptr_.upload(L_.row_ptr, stream_);
+  col_.upload(L_.col, stream_);
+  own_start_.upload(L_.own_start, stream_);
+  own_mask_.upload(L_.own_mask, stream_);
+  diag_pos_.upload(L_.diag_pos, stream_);
+  std::vector<int> sb(ns);
+  std::vector<long> sd(ns);
+  for (const auto& C : L_.comps)
+    for (long l = 0; l < C.nslot; ++l) {
+      sb[C.slot_off + l] = C.b;
+      sd[C.slot_off + l] = C.dof_off + l * C.b;
+    }
+  slot_b_.upload(sb, stream_);
+  slot_dof_.upload(sd, stream_);
+  // SpMV chunks: rows grouped until their blocks x B fill the LDS stage
+  std::vector<int> chunk{0};
+  long acc = 0;
+  for (long r = 0; r < ns; ++r) {
+    long len = static_cast<long>(L_.row_ptr[r + 1] - L_.row_ptr[r]) * B;
+    if (len > k::kSpmvStage) throw NotImplemented("BCSR row too long for the SpMV stage");
+    if (acc + len > k::kSpmvStage) {
+      chunk.push_back(static_cast<int>(r));
+      acc = 0;
+    }
+    acc += len;
+  }
+  if (chunk.back() != ns) chunk.push_back(static_cast<int>(ns));
+  chunk_row_.upload(chunk, stream_);
+  plan_.nchunk = static_cast<int>(chunk.size()) - 1;
+  plan_.chunk_row = chunk_row_.get();
+  plan_.rp = row_ptr_.get();
+  plan_.col = col_.get();
+  // per-compartment assembly work lists
+  comp_dev_.resize(L_.comps.size());
+  for (size_t c = 0; c < L_.comps.size(); ++c) {
+    comp_dev_[c].cells.upload(L_.comps[c].cells, stream_);
+    comp_dev_[c].cell_mask.upload(L_.comps[c].cell_mask, stream_);
+    comp_dev_[c].cell_slots.upload(L_.comps[c].cell_slots, stream_);
+  }
+  // facet records per facet kernel, grouped by colour
+  facet_dev_.resize(gen_.facet_kernels.size());
+  for (size_t k = 0; k < gen_.facet_kernels.size(); ++k) {
+    const auto& fk = gen_.facet_kernels[k];
+    std::vector<int> fi;
+    std::vector<double> fd;
+    auto& dev = facet_dev_[k];
+    int count = 0;
+    for (size_t col = 0; col + 1 < L_.facet_color_ptr.size(); ++col) {
+      int first = count;
+      for (int f = L_.facet_color_ptr[col]; f < L_.facet_color_ptr[col + 1]; ++f) {
+        const Facet& F = L_.facets[f];
+        if (F.comp_in != fk.comp_in || F.comp_out != fk.comp_out || F.boundary != fk.boundary) continue;
+        for (int v = 0; v < fk.nvf; ++v) fi.push_back(F.slot_in[v]);
+        for (int v = 0; v < fk.nvf; ++v) fi.push_back(F.boundary ? F.slot_in[v] : F.slot_out[v]);
+        auto pos = [&](int rs, int cs) -> int {
+          if (rs < 0 || cs < 0) return 0;
+          auto b = L_.col.begin() + L_.row_ptr[rs], e = L_.col.begin() + L_.row_ptr[rs + 1];
+          auto it = std::lower_bound(b, e, cs);
+          if (it == e || *it != cs) throw Error("internal: facet block missing from pattern");
+          return static_cast<int>(it - L_.col.begin());
+        };
+        const auto& si = F.slot_in;
+        const auto& so = F.boundary ? F.slot_in : F.slot_out;
+        for (int a = 0; a < fk.nvf; ++a)
+          for (int b = 0; b < fk.nvf; ++b) fi.push_back(pos(si[a], si[b]));
+        for (int a = 0; a < fk.nvf; ++a)
+          for (int b = 0; b < fk.nvf; ++b) fi.push_back(F.boundary ? 0 : pos(si[a], so[b]));
+        for (int a = 0; a < fk.nvf; ++a)
+          for (int b = 0; b < fk.nvf; ++b) fi.push_back(F.boundary ? 0 : pos(so[a], si[b]));
+        for (int a = 0; a < fk.nvf; ++a)
+          for (int b = 0; b < fk.nvf; ++b) fi.push_back(F.boundary ? 0 : pos(so[a], so[b]));
+        for (int q = 0; q < fk.nqf; ++q) fd.push_back(F.qw[q]);
+        for (int q = 0; q < fk.nqf; ++q)
+          for (int k2 = 0; k2 < 3; ++k2) fd.push_back(F.qx[3 * q + k2]);
+        for (int k2 = 0; k2 < 3; ++k2) fd.push_back(F.qn[k2]);
+        fd.push_back(F.volume);
+        ++count;
+      }
+      if (count > first) dev.color_ranges.push_back({first, count - first});
+    }
+    dev.fint.upload(fi, stream_);
+    dev.fdbl.upload(fd, stream_);
+  }
+  // lumped mass weights (integral of each hat function over its compartment)
+  {
+    std::vector<double> w(ns, 0.0);
+    Rule r = element_rule(L_.grid.dim, L_.grid.cube);
+    double det = 1.0;
+    for (int k2 = 0; k2 < L_.grid.dim; ++k2) det *= L_.grid.h[k2];
+    std::vector<double> wint(L_.grid.nvpe, 0.0);
+    for (size_t q = 0; q < r.w.size(); ++q) {
+      double x[3] = {r.x[q][0], r.x[q][1], r.x[q][2]};
+      // hat-function values on the reference element
+      for (int a = 0; a < L_.grid.nvpe; ++a) {
+        double phi;
+        if (L_.grid.cube) {
+          phi = 1.0;
+          for (int k2 = 0; k2 < L_.grid.dim; ++k2) phi *= ((a >> k2) & 1) ? x[k2] : 1.0 - x[k2];
+        } else {
+          double s = 0;
+          for (int k2 = 0; k2 < L_.grid.dim; ++k2) s += x[k2];
+          phi = a == 0 ? 1.0 - s : x[a - 1];
+        }
+        wint[a] += r.w[q] * det * phi;
+      }
+    }
+    for (const auto& C : L_.comps)
+      for (size_t i = 0; i < C.cells.size(); ++i)
+        for (int cls = 0; cls < L_.grid.nclass; ++cls)
+          if ((C.cell_mask[i] >> cls) & 1)
+            for (int a = 0; a < L_.grid.nvpe; ++a) w[C.cell_slots[i * L_.grid.ncorner + L_.grid.corner[cls][a]]] += wint[a];
+    weights_.upload(w, stream_);
+  }
+  const size_t nv = static_cast<size_t>(ns) * B;
+  u_.resize(nv);
+  hist_.resize(nv);
+  r_.resize(nv);
+  z_.resize(nv);
+  xs_.resize(nv);
+  packed_.resize(std::max<size_t>(1, static_cast<size_t>(L_.ndof)));
+  stage_.resize(rk_.stages + 1);
+  for (auto& s : stage_) s.resize(nv);
+  val_.resize(static_cast<size_t>(matrix_values()));
+  dinv_.resize(static_cast<size_t>(ns) * B * B);
+  for (auto& v : kv_) v.resize(nv);
+  if (cfg_.precision == Precision::Mixed) {
+    val32_.resize(static_cast<size_t>(matrix_values()));
+    for (auto& v : kf_) v.resize(nv);
+  }
+  const size_t nparts = std::max<size_t>(static_cast<size_t>(plan_.nchunk) + 8,
+                                         static_cast<size_t>((ns + 255) / 256) + 8);
+  part_.resize(std::max<size_t>(2 * nparts, 8 * 1024 + 64));
+  scalar_out_.resize(16);
+  sc_.resize(1);
+  status_.resize(4);
+  k::fill(u_.get(), 0.0, static_cast<long>(nv), stream_);
+  k::fill(status_.get(), 0, 4, stream_);
+  if (cfg_.precond == Preconditioner::ILU0) {
+    // level schedules of the block lower / upper triangular factors
+    std::vector<int> lev(ns, 0);
+    int nlev = 0;
+    for (long i = 0; i < ns; ++i) {
+      int l = 0;
+      for (int p2 = L_.row_ptr[i]; p2 < L_.diag_pos[i]; ++p2) l = std::max(l, lev[L_.col[p2]] + 1);
+      lev[i] = l;
+      nlev = std::max(nlev, l + 1);
+    }
+    std::vector<int> cnt(nlev + 1, 0), rows(ns);
+    for (long i = 0; i < ns; ++i) cnt[lev[i] + 1]++;
+    for (int l = 0; l < nlev; ++l) cnt[l + 1] += cnt[l];
+    lo_ptr_ = cnt;
+    for (long i = 0; i < ns; ++i) rows[cnt[lev[i]]++] = static_cast<int>(i);
+    lo_rows_.upload(rows, stream_);
+    std::fill(lev.begin(), lev.end(), 0);
+    nlev = 0;
+    for (long i = ns - 1; i >= 0; --i) {
+      int l = 0;
+      for (int p2 = L_.diag_pos[i] + 1; p2 < L_.row_ptr[i + 1]; ++p2) l = std::max(l, lev[L_.col[p2]] + 1);
+      lev[i] = l;
+      nlev = std::max(nlev, l + 1);
+    }
+    std::vector<int> cu(nlev + 1, 0), rows_u(ns);
+    for (long i = 0; i < ns; ++i) cu[lev[i] + 1]++;
+    for (int l = 0; l < nlev; ++l) cu[l + 1] += cu[l];
+    up_ptr_ = cu;
+    for (long i = 0; i < ns; ++i) rows_u[cu[lev[i]]++] = static_cast<int>(i);
+    up_rows_.upload(rows_u, stream_);
+    lu_.resize(static_cast<size_t>(matrix_values()));
+  }
+  synchronize();
+}
+
+void Model::compile_and_load() {
+  auto code = rtc_compile(gen_.source, "dcopasi_assembly");
+  module_.load(code);
+  vol_fn_.assign(L_.comps.size(), nullptr);
+  for (size_t c = 0; c < L_.comps.size(); ++c)
+    if (!gen_.volume_kernel[c].empty()) vol_fn_[c] = module_.function(gen_.volume_kernel[c]);
+  facet_fn_.clear();
+  for (const auto& fk : gen_.facet_kernels) facet_fn_.push_back(module_.function(fk.name));
+}
+
+// ------------------------------------------------------------------ assembly
+void Model::launch_assembly(const double* x, double t, double cm, double ca, double* r, double* val, int mode) {
+  const Grid& g = L_.grid;
+  int n0 = static_cast<int>(g.n[0]), n1 = g.dim > 1 ? static_cast<int>(g.n[1]) : 1;
+  double ox = g.origin[0], oy = g.origin[1], oz = g.origin[2];
+  double hx = g.h[0], hy = g.dim > 1 ? g.h[1] : 0.0, hz = g.dim > 2 ? g.h[2] : 0.0;
+  const int* own_start = own_start_.get();
+  const unsigned* own_mask = own_mask_.get();
+  for (size_t c = 0; c < L_.comps.size(); ++c) {
+    if (!vol_fn_[c]) continue;
+    const auto& C = L_.comps[c];
+    const int* cells = comp_dev_[c].cells.get();
+    const unsigned char* cmask = comp_dev_[c].cell_mask.get();
+    const int* cslots = comp_dev_[c].cell_slots.get();
+    for (size_t col = 0; col + 1 < C.color_ptr.size(); ++col) {
+      int first = C.color_ptr[col], count = C.color_ptr[col + 1] - first;
+      if (count <= 0) continue;
+      void* args[] = {&cells, &cmask, &cslots, &first, &count, &x, &t, &cm, &ca, &r, &val, &own_start,
+                      &own_mask, &n0, &n1, &ox, &oy, &oz, &hx, &hy, &hz, &mode};
+      const unsigned block = 128, grid = static_cast<unsigned>((count + block - 1) / block);
+      DCG_HIP(hipModuleLaunchKernel(vol_fn_[c], grid, 1, 1, block, 1, 1, 0, stream_, args, nullptr));
+    }
+  }
+  if (ca == 0.0) return;  // transmission terms belong to the stiffness operator
+  for (size_t k = 0; k < facet_fn_.size(); ++k) {
+    const int* fint = facet_dev_[k].fint.get();
+    const double* fdbl = facet_dev_[k].fdbl.get();
+    for (auto [first, count] : facet_dev_[k].color_ranges) {
+      void* args[] = {&fint, &fdbl, &first, &count, &x, &t, &ca, &r, &val, &mode};
+      const unsigned block = 64, grid = static_cast<unsigned>((count + block - 1) / block);
+      DCG_HIP(hipModuleLaunchKernel(facet_fn_[k], grid, 1, 1, block, 1, 1, 0, stream_, args, nullptr));
+    }
+  }
+}
+
+double Model::residual_norm(const double* r) {
+  const long n = L_.nslot * L_.B;
+  k::reduce(n, r, nullptr, cfg_.norm == 0 ? 0 : cfg_.norm == 1 ? 2 : 1, part_.get(), scalar_out_.get(), stream_);
+  double v = 0;
+  DCG_HIP(hipMemcpyAsync(&v, scalar_out_.get(), sizeof(double), hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  return v;  // two_norm2 for l_2 (make_step_operator.hh:274-276)
+}
+
+// ------------------------------------------------------------------ Krylov
+void Model::precond_setup() {
+  const int B = L_.B;
+  const int ns = static_cast<int>(L_.nslot);
+  if (cfg_.precond == Preconditioner::Jacobi) {
+    k::block_inverse(B, ns, val_.get(), diag_pos_.get(), dinv_.get(), status_.get(), stream_);
+  } else if (cfg_.precond == Preconditioner::ILU0) {
+    DCG_HIP(hipMemcpyAsync(lu_.get(), val_.get(), val_.bytes(), hipMemcpyDeviceToDevice, stream_));
+    for (size_t l = 0; l + 1 < lo_ptr_.size(); ++l)
+      k::ilu_factor(B, lo_ptr_[l + 1] - lo_ptr_[l], lo_rows_.get() + lo_ptr_[l], row_ptr_.get(), col_.get(),
+                    diag_pos_.get(), lu_.get(), dinv_.get(), status_.get(), stream_);
+  }
+}
+
+int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, int maxit, int* its) {
+  const int B = L_.B;
+  const int ns = static_cast<int>(L_.nslot);
+  const long n = static_cast<long>(ns) * B;
+  double *r = kv_[0].get(), *rh = kv_[1].get(), *p = kv_[2].get(), *v = kv_[3].get(), *ph = kv_[4].get(),
+         *s = kv_[5].get(), *sh = kv_[6].get(), *t = kv_[7].get(), *tmp = kv_[8].get();
+  k::BicgScalars host{};
+  host.tol = rtol;
+  host.abstol = abstol;
+  host.maxit = maxit;
+  DCG_HIP(hipMemcpyAsync(sc_.get(), &host, sizeof host, hipMemcpyHostToDevice, stream_));
+  const int nblk = (ns + 255) / 256;
+  const int nred = std::max(1, std::min(1024, static_cast<int>((n + 255) / 256)));
+  k::bicg_init<double>(n, b, x, r, rh, p, v, part_.get(), nred, sc_.get(), stream_);
+  const double* dinv = cfg_.precond == Preconditioner::Jacobi ? dinv_.get() : nullptr;
+  const bool ilu = cfg_.precond == Preconditioner::ILU0;
+  const int* skip = &sc_.get()->done;
+  auto apply_ilu = [&](const double* in, double* out) {
+    for (size_t l = 0; l + 1 < lo_ptr_.size(); ++l)
+      k::ilu_lower<double>(B, lo_ptr_[l + 1] - lo_ptr_[l], lo_rows_.get() + lo_ptr_[l], row_ptr_.get(), col_.get(),
+                           diag_pos_.get(), lu_.get(), in, tmp, sc_.get(), stream_);
+    for (size_t l = 0; l + 1 < up_ptr_.size(); ++l)
+      k::ilu_upper<double>(B, up_ptr_[l + 1] - up_ptr_[l], up_rows_.get() + up_ptr_[l], row_ptr_.get(), col_.get(),
+                           diag_pos_.get(), lu_.get(), dinv_.get(), tmp, out, sc_.get(), stream_);
+  };
+  int done_its = 0, batch = std::max(2, std::min(last_krylov_its_, 64));
+  for (;;) {
+    for (int it = 0; it < batch; ++it) {
+      k::bicg_p<double>(B, ns, r, p, v, ph, dinv, sc_.get(), stream_);
+      if (ilu) apply_ilu(p, ph);
+      k::spmv(B, plan_, val_.get(), ph, v, rh, nullptr, part_.get(), skip, stream_);
+      k::bicg_fin(plan_.nchunk, part_.get(), 2, 1, sc_.get(), stream_);
+      k::bicg_s<double>(B, ns, r, v, s, sh, dinv, sc_.get(), part_.get(), stream_);
+      if (ilu) apply_ilu(s, sh);
+      k::bicg_fin(nblk, part_.get(), 1, 2, sc_.get(), stream_);
+      k::spmv(B, plan_, val_.get(), sh, t, s, t, part_.get(), skip, stream_);
+      k::bicg_fin(plan_.nchunk, part_.get(), 2, 3, sc_.get(), stream_);
+      k::bicg_xr<double>(B, ns, x, ph, sh, s, t, r, rh, sc_.get(), part_.get(), stream_);
+      k::bicg_fin(nblk, part_.get(), 2, 4, sc_.get(), stream_);
+    }
+    done_its += batch;
+    DCG_HIP(hipMemcpyAsync(&host, sc_.get(), sizeof host, hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    if (host.done) break;
+    if (done_its > maxit + 4) {
+      host.done = 2;
+      break;
+    }
+    batch = 4;
+  }
+  *its = host.iter;
+  last_krylov_its_ = std::max(2, host.iter);
+  return host.done == 1 ? 0 : host.done;
+}
+
+int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* its) {
+  const int B = L_.B;
+  const int ns = static_cast<int>(L_.nslot);
+  const long n = static_cast<long>(ns) * B;
+  float *r = kf_[0].get(), *rh = kf_[1].get(), *p = kf_[2].get(), *v = kf_[3].get(), *ph = kf_[4].get(),
+        *s = kf_[5].get(), *sh = kf_[6].get(), *t = kf_[7].get(), *tmp = kf_[8].get();
+  k::BicgScalars host{};
+  host.tol = rtol;
+  host.abstol = 0.0;
+  host.maxit = maxit;
+  DCG_HIP(hipMemcpyAsync(sc_.get(), &host, sizeof host, hipMemcpyHostToDevice, stream_));
+  const int nblk = (ns + 255) / 256;
+  const int nred = std::max(1, std::min(1024, static_cast<int>((n + 255) / 256)));
+  k::bicg_init<float>(n, b, x, r, rh, p, v, part_.get(), nred, sc_.get(), stream_);
+  const double* dinv = cfg_.precond == Preconditioner::Jacobi ? dinv_.get() : nullptr;
+  const bool ilu = cfg_.precond == Preconditioner::ILU0;
+  const int* skip = &sc_.get()->done;
+  auto apply_ilu = [&](const float* in, float* out) {
+    for (size_t l = 0; l + 1 < lo_ptr_.size(); ++l)
+      k::ilu_lower<float>(B, lo_ptr_[l + 1] - lo_ptr_[l], lo_rows_.get() + lo_ptr_[l], row_ptr_.get(), col_.get(),
+                          diag_pos_.get(), lu_.get(), in, tmp, sc_.get(), stream_);
+    for (size_t l = 0; l + 1 < up_ptr_.size(); ++l)
+      k::ilu_upper<float>(B, up_ptr_[l + 1] - up_ptr_[l], up_rows_.get() + up_ptr_[l], row_ptr_.get(), col_.get(),
+                          diag_pos_.get(), lu_.get(), dinv_.get(), tmp, out, sc_.get(), stream_);
+  };
+  int done_its = 0, batch = std::max(2, std::min(last_krylov_its_, 64));
+  for (;;) {
+    for (int it = 0; it < batch; ++it) {
+      k::bicg_p<float>(B, ns, r, p, v, ph, dinv, sc_.get(), stream_);
+      if (ilu) apply_ilu(p, ph);
+      k::spmv(B, plan_, val32_.get(), ph, v, rh, nullptr, part_.get(), skip, stream_);
+      k::bicg_fin(plan_.nchunk, part_.get(), 2, 1, sc_.get(), stream_);
+      k::bicg_s<float>(B, ns, r, v, s, sh, dinv, sc_.get(), part_.get(), stream_);
+      if (ilu) apply_ilu(s, sh);
+      k::bicg_fin(nblk, part_.get(), 1, 2, sc_.get(), stream_);
+      k::spmv(B, plan_, val32_.get(), sh, t, s, t, part_.get(), skip, stream_);
+      k::bicg_fin(plan_.nchunk, part_.get(), 2, 3, sc_.get(), stream_);
+      k::bicg_xr<float>(B, ns, x, ph, sh, s, t, r, rh, sc_.get(), part_.get(), stream_);
+      k::bicg_fin(nblk, part_.get(), 2, 4, sc_.get(), stream_);
+    }
+    done_its += batch;
+    DCG_HIP(hipMemcpyAsync(&host, sc_.get(), sizeof host, hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    if (host.done) break;
+    if (done_its > maxit + 4) {
+      host.done = 2;
+      break;
+    }
+    batch = 4;
+  }
+  *its = host.iter;
+  last_krylov_its_ = std::max(2, host.iter);
+  return host.done == 1 ? 0 : host.done;
+}
+
+// Solve val_ * x = b to relative tolerance rtol.  FP64: one preconditioned
+// BiCGSTAB.  Mixed: FP32 BiCGSTAB on an FP32 copy of the matrix inside FP64
+// iterative refinement; the refinement residual b - A x is FP64.
+int Model::krylov(const double* b, double* x, double rtol, int maxit, int* its, int* refinements) {
+  precond_setup();
+  *its = 0;
+  *refinements = 0;
+  const long n = L_.nslot * L_.B;
+  if (cfg_.precision == Precision::FP64) return bicgstab64(b, x, rtol, 0.0, maxit, its);
+  k::to_float(matrix_values(), val_.get(), val32_.get(), stream_);
+  double* res = kv_[8].get();  // FP64 refinement residual
+  double* ax = kv_[7].get();
+  float* d = kf_[9].get();
+  k::fill(x, 0.0, n, stream_);
+  DCG_HIP(hipMemcpyAsync(res, b, n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  k::reduce(n, b, nullptr, 0, part_.get(), scalar_out_.get(), stream_);
+  double bn2 = 0;
+  DCG_HIP(hipMemcpyAsync(&bn2, scalar_out_.get(), sizeof(double), hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  const double target = rtol * std::sqrt(bn2);
+  if (bn2 == 0.0) return 0;
+  double rn = std::sqrt(bn2);
+  for (int k2 = 0; k2 < cfg_.ir_max; ++k2) {
+    // inner tolerance relative to the current refinement residual, never
+    // tighter than what reaches the outer target
+    double inner = std::max(cfg_.ir_inner_rtol, 0.5 * target / rn);
+    int it = 0;
+    int rc = bicgstab32(res, d, std::min(inner, 0.5), maxit, &it);
+    *its += it;
+    *refinements += 1;
+    if (rc == 3) return 3;
+    // x += d (FP64), res = b - A x (FP64)
+    k::to_double(n, d, ax, stream_);
+    k::axpy(n, 1.0, ax, x, stream_);
+    k::spmv(L_.B, plan_, val_.get(), x, ax, nullptr, nullptr, nullptr, nullptr, stream_);
+    DCG_HIP(hipMemcpyAsync(res, b, n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    k::axpy(n, -1.0, ax, res, stream_);
+    k::reduce(n, res, nullptr, 0, part_.get(), scalar_out_.get(), stream_);
+    double r2 = 0;
+    DCG_HIP(hipMemcpyAsync(&r2, scalar_out_.get(), sizeof(double), hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    rn = std::sqrt(r2);
+    if (!std::isfinite(rn)) return 3;
+    if (rn <= target) return 0;
+  }
+  return 2;
+}
+
+// ------------------------------------------------------------------ time step
+int Model::step(double t, double dt, StepStats* st) {
+  require_device();
+  StepStats local;
+  StepStats& S = st ? *st : local;
+  S = StepStats{};
+  const long n = L_.nslot * L_.B;
+  const size_t bytes = static_cast<size_t>(n) * sizeof(double);
+  DCG_HIP(hipMemcpyAsync(stage_[0].get(), u_.get(), bytes, hipMemcpyDeviceToDevice, stream_));
+  for (int s = 1; s <= rk_.stages; ++s) {
+    S.stages = s;
+    // history of earlier stages: sum_{j<s} a_sj M(u_j) + dt b_sj A(u_j)
+    k::fill(hist_.get(), 0.0, n, stream_);
+    for (int j = 0; j < s; ++j) {
+      const double am = rk_.a[s - 1][j], ab = dt * rk_.b[s - 1][j];
+      if (am == 0.0 && ab == 0.0) continue;
+      launch_assembly(stage_[j].get(), t + rk_.d[j] * dt, am, ab, hist_.get(), nullptr, 1);
+    }
+    const double cm = rk_.a[s - 1][s], ca = dt * rk_.b[s - 1][s], ts = t + rk_.d[s] * dt;
+    double* x = xs_.get();
+    DCG_HIP(hipMemcpyAsync(x, stage_[s - 1].get(), bytes, hipMemcpyDeviceToDevice, stream_));
+    auto residual = [&]() {
+      DCG_HIP(hipMemcpyAsync(r_.get(), hist_.get(), bytes, hipMemcpyDeviceToDevice, stream_));
+      launch_assembly(x, ts, cm, ca, r_.get(), nullptr, 1);
+    };
+    auto jacobian = [&]() {
+      k::init_jacobian(val_.get(), matrix_values(), diag_pos_.get(), slot_b_.get(), static_cast<int>(L_.nslot), L_.B,
+                       stream_);
+      launch_assembly(x, ts, cm, ca, nullptr, val_.get(), 2);
+    };
+    auto solve = [&]() -> int {
+      int its = 0, refs = 0;
+      int rc = krylov(r_.get(), z_.get(), cfg_.lin_rtol, cfg_.lin_maxit, &its, &refs);
+      S.linear_iterations += its;
+      S.refinement_steps += refs;
+      return rc;
+    };
+    if (spec_.is_linear) {
+      // linear defect correction (make_step_operator.hh:210-245)
+      residual();
+      jacobian();
+      int rc = solve();
+      if (rc) {
+        S.status = rc == 3 ? 3 : rc == 4 ? 4 : 2;
+        return S.status;
+      }
+      k::axpy(n, -1.0, z_.get(), x, stream_);
+      S.newton_iterations += 1;
+    } else {
+      residual();
+      double nrm0 = residual_norm(r_.get()), nrm = nrm0;
+      int it = 0;
+      for (;;) {
+        S.residual_norm = nrm;
+        if (!std::isfinite(nrm)) {
+          S.status = 3;
+          return 3;
+        }
+        const bool conv = nrm <= cfg_.nl_atol || nrm <= cfg_.nl_rtol * nrm0 || nrm == 0.0;
+        if (conv && it >= cfg_.nl_min) break;
+        if (it >= cfg_.nl_max) {
+          S.status = 1;
+          return 1;
+        }
+        jacobian();
+        int rc = solve();
+        if (rc) {
+          S.status = rc == 3 ? 3 : rc == 4 ? 4 : 2;
+          return S.status;
+        }
+        k::axpy(n, -1.0, z_.get(), x, stream_);
+        ++it;
+        residual();
+        nrm = residual_norm(r_.get());
+      }
+      S.newton_iterations += it;
+    }
+    DCG_HIP(hipMemcpyAsync(stage_[s].get(), x, bytes, hipMemcpyDeviceToDevice, stream_));
+  }
+  DCG_HIP(hipMemcpyAsync(u_.get(), stage_[rk_.stages].get(), bytes, hipMemcpyDeviceToDevice, stream_));
+  time_ = t + dt;
+  synchronize();
+  return 0;
+}
+
+// ------------------------------------------------------------------ state I/O
+void Model::set_state(const double* packed, double t) {
+  require_device();
+  if (L_.ndof > 0)
+    DCG_HIP(hipMemcpyAsync(packed_.get(), packed, L_.ndof * sizeof(double), hipMemcpyHostToDevice, stream_));
+  k::unpack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), packed_.get(), u_.get(), stream_);
+  time_ = t;
+  synchronize();
+}
+
+void Model::get_state(double* packed, double* t) {
+  require_device();
+  k::pack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), u_.get(), packed_.get(), stream_);
+  if (L_.ndof > 0)
+    DCG_HIP(hipMemcpyAsync(packed, packed_.get(), L_.ndof * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  if (t) *t = time_;
+}
+
+void Model::initial_state(double* packed) const {
+  std::map<std::string, int> slots = {{"position_x", 0}, {"position_y", 1}, {"position_z", 2}, {"time", 3}};
+  for (const auto& sp : spec_.species) slots[sp.name] = 4;  // species values are not defined yet: 0
+  for (size_t s = 0; s < spec_.species.size(); ++s) {
+    const auto& sp = spec_.species[s];
+    const auto& C = L_.comps[sp.comp];
+    expr::Bound f;
+    bool have = spec_.initial[s] != nullptr;
+    if (have) f = expr::bind(spec_.initial[s], slots);
+    for (long l = 0; l < C.nslot; ++l) {
+      double v[5] = {0, 0, 0, time_, 0};
+      L_.grid.vertex_pos(C.nodes[l], v);
+      packed[C.dof_off + l * C.b + sp.local] = have ? f(v) : 0.0;
+    }
+  }
+}
+
+void Model::mass(double* out) {
+  require_device();
+  for (const auto& C : L_.comps) {
+    if (!C.b || !C.nslot) continue;
+    k::weighted_species_sum(C.slot_off, C.nslot, L_.B, weights_.get(), u_.get(), part_.get(), scalar_out_.get(),
+                            stream_);
+    double sums[8];
+    DCG_HIP(hipMemcpyAsync(sums, scalar_out_.get(), sizeof sums, hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    for (int i = 0; i < C.b; ++i) out[C.species[i]] = sums[i];
+  }
+}
+
+// ------------------------------------------------------------------ test entry points
+void Model::assemble(const double* x_packed, double t, double cm, double ca, double* r_packed, double* jac) {
+  require_device();
+  const long n = L_.nslot * L_.B;
+  DCG_HIP(hipMemcpyAsync(packed_.get(), x_packed, L_.ndof * sizeof(double), hipMemcpyHostToDevice, stream_));
+  k::unpack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), packed_.get(), xs_.get(), stream_);
+  k::fill(r_.get(), 0.0, n, stream_);
+  int mode = 1;
+  if (jac) {
+    k::init_jacobian(val_.get(), matrix_values(), diag_pos_.get(), slot_b_.get(), static_cast<int>(L_.nslot), L_.B,
+                     stream_);
+    mode = 3;
+  }
+  launch_assembly(xs_.get(), t, cm, ca, r_.get(), jac ? val_.get() : nullptr, mode);
+  k::pack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), r_.get(), packed_.get(), stream_);
+  DCG_HIP(hipMemcpyAsync(r_packed, packed_.get(), L_.ndof * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  if (jac) DCG_HIP(hipMemcpyAsync(jac, val_.get(), val_.bytes(), hipMemcpyDeviceToHost, stream_));
+  synchronize();
+}
+
+void Model::spmv(const double* jac, const double* x_packed, double* y_packed) {
+  require_device();
+  DCG_HIP(hipMemcpyAsync(val_.get(), jac, val_.bytes(), hipMemcpyHostToDevice, stream_));
+  DCG_HIP(hipMemcpyAsync(packed_.get(), x_packed, L_.ndof * sizeof(double), hipMemcpyHostToDevice, stream_));
+  k::unpack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), packed_.get(), xs_.get(), stream_);
+  k::spmv(L_.B, plan_, val_.get(), xs_.get(), r_.get(), nullptr, nullptr, nullptr, nullptr, stream_);
+  k::pack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), r_.get(), packed_.get(), stream_);
+  DCG_HIP(hipMemcpyAsync(y_packed, packed_.get(), L_.ndof * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  synchronize();
+}
+
+int Model::linear_solve(const double* jac, const double* b_packed, double* x_packed, double rtol, int maxit,
+                        int* its) {
+  require_device();
+  DCG_HIP(hipMemcpyAsync(val_.get(), jac, val_.bytes(), hipMemcpyHostToDevice, stream_));
+  DCG_HIP(hipMemcpyAsync(packed_.get(), b_packed, L_.ndof * sizeof(double), hipMemcpyHostToDevice, stream_));
+  k::unpack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), packed_.get(), r_.get(), stream_);
+  int refs = 0;
+  int rc = krylov(r_.get(), z_.get(), rtol, maxit, its, &refs);
+  k::pack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), z_.get(), packed_.get(), stream_);
+  DCG_HIP(hipMemcpyAsync(x_packed, packed_.get(), L_.ndof * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  return rc;
+}
+
+}  // namespace dcopasi

@@ -1,0 +1,153 @@
+// A diffusion-reaction model resident on one GPU: the drop-in for the
+// reference's ModelSingleCompartment / ModelMultiCompartment step operator
+// (model/diffusion_reaction/model_*_compartment.impl.hh, make_step_operator
+// in model/make_step_operator.hh:159-444).  One Model owns its mesh tables,
+// BCSR matrix, state, work vectors and generated kernels; `step` is
+// PDELab::OneStep::apply for one time step of the configured Runge-Kutta
+// scheme with Newton (or one defect correction for linear models) per stage.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../kernels/kernels.hh"
+#include "codegen.hh"
+#include "config.hh"
+#include "device.hh"
+#include "layout.hh"
+#include "rtc.hh"
+#include "spec.hh"
+
+namespace dcopasi {
+
+struct StepStats {
+  int status = 0;  // 0 ok, 1 Newton not converged, 2 Krylov not converged, 3 non-finite, 4 breakdown
+  int stages = 0;
+  int newton_iterations = 0;
+  int linear_iterations = 0;
+  int refinement_steps = 0;
+  double residual_norm = 0.0;  // last Newton residual in the configured norm
+};
+
+struct RkTable {
+  int stages = 1;
+  double a[3][4] = {{0}}, b[3][4] = {{0}}, d[4] = {0};
+};
+RkTable rk_table(const std::string& type);
+
+enum class Preconditioner { None, Jacobi, ILU0 };
+enum class Precision { FP64, Mixed };
+
+struct SolverConfig {
+  std::string rk = "Alexander2";
+  Preconditioner precond = Preconditioner::Jacobi;
+  Precision precision = Precision::FP64;
+  double lin_rtol = 1e-4;
+  int lin_maxit = 500;
+  double nl_rtol = 1e-4, nl_atol = 0.0;
+  int nl_min = 0, nl_max = 40;
+  int norm = 0;  // 0: two_norm2, 1: l_inf, 2: l_1
+  double ir_inner_rtol = 1e-3;  // FP32 Krylov tolerance inside refinement
+  int ir_max = 50;
+  std::vector<std::string> warnings;
+};
+
+class Model {
+ public:
+  Model(const std::string& ini, const std::string& overrides, int device);
+  ~Model();
+  Model(const Model&) = delete;
+  Model& operator=(const Model&) = delete;
+
+  // sizes and layout
+  size_t n_dof() const { return static_cast<size_t>(L_.ndof); }
+  size_t n_species() const { return spec_.species.size(); }
+  size_t n_compartments() const { return spec_.comp_names.size(); }
+  const Layout& layout() const { return L_; }
+  const ModelSpec& spec() const { return spec_; }
+  const SolverConfig& solver() const { return cfg_; }
+  const std::string& source() const { return gen_.source; }
+  bool on_device() const { return device_ >= 0; }
+  double time() const { return time_; }
+  double time_step() const { return dt_initial_; }
+  double time_end() const { return t_end_; }
+
+  void set_state(const double* packed, double t);
+  void get_state(double* packed, double* t);
+  void initial_state(double* packed) const;  // from initial.expression keys
+  int step(double t, double dt, StepStats* st);
+  void mass(double* per_species);
+
+  // kernel-level entry points (parity tests)
+  void assemble(const double* x_packed, double t, double cm, double ca, double* r_packed, double* jac);
+  void spmv(const double* jac, const double* x_packed, double* y_packed);
+  int linear_solve(const double* jac, const double* b_packed, double* x_packed, double rtol, int maxit, int* its);
+
+  // device-side pieces used by batching/benchmarks
+  hipStream_t stream() const { return stream_; }
+  void synchronize() const;
+  long matrix_values() const { return static_cast<long>(L_.nnzb()) * L_.B * L_.B; }
+
+ private:
+  void build(const std::string& ini, const std::string& overrides);
+  void setup_device();
+  void require_device() const;
+  // assembly: r (+)= cm*R^M(x) + ca*R^A(x) and/or its Jacobian into val_
+  void launch_assembly(const double* x, double t, double cm, double ca, double* r, double* val, int mode);
+  double residual_norm(const double* r);
+  // linear solves on the current val_
+  int krylov(const double* b, double* x, double rtol, int maxit, int* its, int* refinements);
+  int bicgstab64(const double* b, double* x, double rtol, double abstol, int maxit, int* its);
+  int bicgstab32(const double* b, float* x, double rtol, int maxit, int* its);
+  void precond_setup();
+  void compile_and_load();
+
+  ParameterTree cfg_tree_;
+  ModelSpec spec_;
+  Layout L_;
+  GeneratedSource gen_;
+  SolverConfig cfg_;
+  RkTable rk_;
+  int device_ = -1;
+  hipStream_t stream_ = nullptr;
+  RtcModule module_;
+  std::vector<hipFunction_t> vol_fn_;
+  std::vector<hipFunction_t> facet_fn_;
+  double time_ = 0.0, dt_initial_ = 0.0, t_end_ = 0.0;
+  int last_krylov_its_ = 8;
+
+  // device data
+  DeviceArray<int> row_ptr_, col_, own_start_, diag_pos_, slot_b_, chunk_row_;
+  DeviceArray<unsigned> own_mask_;
+  DeviceArray<long> slot_dof_;
+  struct CompDev {
+    DeviceArray<int> cells, cell_slots;
+    DeviceArray<unsigned char> cell_mask;
+  };
+  std::vector<CompDev> comp_dev_;
+  struct FacetDev {
+    DeviceArray<int> fint;
+    DeviceArray<double> fdbl;
+    std::vector<std::pair<int, int>> color_ranges;  // [first, count) per colour
+  };
+  std::vector<FacetDev> facet_dev_;
+  DeviceArray<double> weights_;  // lumped mass per slot (mass sums)
+  DeviceArray<double> u_, hist_, r_, z_, xs_, packed_;
+  std::vector<DeviceArray<double>> stage_;
+  DeviceArray<double> val_, lu_, dinv_;
+  DeviceArray<float> val32_;
+  DeviceArray<double> kv_[9];  // r rh p v ph s sh t tmp
+  DeviceArray<float> kf_[10];  // same in FP32 + x
+  DeviceArray<double> part_, scalar_out_;
+  DeviceArray<k::BicgScalars> sc_;
+  DeviceArray<int> status_;
+  k::SpmvPlan plan_;
+  // ILU level schedules (host copies of the level pointers, device row lists)
+  std::vector<int> lo_ptr_, up_ptr_;
+  DeviceArray<int> lo_rows_, up_rows_;
+  bool ilu_ready_ = false;
+};
+
+}  // namespace dcopasi

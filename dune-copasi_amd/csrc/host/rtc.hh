@@ -1,0 +1,28 @@
+// Runtime compilation of a model's generated assembly kernels with hipRTC,
+// with an on-disk cache of code objects keyed by a hash of source + options.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+namespace dcopasi {
+
+// Compile `source` for gfx950; returns the code object bytes.
+std::vector<char> rtc_compile(const std::string& source, const std::string& name);
+
+class RtcModule {
+ public:
+  RtcModule() = default;
+  RtcModule(const RtcModule&) = delete;
+  RtcModule& operator=(const RtcModule&) = delete;
+  ~RtcModule();
+  void load(const std::vector<char>& code);
+  hipFunction_t function(const std::string& name) const;
+  bool loaded() const { return mod_ != nullptr; }
+
+ private:
+  hipModule_t mod_ = nullptr;
+};
+
+}  // namespace dcopasi

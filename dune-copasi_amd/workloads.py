@@ -1,0 +1,280 @@
+"""Benchmark and parity workloads as reference-style INI text plus seeded
+nodal initial states.
+
+Each workload is the model of one benchmark configuration written with the
+reference's own keys (model.scalar_field.*, compartments.*, grid.*,
+model.time_step_operator.*).  Two keys are additions of this build:
+`grid.cell_type = cube` (Q1 cells; the reference builds simplices) and
+`model.time_step_operator.linear_solver.precision = mixed`.
+
+Random initial data come from a counter-based SplitMix64 stream keyed by
+(seed, stream, global vertex id), so the same vertex gets the same value on
+any layout, device or process.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+_M64 = (1 << 64) - 1
+
+
+def splitmix64(x: np.ndarray) -> np.ndarray:
+    """SplitMix64 finaliser applied to uint64 counters (Steele et al. 2014)."""
+    z = (x.astype(np.uint64) + np.uint64(0x9E3779B97F4A7C15))
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def uniform01(seed: int, stream: int, ids: np.ndarray) -> np.ndarray:
+    key = (np.uint64((seed * 0x100000001B3) & _M64) ^ np.uint64((stream << 40) & _M64))
+    z = splitmix64(np.asarray(ids, dtype=np.uint64) ^ key)
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    ini: str
+    dt: float
+    steps: int
+    dim: int
+    cells: tuple
+    init: callable  # (vertex coordinates per compartment, vertex ids) -> packed state
+    stages: int = 1
+
+    def initial_state(self, model) -> np.ndarray:
+        """Packed initial state for a model handle (product or oracle wrapper)."""
+        return self.init(model)
+
+
+def vertex_coords(ids: np.ndarray, cells, ext=None, origin=None) -> np.ndarray:
+    d = len(cells)
+    ext = ext or (1.0,) * d
+    origin = origin or (0.0,) * d
+    ids = np.asarray(ids, dtype=np.int64)
+    out = np.zeros((ids.size, 3))
+    rem = ids.copy()
+    for k in range(d):
+        n = cells[k] + 1
+        idx = rem % n
+        rem //= n
+        out[:, k] = origin[k] + ext[k] * idx / cells[k]
+    return out
+
+
+def _solver_block(rk, precond, nl_rtol, nl_atol, lin_rtol, precision="fp64", maxit=2000):
+    pre = ("linear_solver.preconditioner.type = BlockJacobi\n"
+           "linear_solver.preconditioner.diagonal_solver.type = Loop\n"
+           "linear_solver.preconditioner.diagonal_solver.preconditioner.type = ILU\n"
+           "linear_solver.preconditioner.diagonal_solver.convergence_condition.iteration_range = 1 1\n"
+           if precond == "ILU" else f"linear_solver.preconditioner.type = {precond}\n")
+    return (f"[model.time_step_operator]\n"
+            f"type = {rk}\n"
+            f"linear_solver.type = BiCGSTAB\n"
+            f"{pre}"
+            f"linear_solver.convergence_condition.relative_tolerance = {lin_rtol:g}\n"
+            f"linear_solver.convergence_condition.iteration_range = 1 {maxit}\n"
+            f"linear_solver.precision = {precision}\n"
+            f"nonlinear_solver.convergence_condition.relative_tolerance = {nl_rtol:g}\n"
+            f"nonlinear_solver.convergence_condition.absolute_tolerance = {nl_atol:g}\n"
+            f"nonlinear_solver.convergence_condition.iteration_range = 0 40\n")
+
+
+# --------------------------------------------------------------------- Gray-Scott
+def gray_scott(n=128, cell_type="cube", seed=0, F=0.04, k=0.06, Du=2e-5, Dv=1e-5, precond="ILU",
+               nl_rtol=1e-10, nl_atol=0.0, lin_rtol=1e-10, precision="fp64", steps=200, dt=1.0) -> Workload:
+    """Gray-Scott on [0,1]^2, zero flux, implicit Euler (benchmark config 0 at n = 128)."""
+    ini = f"""[grid]
+dimension = 2
+cells = {n} {n}
+extensions = 1 1
+cell_type = {cell_type}
+
+[compartments]
+domain.expression = 1
+
+[model.scalar_field.u]
+compartment = domain
+storage.expression = 1
+cross_diffusion.u.expression = {Du:g}
+reaction.expression = -u*v^2 + {F:g}*(1-u)
+reaction.jacobian.u.expression = -v^2 - {F:g}
+reaction.jacobian.v.expression = -2*u*v
+
+[model.scalar_field.v]
+compartment = domain
+storage.expression = 1
+cross_diffusion.v.expression = {Dv:g}
+reaction.expression = u*v^2 - ({F:g}+{k:g})*v
+reaction.jacobian.u.expression = v^2
+reaction.jacobian.v.expression = 2*u*v - ({F:g}+{k:g})
+
+""" + _solver_block("ImplicitEuler", precond, nl_rtol, nl_atol, lin_rtol, precision)
+
+    def init(model):
+        ids = model.compartment_nodes(0)
+        x = vertex_coords(ids, (n, n))
+        inside = (x[:, 0] >= 0.4) & (x[:, 0] <= 0.6) & (x[:, 1] >= 0.4) & (x[:, 1] <= 0.6)
+        u = np.where(inside, 0.5, 1.0) + 0.02 * (uniform01(seed, 0, ids) - 0.5)
+        v = np.where(inside, 0.25, 0.0) + 0.02 * (uniform01(seed, 1, ids) - 0.5)
+        out = np.empty(2 * ids.size)
+        out[0::2] = u
+        out[1::2] = v
+        return out
+
+    return Workload(f"gray_scott_{cell_type}_{n}", ini, dt, steps, 2, (n, n), init)
+
+
+# ------------------------------------------------------------- Mitchell-Schaefer
+def mitchell_schaefer(n=96, cell_type="cube", seed=0, D=1e-3, tau_in=0.3, tau_out=6.0, tau_open=120.0,
+                      tau_close=150.0, u_gate=0.13, precond="Jacobi", nl_rtol=1e-12, nl_atol=0.0,
+                      lin_rtol=1e-10, precision="fp64", steps=500, dt=0.1) -> Workload:
+    """Two-species excitable model on [0,1]^3, Alexander2 (benchmark config 1 at n = 96)."""
+    ini = f"""[grid]
+dimension = 3
+cells = {n} {n} {n}
+extensions = 1 1 1
+cell_type = {cell_type}
+
+[parser_context]
+tau_in.type = constant
+tau_in.value = {tau_in:g}
+tau_out.type = constant
+tau_out.value = {tau_out:g}
+tau_open.type = constant
+tau_open.value = {tau_open:g}
+tau_close.type = constant
+tau_close.value = {tau_close:g}
+u_gate.type = constant
+u_gate.value = {u_gate:g}
+
+[compartments]
+domain.expression = 1
+
+[model.scalar_field.u]
+compartment = domain
+storage.expression = 1
+cross_diffusion.u.expression = {D:g}
+reaction.expression = h*u^2*(1-u)/tau_in - u/tau_out
+reaction.jacobian.u.expression = h*(2*u - 3*u^2)/tau_in - 1/tau_out
+reaction.jacobian.h.expression = u^2*(1-u)/tau_in
+
+[model.scalar_field.h]
+compartment = domain
+storage.expression = 1
+reaction.expression = (u < u_gate) ? (1-h)/tau_open : -h/tau_close
+reaction.jacobian.h.expression = (u < u_gate) ? -1/tau_open : -1/tau_close
+
+""" + _solver_block("Alexander2", precond, nl_rtol, nl_atol, lin_rtol, precision)
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    centre = rng.uniform(0.1, 0.9, size=3)
+
+    def init(model):
+        ids = model.compartment_nodes(0)
+        x = vertex_coords(ids, (n, n, n))
+        r = np.linalg.norm(x - centre[None, :], axis=1)
+        out = np.empty(2 * ids.size)
+        out[0::2] = np.where(r <= 0.1, 1.0, 0.0)
+        out[1::2] = 1.0
+        return out
+
+    return Workload(f"mitchell_schaefer_{cell_type}_{n}", ini, dt, steps, 3, (n, n, n), init, stages=2)
+
+
+# ------------------------------------------------------------ two compartments
+def two_compartments(n=1024, cell_type="cube", seed=0, kt=0.5, precond="Jacobi", nl_rtol=1e-12, nl_atol=0.0,
+                     lin_rtol=1e-10, precision="fp64", steps=400, dt=0.05) -> Workload:
+    """Inner disk r < 0.3 and its complement, 4 species each, A+B<->C, C->D inside,
+    D->A outside, transmission k_t (D_in - A_out) (benchmark config 2 at n = 1024)."""
+    ds = np.exp(np.log(1e-4) + (np.log(1e-2) - np.log(1e-4)) * uniform01(seed, 99, np.arange(8)))
+    names_in = ["A_in", "B_in", "C_in", "D_in"]
+    names_out = ["A_out", "B_out", "C_out", "D_out"]
+    R = {
+        "A_in": ("-(1*A_in*B_in - 0.1*C_in)", {"A_in": "-B_in", "B_in": "-A_in", "C_in": "0.1"}),
+        "B_in": ("-(1*A_in*B_in - 0.1*C_in)", {"A_in": "-B_in", "B_in": "-A_in", "C_in": "0.1"}),
+        "C_in": ("(1*A_in*B_in - 0.1*C_in) - 0.05*C_in", {"A_in": "B_in", "B_in": "A_in", "C_in": "-0.15"}),
+        "D_in": ("0.05*C_in", {"C_in": "0.05"}),
+        "A_out": ("0.02*D_out", {"D_out": "0.02"}),
+        "B_out": (None, {}),
+        "C_out": (None, {}),
+        "D_out": ("-0.02*D_out", {"D_out": "-0.02"}),
+    }
+    parts = [f"""[grid]
+dimension = 2
+cells = {n} {n}
+extensions = 1 1
+cell_type = {cell_type}
+
+[compartments]
+inner.expression = (position_x-0.5)^2 + (position_y-0.5)^2 < 0.09
+outer.expression = (position_x-0.5)^2 + (position_y-0.5)^2 >= 0.09
+"""]
+    for i, s in enumerate(names_in + names_out):
+        comp = "inner" if s.endswith("_in") else "outer"
+        sec = [f"\n[model.scalar_field.{s}]", f"compartment = {comp}", "storage.expression = 1",
+               f"cross_diffusion.{s}.expression = {ds[i]:.17g}"]
+        expr, jac = R[s]
+        if expr:
+            sec.append(f"reaction.expression = {expr}")
+            for w, e in jac.items():
+                sec.append(f"reaction.jacobian.{w}.expression = {e}")
+        if s == "D_in":
+            sec += [f"outflow.outer.expression = {kt:g}*(D_in - A_out)",
+                    f"outflow.outer.jacobian.D_in.expression = {kt:g}",
+                    f"outflow.outer.jacobian.A_out.expression = {-kt:g}"]
+        if s == "A_out":
+            sec += [f"outflow.inner.expression = -{kt:g}*(D_in - A_out)",
+                    f"outflow.inner.jacobian.D_in.expression = {-kt:g}",
+                    f"outflow.inner.jacobian.A_out.expression = {kt:g}"]
+        parts.append("\n".join(sec) + "\n")
+    parts.append("\n" + _solver_block("ImplicitEuler", precond, nl_rtol, nl_atol, lin_rtol, precision))
+    ini = "".join(parts)
+
+    def init(model):
+        out = np.empty(model.n_dof)
+        for c, comp in enumerate(model.compartments()):
+            ids = model.compartment_nodes(c)
+            b, off = comp["n_species"], comp["dof_offset"]
+            for i in range(b):
+                out[off + i: off + b * ids.size: b] = uniform01(seed, 10 + 4 * c + i, ids)
+        return out
+
+    return Workload(f"two_compartments_{cell_type}_{n}", ini, dt, steps, 2, (n, n), init)
+
+
+# ------------------------------------------------------------- closed forms
+def exponential_decay(dim=2, n=4, cell_type="simplex", rk="ImplicitEuler", rate=-2.0, D=0.0) -> Workload:
+    """u' = rate * u with storage 1 (test/exp.ini); uniform u stays uniform."""
+    cells = " ".join([str(n)] * dim)
+    diff = f"cross_diffusion.u.expression = {D:g}\n" if D else ""
+    ini = f"""[grid]
+dimension = {dim}
+cells = {cells}
+cell_type = {cell_type}
+
+[compartments]
+domain.expression = 1
+
+[parser_context.grow_rate]
+type = constant
+value = {rate:g}
+
+[model.scalar_field.u]
+compartment = domain
+initial.expression = 1
+storage.expression = 1
+reaction.expression = grow_rate*u
+reaction.jacobian.u.expression = grow_rate
+{diff}
+""" + _solver_block(rk, "ILU", 1e-20, 1e-28, 1e-13)
+
+    def init(model):
+        return np.ones(model.n_dof)
+
+    return Workload(f"exp_{dim}d_{cell_type}_{rk}", ini, 0.1, 10, dim, (n,) * dim, init,
+                    stages=1 if rk == "ImplicitEuler" else 2)

@@ -1,0 +1,123 @@
+/*
+ * C ABI of the MI355X (gfx950) diffusion-reaction time step.
+ *
+ * This is the drop-in boundary for DuneCopasi's per-time-step path: a model
+ * read from the reference's own INI keys, advanced one time step at a time
+ * exactly where the reference calls `one_step.apply(in, out)`.  Every
+ * function returns 0 on success, a positive status for a numerical failure
+ * (the reference's PDELab::ErrorCondition) or a negative value on error; the
+ * error text is then in dcg_last_error().  No function throws.
+ *
+ * Ownership: a handle owns all device memory of its model (mesh tables,
+ * BCSR matrix, state and work vectors, generated kernels).  Host buffers are
+ * the caller's and are copied.  A handle is used by one host thread at a
+ * time; different handles are independent and may run concurrently.
+ *
+ * State layout (matches the reference's node-grouped "EntityGrouping" basis,
+ * dune/copasi/model/diffusion_reaction/model_single_compartment_traits.hh:26-27):
+ * compartments in INI order, inside a compartment its vertices in increasing
+ * vertex order, inside a vertex the compartment's species in INI order.
+ */
+#ifndef DCOPASI_H
+#define DCOPASI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dcg_model* dcg_model_t;
+
+/* statistics of one dcg_step call */
+typedef struct dcg_step_stats {
+  int status;            /* 0 ok, 1 Newton not converged, 2 Krylov not converged, 3 non-finite, 4 breakdown */
+  int stages;            /* Runge-Kutta stages completed */
+  int newton_iterations; /* summed over stages */
+  int linear_iterations; /* BiCGSTAB iterations summed over all solves */
+  int refinement_steps;  /* FP64 refinement steps of the mixed-precision solve */
+  double residual_norm;  /* last Newton residual in the configured norm (two_norm2 for l_2) */
+} dcg_step_stats;
+
+/* Build a model from INI text plus "key=value" override lines.
+ * Replaces DiffusionReaction::make_model + Model::make_state +
+ * Model::make_step_operator (dune/copasi/model/diffusion_reaction/factory.hh:46-136,
+ * dune/copasi/model/model.hh:24-101,
+ * dune/copasi/model/diffusion_reaction/model_single_compartment.impl.hh:195-232,
+ * dune/copasi/model/make_step_operator.hh:159-444).
+ * device >= 0 selects a GPU; device < 0 builds the host side only (grid,
+ * compartments, BCSR pattern, generated kernel source) without touching a GPU. */
+int dcg_model_create(const char* ini_text, const char* overrides, int device, dcg_model_t* out);
+
+/* Release the handle and its device memory. */
+void dcg_model_destroy(dcg_model_t model);
+
+/* Sizes of the state vector (dune/copasi/model/model.hh:36-42 State::coefficients). */
+int dcg_state_size(dcg_model_t model, size_t* n_dof, size_t* n_species, size_t* n_compartments);
+
+/* Copy a host state in / out (Model::interpolate and State::coefficients,
+ * dune/copasi/model/model.hh:60-75). */
+int dcg_set_state(dcg_model_t model, const double* state, size_t n, double time);
+int dcg_get_state(dcg_model_t model, double* state, size_t n, double* time);
+
+/* Evaluate the `initial.expression` keys at the DOF vertices
+ * (dune/copasi/model/make_initial.hh:24-90); no device needed. */
+int dcg_initial_state(dcg_model_t model, double* state, size_t n);
+
+/* One time step t -> t+dt from the current state: PDELab::OneStep::apply as
+ * called by TimeStepper::do_step (dune/copasi/common/stepper.hh:95-125).
+ * On a positive status the stored state is unchanged. */
+int dcg_step(dcg_model_t model, double t, double dt, dcg_step_stats* stats);
+
+/* FP64 integral of every species over its compartment, species in INI order
+ * (the mass sums of dune/copasi/model/diffusion_reaction/reduce.hh:160-185). */
+int dcg_mass(dcg_model_t model, double* per_species);
+
+/* Message of the last failed call on this thread (format_exception text,
+ * dune/copasi/common/exceptions.hh:21-33). */
+const char* dcg_last_error(void);
+
+/* ---- layout queries (no device needed) ---- */
+
+/* Name, compartment and index inside the compartment of species s. */
+int dcg_species_info(dcg_model_t model, size_t s, char* name, size_t name_len, int* compartment, int* local_index);
+
+/* Name, vertex count, first DOF and species count of compartment c. */
+int dcg_compartment_info(dcg_model_t model, size_t c, char* name, size_t name_len, size_t* n_nodes,
+                         size_t* dof_offset, int* n_species);
+
+/* Global vertex ids of compartment c's nodes, in DOF order. */
+int dcg_compartment_nodes(dcg_model_t model, size_t c, long* vertex_ids);
+
+/* Jacobian sparsity (local_operator.hh:276-399, make_step_operator.hh:295-401)
+ * as BCSR over slots (one slot = one vertex of one compartment, slots in DOF
+ * order) with block_size x block_size row-major blocks. */
+int dcg_pattern_info(dcg_model_t model, size_t* n_slots, size_t* n_blocks, int* block_size);
+int dcg_pattern(dcg_model_t model, int* row_ptr, int* col);
+
+/* HIP source generated for the model's assembly kernels; *needed gets the
+ * size including the terminating zero. */
+int dcg_generated_source(dcg_model_t model, char* buf, size_t len, size_t* needed);
+
+/* ---- kernel-level entry points (device required) ---- */
+
+/* residual = mass_weight * R^M(x) + stiffness_weight * R^A(x) at time t and,
+ * when jacobian != NULL, its Jacobian as BCSR values (n_blocks * block_size^2).
+ * The local-operator residual/Jacobian of local_operator.hh:417-1452 combined
+ * as the instationary assembler does (make_step_operator.hh:403-405). */
+int dcg_assemble(dcg_model_t model, const double* x, double t, double mass_weight, double stiffness_weight,
+                 double* residual, double* jacobian);
+
+/* y = J x for BCSR values J on the model's pattern. */
+int dcg_spmv(dcg_model_t model, const double* jacobian, const double* x, double* y);
+
+/* Solve J x = b with the model's configured Krylov solver and preconditioner
+ * (LinearSolver::apply, make_step_operator.hh:102-146). */
+int dcg_linear_solve(dcg_model_t model, const double* jacobian, const double* b, double* x, double rtol, int maxit,
+                     int* iterations);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DCOPASI_H */
