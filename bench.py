@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Newton DOF-steps/s of the device time step on one MI355X.
+
+Workload (default): the 3D Mitchell-Schaefer excitable model on a 96^3 Q1 hex
+mesh of the unit cube (912,673 nodes, 1,825,346 DOF), D = 1e-3 on u, Alexander2
+with dt = 0.1, BiCGSTAB + block-Jacobi, Newton tolerance 1e-12 on the squared
+l2 residual, linear tolerance 1e-10.  A "step" is one time step (two implicit
+Runge-Kutta stages, each a Newton solve); Newton DOF-steps = stages x DOF.
+
+Prints one JSON line with the throughput, per-step p50/p99 latency, the
+roofline of the dominant kernel (BCSR SpMV, timed live with HIP events on the
+model's stream) and a CPU baseline: the oracle (a C++ restatement of the
+reference step, OpenMP-coloured assembly, sequential BiCGSTAB) timed on the
+host cores for one time step of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+PKG = ROOT / "dune-copasi_amd"
+HBM_PEAK_GBS = 8000.0
+
+
+def _load(name, path, pkg=False):
+    spec = importlib.util.spec_from_file_location(name, path, submodule_search_locations=[str(path.parent)] if pkg else None)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def make_workload(wl, name, precision):
+    if name == "ms96":
+        return wl.mitchell_schaefer(n=96, cell_type="cube", precond="Jacobi", nl_rtol=1e-12, lin_rtol=1e-10,
+                                    precision=precision), "mitchell_schaefer_3d_q1_96^3_alexander2_dt0.1"
+    if name == "ms48":
+        return wl.mitchell_schaefer(n=48, cell_type="cube", precond="Jacobi", nl_rtol=1e-12, lin_rtol=1e-10,
+                                    precision=precision), "mitchell_schaefer_3d_q1_48^3_alexander2_dt0.1"
+    if name == "gs128":
+        return wl.gray_scott(n=128, cell_type="cube", precond="ILU", nl_rtol=1e-10, lin_rtol=1e-10,
+                             precision=precision), "gray_scott_2d_q1_128^2_implicit_euler_dt1"
+    if name == "two1024":
+        return wl.two_compartments(n=1024, cell_type="cube", precision=precision), \
+            "two_compartments_2d_q1_1024^2_implicit_euler_dt0.05"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(wl_obj, steps: int):
+    """Oracle on the host cores: `steps` time steps of the same workload."""
+    sys.path.insert(0, str(ROOT))
+    from tests import oracle_lib  # the oracle is only ever the baseline here
+
+    L = oracle_lib.load()
+    cores = os.cpu_count() or 1
+    L.orc_set_threads(cores)
+    o = oracle_lib.OracleModel(wl_obj.ini)
+    o.set_state(wl_obj.initial_state(o), 0.0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rc, _, _ = o.step(wl_obj.dt)
+        if rc:
+            raise RuntimeError(f"oracle step failed: {rc}")
+    el = time.perf_counter() - t0
+    dof_steps = steps * wl_obj.stages * o.n_dof
+    return {"value": dof_steps / el, "unit": "Newton DOF-steps/s", "cores": int(L.orc_max_threads()),
+            "kind": "port",
+            "sample": f"{steps} time step(s) of the same workload from the same initial state "
+                      f"({dof_steps} Newton DOF-steps, {el:.1f} s, OpenMP-coloured assembly, sequential BiCGSTAB)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="ms96")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run a plain device copy of this size (counter calibration under rocprofv3)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        raise SystemExit(2)
+
+    import torch
+
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a GPU")
+    dca = _load("dune_copasi_amd", PKG / "__init__.py", pkg=True)
+    wl = _load("dca_workloads", PKG / "workloads.py")
+    w, wname = make_workload(wl, args.workload, args.precision)
+
+    t_setup = time.perf_counter()
+    m = dca.Model(w.ini, "", 0)
+    u0 = w.initial_state(m)
+    m.set_state(u0, 0.0)
+    setup_s = time.perf_counter() - t_setup
+    print(f"# setup {setup_s:.1f}s, {m.n_dof} DOF, {m.n_blocks} blocks of {m.block_size}x{m.block_size}",
+          file=sys.stderr, flush=True)
+
+    t = 0.0
+    for k in range(args.warmup):
+        st = m.step(t, w.dt)
+        if st.status:
+            raise SystemExit(f"warmup step {k} failed with status {st.status}")
+        t += w.dt
+    print(f"# warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+
+    m.profile_enable(True)
+    torch.cuda.synchronize()
+    lat, newton, lin = [], 0, 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        s0 = time.perf_counter()
+        st = m.step(t, w.dt)
+        lat.append(time.perf_counter() - s0)
+        if st.status:
+            raise SystemExit(f"step {k} failed with status {st.status}")
+        newton += st.newton_iterations
+        lin += st.linear_iterations
+        t += w.dt
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    print(f"# timed {args.steps} steps in {elapsed:.3f}s", file=sys.stderr, flush=True)
+
+    kinds = {}
+    for kname in ["spmv64", "spmv32", "assembly_jacobian", "assembly_residual", "precond_setup"]:
+        n, ms, b = m.profile(kname)
+        if n:
+            kinds[kname] = {"launches": n, "avg_us": 1e3 * ms / n, "gbs": (b / n) / (ms / n * 1e-3) / 1e9 if ms else 0.0,
+                            "bytes_per_launch": b / n, "share_of_step_time": ms / (1e3 * elapsed)}
+    spmv_kind = "spmv32" if args.precision == "mixed" and "spmv32" in kinds else "spmv64"
+    sp = kinds.get(spmv_kind)
+    roofline = None
+    if sp:
+        roofline = {"bound": "hbm", "kernel": spmv_kind, "achieved": sp["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": sp["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                    "avg_launch_us": sp["avg_us"], "algorithmic_bytes_per_launch": sp["bytes_per_launch"]}
+
+    if args.copy_probe_mib:
+        n = args.copy_probe_mib * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+
+    value = args.steps * w.stages * m.n_dof / elapsed
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("# cpu baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(w, args.cpu_steps)
+    out = {
+        "metric": "Newton DOF-steps/s (plain FP64 vs mixed/batched), p50/p99 latency, HBM GB/s",
+        "value": value,
+        "unit": "Newton DOF-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-krylov/f64-refinement",
+        "data": "synthetic (seeded initial state)",
+        "config": {"workload": wname, "n_dof": m.n_dof, "stages_per_step": w.stages,
+                   "precision": args.precision, "preconditioner": "block-Jacobi"},
+        "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
+        "newton_iterations_per_step": newton / args.steps,
+        "krylov_iterations_per_step": lin / args.steps,
+        "kernels": kinds,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

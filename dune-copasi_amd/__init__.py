@@ -60,7 +60,11 @@ _SIGS = {
     "dcg_assemble": (_I, [_P, _P, _D, _D, _D, _P, _P]),
     "dcg_spmv": (_I, [_P, _P, _P, _P]),
     "dcg_linear_solve": (_I, [_P, _P, _P, _P, _D, _I, ctypes.POINTER(_I)]),
+    "dcg_profile_enable": (_I, [_P, _I]),
+    "dcg_profile_read": (_I, [_P, _I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
 }
+
+PROFILE_KINDS = {"spmv64": 0, "spmv32": 1, "assembly_jacobian": 2, "assembly_residual": 3, "precond_setup": 4}
 
 
 def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
@@ -215,6 +219,17 @@ class Model:
         its = _I()
         rc = _check(self._lib.dcg_linear_solve(self._h, _ptr(J), _ptr(b), _ptr(x), rtol, maxit, ctypes.byref(its)))
         return x, rc, its.value
+
+    # ---- measurement
+    def profile_enable(self, on: bool = True):
+        _check(self._lib.dcg_profile_enable(self._h, 1 if on else 0))
+
+    def profile(self, kind: str):
+        """(launches, total milliseconds, total algorithmic bytes) of one kernel class."""
+        l, ms, b = ctypes.c_long(), _D(), _D()
+        _check(self._lib.dcg_profile_read(self._h, PROFILE_KINDS[kind], ctypes.byref(l), ctypes.byref(ms),
+                                          ctypes.byref(b)))
+        return l.value, ms.value, b.value
 
     # ---- helpers on the layout
     def slot_dofs(self):

@@ -207,4 +207,25 @@ int dcg_linear_solve(dcg_model_t h, const double* jacobian, const double* b, dou
   });
 }
 
+int dcg_profile_enable(dcg_model_t h, int on) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    h->m->set_profiling(on != 0);
+    return 0;
+  });
+}
+
+int dcg_profile_read(dcg_model_t h, int kind, long* launches, double* milliseconds, double* bytes) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    long l = 0;
+    double ms = 0, b = 0;
+    h->m->profile(kind, &l, &ms, &b);
+    if (launches) *launches = l;
+    if (milliseconds) *milliseconds = ms;
+    if (bytes) *bytes = b;
+    return 0;
+  });
+}
+
 }  // extern "C"

@@ -47,6 +47,8 @@ Model::Model(const std::string& ini, const std::string& overrides, int device) :
 }
 
 Model::~Model() {
+  for (auto& p : prof_data_)
+    for (auto e : p.pool) (void)hipEventDestroy(e);
   if (stream_) {
     (void)hipStreamSynchronize(stream_);
     (void)hipStreamDestroy(stream_);
@@ -404,12 +406,16 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
     for (int it = 0; it < batch; ++it) {
       k::bicg_p<double>(B, ns, r, p, v, ph, dinv, sc_.get(), stream_);
       if (ilu) apply_ilu(p, ph);
+      prof_begin(kSpmv64);
       k::spmv(B, plan_, val_.get(), ph, v, rh, nullptr, part_.get(), skip, stream_);
+      prof_end(kSpmv64, spmv_bytes(8, 8, 1));
       k::bicg_fin(plan_.nchunk, part_.get(), 2, 1, sc_.get(), stream_);
       k::bicg_s<double>(B, ns, r, v, s, sh, dinv, sc_.get(), part_.get(), stream_);
       if (ilu) apply_ilu(s, sh);
       k::bicg_fin(nblk, part_.get(), 1, 2, sc_.get(), stream_);
+      prof_begin(kSpmv64);
       k::spmv(B, plan_, val_.get(), sh, t, s, t, part_.get(), skip, stream_);
+      prof_end(kSpmv64, spmv_bytes(8, 8, 1));
       k::bicg_fin(plan_.nchunk, part_.get(), 2, 3, sc_.get(), stream_);
       k::bicg_xr<double>(B, ns, x, ph, sh, s, t, r, rh, sc_.get(), part_.get(), stream_);
       k::bicg_fin(nblk, part_.get(), 2, 4, sc_.get(), stream_);
@@ -459,12 +465,16 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
     for (int it = 0; it < batch; ++it) {
       k::bicg_p<float>(B, ns, r, p, v, ph, dinv, sc_.get(), stream_);
       if (ilu) apply_ilu(p, ph);
+      prof_begin(kSpmv32);
       k::spmv(B, plan_, val32_.get(), ph, v, rh, nullptr, part_.get(), skip, stream_);
+      prof_end(kSpmv32, spmv_bytes(4, 4, 1));
       k::bicg_fin(plan_.nchunk, part_.get(), 2, 1, sc_.get(), stream_);
       k::bicg_s<float>(B, ns, r, v, s, sh, dinv, sc_.get(), part_.get(), stream_);
       if (ilu) apply_ilu(s, sh);
       k::bicg_fin(nblk, part_.get(), 1, 2, sc_.get(), stream_);
+      prof_begin(kSpmv32);
       k::spmv(B, plan_, val32_.get(), sh, t, s, t, part_.get(), skip, stream_);
+      prof_end(kSpmv32, spmv_bytes(4, 4, 1));
       k::bicg_fin(plan_.nchunk, part_.get(), 2, 3, sc_.get(), stream_);
       k::bicg_xr<float>(B, ns, x, ph, sh, s, t, r, rh, sc_.get(), part_.get(), stream_);
       k::bicg_fin(nblk, part_.get(), 2, 4, sc_.get(), stream_);
@@ -488,7 +498,9 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
 // BiCGSTAB.  Mixed: FP32 BiCGSTAB on an FP32 copy of the matrix inside FP64
 // iterative refinement; the refinement residual b - A x is FP64.
 int Model::krylov(const double* b, double* x, double rtol, int maxit, int* its, int* refinements) {
+  prof_begin(kPrecond);
   precond_setup();
+  prof_end(kPrecond, 0.0);
   *its = 0;
   *refinements = 0;
   const long n = L_.nslot * L_.B;
@@ -555,12 +567,16 @@ int Model::step(double t, double dt, StepStats* st) {
     DCG_HIP(hipMemcpyAsync(x, stage_[s - 1].get(), bytes, hipMemcpyDeviceToDevice, stream_));
     auto residual = [&]() {
       DCG_HIP(hipMemcpyAsync(r_.get(), hist_.get(), bytes, hipMemcpyDeviceToDevice, stream_));
+      prof_begin(kAssemblyRes);
       launch_assembly(x, ts, cm, ca, r_.get(), nullptr, 1);
+      prof_end(kAssemblyRes, assembly_bytes(false));
     };
     auto jacobian = [&]() {
       k::init_jacobian(val_.get(), matrix_values(), diag_pos_.get(), slot_b_.get(), static_cast<int>(L_.nslot), L_.B,
                        stream_);
+      prof_begin(kAssemblyJac);
       launch_assembly(x, ts, cm, ca, nullptr, val_.get(), 2);
+      prof_end(kAssemblyJac, assembly_bytes(true));
     };
     auto solve = [&]() -> int {
       int its = 0, refs = 0;
@@ -614,6 +630,7 @@ int Model::step(double t, double dt, StepStats* st) {
   DCG_HIP(hipMemcpyAsync(u_.get(), stage_[rk_.stages].get(), bytes, hipMemcpyDeviceToDevice, stream_));
   time_ = t + dt;
   synchronize();
+  if (prof_) prof_collect();
   return 0;
 }
 
@@ -664,6 +681,84 @@ void Model::mass(double* out) {
     synchronize();
     for (int i = 0; i < C.b; ++i) out[C.species[i]] = sums[i];
   }
+}
+
+// ------------------------------------------------------------------ profiling
+void Model::set_profiling(bool on) {
+  require_device();
+  synchronize();
+  prof_collect();
+  prof_ = on;
+  for (auto& p : prof_data_) {
+    p.launches = 0;
+    p.ms = 0.0;
+    p.bytes = 0.0;
+  }
+}
+
+void Model::prof_begin(int kind) {
+  if (!prof_) return;
+  Prof& p = prof_data_[kind];
+  if (p.used + 2 > p.pool.size()) {
+    hipEvent_t a, b;
+    DCG_HIP(hipEventCreate(&a));
+    DCG_HIP(hipEventCreate(&b));
+    p.pool.push_back(a);
+    p.pool.push_back(b);
+  }
+  DCG_HIP(hipEventRecord(p.pool[p.used], stream_));
+}
+
+void Model::prof_end(int kind, double bytes) {
+  if (!prof_) return;
+  Prof& p = prof_data_[kind];
+  DCG_HIP(hipEventRecord(p.pool[p.used + 1], stream_));
+  p.used += 2;
+  p.pending_bytes.push_back(bytes);
+}
+
+void Model::prof_collect() {
+  for (auto& p : prof_data_) {
+    for (size_t i = 0; i + 1 < p.used; i += 2) {
+      float ms = 0.f;
+      DCG_HIP(hipEventElapsedTime(&ms, p.pool[i], p.pool[i + 1]));
+      p.ms += ms;
+      p.launches += 1;
+      p.bytes += p.pending_bytes[i / 2];
+    }
+    p.used = 0;
+    p.pending_bytes.clear();
+  }
+}
+
+void Model::profile(int kind, long* launches, double* ms, double* bytes) {
+  require_device();
+  if (kind < 0 || kind >= kProfKinds) throw Error("profile kind out of range");
+  synchronize();
+  prof_collect();
+  *launches = prof_data_[kind].launches;
+  *ms = prof_data_[kind].ms;
+  *bytes = prof_data_[kind].bytes;
+}
+
+// Algorithmic bytes of one BCSR SpMV: values, column indices, row pointer,
+// x read once, y written once, plus fused dot-product operands.
+double Model::spmv_bytes(int vb, int xb, int extra_vectors) const {
+  const double nb = static_cast<double>(L_.nnzb()), N = static_cast<double>(L_.nslot), B = L_.B;
+  return vb * B * B * nb + 4.0 * nb + 4.0 * (N + 1) + xb * B * N * (2.0 + extra_vectors);
+}
+
+// Algorithmic bytes of one assembly pass: state read once, residual or
+// Jacobian values written once, plus the cell tables (cell id, class mask,
+// corner slots) and the per-row stencil tables the Jacobian scatter reads.
+double Model::assembly_bytes(bool jacobian) const {
+  const double N = static_cast<double>(L_.nslot), B = L_.B;
+  double cells = 0;
+  for (const auto& C : L_.comps) cells += static_cast<double>(C.cells.size());
+  double b = 8.0 * B * N + cells * (4.0 + 1.0 + 4.0 * L_.grid.ncorner);
+  if (jacobian) b += 8.0 * B * B * static_cast<double>(L_.nnzb()) + 8.0 * N;
+  else b += 8.0 * B * N * 2.0;  // residual: history read + residual written
+  return b;
 }
 
 // ------------------------------------------------------------------ test entry points

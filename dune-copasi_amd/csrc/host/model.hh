@@ -85,6 +85,12 @@ class Model {
   void spmv(const double* jac, const double* x_packed, double* y_packed);
   int linear_solve(const double* jac, const double* b_packed, double* x_packed, double rtol, int maxit, int* its);
 
+  // kernel timing with HIP events on the model's stream (benchmarks)
+  enum ProfKind { kSpmv64 = 0, kSpmv32 = 1, kAssemblyJac = 2, kAssemblyRes = 3, kPrecond = 4, kProfKinds = 5 };
+  void set_profiling(bool on);
+  // launches, summed milliseconds and summed algorithmic bytes of one kind
+  void profile(int kind, long* launches, double* ms, double* bytes);
+
   // device-side pieces used by batching/benchmarks
   hipStream_t stream() const { return stream_; }
   void synchronize() const;
@@ -103,6 +109,11 @@ class Model {
   int bicgstab32(const double* b, float* x, double rtol, int maxit, int* its);
   void precond_setup();
   void compile_and_load();
+  void prof_begin(int kind);
+  void prof_end(int kind, double bytes);
+  void prof_collect();
+  double spmv_bytes(int value_bytes, int vector_bytes, int extra_vectors) const;
+  double assembly_bytes(bool jacobian) const;
 
   ParameterTree cfg_tree_;
   ModelSpec spec_;
@@ -148,6 +159,16 @@ class Model {
   std::vector<int> lo_ptr_, up_ptr_;
   DeviceArray<int> lo_rows_, up_rows_;
   bool ilu_ready_ = false;
+  // profiling
+  bool prof_ = false;
+  struct Prof {
+    std::vector<hipEvent_t> pool;
+    std::vector<double> pending_bytes;
+    size_t used = 0;
+    long launches = 0;
+    double ms = 0.0, bytes = 0.0;
+  };
+  Prof prof_data_[kProfKinds];
 };
 
 }  // namespace dcopasi

@@ -116,6 +116,16 @@ int dcg_spmv(dcg_model_t model, const double* jacobian, const double* x, double*
 int dcg_linear_solve(dcg_model_t model, const double* jacobian, const double* b, double* x, double rtol, int maxit,
                      int* iterations);
 
+/* ---- measurement ----
+ * HIP-event timing on the model's own stream, the device counterpart of the
+ * reference's Perfetto counters (dune/copasi/common/stepper.hh:100-102,
+ * dune/copasi/model/make_step_operator.hh:137-139).  kind: 0 FP64 SpMV,
+ * 1 FP32 SpMV, 2 Jacobian assembly pass, 3 residual assembly pass,
+ * 4 preconditioner set-up.  Enabling resets the counters; `bytes` sums the
+ * algorithmic bytes of the timed launches. */
+int dcg_profile_enable(dcg_model_t model, int on);
+int dcg_profile_read(dcg_model_t model, int kind, long* launches, double* milliseconds, double* bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1730,3 +1730,43 @@ int orc_mass(void* h, const double* x, double* out) {
 }
 
 }  // extern "C"
+
+// Evaluate one expression with named variables (parser tests).  `context` is
+// INI text whose [parser_context] constants/functions are visible.
+extern "C" int orc_eval(const char* context, const char* text, const char* const* names, const double* values, int n,
+                        double* out) {
+  try {
+    orc::Model M;
+    M.ini.parse(context ? context : "");
+    orc::Scope sc;
+    for (int i = 0; i < n; ++i) sc.vars[names[i]] = i;
+    // constants and functions of the context, as build_scope reads them
+    for (auto& k : M.ini.subkeys("parser_context")) {
+      std::string p = "parser_context." + k;
+      std::string type = M.ini.get(p + ".type", "");
+      if (type == "constant") sc.consts[k] = std::stod(M.ini.get(p + ".value"));
+      if (type == "function") {
+        std::string e = M.ini.get(p + ".expression");
+        auto colon = e.find(':');
+        orc::UserFunction f;
+        std::string args = e.substr(0, colon);
+        size_t st = 0;
+        while (st <= args.size()) {
+          size_t c = args.find(',', st);
+          std::string a = orc::trim(args.substr(st, c == std::string::npos ? std::string::npos : c - st));
+          if (!a.empty()) f.args.push_back(a);
+          if (c == std::string::npos) break;
+          st = c + 1;
+        }
+        f.body = e.substr(colon + 1);
+        sc.funcs[k] = f;
+      }
+    }
+    orc::Expr e = orc::compile(text, sc);
+    *out = e.null() ? 0.0 : e(values);
+    return e.null() ? 1 : 0;
+  } catch (const std::exception& ex) {
+    g_err = ex.what();
+    return -1;
+  }
+}

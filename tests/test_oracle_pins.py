@@ -1,0 +1,344 @@
+"""Pins of the CPU oracle against what the mathematics fixes.
+
+The reference ships no golden vectors for this path (its test/data is Git-LFS
+stubs), so the oracle is checked against:
+  * closed forms of the reference's own test/exp.ini (u' = -2u): implicit
+    Euler gives exactly (1 + 2 dt)^-n at every node, Alexander2 its stability
+    function, on P1 and Q1 meshes in 1D/2D/3D;
+  * textbook element matrices (P1 triangle/tetrahedron and Q1 square mass and
+    stiffness), assembled by numpy on 1-cell and 2x2-cell meshes;
+  * invariants: stiffness rows sum to zero, mass entries sum to |Omega|,
+    species mass is conserved under pure diffusion;
+  * central finite differences of the oracle's residual for its Jacobian
+    (reaction, transmission and discontinuous-gate models);
+  * the exact discrete propagator of implicit Euler and Alexander2 for a
+    linear two-species ODE (matrix functions in numpy), which pins the
+    Runge-Kutta coefficients and the signs of the coupling Jacobian;
+  * the root property: the state after a step makes the stage residual vanish.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+from tests.oracle_lib import OracleModel, load
+
+ALPHA = 1.0 - math.sqrt(2.0) / 2.0
+
+
+def _exp_ini(dim, cell_type, rk, n=3, D=0.0):
+    cells = " ".join([str(n)] * dim)
+    diff = f"cross_diffusion.u.expression = {D}\n" if D else ""
+    return f"""[grid]
+dimension = {dim}
+cells = {cells}
+cell_type = {cell_type}
+[compartments]
+domain.expression = 1
+[parser_context.grow_rate]
+type = constant
+value = -2.
+[model.scalar_field.u]
+compartment = domain
+storage.expression = 1
+reaction.expression = grow_rate*u
+reaction.jacobian.u.expression = grow_rate
+{diff}[model.time_step_operator]
+type = {rk}
+linear_solver.convergence_condition.relative_tolerance = 1e-14
+nonlinear_solver.convergence_condition.relative_tolerance = 1e-24
+nonlinear_solver.convergence_condition.absolute_tolerance = 1e-30
+"""
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("cell_type", ["simplex", "cube"])
+@pytest.mark.parametrize("rk", ["ImplicitEuler", "Alexander2"])
+@pytest.mark.parametrize("D", [0.0, 0.3])
+def test_exp_ini_closed_form(dim, cell_type, rk, D):
+    o = OracleModel(_exp_ini(dim, cell_type, rk, D=D))
+    o.set_state(np.ones(o.n_dof), 0.0)
+    dt, n = 0.1, 7
+    for _ in range(n):
+        rc, _, _ = o.step(dt)
+        assert rc == 0
+    u, t = o.get_state()
+    z = -2.0 * dt
+    R = 1.0 / (1.0 - z) if rk == "ImplicitEuler" else (1 + (1 - 2 * ALPHA) * z) / (1 - ALPHA * z) ** 2
+    assert abs(t - n * dt) < 1e-12
+    np.testing.assert_allclose(u, R ** n, rtol=1e-12, atol=1e-14)
+
+
+def _diffusion_ini(dim, cell_type, cells, D=1.0, ext=None):
+    ext = ext or " ".join(["1"] * dim)
+    return f"""[grid]
+dimension = {dim}
+cells = {cells}
+extensions = {ext}
+cell_type = {cell_type}
+[compartments]
+domain.expression = 1
+[model.scalar_field.u]
+compartment = domain
+storage.expression = 1
+cross_diffusion.u.expression = {D}
+[model.time_step_operator]
+type = ImplicitEuler
+linear_solver.convergence_condition.relative_tolerance = 1e-14
+nonlinear_solver.convergence_condition.relative_tolerance = 1e-24
+nonlinear_solver.convergence_condition.absolute_tolerance = 1e-30
+"""
+
+
+def _matrices(o):
+    x = np.zeros(o.n_dof)
+    _, vm = o.assemble(x, 0.0, 1.0, 0.0)
+    _, vk = o.assemble(x, 0.0, 0.0, 1.0)
+    return o.csr(vm).toarray(), o.csr(vk).toarray()
+
+
+def test_q1_square_textbook_matrices():
+    h = 0.5
+    o = OracleModel(_diffusion_ini(2, "cube", "1 1", ext="0.5 0.5"))
+    M, K = _matrices(o)
+    m1 = h / 6 * np.array([[2.0, 1.0], [1.0, 2.0]])
+    k1 = 1 / h * np.array([[1.0, -1.0], [-1.0, 1.0]])
+    # vertex order x fastest: index = i + 2 j  ->  kron(y-matrix, x-matrix)
+    np.testing.assert_allclose(M, np.kron(m1, m1), rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(K, np.kron(m1, k1) + np.kron(k1, m1), rtol=1e-13, atol=1e-15)
+
+
+def test_p1_triangles_textbook_matrices():
+    # unit square split along the (0,0)-(1,1) diagonal: triangles {0,1,3}, {0,2,3}
+    o = OracleModel(_diffusion_ini(2, "simplex", "1 1"))
+    M, K = _matrices(o)
+    Me = 1.0 / 24 * np.array([[2, 1, 1], [1, 2, 1], [1, 1, 2]], dtype=float)  # area 1/2
+    Mref = np.zeros((4, 4))
+    Kref = np.zeros((4, 4))
+    # right angle at vertex 1 in {0,1,3}; at vertex 2 in {0,2,3}
+    for tri, right in (([0, 1, 3], 1), ([0, 2, 3], 2)):
+        Ke = np.zeros((3, 3))
+        # stiffness of a right isosceles triangle with unit legs
+        others = [v for v in tri if v != right]
+        li = {v: i for i, v in enumerate(tri)}
+        r = li[right]
+        a, b = li[others[0]], li[others[1]]
+        Ke[r, r] = 1.0
+        Ke[a, a] = Ke[b, b] = 0.5
+        Ke[r, a] = Ke[a, r] = Ke[r, b] = Ke[b, r] = -0.5
+        for i, vi in enumerate(tri):
+            for j, vj in enumerate(tri):
+                Mref[vi, vj] += Me[i, j]
+                Kref[vi, vj] += Ke[i, j]
+    np.testing.assert_allclose(M, Mref, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(K, Kref, rtol=1e-13, atol=1e-15)
+
+
+def test_p1_tetrahedra_mass_matrix():
+    # each Kuhn tetrahedron of the unit cube has volume 1/6 and mass V/20 (2 on the diagonal)
+    o = OracleModel(_diffusion_ini(3, "simplex", "1 1 1"))
+    M, K = _matrices(o)
+    Mref = np.zeros((8, 8))
+    Me = (1.0 / 6) / 20 * (np.ones((4, 4)) + np.eye(4))
+    import itertools
+    for p in itertools.permutations(range(3)):
+        verts, bits = [0], 0
+        for ax in p:
+            bits |= 1 << ax
+            verts.append(bits)
+        for i, vi in enumerate(verts):
+            for j, vj in enumerate(verts):
+                Mref[vi, vj] += Me[i, j]
+    np.testing.assert_allclose(M, Mref, rtol=1e-13, atol=1e-16)
+    np.testing.assert_allclose(K.sum(axis=1), 0.0, atol=1e-13)
+
+
+@pytest.mark.parametrize("dim,cell_type,cells", [(2, "simplex", "5 3"), (2, "cube", "4 6"), (3, "simplex", "3 2 2"),
+                                                 (3, "cube", "3 3 2"), (1, "cube", "7")])
+def test_mass_and_stiffness_invariants(dim, cell_type, cells):
+    ext = " ".join(["0.7"] * dim)
+    o = OracleModel(_diffusion_ini(dim, cell_type, cells, ext=ext))
+    M, K = _matrices(o)
+    np.testing.assert_allclose(M.sum(), 0.7 ** dim, rtol=1e-13)
+    np.testing.assert_allclose(K.sum(axis=1), 0.0, atol=1e-12)
+    np.testing.assert_allclose(M, M.T, atol=1e-16)
+    np.testing.assert_allclose(K, K.T, atol=1e-13)
+    assert np.linalg.eigvalsh(M).min() > 0
+
+
+@pytest.mark.parametrize("cell_type", ["simplex", "cube"])
+def test_pure_diffusion_conserves_mass(wl, cell_type):
+    o = OracleModel(_diffusion_ini(2, cell_type, "9 7", D=0.05))
+    rng = np.random.default_rng(0)
+    u0 = rng.uniform(0, 1, o.n_dof)
+    o.set_state(u0, 0.0)
+    m0 = o.mass(u0)
+    for _ in range(5):
+        assert o.step(0.3)[0] == 0
+    u, _ = o.get_state()
+    np.testing.assert_allclose(o.mass(u), m0, rtol=1e-11)
+    assert np.std(u) < np.std(u0)  # and it does diffuse
+
+
+def _fd_check(o, x, t, cm, ca, cols, h=1e-6, rtol=2e-6):
+    _, v = o.assemble(x, t, cm, ca)
+    A = o.csr(v)
+    for j in cols:
+        e = np.zeros_like(x)
+        e[j] = h
+        rp, _ = o.assemble(x + e, t, cm, ca, jacobian=False)
+        rm, _ = o.assemble(x - e, t, cm, ca, jacobian=False)
+        fd = (rp - rm) / (2 * h)
+        col = A[:, j].toarray().ravel()
+        scale = max(np.abs(col).max(), np.abs(fd).max(), 1e-30)
+        assert np.abs(fd - col).max() <= rtol * scale, (j, np.abs(fd - col).max(), scale)
+
+
+@pytest.mark.parametrize("make", [
+    lambda wl: wl.gray_scott(n=5, cell_type="simplex"),
+    lambda wl: wl.gray_scott(n=4, cell_type="cube"),
+    lambda wl: wl.mitchell_schaefer(n=3, cell_type="cube"),
+    lambda wl: wl.two_compartments(n=10, cell_type="cube"),
+    lambda wl: wl.two_compartments(n=8, cell_type="simplex"),
+], ids=["gs_p1", "gs_q1", "ms_q1", "two_q1", "two_p1"])
+def test_jacobian_matches_finite_differences(wl, make):
+    w = make(wl)
+    o = OracleModel(w.ini)
+    x = w.initial_state(o)
+    rng = np.random.default_rng(2)
+    x = x + 0.05 * rng.standard_normal(x.size)
+    cols = rng.choice(o.n_dof, size=min(40, o.n_dof), replace=False)
+    _fd_check(o, x, 0.2, 1.0, 0.7, cols)
+
+
+def test_numerical_jacobian_agrees_with_analytical(wl):
+    w = wl.gray_scott(n=4, cell_type="cube")
+    o_a = OracleModel(w.ini)
+    o_n = OracleModel(w.ini, "model.jacobian.type=numerical")
+    x = w.initial_state(o_a) + 0.01
+    _, va = o_a.assemble(x, 0.0, 1.0, 1.0)
+    _, vn = o_n.assemble(x, 0.0, 1.0, 1.0)
+    assert np.abs(va - vn).max() <= 1e-6 * np.abs(va).max()
+
+
+def _linear_ode_ini(rk):
+    # u' = -0.3 u + 0.8 v ; v' = -0.5 u - 0.2 v (rotation with decay), no diffusion
+    return f"""[grid]
+dimension = 2
+cells = 2 2
+[compartments]
+domain.expression = 1
+[model.scalar_field.u]
+compartment = domain
+storage.expression = 1
+reaction.expression = -0.3*u + 0.8*v
+reaction.jacobian.u.expression = -0.3
+reaction.jacobian.v.expression = 0.8
+[model.scalar_field.v]
+compartment = domain
+storage.expression = 1
+reaction.expression = -0.5*u - 0.2*v
+reaction.jacobian.u.expression = -0.5
+reaction.jacobian.v.expression = -0.2
+[model.time_step_operator]
+type = {rk}
+linear_solver.convergence_condition.relative_tolerance = 1e-14
+nonlinear_solver.convergence_condition.relative_tolerance = 1e-26
+nonlinear_solver.convergence_condition.absolute_tolerance = 1e-30
+"""
+
+
+@pytest.mark.parametrize("rk", ["ImplicitEuler", "Alexander2"])
+def test_linear_ode_discrete_propagator(rk):
+    import scipy.linalg
+
+    A = np.array([[-0.3, 0.8], [-0.5, -0.2]])
+    dt, n = 0.25, 8
+    Z = dt * A
+    I = np.eye(2)
+    if rk == "ImplicitEuler":
+        R = np.linalg.inv(I - Z)
+    else:
+        Ri = np.linalg.inv(I - ALPHA * Z)
+        R = Ri @ Ri @ (I + (1 - 2 * ALPHA) * Z)
+    o = OracleModel(_linear_ode_ini(rk))
+    u0 = np.array([1.0, 0.5])
+    o.set_state(np.tile(u0, o.n_dof // 2), 0.0)
+    for _ in range(n):
+        assert o.step(dt)[0] == 0
+    u, _ = o.get_state()
+    ref = np.linalg.matrix_power(R, n) @ u0
+    np.testing.assert_allclose(u.reshape(-1, 2), np.tile(ref, (o.n_dof // 2, 1)), rtol=1e-12, atol=1e-14)
+    # and the scheme approximates the exact flow to its order
+    exact = scipy.linalg.expm(n * dt * A) @ u0
+    err = np.abs(ref - exact).max()
+    assert err < (0.1 if rk == "ImplicitEuler" else 5e-3)
+
+
+@pytest.mark.parametrize("make", [lambda wl: wl.gray_scott(n=6, cell_type="cube", nl_rtol=1e-24, nl_atol=1e-28,
+                                                           lin_rtol=1e-13),
+                                  lambda wl: wl.two_compartments(n=8, cell_type="cube", nl_rtol=1e-24, nl_atol=1e-28,
+                                                                 lin_rtol=1e-13)], ids=["gs", "two"])
+def test_step_solves_the_stage_equation(wl, make):
+    # implicit Euler: M(u1) - M(u0) + dt A(u1) = 0 at the returned state
+    w = make(wl)
+    o = OracleModel(w.ini)
+    u0 = w.initial_state(o)
+    o.set_state(u0, 0.0)
+    assert o.step(w.dt)[0] == 0
+    u1, _ = o.get_state()
+    r1, _ = o.assemble(u1, w.dt, 1.0, w.dt, jacobian=False)
+    r0, _ = o.assemble(u0, 0.0, 1.0, 0.0, jacobian=False)
+    ref = np.abs(r0).max()
+    assert np.abs(r1 - r0).max() <= 1e-11 * ref
+
+
+def _eval(expr, names=(), values=(), context=""):
+    L = load()
+    L.orc_eval.restype = ctypes.c_int
+    L.orc_eval.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p),
+                           ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    n = len(names)
+    nm = (ctypes.c_char_p * max(n, 1))(*[s.encode() for s in names])
+    vv = (ctypes.c_double * max(n, 1))(*values)
+    out = ctypes.c_double()
+    rc = L.orc_eval(context.encode(), expr.encode(), nm, vv, n, ctypes.byref(out))
+    if rc < 0:
+        raise ValueError(L.orc_last_error().decode())
+    return out.value, rc
+
+
+EXPRESSIONS = [
+    ("-x^2", lambda x, y: -(x ** 2)),
+    ("x^y^0.5", lambda x, y: x ** (y ** 0.5)),
+    ("2^-y", lambda x, y: 2 ** -y),
+    ("x*y - y/x + 3", lambda x, y: x * y - y / x + 3),
+    ("x < y ? x : y", lambda x, y: x if x < y else y),
+    ("(x > 1) && (y <= 3)", lambda x, y: float(x > 1 and y <= 3)),
+    ("x == 2 || y != 3", lambda x, y: float(x == 2 or y != 3)),
+    ("not (x > y)", lambda x, y: float(not (x > y))),
+    ("min(x, y, 0.5) + max(x, -y)", lambda x, y: min(x, y, 0.5) + max(x, -y)),
+    ("exp(-x) * log(y) + sqrt(x*y) - abs(-x)", lambda x, y: math.exp(-x) * math.log(y) + math.sqrt(x * y) - x),
+    ("sin(x)^2 + cos(x)^2", lambda x, y: 1.0),
+    ("atan2(y, x) + tanh(x) - pow(x, y)", lambda x, y: math.atan2(y, x) + math.tanh(x) - x ** y),
+    ("1e-3 * .5e2 + 2.5E+1", lambda x, y: 1e-3 * 50 + 25),
+    ("if(x > y, 1, 2) + floor(y / x)", lambda x, y: (1 if x > y else 2) + math.floor(y / x)),
+]
+
+
+@pytest.mark.parametrize("text,ref", EXPRESSIONS, ids=[e[0] for e in EXPRESSIONS])
+def test_oracle_expressions(text, ref):
+    for x, y in [(2.0, 3.0), (0.7, 0.2), (1.5, 4.5)]:
+        v, rc = _eval(text, ("x", "y"), (x, y))
+        assert rc == 0
+        assert v == pytest.approx(ref(x, y), rel=1e-14, abs=1e-14)
+
+
+def test_oracle_context_functions_and_zero_literal():
+    ctx = "[parser_context.f]\ntype = function\nexpression = a, b: a*10 + b\n[parser_context.k]\ntype = constant\nvalue = 4\n"
+    v, rc = _eval("f(x, k) + f(1, 2)", ("x",), (3.0,), ctx)
+    assert (v, rc) == (34.0 + 12.0, 0)
+    for zero in ["0", "0.0", "-0", "", "  0.  "]:
+        assert _eval(zero)[1] == 1  # term absent, as the reference treats it
+    assert _eval("no_value")[0] == np.finfo(float).max
