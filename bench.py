@@ -106,8 +106,9 @@ def main():
     u0 = w.initial_state(m)
     m.set_state(u0, 0.0)
     setup_s = time.perf_counter() - t_setup
-    print(f"# setup {setup_s:.1f}s, {m.n_dof} DOF, {m.n_blocks} blocks of {m.block_size}x{m.block_size}",
-          file=sys.stderr, flush=True)
+    tuning, chosen = m.spmv_tuning()
+    print(f"# setup {setup_s:.1f}s, {m.n_dof} DOF, {m.n_blocks} blocks of {m.block_size}x{m.block_size}; "
+          f"SpMV autotune {tuning} -> {chosen}", file=sys.stderr, flush=True)
 
     t = 0.0
     for k in range(args.warmup):
@@ -181,6 +182,7 @@ def main():
         "krylov_iterations_per_step": lin / args.steps,
         "kernels": kinds,
         "roofline": roofline,
+        "spmv_autotune": {"candidates_us": tuning, "chosen": chosen},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -62,6 +62,7 @@ _SIGS = {
     "dcg_linear_solve": (_I, [_P, _P, _P, _P, _D, _I, ctypes.POINTER(_I)]),
     "dcg_profile_enable": (_I, [_P, _I]),
     "dcg_profile_read": (_I, [_P, _I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "dcg_spmv_tuning": (_I, [_P, ctypes.POINTER(_I), _P, _P, ctypes.POINTER(_I)]),
 }
 
 PROFILE_KINDS = {"spmv64": 0, "spmv32": 1, "assembly_jacobian": 2, "assembly_residual": 3, "precond_setup": 4}
@@ -230,6 +231,18 @@ class Model:
         _check(self._lib.dcg_profile_read(self._h, PROFILE_KINDS[kind], ctypes.byref(l), ctypes.byref(ms),
                                           ctypes.byref(b)))
         return l.value, ms.value, b.value
+
+    def spmv_tuning(self):
+        """{variant name: microseconds} of the set-up autotuner and the chosen variant."""
+        n, ch = _I(), _I()
+        var = np.zeros(16, dtype=np.int32)
+        us = np.zeros(16)
+        _check(self._lib.dcg_spmv_tuning(self._h, ctypes.byref(n), _ptr(var), _ptr(us), ctypes.byref(ch)))
+        names = {0: "lds_entries", 1: "rows_g8", 2: "rows_g16", 3: "rows_g32", 4: "rows_g64", 5: "lds_blocks"}
+        out = {}
+        for v, t in zip(var[:n.value], us[:n.value]):
+            out[names[int(v) % 100] + ("_fp32" if v >= 100 else "")] = float(t)
+        return out, names[ch.value]
 
     # ---- helpers on the layout
     def slot_dofs(self):

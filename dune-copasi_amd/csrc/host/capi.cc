@@ -228,4 +228,17 @@ int dcg_profile_read(dcg_model_t h, int kind, long* launches, double* millisecon
   });
 }
 
+int dcg_spmv_tuning(dcg_model_t h, int* count, int* variant, double* microseconds, int* chosen) {
+  if (check(h)) return -1;
+  const auto& t = h->m->spmv_tuning();
+  const int n = static_cast<int>(std::min<size_t>(t.size(), 16));
+  if (count) *count = n;
+  for (int k = 0; k < n; ++k) {
+    if (variant) variant[k] = t[k].first;
+    if (microseconds) microseconds[k] = t[k].second;
+  }
+  if (chosen) *chosen = h->m->spmv_variant();
+  return 0;
+}
+
 }  // extern "C"

@@ -603,8 +603,10 @@ std::string gen_volume(const ModelSpec& m, const Grid& g, int comp, const std::s
     if (any) {
       o << "__device__ constexpr double KD[NCLASS][NV][NV][BC][BC] = " << array_init(kd) << ";\n";
       o << "template <int P> __device__ __forceinline__ double const_diffusion(int a, int c, int i, int j) { return KD[P][a][c][i][j]; }\n";
+      o << "__device__ __forceinline__ double kd_rt(int p, int a, int c, int i, int j) { return KD[p][a][c][i][j]; }\n";
     } else {
       o << "template <int P> __device__ __forceinline__ double const_diffusion(int, int, int, int) { return 0.0; }\n";
+      o << "__device__ __forceinline__ double kd_rt(int, int, int, int, int) { return 0.0; }\n";
     }
   }
   // quadrature point kernel
@@ -822,6 +824,297 @@ std::string gen_facet(const ModelSpec& m, const Grid& g, int cin, int cout, bool
   return full.str();
 }
 
+
+// Tiled write-once volume kernel.  One workgroup owns a TX x TY x TZ box of
+// vertices; it stages the state of the surrounding vertex box and the class
+// masks of the surrounding cells in LDS, computes per (cell, class, point)
+// Jacobian coefficients (or per-cell element residuals) once, and then every
+// thread gathers one BCSR block (or one residual entry) of its rows from the
+// cells that touch it and stores it exactly once.  No zero fill, no colours,
+// no atomics; the summation order is fixed.
+const char* kTiled = R"(
+namespace %C% {
+constexpr int XB = TX + 2, YB = DIM > 1 ? TY + 2 : 1, ZB = DIM > 2 ? TZ + 2 : 1;
+constexpr int CXB = TX + 1, CYB = DIM > 1 ? TY + 1 : 1, CZB = DIM > 2 ? TZ + 1 : 1;
+constexpr int NVB = XB * YB * ZB, NCB = CXB * CYB * CZB, NROW = TX * TY * TZ;
+constexpr int NST = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
+constexpr int OY = DIM > 1 ? 1 : 0, OZ = DIM > 2 ? 1 : 0;
+constexpr int MC_WORDS = NCB * NCLASS * NQ * BC * BC;
+constexpr int RL_WORDS = NCB * NCORNER * BC;
+constexpr int UNION_WORDS = MC_WORDS > RL_WORDS ? MC_WORDS : RL_WORDS;
+
+// element residual of class p (runtime) from corner values
+__device__ __forceinline__ void residual_rt(int p, const double (&xl)[NV][BC], double cx, double cy, double cz,
+                                            double t, double cm, double ca, double (&rl)[NV][BC]) {
+  for (int a = 0; a < NV; ++a)
+    for (int i = 0; i < BC; ++i) rl[a][i] = 0.0;
+  for (int q = 0; q < NQ; ++q) {
+    double u[BC], g[BC][3];
+    for (int i = 0; i < BC; ++i) {
+      double s = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+      for (int a = 0; a < NV; ++a) {
+        s += PHI[q][a] * xl[a][i];
+        g0 += GPHI[p][q][a][0] * xl[a][i];
+        if (DIM > 1) g1 += GPHI[p][q][a][DIM > 1 ? 1 : 0] * xl[a][i];
+        if (DIM > 2) g2 += GPHI[p][q][a][DIM > 2 ? 2 : 0] * xl[a][i];
+      }
+      u[i] = s;
+      g[i][0] = g0;
+      g[i][1] = g1;
+      g[i][2] = g2;
+    }
+    const double px = cx + XQ[p][q][0], py = cy + XQ[p][q][1], pz = cz + XQ[p][q][2];
+    double rs[BC], rf[BC][DIM];
+    qp(q, u, g, px, py, pz, t, cm, ca, rs, rf, nullptr, nullptr, nullptr);
+    for (int a = 0; a < NV; ++a)
+      for (int i = 0; i < BC; ++i) {
+        double s = rs[i] * PHI[q][a];
+        for (int k = 0; k < DIM; ++k) s += rf[i][k] * GPHI[p][q][a][k];
+        rl[a][i] += s;
+      }
+  }
+}
+}  // namespace %C%
+
+extern "C" __global__ void __launch_bounds__(256)
+%KERNEL%(const int* __restrict__ tiles, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,
+         const double* __restrict__ x, const double* __restrict__ base, double t, double cm, double ca,
+         double* __restrict__ r, double* __restrict__ val, const int* __restrict__ own_start,
+         const unsigned* __restrict__ own_mask, int n0, int n1, int n2, double ox, double oy, double oz, double hx,
+         double hy, double hz, int mode) {
+  using namespace %C%;
+  __shared__ double xs[NVB * BC];
+  __shared__ int sl[NVB];
+  __shared__ unsigned char cms[NCB];
+  __shared__ double work[UNION_WORDS];
+  const int ntx = (n0 + 1 + TX - 1) / TX;
+  const int nty = DIM > 1 ? (n1 + 1 + TY - 1) / TY : 1;
+  const int tile = tiles[blockIdx.x];
+  const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+  const int vi0 = tx * TX - 1, vj0 = DIM > 1 ? ty * TY - 1 : 0, vk0 = DIM > 2 ? tz * TZ - 1 : 0;
+  // phase 0: vertex box (state, slots) and cell box (class masks)
+  for (int b = threadIdx.x; b < NVB; b += blockDim.x) {
+    const int bi = b % XB, bj = (b / XB) % YB, bk = b / (XB * YB);
+    const int gi = vi0 + bi, gj = vj0 + bj, gk = vk0 + bk;
+    int s = -1;
+    if (gi >= 0 && gi <= n0 && gj >= 0 && gj <= (DIM > 1 ? n1 : 0) && gk >= 0 && gk <= (DIM > 2 ? n2 : 0))
+      s = g2l[gi + (n0 + 1) * (gj + (n1 + 1) * gk)];
+    sl[b] = s;
+    for (int i = 0; i < BC; ++i) xs[b * BC + i] = s >= 0 ? x[(size_t)s * DCG_B + i] : 0.0;
+  }
+  for (int c = threadIdx.x; c < NCB; c += blockDim.x) {
+    const int ci = c % CXB, cj = (c / CXB) % CYB, ck = c / (CXB * CYB);
+    const int gi = vi0 + ci, gj = vj0 + cj, gk = vk0 + ck;
+    unsigned char m = 0;
+    if (gi >= 0 && gi < n0 && gj >= 0 && gj < (DIM > 1 ? n1 : 1) && gk >= 0 && gk < (DIM > 2 ? n2 : 1))
+      m = cmask[gi + n0 * (gj + n1 * gk)];
+    cms[c] = m;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  if (mode & 2) {
+    // phase 1: Jacobian coefficients per (cell, class, point); class and point
+    // are uniform per wave so the reference-element tables are scalar loads.
+    // LDS layout is cell-minor: work[((p*NQ + q)*BC*BC + i*BC + j)*NCB + cell].
+    for (int pq0 = wave; pq0 < NCLASS * NQ; pq0 += nwave) {
+      const int pq = __builtin_amdgcn_readfirstlane(pq0);
+      const int p = pq / NQ, q = pq % NQ;
+      for (int cl = lane; cl < NCB; cl += 64) {
+        if (!((cms[cl] >> p) & 1u)) continue;
+        const int ci = cl % CXB, cj = (cl / CXB) % CYB, ck = cl / (CXB * CYB);
+        double xl[NV][BC];
+        for (int a = 0; a < NV; ++a) {
+          const int c = CORNER[p][a];
+          const int b = (ci + (c & 1)) + XB * ((cj + ((c >> 1) & 1)) + YB * (ck + ((c >> 2) & 1)));
+          for (int i = 0; i < BC; ++i) xl[a][i] = xs[b * BC + i];
+        }
+        double u[BC], g[BC][3];
+        for (int i = 0; i < BC; ++i) {
+          double s = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+          for (int a = 0; a < NV; ++a) {
+            s += PHI[q][a] * xl[a][i];
+            g0 += GPHI[p][q][a][0] * xl[a][i];
+            if (DIM > 1) g1 += GPHI[p][q][a][DIM > 1 ? 1 : 0] * xl[a][i];
+            if (DIM > 2) g2 += GPHI[p][q][a][DIM > 2 ? 2 : 0] * xl[a][i];
+          }
+          u[i] = s;
+          g[i][0] = g0;
+          g[i][1] = g1;
+          g[i][2] = g2;
+        }
+        const double px = ox + (vi0 + ci) * hx + XQ[p][q][0];
+        const double py = oy + (vj0 + cj) * hy + XQ[p][q][1];
+        const double pz = oz + (vk0 + ck) * hz + XQ[p][q][2];
+        double rs[BC], rf[BC][DIM], mc[BC][BC];
+        qp(q, u, g, px, py, pz, t, cm, ca, rs, rf, mc, nullptr, nullptr);
+        for (int i = 0; i < BC; ++i)
+          for (int j = 0; j < BC; ++j) work[(size_t)(pq * BC * BC + i * BC + j) * NCB + cl] = mc[i][j];
+      }
+    }
+    __syncthreads();
+    // phase 2: one BCSR block per (row, stencil offset), written once; the
+    // offset is uniform per wave, lanes run over the tile's rows.
+    for (int o0 = wave; o0 < NST; o0 += nwave) {
+      const int o = __builtin_amdgcn_readfirstlane(o0);
+      const int dx = o % 3 - 1, dy = DIM > 1 ? (o / 3) % 3 - 1 : 0, dz = DIM > 2 ? o / 9 - 1 : 0;
+      const bool diag = o == NST / 2;
+      for (int lr = lane; lr < NROW; lr += 64) {
+        const int li = lr % TX, lj = DIM > 1 ? (lr / TX) % TY : 0, lk = DIM > 2 ? lr / (TX * TY) : 0;
+        const int vb = (li + 1) + XB * ((lj + OY) + YB * (lk + OZ));
+        const int s = sl[vb];
+        if (s < 0) continue;
+        const unsigned mk = own_mask[s];
+        if (!((mk >> o) & 1u)) continue;
+        const int pos = own_start[s] + __popc(mk & ((1u << o) - 1u));
+        double acc[BC][BC];
+        for (int i = 0; i < BC; ++i)
+          for (int j = 0; j < BC; ++j) acc[i][j] = 0.0;
+        for (int ez = 0; ez < (DIM > 2 ? 2 : 1); ++ez) {
+          if (DIM > 2 && ((dz == 1 && ez == 0) || (dz == -1 && ez == 1))) continue;
+          for (int ey = 0; ey < (DIM > 1 ? 2 : 1); ++ey) {
+            if (DIM > 1 && ((dy == 1 && ey == 0) || (dy == -1 && ey == 1))) continue;
+            for (int ex = 0; ex < 2; ++ex) {
+              if ((dx == 1 && ex == 0) || (dx == -1 && ex == 1)) continue;
+              const int cl = (li + ex) + CXB * ((lj + ey) + CYB * (lk + ez));
+              const unsigned m = cms[cl];
+              if (!m) continue;
+              // corner bits of v and w in this cell (uniform per wave)
+              const int cv = (1 - ex) | (DIM > 1 ? (1 - ey) << 1 : 0) | (DIM > 2 ? (1 - ez) << 2 : 0);
+              const int cw = (1 - ex + dx) | (DIM > 1 ? (1 - ey + dy) << 1 : 0) | (DIM > 2 ? (1 - ez + dz) << 2 : 0);
+              for (int p = 0; p < NCLASS; ++p) {
+                const int la = LOC[p][cv], lc = LOC[p][cw];
+                if (la < 0 || lc < 0) continue;
+                if (!((m >> p) & 1u)) continue;
+                for (int q = 0; q < NQ; ++q) {
+                  const double w = PP[q][la][lc];
+                  const double* mcp = work + (size_t)((p * NQ + q) * BC * BC) * NCB + cl;
+                  for (int i = 0; i < BC; ++i)
+                    for (int j = 0; j < BC; ++j) acc[i][j] += w * mcp[(size_t)(i * BC + j) * NCB];
+                }
+                for (int i = 0; i < BC; ++i)
+                  for (int j = 0; j < BC; ++j) acc[i][j] += ca * kd_rt(p, la, lc, i, j);
+              }
+            }
+          }
+        }
+        double* blk = val + (size_t)pos * (DCG_B * DCG_B);
+        for (int i = 0; i < DCG_B; ++i)
+          for (int j = 0; j < DCG_B; ++j)
+            blk[i * DCG_B + j] = (i < BC && j < BC) ? acc[i < BC ? i : 0][j < BC ? j : 0] : ((diag && i == j) ? 1.0 : 0.0);
+      }
+    }
+    __syncthreads();
+  }
+  if (mode & 1) {
+    // phase 1: element residual per cell, folded onto the cell's corners
+    for (int cl = threadIdx.x; cl < NCB; cl += blockDim.x) {
+      double* dst = work + (size_t)cl * (NCORNER * BC);
+      for (int k = 0; k < NCORNER * BC; ++k) dst[k] = 0.0;
+      const unsigned m = cms[cl];
+      if (!m) continue;
+      const int ci = cl % CXB, cj = (cl / CXB) % CYB, ck = cl / (CXB * CYB);
+      const double cx = ox + (vi0 + ci) * hx, cy = oy + (vj0 + cj) * hy, cz = oz + (vk0 + ck) * hz;
+      for (int p = 0; p < NCLASS; ++p) {
+        if (!((m >> p) & 1u)) continue;
+        double xl[NV][BC], rl[NV][BC];
+        for (int a = 0; a < NV; ++a) {
+          const int c = CORNER[p][a];
+          const int b = (ci + (c & 1)) + XB * ((cj + ((c >> 1) & 1)) + YB * (ck + ((c >> 2) & 1)));
+          for (int i = 0; i < BC; ++i) xl[a][i] = xs[b * BC + i];
+        }
+        residual_rt(p, xl, cx, cy, cz, t, cm, ca, rl);
+        for (int a = 0; a < NV; ++a)
+          for (int i = 0; i < BC; ++i) dst[CORNER[p][a] * BC + i] += rl[a][i];
+      }
+    }
+    __syncthreads();
+    // phase 2: r = base + sum of the cells around each row vertex
+    for (int it = threadIdx.x; it < NROW * DCG_B; it += blockDim.x) {
+      const int lr = it / DCG_B, i = it % DCG_B;
+      const int li = lr % TX, lj = DIM > 1 ? (lr / TX) % TY : 0, lk = DIM > 2 ? lr / (TX * TY) : 0;
+      const int vb = (li + 1) + XB * ((lj + OY) + YB * (lk + OZ));
+      const int s = sl[vb];
+      if (s < 0) continue;
+      double v = base ? base[(size_t)s * DCG_B + i] : 0.0;
+      if (i < BC) {
+        for (int ez = 0; ez < (DIM > 2 ? 2 : 1); ++ez)
+          for (int ey = 0; ey < (DIM > 1 ? 2 : 1); ++ey)
+            for (int ex = 0; ex < 2; ++ex) {
+              const int cl = (li + ex) + CXB * ((lj + ey) + CYB * (lk + ez));
+              if (!cms[cl]) continue;
+              const int cv = (1 - ex) | (DIM > 1 ? (1 - ey) << 1 : 0) | (DIM > 2 ? (1 - ez) << 2 : 0);
+              v += work[(size_t)cl * (NCORNER * BC) + cv * BC + i];
+            }
+      }
+      r[(size_t)s * DCG_B + i] = v;
+    }
+  }
+}
+)";
+
+}  // namespace
+
+bool tile_eligible(const ModelSpec& m, int comp) {
+  if (m.jacobian == JacobianType::Numerical && !m.is_linear) return false;
+  for (int s : m.comp_species[comp]) {
+    const auto& sp = m.species[s];
+    for (int k = 0; k < 3; ++k)
+      if (sp.velocity[k]) return false;
+    for (const auto& dt : sp.diffusion) {
+      if (!dt.jac.empty() || !tensor_const(dt.D)) return false;
+    }
+  }
+  return true;
+}
+
+std::array<int, 3> choose_tile(const Grid& g, int bc) {
+  // largest candidate whose LDS (staging + coefficient union) stays under 48 KiB
+  std::vector<std::array<int, 3>> cand;
+  if (g.dim == 3) cand = {{8, 4, 4}, {4, 4, 4}, {4, 4, 2}, {4, 2, 2}, {2, 2, 2}};
+  else if (g.dim == 2) cand = {{16, 8, 1}, {8, 8, 1}, {8, 4, 1}, {4, 4, 1}, {2, 2, 1}};
+  else cand = {{128, 1, 1}, {64, 1, 1}, {32, 1, 1}};
+  const int nq = static_cast<int>(element_rule(g.dim, g.cube).w.size());
+  for (auto c : cand) {
+    long xb = c[0] + 2, yb = g.dim > 1 ? c[1] + 2 : 1, zb = g.dim > 2 ? c[2] + 2 : 1;
+    long cxb = c[0] + 1, cyb = g.dim > 1 ? c[1] + 1 : 1, czb = g.dim > 2 ? c[2] + 1 : 1;
+    long ncb = cxb * cyb * czb;
+    long mc = ncb * g.nclass * nq * bc * bc, rl = ncb * g.ncorner * bc;
+    long bytes = 8 * std::max(mc, rl) + 8 * xb * yb * zb * bc + 4 * xb * yb * zb + ncb;
+    if (bytes <= 48 * 1024) return c;
+  }
+  return cand.back();
+}
+
+std::string gen_tiled(const ModelSpec& m, const Grid& g, int comp, const std::string& ns, const std::string& kname,
+                      std::array<int, 3> tile) {
+  const auto T = element_tables(g);
+  std::ostringstream o;
+  o << "namespace " << ns << " {\n";
+  o << "constexpr int TX = " << tile[0] << ", TY = " << tile[1] << ", TZ = " << tile[2] << ";\n";
+  std::vector<double> pp;
+  for (int q = 0; q < T.nq; ++q)
+    for (int a = 0; a < T.nv; ++a)
+      for (int c = 0; c < T.nv; ++c) pp.push_back(T.phi[q * T.nv + a] * T.phi[q * T.nv + c]);
+  o << "__device__ constexpr double PP[NQ][NV][NV] = " << array_init(pp) << ";\n";
+  std::vector<int> loc;
+  for (int p = 0; p < g.nclass; ++p)
+    for (int c = 0; c < g.ncorner; ++c) {
+      int l = -1;
+      for (int a = 0; a < g.nvpe; ++a)
+        if (g.corner[p][a] == c) l = a;
+      loc.push_back(l);
+    }
+  o << "__device__ constexpr int LOC[NCLASS][NCORNER] = " << int_init(loc) << ";\n";
+  o << "}  // namespace " << ns << "\n";
+  std::string body = kTiled;
+  replace_all(body, "%C%", ns);
+  replace_all(body, "%KERNEL%", kname);
+  (void)m;
+  (void)comp;
+  return o.str() + body;
+}
+
+namespace {
 }  // namespace
 
 GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L) {
@@ -834,12 +1127,24 @@ GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L) {
   for (size_t c = 0; c < spec.comp_names.size(); ++c) {
     if (spec.comp_species[c].empty()) {
       out.volume_kernel.push_back("");
+      out.tile_kernel.push_back("");
+      out.tile_dims.push_back({0, 0, 0});
       continue;
     }
     std::string ns = "vol" + std::to_string(c);
     std::string kn = "dcg_volume_" + std::to_string(c);
     src << gen_volume(spec, L.grid, static_cast<int>(c), ns, kn, numerical);
     out.volume_kernel.push_back(kn);
+    if (tile_eligible(spec, static_cast<int>(c))) {
+      auto tile = choose_tile(L.grid, static_cast<int>(spec.comp_species[c].size()));
+      std::string tk = "dcg_tile_" + std::to_string(c);
+      src << gen_tiled(spec, L.grid, static_cast<int>(c), ns, tk, tile);
+      out.tile_kernel.push_back(tk);
+      out.tile_dims.push_back(tile);
+    } else {
+      out.tile_kernel.push_back("");
+      out.tile_dims.push_back({0, 0, 0});
+    }
   }
   // facet kernels for every (in, out, boundary) combination present
   std::vector<std::tuple<int, int, bool>> combos;

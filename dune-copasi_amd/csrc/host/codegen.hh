@@ -7,6 +7,7 @@
 // localAssembleSkeleton / localAssembleBoundary and their Jacobians
 // (local_operator.hh:777-1452) with exact derivatives in all four blocks.
 #pragma once
+#include <array>
 #include <string>
 #include <vector>
 
@@ -25,6 +26,9 @@ struct FacetKernel {
 struct GeneratedSource {
   std::string source;
   std::vector<std::string> volume_kernel;  // per compartment ("" when it has no species)
+  // tiled write-once variant per compartment ("" when the model needs the coloured kernel)
+  std::vector<std::string> tile_kernel;
+  std::vector<std::array<int, 3>> tile_dims;
   std::vector<FacetKernel> facet_kernels;
 };
 

@@ -113,6 +113,9 @@ void Model::build(const std::string& ini, const std::string& overrides) {
     cfg_.nl_min = static_cast<int>(nr[0]);
     cfg_.nl_max = static_cast<int>(nr[1]);
   }
+  cfg_.fixed_linear_tolerance = c.get(p + "nonlinear_solver.dx_inverse_fixed_tolerance", false);
+  cfg_.min_linear_reduction = c.get(p + "nonlinear_solver.dx_inverse_min_relative_tolerance", 0.1);
+  cfg_.linearization_threshold = c.get(p + "nonlinear_solver.linearization_threshold", 0.0);
   const std::string norm = c.get(p + "nonlinear_solver.norm", std::string("l_2"));
   if (norm == "l_2") cfg_.norm = 0;
   else if (norm == "l_inf") cfg_.norm = 1;
@@ -168,12 +171,60 @@ void Model::setup_device() {
   plan_.chunk_row = chunk_row_.get();
   plan_.rp = row_ptr_.get();
   plan_.col = col_.get();
+  plan_.nrows = static_cast<int>(ns);
+  // row-split launches: about 8 rows per thread group pass, grid a multiple of 8
+  {
+    long wgs = std::max<long>(8, std::min<long>(4096, (ns + 63) / 64));
+    wgs = (wgs + 7) / 8 * 8;
+    plan_.row_wgs = static_cast<int>(wgs);
+    plan_.rows_per_wg = static_cast<int>((ns + wgs - 1) / wgs);
+  }
   // per-compartment assembly work lists
   comp_dev_.resize(L_.comps.size());
+  all_tiled_ = true;
+  const Grid& gr = L_.grid;
   for (size_t c = 0; c < L_.comps.size(); ++c) {
-    comp_dev_[c].cells.upload(L_.comps[c].cells, stream_);
-    comp_dev_[c].cell_mask.upload(L_.comps[c].cell_mask, stream_);
-    comp_dev_[c].cell_slots.upload(L_.comps[c].cell_slots, stream_);
+    const auto& C = L_.comps[c];
+    comp_dev_[c].cells.upload(C.cells, stream_);
+    comp_dev_[c].cell_mask.upload(C.cell_mask, stream_);
+    comp_dev_[c].cell_slots.upload(C.cell_slots, stream_);
+    if (!C.b) continue;
+    if (gen_.tile_kernel[c].empty()) {
+      all_tiled_ = false;
+      continue;
+    }
+    std::vector<int> g2l(gr.nvert, -1);
+    for (long l = 0; l < C.nslot; ++l) g2l[C.nodes[l]] = static_cast<int>(C.slot_off + l);
+    std::vector<unsigned char> cm(gr.ncell, 0);
+    for (size_t i = 0; i < C.cells.size(); ++i) cm[C.cells[i]] = C.cell_mask[i];
+    const auto td = gen_.tile_dims[c];
+    const long ntx = (gr.n[0] + 1 + td[0] - 1) / td[0];
+    const long nty = gr.dim > 1 ? (gr.n[1] + 1 + td[1] - 1) / td[1] : 1;
+    const long ntz = gr.dim > 2 ? (gr.n[2] + 1 + td[2] - 1) / td[2] : 1;
+    std::vector<char> used(ntx * nty * ntz, 0);
+    for (long l = 0; l < C.nslot; ++l) {
+      long v = C.nodes[l];
+      long i = v % (gr.n[0] + 1), rest = v / (gr.n[0] + 1);
+      long j = gr.dim > 1 ? rest % (gr.n[1] + 1) : 0, k = gr.dim > 2 ? rest / (gr.n[1] + 1) : 0;
+      used[(i / td[0]) + ntx * ((j / td[1]) + nty * (k / td[2]))] = 1;
+    }
+    std::vector<int> tiles;
+    for (long t = 0; t < ntx * nty * ntz; ++t)
+      if (used[t]) tiles.push_back(static_cast<int>(t));
+    comp_dev_[c].g2l.upload(g2l, stream_);
+    comp_dev_[c].cell_mask_grid.upload(cm, stream_);
+    comp_dev_[c].tiles.upload(tiles, stream_);
+    comp_dev_[c].ntiles = static_cast<int>(tiles.size());
+  }
+  {
+    // blocks outside a row's own-compartment run couple two compartments
+    std::vector<int> extra;
+    for (long s = 0; s < ns; ++s) {
+      int own_end = L_.own_start[s] + __builtin_popcount(L_.own_mask[s]);
+      for (int p2 = L_.row_ptr[s]; p2 < L_.row_ptr[s + 1]; ++p2)
+        if (p2 < L_.own_start[s] || p2 >= own_end) extra.push_back(p2);
+    }
+    extra_pos_.upload(extra, stream_);
   }
   // facet records per facet kernel, grouped by colour
   facet_dev_.resize(gen_.facet_kernels.size());
@@ -265,7 +316,7 @@ void Model::setup_device() {
     val32_.resize(static_cast<size_t>(matrix_values()));
     for (auto& v : kf_) v.resize(nv);
   }
-  const size_t nparts = std::max<size_t>(static_cast<size_t>(plan_.nchunk) + 8,
+  const size_t nparts = std::max<size_t>(std::max<size_t>(static_cast<size_t>(plan_.nchunk), plan_.row_wgs) + 8,
                                          static_cast<size_t>((ns + 255) / 256) + 8);
   part_.resize(std::max<size_t>(2 * nparts, 8 * 1024 + 64));
   scalar_out_.resize(16);
@@ -305,23 +356,102 @@ void Model::setup_device() {
     up_rows_.upload(rows_u, stream_);
     lu_.resize(static_cast<size_t>(matrix_values()));
   }
+  tune_spmv();
   synchronize();
+}
+
+// Times every SpMV variant valid for this block size on the model's own
+// pattern (zero values, as in the solver: x, y and two dot operands) and keeps
+// the fastest.
+void Model::tune_spmv() {
+  tuning_.clear();
+  plan_.variant = tune_spmv_plan(false);
+  plan32_ = plan_;
+  if (cfg_.precision == Precision::Mixed) plan32_.variant = tune_spmv_plan(true);
+}
+
+int Model::tune_spmv_plan(bool fp32) {
+  const long nv = L_.nslot * L_.B;
+  k::fill(val_.get(), 0.0, matrix_values(), stream_);
+  for (int v = 0; v < 4; ++v) k::fill(kv_[v].get(), 1.0, nv, stream_);
+  if (fp32) {
+    k::fill(val32_.get(), 0.0f, matrix_values(), stream_);
+    for (int v = 0; v < 4; ++v) k::fill(kf_[v].get(), 1.0f, nv, stream_);
+  }
+  k::SpmvPlan pl = plan_;
+  hipEvent_t e0, e1;
+  DCG_HIP(hipEventCreate(&e0));
+  DCG_HIP(hipEventCreate(&e1));
+  double best = 1e30;
+  int best_v = 0;
+  auto run = [&]() {
+    if (fp32)
+      k::spmv(L_.B, pl, val32_.get(), kf_[0].get(), kf_[1].get(), kf_[2].get(), kf_[3].get(), part_.get(), nullptr, 0,
+              stream_);
+    else
+      k::spmv(L_.B, pl, val_.get(), kv_[0].get(), kv_[1].get(), kv_[2].get(), kv_[3].get(), part_.get(), nullptr, 0,
+              stream_);
+  };
+  for (int variant : {5, 0, 1, 2, 3, 4}) {
+    const int G = (variant == 0 || variant == 5) ? 0 : (4 << variant);  // 8, 16, 32, 64
+    if (G && G % L_.B != 0) continue;
+    pl.variant = variant;
+    for (int w = 0; w < 3; ++w) run();
+    const int reps = 5;
+    DCG_HIP(hipEventRecord(e0, stream_));
+    for (int w = 0; w < reps; ++w) run();
+    DCG_HIP(hipEventRecord(e1, stream_));
+    DCG_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    DCG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    const double us = 1e3 * ms / reps;
+    tuning_.push_back({variant + (fp32 ? 100 : 0), us});
+    if (us < best) {
+      best = us;
+      best_v = variant;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best_v;
 }
 
 void Model::compile_and_load() {
   auto code = rtc_compile(gen_.source, "dcopasi_assembly");
   module_.load(code);
   vol_fn_.assign(L_.comps.size(), nullptr);
-  for (size_t c = 0; c < L_.comps.size(); ++c)
+  tile_fn_.assign(L_.comps.size(), nullptr);
+  for (size_t c = 0; c < L_.comps.size(); ++c) {
     if (!gen_.volume_kernel[c].empty()) vol_fn_[c] = module_.function(gen_.volume_kernel[c]);
+    if (!gen_.tile_kernel[c].empty()) tile_fn_[c] = module_.function(gen_.tile_kernel[c]);
+  }
   facet_fn_.clear();
   for (const auto& fk : gen_.facet_kernels) facet_fn_.push_back(module_.function(fk.name));
 }
 
 // ------------------------------------------------------------------ assembly
-void Model::launch_assembly(const double* x, double t, double cm, double ca, double* r, double* val, int mode) {
+void Model::launch_assembly(const double* x, double t, double cm, double ca, const double* base, double* r,
+                            double* val, int mode) {
   const Grid& g = L_.grid;
+  const long n = L_.nslot * L_.B;
+  const double* base_eff = base;
+  if ((mode & 1) && !all_tiled_) {
+    // the coloured kernels accumulate: start from base
+    if (!base) k::fill(r, 0.0, n, stream_);
+    else if (base != r)
+      DCG_HIP(hipMemcpyAsync(r, base, n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    base_eff = r;
+  }
+  if (mode & 2) {
+    if (!all_tiled_) {
+      k::init_jacobian(val, matrix_values(), diag_pos_.get(), slot_b_.get(), static_cast<int>(L_.nslot), L_.B,
+                       stream_);
+    } else if (extra_pos_.size()) {
+      k::zero_blocks(val, extra_pos_.get(), static_cast<long>(extra_pos_.size()), L_.B, stream_);
+    }
+  }
   int n0 = static_cast<int>(g.n[0]), n1 = g.dim > 1 ? static_cast<int>(g.n[1]) : 1;
+  int n2 = g.dim > 2 ? static_cast<int>(g.n[2]) : 1;
   double ox = g.origin[0], oy = g.origin[1], oz = g.origin[2];
   double hx = g.h[0], hy = g.dim > 1 ? g.h[1] : 0.0, hz = g.dim > 2 ? g.h[2] : 0.0;
   const int* own_start = own_start_.get();
@@ -329,6 +459,16 @@ void Model::launch_assembly(const double* x, double t, double cm, double ca, dou
   for (size_t c = 0; c < L_.comps.size(); ++c) {
     if (!vol_fn_[c]) continue;
     const auto& C = L_.comps[c];
+    if (tile_fn_[c]) {
+      const int* tiles = comp_dev_[c].tiles.get();
+      const int* g2l = comp_dev_[c].g2l.get();
+      const unsigned char* cmask = comp_dev_[c].cell_mask_grid.get();
+      void* args[] = {&tiles, &g2l, &cmask, &x, &base_eff, &t, &cm, &ca, &r, &val, &own_start, &own_mask,
+                      &n0, &n1, &n2, &ox, &oy, &oz, &hx, &hy, &hz, &mode};
+      if (comp_dev_[c].ntiles > 0)
+        DCG_HIP(hipModuleLaunchKernel(tile_fn_[c], comp_dev_[c].ntiles, 1, 1, 256, 1, 1, 0, stream_, args, nullptr));
+      continue;
+    }
     const int* cells = comp_dev_[c].cells.get();
     const unsigned char* cmask = comp_dev_[c].cell_mask.get();
     const int* cslots = comp_dev_[c].cell_slots.get();
@@ -387,12 +527,11 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
   host.abstol = abstol;
   host.maxit = maxit;
   DCG_HIP(hipMemcpyAsync(sc_.get(), &host, sizeof host, hipMemcpyHostToDevice, stream_));
-  const int nblk = (ns + 255) / 256;
   const int nred = std::max(1, std::min(1024, static_cast<int>((n + 255) / 256)));
+  const int nblk = (ns + 255) / 256;
   k::bicg_init<double>(n, b, x, r, rh, p, v, part_.get(), nred, sc_.get(), stream_);
   const double* dinv = cfg_.precond == Preconditioner::Jacobi ? dinv_.get() : nullptr;
   const bool ilu = cfg_.precond == Preconditioner::ILU0;
-  const int* skip = &sc_.get()->done;
   auto apply_ilu = [&](const double* in, double* out) {
     for (size_t l = 0; l + 1 < lo_ptr_.size(); ++l)
       k::ilu_lower<double>(B, lo_ptr_[l + 1] - lo_ptr_[l], lo_rows_.get() + lo_ptr_[l], row_ptr_.get(), col_.get(),
@@ -407,16 +546,16 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
       k::bicg_p<double>(B, ns, r, p, v, ph, dinv, sc_.get(), stream_);
       if (ilu) apply_ilu(p, ph);
       prof_begin(kSpmv64);
-      k::spmv(B, plan_, val_.get(), ph, v, rh, nullptr, part_.get(), skip, stream_);
+      k::spmv(B, plan_, val_.get(), ph, v, rh, nullptr, part_.get(), sc_.get(), 1, stream_);
+      k::bicg_fin(plan_.parts(), part_.get(), 2, 1, sc_.get(), stream_);
       prof_end(kSpmv64, spmv_bytes(8, 8, 1));
-      k::bicg_fin(plan_.nchunk, part_.get(), 2, 1, sc_.get(), stream_);
       k::bicg_s<double>(B, ns, r, v, s, sh, dinv, sc_.get(), part_.get(), stream_);
       if (ilu) apply_ilu(s, sh);
       k::bicg_fin(nblk, part_.get(), 1, 2, sc_.get(), stream_);
       prof_begin(kSpmv64);
-      k::spmv(B, plan_, val_.get(), sh, t, s, t, part_.get(), skip, stream_);
+      k::spmv(B, plan_, val_.get(), sh, t, s, t, part_.get(), sc_.get(), 3, stream_);
+      k::bicg_fin(plan_.parts(), part_.get(), 2, 3, sc_.get(), stream_);
       prof_end(kSpmv64, spmv_bytes(8, 8, 1));
-      k::bicg_fin(plan_.nchunk, part_.get(), 2, 3, sc_.get(), stream_);
       k::bicg_xr<double>(B, ns, x, ph, sh, s, t, r, rh, sc_.get(), part_.get(), stream_);
       k::bicg_fin(nblk, part_.get(), 2, 4, sc_.get(), stream_);
     }
@@ -446,12 +585,11 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
   host.abstol = 0.0;
   host.maxit = maxit;
   DCG_HIP(hipMemcpyAsync(sc_.get(), &host, sizeof host, hipMemcpyHostToDevice, stream_));
-  const int nblk = (ns + 255) / 256;
   const int nred = std::max(1, std::min(1024, static_cast<int>((n + 255) / 256)));
+  const int nblk = (ns + 255) / 256;
   k::bicg_init<float>(n, b, x, r, rh, p, v, part_.get(), nred, sc_.get(), stream_);
   const double* dinv = cfg_.precond == Preconditioner::Jacobi ? dinv_.get() : nullptr;
   const bool ilu = cfg_.precond == Preconditioner::ILU0;
-  const int* skip = &sc_.get()->done;
   auto apply_ilu = [&](const float* in, float* out) {
     for (size_t l = 0; l + 1 < lo_ptr_.size(); ++l)
       k::ilu_lower<float>(B, lo_ptr_[l + 1] - lo_ptr_[l], lo_rows_.get() + lo_ptr_[l], row_ptr_.get(), col_.get(),
@@ -466,16 +604,16 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
       k::bicg_p<float>(B, ns, r, p, v, ph, dinv, sc_.get(), stream_);
       if (ilu) apply_ilu(p, ph);
       prof_begin(kSpmv32);
-      k::spmv(B, plan_, val32_.get(), ph, v, rh, nullptr, part_.get(), skip, stream_);
+      k::spmv(B, plan32_, val32_.get(), ph, v, rh, nullptr, part_.get(), sc_.get(), 1, stream_);
+      k::bicg_fin(plan32_.parts(), part_.get(), 2, 1, sc_.get(), stream_);
       prof_end(kSpmv32, spmv_bytes(4, 4, 1));
-      k::bicg_fin(plan_.nchunk, part_.get(), 2, 1, sc_.get(), stream_);
       k::bicg_s<float>(B, ns, r, v, s, sh, dinv, sc_.get(), part_.get(), stream_);
       if (ilu) apply_ilu(s, sh);
       k::bicg_fin(nblk, part_.get(), 1, 2, sc_.get(), stream_);
       prof_begin(kSpmv32);
-      k::spmv(B, plan_, val32_.get(), sh, t, s, t, part_.get(), skip, stream_);
+      k::spmv(B, plan32_, val32_.get(), sh, t, s, t, part_.get(), sc_.get(), 3, stream_);
+      k::bicg_fin(plan32_.parts(), part_.get(), 2, 3, sc_.get(), stream_);
       prof_end(kSpmv32, spmv_bytes(4, 4, 1));
-      k::bicg_fin(plan_.nchunk, part_.get(), 2, 3, sc_.get(), stream_);
       k::bicg_xr<float>(B, ns, x, ph, sh, s, t, r, rh, sc_.get(), part_.get(), stream_);
       k::bicg_fin(nblk, part_.get(), 2, 4, sc_.get(), stream_);
     }
@@ -530,7 +668,7 @@ int Model::krylov(const double* b, double* x, double rtol, int maxit, int* its, 
     // x += d (FP64), res = b - A x (FP64)
     k::to_double(n, d, ax, stream_);
     k::axpy(n, 1.0, ax, x, stream_);
-    k::spmv(L_.B, plan_, val_.get(), x, ax, nullptr, nullptr, nullptr, nullptr, stream_);
+    k::spmv(L_.B, plan_, val_.get(), x, ax, nullptr, nullptr, nullptr, nullptr, 0, stream_);
     DCG_HIP(hipMemcpyAsync(res, b, n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
     k::axpy(n, -1.0, ax, res, stream_);
     k::reduce(n, res, nullptr, 0, part_.get(), scalar_out_.get(), stream_);
@@ -560,27 +698,24 @@ int Model::step(double t, double dt, StepStats* st) {
     for (int j = 0; j < s; ++j) {
       const double am = rk_.a[s - 1][j], ab = dt * rk_.b[s - 1][j];
       if (am == 0.0 && ab == 0.0) continue;
-      launch_assembly(stage_[j].get(), t + rk_.d[j] * dt, am, ab, hist_.get(), nullptr, 1);
+      launch_assembly(stage_[j].get(), t + rk_.d[j] * dt, am, ab, hist_.get(), hist_.get(), nullptr, 1);
     }
     const double cm = rk_.a[s - 1][s], ca = dt * rk_.b[s - 1][s], ts = t + rk_.d[s] * dt;
     double* x = xs_.get();
     DCG_HIP(hipMemcpyAsync(x, stage_[s - 1].get(), bytes, hipMemcpyDeviceToDevice, stream_));
     auto residual = [&]() {
-      DCG_HIP(hipMemcpyAsync(r_.get(), hist_.get(), bytes, hipMemcpyDeviceToDevice, stream_));
       prof_begin(kAssemblyRes);
-      launch_assembly(x, ts, cm, ca, r_.get(), nullptr, 1);
+      launch_assembly(x, ts, cm, ca, hist_.get(), r_.get(), nullptr, 1);
       prof_end(kAssemblyRes, assembly_bytes(false));
     };
     auto jacobian = [&]() {
-      k::init_jacobian(val_.get(), matrix_values(), diag_pos_.get(), slot_b_.get(), static_cast<int>(L_.nslot), L_.B,
-                       stream_);
       prof_begin(kAssemblyJac);
-      launch_assembly(x, ts, cm, ca, nullptr, val_.get(), 2);
+      launch_assembly(x, ts, cm, ca, nullptr, nullptr, val_.get(), 2);
       prof_end(kAssemblyJac, assembly_bytes(true));
     };
-    auto solve = [&]() -> int {
+    auto solve = [&](double rtol) -> int {
       int its = 0, refs = 0;
-      int rc = krylov(r_.get(), z_.get(), cfg_.lin_rtol, cfg_.lin_maxit, &its, &refs);
+      int rc = krylov(r_.get(), z_.get(), rtol, cfg_.lin_maxit, &its, &refs);
       S.linear_iterations += its;
       S.refinement_steps += refs;
       return rc;
@@ -589,7 +724,7 @@ int Model::step(double t, double dt, StepStats* st) {
       // linear defect correction (make_step_operator.hh:210-245)
       residual();
       jacobian();
-      int rc = solve();
+      int rc = solve(cfg_.lin_rtol);
       if (rc) {
         S.status = rc == 3 ? 3 : rc == 4 ? 4 : 2;
         return S.status;
@@ -597,8 +732,16 @@ int Model::step(double t, double dt, StepStats* st) {
       k::axpy(n, -1.0, z_.get(), x, stream_);
       S.newton_iterations += 1;
     } else {
+      // Newton with the control of the reference's configuration
+      // (make_step_operator.hh:246-288): stop when the defect (in the chosen
+      // norm) falls below max(absolute, relative * defect_0); unless the
+      // linear tolerance is fixed, each correction is solved to
+      //   max(stop / (10 d_k), min(min_reduction, d_k^2 / d_{k-1}^2)),
+      // and the Jacobian is re-assembled unless the defect rate stays below
+      // linearization_threshold.
       residual();
-      double nrm0 = residual_norm(r_.get()), nrm = nrm0;
+      double nrm0 = residual_norm(r_.get()), nrm = nrm0, prev = nrm0;
+      const double stop = std::max(cfg_.nl_atol, cfg_.nl_rtol * nrm0);
       int it = 0;
       for (;;) {
         S.residual_norm = nrm;
@@ -606,14 +749,21 @@ int Model::step(double t, double dt, StepStats* st) {
           S.status = 3;
           return 3;
         }
-        const bool conv = nrm <= cfg_.nl_atol || nrm <= cfg_.nl_rtol * nrm0 || nrm == 0.0;
+        const bool conv = nrm <= stop || nrm == 0.0;
         if (conv && it >= cfg_.nl_min) break;
         if (it >= cfg_.nl_max) {
           S.status = 1;
           return 1;
         }
-        jacobian();
-        int rc = solve();
+        const bool relinearize = it == 0 || cfg_.linearization_threshold <= 0.0 ||
+                                 nrm / prev > cfg_.linearization_threshold;
+        if (relinearize) jacobian();
+        double rtol = cfg_.lin_rtol;
+        if (!cfg_.fixed_linear_tolerance) {
+          const double ratio = it == 0 ? 1.0 : (nrm * nrm) / (prev * prev);
+          rtol = std::max(stop / (10.0 * nrm), std::min(cfg_.min_linear_reduction, ratio));
+        }
+        int rc = solve(rtol);
         if (rc) {
           S.status = rc == 3 ? 3 : rc == 4 ? 4 : 2;
           return S.status;
@@ -621,6 +771,7 @@ int Model::step(double t, double dt, StepStats* st) {
         k::axpy(n, -1.0, z_.get(), x, stream_);
         ++it;
         residual();
+        prev = nrm;
         nrm = residual_norm(r_.get());
       }
       S.newton_iterations += it;
@@ -767,14 +918,8 @@ void Model::assemble(const double* x_packed, double t, double cm, double ca, dou
   const long n = L_.nslot * L_.B;
   DCG_HIP(hipMemcpyAsync(packed_.get(), x_packed, L_.ndof * sizeof(double), hipMemcpyHostToDevice, stream_));
   k::unpack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), packed_.get(), xs_.get(), stream_);
-  k::fill(r_.get(), 0.0, n, stream_);
-  int mode = 1;
-  if (jac) {
-    k::init_jacobian(val_.get(), matrix_values(), diag_pos_.get(), slot_b_.get(), static_cast<int>(L_.nslot), L_.B,
-                     stream_);
-    mode = 3;
-  }
-  launch_assembly(xs_.get(), t, cm, ca, r_.get(), jac ? val_.get() : nullptr, mode);
+  (void)n;
+  launch_assembly(xs_.get(), t, cm, ca, nullptr, r_.get(), jac ? val_.get() : nullptr, jac ? 3 : 1);
   k::pack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), r_.get(), packed_.get(), stream_);
   DCG_HIP(hipMemcpyAsync(r_packed, packed_.get(), L_.ndof * sizeof(double), hipMemcpyDeviceToHost, stream_));
   if (jac) DCG_HIP(hipMemcpyAsync(jac, val_.get(), val_.bytes(), hipMemcpyDeviceToHost, stream_));
@@ -786,7 +931,7 @@ void Model::spmv(const double* jac, const double* x_packed, double* y_packed) {
   DCG_HIP(hipMemcpyAsync(val_.get(), jac, val_.bytes(), hipMemcpyHostToDevice, stream_));
   DCG_HIP(hipMemcpyAsync(packed_.get(), x_packed, L_.ndof * sizeof(double), hipMemcpyHostToDevice, stream_));
   k::unpack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), packed_.get(), xs_.get(), stream_);
-  k::spmv(L_.B, plan_, val_.get(), xs_.get(), r_.get(), nullptr, nullptr, nullptr, nullptr, stream_);
+  k::spmv(L_.B, plan_, val_.get(), xs_.get(), r_.get(), nullptr, nullptr, nullptr, nullptr, 0, stream_);
   k::pack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), r_.get(), packed_.get(), stream_);
   DCG_HIP(hipMemcpyAsync(y_packed, packed_.get(), L_.ndof * sizeof(double), hipMemcpyDeviceToHost, stream_));
   synchronize();

@@ -49,6 +49,9 @@ struct SolverConfig {
   double nl_rtol = 1e-4, nl_atol = 0.0;
   int nl_min = 0, nl_max = 40;
   int norm = 0;  // 0: two_norm2, 1: l_inf, 2: l_1
+  bool fixed_linear_tolerance = false;  // nonlinear_solver.dx_inverse_fixed_tolerance
+  double min_linear_reduction = 0.1;    // nonlinear_solver.dx_inverse_min_relative_tolerance
+  double linearization_threshold = 0.0; // re-linearize when defect_k/defect_{k-1} exceeds it
   double ir_inner_rtol = 1e-3;  // FP32 Krylov tolerance inside refinement
   int ir_max = 50;
   std::vector<std::string> warnings;
@@ -91,6 +94,10 @@ class Model {
   // launches, summed milliseconds and summed algorithmic bytes of one kind
   void profile(int kind, long* launches, double* ms, double* bytes);
 
+  // SpMV autotuning result: (variant, microseconds) per candidate; chosen variant
+  const std::vector<std::pair<int, double>>& spmv_tuning() const { return tuning_; }
+  int spmv_variant() const { return plan_.variant; }
+
   // device-side pieces used by batching/benchmarks
   hipStream_t stream() const { return stream_; }
   void synchronize() const;
@@ -100,8 +107,10 @@ class Model {
   void build(const std::string& ini, const std::string& overrides);
   void setup_device();
   void require_device() const;
-  // assembly: r (+)= cm*R^M(x) + ca*R^A(x) and/or its Jacobian into val_
-  void launch_assembly(const double* x, double t, double cm, double ca, double* r, double* val, int mode);
+  // assembly: r = base + cm*R^M(x) + ca*R^A(x) (base may be null or r itself)
+  // and/or val = d/dx of it, complete (padding and interface blocks included)
+  void launch_assembly(const double* x, double t, double cm, double ca, const double* base, double* r, double* val,
+                       int mode);
   double residual_norm(const double* r);
   // linear solves on the current val_
   int krylov(const double* b, double* x, double rtol, int maxit, int* its, int* refinements);
@@ -109,6 +118,8 @@ class Model {
   int bicgstab32(const double* b, float* x, double rtol, int maxit, int* its);
   void precond_setup();
   void compile_and_load();
+  void tune_spmv();
+  int tune_spmv_plan(bool fp32);
   void prof_begin(int kind);
   void prof_end(int kind, double bytes);
   void prof_collect();
@@ -125,6 +136,7 @@ class Model {
   hipStream_t stream_ = nullptr;
   RtcModule module_;
   std::vector<hipFunction_t> vol_fn_;
+  std::vector<hipFunction_t> tile_fn_;
   std::vector<hipFunction_t> facet_fn_;
   double time_ = 0.0, dt_initial_ = 0.0, t_end_ = 0.0;
   int last_krylov_its_ = 8;
@@ -136,7 +148,13 @@ class Model {
   struct CompDev {
     DeviceArray<int> cells, cell_slots;
     DeviceArray<unsigned char> cell_mask;
+    // tiled kernel: vertex -> slot, cell -> class mask over the whole grid, tile list
+    DeviceArray<int> g2l, tiles;
+    DeviceArray<unsigned char> cell_mask_grid;
+    int ntiles = 0;
   };
+  bool all_tiled_ = false;
+  DeviceArray<int> extra_pos_;  // BCSR blocks coupling two compartments (zeroed before facets)
   std::vector<CompDev> comp_dev_;
   struct FacetDev {
     DeviceArray<int> fint;
@@ -154,11 +172,12 @@ class Model {
   DeviceArray<double> part_, scalar_out_;
   DeviceArray<k::BicgScalars> sc_;
   DeviceArray<int> status_;
-  k::SpmvPlan plan_;
+  k::SpmvPlan plan_, plan32_;
   // ILU level schedules (host copies of the level pointers, device row lists)
   std::vector<int> lo_ptr_, up_ptr_;
   DeviceArray<int> lo_rows_, up_rows_;
   bool ilu_ready_ = false;
+  std::vector<std::pair<int, double>> tuning_;
   // profiling
   bool prof_ = false;
   struct Prof {

@@ -9,10 +9,19 @@ constexpr int kSpmvStage = 4096;
 
 // Chunking of BCSR rows so each chunk's blocks x B fit the LDS stage.
 struct SpmvPlan {
+  // variant 0: LDS-staged chunks, one thread per block row entry; 5: LDS-staged
+  // chunks, one thread per whole block; 1..4: row-split with 8/16/32/64 lanes
+  // per row (only when the block size divides the group); chosen by the autotuner
+  int variant = 0;
   int nchunk = 0;
   const int* chunk_row = nullptr;  // nchunk + 1 row boundaries (device)
   const int* rp = nullptr;         // BCSR row pointer (device)
   const int* col = nullptr;        // BCSR columns (device)
+  int nrows = 0;                   // row-split: rows, workgroups (multiple of 8), rows per workgroup
+  int row_wgs = 0;
+  int rows_per_wg = 0;
+  // number of partial-sum pairs a launch writes
+  int parts() const { return (variant == 0 || variant == 5) ? nchunk : row_wgs; }
 };
 
 // Device-resident BiCGSTAB state shared by its kernels.
@@ -25,16 +34,21 @@ void fill(double* p, double v, long n, hipStream_t s);
 void fill(float* p, float v, long n, hipStream_t s);
 void fill(int* p, int v, long n, hipStream_t s);
 void init_jacobian(double* val, long nval, const int* diag_pos, const int* slot_b, int nslot, int B, hipStream_t s);
+// zero the B x B blocks at the given block positions
+void zero_blocks(double* val, const int* pos, long n, int B, hipStream_t s);
 void axpy(long n, double a, const double* x, double* y, hipStream_t s);
 void axpby(long n, double a, const double* x, double b, double* y, hipStream_t s);
 int reduce_parts(long n);
 // op 0: sum x*y (y null: x*x), 1: sum |x|, 2: max |x|; `part` holds reduce_parts(n) doubles
 void reduce(long n, const double* x, const double* y, int op, double* part, double* out, hipStream_t s);
 
+// y = A x; with `part` the per-chunk partial dots y.w1, y.w2 are written
+// (part[2c], part[2c+1]); with `sc` the launch is skipped once the solve is
+// done.  `stage` is unused by the kernel.
 void spmv(int B, const SpmvPlan& pl, const double* val, const double* x, double* y, const double* w1,
-          const double* w2, double* part, const int* skip, hipStream_t s);
+          const double* w2, double* part, BicgScalars* sc, int stage, hipStream_t s);
 void spmv(int B, const SpmvPlan& pl, const float* val, const float* x, float* y, const float* w1, const float* w2,
-          double* part, const int* skip, hipStream_t s);
+          double* part, BicgScalars* sc, int stage, hipStream_t s);
 
 void block_inverse(int B, int nslot, const double* val, const int* diag_pos, double* dinv, int* status, hipStream_t s);
 
@@ -50,7 +64,6 @@ void bicg_s(int B, int nslot, const V* r, const V* v, V* sv, V* sh, const double
 template <typename V>
 void bicg_xr(int B, int nslot, V* x, const V* ph, const V* sh, const V* sv, const V* t, V* r, const V* rh,
              const BicgScalars* sc, double* part, hipStream_t s);
-void bicg_fin(int nparts, const double* part, int stride, int stage, BicgScalars* sc, hipStream_t s);
 
 void ilu_factor(int B, int nrows, const int* rows, const int* rp, const int* col, const int* diag_pos, double* lu,
                 double* dinv, int* status, hipStream_t s);
@@ -68,6 +81,8 @@ void pack_state(long nslot, int B, const int* slot_b, const long* slot_dof, cons
 // out[i] = sum over slots [first, first+n) of w[s] * x[s*B+i], i < B (<= 8); part: 8*reduce_parts(n)
 void weighted_species_sum(long first, long n, int B, const double* w, const double* x, double* part, double* out,
                           hipStream_t s);
+// reduce per-workgroup partials (stride 1 or 2) and update the scalars of `stage`
+void bicg_fin(int nparts, const double* part, int stride, int stage, BicgScalars* sc, hipStream_t s);
 void to_float(long n, const double* a, float* b, hipStream_t s);
 void to_double(long n, const float* a, double* b, hipStream_t s);
 

@@ -71,6 +71,11 @@ __global__ void pad_identity_k(double* val, const int* diag_pos, const int* slot
   if (c >= slot_b[s]) val[(size_t)diag_pos[s] * B * B + c * B + c] = 1.0;
 }
 
+__global__ void zero_blocks_k(double* __restrict__ val, const int* __restrict__ pos, long n, int bb) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n * bb) val[(size_t)pos[i / bb] * bb + i % bb] = 0.0;
+}
+
 // y = a*x + y
 __global__ void axpy_k(long n, double a, const double* __restrict__ x, double* __restrict__ y) {
   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -129,6 +134,93 @@ __global__ void reduce_final_k(int nparts, const double* __restrict__ part, int 
   if (threadIdx.x == 0) *out = v[0];
 }
 
+// ----------------------------------------------------------------- BiCGSTAB scalars
+// Updates the device-resident scalars after one reduction stage; v0/v1 are the
+// two reduced sums of that stage.
+__device__ void bicg_update(BicgScalars* sc, int stage, double v0, double v1) {
+  switch (stage) {
+    case 0: {  // initial: v0 = r.r ; rho = r.r since rh = r
+      sc->r0 = sqrt(v0);
+      sc->rnorm = sc->r0;
+      sc->rho = v0;
+      sc->alpha = sc->omega = 1.0;
+      sc->beta = 0.0;
+      sc->iter = 0;
+      sc->half = 0;
+      if (!(sc->r0 > 0.0)) sc->done = isfinite(sc->r0) ? 1 : 3;
+      if (sc->r0 <= sc->abstol) sc->done = 1;
+      break;
+    }
+    case 1: {  // after v = A ph: alpha = rho / (rh.v)
+      if (v0 == 0.0 || !isfinite(v0)) {
+        sc->done = isfinite(v0) ? 4 : 3;
+        break;
+      }
+      sc->alpha = sc->rho / v0;
+      break;
+    }
+    case 2: {  // after s: half-step convergence test
+      const double ns = sqrt(v0);
+      sc->iter += 1;
+      if (!isfinite(ns)) {
+        sc->done = 3;
+        break;
+      }
+      if (ns <= sc->tol * sc->r0 || ns <= sc->abstol) {
+        sc->half = 1;
+        sc->rnorm = ns;
+      }
+      break;
+    }
+    case 3: {  // after t = A sh: omega = (t.s)/(t.t)
+      if (sc->half) break;
+      sc->omega = v1 == 0.0 ? 0.0 : v0 / v1;
+      break;
+    }
+    case 4: {  // after x, r update: convergence, beta for the next step
+      if (sc->half) {
+        sc->done = 1;
+        break;
+      }
+      const double nr = sqrt(v1);
+      sc->rnorm = nr;
+      if (!isfinite(nr)) {
+        sc->done = 3;
+        break;
+      }
+      if (nr <= sc->tol * sc->r0 || nr <= sc->abstol) {
+        sc->done = 1;
+        break;
+      }
+      if (sc->omega == 0.0 || v0 == 0.0) {
+        sc->done = 4;
+        break;
+      }
+      sc->beta = (v0 / sc->rho) * (sc->alpha / sc->omega);
+      sc->rho = v0;
+      if (sc->iter >= sc->maxit) sc->done = 2;
+      break;
+    }
+  }
+}
+
+// Finaliser: one workgroup of 1024 threads sums the partial sums in a fixed
+// order and updates the scalars.  (Fusing this into the producers with a
+// last-workgroup ticket needs an agent-scope release per workgroup, which on
+// a multi-XCD part writes back L2 and costs far more than a launch.)
+__global__ void __launch_bounds__(1024) bicg_fin_k(int nparts, const double* __restrict__ part, int stride, int stage,
+                                                   BicgScalars* __restrict__ sc) {
+  if (sc->done) return;
+  __shared__ double red[32];
+  double v[2] = {0.0, 0.0};
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+    v[0] += part[(size_t)i * stride];
+    if (stride == 2) v[1] += part[(size_t)i * stride + 1];
+  }
+  block_sum(v, red);
+  if (threadIdx.x == 0) bicg_update(sc, stage, v[0], v[1]);
+}
+
 // ----------------------------------------------------------------- SpMV
 // y = A x over BCSR, one workgroup per chunk of rows whose block-row entries
 // (blocks x B) fit the LDS stage.  Phase 1 streams the chunk's blocks with
@@ -141,9 +233,9 @@ __global__ void __launch_bounds__(kThreads) spmv_k(int nchunk, const int* __rest
                                                    const T* __restrict__ val, const V* __restrict__ x,
                                                    V* __restrict__ y, const V* __restrict__ w1,
                                                    const V* __restrict__ w2, double* __restrict__ part,
-                                                   const int* __restrict__ skip) {
-  if (skip && *skip) return;
-  __shared__ double stage[kSpmvStage];
+                                                   BicgScalars* __restrict__ sc, int stage) {
+  if (sc && sc->done) return;
+  __shared__ double staged[kSpmvStage];
   __shared__ double red[32];
   const int c = xcd_chunk(blockIdx.x, gridDim.x);
   double acc[2] = {0.0, 0.0};
@@ -151,21 +243,174 @@ __global__ void __launch_bounds__(kThreads) spmv_k(int nchunk, const int* __rest
     const int r0 = chunk_row[c], r1 = chunk_row[c + 1];
     const int k0 = rp[r0], k1 = rp[r1];
     const int ne = (k1 - k0) * B;
-    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-      const int kb = k0 + e / B, i = e % B;
-      const T* a = val + (size_t)kb * (B * B) + i * B;
-      const V* xv = x + (size_t)col[kb] * B;
-      double s = 0.0;
+    constexpr int U = 4;  // independent entries per thread in flight
+    for (int e0 = threadIdx.x; e0 < ne; e0 += U * kThreads) {
+      T av[U][B];
+      int cv[U];
 #pragma unroll
-      for (int j = 0; j < B; ++j) s += (double)a[j] * (double)xv[j];
-      stage[e] = s;
+      for (int u = 0; u < U; ++u) {
+        const int e = e0 + u * kThreads;
+        const int kb = k0 + (e < ne ? e : 0) / B, i = (e < ne ? e : 0) % B;
+        const T* a = val + (size_t)kb * (B * B) + i * B;
+#pragma unroll
+        for (int j = 0; j < B; ++j) av[u][j] = __builtin_nontemporal_load(a + j);
+        cv[u] = col[kb];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e0 + u * kThreads;
+        if (e >= ne) break;
+        const V* xv = x + (size_t)cv[u] * B;
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < B; ++j) s += (double)av[u][j] * (double)xv[j];
+        staged[e] = s;
+      }
     }
     __syncthreads();
     const int nout = (r1 - r0) * B;
     for (int o = threadIdx.x; o < nout; o += blockDim.x) {
       const int row = r0 + o / B, i = o % B;
       double s = 0.0;
-      for (int kb = rp[row]; kb < rp[row + 1]; ++kb) s += stage[(kb - k0) * B + i];
+      for (int kb = rp[row]; kb < rp[row + 1]; ++kb) s += staged[(kb - k0) * B + i];
+      const size_t yi = (size_t)row * B + i;
+      y[yi] = (V)s;
+      const double ys = (double)(V)s;
+      if (w1) acc[0] += ys * (double)w1[yi];
+      if (w2) acc[1] += ys * (double)w2[yi];
+    }
+  }
+  if (part) {
+    block_sum(acc, red);
+    if (threadIdx.x == 0 && c < nchunk) {
+      part[2 * c] = acc[0];
+      part[2 * c + 1] = acc[1];
+    }
+  }
+}
+
+// Row-split variant: G consecutive lanes share one BCSR row; lane l of the
+// group takes the row's entries e = l, l + G, ... (entry = block x
+// block-row), so with B | G every lane accumulates one fixed species and a
+// shuffle-xor tree over the lanes of equal species finishes the row.  No LDS,
+// no barrier; each workgroup owns a contiguous, XCD-local range of rows.
+template <int B, int G, typename T, typename V>
+__global__ void __launch_bounds__(kThreads) spmv_rows_k(int nrows, int rows_per_wg, const int* __restrict__ rp,
+                                                        const int* __restrict__ col, const T* __restrict__ val,
+                                                        const V* __restrict__ x, V* __restrict__ y,
+                                                        const V* __restrict__ w1, const V* __restrict__ w2,
+                                                        double* __restrict__ part, const BicgScalars* __restrict__ sc) {
+  if (sc && sc->done) return;
+  static_assert(G % B == 0 && G <= kWave, "group size");
+  __shared__ double red[32];
+  const int c = xcd_chunk(blockIdx.x, gridDim.x);
+  const int r_begin = c * rows_per_wg;
+  const int r_end = min(nrows, r_begin + rows_per_wg);
+  const int lg = threadIdx.x % G, gid = threadIdx.x / G;
+  constexpr int NG = kThreads / G;
+  const int i = lg % B;
+  double acc2[2] = {0.0, 0.0};
+  for (int r = r_begin + gid; r < r_end; r += NG) {
+    const int k0 = rp[r], ne = (rp[r + 1] - k0) * B;
+    double acc = 0.0;
+    int e = lg;
+    for (; e + G < ne; e += 2 * G) {
+      const int kb0 = k0 + e / B, kb1 = k0 + (e + G) / B;
+      const T* a0 = val + (size_t)kb0 * (B * B) + i * B;
+      const T* a1 = val + (size_t)kb1 * (B * B) + i * B;
+      T v0[B], v1[B];
+#pragma unroll
+      for (int j = 0; j < B; ++j) {
+        v0[j] = __builtin_nontemporal_load(a0 + j);
+        v1[j] = __builtin_nontemporal_load(a1 + j);
+      }
+      const V* x0 = x + (size_t)col[kb0] * B;
+      const V* x1 = x + (size_t)col[kb1] * B;
+#pragma unroll
+      for (int j = 0; j < B; ++j) acc += (double)v0[j] * (double)x0[j] + (double)v1[j] * (double)x1[j];
+    }
+    if (e < ne) {
+      const int kb = k0 + e / B;
+      const T* a = val + (size_t)kb * (B * B) + i * B;
+      const V* xv = x + (size_t)col[kb] * B;
+#pragma unroll
+      for (int j = 0; j < B; ++j) acc += (double)__builtin_nontemporal_load(a + j) * (double)xv[j];
+    }
+#pragma unroll
+    for (int off = B; off < G; off <<= 1) acc += __shfl_xor(acc, off, G);
+    if (lg < B) {
+      const size_t yi = (size_t)r * B + lg;
+      y[yi] = (V)acc;
+      const double ys = (double)(V)acc;
+      if (w1) acc2[0] += ys * (double)w1[yi];
+      if (w2) acc2[1] += ys * (double)w2[yi];
+    }
+  }
+  if (part) {
+    block_sum(acc2, red);
+    if (threadIdx.x == 0) {
+      part[2 * c] = acc2[0];
+      part[2 * c + 1] = acc2[1];
+    }
+  }
+}
+
+// Block variant of the LDS-staged SpMV: one thread per whole B x B block
+// (B*B contiguous values, one column index, one B-vector of x), writing B
+// products to the stage; halves the per-entry index and gather work of the
+// entry-per-thread form for B = 2 and makes FP32 loads 16 bytes wide.
+template <int B, typename T, typename V>
+__global__ void __launch_bounds__(kThreads) spmv_blk_k(int nchunk, const int* __restrict__ chunk_row,
+                                                       const int* __restrict__ rp, const int* __restrict__ col,
+                                                       const T* __restrict__ val, const V* __restrict__ x,
+                                                       V* __restrict__ y, const V* __restrict__ w1,
+                                                       const V* __restrict__ w2, double* __restrict__ part,
+                                                       const BicgScalars* __restrict__ sc) {
+  if (sc && sc->done) return;
+  __shared__ double staged[kSpmvStage];
+  __shared__ double red[32];
+  const int c = xcd_chunk(blockIdx.x, gridDim.x);
+  double acc[2] = {0.0, 0.0};
+  if (c < nchunk) {
+    const int r0 = chunk_row[c], r1 = chunk_row[c + 1];
+    const int k0 = rp[r0], k1 = rp[r1];
+    const int nb = k1 - k0;
+    constexpr int U = 2;
+    for (int b0 = threadIdx.x; b0 < nb; b0 += U * kThreads) {
+      T av[U][B * B];
+      int cv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int b = b0 + u * kThreads;
+        const int kb = k0 + (b < nb ? b : 0);
+        const T* a = val + (size_t)kb * (B * B);
+#pragma unroll
+        for (int j = 0; j < B * B; ++j) av[u][j] = __builtin_nontemporal_load(a + j);
+        cv[u] = col[kb];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int b = b0 + u * kThreads;
+        if (b >= nb) break;
+        const V* xv = x + (size_t)cv[u] * B;
+        double xl[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) xl[j] = (double)xv[j];
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+          double s = 0.0;
+#pragma unroll
+          for (int j = 0; j < B; ++j) s += (double)av[u][i * B + j] * xl[j];
+          staged[b * B + i] = s;
+        }
+      }
+    }
+    __syncthreads();
+    const int nout = (r1 - r0) * B;
+    for (int o = threadIdx.x; o < nout; o += blockDim.x) {
+      const int row = r0 + o / B, i = o % B;
+      double s = 0.0;
+      for (int kb = rp[row]; kb < rp[row + 1]; ++kb) s += staged[(kb - k0) * B + i];
       const size_t yi = (size_t)row * B + i;
       y[yi] = (V)s;
       const double ys = (double)(V)s;
@@ -348,89 +593,12 @@ __global__ void __launch_bounds__(kThreads) bicg_xr_k(int nslot, V* __restrict__
   }
 }
 
-// single-workgroup finalisers: reduce partials in fixed order, update scalars
-__global__ void bicg_fin_k(int nparts, const double* __restrict__ part, int stride, int stage,
-                           BicgScalars* __restrict__ sc) {
-  if (sc->done) return;
-  __shared__ double red[32];
-  double v[2] = {0.0, 0.0};
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
-    v[0] += part[i * stride];
-    if (stride == 2) v[1] += part[i * stride + 1];
-  }
-  block_sum(v, red);
-  if (threadIdx.x != 0) return;
-  switch (stage) {
-    case 0: {  // initial: v0 = r.r ; rho = r.r since rh = r
-      sc->r0 = sqrt(v[0]);
-      sc->rnorm = sc->r0;
-      sc->rho = v[0];
-      sc->alpha = sc->omega = 1.0;
-      sc->beta = 0.0;
-      sc->iter = 0;
-      sc->half = 0;
-      if (!(sc->r0 > 0.0)) sc->done = isfinite(sc->r0) ? 1 : 3;
-      if (sc->r0 <= sc->abstol) sc->done = 1;
-      break;
-    }
-    case 1: {  // after v = A ph: alpha = rho / (rh.v)
-      if (v[0] == 0.0 || !isfinite(v[0])) {
-        sc->done = isfinite(v[0]) ? 4 : 3;
-        break;
-      }
-      sc->alpha = sc->rho / v[0];
-      break;
-    }
-    case 2: {  // after s: half-step convergence test
-      const double ns = sqrt(v[0]);
-      sc->iter += 1;
-      if (!isfinite(ns)) {
-        sc->done = 3;
-        break;
-      }
-      if (ns <= sc->tol * sc->r0 || ns <= sc->abstol) {
-        sc->half = 1;
-        sc->rnorm = ns;
-      }
-      break;
-    }
-    case 3: {  // after t = A sh: omega = (t.s)/(t.t)
-      if (sc->half) break;
-      sc->omega = v[1] == 0.0 ? 0.0 : v[0] / v[1];
-      break;
-    }
-    case 4: {  // after x, r update: convergence, beta for the next step
-      if (sc->half) {
-        sc->done = 1;
-        break;
-      }
-      const double nr = sqrt(v[1]);
-      sc->rnorm = nr;
-      if (!isfinite(nr)) {
-        sc->done = 3;
-        break;
-      }
-      if (nr <= sc->tol * sc->r0 || nr <= sc->abstol) {
-        sc->done = 1;
-        break;
-      }
-      if (sc->omega == 0.0 || v[0] == 0.0) {
-        sc->done = 4;
-        break;
-      }
-      sc->beta = (v[0] / sc->rho) * (sc->alpha / sc->omega);
-      sc->rho = v[0];
-      if (sc->iter >= sc->maxit) sc->done = 2;
-      break;
-    }
-  }
-}
-
 // x = 0, r = rh = b, p = v = 0, partial b.b
 template <typename V, typename W>
 __global__ void __launch_bounds__(kThreads) bicg_init_k(long n, const W* __restrict__ b, V* __restrict__ x,
                                                         V* __restrict__ r, V* __restrict__ rh, V* __restrict__ p,
-                                                        V* __restrict__ v, double* __restrict__ part) {
+                                                        V* __restrict__ v, double* __restrict__ part,
+                                                        BicgScalars* __restrict__ sc) {
   __shared__ double red[16];
   double acc[1] = {0.0};
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
@@ -685,6 +853,10 @@ void init_jacobian(double* val, long nval, const int* diag_pos, const int* slot_
   pad_identity_k<<<grid_for((long)nslot * B), kThreads, 0, s>>>(val, diag_pos, slot_b, nslot, B);
 }
 
+void zero_blocks(double* val, const int* pos, long n, int B, hipStream_t s) {
+  if (n > 0) zero_blocks_k<<<grid_for(n * B * B), kThreads, 0, s>>>(val, pos, n, B * B);
+}
+
 void axpy(long n, double a, const double* x, double* y, hipStream_t s) {
   if (n > 0) axpy_k<<<grid_for(n), kThreads, 0, s>>>(n, a, x, y);
 }
@@ -703,12 +875,36 @@ void reduce(long n, const double* x, const double* y, int op, double* part, doub
   reduce_final_k<<<1, kThreads, 0, s>>>(g, part, op, out);
 }
 
+template <int B, int G, typename T, typename V>
+static void spmv_rows_t(const SpmvPlan& pl, const T* val, const V* x, V* y, const V* w1, const V* w2, double* part,
+                        BicgScalars* sc, hipStream_t s) {
+  if constexpr (G % B == 0) {
+    const unsigned grid = (unsigned)pl.row_wgs;  // a multiple of 8
+    spmv_rows_k<B, G, T, V><<<grid, kThreads, 0, s>>>(pl.nrows, pl.rows_per_wg, pl.rp, pl.col, val, x, y, w1, w2, part,
+                                                      sc);
+  }
+}
+
 template <int B, typename T, typename V>
 static void spmv_t(const SpmvPlan& pl, const T* val, const V* x, V* y, const V* w1, const V* w2, double* part,
-                   const int* skip, hipStream_t s) {
+                   BicgScalars* sc, int stage, hipStream_t s) {
+  (void)stage;
+  switch (pl.variant) {
+    case 5: {
+      const unsigned grid = (unsigned)(((pl.nchunk + 7) / 8) * 8);
+      spmv_blk_k<B, T, V><<<grid, kThreads, 0, s>>>(pl.nchunk, pl.chunk_row, pl.rp, pl.col, val, x, y, w1, w2, part,
+                                                     sc);
+      return;
+    }
+    case 1: spmv_rows_t<B, 8, T, V>(pl, val, x, y, w1, w2, part, sc, s); return;
+    case 2: spmv_rows_t<B, 16, T, V>(pl, val, x, y, w1, w2, part, sc, s); return;
+    case 3: spmv_rows_t<B, 32, T, V>(pl, val, x, y, w1, w2, part, sc, s); return;
+    case 4: spmv_rows_t<B, 64, T, V>(pl, val, x, y, w1, w2, part, sc, s); return;
+    default: break;
+  }
   // grid rounded up to a multiple of 8 so the XCD remap covers every chunk
   const unsigned grid = (unsigned)(((pl.nchunk + 7) / 8) * 8);
-  spmv_k<B, T, V><<<grid, kThreads, 0, s>>>(pl.nchunk, pl.chunk_row, pl.rp, pl.col, val, x, y, w1, w2, part, skip);
+  spmv_k<B, T, V><<<grid, kThreads, 0, s>>>(pl.nchunk, pl.chunk_row, pl.rp, pl.col, val, x, y, w1, w2, part, sc, 0);
 }
 
 #define DCG_DISPATCH_B(B, CALL)                     \
@@ -725,12 +921,12 @@ static void spmv_t(const SpmvPlan& pl, const T* val, const V* x, V* y, const V* 
   }
 
 void spmv(int B, const SpmvPlan& pl, const double* val, const double* x, double* y, const double* w1,
-          const double* w2, double* part, const int* skip, hipStream_t s) {
-  DCG_DISPATCH_B(B, (spmv_t<BB, double, double>(pl, val, x, y, w1, w2, part, skip, s)));
+          const double* w2, double* part, BicgScalars* sc, int stage, hipStream_t s) {
+  DCG_DISPATCH_B(B, (spmv_t<BB, double, double>(pl, val, x, y, w1, w2, part, sc, stage, s)));
 }
 void spmv(int B, const SpmvPlan& pl, const float* val, const float* x, float* y, const float* w1, const float* w2,
-          double* part, const int* skip, hipStream_t s) {
-  DCG_DISPATCH_B(B, (spmv_t<BB, float, float>(pl, val, x, y, w1, w2, part, skip, s)));
+          double* part, BicgScalars* sc, int stage, hipStream_t s) {
+  DCG_DISPATCH_B(B, (spmv_t<BB, float, float>(pl, val, x, y, w1, w2, part, sc, stage, s)));
 }
 
 void block_inverse(int B, int nslot, const double* val, const int* diag_pos, double* dinv, int* status, hipStream_t s) {
@@ -740,8 +936,8 @@ void block_inverse(int B, int nslot, const double* val, const int* diag_pos, dou
 template <typename V>
 void bicg_init(long n, const double* b, V* x, V* r, V* rh, V* p, V* v, double* part, int nparts, BicgScalars* sc,
                hipStream_t s) {
-  bicg_init_k<V, double><<<nparts, kThreads, 0, s>>>(n, b, x, r, rh, p, v, part);
-  bicg_fin_k<<<1, kThreads, 0, s>>>(nparts, part, 1, 0, sc);
+  bicg_init_k<V, double><<<nparts, kThreads, 0, s>>>(n, b, x, r, rh, p, v, part, sc);
+  bicg_fin_k<<<1, 1024, 0, s>>>(nparts, part, 1, 0, sc);
 }
 template void bicg_init<double>(long, const double*, double*, double*, double*, double*, double*, double*, int,
                                 BicgScalars*, hipStream_t);
@@ -779,7 +975,7 @@ template void bicg_xr<float>(int, int, float*, const float*, const float*, const
                              const float*, const BicgScalars*, double*, hipStream_t);
 
 void bicg_fin(int nparts, const double* part, int stride, int stage, BicgScalars* sc, hipStream_t s) {
-  bicg_fin_k<<<1, kThreads, 0, s>>>(nparts, part, stride, stage, sc);
+  bicg_fin_k<<<1, 1024, 0, s>>>(nparts, part, stride, stage, sc);
 }
 
 void ilu_factor(int B, int nrows, const int* rows, const int* rp, const int* col, const int* diag_pos, double* lu,

@@ -126,6 +126,13 @@ int dcg_linear_solve(dcg_model_t model, const double* jacobian, const double* b,
 int dcg_profile_enable(dcg_model_t model, int on);
 int dcg_profile_read(dcg_model_t model, int kind, long* launches, double* milliseconds, double* bytes);
 
+/* SpMV variants the set-up autotuner timed on this model's pattern: for each
+ * candidate k < *count, variant[k] (0 LDS-staged entries, 5 LDS-staged whole
+ * blocks, 1..4 row-split with 8/16/32/64 lanes per row; +100 for the FP32
+ * matrix of the mixed-precision solve) and microseconds[k]; *chosen is the
+ * FP64 variant used.  Arrays hold at least 16 entries. */
+int dcg_spmv_tuning(dcg_model_t model, int* count, int* variant, double* microseconds, int* chosen);
+
 #ifdef __cplusplus
 }
 #endif

@@ -372,6 +372,8 @@ struct Model {
   double nl_rtol = 1e-4, nl_atol = 0.0;
   int nl_min = 0, nl_max = 40;
   std::string nl_norm = "l_2";
+  bool fixed_lin_tol = false;
+  double min_lin_red = 0.1, lin_threshold = 0.0;
   // last statistics
   int last_newton = 0, last_linear = 0;
   double last_assembly_s = 0, last_solve_s = 0;
@@ -1450,39 +1452,56 @@ static int step(Model& M, double dt, StepStats& st) {
       assemble(M, x.data(), ts, cm, ca, r.data(), jac ? vals.data() : nullptr, true);
       tasm += std::chrono::duration<double>(std::chrono::steady_clock::now() - a0).count();
     };
-    auto solve = [&]() -> int {
+    auto solve = [&](double rtol) -> int {
       auto a0 = std::chrono::steady_clock::now();
       Precond P;
       P.setup(A, M.precond);  // rebuilt every call, make_step_operator.hh:120-127
       int its;
       double red;
-      int rc = bicgstab(A, P, r.data(), z.data(), M.lin_rtol, M.lin_maxit, &its, &red);
+      int rc = bicgstab(A, P, r.data(), z.data(), rtol, M.lin_maxit, &its, &red);
       st.linear += its;
       tsol += std::chrono::duration<double>(std::chrono::steady_clock::now() - a0).count();
       return rc;
     };
     if (M.is_linear) {
       F(true);
-      int rc = solve();
+      int rc = solve(M.lin_rtol);
       if (rc) return rc == 2 ? 2 : rc;
       for (long i = 0; i < n; ++i) x[i] -= z[i];
       st.newton += 1;
     } else {
+      // classic PDELab Newton control: stop below max(abs, rel * d0); linear
+      // reduction max(stop/(10 d_k), min(min_red, d_k^2/d_{k-1}^2)) unless
+      // fixed; re-linearize unless the defect rate stays below the threshold
       F(false);
-      double nrm0 = nl_norm(M, r), nrm = nrm0;
+      double nrm0 = nl_norm(M, r), nrm = nrm0, prev = nrm0;
+      const double stop = std::max(M.nl_atol, M.nl_rtol * nrm0);
       int it = 0;
+      bool have_jac = false;
       for (;;) {
         if (!std::isfinite(nrm)) return 3;
-        bool conv = (nrm <= M.nl_atol) || (nrm <= M.nl_rtol * nrm0) || nrm == 0.0;
+        bool conv = nrm <= stop || nrm == 0.0;
         if (conv && it >= M.nl_min) break;
         if (it >= M.nl_max) return 1;
-        F(true);
-        int rc = solve();
+        bool relin = !have_jac || M.lin_threshold <= 0.0 || nrm / prev > M.lin_threshold;
+        if (relin) {
+          std::vector<double> keep = r;
+          F(true);  // assembles the residual again with the Jacobian
+          have_jac = true;
+          (void)keep;
+        }
+        double rtol = M.lin_rtol;
+        if (!M.fixed_lin_tol) {
+          double ratio = it == 0 ? 1.0 : (nrm * nrm) / (prev * prev);
+          rtol = std::max(stop / (10.0 * nrm), std::min(M.min_lin_red, ratio));
+        }
+        int rc = solve(rtol);
         if (rc == 3 || rc == 4) return rc;
         if (rc == 2) return 2;
         for (long i = 0; i < n; ++i) x[i] -= z[i];
         ++it;
         F(false);
+        prev = nrm;
         nrm = nl_norm(M, r);
       }
       st.newton += it;
@@ -1556,6 +1575,10 @@ static void configure(Model& M, const std::string& ini, const std::string& over)
     M.nl_max = (int)nr[1];
   }
   M.nl_norm = I.get(p + "nonlinear_solver.norm", "l_2");
+  std::string fx = I.get(p + "nonlinear_solver.dx_inverse_fixed_tolerance", "false");
+  M.fixed_lin_tol = (fx == "true" || fx == "1");
+  M.min_lin_red = I.get_double(p + "nonlinear_solver.dx_inverse_min_relative_tolerance", 0.1);
+  M.lin_threshold = I.get_double(p + "nonlinear_solver.linearization_threshold", 0.0);
 }
 
 }  // namespace orc

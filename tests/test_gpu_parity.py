@@ -12,6 +12,85 @@ from tests.oracle_lib import OracleModel, bcsr_to_dense_csr, rel_l2_per_species
 
 pytestmark = pytest.mark.gpu
 
+def _fallback_model(kind, cell_type):
+    """Gray-Scott variants that need the coloured assembly kernel."""
+    extra_u = {"adv": "velocity.x.expression = 0.02\nvelocity.y.expression = -0.01*v\n"
+                      "velocity.jacobian.v.y.expression = -0.01\n",
+               "nld": "",
+               "num": ""}[kind]
+    du = ("cross_diffusion.u.expression = 1e-3*(1+u^2)\ncross_diffusion.u.jacobian.u.expression = 2e-3*u\n"
+          if kind == "nld" else "cross_diffusion.u.expression = 2e-3\n")
+    jt = "[model]\njacobian.type = numerical\n" if kind == "num" else ""
+    return f"""[grid]
+dimension = 2
+cells = 10 10
+cell_type = {cell_type}
+[compartments]
+domain.expression = 1
+{jt}[model.scalar_field.u]
+compartment = domain
+storage.expression = 1
+{du}{extra_u}reaction.expression = -u*v^2 + 0.04*(1-u)
+reaction.jacobian.u.expression = -v^2 - 0.04
+reaction.jacobian.v.expression = -2*u*v
+[model.scalar_field.v]
+compartment = domain
+storage.expression = 1
+cross_diffusion.v.expression = 1e-3
+reaction.expression = u*v^2 - 0.1*v
+reaction.jacobian.u.expression = v^2
+reaction.jacobian.v.expression = 2*u*v - 0.1
+[model.time_step_operator]
+type = ImplicitEuler
+linear_solver.preconditioner.type = Jacobi
+linear_solver.convergence_condition.relative_tolerance = 1e-12
+linear_solver.convergence_condition.iteration_range = 1 2000
+nonlinear_solver.convergence_condition.relative_tolerance = 1e-20
+nonlinear_solver.convergence_condition.absolute_tolerance = 1e-26
+"""
+
+
+class _Inline:
+    def __init__(self, ini, dt=0.5):
+        self.ini, self.dt = ini, dt
+
+    def initial_state(self, model):
+        ids = model.compartment_nodes(0)
+        out = np.empty(2 * ids.size)
+        out[0::2] = 1.0 - 0.5 * (ids % 7) / 7.0
+        out[1::2] = 0.25 * (ids % 5) / 5.0
+        return out
+
+
+FALLBACK = [(f"{k}_{c}", (lambda k=k, c=c: _Inline(_fallback_model(k, c))))
+            for k in ("adv", "nld", "num") for c in ("cube", "simplex")]
+
+
+@pytest.mark.parametrize("name,make", FALLBACK, ids=[f[0] for f in FALLBACK])
+def test_coloured_kernel_models(dca, name, make):
+    w = make()
+    gpu = dca.Model(w.ini, "", 0)
+    orc = OracleModel(w.ini)
+    x = _perturbed_state(w, gpu)
+    r_g, J_g = gpu.assemble(x, 0.1, 0.8, 0.6)
+    r_o, v_o = orc.assemble(x, 0.1, 0.8, 0.6)
+    assert np.abs(r_g - r_o).max() <= 1e-12 * np.abs(r_o).max()
+    A_g = bcsr_to_dense_csr(gpu, J_g).toarray()
+    A_o = orc.csr(v_o).toarray()
+    tol = 1e-6 if name.startswith("num") else 1e-12  # forward differences round differently
+    assert np.abs(A_g - A_o).max() <= tol * np.abs(A_o).max()
+    u0 = w.initial_state(gpu)
+    gpu.set_state(u0, 0.0)
+    orc.set_state(u0, 0.0)
+    t = 0.0
+    for k in range(4):
+        assert gpu.step(t, w.dt).status == 0
+        assert orc.step(w.dt)[0] == 0
+        t += w.dt
+        err = max(rel_l2_per_species(gpu.compartments(), gpu.get_state()[0], orc.get_state()[0]))
+        assert err <= 1e-8, (k, err)
+
+
 CASES = [
     ("gs_q1", lambda wl: wl.gray_scott(n=12, cell_type="cube")),
     ("gs_p1", lambda wl: wl.gray_scott(n=12, cell_type="simplex")),
