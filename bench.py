@@ -54,6 +54,28 @@ def make_workload(wl, name, precision):
     raise SystemExit(f"unknown workload {name}")
 
 
+TRAFFIC_FILES = {("ms96", "fp64"): "traffic_mitchell_schaefer_q1_96_fp64.txt"}
+VARIANT_KERNELS = {"lds_blocks": "spmv_blk_k", "lds_entries": "spmv_k", "rows_g8": "spmv_rows_k",
+                   "rows_g16": "spmv_rows_k", "rows_g32": "spmv_rows_k", "rows_g64": "spmv_rows_k"}
+
+
+def committed_traffic(workload, precision, chosen):
+    """HBM bytes per launch of the SpMV from the committed counter summary
+    (rocprofv3 FETCH_SIZE/WRITE_SIZE passes calibrated on a plain copy), when
+    it covers the kernel variant this run uses; None otherwise."""
+    name = TRAFFIC_FILES.get((workload, precision))
+    if not name or not (ROOT / "profiles" / name).exists():
+        return None
+    kv = {}
+    for line in (ROOT / "profiles" / name).read_text().splitlines():
+        if "=" in line and not line.startswith("#"):
+            k, v = line.split("=", 1)
+            kv[k.strip()] = v.strip()
+    if not kv.get("kernel", "").startswith(VARIANT_KERNELS.get(chosen, "?")):
+        return None
+    return float(kv["traffic_bytes"])
+
+
 def cpu_baseline(wl_obj, steps: int):
     """Oracle on the host cores: `steps` time steps of the same workload."""
     sys.path.insert(0, str(ROOT))
@@ -146,16 +168,14 @@ def main():
     roofline = None
     if sp:
         roofline = {"bound": "hbm", "kernel": spmv_kind, "achieved": sp["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": sp["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                    "frac": sp["gbs"] / HBM_PEAK_GBS, "traffic": committed_traffic(args.workload, args.precision, chosen),
                     "avg_launch_us": sp["avg_us"], "algorithmic_bytes_per_launch": sp["bytes_per_launch"]}
 
+    probe = None
     if args.copy_probe_mib:
-        n = args.copy_probe_mib * (1 << 20) // 8
-        a = torch.ones(n, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
+        nbytes = args.copy_probe_mib * (1 << 20)
+        ms = m.probe_copy(nbytes)
+        probe = {"bytes_each_way": nbytes, "ms": ms, "gbs_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}
 
     value = args.steps * w.stages * m.n_dof / elapsed
     cpu = None
@@ -183,6 +203,7 @@ def main():
         "kernels": kinds,
         "roofline": roofline,
         "spmv_autotune": {"candidates_us": tuning, "chosen": chosen},
+        "copy_probe": probe,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

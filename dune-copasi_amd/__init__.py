@@ -63,6 +63,7 @@ _SIGS = {
     "dcg_profile_enable": (_I, [_P, _I]),
     "dcg_profile_read": (_I, [_P, _I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "dcg_spmv_tuning": (_I, [_P, ctypes.POINTER(_I), _P, _P, ctypes.POINTER(_I)]),
+    "dcg_probe_copy": (_I, [_P, _S, ctypes.POINTER(_D)]),
 }
 
 PROFILE_KINDS = {"spmv64": 0, "spmv32": 1, "assembly_jacobian": 2, "assembly_residual": 3, "precond_setup": 4}
@@ -231,6 +232,12 @@ class Model:
         _check(self._lib.dcg_profile_read(self._h, PROFILE_KINDS[kind], ctypes.byref(l), ctypes.byref(ms),
                                           ctypes.byref(b)))
         return l.value, ms.value, b.value
+
+    def probe_copy(self, nbytes: int) -> float:
+        """Milliseconds of a plain device copy of nbytes (probe_copy_k)."""
+        ms = _D()
+        _check(self._lib.dcg_probe_copy(self._h, nbytes, ctypes.byref(ms)))
+        return ms.value
 
     def spmv_tuning(self):
         """{variant name: microseconds} of the set-up autotuner and the chosen variant."""

@@ -228,6 +228,15 @@ int dcg_profile_read(dcg_model_t h, int kind, long* launches, double* millisecon
   });
 }
 
+int dcg_probe_copy(dcg_model_t h, size_t bytes, double* milliseconds) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    double ms = h->m->probe_copy(static_cast<long>(bytes));
+    if (milliseconds) *milliseconds = ms;
+    return 0;
+  });
+}
+
 int dcg_spmv_tuning(dcg_model_t h, int* count, int* variant, double* microseconds, int* chosen) {
   if (check(h)) return -1;
   const auto& t = h->m->spmv_tuning();

@@ -892,6 +892,29 @@ void Model::profile(int kind, long* launches, double* ms, double* bytes) {
   *bytes = prof_data_[kind].bytes;
 }
 
+double Model::probe_copy(long bytes) {
+  require_device();
+  bytes = bytes / 32 * 32;
+  DeviceArray<char> a, b;
+  a.resize(static_cast<size_t>(bytes));
+  b.resize(static_cast<size_t>(bytes));
+  DCG_HIP(hipMemsetAsync(a.get(), 0, static_cast<size_t>(bytes), stream_));
+  hipEvent_t e0, e1;
+  DCG_HIP(hipEventCreate(&e0));
+  DCG_HIP(hipEventCreate(&e1));
+  k::probe_copy(a.get(), b.get(), bytes, stream_);
+  DCG_HIP(hipEventRecord(e0, stream_));
+  k::probe_copy(a.get(), b.get(), bytes, stream_);
+  DCG_HIP(hipEventRecord(e1, stream_));
+  DCG_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  DCG_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  synchronize();
+  return ms;
+}
+
 // Algorithmic bytes of one BCSR SpMV: values, column indices, row pointer,
 // x read once, y written once, plus fused dot-product operands.
 double Model::spmv_bytes(int vb, int xb, int extra_vectors) const {

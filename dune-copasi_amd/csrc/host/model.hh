@@ -93,6 +93,8 @@ class Model {
   void set_profiling(bool on);
   // launches, summed milliseconds and summed algorithmic bytes of one kind
   void profile(int kind, long* launches, double* ms, double* bytes);
+  // device copy of `bytes` through a plain kernel; returns its milliseconds
+  double probe_copy(long bytes);
 
   // SpMV autotuning result: (variant, microseconds) per candidate; chosen variant
   const std::vector<std::pair<int, double>>& spmv_tuning() const { return tuning_; }

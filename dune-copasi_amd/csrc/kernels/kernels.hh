@@ -34,6 +34,8 @@ void fill(double* p, double v, long n, hipStream_t s);
 void fill(float* p, float v, long n, hipStream_t s);
 void fill(int* p, int v, long n, hipStream_t s);
 void init_jacobian(double* val, long nval, const int* diag_pos, const int* slot_b, int nslot, int B, hipStream_t s);
+// copy `bytes` (multiple of 32) from a to b with a plain streaming kernel
+void probe_copy(const void* a, void* b, long bytes, hipStream_t s);
 // zero the B x B blocks at the given block positions
 void zero_blocks(double* val, const int* pos, long n, int B, hipStream_t s);
 void axpy(long n, double a, const double* x, double* y, hipStream_t s);

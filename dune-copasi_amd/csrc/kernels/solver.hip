@@ -76,6 +76,11 @@ __global__ void zero_blocks_k(double* __restrict__ val, const int* __restrict__ 
   if (i < n * bb) val[(size_t)pos[i / bb] * bb + i % bb] = 0.0;
 }
 
+// plain streaming copy of known size: calibrates HBM traffic counters
+__global__ void probe_copy_k(long n, const double4* __restrict__ a, double4* __restrict__ b) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
+}
+
 // y = a*x + y
 __global__ void axpy_k(long n, double a, const double* __restrict__ x, double* __restrict__ y) {
   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -851,6 +856,11 @@ void fill(int* p, int v, long n, hipStream_t s) {
 void init_jacobian(double* val, long nval, const int* diag_pos, const int* slot_b, int nslot, int B, hipStream_t s) {
   fill(val, 0.0, nval, s);
   pad_identity_k<<<grid_for((long)nslot * B), kThreads, 0, s>>>(val, diag_pos, slot_b, nslot, B);
+}
+
+void probe_copy(const void* a, void* b, long bytes, hipStream_t s) {
+  const long n = bytes / (long)sizeof(double4);
+  probe_copy_k<<<2048, kThreads, 0, s>>>(n, static_cast<const double4*>(a), static_cast<double4*>(b));
 }
 
 void zero_blocks(double* val, const int* pos, long n, int B, hipStream_t s) {

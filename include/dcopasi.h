@@ -131,6 +131,11 @@ int dcg_profile_read(dcg_model_t model, int kind, long* launches, double* millis
  * blocks, 1..4 row-split with 8/16/32/64 lanes per row; +100 for the FP32
  * matrix of the mixed-precision solve) and microseconds[k]; *chosen is the
  * FP64 variant used.  Arrays hold at least 16 entries. */
+/* Copy `bytes` on the model's device with a plain streaming kernel
+ * (probe_copy_k) and return its duration: the reference point that HBM
+ * traffic counters and bandwidth figures are calibrated against. */
+int dcg_probe_copy(dcg_model_t model, size_t bytes, double* milliseconds);
+
 int dcg_spmv_tuning(dcg_model_t model, int* count, int* variant, double* microseconds, int* chosen);
 
 #ifdef __cplusplus
