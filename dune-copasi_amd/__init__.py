@@ -64,7 +64,25 @@ _SIGS = {
     "dcg_profile_read": (_I, [_P, _I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "dcg_spmv_tuning": (_I, [_P, ctypes.POINTER(_I), _P, _P, ctypes.POINTER(_I)]),
     "dcg_probe_copy": (_I, [_P, _S, ctypes.POINTER(_D)]),
+    "dcg_model_create_ex": (_I, [ctypes.c_char_p, ctypes.c_char_p, _I, ctypes.c_uint, ctypes.POINTER(_P)]),
+    "dcg_param_info": (_I, [_P, _S, ctypes.c_char_p, _S, ctypes.POINTER(_S)]),
+    "dcg_set_params": (_I, [_P, _P, _S]),
+    "dcg_get_params": (_I, [_P, _P, _S]),
+    "dcg_device_bytes": (_I, [_P, ctypes.POINTER(_S)]),
+    "dcg_estimate_device_bytes": (_I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_S)]),
+    "dcg_device_memory": (_I, [_I, ctypes.POINTER(_S), ctypes.POINTER(_S)]),
+    "dcg_batch_create": (_I, [ctypes.c_char_p, _I, ctypes.POINTER(_P)]),
+    "dcg_batch_destroy": (None, [_P]),
+    "dcg_batch_add": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_I)]),
+    "dcg_batch_remove": (_I, [_P, _I]),
+    "dcg_batch_ids": (_I, [_P, _P, _S, ctypes.POINTER(_S)]),
+    "dcg_batch_member": (_I, [_P, _I, ctypes.POINTER(_P)]),
+    "dcg_batch_set_time_step": (_I, [_P, _I, _D]),
+    "dcg_batch_step": (_I, [_P, _P, _S]),
+    "dcg_batch_device_bytes": (_I, [_P, ctypes.POINTER(_S)]),
 }
+
+RUNTIME_PARAMETERS = 1
 
 PROFILE_KINDS = {"spmv64": 0, "spmv32": 1, "assembly_jacobian": 2, "assembly_residual": 3, "precond_setup": 4}
 
@@ -102,10 +120,18 @@ def _check(rc: int) -> int:
 class Model:
     """One model handle (dcg_model_t)."""
 
-    def __init__(self, ini: str, overrides: str = "", device: int = 0):
+    def __init__(self, ini: str, overrides: str = "", device: int = 0, runtime_parameters: bool = False,
+                 _handle=None):
         lib = load_library()
-        h = ctypes.c_void_p()
-        _check(lib.dcg_model_create(ini.encode(), overrides.encode(), device, ctypes.byref(h)))
+        if _handle is not None:
+            h = _handle
+        else:
+            h = ctypes.c_void_p()
+            if runtime_parameters:
+                _check(lib.dcg_model_create_ex(ini.encode(), overrides.encode(), device, RUNTIME_PARAMETERS,
+                                               ctypes.byref(h)))
+            else:
+                _check(lib.dcg_model_create(ini.encode(), overrides.encode(), device, ctypes.byref(h)))
         self._h = h
         self._lib = lib
         self.device = device
@@ -251,6 +277,31 @@ class Model:
             out[names[int(v) % 100] + ("_fp32" if v >= 100 else "")] = float(t)
         return out, names[ch.value]
 
+    # ---- run-time parameters and footprint
+    def param_names(self):
+        n = _S()
+        _check(self._lib.dcg_param_info(self._h, 0, None, 0, ctypes.byref(n)))
+        out = []
+        for i in range(n.value):
+            buf = ctypes.create_string_buffer(256)
+            _check(self._lib.dcg_param_info(self._h, i, buf, 256, None))
+            out.append(buf.value.decode())
+        return out
+
+    def params(self) -> np.ndarray:
+        v = np.zeros(len(self.param_names()))
+        _check(self._lib.dcg_get_params(self._h, _ptr(v), v.size))
+        return v
+
+    def set_params(self, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        _check(self._lib.dcg_set_params(self._h, _ptr(v), v.size))
+
+    def device_bytes(self) -> int:
+        b = _S()
+        _check(self._lib.dcg_device_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
     # ---- helpers on the layout
     def slot_dofs(self):
         """For every slot: (first packed dof, species count)."""
@@ -263,3 +314,73 @@ class Model:
             cnt[s:s + n] = b
             s += n
         return first, cnt
+
+
+def estimate_device_bytes(ini: str, overrides: str = "") -> int:
+    """HBM bytes a model of this INI would hold (computed on the host)."""
+    b = _S()
+    _check(load_library().dcg_estimate_device_bytes(ini.encode(), overrides.encode(), ctypes.byref(b)))
+    return b.value
+
+
+def device_memory(device: int = 0):
+    """(free, total) bytes of a device."""
+    f, t = _S(), _S()
+    _check(load_library().dcg_device_memory(device, ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
+
+
+class Batch:
+    """Many independent models advanced together (dcg_batch_t)."""
+
+    def __init__(self, options: str = "", device: int = 0):
+        lib = load_library()
+        h = ctypes.c_void_p()
+        _check(lib.dcg_batch_create(options.encode(), device, ctypes.byref(h)))
+        self._h = h
+        self._lib = lib
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.dcg_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def add(self, ini: str, overrides: str = "") -> int:
+        i = _I()
+        _check(self._lib.dcg_batch_add(self._h, ini.encode(), overrides.encode(), ctypes.byref(i)))
+        return i.value
+
+    def remove(self, member_id: int):
+        _check(self._lib.dcg_batch_remove(self._h, member_id))
+
+    def ids(self):
+        n = _S()
+        _check(self._lib.dcg_batch_ids(self._h, None, 0, ctypes.byref(n)))
+        v = np.zeros(max(1, n.value), dtype=np.int32)
+        _check(self._lib.dcg_batch_ids(self._h, _ptr(v), v.size, None))
+        return [int(x) for x in v[:n.value]]
+
+    def member(self, member_id: int) -> Model:
+        """A Model view of a member (state, parameters, mass); the member stays in the batch."""
+        h = ctypes.c_void_p()
+        _check(self._lib.dcg_batch_member(self._h, member_id, ctypes.byref(h)))
+        return Model("", "", self.device, _handle=h)
+
+    def set_time_step(self, member_id: int, dt: float):
+        _check(self._lib.dcg_batch_set_time_step(self._h, member_id, dt))
+
+    def step(self):
+        """One step of every member; list of StepStats in ids() order."""
+        n = len(self.ids())
+        st = (StepStats * max(1, n))()
+        _check(self._lib.dcg_batch_step(self._h, ctypes.cast(st, ctypes.c_void_p), max(1, n)))
+        return [st[i] for i in range(n)]
+
+    def device_bytes(self) -> int:
+        b = _S()
+        _check(self._lib.dcg_batch_device_bytes(self._h, ctypes.byref(b)))
+        return b.value

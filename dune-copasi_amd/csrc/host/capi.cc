@@ -4,10 +4,17 @@
 #include <string>
 
 #include "../../../include/dcopasi.h"
+#include "batch.hh"
 #include "model.hh"
 
+// owning handle, or a view of a batch member (own empty)
 struct dcg_model {
-  std::unique_ptr<dcopasi::Model> m;
+  std::unique_ptr<dcopasi::Model> own;
+  dcopasi::Model* m = nullptr;
+};
+
+struct dcg_batch {
+  std::unique_ptr<dcopasi::Batch> b;
 };
 
 namespace {
@@ -50,7 +57,23 @@ int dcg_model_create(const char* ini_text, const char* overrides, int device, dc
     if (!out) throw dcopasi::Error("null output handle");
     *out = nullptr;
     auto h = std::make_unique<dcg_model>();
-    h->m = std::make_unique<dcopasi::Model>(ini_text ? ini_text : "", overrides ? overrides : "", device);
+    h->own = std::make_unique<dcopasi::Model>(ini_text ? ini_text : "", overrides ? overrides : "", device);
+    h->m = h->own.get();
+    *out = h.release();
+    return 0;
+  });
+}
+
+int dcg_model_create_ex(const char* ini_text, const char* overrides, int device, unsigned flags, dcg_model_t* out) {
+  return guarded([&] {
+    if (!out) throw dcopasi::Error("null output handle");
+    *out = nullptr;
+    if (flags & ~DCG_RUNTIME_PARAMETERS) throw dcopasi::Error("unknown creation flags");
+    dcopasi::ModelOptions opt;
+    opt.runtime_parameters = (flags & DCG_RUNTIME_PARAMETERS) != 0;
+    auto h = std::make_unique<dcg_model>();
+    h->own = std::make_unique<dcopasi::Model>(ini_text ? ini_text : "", overrides ? overrides : "", device, opt);
+    h->m = h->own.get();
     *out = h.release();
     return 0;
   });
@@ -248,6 +271,169 @@ int dcg_spmv_tuning(dcg_model_t h, int* count, int* variant, double* microsecond
   }
   if (chosen) *chosen = h->m->spmv_variant();
   return 0;
+}
+
+int dcg_param_info(dcg_model_t h, size_t i, char* name, size_t name_len, size_t* count) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    const auto& p = h->m->param_names();
+    if (count) *count = p.size();
+    if (name && name_len) {
+      if (i >= p.size()) throw dcopasi::Error("parameter index out of range");
+      copy_name(p[i], name, name_len);
+    }
+    return 0;
+  });
+}
+
+int dcg_set_params(dcg_model_t h, const double* values, size_t n) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    h->m->set_params(values, n);
+    return 0;
+  });
+}
+
+int dcg_get_params(dcg_model_t h, double* values, size_t n) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    h->m->get_params(values, n);
+    return 0;
+  });
+}
+
+int dcg_device_bytes(dcg_model_t h, size_t* bytes) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    if (!bytes) throw dcopasi::Error("null output");
+    *bytes = h->m->device_bytes();
+    return 0;
+  });
+}
+
+int dcg_estimate_device_bytes(const char* ini_text, const char* overrides, size_t* bytes) {
+  return guarded([&] {
+    if (!bytes) throw dcopasi::Error("null output");
+    dcopasi::Model m(ini_text ? ini_text : "", overrides ? overrides : "", -1);
+    const auto& c = m.solver();
+    *bytes = dcopasi::Model::estimate_device_bytes(m.layout(), m.rk().stages, c.precision == dcopasi::Precision::Mixed,
+                                                   c.precond == dcopasi::Preconditioner::ILU0);
+    return 0;
+  });
+}
+
+int dcg_device_memory(int device, size_t* free_bytes, size_t* total_bytes) {
+  return guarded([&] {
+    DCG_HIP(hipSetDevice(device));
+    size_t f = 0, t = 0;
+    DCG_HIP(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return 0;
+  });
+}
+
+static int check_batch(dcg_batch_t b) {
+  if (!b || !b->b) {
+    g_error = "null batch handle";
+    return -1;
+  }
+  return 0;
+}
+
+int dcg_batch_create(const char* options_ini, int device, dcg_batch_t* out) {
+  return guarded([&] {
+    if (!out) throw dcopasi::Error("null output handle");
+    *out = nullptr;
+    if (device < 0) throw dcopasi::Error("a batch needs a device");
+    auto h = std::make_unique<dcg_batch>();
+    h->b = std::make_unique<dcopasi::Batch>(device, dcopasi::BatchOptions::from_ini(options_ini ? options_ini : ""));
+    *out = h.release();
+    return 0;
+  });
+}
+
+void dcg_batch_destroy(dcg_batch_t batch) { delete batch; }
+
+int dcg_batch_add(dcg_batch_t b, const char* ini_text, const char* overrides, int* id) {
+  if (check_batch(b)) return -1;
+  return guarded([&] {
+    const int i = b->b->add(ini_text ? ini_text : "", overrides ? overrides : "");
+    if (id) *id = i;
+    return 0;
+  });
+}
+
+int dcg_batch_remove(dcg_batch_t b, int id) {
+  if (check_batch(b)) return -1;
+  return guarded([&] {
+    b->b->remove(id);
+    return 0;
+  });
+}
+
+int dcg_batch_ids(dcg_batch_t b, int* ids, size_t capacity, size_t* count) {
+  if (check_batch(b)) return -1;
+  return guarded([&] {
+    const auto v = b->b->ids();
+    if (count) *count = v.size();
+    if (ids) {
+      if (capacity < v.size()) throw dcopasi::Error("id buffer too small");
+      for (size_t i = 0; i < v.size(); ++i) ids[i] = v[i];
+    }
+    return 0;
+  });
+}
+
+int dcg_batch_member(dcg_batch_t b, int id, dcg_model_t* out) {
+  if (check_batch(b)) return -1;
+  return guarded([&] {
+    if (!out) throw dcopasi::Error("null output handle");
+    auto h = std::make_unique<dcg_model>();
+    h->m = &b->b->member(id);
+    *out = h.release();
+    return 0;
+  });
+}
+
+int dcg_batch_set_time_step(dcg_batch_t b, int id, double dt) {
+  if (check_batch(b)) return -1;
+  return guarded([&] {
+    if (!(dt > 0.0)) throw dcopasi::Error("time step must be positive");
+    b->b->set_time_step(id, dt);
+    return 0;
+  });
+}
+
+int dcg_batch_step(dcg_batch_t b, dcg_step_stats* stats, size_t capacity) {
+  if (check_batch(b)) return -1;
+  return guarded([&] {
+    if (stats && capacity < b->b->size()) throw dcopasi::Error("stats buffer too small");
+    std::vector<dcopasi::StepStats> st;
+    b->b->step(&st);
+    int worst = 0;
+    for (size_t i = 0; i < st.size(); ++i) {
+      if (stats) {
+        stats[i].status = st[i].status;
+        stats[i].stages = st[i].stages;
+        stats[i].newton_iterations = st[i].newton_iterations;
+        stats[i].linear_iterations = st[i].linear_iterations;
+        stats[i].refinement_steps = st[i].refinement_steps;
+        stats[i].residual_norm = st[i].residual_norm;
+      }
+      worst = std::max(worst, st[i].status);
+    }
+    return worst;
+  });
+}
+
+int dcg_batch_device_bytes(dcg_batch_t b, size_t* bytes) {
+  if (check_batch(b)) return -1;
+  return guarded([&] {
+    if (!bytes) throw dcopasi::Error("null output");
+    *bytes = b->b->device_bytes();
+    return 0;
+  });
 }
 
 }  // extern "C"

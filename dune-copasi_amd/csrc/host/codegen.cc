@@ -161,8 +161,47 @@ bool tensor_const(const TensorTerm& t) {
   return true;
 }
 
+// diffusion whose entries depend on nothing but run-time parameters and time:
+// its stiffness contribution is D times a reference table
+bool tensor_uniform(const TensorTerm& t) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      if (!t.d[i][j]) continue;
+      std::vector<std::string> vars;
+      expr::collect_vars(t.d[i][j], vars);
+      for (const auto& v : vars)
+        if (v != "time" && v.rfind("$", 0) != 0) return false;
+    }
+  return true;
+}
+
 const char* kPrelude = R"(
 #define DCG_B %B%
+// DCG_GEO: element tables are those of the unit cell and the spacing comes at
+// run time in prm[0..7] = {h_x, h_y, h_z, 1/h_x, 1/h_y, 1/h_z, |cell|, 0}
+#define DCG_GEO %GEO%
+#if DCG_GEO
+#define GP(p, q, a, k) (GPHI[p][q][a][k] * prm[3 + (k)])
+#define FACQ(q) (FAC[q] * prm[6])
+#define XQP(p, q, k) (XQ[p][q][k] * prm[k])
+#else
+#define GP(p, q, a, k) GPHI[p][q][a][k]
+#define FACQ(q) FAC[q]
+#define XQP(p, q, k) XQ[p][q][k]
+#endif
+// descriptor of one batch member for the batched tile kernels
+struct dcg_asm_member {
+  double* vec[8];
+  double* val;
+  const int* own_start;
+  const unsigned* own_mask;
+  const double* prm;
+  const int* g2l[4];
+  const unsigned char* cmask[4];
+  int n[4];
+  double origin[3];
+  double h[3];
+};
 __device__ __forceinline__ double dcg_sign(double v) { return (double)((v > 0.0) - (v < 0.0)); }
 __device__ __forceinline__ int dcg_pos(const int* own_start, const unsigned* own_mask, int s, int o) {
   return own_start[s] + __popc(own_mask[s] & ((1u << o) - 1u));
@@ -175,7 +214,8 @@ namespace %C% {
 template <int P>
 __device__ __forceinline__ void residual(const double (&xl)[NV][BC], double cx, double cy, double cz, double t,
                                          double cm, double ca, double (&rl)[NV][BC], double (*mc)[BC][BC],
-                                         double (*gc)[BC][BC][DIM], double (*dc)[BC][BC][DIM][DIM]) {
+                                         double (*gc)[BC][BC][DIM], double (*dc)[BC][BC][DIM][DIM],
+                                         const double* __restrict__ prm) {
 #pragma unroll
   for (int a = 0; a < NV; ++a)
 #pragma unroll
@@ -194,22 +234,22 @@ __device__ __forceinline__ void residual(const double (&xl)[NV][BC], double cx, 
         double gs = 0.0;
         if (k < DIM) {
 #pragma unroll
-          for (int a = 0; a < NV; ++a) gs += GPHI[P][q][a][k < DIM ? k : 0] * xl[a][i];
+          for (int a = 0; a < NV; ++a) gs += GP(P, q, a, k < DIM ? k : 0) * xl[a][i];
         }
         g[i][k] = gs;
       }
     }
-    const double px = cx + XQ[P][q][0], py = cy + XQ[P][q][1], pz = cz + XQ[P][q][2];
+    const double px = cx + XQP(P, q, 0), py = cy + XQP(P, q, 1), pz = cz + XQP(P, q, 2);
     (void)px; (void)py; (void)pz;
     double rs[BC], rf[BC][DIM];
-    qp(q, u, g, px, py, pz, t, cm, ca, rs, rf, mc ? mc[q] : nullptr, gc ? gc[q] : nullptr, dc ? dc[q] : nullptr);
+    qp(q, u, g, px, py, pz, t, cm, ca, rs, rf, mc ? mc[q] : nullptr, gc ? gc[q] : nullptr, dc ? dc[q] : nullptr, prm);
 #pragma unroll
     for (int a = 0; a < NV; ++a)
 #pragma unroll
       for (int i = 0; i < BC; ++i) {
         double s = rs[i] * PHI[q][a];
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) s += rf[i][k] * GPHI[P][q][a][k];
+        for (int k = 0; k < DIM; ++k) s += rf[i][k] * GP(P, q, a, k);
         rl[a][i] += s;
       }
   }
@@ -218,7 +258,8 @@ __device__ __forceinline__ void residual(const double (&xl)[NV][BC], double cx, 
 template <int P>
 __device__ __forceinline__ void element(const double (&xc)[NCORNER][BC], const int (&sl)[NCORNER], double cx,
                                         double cy, double cz, double t, double cm, double ca, double* r,
-                                        double* val, const int* own_start, const unsigned* own_mask, int mode) {
+                                        double* val, const int* own_start, const unsigned* own_mask, int mode,
+                                        const double* __restrict__ prm) {
   double xl[NV][BC];
 #pragma unroll
   for (int a = 0; a < NV; ++a)
@@ -227,7 +268,7 @@ __device__ __forceinline__ void element(const double (&xc)[NCORNER][BC], const i
   double rl[NV][BC];
   const bool want_j = (mode & 2) != 0;
 #if NUMERICAL_JAC
-  residual<P>(xl, cx, cy, cz, t, cm, ca, rl, nullptr, nullptr, nullptr);
+  residual<P>(xl, cx, cy, cz, t, cm, ca, rl, nullptr, nullptr, nullptr, prm);
   if (want_j) {
 #pragma unroll 1
     for (int c = 0; c < NV; ++c) {
@@ -241,7 +282,7 @@ __device__ __forceinline__ void element(const double (&xc)[NCORNER][BC], const i
         const double delta = FD_EPS * (1.0 + fabs(xl[c][j]));
         xp[c][j] += delta;
         double rp[NV][BC];
-        residual<P>(xp, cx, cy, cz, t, cm, ca, rp, nullptr, nullptr, nullptr);
+        residual<P>(xp, cx, cy, cz, t, cm, ca, rp, nullptr, nullptr, nullptr, prm);
 #pragma unroll
         for (int a = 0; a < NV; ++a) {
           const int sa = sl[CORNER[P][a]];
@@ -268,11 +309,11 @@ __device__ __forceinline__ void element(const double (&xc)[NCORNER][BC], const i
               nullptr,
 #endif
 #if HAS_DC
-              want_j ? dc : nullptr
+              want_j ? dc : nullptr,
 #else
-              nullptr
+              nullptr,
 #endif
-  );
+              prm);
   if (want_j) {
 #pragma unroll
     for (int a = 0; a < NV; ++a) {
@@ -294,7 +335,7 @@ __device__ __forceinline__ void element(const double (&xc)[NCORNER][BC], const i
 #pragma unroll
             for (int q = 0; q < NQ; ++q)
 #pragma unroll
-              for (int k = 0; k < DIM; ++k) s += PHI[q][c] * GPHI[P][q][a][k] * gc[q][i][j][k];
+              for (int k = 0; k < DIM; ++k) s += PHI[q][c] * GP(P, q, a, k) * gc[q][i][j][k];
 #endif
 #if HAS_DC
 #pragma unroll
@@ -302,7 +343,7 @@ __device__ __forceinline__ void element(const double (&xc)[NCORNER][BC], const i
 #pragma unroll
               for (int k = 0; k < DIM; ++k)
 #pragma unroll
-                for (int l = 0; l < DIM; ++l) s += GPHI[P][q][a][k] * dc[q][i][j][k][l] * GPHI[P][q][c][l];
+                for (int l = 0; l < DIM; ++l) s += GP(P, q, a, k) * dc[q][i][j][k][l] * GP(P, q, c, l);
 #endif
             s += ca * const_diffusion<P>(a, c, i, j);
             blk[i * DCG_B + j] += s;
@@ -327,7 +368,7 @@ extern "C" __global__ void __launch_bounds__(128)
          int first, int count, const double* __restrict__ x, double t, double cm, double ca,
          double* __restrict__ r, double* __restrict__ val, const int* __restrict__ own_start,
          const unsigned* __restrict__ own_mask, int n0, int n1, double ox, double oy, double oz, double hx,
-         double hy, double hz, int mode) {
+         double hy, double hz, int mode, const double* __restrict__ prm) {
   using namespace %C%;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= count) return;
@@ -355,7 +396,8 @@ __device__ __forceinline__ void facet_residual(const double (&xi)[NVF][BI > 0 ? 
                                                const double (&xo)[NVF][BO > 0 ? BO : 1], const double* fd,
                                                double t, double ca, double (&ri)[NVF][BI > 0 ? BI : 1],
                                                double (&ro)[NVF][BO > 0 ? BO : 1], double* Jii, double* Jio,
-                                               double* Joi, double* Joo) {
+                                               double* Joi, double* Joo, const double* __restrict__ prm) {
+  (void)prm;
 #pragma unroll
   for (int f = 0; f < NVF; ++f) {
 #pragma unroll
@@ -394,7 +436,7 @@ __device__ __forceinline__ void facet_residual(const double (&xi)[NVF][BI > 0 ? 
 extern "C" __global__ void __launch_bounds__(64)
 %KERNEL%(const int* __restrict__ fint, const double* __restrict__ fdbl, int first, int count,
          const double* __restrict__ x, double t, double ca, double* __restrict__ r, double* __restrict__ val,
-         int mode) {
+         int mode, const double* __restrict__ prm) {
   using namespace %F%;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= count) return;
@@ -412,7 +454,7 @@ extern "C" __global__ void __launch_bounds__(64)
   double ri[NVF][BI > 0 ? BI : 1], ro[NVF][BO > 0 ? BO : 1];
   const bool want_j = (mode & 2) != 0;
 #if NUMERICAL_JAC
-  facet_residual(xi, xo, fd, t, ca, ri, ro, nullptr, nullptr, nullptr, nullptr);
+  facet_residual(xi, xo, fd, t, ca, ri, ro, nullptr, nullptr, nullptr, nullptr, prm);
   if (want_j) {
     const int* pos = fi + 2 * NVF;
 #pragma unroll 1
@@ -434,7 +476,7 @@ extern "C" __global__ void __launch_bounds__(64)
           const double delta = FD_EPS * (1.0 + fabs(xv));
           xv += delta;
           double ri2[NVF][BI > 0 ? BI : 1], ro2[NVF][BO > 0 ? BO : 1];
-          facet_residual(xi2, xo2, fd, t, ca, ri2, ro2, nullptr, nullptr, nullptr, nullptr);
+          facet_residual(xi2, xo2, fd, t, ca, ri2, ro2, nullptr, nullptr, nullptr, nullptr, prm);
 #pragma unroll
           for (int f = 0; f < NVF; ++f) {
 #pragma unroll
@@ -460,7 +502,7 @@ extern "C" __global__ void __launch_bounds__(64)
   for (int k = 0; k < NVF * NVF * BI * BO; ++k) { Jio[k] = 0.0; Joi[k] = 0.0; }
 #pragma unroll
   for (int k = 0; k < NVF * NVF * BO * BO; ++k) Joo[k] = 0.0;
-  facet_residual(xi, xo, fd, t, ca, ri, ro, Jii, Jio, Joi, Joo);
+  facet_residual(xi, xo, fd, t, ca, ri, ro, Jii, Jio, Joi, Joo, prm);
   if (want_j) {
     const int* pos = fi + 2 * NVF;
 #pragma unroll
@@ -513,10 +555,17 @@ void replace_all(std::string& s, const std::string& from, const std::string& to)
 }
 
 // variable names -> C expressions for one compartment's volume expressions
-std::map<std::string, std::string> volume_vars(const ModelSpec& m, int comp, double evol) {
+// run-time parameters follow the eight geometry words of prm
+void param_vars(const ModelSpec& m, std::map<std::string, std::string>& v) {
+  for (size_t k = 0; k < m.params.size(); ++k)
+    v["$" + m.params[k]] = "prm[" + std::to_string(kGeoWords + k) + "]";
+}
+
+std::map<std::string, std::string> volume_vars(const ModelSpec& m, int comp, double evol, bool geo) {
   std::map<std::string, std::string> v;
+  param_vars(m, v);
   v["time"] = "t";
-  v["entity_volume"] = D(evol);
+  v["entity_volume"] = geo ? "(" + D(evol) + " * prm[6])" : D(evol);
   v["in_volume"] = "1.0";
   v["in_boundary"] = "0.0";
   v["in_skeleton"] = "0.0";
@@ -534,13 +583,22 @@ std::map<std::string, std::string> volume_vars(const ModelSpec& m, int comp, dou
   return v;
 }
 
+// unit-spacing copy of a grid: the element tables of run-time geometry
+Grid unit_cell(const Grid& g) {
+  Grid u = g;
+  u.h = {1.0, 1.0, 1.0};
+  return u;
+}
+
 std::string gen_volume(const ModelSpec& m, const Grid& g, int comp, const std::string& ns, const std::string& kname,
-                       bool numerical) {
-  const auto T = element_tables(g);
+                       bool numerical, bool geo) {
+  const auto T = element_tables(geo ? unit_cell(g) : g);
   const auto& species = m.comp_species[comp];
   const int bc = static_cast<int>(species.size());
   const int d = m.dim;
-  auto vars = volume_vars(m, comp, T.evol);
+  auto vars = volume_vars(m, comp, T.evol, geo);
+  // numeric diffusion is folded into tables only when the spacing is known
+  auto folded = [&](const TensorTerm& t) { return !geo && tensor_const(t); };
   auto E = [&](const Ex& e) { return expr::emit(e, vars); };
 
   bool has_gc = false, has_dc = false;
@@ -549,7 +607,7 @@ std::string gen_volume(const ModelSpec& m, const Grid& g, int comp, const std::s
     for (int k = 0; k < d; ++k) has_gc = has_gc || sp.velocity[k];
     for (const auto& dt : sp.diffusion) {
       if (!dt.jac.empty()) has_gc = true;
-      if (!tensor_const(dt.D)) has_dc = true;
+      if (!folded(dt.D)) has_dc = true;
     }
   }
   std::ostringstream o;
@@ -574,7 +632,7 @@ std::string gen_volume(const ModelSpec& m, const Grid& g, int comp, const std::s
     for (int i = 0; i < bc; ++i) {
       const auto& sp = m.species[species[i]];
       for (const auto& dt : sp.diffusion) {
-        if (!tensor_const(dt.D)) continue;
+        if (!folded(dt.D)) continue;
         int j = m.species[dt.wrt].local;
         double Dm[3][3] = {{0}};
         for (int k = 0; k < d; ++k)
@@ -612,9 +670,9 @@ std::string gen_volume(const ModelSpec& m, const Grid& g, int comp, const std::s
   // quadrature point kernel
   o << "__device__ __forceinline__ void qp(int q, const double (&u)[BC], const double (&g)[BC][3], double px, double py, "
        "double pz, double t, double cm, double ca, double (&rs)[BC], double (&rf)[BC][DIM], double (*mc)[BC], "
-       "double (*gc)[BC][DIM], double (*dc)[BC][DIM][DIM]) {\n";
-  o << "  (void)px; (void)py; (void)pz; (void)t; (void)g; (void)gc; (void)dc;\n";
-  o << "  const double fac = FAC[q];\n";
+       "double (*gc)[BC][DIM], double (*dc)[BC][DIM][DIM], const double* __restrict__ prm) {\n";
+  o << "  (void)px; (void)py; (void)pz; (void)t; (void)g; (void)gc; (void)dc; (void)prm;\n";
+  o << "  const double fac = FACQ(q);\n";
   o << "#pragma unroll\n  for (int i = 0; i < BC; ++i) { rs[i] = 0.0;\n#pragma unroll\n for (int k = 0; k < DIM; ++k) rf[i][k] = 0.0; }\n";
   o << "  if (mc) {\n#pragma unroll\n  for (int i = 0; i < BC; ++i)\n#pragma unroll\n for (int j = 0; j < BC; ++j) mc[i][j] = 0.0;}\n";
   if (has_gc)
@@ -672,7 +730,7 @@ std::string gen_volume(const ModelSpec& m, const Grid& g, int comp, const std::s
           if (!e) continue;
           o << "      { const double Dv = " << E(e) << ";\n";
           o << "        rf[" << I << "][" << k << "] += fac * ca * Dv * g[" << J << "][" << l << "];\n";
-          if (!tensor_const(dt.D)) o << "        if (dc) dc[" << I << "][" << J << "][" << k << "][" << l << "] += fac * ca * Dv;\n";
+          if (!folded(dt.D)) o << "        if (dc) dc[" << I << "][" << J << "][" << k << "][" << l << "] += fac * ca * Dv;\n";
           o << "      }\n";
         }
       for (const auto& dj : dt.jac) {
@@ -696,7 +754,7 @@ std::string gen_volume(const ModelSpec& m, const Grid& g, int comp, const std::s
   std::ostringstream disp;
   for (int p = 0; p < g.nclass; ++p)
     disp << "  if ((msk >> " << p << ") & 1u) " << ns << "::element<" << p
-         << ">(xc, sl, cx, cy, cz, t, cm, ca, r, val, own_start, own_mask, mode);\n";
+         << ">(xc, sl, cx, cy, cz, t, cm, ca, r, val, own_start, own_mask, mode, prm);\n";
   replace_all(body, "%DISPATCH%", disp.str());
   std::ostringstream full;
   full << "#define NUMERICAL_JAC " << (numerical ? 1 : 0) << "\n#define HAS_GC " << (has_gc ? 1 : 0) << "\n#define HAS_DC "
@@ -732,6 +790,7 @@ std::string gen_facet(const ModelSpec& m, const Grid& g, int cin, int cout, bool
     }
   auto vars_for = [&](bool inside) {
     std::map<std::string, std::string> v;
+    param_vars(m, v);
     v["time"] = "t";
     v["entity_volume"] = "fvol";
     v["in_volume"] = "0.0";
@@ -845,7 +904,8 @@ constexpr int UNION_WORDS = MC_WORDS > RL_WORDS ? MC_WORDS : RL_WORDS;
 
 // element residual of class p (runtime) from corner values
 __device__ __forceinline__ void residual_rt(int p, const double (&xl)[NV][BC], double cx, double cy, double cz,
-                                            double t, double cm, double ca, double (&rl)[NV][BC]) {
+                                            double t, double cm, double ca, double (&rl)[NV][BC],
+                                            const double* __restrict__ prm) {
   for (int a = 0; a < NV; ++a)
     for (int i = 0; i < BC; ++i) rl[a][i] = 0.0;
   for (int q = 0; q < NQ; ++q) {
@@ -854,42 +914,39 @@ __device__ __forceinline__ void residual_rt(int p, const double (&xl)[NV][BC], d
       double s = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
       for (int a = 0; a < NV; ++a) {
         s += PHI[q][a] * xl[a][i];
-        g0 += GPHI[p][q][a][0] * xl[a][i];
-        if (DIM > 1) g1 += GPHI[p][q][a][DIM > 1 ? 1 : 0] * xl[a][i];
-        if (DIM > 2) g2 += GPHI[p][q][a][DIM > 2 ? 2 : 0] * xl[a][i];
+        g0 += GP(p, q, a, 0) * xl[a][i];
+        if (DIM > 1) g1 += GP(p, q, a, DIM > 1 ? 1 : 0) * xl[a][i];
+        if (DIM > 2) g2 += GP(p, q, a, DIM > 2 ? 2 : 0) * xl[a][i];
       }
       u[i] = s;
       g[i][0] = g0;
       g[i][1] = g1;
       g[i][2] = g2;
     }
-    const double px = cx + XQ[p][q][0], py = cy + XQ[p][q][1], pz = cz + XQ[p][q][2];
+    const double px = cx + XQP(p, q, 0), py = cy + XQP(p, q, 1), pz = cz + XQP(p, q, 2);
     double rs[BC], rf[BC][DIM];
-    qp(q, u, g, px, py, pz, t, cm, ca, rs, rf, nullptr, nullptr, nullptr);
+    qp(q, u, g, px, py, pz, t, cm, ca, rs, rf, nullptr, nullptr, nullptr, prm);
     for (int a = 0; a < NV; ++a)
       for (int i = 0; i < BC; ++i) {
         double s = rs[i] * PHI[q][a];
-        for (int k = 0; k < DIM; ++k) s += rf[i][k] * GPHI[p][q][a][k];
+        for (int k = 0; k < DIM; ++k) s += rf[i][k] * GP(p, q, a, k);
         rl[a][i] += s;
       }
   }
 }
-}  // namespace %C%
 
-extern "C" __global__ void __launch_bounds__(256)
-%KERNEL%(const int* __restrict__ tiles, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,
+// one tile of vertices: the residual rows and/or the BCSR block rows it owns
+__device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,
          const double* __restrict__ x, const double* __restrict__ base, double t, double cm, double ca,
          double* __restrict__ r, double* __restrict__ val, const int* __restrict__ own_start,
          const unsigned* __restrict__ own_mask, int n0, int n1, int n2, double ox, double oy, double oz, double hx,
-         double hy, double hz, int mode) {
-  using namespace %C%;
+         double hy, double hz, int mode, const double* __restrict__ prm) {
   __shared__ double xs[NVB * BC];
   __shared__ int sl[NVB];
   __shared__ unsigned char cms[NCB];
   __shared__ double work[UNION_WORDS];
   const int ntx = (n0 + 1 + TX - 1) / TX;
   const int nty = DIM > 1 ? (n1 + 1 + TY - 1) / TY : 1;
-  const int tile = tiles[blockIdx.x];
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
   const int vi0 = tx * TX - 1, vj0 = DIM > 1 ? ty * TY - 1 : 0, vk0 = DIM > 2 ? tz * TZ - 1 : 0;
   // phase 0: vertex box (state, slots) and cell box (class masks)
@@ -913,6 +970,8 @@ extern "C" __global__ void __launch_bounds__(256)
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   if (mode & 2) {
+    double Dv[BC][BC][DIM][DIM];
+    uniform_diffusion(t, prm, Dv);
     // phase 1: Jacobian coefficients per (cell, class, point); class and point
     // are uniform per wave so the reference-element tables are scalar loads.
     // LDS layout is cell-minor: work[((p*NQ + q)*BC*BC + i*BC + j)*NCB + cell].
@@ -933,20 +992,20 @@ extern "C" __global__ void __launch_bounds__(256)
           double s = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
           for (int a = 0; a < NV; ++a) {
             s += PHI[q][a] * xl[a][i];
-            g0 += GPHI[p][q][a][0] * xl[a][i];
-            if (DIM > 1) g1 += GPHI[p][q][a][DIM > 1 ? 1 : 0] * xl[a][i];
-            if (DIM > 2) g2 += GPHI[p][q][a][DIM > 2 ? 2 : 0] * xl[a][i];
+            g0 += GP(p, q, a, 0) * xl[a][i];
+            if (DIM > 1) g1 += GP(p, q, a, DIM > 1 ? 1 : 0) * xl[a][i];
+            if (DIM > 2) g2 += GP(p, q, a, DIM > 2 ? 2 : 0) * xl[a][i];
           }
           u[i] = s;
           g[i][0] = g0;
           g[i][1] = g1;
           g[i][2] = g2;
         }
-        const double px = ox + (vi0 + ci) * hx + XQ[p][q][0];
-        const double py = oy + (vj0 + cj) * hy + XQ[p][q][1];
-        const double pz = oz + (vk0 + ck) * hz + XQ[p][q][2];
+        const double px = ox + (vi0 + ci) * hx + XQP(p, q, 0);
+        const double py = oy + (vj0 + cj) * hy + XQP(p, q, 1);
+        const double pz = oz + (vk0 + ck) * hz + XQP(p, q, 2);
         double rs[BC], rf[BC][DIM], mc[BC][BC];
-        qp(q, u, g, px, py, pz, t, cm, ca, rs, rf, mc, nullptr, nullptr);
+        qp(q, u, g, px, py, pz, t, cm, ca, rs, rf, mc, nullptr, nullptr, prm);
         for (int i = 0; i < BC; ++i)
           for (int j = 0; j < BC; ++j) work[(size_t)(pq * BC * BC + i * BC + j) * NCB + cl] = mc[i][j];
       }
@@ -992,7 +1051,16 @@ extern "C" __global__ void __launch_bounds__(256)
                     for (int j = 0; j < BC; ++j) acc[i][j] += w * mcp[(size_t)(i * BC + j) * NCB];
                 }
                 for (int i = 0; i < BC; ++i)
-                  for (int j = 0; j < BC; ++j) acc[i][j] += ca * kd_rt(p, la, lc, i, j);
+                  for (int j = 0; j < BC; ++j) {
+                    if constexpr (CONST_DIFFUSION) {
+                      acc[i][j] += ca * kd_rt(p, la, lc, i, j);
+                    } else {
+                      double kd = 0.0;
+                      for (int k = 0; k < DIM; ++k)
+                        for (int l = 0; l < DIM; ++l) kd += Dv[i][j][k][l] * KDU[p][la][lc][k][l];
+                      acc[i][j] += ca * kd;
+                    }
+                  }
               }
             }
           }
@@ -1022,7 +1090,7 @@ extern "C" __global__ void __launch_bounds__(256)
           const int b = (ci + (c & 1)) + XB * ((cj + ((c >> 1) & 1)) + YB * (ck + ((c >> 2) & 1)));
           for (int i = 0; i < BC; ++i) xl[a][i] = xs[b * BC + i];
         }
-        residual_rt(p, xl, cx, cy, cz, t, cm, ca, rl);
+        residual_rt(p, xl, cx, cy, cz, t, cm, ca, rl, prm);
         for (int a = 0; a < NV; ++a)
           for (int i = 0; i < BC; ++i) dst[CORNER[p][a] * BC + i] += rl[a][i];
       }
@@ -1050,6 +1118,36 @@ extern "C" __global__ void __launch_bounds__(256)
     }
   }
 }
+}  // namespace %C%
+
+extern "C" __global__ void __launch_bounds__(256)
+%KERNEL%(const int* __restrict__ tiles, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,
+         const double* __restrict__ x, const double* __restrict__ base, double t, double cm, double ca,
+         double* __restrict__ r, double* __restrict__ val, const int* __restrict__ own_start,
+         const unsigned* __restrict__ own_mask, int n0, int n1, int n2, double ox, double oy, double oz, double hx,
+         double hy, double hz, int mode, const double* __restrict__ prm) {
+  %C%::tile_body(tiles[blockIdx.x], g2l, cmask, x, base, t, cm, ca, r, val, own_start, own_mask, n0, n1, n2, ox, oy,
+                 oz, hx, hy, hz, mode, prm);
+}
+)";
+
+// Batched form: one launch over (member, tile) pairs of many models that share
+// this source; each member's tables, spacing, parameters and vectors come from
+// its descriptor, its time and mass / stiffness weights from coef[4 m .. 4 m + 3]
+// = (t, cm, ca, on).  Members with act[m] == 0 or on == 0 are skipped.
+const char* kTiledBatch = R"(
+extern "C" __global__ void __launch_bounds__(256)
+%KERNEL%_batch(const int2* __restrict__ work, const dcg_asm_member* __restrict__ mem,
+               const double* __restrict__ coef, const int* __restrict__ act, int xsel, int bsel, int rsel, int mode) {
+  const int2 w = work[blockIdx.x];
+  if (act && !act[w.x]) return;
+  const double* cf = coef + 4 * (size_t)w.x;
+  if (cf[3] == 0.0) return;
+  const dcg_asm_member& M = mem[w.x];
+  %C%::tile_body(w.y, M.g2l[%COMP%], M.cmask[%COMP%], M.vec[xsel], bsel >= 0 ? M.vec[bsel] : nullptr, cf[0], cf[1],
+                 cf[2], rsel >= 0 ? M.vec[rsel] : nullptr, (mode & 2) ? M.val : nullptr, M.own_start, M.own_mask,
+                 M.n[0], M.n[1], M.n[2], M.origin[0], M.origin[1], M.origin[2], M.h[0], M.h[1], M.h[2], mode, M.prm);
+}
 )";
 
 }  // namespace
@@ -1061,7 +1159,7 @@ bool tile_eligible(const ModelSpec& m, int comp) {
     for (int k = 0; k < 3; ++k)
       if (sp.velocity[k]) return false;
     for (const auto& dt : sp.diffusion) {
-      if (!dt.jac.empty() || !tensor_const(dt.D)) return false;
+      if (!dt.jac.empty() || !tensor_uniform(dt.D)) return false;
     }
   }
   return true;
@@ -1086,8 +1184,8 @@ std::array<int, 3> choose_tile(const Grid& g, int bc) {
 }
 
 std::string gen_tiled(const ModelSpec& m, const Grid& g, int comp, const std::string& ns, const std::string& kname,
-                      std::array<int, 3> tile) {
-  const auto T = element_tables(g);
+                      std::array<int, 3> tile, bool geo) {
+  const auto T = element_tables(geo ? unit_cell(g) : g);
   std::ostringstream o;
   o << "namespace " << ns << " {\n";
   o << "constexpr int TX = " << tile[0] << ", TY = " << tile[1] << ", TZ = " << tile[2] << ";\n";
@@ -1105,22 +1203,65 @@ std::string gen_tiled(const ModelSpec& m, const Grid& g, int comp, const std::st
       loc.push_back(l);
     }
   o << "__device__ constexpr int LOC[NCLASS][NCORNER] = " << int_init(loc) << ";\n";
+  // unit stiffness tables: sum_q FAC grad_k phi_a grad_l phi_c per class
+  const int d = g.dim;
+  std::vector<double> kdu;
+  for (int p = 0; p < g.nclass; ++p)
+    for (int a = 0; a < T.nv; ++a)
+      for (int c = 0; c < T.nv; ++c)
+        for (int k = 0; k < d; ++k)
+          for (int l = 0; l < d; ++l) {
+            double sum = 0;
+            for (int q = 0; q < T.nq; ++q)
+              sum += T.fac[q] * T.gphi[((static_cast<size_t>(p) * T.nq + q) * T.nv + a) * d + k] *
+                     T.gphi[((static_cast<size_t>(p) * T.nq + q) * T.nv + c) * d + l];
+            kdu.push_back(sum);
+          }
+  o << "__device__ constexpr double KDU[NCLASS][NV][NV][DIM][DIM] = " << array_init(kdu) << ";\n";
+  // all diffusion numeric: the folded per-pair table KD is used instead
+  bool all_const = !geo;
+  for (int s : m.comp_species[comp])
+    for (const auto& dt : m.species[s].diffusion) all_const = all_const && tensor_const(dt.D);
+  o << "constexpr bool CONST_DIFFUSION = " << (all_const ? "true" : "false") << ";\n";
+  // values of the spatially uniform diffusion tensors D_ij
+  auto vars = volume_vars(m, comp, T.evol, geo);
+  o << "__device__ __forceinline__ void uniform_diffusion(double t, const double* __restrict__ prm, "
+       "double (&Dv)[BC][BC][DIM][DIM]) {\n  (void)t; (void)prm;\n";
+  o << "  for (int i = 0; i < BC; ++i) for (int j = 0; j < BC; ++j) for (int k = 0; k < DIM; ++k) "
+       "for (int l = 0; l < DIM; ++l) Dv[i][j][k][l] = 0.0;\n";
+  const auto& species = m.comp_species[comp];
+  for (size_t i = 0; i < species.size(); ++i)
+    for (const auto& dt : m.species[species[i]].diffusion) {
+      const int j = m.species[dt.wrt].local;
+      for (int k = 0; k < d; ++k)
+        for (int l = 0; l < d; ++l) {
+          Ex e = dt.D.tensor ? dt.D.d[k][l] : (k == l ? dt.D.d[0][0] : nullptr);
+          if (!e) continue;
+          o << "  Dv[" << i << "][" << j << "][" << k << "][" << l << "] += " << expr::emit(e, vars) << ";\n";
+        }
+    }
+  if (geo)  // unit-cell table -> cell of spacing h: |cell| / (h_k h_l)
+    o << "  for (int i = 0; i < BC; ++i) for (int j = 0; j < BC; ++j) for (int k = 0; k < DIM; ++k) "
+         "for (int l = 0; l < DIM; ++l) Dv[i][j][k][l] *= prm[6] * prm[3 + k] * prm[3 + l];\n";
+  o << "}\n";
   o << "}  // namespace " << ns << "\n";
   std::string body = kTiled;
+  if (geo && comp < 4) body += kTiledBatch;
   replace_all(body, "%C%", ns);
   replace_all(body, "%KERNEL%", kname);
-  (void)m;
-  (void)comp;
+  replace_all(body, "%COMP%", std::to_string(comp));
   return o.str() + body;
 }
 
 namespace {
 }  // namespace
 
-GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L) {
+GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L, bool runtime_geometry) {
   GeneratedSource out;
+  const bool geo = runtime_geometry;
   std::string pre = kPrelude;
   replace_all(pre, "%B%", std::to_string(L.B));
+  replace_all(pre, "%GEO%", geo ? "1" : "0");
   std::ostringstream src;
   src << pre;
   const bool numerical = spec.jacobian == JacobianType::Numerical && !spec.is_linear;
@@ -1128,21 +1269,24 @@ GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L) {
     if (spec.comp_species[c].empty()) {
       out.volume_kernel.push_back("");
       out.tile_kernel.push_back("");
+      out.tile_batch_kernel.push_back("");
       out.tile_dims.push_back({0, 0, 0});
       continue;
     }
     std::string ns = "vol" + std::to_string(c);
     std::string kn = "dcg_volume_" + std::to_string(c);
-    src << gen_volume(spec, L.grid, static_cast<int>(c), ns, kn, numerical);
+    src << gen_volume(spec, L.grid, static_cast<int>(c), ns, kn, numerical, geo);
     out.volume_kernel.push_back(kn);
     if (tile_eligible(spec, static_cast<int>(c))) {
       auto tile = choose_tile(L.grid, static_cast<int>(spec.comp_species[c].size()));
       std::string tk = "dcg_tile_" + std::to_string(c);
-      src << gen_tiled(spec, L.grid, static_cast<int>(c), ns, tk, tile);
+      src << gen_tiled(spec, L.grid, static_cast<int>(c), ns, tk, tile, geo);
       out.tile_kernel.push_back(tk);
+      out.tile_batch_kernel.push_back(geo && c < 4 ? tk + "_batch" : "");
       out.tile_dims.push_back(tile);
     } else {
       out.tile_kernel.push_back("");
+      out.tile_batch_kernel.push_back("");
       out.tile_dims.push_back({0, 0, 0});
     }
   }

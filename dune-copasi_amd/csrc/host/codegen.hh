@@ -28,10 +28,19 @@ struct GeneratedSource {
   std::vector<std::string> volume_kernel;  // per compartment ("" when it has no species)
   // tiled write-once variant per compartment ("" when the model needs the coloured kernel)
   std::vector<std::string> tile_kernel;
+  // batched tiled kernel per compartment (runtime geometry only, else "")
+  std::vector<std::string> tile_batch_kernel;
   std::vector<std::array<int, 3>> tile_dims;
   std::vector<FacetKernel> facet_kernels;
 };
 
-GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& layout);
+// Words of the run-time parameter array before the model parameters: the
+// cell spacing, its reciprocals and the cell measure (read by kernels generated
+// with runtime_geometry, zero-cost constants otherwise).
+constexpr int kGeoWords = 8;
+
+// runtime_geometry: element tables of the unit cell, spacing from prm[0..7],
+// so models that differ only in mesh size and parameters share one source.
+GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& layout, bool runtime_geometry = false);
 
 }  // namespace dcopasi

@@ -316,7 +316,8 @@ class Reader {
     }
     if (auto it = locals_.find(t.text); it != locals_.end()) return it->second;
     if (t.text == "no_value") return constant(DBL_MAX);
-    if (auto it = ctx_.constants.find(t.text); it != ctx_.constants.end()) return constant(it->second);
+    if (auto it = ctx_.constants.find(t.text); it != ctx_.constants.end())
+      return ctx_.fold_constants ? constant(it->second) : variable("$" + t.text);
     return variable(t.text);
   }
 

@@ -40,6 +40,9 @@ struct Function {
 struct Context {
   std::map<std::string, double> constants;
   std::map<std::string, Function> functions;
+  // false: constants stay symbolic as variables named "$<name>" (model
+  // parameters read at run time, shared code for a batch of models)
+  bool fold_constants = true;
 };
 
 // Parses `text`; variables are left symbolic, context constants fold to

@@ -38,7 +38,8 @@ RkTable rk_table(const std::string& type) {
 }
 
 // ------------------------------------------------------------------ set-up
-Model::Model(const std::string& ini, const std::string& overrides, int device) : device_(device) {
+Model::Model(const std::string& ini, const std::string& overrides, int device, const ModelOptions& opt)
+    : device_(device), opt_(opt) {
   build(ini, overrides);
   if (device_ >= 0) {
     setup_device();
@@ -51,7 +52,7 @@ Model::~Model() {
     for (auto e : p.pool) (void)hipEventDestroy(e);
   if (stream_) {
     (void)hipStreamSynchronize(stream_);
-    (void)hipStreamDestroy(stream_);
+    if (own_stream_) (void)hipStreamDestroy(stream_);
   }
 }
 
@@ -60,7 +61,7 @@ void Model::build(const std::string& ini, const std::string& overrides) {
   const auto& c = cfg_tree_;
   const int dim = c.get("grid.dimension", 2);
   if (c.has_key("grid.path")) throw NotImplemented("grid files (grid.path) are not read by this build");
-  spec_ = read_model(c, dim);
+  spec_ = read_model(c, dim, !opt_.runtime_parameters);
   const std::string cell = c.get("grid.cell_type", std::string("simplex"));
   if (cell != "simplex" && cell != "cube") throw IOError("grid.cell_type must be 'simplex' or 'cube'");
   Grid g = make_grid(dim, c.get_list("grid.cells"), c.get_list("grid.extensions"), c.get_list("grid.origin"),
@@ -73,7 +74,7 @@ void Model::build(const std::string& ini, const std::string& overrides) {
     comps[i].b = static_cast<int>(comps[i].species.size());
   }
   L_ = make_layout(g, elem_comp, std::move(comps), spec_.has_outflow_boundary, spec_.has_outflow_interface);
-  gen_ = generate_assembly(spec_, L_);
+  gen_ = generate_assembly(spec_, L_, opt_.runtime_parameters);
 
   // solver configuration, keys of model/make_step_operator.hh:183-288
   const std::string p = "model.time_step_operator.";
@@ -96,6 +97,10 @@ void Model::build(const std::string& ini, const std::string& overrides) {
     cfg_.warnings.push_back("preconditioner '" + pre + "' runs as block ILU0 on the device");
   } else {
     throw NotImplemented("preconditioner '" + pre + "'");
+  }
+  if (opt_.batch_member && cfg_.precond == Preconditioner::ILU0) {
+    cfg_.precond = Preconditioner::Jacobi;
+    cfg_.warnings.push_back("batch members precondition with block-Jacobi");
   }
   const std::string prec = c.get(p + "linear_solver.precision", std::string("fp64"));
   if (prec == "fp64") cfg_.precision = Precision::FP64;
@@ -136,7 +141,12 @@ void Model::synchronize() const {
 
 void Model::setup_device() {
   DCG_HIP(hipSetDevice(device_));
-  DCG_HIP(hipStreamCreate(&stream_));
+  if (opt_.stream) {
+    stream_ = opt_.stream;
+    own_stream_ = false;
+  } else {
+    DCG_HIP(hipStreamCreate(&stream_));
+  }
   const long ns = L_.nslot;
   const int B = L_.B;
   row_ptr_.upload(L_.row_ptr, stream_);
@@ -356,7 +366,21 @@ void Model::setup_device() {
     up_rows_.upload(rows_u, stream_);
     lu_.resize(static_cast<size_t>(matrix_values()));
   }
-  tune_spmv();
+  {
+    // geometry words (spacing, reciprocals, cell measure) + parameters
+    std::vector<double> prm(kGeoWords, 0.0);
+    double cell = 1.0;
+    for (int k2 = 0; k2 < 3; ++k2) {
+      const bool on = k2 < L_.grid.dim;
+      prm[k2] = on ? L_.grid.h[k2] : 0.0;
+      prm[3 + k2] = on ? 1.0 / L_.grid.h[k2] : 0.0;
+      if (on) cell *= L_.grid.h[k2];
+    }
+    prm[6] = cell;
+    prm.insert(prm.end(), spec_.param_values.begin(), spec_.param_values.end());
+    prm_.upload(prm, stream_);
+  }
+  if (!opt_.batch_member) tune_spmv();
   synchronize();
 }
 
@@ -417,32 +441,33 @@ int Model::tune_spmv_plan(bool fp32) {
 }
 
 void Model::compile_and_load() {
-  auto code = rtc_compile(gen_.source, "dcopasi_assembly");
-  module_.load(code);
+  module_ = rtc_module(gen_.source, "dcopasi_assembly", device_);
   vol_fn_.assign(L_.comps.size(), nullptr);
   tile_fn_.assign(L_.comps.size(), nullptr);
   for (size_t c = 0; c < L_.comps.size(); ++c) {
-    if (!gen_.volume_kernel[c].empty()) vol_fn_[c] = module_.function(gen_.volume_kernel[c]);
-    if (!gen_.tile_kernel[c].empty()) tile_fn_[c] = module_.function(gen_.tile_kernel[c]);
+    if (!gen_.volume_kernel[c].empty()) vol_fn_[c] = module_->function(gen_.volume_kernel[c]);
+    if (!gen_.tile_kernel[c].empty()) tile_fn_[c] = module_->function(gen_.tile_kernel[c]);
   }
   facet_fn_.clear();
-  for (const auto& fk : gen_.facet_kernels) facet_fn_.push_back(module_.function(fk.name));
+  for (const auto& fk : gen_.facet_kernels) facet_fn_.push_back(module_->function(fk.name));
 }
 
 // ------------------------------------------------------------------ assembly
 void Model::launch_assembly(const double* x, double t, double cm, double ca, const double* base, double* r,
-                            double* val, int mode) {
+                            double* val, int mode, bool skip_tiled, int phases) {
   const Grid& g = L_.grid;
   const long n = L_.nslot * L_.B;
   const double* base_eff = base;
   if ((mode & 1) && !all_tiled_) {
     // the coloured kernels accumulate: start from base
-    if (!base) k::fill(r, 0.0, n, stream_);
-    else if (base != r)
-      DCG_HIP(hipMemcpyAsync(r, base, n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    if (phases & 1) {
+      if (!base) k::fill(r, 0.0, n, stream_);
+      else if (base != r)
+        DCG_HIP(hipMemcpyAsync(r, base, n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    }
     base_eff = r;
   }
-  if (mode & 2) {
+  if ((mode & 2) && (phases & 1)) {
     if (!all_tiled_) {
       k::init_jacobian(val, matrix_values(), diag_pos_.get(), slot_b_.get(), static_cast<int>(L_.nslot), L_.B,
                        stream_);
@@ -456,15 +481,17 @@ void Model::launch_assembly(const double* x, double t, double cm, double ca, con
   double hx = g.h[0], hy = g.dim > 1 ? g.h[1] : 0.0, hz = g.dim > 2 ? g.h[2] : 0.0;
   const int* own_start = own_start_.get();
   const unsigned* own_mask = own_mask_.get();
-  for (size_t c = 0; c < L_.comps.size(); ++c) {
+  const double* prm = prm_.get();
+  for (size_t c = 0; c < L_.comps.size() && (phases & 2); ++c) {
     if (!vol_fn_[c]) continue;
     const auto& C = L_.comps[c];
     if (tile_fn_[c]) {
+      if (skip_tiled) continue;
       const int* tiles = comp_dev_[c].tiles.get();
       const int* g2l = comp_dev_[c].g2l.get();
       const unsigned char* cmask = comp_dev_[c].cell_mask_grid.get();
       void* args[] = {&tiles, &g2l, &cmask, &x, &base_eff, &t, &cm, &ca, &r, &val, &own_start, &own_mask,
-                      &n0, &n1, &n2, &ox, &oy, &oz, &hx, &hy, &hz, &mode};
+                      &n0, &n1, &n2, &ox, &oy, &oz, &hx, &hy, &hz, &mode, &prm};
       if (comp_dev_[c].ntiles > 0)
         DCG_HIP(hipModuleLaunchKernel(tile_fn_[c], comp_dev_[c].ntiles, 1, 1, 256, 1, 1, 0, stream_, args, nullptr));
       continue;
@@ -476,17 +503,17 @@ void Model::launch_assembly(const double* x, double t, double cm, double ca, con
       int first = C.color_ptr[col], count = C.color_ptr[col + 1] - first;
       if (count <= 0) continue;
       void* args[] = {&cells, &cmask, &cslots, &first, &count, &x, &t, &cm, &ca, &r, &val, &own_start,
-                      &own_mask, &n0, &n1, &ox, &oy, &oz, &hx, &hy, &hz, &mode};
+                      &own_mask, &n0, &n1, &ox, &oy, &oz, &hx, &hy, &hz, &mode, &prm};
       const unsigned block = 128, grid = static_cast<unsigned>((count + block - 1) / block);
       DCG_HIP(hipModuleLaunchKernel(vol_fn_[c], grid, 1, 1, block, 1, 1, 0, stream_, args, nullptr));
     }
   }
-  if (ca == 0.0) return;  // transmission terms belong to the stiffness operator
+  if (ca == 0.0 || !(phases & 4)) return;  // transmission terms belong to the stiffness operator
   for (size_t k = 0; k < facet_fn_.size(); ++k) {
     const int* fint = facet_dev_[k].fint.get();
     const double* fdbl = facet_dev_[k].fdbl.get();
     for (auto [first, count] : facet_dev_[k].color_ranges) {
-      void* args[] = {&fint, &fdbl, &first, &count, &x, &t, &ca, &r, &val, &mode};
+      void* args[] = {&fint, &fdbl, &first, &count, &x, &t, &ca, &r, &val, &mode, &prm};
       const unsigned block = 64, grid = static_cast<unsigned>((count + block - 1) / block);
       DCG_HIP(hipModuleLaunchKernel(facet_fn_[k], grid, 1, 1, block, 1, 1, 0, stream_, args, nullptr));
     }
@@ -972,6 +999,76 @@ int Model::linear_solve(const double* jac, const double* b_packed, double* x_pac
   DCG_HIP(hipMemcpyAsync(x_packed, packed_.get(), L_.ndof * sizeof(double), hipMemcpyDeviceToHost, stream_));
   synchronize();
   return rc;
+}
+
+// ------------------------------------------------------------------ parameters, footprint, batch view
+void Model::set_params(const double* values, size_t n) {
+  require_device();
+  if (!opt_.runtime_parameters) throw Error("model was created with folded constants: no run-time parameters");
+  if (n != spec_.params.size())
+    throw Error("expected " + std::to_string(spec_.params.size()) + " parameters, got " + std::to_string(n));
+  spec_.param_values.assign(values, values + n);
+  if (n) DCG_HIP(hipMemcpyAsync(prm_.get() + kGeoWords, values, n * sizeof(double), hipMemcpyHostToDevice, stream_));
+  synchronize();
+}
+
+void Model::get_params(double* values, size_t n) const {
+  if (n != spec_.param_values.size())
+    throw Error("expected room for " + std::to_string(spec_.param_values.size()) + " parameters");
+  std::copy(spec_.param_values.begin(), spec_.param_values.end(), values);
+}
+
+size_t Model::device_bytes() const {
+  size_t b = row_ptr_.bytes() + col_.bytes() + own_start_.bytes() + diag_pos_.bytes() + slot_b_.bytes() +
+             chunk_row_.bytes() + own_mask_.bytes() + slot_dof_.bytes() + extra_pos_.bytes() + weights_.bytes() +
+             prm_.bytes() + u_.bytes() + hist_.bytes() + r_.bytes() + z_.bytes() + xs_.bytes() + packed_.bytes() +
+             val_.bytes() + lu_.bytes() + dinv_.bytes() + val32_.bytes() + part_.bytes() + scalar_out_.bytes() +
+             sc_.bytes() + status_.bytes() + lo_rows_.bytes() + up_rows_.bytes();
+  for (const auto& v : stage_) b += v.bytes();
+  for (const auto& v : kv_) b += v.bytes();
+  for (const auto& v : kf_) b += v.bytes();
+  for (const auto& c : comp_dev_)
+    b += c.cells.bytes() + c.cell_slots.bytes() + c.cell_mask.bytes() + c.g2l.bytes() + c.tiles.bytes() +
+         c.cell_mask_grid.bytes();
+  for (const auto& f : facet_dev_) b += f.fint.bytes() + f.fdbl.bytes();
+  return b;
+}
+
+// Device bytes of a model with layout L, before anything is allocated: the
+// matrix (FP64, its FP32 copy when mixed, the ILU factor when used), the
+// block-diagonal inverse, the per-slot vectors and the mesh tables.
+size_t Model::estimate_device_bytes(const Layout& L, int stages, bool mixed, bool ilu) {
+  const double nval = static_cast<double>(L.nnzb()) * L.B * L.B;
+  const double ns = static_cast<double>(L.nslot), nv = ns * L.B;
+  double b = 8.0 * nval * (ilu ? 2.0 : 1.0) + (mixed ? 4.0 * nval : 0.0) + 8.0 * ns * L.B * L.B;
+  b += 8.0 * nv * (5 + stages + 1 + 9) + (mixed ? 4.0 * nv * 10 : 0.0) + 8.0 * L.ndof;
+  b += 4.0 * (ns + 1) + 4.0 * L.nnzb() + 4.0 * ns * 4 + 8.0 * ns * 2;
+  double cells = 0;
+  for (const auto& C : L.comps) cells += static_cast<double>(C.cells.size());
+  b += cells * (4.0 + 1.0 + 4.0 * L.grid.ncorner) + (4.0 + 1.0) * static_cast<double>(L.grid.nvert) * L.comps.size();
+  return static_cast<size_t>(b);
+}
+
+k::MemberView Model::view() const {
+  k::MemberView v;
+  v.nslot = static_cast<int>(L_.nslot);
+  v.nchunk = plan_.nchunk;
+  v.rp = row_ptr_.get();
+  v.col = col_.get();
+  v.diag_pos = diag_pos_.get();
+  v.chunk_row = chunk_row_.get();
+  v.val = val_.get();
+  v.val32 = val32_.get();
+  v.dinv = dinv_.get();
+  v.v[k::kVU] = u_.get();
+  v.v[k::kVHist] = hist_.get();
+  v.v[k::kVR] = r_.get();
+  v.v[k::kVZ] = z_.get();
+  v.v[k::kVXs] = xs_.get();
+  for (size_t j = 0; j < stage_.size() && j < 3; ++j) v.v[k::kVStage + j] = stage_[j].get();
+  for (int i = 0; i < 9; ++i) v.v[k::kVKry + i] = kv_[i].get();
+  for (int i = 0; i < k::kBatchVecF; ++i) v.f[i] = kf_[i].get();
+  return v;
 }
 
 }  // namespace dcopasi

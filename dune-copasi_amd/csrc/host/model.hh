@@ -57,9 +57,23 @@ struct SolverConfig {
   std::vector<std::string> warnings;
 };
 
+// Creation options beyond the INI keys.
+struct ModelOptions {
+  // parser_context constants and the mesh spacing are read at run time, so
+  // models that differ only in them share one generated module (batches,
+  // parameter sweeps); set_params changes the constants without recompiling
+  bool runtime_parameters = false;
+  // member of a Batch: the batch tunes the SpMV and preconditions with
+  // block-Jacobi; `stream` (owned by the batch) replaces the model's own
+  bool batch_member = false;
+  hipStream_t stream = nullptr;
+};
+
+class Batch;
+
 class Model {
  public:
-  Model(const std::string& ini, const std::string& overrides, int device);
+  Model(const std::string& ini, const std::string& overrides, int device, const ModelOptions& opt = {});
   ~Model();
   Model(const Model&) = delete;
   Model& operator=(const Model&) = delete;
@@ -82,6 +96,14 @@ class Model {
   void initial_state(double* packed) const;  // from initial.expression keys
   int step(double t, double dt, StepStats* st);
   void mass(double* per_species);
+  // run-time parameters (parser_context constants by name); only models
+  // created with runtime_parameters have them
+  const std::vector<std::string>& param_names() const { return spec_.params; }
+  void set_params(const double* values, size_t n);
+  void get_params(double* values, size_t n) const;
+  // bytes of device memory the model holds (after set-up) or would hold
+  size_t device_bytes() const;
+  static size_t estimate_device_bytes(const Layout& L, int stages, bool mixed, bool ilu);
 
   // kernel-level entry points (parity tests)
   void assemble(const double* x_packed, double t, double cm, double ca, double* r_packed, double* jac);
@@ -104,15 +126,24 @@ class Model {
   hipStream_t stream() const { return stream_; }
   void synchronize() const;
   long matrix_values() const { return static_cast<long>(L_.nnzb()) * L_.B * L_.B; }
+  const RkTable& rk() const { return rk_; }
+  void set_time(double t) { time_ = t; }
+  // device buffers of the model for batched kernels
+  k::MemberView view() const;
 
  private:
+  friend class Batch;
   void build(const std::string& ini, const std::string& overrides);
   void setup_device();
   void require_device() const;
   // assembly: r = base + cm*R^M(x) + ca*R^A(x) (base may be null or r itself)
   // and/or val = d/dx of it, complete (padding and interface blocks included)
+  // skip_tiled: leave the tiled compartments to a batched launch; phases:
+  // 1 prepare r / val, 2 volume kernels, 4 facet kernels
   void launch_assembly(const double* x, double t, double cm, double ca, const double* base, double* r, double* val,
-                       int mode);
+                       int mode, bool skip_tiled = false, int phases = 7);
+  // compartments without a batched tile kernel, or facets: per-model launches
+  bool needs_own_assembly() const;
   double residual_norm(const double* r);
   // linear solves on the current val_
   int krylov(const double* b, double* x, double rtol, int maxit, int* its, int* refinements);
@@ -135,8 +166,10 @@ class Model {
   SolverConfig cfg_;
   RkTable rk_;
   int device_ = -1;
+  ModelOptions opt_;
   hipStream_t stream_ = nullptr;
-  RtcModule module_;
+  bool own_stream_ = true;
+  std::shared_ptr<RtcModule> module_;
   std::vector<hipFunction_t> vol_fn_;
   std::vector<hipFunction_t> tile_fn_;
   std::vector<hipFunction_t> facet_fn_;
@@ -165,6 +198,7 @@ class Model {
   };
   std::vector<FacetDev> facet_dev_;
   DeviceArray<double> weights_;  // lumped mass per slot (mass sums)
+  DeviceArray<double> prm_;      // geometry words + run-time parameters
   DeviceArray<double> u_, hist_, r_, z_, xs_, packed_;
   std::vector<DeviceArray<double>> stage_;
   DeviceArray<double> val_, lu_, dinv_;

@@ -7,6 +7,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <map>
+#include <mutex>
 #include <sstream>
 
 #include "error.hh"
@@ -94,6 +96,19 @@ RtcModule::~RtcModule() {
 
 void RtcModule::load(const std::vector<char>& code) {
   DCG_HIP(hipModuleLoadData(&mod_, code.data()));
+}
+
+std::shared_ptr<RtcModule> rtc_module(const std::string& source, const std::string& name, int device) {
+  static std::mutex mu;
+  static std::map<std::pair<int, std::string>, std::weak_ptr<RtcModule>> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  auto key = std::make_pair(device, source);
+  if (auto it = cache.find(key); it != cache.end())
+    if (auto m = it->second.lock()) return m;
+  auto m = std::make_shared<RtcModule>();
+  m->load(rtc_compile(source, name));
+  cache[key] = m;
+  return m;
 }
 
 hipFunction_t RtcModule::function(const std::string& name) const {

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -24,5 +25,9 @@ class RtcModule {
  private:
   hipModule_t mod_ = nullptr;
 };
+
+// Process-wide cache of loaded modules: models with identical generated
+// source on one device share one module.
+std::shared_ptr<RtcModule> rtc_module(const std::string& source, const std::string& name, int device);
 
 }  // namespace dcopasi

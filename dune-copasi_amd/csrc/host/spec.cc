@@ -81,16 +81,24 @@ static std::vector<Partial> symbolic(const Ex& term, const ModelSpec& m, const s
   return out;
 }
 
-ModelSpec read_model(const ParameterTree& cfg, int dim) {
+ModelSpec read_model(const ParameterTree& cfg, int dim, bool fold_constants) {
   ModelSpec m;
   m.dim = dim;
   m.ctx = read_context(cfg);
+  // compartment and initial expressions are evaluated on the host: always folded
+  const expr::Context folded = m.ctx;
+  m.ctx.fold_constants = fold_constants;
+  if (!fold_constants)
+    for (const auto& [name, value] : m.ctx.constants) {
+      m.params.push_back(name);
+      m.param_values.push_back(value);
+    }
   // compartments in INI order (make_multi_domain_grid.hh:128-160)
   for (const auto& c : cfg.sub_keys("compartments")) {
     const std::string type = cfg.get("compartments." + c + ".type", std::string("expression"));
     if (type != "expression") throw NotImplemented("Not know type '" + type + "' of compartment");
     m.comp_names.push_back(c);
-    m.comp_expr.push_back(expression(cfg, "compartments." + c + ".expression", m.ctx));
+    m.comp_expr.push_back(expression(cfg, "compartments." + c + ".expression", folded));
   }
   m.comp_species.resize(m.comp_names.size());
   for (const auto& s : cfg.sub_keys("model.scalar_field")) {
@@ -183,7 +191,7 @@ ModelSpec read_model(const ParameterTree& cfg, int dim) {
       else m.has_outflow_interface = true;
       sp.outflow.push_back(o);
     }
-    m.initial.push_back(expression(cfg, p + ".initial.expression", m.ctx));
+    m.initial.push_back(expression(cfg, p + ".initial.expression", folded));
   }
   return m;
 }

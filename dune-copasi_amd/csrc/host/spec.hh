@@ -70,8 +70,14 @@ struct ModelSpec {
   double fd_epsilon = 1e-7;
   bool is_linear = false;
   bool has_outflow_boundary = false, has_outflow_interface = false;
+  // run-time parameters (parser_context constants when not folded): the
+  // expressions reference "$<name>", constants sorted by name
+  std::vector<std::string> params;
+  std::vector<double> param_values;
 };
 
-ModelSpec read_model(const ParameterTree& cfg, int dim);
+// fold_constants = false keeps parser_context constants as run-time
+// parameters so models that differ only in them share generated code.
+ModelSpec read_model(const ParameterTree& cfg, int dim, bool fold_constants = true);
 
 }  // namespace dcopasi

@@ -30,6 +30,113 @@ struct BicgScalars {
   int iter, maxit, done, half;
 };
 
+// ----------------------------------------------------------------- batches
+// A batch advances many independent models with one launch per operation:
+// every batched kernel walks a work list of (member, block) pairs and finds
+// the member's buffers in a device-resident MemberView array, so members keep
+// their own allocations and join or leave without moving data.
+
+// FP64 vectors of a member, by selector
+enum BatchVec : int {
+  kVU = 0,      // state u_n
+  kVHist = 1,   // history of earlier stages
+  kVR = 2,      // Newton residual
+  kVZ = 3,      // Newton correction
+  kVXs = 4,     // stage iterate
+  kVStage = 5,  // stage values 0..2 (5, 6, 7)
+  kVKry = 8,    // FP64 Krylov vectors r rh p v ph s sh t tmp (8..16)
+  kBatchVec = 17
+};
+constexpr int kBatchVecF = 10;  // FP32 Krylov vectors r rh p v ph s sh t tmp x
+
+struct MemberView {
+  int nslot = 0, nchunk = 0;
+  const int* rp = nullptr;
+  const int* col = nullptr;
+  const int* diag_pos = nullptr;
+  const int* chunk_row = nullptr;
+  double* val = nullptr;
+  float* val32 = nullptr;
+  double* dinv = nullptr;
+  double* v[kBatchVec] = {};
+  float* f[kBatchVecF] = {};
+  BicgScalars* sc = nullptr;
+};
+
+// per-member state of FP64 iterative refinement around the FP32 Krylov solve
+struct IrState {
+  double target, rn;
+  int done, status, its, refs;
+};
+
+// skip member m when (p[m * stride] != 0) == when; p == nullptr: never
+struct Skip {
+  const int* p = nullptr;
+  int stride = 0;
+  int when = 0;
+};
+
+// work lists: (member, 256-slot block) for vectors, (member, chunk) for SpMV;
+// range[m] = [first, last) work items of member m
+struct BatchWork {
+  const int2* vwork = nullptr;
+  int nvwork = 0;
+  const int2* vrange = nullptr;
+  const int2* cwork = nullptr;
+  int ncwork = 0;
+  const int2* crange = nullptr;
+  int members = 0;
+};
+
+// Host twin of the generated dcg_asm_member (same field order and types):
+// one batch member as the batched tile kernels see it.
+constexpr int kAsmComps = 4;
+struct AsmMember {
+  double* vec[8] = {};  // BatchVec selectors kVU .. kVStage + 2
+  double* val = nullptr;
+  const int* own_start = nullptr;
+  const unsigned* own_mask = nullptr;
+  const double* prm = nullptr;
+  const int* g2l[kAsmComps] = {};
+  const unsigned char* cmask[kAsmComps] = {};
+  int n[4] = {1, 1, 1, 0};
+  double origin[3] = {0, 0, 0};
+  double h[3] = {0, 0, 0};
+};
+
+void b_copy(int B, const BatchWork& w, const MemberView* mv, Skip sk, int dst, int src, hipStream_t s);
+void b_axpy(int B, const BatchWork& w, const MemberView* mv, Skip sk, double a, int x, int y, hipStream_t s);
+// per-member reduction of vector x (op 0: x.x, 1: sum |x|, 2: max |x|) into out[m]
+void b_norm(int B, const BatchWork& w, const MemberView* mv, Skip sk, int x, int op, double* part, double* out,
+            hipStream_t s);
+void b_block_inverse(int B, const BatchWork& w, const MemberView* mv, Skip sk, int* status, hipStream_t s);
+void b_to_float(int B, const BatchWork& w, const MemberView* mv, Skip sk, hipStream_t s);
+// Krylov set-up: tolerances from tol[m] (or, with ir, the refinement's inner
+// tolerance max(inner, target/(2 rn)) capped at 0.5); members skipped by sk or
+// finished refinements start done
+void b_bicg_setup(const BatchWork& w, Skip sk, const double* tol, const IrState* ir, double inner, int maxit,
+                  BicgScalars* sc, hipStream_t s);
+// b: FP64 selector; x: FP64 selector (FP64 Krylov) or the FP32 x vector
+template <typename V>
+void b_bicg_init(int B, const BatchWork& w, const MemberView* mv, int b, int x, double* part, BicgScalars* sc,
+                 hipStream_t st);
+// one BiCGSTAB iteration on every unfinished member (jacobi: apply dinv)
+template <typename V>
+void b_bicg_iteration(int B, int variant, const BatchWork& w, const MemberView* mv, int x, bool jacobi,
+                      double* part, BicgScalars* sc, hipStream_t st);
+// y = A x on the FP64 matrix (selectors), members skipped by sk
+void b_spmv64(int B, int variant, const BatchWork& w, const MemberView* mv, Skip sk, int x, int y, hipStream_t st);
+// y = A x on the FP32 matrix copy, FP32 Krylov vectors (selectors kVKry + i)
+void b_spmv32(int B, int variant, const BatchWork& w, const MemberView* mv, Skip sk, int x, int y, hipStream_t st);
+// zero the FP32 Krylov vector at selector sel (kVKry + i)
+void b_copy_f(int B, const BatchWork& w, const MemberView* mv, Skip sk, int sel, hipStream_t st);
+// refinement: ir from ||b||^2 in sums; x += d; res = b - A x; residual check
+void b_ir_begin(const BatchWork& w, Skip sk, const double* rtol, const double* sums, IrState* ir, hipStream_t st);
+void b_ir_update(int B, const BatchWork& w, const MemberView* mv, const IrState* ir, int x, hipStream_t st);
+void b_ir_residual(int B, const BatchWork& w, const MemberView* mv, const IrState* ir, int b, int ax, int res,
+                   double* part, double* sums, hipStream_t st);
+void b_ir_check(const BatchWork& w, const double* sums, const BicgScalars* sc, IrState* ir, hipStream_t st);
+
 void fill(double* p, double v, long n, hipStream_t s);
 void fill(float* p, float v, long n, hipStream_t s);
 void fill(int* p, int v, long n, hipStream_t s);

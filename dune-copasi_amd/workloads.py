@@ -86,32 +86,43 @@ def _solver_block(rk, precond, nl_rtol, nl_atol, lin_rtol, precision="fp64", max
 
 # --------------------------------------------------------------------- Gray-Scott
 def gray_scott(n=128, cell_type="cube", seed=0, F=0.04, k=0.06, Du=2e-5, Dv=1e-5, precond="ILU",
-               nl_rtol=1e-10, nl_atol=0.0, lin_rtol=1e-10, precision="fp64", steps=200, dt=1.0) -> Workload:
-    """Gray-Scott on [0,1]^2, zero flux, implicit Euler (benchmark config 0 at n = 128)."""
+               nl_rtol=1e-10, nl_atol=0.0, lin_rtol=1e-10, precision="fp64", steps=200, dt=1.0,
+               constants=False) -> Workload:
+    """Gray-Scott on [0,1]^2, zero flux, implicit Euler (benchmark config 0 at n = 128).
+
+    constants=True writes F, k, Du, Dv as parser_context constants (the form a
+    parameter sweep uses); the model is the same."""
+    if constants:
+        ctx = (f"[parser_context]\nF.type = constant\nF.value = {F!r}\nk.type = constant\nk.value = {k!r}\n"
+               f"Du.type = constant\nDu.value = {Du!r}\nDv.type = constant\nDv.value = {Dv!r}\n\n")
+        sF, sk, sDu, sDv = "F", "k", "Du", "Dv"
+    else:
+        ctx = ""
+        sF, sk, sDu, sDv = f"{F:g}", f"{k:g}", f"{Du:g}", f"{Dv:g}"
     ini = f"""[grid]
 dimension = 2
 cells = {n} {n}
 extensions = 1 1
 cell_type = {cell_type}
 
-[compartments]
+{ctx}[compartments]
 domain.expression = 1
 
 [model.scalar_field.u]
 compartment = domain
 storage.expression = 1
-cross_diffusion.u.expression = {Du:g}
-reaction.expression = -u*v^2 + {F:g}*(1-u)
-reaction.jacobian.u.expression = -v^2 - {F:g}
+cross_diffusion.u.expression = {sDu}
+reaction.expression = -u*v^2 + {sF}*(1-u)
+reaction.jacobian.u.expression = -v^2 - {sF}
 reaction.jacobian.v.expression = -2*u*v
 
 [model.scalar_field.v]
 compartment = domain
 storage.expression = 1
-cross_diffusion.v.expression = {Dv:g}
-reaction.expression = u*v^2 - ({F:g}+{k:g})*v
+cross_diffusion.v.expression = {sDv}
+reaction.expression = u*v^2 - ({sF}+{sk})*v
 reaction.jacobian.u.expression = v^2
-reaction.jacobian.v.expression = 2*u*v - ({F:g}+{k:g})
+reaction.jacobian.v.expression = 2*u*v - ({sF}+{sk})
 
 """ + _solver_block("ImplicitEuler", precond, nl_rtol, nl_atol, lin_rtol, precision)
 
@@ -127,6 +138,26 @@ reaction.jacobian.v.expression = 2*u*v - ({F:g}+{k:g})
         return out
 
     return Workload(f"gray_scott_{cell_type}_{n}", ini, dt, steps, 2, (n, n), init)
+
+
+def gray_scott_batch(count=256, seed=0, side_min=32, side_max=512, precision="mixed", precond="Jacobi",
+                     nl_rtol=1e-10, lin_rtol=1e-10, steps=100, dt=1.0, cell_type="cube"):
+    """Independent Gray-Scott instances (benchmark config 3 at count = 256):
+    sides log-uniform in [side_min, side_max], F ~ U[0.01, 0.08],
+    k ~ U[0.04, 0.07], D_u / D_v ~ U[1.5, 3] with D_v = 1e-5, initial state as
+    in gray_scott seeded per instance, implicit Euler."""
+    ids = np.arange(count)
+    side = np.exp(np.log(side_min) + (np.log(side_max) - np.log(side_min)) * uniform01(seed, 200, ids))
+    side = np.clip(np.rint(side).astype(int), side_min, side_max)
+    F = 0.01 + 0.07 * uniform01(seed, 201, ids)
+    k = 0.04 + 0.03 * uniform01(seed, 202, ids)
+    ratio = 1.5 + 1.5 * uniform01(seed, 203, ids)
+    out = []
+    for i in range(count):
+        out.append(gray_scott(n=int(side[i]), cell_type=cell_type, seed=seed * 100003 + i, F=float(F[i]),
+                              k=float(k[i]), Du=float(ratio[i]) * 1e-5, Dv=1e-5, precond=precond, nl_rtol=nl_rtol,
+                              lin_rtol=lin_rtol, precision=precision, steps=steps, dt=dt, constants=True))
+    return out
 
 
 # ------------------------------------------------------------- Mitchell-Schaefer

@@ -126,17 +126,67 @@ int dcg_linear_solve(dcg_model_t model, const double* jacobian, const double* b,
 int dcg_profile_enable(dcg_model_t model, int on);
 int dcg_profile_read(dcg_model_t model, int kind, long* launches, double* milliseconds, double* bytes);
 
-/* SpMV variants the set-up autotuner timed on this model's pattern: for each
- * candidate k < *count, variant[k] (0 LDS-staged entries, 5 LDS-staged whole
- * blocks, 1..4 row-split with 8/16/32/64 lanes per row; +100 for the FP32
- * matrix of the mixed-precision solve) and microseconds[k]; *chosen is the
- * FP64 variant used.  Arrays hold at least 16 entries. */
 /* Copy `bytes` on the model's device with a plain streaming kernel
  * (probe_copy_k) and return its duration: the reference point that HBM
  * traffic counters and bandwidth figures are calibrated against. */
 int dcg_probe_copy(dcg_model_t model, size_t bytes, double* milliseconds);
 
+/* SpMV variants the set-up autotuner timed on this model's pattern: for each
+ * candidate k < *count, variant[k] (0 LDS-staged entries, 5 LDS-staged whole
+ * blocks, 1..4 row-split with 8/16/32/64 lanes per row; +100 for the FP32
+ * matrix of the mixed-precision solve) and microseconds[k]; *chosen is the
+ * FP64 variant used.  Arrays hold at least 16 entries. */
 int dcg_spmv_tuning(dcg_model_t model, int* count, int* variant, double* microseconds, int* chosen);
+
+/* ---------------------------------------------------------------- run-time parameters
+ * The reference re-parses `parser_context` constants whenever it builds a
+ * model (dune/copasi/parser/context.cc:56-173); parameter sweeps and fits
+ * rebuild it per value.  With DCG_RUNTIME_PARAMETERS the constants and the
+ * mesh spacing are read by the generated kernels at run time instead of being
+ * folded in, so one compiled module serves every parameter value and mesh
+ * size, and dcg_set_params changes values in place. */
+#define DCG_RUNTIME_PARAMETERS 1u
+int dcg_model_create_ex(const char* ini_text, const char* overrides, int device, unsigned flags, dcg_model_t* out);
+/* count = number of parameters; name of parameter i (sorted by name) when i < count */
+int dcg_param_info(dcg_model_t model, size_t i, char* name, size_t name_len, size_t* count);
+int dcg_set_params(dcg_model_t model, const double* values, size_t n);
+int dcg_get_params(dcg_model_t model, double* values, size_t n);
+
+/* Device memory held by a model, and the estimate for an INI before anything
+ * is allocated (admission by HBM footprint). */
+int dcg_device_bytes(dcg_model_t model, size_t* bytes);
+int dcg_estimate_device_bytes(const char* ini_text, const char* overrides, size_t* bytes);
+/* hipMemGetInfo of a device */
+int dcg_device_memory(int device, size_t* free_bytes, size_t* total_bytes);
+
+/* ---------------------------------------------------------------- batches
+ * Many independent models advanced together: every member is built with
+ * run-time parameters on the batch's device and stream and keeps its own HBM
+ * buffers; dcg_batch_step advances every member by one step of its own
+ * scheme and step size from its own time, with one launch per device
+ * operation for all members (batched assembly per shared module, batched
+ * BiCGSTAB with per-member scalars, block-Jacobi, per-member FP64 refinement
+ * of the FP32 solve, hipGraph-captured iteration blocks).  The reference
+ * has no counterpart: it advances one model per process
+ * (dune/copasi/common/stepper.hh:95-175).
+ * options_ini keys: precision = mixed|fp64, preconditioner = Jacobi|None,
+ * refinement.inner_relative_tolerance, refinement.max_steps,
+ * iteration_limit, graph_iterations (0: no graphs). */
+typedef struct dcg_batch* dcg_batch_t;
+int dcg_batch_create(const char* options_ini, int device, dcg_batch_t* out);
+void dcg_batch_destroy(dcg_batch_t batch);
+/* add a member; *id identifies it until removed */
+int dcg_batch_add(dcg_batch_t batch, const char* ini_text, const char* overrides, int* id);
+int dcg_batch_remove(dcg_batch_t batch, int id);
+/* member ids in batch order (the order of dcg_batch_step's stats) */
+int dcg_batch_ids(dcg_batch_t batch, int* ids, size_t capacity, size_t* count);
+/* a model handle onto a member (state, parameters, mass, sizes); release it
+ * with dcg_model_destroy, which leaves the member in the batch */
+int dcg_batch_member(dcg_batch_t batch, int id, dcg_model_t* out);
+int dcg_batch_set_time_step(dcg_batch_t batch, int id, double dt);
+/* one step of every member; stats[k] for the k-th id (capacity >= members) */
+int dcg_batch_step(dcg_batch_t batch, dcg_step_stats* stats, size_t capacity);
+int dcg_batch_device_bytes(dcg_batch_t batch, size_t* bytes);
 
 #ifdef __cplusplus
 }

@@ -170,3 +170,41 @@ def test_generated_kernels_compile_for_gfx950(dca, wl, make):
     assert "dcg_volume_0" in src
     rc, log = _hiprtc_compile(src)
     assert rc == 0, log
+
+
+def test_runtime_parameter_source_is_shared(dca, wl):
+    """With run-time parameters, models that differ only in mesh size and
+    parameter values generate the same source (one hipRTC module for a whole
+    batch), and that source holds the batched tile kernel and compiles."""
+    ws = wl.gray_scott_batch(count=4, seed=1, side_min=8, side_max=64)
+    srcs = []
+    for w in ws:
+        m = dca.Model(w.ini, "", -1, runtime_parameters=True)
+        assert m.param_names() == ["Du", "Dv", "F", "k"]
+        p = m.params()
+        assert 0.01 <= p[2] <= 0.08 and 0.04 <= p[3] <= 0.07 and 1.5 <= p[0] / p[1] <= 3.0
+        srcs.append(m.generated_source())
+    assert all(s == srcs[0] for s in srcs)
+    assert "dcg_tile_0_batch" in srcs[0] and "#define DCG_GEO 1" in srcs[0]
+    rc, log = _hiprtc_compile(srcs[0])
+    assert rc == 0, log
+    # folded models embed their constants and spacing instead
+    folded = dca.Model(ws[0].ini, "", -1).generated_source()
+    assert "#define DCG_GEO 0" in folded and "_batch" not in folded
+
+
+def test_gray_scott_batch_workload(wl):
+    ws = wl.gray_scott_batch(count=256)
+    sides = np.array([w.cells[0] for w in ws])
+    assert sides.min() >= 32 and sides.max() <= 512
+    # log-uniform: about half the instances below the geometric mean 128
+    assert 0.35 < np.mean(sides < 128) < 0.65
+    assert len({w.ini for w in ws}) == 256
+
+
+def test_footprint_estimate_scales_with_mesh(dca, wl):
+    small = dca.estimate_device_bytes(wl.gray_scott(n=32, precision="mixed").ini)
+    big = dca.estimate_device_bytes(wl.gray_scott(n=128, precision="mixed").ini)
+    assert small > 0 and 12 < big / small < 20
+    fp64 = dca.estimate_device_bytes(wl.gray_scott(n=128, precision="fp64").ini)
+    assert fp64 < big
