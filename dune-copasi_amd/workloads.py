@@ -66,7 +66,7 @@ def vertex_coords(ids: np.ndarray, cells, ext=None, origin=None) -> np.ndarray:
     return out
 
 
-def _solver_block(rk, precond, nl_rtol, nl_atol, lin_rtol, precision="fp64", maxit=2000):
+def _solver_block(rk, precond, nl_rtol, nl_atol, lin_rtol, precision="fp64", maxit=2000, dt=None):
     pre = ("linear_solver.preconditioner.type = BlockJacobi\n"
            "linear_solver.preconditioner.diagonal_solver.type = Loop\n"
            "linear_solver.preconditioner.diagonal_solver.preconditioner.type = ILU\n"
@@ -74,6 +74,7 @@ def _solver_block(rk, precond, nl_rtol, nl_atol, lin_rtol, precision="fp64", max
            if precond == "ILU" else f"linear_solver.preconditioner.type = {precond}\n")
     return (f"[model.time_step_operator]\n"
             f"type = {rk}\n"
+            + (f"time_step_initial = {dt!r}\n" if dt is not None else "") +
             f"linear_solver.type = BiCGSTAB\n"
             f"{pre}"
             f"linear_solver.convergence_condition.relative_tolerance = {lin_rtol:g}\n"
@@ -124,7 +125,7 @@ reaction.expression = u*v^2 - ({sF}+{sk})*v
 reaction.jacobian.u.expression = v^2
 reaction.jacobian.v.expression = 2*u*v - ({sF}+{sk})
 
-""" + _solver_block("ImplicitEuler", precond, nl_rtol, nl_atol, lin_rtol, precision)
+""" + _solver_block("ImplicitEuler", precond, nl_rtol, nl_atol, lin_rtol, precision, dt=dt)
 
     def init(model):
         ids = model.compartment_nodes(0)
@@ -141,7 +142,7 @@ reaction.jacobian.v.expression = 2*u*v - ({sF}+{sk})
 
 
 def gray_scott_batch(count=256, seed=0, side_min=32, side_max=512, precision="mixed", precond="Jacobi",
-                     nl_rtol=1e-10, lin_rtol=1e-10, steps=100, dt=1.0, cell_type="cube"):
+                     nl_rtol=1e-10, nl_atol=0.0, lin_rtol=1e-10, steps=100, dt=1.0, cell_type="cube"):
     """Independent Gray-Scott instances (benchmark config 3 at count = 256):
     sides log-uniform in [side_min, side_max], F ~ U[0.01, 0.08],
     k ~ U[0.04, 0.07], D_u / D_v ~ U[1.5, 3] with D_v = 1e-5, initial state as
@@ -156,7 +157,7 @@ def gray_scott_batch(count=256, seed=0, side_min=32, side_max=512, precision="mi
     for i in range(count):
         out.append(gray_scott(n=int(side[i]), cell_type=cell_type, seed=seed * 100003 + i, F=float(F[i]),
                               k=float(k[i]), Du=float(ratio[i]) * 1e-5, Dv=1e-5, precond=precond, nl_rtol=nl_rtol,
-                              lin_rtol=lin_rtol, precision=precision, steps=steps, dt=dt, constants=True))
+                              nl_atol=nl_atol, lin_rtol=lin_rtol, precision=precision, steps=steps, dt=dt, constants=True))
     return out
 
 
@@ -200,7 +201,7 @@ storage.expression = 1
 reaction.expression = (u < u_gate) ? (1-h)/tau_open : -h/tau_close
 reaction.jacobian.h.expression = (u < u_gate) ? -1/tau_open : -1/tau_close
 
-""" + _solver_block("Alexander2", precond, nl_rtol, nl_atol, lin_rtol, precision)
+""" + _solver_block("Alexander2", precond, nl_rtol, nl_atol, lin_rtol, precision, dt=dt)
 
     rng = np.random.Generator(np.random.PCG64(seed))
     centre = rng.uniform(0.1, 0.9, size=3)
@@ -263,7 +264,7 @@ outer.expression = (position_x-0.5)^2 + (position_y-0.5)^2 >= 0.09
                     f"outflow.inner.jacobian.D_in.expression = {-kt:g}",
                     f"outflow.inner.jacobian.A_out.expression = {kt:g}"]
         parts.append("\n".join(sec) + "\n")
-    parts.append("\n" + _solver_block("ImplicitEuler", precond, nl_rtol, nl_atol, lin_rtol, precision))
+    parts.append("\n" + _solver_block("ImplicitEuler", precond, nl_rtol, nl_atol, lin_rtol, precision, dt=dt))
     ini = "".join(parts)
 
     def init(model):
