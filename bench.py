@@ -99,6 +99,158 @@ def cpu_baseline(wl_obj, steps: int):
                       f"({dof_steps} Newton DOF-steps, {el:.1f} s, OpenMP-coloured assembly, sequential BiCGSTAB)"}
 
 
+def cpu_baseline_batch(ws, dof_steps_budget: float):
+    """Oracle on the host cores: one time step of as many members (in batch
+    order) as fit the DOF-step budget."""
+    sys.path.insert(0, str(ROOT))
+    from tests import oracle_lib  # the oracle is only ever the baseline here
+
+    L = oracle_lib.load()
+    L.orc_set_threads(os.cpu_count() or 1)
+    done, el, used = 0, 0.0, 0
+    for w in ws:
+        o = oracle_lib.OracleModel(w.ini)
+        o.set_state(w.initial_state(o), 0.0)
+        t0 = time.perf_counter()
+        rc, _, _ = o.step(w.dt)
+        el += time.perf_counter() - t0
+        if rc:
+            raise RuntimeError(f"oracle step failed: {rc}")
+        done += o.n_dof * w.stages
+        used += 1
+        if done >= dof_steps_budget:
+            break
+    return {"value": done / el, "unit": "Newton DOF-steps/s", "cores": int(L.orc_max_threads()), "kind": "port",
+            "sample": f"one time step of each of the first {used} members ({done} Newton DOF-steps, {el:.1f} s, "
+                      f"OpenMP-coloured assembly, sequential BiCGSTAB)"}
+
+
+def run_batch(args, dca, wl):
+    """Config: 256 independent Gray-Scott instances advanced by one Batch."""
+    import torch
+
+    precision = args.precision
+    ws = wl.gray_scott_batch(count=args.members, seed=0, precision=precision, nl_rtol=args.batch_nl_rtol,
+                             nl_atol=0.0, lin_rtol=1e-10)
+    t_setup = time.perf_counter()
+    b = dca.Batch(f"precision = {precision}\ngraph_iterations = {args.graph_iterations}\n", 0)
+    ids = []
+    for w in ws:
+        i = b.add(w.ini)
+        ids.append(i)
+        mv = b.member(i)
+        mv.set_state(w.initial_state(mv), 0.0)
+        mv.close()
+    n_dof = 0
+    for i in ids:
+        mv = b.member(i)
+        n_dof += mv.n_dof
+        mv.close()
+    setup_s = time.perf_counter() - t_setup
+    print(f"# batch set-up {setup_s:.1f}s: {len(ids)} members, {n_dof} DOF, "
+          f"{b.device_bytes() / 2**30:.2f} GiB on the device", file=sys.stderr, flush=True)
+
+    def one_step(k):
+        st = b.step()
+        bad = [x.status for x in st if x.status]
+        if bad:
+            raise SystemExit(f"batch step {k}: {len(bad)} members failed (status {bad[:5]})")
+        return st
+
+    for k in range(args.warmup):
+        one_step(k)
+    print(f"# warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    lat, newton, lin, refs = [], 0, 0, 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        s0 = time.perf_counter()
+        st = one_step(k)
+        lat.append(time.perf_counter() - s0)
+        newton += max(x.newton_iterations for x in st)
+        lin += max(x.linear_iterations for x in st)
+        refs += max(x.refinement_steps for x in st)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    print(f"# timed {args.steps} steps in {elapsed:.3f}s", file=sys.stderr, flush=True)
+    # kernel roofline: profiled steps after the timed region (plain launches,
+    # HIP events around every batched SpMV on the batch stream)
+    b.profile_enable(True)
+    for k in range(args.profile_steps):
+        one_step(k)
+    kind = "spmv32" if precision == "mixed" else "spmv64"
+    n, ms, by = b.profile(kind)
+    b.profile_enable(False)
+    roofline = None
+    if n:
+        gbs = (by / n) / (ms / n * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": f"batched {kind} (bspmv_k)", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None, "avg_launch_us": 1e3 * ms / n,
+                    "algorithmic_bytes_per_launch": by / n, "launches": n}
+    total_steps = args.warmup + args.steps + args.profile_steps
+    # error of sampled members against single FP64 models converged to round-off
+    sample = sorted(range(len(ws)), key=lambda j: ws[j].cells[0])
+    sample = sorted(set([sample[0], sample[len(sample) // 2], sample[-1]]))[: args.error_members]
+    errs = {}
+    tight = ("model.time_step_operator.linear_solver.precision = fp64\n"
+             "model.time_step_operator.nonlinear_solver.convergence_condition.relative_tolerance = 1e-20\n"
+             "model.time_step_operator.nonlinear_solver.convergence_condition.absolute_tolerance = 1e-28\n")
+    for j in sample:
+        w = ws[j]
+        mv = b.member(ids[j])
+        u, _ = mv.get_state()
+        info = mv.compartments()
+        mv.close()
+        ref = dca.Model(w.ini, tight + "model.time_step_operator.linear_solver.preconditioner.type = Jacobi\n", 0)
+        ref.set_state(w.initial_state(ref), 0.0)
+        t = 0.0
+        for _ in range(total_steps):
+            if ref.step(t, w.dt).status:
+                raise SystemExit("FP64 reference step failed")
+            t += w.dt
+        r, _ = ref.get_state()
+        ref.close()
+        e = []
+        for c in info:
+            for sp in range(c["n_species"]):
+                a = u[c["dof_offset"] + sp: c["dof_offset"] + c["n_species"] * c["n_nodes"]: c["n_species"]]
+                bb = r[c["dof_offset"] + sp: c["dof_offset"] + c["n_species"] * c["n_nodes"]: c["n_species"]]
+                e.append(float(np.linalg.norm(a - bb) / max(np.linalg.norm(bb), 1e-300)))
+        errs[w.name] = max(e)
+    cpu = None if args.no_cpu_baseline else cpu_baseline_batch(ws, args.cpu_dof_steps)
+    value = args.steps * n_dof / elapsed
+    out = {
+        "metric": "Newton DOF-steps/s (plain FP64 vs mixed/batched), p50/p99 latency, HBM GB/s",
+        "value": value,
+        "unit": "Newton DOF-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if precision == "fp64" else "f32-krylov/f64-refinement",
+        "data": "synthetic (seeded initial states and parameters)",
+        "config": {"workload": f"gray_scott_batch_{len(ws)}_2d_q1_sides32-512_implicit_euler_dt1",
+                   "members": len(ws), "n_dof": n_dof, "precision": precision, "preconditioner": "block-Jacobi",
+                   "graph_iterations": args.graph_iterations,
+                   "newton_relative_tolerance_two_norm2": args.batch_nl_rtol},
+        "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
+        "newton_iterations_per_step_max_member": newton / args.steps,
+        "krylov_iterations_per_step_max_member": lin / args.steps,
+        "refinements_per_step_max_member": refs / args.steps,
+        "error_vs_fp64": {"max_rel_l2_per_species": max(errs.values()) if errs else None, "members": errs,
+                          "steps": total_steps},
+        "setup_s": setup_s,
+        "device_bytes": b.device_bytes(),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    b.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +262,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this size (counter calibration under rocprofv3)")
+    ap.add_argument("--members", type=int, default=256, help="batch workload: number of models")
+    ap.add_argument("--graph-iterations", type=int, default=4, help="batch workload: iterations per hipGraph")
+    ap.add_argument("--batch-nl-rtol", type=float, default=1e-16,
+                    help="batch workload: Newton tolerance on the squared l2 residual")
+    ap.add_argument("--profile-steps", type=int, default=1, help="batch workload: profiled steps after timing")
+    ap.add_argument("--error-members", type=int, default=3, help="batch workload: members checked against FP64")
+    ap.add_argument("--cpu-dof-steps", type=float, default=4e6, help="batch workload: CPU baseline sample size")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -121,6 +280,8 @@ def main():
         raise SystemExit("bench.py needs a GPU")
     dca = _load("dune_copasi_amd", PKG / "__init__.py", pkg=True)
     wl = _load("dca_workloads", PKG / "workloads.py")
+    if args.workload == "gs_batch":
+        return run_batch(args, dca, wl)
     w, wname = make_workload(wl, args.workload, args.precision)
 
     t_setup = time.perf_counter()

@@ -80,6 +80,8 @@ _SIGS = {
     "dcg_batch_set_time_step": (_I, [_P, _I, _D]),
     "dcg_batch_step": (_I, [_P, _P, _S]),
     "dcg_batch_device_bytes": (_I, [_P, ctypes.POINTER(_S)]),
+    "dcg_batch_profile_enable": (_I, [_P, _I]),
+    "dcg_batch_profile_read": (_I, [_P, _I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
 }
 
 RUNTIME_PARAMETERS = 1
@@ -384,3 +386,13 @@ class Batch:
         b = _S()
         _check(self._lib.dcg_batch_device_bytes(self._h, ctypes.byref(b)))
         return b.value
+
+    def profile_enable(self, on: bool = True):
+        _check(self._lib.dcg_batch_profile_enable(self._h, 1 if on else 0))
+
+    def profile(self, kind: str):
+        """(launches, total ms, total algorithmic bytes) of the batched SpMV ("spmv64" or "spmv32")."""
+        l, ms, b = ctypes.c_long(), _D(), _D()
+        _check(self._lib.dcg_batch_profile_read(self._h, {"spmv64": 0, "spmv32": 1}[kind], ctypes.byref(l),
+                                                ctypes.byref(ms), ctypes.byref(b)))
+        return l.value, ms.value, b.value

@@ -38,6 +38,8 @@ Batch::Batch(int device, const BatchOptions& opt) : device_(device), opt_(opt) {
 
 Batch::~Batch() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto& e : ev_)
+    if (e) (void)hipEventDestroy(e);
   destroy_graphs();
   members_.clear();
   sgroups_.clear();
@@ -300,6 +302,27 @@ void Batch::capture(SolverGroup& g, bool fp32) {
   g.graph[fp32 ? 1 : 0] = exec;
 }
 
+void Batch::set_profiling(bool on) {
+  DCG_HIP(hipStreamSynchronize(stream_));
+  if (on && !ev_[0])
+    for (auto& e : ev_) DCG_HIP(hipEventCreate(&e));
+  prof_ = on;
+  for (auto& p : prof_data_) p = Prof{};
+}
+
+void Batch::profile(int kind, long* launches, double* ms, double* bytes) const {
+  if (kind < 0 || kind > 1) throw Error("batch profile kind must be 0 (FP64 SpMV) or 1 (FP32 SpMV)");
+  *launches = prof_data_[kind].launches;
+  *ms = prof_data_[kind].ms;
+  *bytes = prof_data_[kind].bytes;
+}
+
+size_t Batch::n_dof() const {
+  size_t n = 0;
+  for (const auto& e : members_) n += e.m->n_dof();
+  return n;
+}
+
 std::vector<std::array<int, 3>> Batch::spmv_choice() {
   if (dirty_) rebuild();
   std::vector<std::array<int, 3>> v;
@@ -407,6 +430,40 @@ void Batch::iterate(SolverGroup& g, bool fp32) {
   int done_its = 0;
   int blocks = std::max(1, (g.est_its + K - 1) / K);
   int most = 0;
+  if (prof_) {
+    // one iteration at a time, the SpMVs bracketed by events; the bytes are
+    // those of the members still iterating
+    for (;;) {
+      DCG_HIP(hipMemcpyAsync(sc.data(), g.sc.get(), M * sizeof(k::BicgScalars), hipMemcpyDeviceToHost, stream_));
+      DCG_HIP(hipStreamSynchronize(stream_));
+      double bytes = 0.0;
+      bool all = true;
+      for (size_t j = 0; j < M; ++j) {
+        if (sc[j].done) continue;
+        all = false;
+        bytes += members_[g.mem[j]].m->spmv_bytes(fp32 ? 4 : 8, fp32 ? 4 : 8, 1);
+      }
+      if (all || done_its > opt_.lin_maxit + 4) break;
+      if (fp32)
+        k::b_bicg_iteration<float>(g.B, g.variant32, g.w, g.mv.get(), k::kVKry + 9, opt_.jacobi, g.part.get(),
+                                   g.sc.get(), stream_, ev_);
+      else
+        k::b_bicg_iteration<double>(g.B, g.variant64, g.w, g.mv.get(), k::kVZ, opt_.jacobi, g.part.get(),
+                                    g.sc.get(), stream_, ev_);
+      DCG_HIP(hipEventSynchronize(ev_[3]));
+      float a = 0.f, b = 0.f;
+      DCG_HIP(hipEventElapsedTime(&a, ev_[0], ev_[1]));
+      DCG_HIP(hipEventElapsedTime(&b, ev_[2], ev_[3]));
+      Prof& p = prof_data_[fp32 ? 1 : 0];
+      p.launches += 2;
+      p.ms += a + b;
+      p.bytes += 2.0 * bytes;
+      ++done_its;
+    }
+    for (const auto& s2 : sc) most = std::max(most, s2.iter);
+    g.est_its = std::max(2, most);
+    return;
+  }
   for (;;) {
     for (int b = 0; b < blocks; ++b) {
       if (opt_.graph_iterations > 0) {

@@ -70,6 +70,14 @@ class Batch {
   // SpMV variant chosen for each block-size group: (B, FP64 variant, FP32 variant)
   std::vector<std::array<int, 3>> spmv_choice();
 
+  // Kernel timing: with profiling on, BiCGSTAB iterations run as plain
+  // launches with HIP events around every batched SpMV (kind 0: FP64, 1:
+  // FP32); launches, summed milliseconds and summed algorithmic bytes.
+  void set_profiling(bool on);
+  void profile(int kind, long* launches, double* ms, double* bytes) const;
+  // Newton DOF of all members (state entries, no padding)
+  size_t n_dof() const;
+
  private:
   struct Entry {
     int id;
@@ -128,6 +136,12 @@ class Batch {
 
   int device_;
   BatchOptions opt_;
+  bool prof_ = false;
+  hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+  struct Prof {
+    long launches = 0;
+    double ms = 0.0, bytes = 0.0;
+  } prof_data_[2];
   hipStream_t stream_ = nullptr;
   std::vector<Entry> members_;
   int next_id_ = 0;

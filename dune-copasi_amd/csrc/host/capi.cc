@@ -436,4 +436,25 @@ int dcg_batch_device_bytes(dcg_batch_t b, size_t* bytes) {
   });
 }
 
+int dcg_batch_profile_enable(dcg_batch_t b, int on) {
+  if (check_batch(b)) return -1;
+  return guarded([&] {
+    b->b->set_profiling(on != 0);
+    return 0;
+  });
+}
+
+int dcg_batch_profile_read(dcg_batch_t b, int kind, long* launches, double* milliseconds, double* bytes) {
+  if (check_batch(b)) return -1;
+  return guarded([&] {
+    long l = 0;
+    double ms = 0, by = 0;
+    b->b->profile(kind, &l, &ms, &by);
+    if (launches) *launches = l;
+    if (milliseconds) *milliseconds = ms;
+    if (bytes) *bytes = by;
+    return 0;
+  });
+}
+
 }  // extern "C"

@@ -156,7 +156,12 @@ class Model {
   void prof_begin(int kind);
   void prof_end(int kind, double bytes);
   void prof_collect();
+
+ public:
+  // algorithmic bytes of one SpMV on this model's matrix
   double spmv_bytes(int value_bytes, int vector_bytes, int extra_vectors) const;
+
+ private:
   double assembly_bytes(bool jacobian) const;
 
   ParameterTree cfg_tree_;

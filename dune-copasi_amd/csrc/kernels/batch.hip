@@ -525,27 +525,34 @@ template void b_bicg_init<float>(int, const BatchWork&, const MemberView*, int, 
 
 template <typename V>
 void b_bicg_iteration(int B, int variant, const BatchWork& w, const MemberView* mv, int x, bool jacobi,
-                      double* part, BicgScalars* sc, hipStream_t st) {
+                      double* part, BicgScalars* sc, hipStream_t st, hipEvent_t* ev) {
   if (w.nvwork <= 0) return;
   using T = V;  // the matrix copy has the precision of the Krylov vectors
   const int jac = jacobi ? 1 : 0;
   const Skip none{};
+  auto mark = [&](int i) {
+    if (ev) (void)hipEventRecord(ev[i], st);
+  };
   DCG_BATCH_B(B, {
     bp_k<BB, V><<<w.nvwork, kThreads, 0, st>>>(w.vwork, mv, jac, sc);
+    mark(0);
     bspmv_launch<BB, T, V>(variant, w, mv, none, sc, kVKry + 4, kVKry + 3, kVKry + 1, -1, part, st);
+    mark(1);
     bfin_k<<<w.members, kThreads, 0, st>>>(w.crange, part, 2, 1, sc);
     bs_k<BB, V><<<w.nvwork, kThreads, 0, st>>>(w.vwork, mv, jac, sc, part);
     bfin_k<<<w.members, kThreads, 0, st>>>(w.vrange, part, 1, 2, sc);
+    mark(2);
     bspmv_launch<BB, T, V>(variant, w, mv, none, sc, kVKry + 6, kVKry + 7, kVKry + 5, kVKry + 7, part, st);
+    mark(3);
     bfin_k<<<w.members, kThreads, 0, st>>>(w.crange, part, 2, 3, sc);
     bxr_k<BB, V><<<w.nvwork, kThreads, 0, st>>>(w.vwork, mv, x, sc, part);
     bfin_k<<<w.members, kThreads, 0, st>>>(w.vrange, part, 2, 4, sc);
   });
 }
 template void b_bicg_iteration<double>(int, int, const BatchWork&, const MemberView*, int, bool, double*,
-                                       BicgScalars*, hipStream_t);
+                                       BicgScalars*, hipStream_t, hipEvent_t*);
 template void b_bicg_iteration<float>(int, int, const BatchWork&, const MemberView*, int, bool, double*,
-                                      BicgScalars*, hipStream_t);
+                                      BicgScalars*, hipStream_t, hipEvent_t*);
 
 void b_spmv64(int B, int variant, const BatchWork& w, const MemberView* mv, Skip sk, int x, int y, hipStream_t st) {
   DCG_BATCH_B(B, (bspmv_launch<BB, double, double>(variant, w, mv, sk, nullptr, x, y, -1, -1, nullptr, st)));

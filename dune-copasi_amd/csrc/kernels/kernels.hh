@@ -121,9 +121,10 @@ template <typename V>
 void b_bicg_init(int B, const BatchWork& w, const MemberView* mv, int b, int x, double* part, BicgScalars* sc,
                  hipStream_t st);
 // one BiCGSTAB iteration on every unfinished member (jacobi: apply dinv)
+// ev (optional, 4 events): recorded before and after each of the two SpMVs
 template <typename V>
 void b_bicg_iteration(int B, int variant, const BatchWork& w, const MemberView* mv, int x, bool jacobi,
-                      double* part, BicgScalars* sc, hipStream_t st);
+                      double* part, BicgScalars* sc, hipStream_t st, hipEvent_t* ev = nullptr);
 // y = A x on the FP64 matrix (selectors), members skipped by sk
 void b_spmv64(int B, int variant, const BatchWork& w, const MemberView* mv, Skip sk, int x, int y, hipStream_t st);
 // y = A x on the FP32 matrix copy, FP32 Krylov vectors (selectors kVKry + i)

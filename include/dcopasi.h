@@ -187,6 +187,12 @@ int dcg_batch_set_time_step(dcg_batch_t batch, int id, double dt);
 /* one step of every member; stats[k] for the k-th id (capacity >= members) */
 int dcg_batch_step(dcg_batch_t batch, dcg_step_stats* stats, size_t capacity);
 int dcg_batch_device_bytes(dcg_batch_t batch, size_t* bytes);
+/* HIP-event timing of the batched SpMVs (kind 0: FP64, 1: FP32); while it is
+ * on, BiCGSTAB iterations run as plain launches instead of graphs.  Enabling
+ * resets the counters; bytes are the algorithmic bytes of the members that
+ * were still iterating. */
+int dcg_batch_profile_enable(dcg_batch_t batch, int on);
+int dcg_batch_profile_read(dcg_batch_t batch, int kind, long* launches, double* milliseconds, double* bytes);
 
 #ifdef __cplusplus
 }
