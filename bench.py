@@ -48,6 +48,9 @@ def make_workload(wl, name, precision):
     if name == "gs128":
         return wl.gray_scott(n=128, cell_type="cube", precond="ILU", nl_rtol=1e-10, lin_rtol=1e-10,
                              precision=precision), "gray_scott_2d_q1_128^2_implicit_euler_dt1"
+    if name == "gs1024":
+        return wl.gray_scott(n=1024, cell_type="cube", precond="Jacobi", nl_rtol=1e-16, lin_rtol=1e-10,
+                             precision=precision), "gray_scott_2d_q1_1024^2_implicit_euler_dt1"
     if name == "two1024":
         return wl.two_compartments(n=1024, cell_type="cube", precision=precision), \
             "two_compartments_2d_q1_1024^2_implicit_euler_dt0.05"

@@ -149,7 +149,8 @@ void Batch::rebuild() {
   for (auto& g : sgroups_) {
     const int M = static_cast<int>(g.mem.size());
     std::vector<k::MemberView> mv(M);
-    std::vector<int2> vwork, vrange(M), cwork, crange(M);
+    std::vector<int2> vwork, vrange(M), crange(M);
+    std::vector<int4> cwork;
     for (int j = 0; j < M; ++j) {
       const Model& m = *members_[g.mem[j]].m;
       mv[j] = m.view();
@@ -158,7 +159,13 @@ void Batch::rebuild() {
       for (int b = 0; b < nblk; ++b) vwork.push_back(make_int2(j, b));
       vrange[j].y = static_cast<int>(vwork.size());
       crange[j].x = static_cast<int>(cwork.size());
-      for (int c = 0; c < mv[j].nchunk; ++c) cwork.push_back(make_int2(j, c));
+      const auto& ch = m.chunk_host_;
+      const auto& rp = m.L_.row_ptr;
+      for (int c = 0; c < mv[j].nchunk; ++c) {
+        const int r0 = ch[c], r1 = ch[c + 1], k0 = rp[r0], k1 = rp[r1];
+        if (r1 - r0 > 0xffff || k1 - k0 > 0xffff) throw Error("internal: SpMV chunk too large for the work list");
+        cwork.push_back(make_int4(j, r0, k0, ((r1 - r0) << 16) | (k1 - k0)));
+      }
       crange[j].y = static_cast<int>(cwork.size());
     }
     g.h_act.assign(M, 0);

@@ -91,7 +91,8 @@ class Batch {
     int B = 0;
     std::vector<int> mem;  // indices into members_
     DeviceArray<k::MemberView> mv;
-    DeviceArray<int2> vwork, vrange, cwork, crange;
+    DeviceArray<int2> vwork, vrange, crange;
+    DeviceArray<int4> cwork;
     k::BatchWork w;
     DeviceArray<k::BicgScalars> sc;
     DeviceArray<k::IrState> ir;

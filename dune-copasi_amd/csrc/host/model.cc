@@ -177,6 +177,7 @@ void Model::setup_device() {
   }
   if (chunk.back() != ns) chunk.push_back(static_cast<int>(ns));
   chunk_row_.upload(chunk, stream_);
+  chunk_host_ = chunk;
   plan_.nchunk = static_cast<int>(chunk.size()) - 1;
   plan_.chunk_row = chunk_row_.get();
   plan_.rp = row_ptr_.get();

@@ -183,6 +183,7 @@ class Model {
 
   // device data
   DeviceArray<int> row_ptr_, col_, own_start_, diag_pos_, slot_b_, chunk_row_;
+  std::vector<int> chunk_host_;  // SpMV chunk row boundaries
   DeviceArray<unsigned> own_mask_;
   DeviceArray<long> slot_dof_;
   struct CompDev {

@@ -333,25 +333,25 @@ __global__ void __launch_bounds__(kThreads) bxr_k(const int2* __restrict__ work,
 // y = A x per (member, chunk); partial dots y.w1, y.w2 per work item.  The
 // work index goes through the XCD map like the single-model SpMV.
 template <int B, typename T, typename V, bool BLOCKS>
-__global__ void __launch_bounds__(kThreads) bspmv_k(int nwork, const int2* __restrict__ cwork,
+__global__ void __launch_bounds__(kThreads) bspmv_k(int nwork, const int4* __restrict__ cwork,
                                                     const MemberView* __restrict__ mv, Skip sk,
                                                     const BicgScalars* __restrict__ sc, int x, int y, int w1,
                                                     int w2, double* __restrict__ part) {
-  __shared__ double staged[kSpmvStage];
+  __shared__ StageT<T> staged[kSpmvStage];
   __shared__ double red[32];
   const int c = xcd_chunk(blockIdx.x, gridDim.x);
   if (c >= nwork) return;
-  const int2 w = cwork[c];
+  const int4 w = cwork[c];
   if (skipped(sk, w.x) || (sc && sc[w.x].done)) return;
   const MemberView& m = mv[w.x];
   double acc[2] = {0.0, 0.0};
-  const int r0 = m.chunk_row[w.y], r1 = m.chunk_row[w.y + 1];
+  const int r0 = w.y, r1 = w.y + (w.w >> 16), k0 = w.z, k1 = w.z + (w.w & 0xffff);
   if (BLOCKS)
-    spmv_blocks_chunk<B, T, V>(r0, r1, m.rp, m.col, matrix<T>(m), vsel<V>(m, x), vsel<V>(m, y), vsel<V>(m, w1),
-                               vsel<V>(m, w2), acc, staged);
+    spmv_blocks_chunk<B, T, V, StageT<T>>(r0, r1, k0, k1, m.rp, m.col, matrix<T>(m), vsel<V>(m, x), vsel<V>(m, y),
+                               vsel<V>(m, w1), vsel<V>(m, w2), acc, staged);
   else
-    spmv_entries_chunk<B, T, V>(r0, r1, m.rp, m.col, matrix<T>(m), vsel<V>(m, x), vsel<V>(m, y), vsel<V>(m, w1),
-                                vsel<V>(m, w2), acc, staged);
+    spmv_entries_chunk<B, T, V, StageT<T>>(r0, r1, k0, k1, m.rp, m.col, matrix<T>(m), vsel<V>(m, x), vsel<V>(m, y),
+                                vsel<V>(m, w1), vsel<V>(m, w2), acc, staged);
   if (part) {
     block_sum(acc, red);
     if (threadIdx.x == 0) {

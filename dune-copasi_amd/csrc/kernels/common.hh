@@ -4,6 +4,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.hh"
 
 namespace dcopasi::k {
@@ -193,18 +195,23 @@ __device__ __forceinline__ void block_inverse_dev(const double* __restrict__ d, 
 
 
 // ----------------------------------------------------------------- SpMV chunk bodies
-// One workgroup computes rows [r0, r1) of y = A x; the chunk's block-row
-// entries (blocks x B) fit the LDS stage.  Phase 1 streams the chunk's blocks
+// LDS stage element: the FP32 matrix stages its per-block products in FP32
+// (half the LDS, twice the resident workgroups), rows are summed in FP64.
+template <typename T>
+using StageT = typename std::conditional<std::is_same<T, float>::value, float, double>::type;
+
+// One workgroup computes rows [r0, r1) of y = A x, whose blocks are
+// [k0, k1) = [rp[r0], rp[r1]); the chunk's block-row entries (blocks x B)
+// fit the LDS stage.  Phase 1 streams the chunk's blocks
 // with consecutive threads on consecutive (block, row-of-block) pairs, so
 // value loads are fully coalesced; products go to LDS.  Phase 2 sums each row
 // from LDS and adds y.w1, y.w2 to acc.
-template <int B, typename T, typename V>
-__device__ __forceinline__ void spmv_entries_chunk(int r0, int r1, const int* __restrict__ rp,
+template <int B, typename T, typename V, typename S>
+__device__ __forceinline__ void spmv_entries_chunk(int r0, int r1, int k0, int k1, const int* __restrict__ rp,
                                                    const int* __restrict__ col, const T* __restrict__ val,
                                                    const V* __restrict__ x, V* __restrict__ y,
                                                    const V* __restrict__ w1, const V* __restrict__ w2,
-                                                   double (&acc)[2], double* __restrict__ staged) {
-  const int k0 = rp[r0], k1 = rp[r1];
+                                                   double (&acc)[2], S* __restrict__ staged) {
   const int ne = (k1 - k0) * B;
   constexpr int U = 4;  // independent entries per thread in flight
   for (int e0 = threadIdx.x; e0 < ne; e0 += U * kThreads) {
@@ -227,7 +234,7 @@ __device__ __forceinline__ void spmv_entries_chunk(int r0, int r1, const int* __
       double s = 0.0;
 #pragma unroll
       for (int j = 0; j < B; ++j) s += (double)av[u][j] * (double)xv[j];
-      staged[e] = s;
+      staged[e] = (S)s;
     }
   }
   __syncthreads();
@@ -235,7 +242,7 @@ __device__ __forceinline__ void spmv_entries_chunk(int r0, int r1, const int* __
   for (int o = threadIdx.x; o < nout; o += blockDim.x) {
     const int row = r0 + o / B, i = o % B;
     double s = 0.0;
-    for (int kb = rp[row]; kb < rp[row + 1]; ++kb) s += staged[(kb - k0) * B + i];
+    for (int kb = rp[row]; kb < rp[row + 1]; ++kb) s += (double)staged[(kb - k0) * B + i];
     const size_t yi = (size_t)row * B + i;
     y[yi] = (V)s;
     const double ys = (double)(V)s;
@@ -247,13 +254,12 @@ __device__ __forceinline__ void spmv_entries_chunk(int r0, int r1, const int* __
 // Block form: one thread per whole B x B block (B*B contiguous values, one
 // column index, one B-vector of x), writing B products to the stage; halves
 // the per-entry index and gather work for B = 2 and makes FP32 loads 16 bytes.
-template <int B, typename T, typename V>
-__device__ __forceinline__ void spmv_blocks_chunk(int r0, int r1, const int* __restrict__ rp,
+template <int B, typename T, typename V, typename S>
+__device__ __forceinline__ void spmv_blocks_chunk(int r0, int r1, int k0, int k1, const int* __restrict__ rp,
                                                   const int* __restrict__ col, const T* __restrict__ val,
                                                   const V* __restrict__ x, V* __restrict__ y,
                                                   const V* __restrict__ w1, const V* __restrict__ w2,
-                                                  double (&acc)[2], double* __restrict__ staged) {
-  const int k0 = rp[r0], k1 = rp[r1];
+                                                  double (&acc)[2], S* __restrict__ staged) {
   const int nb = k1 - k0;
   constexpr int U = 2;
   for (int b0 = threadIdx.x; b0 < nb; b0 += U * kThreads) {
@@ -281,7 +287,7 @@ __device__ __forceinline__ void spmv_blocks_chunk(int r0, int r1, const int* __r
         double s = 0.0;
 #pragma unroll
         for (int j = 0; j < B; ++j) s += (double)av[u][i * B + j] * xl[j];
-        staged[b * B + i] = s;
+        staged[b * B + i] = (S)s;
       }
     }
   }
@@ -290,7 +296,7 @@ __device__ __forceinline__ void spmv_blocks_chunk(int r0, int r1, const int* __r
   for (int o = threadIdx.x; o < nout; o += blockDim.x) {
     const int row = r0 + o / B, i = o % B;
     double s = 0.0;
-    for (int kb = rp[row]; kb < rp[row + 1]; ++kb) s += staged[(kb - k0) * B + i];
+    for (int kb = rp[row]; kb < rp[row + 1]; ++kb) s += (double)staged[(kb - k0) * B + i];
     const size_t yi = (size_t)row * B + i;
     y[yi] = (V)s;
     const double ys = (double)(V)s;

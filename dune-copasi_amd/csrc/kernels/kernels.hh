@@ -76,13 +76,15 @@ struct Skip {
   int when = 0;
 };
 
-// work lists: (member, 256-slot block) for vectors, (member, chunk) for SpMV;
-// range[m] = [first, last) work items of member m
+// work lists: (member, 256-slot block) for vectors; for SpMV one item per
+// chunk of rows, {member, first row, first block, rows << 16 | blocks}, so a
+// workgroup starts streaming after a single dependent load; range[m] =
+// [first, last) work items of member m
 struct BatchWork {
   const int2* vwork = nullptr;
   int nvwork = 0;
   const int2* vrange = nullptr;
-  const int2* cwork = nullptr;
+  const int4* cwork = nullptr;
   int ncwork = 0;
   const int2* crange = nullptr;
   int members = 0;

@@ -128,11 +128,14 @@ __global__ void __launch_bounds__(kThreads) spmv_k(int nchunk, const int* __rest
                                                    const V* __restrict__ w2, double* __restrict__ part,
                                                    BicgScalars* __restrict__ sc, int stage) {
   if (sc && sc->done) return;
-  __shared__ double staged[kSpmvStage];
+  __shared__ StageT<T> staged[kSpmvStage];
   __shared__ double red[32];
   const int c = xcd_chunk(blockIdx.x, gridDim.x);
   double acc[2] = {0.0, 0.0};
-  if (c < nchunk) spmv_entries_chunk<B, T, V>(chunk_row[c], chunk_row[c + 1], rp, col, val, x, y, w1, w2, acc, staged);
+  if (c < nchunk) {
+    const int r0 = chunk_row[c], r1 = chunk_row[c + 1];
+    spmv_entries_chunk<B, T, V, StageT<T>>(r0, r1, rp[r0], rp[r1], rp, col, val, x, y, w1, w2, acc, staged);
+  }
   if (part) {
     block_sum(acc, red);
     if (threadIdx.x == 0 && c < nchunk) {
@@ -217,11 +220,14 @@ __global__ void __launch_bounds__(kThreads) spmv_blk_k(int nchunk, const int* __
                                                        const V* __restrict__ w2, double* __restrict__ part,
                                                        const BicgScalars* __restrict__ sc) {
   if (sc && sc->done) return;
-  __shared__ double staged[kSpmvStage];
+  __shared__ StageT<T> staged[kSpmvStage];
   __shared__ double red[32];
   const int c = xcd_chunk(blockIdx.x, gridDim.x);
   double acc[2] = {0.0, 0.0};
-  if (c < nchunk) spmv_blocks_chunk<B, T, V>(chunk_row[c], chunk_row[c + 1], rp, col, val, x, y, w1, w2, acc, staged);
+  if (c < nchunk) {
+    const int r0 = chunk_row[c], r1 = chunk_row[c + 1];
+    spmv_blocks_chunk<B, T, V, StageT<T>>(r0, r1, rp[r0], rp[r1], rp, col, val, x, y, w1, w2, acc, staged);
+  }
   if (part) {
     block_sum(acc, red);
     if (threadIdx.x == 0 && c < nchunk) {
