@@ -36,6 +36,23 @@ class StepStats(ctypes.Structure):
     ]
 
 
+class RequestInfo(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int),
+        ("finished", ctypes.c_int),
+        ("steps_done", ctypes.c_int),
+        ("steps", ctypes.c_int),
+        ("newton_iterations", ctypes.c_long),
+        ("linear_iterations", ctypes.c_long),
+        ("n_dof", ctypes.c_size_t),
+        ("device_bytes", ctypes.c_size_t),
+        ("t_arrival", ctypes.c_double),
+        ("t_ready", ctypes.c_double),
+        ("t_admitted", ctypes.c_double),
+        ("t_finished", ctypes.c_double),
+    ]
+
+
 _P = ctypes.c_void_p
 _D = ctypes.c_double
 _I = ctypes.c_int
@@ -82,6 +99,14 @@ _SIGS = {
     "dcg_batch_device_bytes": (_I, [_P, ctypes.POINTER(_S)]),
     "dcg_batch_profile_enable": (_I, [_P, _I]),
     "dcg_batch_profile_read": (_I, [_P, _I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "dcg_server_create": (_I, [ctypes.c_char_p, _I, ctypes.POINTER(_P)]),
+    "dcg_server_destroy": (None, [_P]),
+    "dcg_server_submit": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p, _I, ctypes.POINTER(_I)]),
+    "dcg_server_tick": (_I, [_P, ctypes.POINTER(_I)]),
+    "dcg_server_result": (_I, [_P, _I, _P, _S, ctypes.POINTER(RequestInfo)]),
+    "dcg_server_forget": (_I, [_P, _I]),
+    "dcg_server_stats": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I),
+                              ctypes.POINTER(_S), ctypes.POINTER(_S), ctypes.POINTER(_D)]),
 }
 
 RUNTIME_PARAMETERS = 1
@@ -396,3 +421,63 @@ class Batch:
         _check(self._lib.dcg_batch_profile_read(self._h, {"spmv64": 0, "spmv32": 1}[kind], ctypes.byref(l),
                                                 ctypes.byref(ms), ctypes.byref(b)))
         return l.value, ms.value, b.value
+
+
+class Server:
+    """Request loop over one GPU with continuous batching (dcg_server_t)."""
+
+    def __init__(self, options: str = "", device: int = 0):
+        lib = load_library()
+        h = ctypes.c_void_p()
+        _check(lib.dcg_server_create(options.encode(), device, ctypes.byref(h)))
+        self._h = h
+        self._lib = lib
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.dcg_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, ini: str, steps: int, overrides: str = "") -> int:
+        i = _I()
+        _check(self._lib.dcg_server_submit(self._h, ini.encode(), overrides.encode(), steps, ctypes.byref(i)))
+        return i.value
+
+    def tick(self) -> int:
+        """Admit, advance every resident request one step, retire; returns unfinished requests."""
+        o = _I()
+        _check(self._lib.dcg_server_tick(self._h, ctypes.byref(o)))
+        return o.value
+
+    def info(self, rid: int) -> RequestInfo:
+        ri = RequestInfo()
+        rc = self._lib.dcg_server_result(self._h, rid, None, 0, ctypes.byref(ri))
+        if rc < 0:
+            raise DcgError(self._lib.dcg_last_error().decode())
+        return ri
+
+    def result(self, rid: int):
+        """(state, info) of a finished request, or (None, info) while it runs."""
+        ri = self.info(rid)
+        if not ri.finished:
+            return None, ri
+        if ri.status < 0:
+            raise DcgError(self._lib.dcg_last_error().decode())
+        u = np.zeros(ri.n_dof)
+        _check(self._lib.dcg_server_result(self._h, rid, _ptr(u), u.size, ctypes.byref(ri)))
+        return u, ri
+
+    def forget(self, rid: int):
+        _check(self._lib.dcg_server_forget(self._h, rid))
+
+    def stats(self):
+        q, r, res, fin = _I(), _I(), _I(), _I()
+        used, budget, sec = _S(), _S(), _D()
+        _check(self._lib.dcg_server_stats(self._h, ctypes.byref(q), ctypes.byref(r), ctypes.byref(res),
+                                          ctypes.byref(fin), ctypes.byref(used), ctypes.byref(budget),
+                                          ctypes.byref(sec)))
+        return {"queued": q.value, "ready": r.value, "resident": res.value, "finished": fin.value,
+                "used_bytes": used.value, "budget_bytes": budget.value, "seconds": sec.value}

@@ -26,7 +26,9 @@ BatchOptions BatchOptions::from_ini(const std::string& text) {
 
 bool Model::needs_own_assembly() const {
   for (size_t c = 0; c < L_.comps.size(); ++c)
-    if (vol_fn_[c] && (c >= static_cast<size_t>(k::kAsmComps) || gen_.tile_batch_kernel[c].empty())) return true;
+    if ((vol_fn_[c] || tile_fn_[c]) &&
+        (c >= static_cast<size_t>(k::kAsmComps) || gen_.tile_batch_kernel[c].empty()))
+      return true;
   return !facet_fn_.empty() || extra_pos_.size() > 0;
 }
 
@@ -57,23 +59,50 @@ void Batch::destroy_graphs() {
       }
 }
 
-int Batch::add(const std::string& ini, const std::string& overrides) {
-  DCG_HIP(hipSetDevice(device_));
+ModelOptions Batch::member_options() const {
   ModelOptions mo;
   mo.runtime_parameters = true;
   mo.batch_member = true;
-  mo.stream = stream_;
-  // the batch decides the linear solver: members allocate what it needs
+  return mo;
+}
+
+// the batch decides the linear solver: members allocate what it needs
+std::string Batch::member_overrides(const std::string& overrides) const {
   std::string ov = overrides;
   ov += "\nmodel.time_step_operator.linear_solver.precision = ";
   ov += opt_.precision == Precision::Mixed ? "mixed" : "fp64";
   ov += "\nmodel.time_step_operator.linear_solver.preconditioner.type = ";
   ov += opt_.jacobi ? "Jacobi" : "None";
   ov += "\n";
+  return ov;
+}
+
+int Batch::add(const std::string& ini, const std::string& overrides) {
+  DCG_HIP(hipSetDevice(device_));
+  ModelOptions mo = member_options();
+  mo.stream = stream_;
+  auto m = std::make_unique<Model>(ini, member_overrides(overrides), device_, mo);
+  if (m->rk().stages > 2) throw NotImplemented("batch members use schemes of at most two stages");
   Entry e;
   e.id = next_id_++;
-  e.m = std::make_unique<Model>(ini, ov, device_, mo);
-  if (e.m->rk().stages > 2) throw NotImplemented("batch members use schemes of at most two stages");
+  e.m = std::move(m);
+  e.dt = e.m->time_step();
+  members_.push_back(std::move(e));
+  dirty_ = true;
+  return members_.back().id;
+}
+
+int Batch::add(std::unique_ptr<Model> m) {
+  if (!m) throw Error("null model");
+  if (!m->opt_.runtime_parameters || !m->opt_.batch_member)
+    throw Error("batch members are built with Batch::member_options()");
+  if (m->rk().stages > 2) throw NotImplemented("batch members use schemes of at most two stages");
+  DCG_HIP(hipSetDevice(device_));
+  if (!m->on_device()) m->attach(device_, stream_);
+  else if (m->stream() != stream_) throw Error("model lives on another stream");
+  Entry e;
+  e.id = next_id_++;
+  e.m = std::move(m);
   e.dt = e.m->time_step();
   members_.push_back(std::move(e));
   dirty_ = true;

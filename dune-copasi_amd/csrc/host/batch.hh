@@ -52,6 +52,13 @@ class Batch {
 
   // builds a member on the batch's device and stream; returns its id
   int add(const std::string& ini, const std::string& overrides);
+  // member options and the overrides that put the batch's linear solver in
+  // a member's INI (for members built elsewhere, e.g. on worker threads)
+  ModelOptions member_options() const;
+  std::string member_overrides(const std::string& overrides) const;
+  // adopts a host-built member (created with member_options(), device < 0):
+  // attaches it to the batch's device and stream
+  int add(std::unique_ptr<Model> m);
   void remove(int id);
   Model& member(int id);
   std::vector<int> ids() const;

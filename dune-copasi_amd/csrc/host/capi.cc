@@ -6,6 +6,7 @@
 #include "../../../include/dcopasi.h"
 #include "batch.hh"
 #include "model.hh"
+#include "serve.hh"
 
 // owning handle, or a view of a batch member (own empty)
 struct dcg_model {
@@ -15,6 +16,10 @@ struct dcg_model {
 
 struct dcg_batch {
   std::unique_ptr<dcopasi::Batch> b;
+};
+
+struct dcg_server {
+  std::unique_ptr<dcopasi::Server> s;
 };
 
 namespace {
@@ -453,6 +458,100 @@ int dcg_batch_profile_read(dcg_batch_t b, int kind, long* launches, double* mill
     if (launches) *launches = l;
     if (milliseconds) *milliseconds = ms;
     if (bytes) *bytes = by;
+    return 0;
+  });
+}
+
+static int check_server(dcg_server_t s) {
+  if (!s || !s->s) {
+    g_error = "null server handle";
+    return -1;
+  }
+  return 0;
+}
+
+int dcg_server_create(const char* options_ini, int device, dcg_server_t* out) {
+  return guarded([&] {
+    if (!out) throw dcopasi::Error("null output handle");
+    *out = nullptr;
+    if (device < 0) throw dcopasi::Error("a server needs a device");
+    auto h = std::make_unique<dcg_server>();
+    h->s = std::make_unique<dcopasi::Server>(device, dcopasi::ServerOptions::from_ini(options_ini ? options_ini : ""));
+    *out = h.release();
+    return 0;
+  });
+}
+
+void dcg_server_destroy(dcg_server_t server) { delete server; }
+
+int dcg_server_submit(dcg_server_t h, const char* ini_text, const char* overrides, int steps, int* id) {
+  if (check_server(h)) return -1;
+  return guarded([&] {
+    const int i = h->s->submit(ini_text ? ini_text : "", overrides ? overrides : "", steps);
+    if (id) *id = i;
+    return 0;
+  });
+}
+
+int dcg_server_tick(dcg_server_t h, int* open) {
+  if (check_server(h)) return -1;
+  return guarded([&] {
+    const int o = h->s->tick();
+    if (open) *open = o;
+    return 0;
+  });
+}
+
+int dcg_server_result(dcg_server_t h, int id, double* state, size_t n, dcg_request_info* info) {
+  if (check_server(h)) return -1;
+  return guarded([&] {
+    std::vector<double> u;
+    dcopasi::RequestInfo ri;
+    const bool done = h->s->result(id, state ? &u : nullptr, &ri);
+    if (info) {
+      info->status = ri.status;
+      info->finished = ri.finished ? 1 : 0;
+      info->steps_done = ri.steps_done;
+      info->steps = ri.steps;
+      info->newton_iterations = ri.newton_iterations;
+      info->linear_iterations = ri.linear_iterations;
+      info->n_dof = ri.n_dof;
+      info->device_bytes = ri.device_bytes;
+      info->t_arrival = ri.t_arrival;
+      info->t_ready = ri.t_ready;
+      info->t_admitted = ri.t_admitted;
+      info->t_finished = ri.t_finished;
+    }
+    if (ri.status < 0) g_error = ri.error;
+    if (!done) return 1;
+    if (state) {
+      if (n != u.size()) throw dcopasi::Error("state size mismatch");
+      std::copy(u.begin(), u.end(), state);
+    }
+    return 0;
+  });
+}
+
+int dcg_server_forget(dcg_server_t h, int id) {
+  if (check_server(h)) return -1;
+  return guarded([&] {
+    h->s->forget(id);
+    return 0;
+  });
+}
+
+int dcg_server_stats(dcg_server_t h, int* queued, int* ready, int* resident, int* finished, size_t* used_bytes,
+                     size_t* budget_bytes, double* seconds) {
+  if (check_server(h)) return -1;
+  return guarded([&] {
+    const auto st = h->s->stats();
+    if (queued) *queued = st.queued;
+    if (ready) *ready = st.ready;
+    if (resident) *resident = st.resident;
+    if (finished) *finished = st.finished;
+    if (used_bytes) *used_bytes = st.used_bytes;
+    if (budget_bytes) *budget_bytes = st.budget_bytes;
+    if (seconds) *seconds = h->s->now();
     return 0;
   });
 }

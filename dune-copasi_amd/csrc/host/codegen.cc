@@ -590,8 +590,11 @@ Grid unit_cell(const Grid& g) {
   return u;
 }
 
+// with_kernel = false: only the compartment's tables and point function
+// (what the tiled kernel builds on); the coloured kernel dominates hipRTC
+// time and is left out when the tiled kernel serves the compartment.
 std::string gen_volume(const ModelSpec& m, const Grid& g, int comp, const std::string& ns, const std::string& kname,
-                       bool numerical, bool geo) {
+                       bool numerical, bool geo, bool with_kernel) {
   const auto T = element_tables(geo ? unit_cell(g) : g);
   const auto& species = m.comp_species[comp];
   const int bc = static_cast<int>(species.size());
@@ -759,7 +762,7 @@ std::string gen_volume(const ModelSpec& m, const Grid& g, int comp, const std::s
   std::ostringstream full;
   full << "#define NUMERICAL_JAC " << (numerical ? 1 : 0) << "\n#define HAS_GC " << (has_gc ? 1 : 0) << "\n#define HAS_DC "
        << (has_dc ? 1 : 0) << "\n";
-  full << o.str() << body << "#undef NUMERICAL_JAC\n#undef HAS_GC\n#undef HAS_DC\n";
+  full << o.str() << (with_kernel ? body : std::string()) << "#undef NUMERICAL_JAC\n#undef HAS_GC\n#undef HAS_DC\n";
   return full.str();
 }
 
@@ -1275,9 +1278,10 @@ GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L, bool r
     }
     std::string ns = "vol" + std::to_string(c);
     std::string kn = "dcg_volume_" + std::to_string(c);
-    src << gen_volume(spec, L.grid, static_cast<int>(c), ns, kn, numerical, geo);
-    out.volume_kernel.push_back(kn);
-    if (tile_eligible(spec, static_cast<int>(c))) {
+    const bool tiled = tile_eligible(spec, static_cast<int>(c));
+    src << gen_volume(spec, L.grid, static_cast<int>(c), ns, kn, numerical, geo, !tiled);
+    out.volume_kernel.push_back(tiled ? "" : kn);
+    if (tiled) {
       auto tile = choose_tile(L.grid, static_cast<int>(spec.comp_species[c].size()));
       std::string tk = "dcg_tile_" + std::to_string(c);
       src << gen_tiled(spec, L.grid, static_cast<int>(c), ns, tk, tile, geo);

@@ -25,7 +25,8 @@ struct FacetKernel {
 
 struct GeneratedSource {
   std::string source;
-  std::vector<std::string> volume_kernel;  // per compartment ("" when it has no species)
+  // coloured kernel per compartment ("" when it has no species or is tiled)
+  std::vector<std::string> volume_kernel;
   // tiled write-once variant per compartment ("" when the model needs the coloured kernel)
   std::vector<std::string> tile_kernel;
   // batched tiled kernel per compartment (runtime geometry only, else "")

@@ -47,6 +47,17 @@ Model::Model(const std::string& ini, const std::string& overrides, int device, c
   }
 }
 
+void Model::precompile() const { (void)rtc_compile(gen_.source, "dcopasi_assembly"); }
+
+void Model::attach(int device, hipStream_t stream) {
+  if (device_ >= 0) throw Error("model is already on a device");
+  if (device < 0) throw Error("attach needs a device");
+  device_ = device;
+  opt_.stream = stream;
+  setup_device();
+  compile_and_load();
+}
+
 Model::~Model() {
   for (auto& p : prof_data_)
     for (auto e : p.pool) (void)hipEventDestroy(e);
@@ -484,7 +495,7 @@ void Model::launch_assembly(const double* x, double t, double cm, double ca, con
   const unsigned* own_mask = own_mask_.get();
   const double* prm = prm_.get();
   for (size_t c = 0; c < L_.comps.size() && (phases & 2); ++c) {
-    if (!vol_fn_[c]) continue;
+    if (!vol_fn_[c] && !tile_fn_[c]) continue;
     const auto& C = L_.comps[c];
     if (tile_fn_[c]) {
       if (skip_tiled) continue;

@@ -91,6 +91,12 @@ class Model {
   double time_step() const { return dt_initial_; }
   double time_end() const { return t_end_; }
 
+  // two-phase construction (device < 0 first, e.g. on a worker thread):
+  // compile the generated kernels into the on-disk cache, then allocate and
+  // upload on `device`, on `stream` when given
+  void precompile() const;
+  void attach(int device, hipStream_t stream = nullptr);
+
   void set_state(const double* packed, double t);
   void get_state(double* packed, double* t);
   void initial_state(double* packed) const;  // from initial.expression keys

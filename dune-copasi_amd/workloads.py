@@ -279,6 +279,114 @@ outer.expression = (position_x-0.5)^2 + (position_y-0.5)^2 >= 0.09
     return Workload(f"two_compartments_{cell_type}_{n}", ini, dt, steps, 2, (n, n), init)
 
 
+# ------------------------------------------------------------- serving requests
+def random_request(seed: int, precision="mixed", nl_rtol=1e-16, nl_atol=0.0, side2=(64, 512), side3=(32, 96),
+                   steps_range=(50, 500), dt=0.05) -> Workload:
+    """One seeded random model of the serving stream (benchmark config 4):
+    1 or 2 compartments (inner disk / ball of radius 0.3 and its complement)
+    on a 2D Q1 mesh with side log-uniform in side2 or a 3D Q1 mesh with side
+    log-uniform in side3; 2-8 species; a random mass-action network of 2-12
+    reactions with rates log-uniform in [1e-3, 1]; diffusion log-uniform in
+    [1e-4, 1e-2]; with two compartments one transmission flux
+    k_t (a_in - b_out) across the interface; implicit Euler or Alexander2;
+    a step count uniform in steps_range.  Rates, diffusion and k_t are
+    parser_context constants; Jacobians are written out (mass action)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dim = int(rng.integers(2, 4))
+    lo, hi = side2 if dim == 2 else side3
+    n = int(np.clip(np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi)))), lo, hi))
+    ncomp = int(rng.integers(1, 3))
+    nsp = int(rng.integers(2, 9))
+    if ncomp == 2:
+        nin = int(rng.integers(1, nsp))
+        comps = ["inner"] * nin + ["outer"] * (nsp - nin)
+    else:
+        comps = ["domain"] * nsp
+    names = [f"s{i}" for i in range(nsp)]
+    nreac = int(rng.integers(2, 13))
+    logu = lambda a, b: float(np.exp(rng.uniform(np.log(a), np.log(b))))
+    consts = {}
+    terms = {s: [] for s in names}      # (sign, rate expr, reactants)
+    for r in range(nreac):
+        c = comps[int(rng.integers(0, nsp))]
+        pool = [s for s, cc in zip(names, comps) if cc == c]
+        nr = int(rng.integers(1, min(2, len(pool)) + 1))
+        reac = list(rng.choice(pool, size=nr, replace=False))
+        npr = int(rng.integers(0, min(2, len(pool)) + 1))
+        prod = list(rng.choice(pool, size=npr, replace=False)) if npr else []
+        k = f"k{r}"
+        consts[k] = logu(1e-3, 1.0)
+        for s in reac:
+            terms[s].append((-1, k, reac))
+        for s in prod:
+            terms[s].append((+1, k, reac))
+    for i, s in enumerate(names):
+        consts[f"D{i}"] = logu(1e-4, 1e-2)
+    trans = None
+    if ncomp == 2:
+        a = names[int(rng.integers(0, nin))]
+        b = names[int(rng.integers(nin, nsp))]
+        consts["kt"] = logu(1e-3, 1.0)
+        trans = (a, b)
+    rk = "ImplicitEuler" if rng.integers(0, 2) == 0 else "Alexander2"
+    steps = int(rng.integers(steps_range[0], steps_range[1] + 1))
+    ic = {s: (float(rng.uniform(0.2, 0.8)), float(rng.uniform(0.0, 0.2)), rng.uniform(1.0, 6.0, size=3))
+          for s in names}
+
+    axes = ["position_x", "position_y", "position_z"][:dim]
+    cells = " ".join([str(n)] * dim)
+    ext = " ".join(["1"] * dim)
+    r2 = " + ".join(f"({ax}-0.5)^2" for ax in axes)
+    lines = [f"[grid]\ndimension = {dim}\ncells = {cells}\nextensions = {ext}\ncell_type = cube\n",
+             "[parser_context]"]
+    for k2, v in consts.items():
+        lines.append(f"{k2}.type = constant\n{k2}.value = {v!r}")
+    lines.append("\n[compartments]")
+    if ncomp == 2:
+        lines.append(f"inner.expression = {r2} < 0.09\nouter.expression = {r2} >= 0.09")
+    else:
+        lines.append("domain.expression = 1")
+
+    def mono(reac, skip=None):
+        f = [x for x in reac if x != skip]
+        return "*".join(f) if f else "1"
+
+    for i, s in enumerate(names):
+        sec = [f"\n[model.scalar_field.{s}]", f"compartment = {comps[i]}", "storage.expression = 1",
+               f"cross_diffusion.{s}.expression = D{i}"]
+        a0, a1, w = ic[s]
+        arg = " + ".join(f"{w[j]:.6f}*{ax}" for j, ax in enumerate(axes))
+        sec.append(f"initial.expression = {a0:.6f} + {a1:.6f}*sin({arg})")
+        if terms[s]:
+            expr = " ".join(f"{'+' if sg > 0 else '-'} {k}*{mono(reac)}" for sg, k, reac in terms[s])
+            sec.append(f"reaction.expression = {expr}")
+            jac = {}
+            for sg, k, reac in terms[s]:
+                for x in reac:
+                    jac.setdefault(x, []).append(f"{'+' if sg > 0 else '-'} {k}*{mono(reac, x)}")
+            for x, parts in jac.items():
+                sec.append(f"reaction.jacobian.{x}.expression = {' '.join(parts)}")
+        if trans and s == trans[0]:
+            a, b = trans
+            sec += [f"outflow.outer.expression = kt*({a} - {b})", f"outflow.outer.jacobian.{a}.expression = kt",
+                    f"outflow.outer.jacobian.{b}.expression = -kt"]
+        if trans and s == trans[1]:
+            a, b = trans
+            sec += [f"outflow.inner.expression = -kt*({a} - {b})", f"outflow.inner.jacobian.{a}.expression = -kt",
+                    f"outflow.inner.jacobian.{b}.expression = kt"]
+        lines.append("\n".join(sec))
+    ini = "\n".join(lines) + "\n\n" + _solver_block(rk, "Jacobi", nl_rtol, nl_atol, 1e-10, precision, dt=dt)
+
+    def init(model):
+        out = np.zeros(model.n_dof)
+        model_init = getattr(model, "initial_state", None)
+        return model_init() if model_init is not None else out
+
+    w = Workload(f"request_{seed}_{dim}d_{n}_{nsp}sp_{ncomp}c_{rk}", ini, dt, steps, dim, (n,) * dim, init,
+                 stages=1 if rk == "ImplicitEuler" else 2)
+    return w
+
+
 # ------------------------------------------------------------- closed forms
 def exponential_decay(dim=2, n=4, cell_type="simplex", rk="ImplicitEuler", rate=-2.0, D=0.0) -> Workload:
     """u' = rate * u with storage 1 (test/exp.ini); uniform u stays uniform."""

@@ -194,6 +194,40 @@ int dcg_batch_device_bytes(dcg_batch_t batch, size_t* bytes);
 int dcg_batch_profile_enable(dcg_batch_t batch, int on);
 int dcg_batch_profile_read(dcg_batch_t batch, int kind, long* launches, double* milliseconds, double* bytes);
 
+/* ---------------------------------------------------------------- serving
+ * A request loop over one GPU: requests (an INI model and a number of time
+ * steps from its initial state) are built and compiled on host worker
+ * threads, admitted to a resident batch while their estimated HBM footprint
+ * fits the budget and a member slot is free, advanced one step per
+ * dcg_server_tick together with every other resident request (continuous
+ * batching), and retired with their final state kept for dcg_server_result.
+ * No counterpart in the reference, which solves one model per process
+ * (src/dune_copasi.cc:229-290).
+ * options_ini keys: the batch keys plus hbm_budget_bytes (0: 90% of free
+ * memory), max_members, workers. */
+typedef struct dcg_server* dcg_server_t;
+
+typedef struct dcg_request_info {
+  int status;      /* 0 ok, > 0 numerical failure (dcg_step_stats codes), < 0 set-up error */
+  int finished;    /* 1 once the result is available */
+  int steps_done, steps;
+  long newton_iterations, linear_iterations;
+  size_t n_dof, device_bytes;
+  double t_arrival, t_ready, t_admitted, t_finished; /* seconds since the server started */
+} dcg_request_info;
+
+int dcg_server_create(const char* options_ini, int device, dcg_server_t* out);
+void dcg_server_destroy(dcg_server_t server);
+int dcg_server_submit(dcg_server_t server, const char* ini_text, const char* overrides, int steps, int* id);
+/* admit, advance every resident request one step, retire; *open = requests not finished */
+int dcg_server_tick(dcg_server_t server, int* open);
+/* info always; state (n == info.n_dof) once finished; returns 1 while unfinished */
+int dcg_server_result(dcg_server_t server, int id, double* state, size_t n, dcg_request_info* info);
+int dcg_server_forget(dcg_server_t server, int id);
+/* queued (waiting for a worker), ready (built, waiting for memory), resident, finished; bytes */
+int dcg_server_stats(dcg_server_t server, int* queued, int* ready, int* resident, int* finished, size_t* used_bytes,
+                     size_t* budget_bytes, double* seconds);
+
 #ifdef __cplusplus
 }
 #endif

@@ -167,7 +167,33 @@ def _hiprtc_compile(src):
 def test_generated_kernels_compile_for_gfx950(dca, wl, make):
     m = dca.Model(make(wl).ini, "", -1)
     src = m.generated_source()
-    assert "dcg_volume_0" in src
+    # tile-eligible compartments get only the tiled kernel (the coloured one
+    # would be dead code and dominates compile time)
+    assert "dcg_tile_0" in src and "dcg_volume_0" not in src
+    rc, log = _hiprtc_compile(src)
+    assert rc == 0, log
+
+
+def test_coloured_kernel_source_compiles(dca):
+    """A model with advection needs the coloured kernel; it is generated and compiles."""
+    ini = """[grid]
+dimension = 2
+cells = 8 8
+cell_type = cube
+[compartments]
+domain.expression = 1
+[model.scalar_field.u]
+compartment = domain
+storage.expression = 1
+cross_diffusion.u.expression = 1e-3
+velocity.x.expression = 0.02
+reaction.expression = -u
+reaction.jacobian.u.expression = -1
+[model.time_step_operator]
+type = ImplicitEuler
+"""
+    src = dca.Model(ini, "", -1).generated_source()
+    assert "dcg_volume_0" in src and "dcg_tile_0" not in src
     rc, log = _hiprtc_compile(src)
     assert rc == 0, log
 
