@@ -69,6 +69,7 @@ void Server::worker() {
       if (stop_) return;
       r = req_[todo_.front()].get();
       todo_.pop_front();
+      ++building_;
     }
     std::unique_ptr<Model> m;
     size_t est = 0;
@@ -82,23 +83,32 @@ void Server::worker() {
     } catch (const std::exception& e) {
       err = e.what();
     }
-    std::lock_guard<std::mutex> lock(mu_);
-    r->info.t_ready = now();
-    if (!err.empty()) {
-      r->info.status = -1;
-      r->info.error = err;
-      r->info.finished = true;
-      r->info.t_finished = r->info.t_ready;
-      continue;
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      --building_;
+      r->info.t_ready = now();
+      if (!err.empty()) {
+        r->info.status = -1;
+        r->info.error = err;
+        r->info.finished = true;
+        r->info.t_finished = r->info.t_ready;
+      } else {
+        r->model = std::move(m);
+        r->estimate = est;
+        r->info.n_dof = r->model->n_dof();
+        ready_.push_back(r->id);
+      }
     }
-    r->model = std::move(m);
-    r->estimate = est;
-    r->info.n_dof = r->model->n_dof();
-    ready_.push_back(r->id);
+    ready_cv_.notify_all();
   }
 }
 
 int Server::tick() {
+  if (batch_->size() == 0) {
+    std::unique_lock<std::mutex> lock(mu_);
+    ready_cv_.wait_for(lock, std::chrono::milliseconds(100),
+                       [this] { return !ready_.empty() || (todo_.empty() && building_ == 0); });
+  }
   // admission in arrival order while the footprint fits and a slot is free
   for (;;) {
     Request* r = nullptr;

@@ -53,7 +53,9 @@ class Server {
   // queue a request: `steps` time steps of the model from its initial state
   int submit(const std::string& ini, const std::string& overrides, int steps);
   // one scheduling round: admit what fits, one step of every resident
-  // request, retire finished ones; returns the requests not yet finished
+  // request, retire finished ones; returns the requests not yet finished.
+  // With nothing resident and nothing built yet it first waits (at most
+  // 100 ms) for a worker to finish a request.
   int tick();
   // false while the request is unfinished; state may be null
   bool result(int id, std::vector<double>* state, RequestInfo* info);
@@ -86,6 +88,7 @@ class Server {
   std::unique_ptr<Batch> batch_;
   std::mutex mu_;
   std::condition_variable cv_;
+  std::condition_variable ready_cv_;  // a worker finished building a request
   bool stop_ = false;
   std::deque<int> todo_;   // submitted, waiting for a worker
   std::deque<int> ready_;  // built, waiting for admission (FIFO)
@@ -93,6 +96,7 @@ class Server {
   std::map<int, int> by_member_;  // batch member id -> request id
   std::vector<std::thread> threads_;
   int next_ = 0;
+  int building_ = 0;  // taken by a worker, not yet ready
   size_t used_ = 0;
   long ticks_ = 0;
 };
