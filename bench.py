@@ -254,6 +254,100 @@ def run_batch(args, dca, wl):
     print(json.dumps(out))
 
 
+def run_serve(args, dca, wl):
+    """Config: a Poisson stream of seeded random models served by one Server."""
+    import torch
+
+    R = args.requests
+    reqs = [wl.random_request(args.serve_seed * 100000 + i, precision=args.precision,
+                              steps_range=(args.min_steps, args.max_steps)) for i in range(R)]
+    rng = np.random.Generator(np.random.PCG64(args.serve_seed))
+    arrivals = np.cumsum(rng.exponential(1.0 / args.qps, size=R))
+    arrivals -= arrivals[0]
+    srv = dca.Server(f"precision = {args.precision}\nmax_members = {args.max_members}\n"
+                     f"workers = {args.workers}\ngraph_iterations = {args.graph_iterations}\n", 0)
+    ids = [None] * R
+    t0 = time.perf_counter()
+    sub, last_print = 0, 0.0
+    while True:
+        now = time.perf_counter() - t0
+        while sub < R and arrivals[sub] <= now:
+            ids[sub] = srv.submit(reqs[sub].ini, reqs[sub].steps)
+            sub += 1
+        open_ = srv.tick()
+        if sub == R and open_ == 0:
+            break
+        if now - last_print > 10.0:
+            st = srv.stats()
+            print(f"# t={now:.0f}s submitted {sub}/{R} resident {st['resident']} finished {st['finished']} "
+                  f"used {st['used_bytes'] / 2**30:.1f} GiB", file=sys.stderr, flush=True)
+            last_print = now
+        if open_ == 0 and sub < R:
+            time.sleep(max(0.0, min(0.05, arrivals[sub] - (time.perf_counter() - t0))))
+    torch.cuda.synchronize()
+    infos = [srv.result(i)[1] for i in ids]
+    bad = [(reqs[k].name, x.status) for k, x in enumerate(infos) if x.status]
+    if bad:
+        raise SystemExit(f"failed requests: {bad}")
+    first = min(x.t_arrival for x in infos)
+    makespan = max(x.t_finished for x in infos) - first
+    work = sum(x.n_dof * x.steps * reqs[k].stages for k, x in enumerate(infos))
+    lat = np.array([x.t_finished - x.t_arrival for x in infos])
+    queue = np.array([x.t_admitted - x.t_arrival for x in infos])
+    print(f"# served {R} requests in {makespan:.1f}s", file=sys.stderr, flush=True)
+    # error of sampled results against single FP64 models converged to round-off
+    tight = ("model.time_step_operator.linear_solver.precision = fp64\n"
+             "model.time_step_operator.linear_solver.preconditioner.type = Jacobi\n"
+             "model.time_step_operator.nonlinear_solver.convergence_condition.relative_tolerance = 1e-20\n"
+             "model.time_step_operator.nonlinear_solver.convergence_condition.absolute_tolerance = 1e-28\n")
+    order = sorted(range(R), key=lambda k: infos[k].n_dof * infos[k].steps)
+    errs = {}
+    for k in order[: args.error_members]:
+        u, _ = srv.result(ids[k])
+        ref = dca.Model(reqs[k].ini, tight, 0)
+        ref.set_state(ref.initial_state(), 0.0)
+        t = 0.0
+        for _ in range(reqs[k].steps):
+            if ref.step(t, reqs[k].dt).status:
+                raise SystemExit("FP64 reference step failed")
+            t += reqs[k].dt
+        r, _ = ref.get_state()
+        e = []
+        for c in ref.compartments():
+            for sp in range(c["n_species"]):
+                sl = slice(c["dof_offset"] + sp, c["dof_offset"] + c["n_species"] * c["n_nodes"], c["n_species"])
+                e.append(float(np.linalg.norm(u[sl] - r[sl]) / max(np.linalg.norm(r[sl]), 1e-300)))
+        ref.close()
+        errs[reqs[k].name] = max(e)
+    srv.close()
+    out = {
+        "metric": "Newton DOF-steps/s (plain FP64 vs mixed/batched), p50/p99 latency, HBM GB/s",
+        "value": work / makespan,
+        "unit": "Newton DOF-steps/s",
+        "n_gpus": 1,
+        "steps": R,
+        "warmup": 0,
+        "ms_per_step": 1e3 * makespan / max(1, sum(x.steps for x in infos)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-krylov/f64-refinement",
+        "data": "synthetic (seeded random models and Poisson arrivals)",
+        "config": {"workload": "serving_poisson_random_mass_action_2d_64-512_3d_32-96",
+                   "requests": R, "qps": args.qps, "steps_range": [args.min_steps, args.max_steps],
+                   "precision": args.precision, "max_members": args.max_members, "workers": args.workers},
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                      "max": float(lat.max())},
+        "queue_s": {"p50": float(np.percentile(queue, 50)), "p99": float(np.percentile(queue, 99))},
+        "makespan_s": makespan,
+        "newton_dof_steps": work,
+        "error_vs_fp64": {"max_rel_l2_per_species": max(errs.values()) if errs else None, "requests": errs},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,6 +366,13 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=1, help="batch workload: profiled steps after timing")
     ap.add_argument("--error-members", type=int, default=3, help="batch workload: members checked against FP64")
     ap.add_argument("--cpu-dof-steps", type=float, default=4e6, help="batch workload: CPU baseline sample size")
+    ap.add_argument("--requests", type=int, default=16, help="serve workload: number of requests")
+    ap.add_argument("--qps", type=float, default=1.0, help="serve workload: Poisson arrival rate")
+    ap.add_argument("--serve-seed", type=int, default=1, help="serve workload: seed of models and arrivals")
+    ap.add_argument("--min-steps", type=int, default=50, help="serve workload: fewest steps of a request")
+    ap.add_argument("--max-steps", type=int, default=500, help="serve workload: most steps of a request")
+    ap.add_argument("--max-members", type=int, default=64, help="serve workload: resident request slots")
+    ap.add_argument("--workers", type=int, default=8, help="serve workload: host build/compile threads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -285,6 +386,8 @@ def main():
     wl = _load("dca_workloads", PKG / "workloads.py")
     if args.workload == "gs_batch":
         return run_batch(args, dca, wl)
+    if args.workload == "serve":
+        return run_serve(args, dca, wl)
     w, wname = make_workload(wl, args.workload, args.precision)
 
     t_setup = time.perf_counter()
