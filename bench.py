@@ -354,7 +354,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="ms96")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
+                    help="default: fp64 for single models, mixed for gs_batch and serve")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -374,6 +375,8 @@ def main():
     ap.add_argument("--max-members", type=int, default=64, help="serve workload: resident request slots")
     ap.add_argument("--workers", type=int, default=8, help="serve workload: host build/compile threads")
     args = ap.parse_args()
+    if args.precision is None:
+        args.precision = "mixed" if args.workload in ("gs_batch", "serve") else "fp64"
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         raise SystemExit(2)

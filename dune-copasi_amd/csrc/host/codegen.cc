@@ -973,8 +973,8 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   if (mode & 2) {
-    double Dv[BC][BC][DIM][DIM];
-    uniform_diffusion(t, prm, Dv);
+    double dv[NDV > 0 ? NDV : 1];
+    uniform_diffusion(t, prm, dv);
     // phase 1: Jacobian coefficients per (cell, class, point); class and point
     // are uniform per wave so the reference-element tables are scalar loads.
     // LDS layout is cell-minor: work[((p*NQ + q)*BC*BC + i*BC + j)*NCB + cell].
@@ -1015,12 +1015,16 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
     }
     __syncthreads();
     // phase 2: one BCSR block per (row, stencil offset), written once; the
-    // offset is uniform per wave, lanes run over the tile's rows.
+    // offset is uniform per wave.  Blocks up to 4 x 4: a lane per row builds
+    // the whole block; larger blocks: a lane per (row, block row), so the
+    // accumulator stays at BC registers and small tiles still fill a wave.
+    constexpr int RL = BC > 4 ? DCG_B : 1;  // lanes per row
     for (int o0 = wave; o0 < NST; o0 += nwave) {
       const int o = __builtin_amdgcn_readfirstlane(o0);
       const int dx = o % 3 - 1, dy = DIM > 1 ? (o / 3) % 3 - 1 : 0, dz = DIM > 2 ? o / 9 - 1 : 0;
       const bool diag = o == NST / 2;
-      for (int lr = lane; lr < NROW; lr += 64) {
+      for (int it = lane; it < NROW * RL; it += 64) {
+        const int lr = it / RL, ib = it % RL;
         const int li = lr % TX, lj = DIM > 1 ? (lr / TX) % TY : 0, lk = DIM > 2 ? lr / (TX * TY) : 0;
         const int vb = (li + 1) + XB * ((lj + OY) + YB * (lk + OZ));
         const int s = sl[vb];
@@ -1028,50 +1032,56 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
         const unsigned mk = own_mask[s];
         if (!((mk >> o) & 1u)) continue;
         const int pos = own_start[s] + __popc(mk & ((1u << o) - 1u));
-        double acc[BC][BC];
-        for (int i = 0; i < BC; ++i)
+        constexpr int AI = BC > 4 ? 1 : BC;  // block rows per lane
+        double acc[AI][BC];
+        for (int i = 0; i < AI; ++i)
           for (int j = 0; j < BC; ++j) acc[i][j] = 0.0;
-        for (int ez = 0; ez < (DIM > 2 ? 2 : 1); ++ez) {
-          if (DIM > 2 && ((dz == 1 && ez == 0) || (dz == -1 && ez == 1))) continue;
-          for (int ey = 0; ey < (DIM > 1 ? 2 : 1); ++ey) {
-            if (DIM > 1 && ((dy == 1 && ey == 0) || (dy == -1 && ey == 1))) continue;
-            for (int ex = 0; ex < 2; ++ex) {
-              if ((dx == 1 && ex == 0) || (dx == -1 && ex == 1)) continue;
-              const int cl = (li + ex) + CXB * ((lj + ey) + CYB * (lk + ez));
-              const unsigned m = cms[cl];
-              if (!m) continue;
-              // corner bits of v and w in this cell (uniform per wave)
-              const int cv = (1 - ex) | (DIM > 1 ? (1 - ey) << 1 : 0) | (DIM > 2 ? (1 - ez) << 2 : 0);
-              const int cw = (1 - ex + dx) | (DIM > 1 ? (1 - ey + dy) << 1 : 0) | (DIM > 2 ? (1 - ez + dz) << 2 : 0);
-              for (int p = 0; p < NCLASS; ++p) {
-                const int la = LOC[p][cv], lc = LOC[p][cw];
-                if (la < 0 || lc < 0) continue;
-                if (!((m >> p) & 1u)) continue;
-                for (int q = 0; q < NQ; ++q) {
-                  const double w = PP[q][la][lc];
-                  const double* mcp = work + (size_t)((p * NQ + q) * BC * BC) * NCB + cl;
-                  for (int i = 0; i < BC; ++i)
-                    for (int j = 0; j < BC; ++j) acc[i][j] += w * mcp[(size_t)(i * BC + j) * NCB];
-                }
-                for (int i = 0; i < BC; ++i)
-                  for (int j = 0; j < BC; ++j) {
-                    if constexpr (CONST_DIFFUSION) {
-                      acc[i][j] += ca * kd_rt(p, la, lc, i, j);
-                    } else {
-                      double kd = 0.0;
-                      for (int k = 0; k < DIM; ++k)
-                        for (int l = 0; l < DIM; ++l) kd += Dv[i][j][k][l] * KDU[p][la][lc][k][l];
-                      acc[i][j] += ca * kd;
+        const int i0 = BC > 4 ? ib : 0;  // first block row of this lane
+        if (i0 < BC) {
+          for (int ez = 0; ez < (DIM > 2 ? 2 : 1); ++ez) {
+            if (DIM > 2 && ((dz == 1 && ez == 0) || (dz == -1 && ez == 1))) continue;
+            for (int ey = 0; ey < (DIM > 1 ? 2 : 1); ++ey) {
+              if (DIM > 1 && ((dy == 1 && ey == 0) || (dy == -1 && ey == 1))) continue;
+              for (int ex = 0; ex < 2; ++ex) {
+                if ((dx == 1 && ex == 0) || (dx == -1 && ex == 1)) continue;
+                const int cl = (li + ex) + CXB * ((lj + ey) + CYB * (lk + ez));
+                const unsigned m = cms[cl];
+                if (!m) continue;
+                // corner bits of v and w in this cell (uniform per wave)
+                const int cv = (1 - ex) | (DIM > 1 ? (1 - ey) << 1 : 0) | (DIM > 2 ? (1 - ez) << 2 : 0);
+                const int cw = (1 - ex + dx) | (DIM > 1 ? (1 - ey + dy) << 1 : 0) | (DIM > 2 ? (1 - ez + dz) << 2 : 0);
+                for (int p = 0; p < NCLASS; ++p) {
+                  const int la = LOC[p][cv], lc = LOC[p][cw];
+                  if (la < 0 || lc < 0) continue;
+                  if (!((m >> p) & 1u)) continue;
+                  for (int q = 0; q < NQ; ++q) {
+                    const double w = PP[q][la][lc];
+                    const double* mcp = work + (size_t)((p * NQ + q) * BC * BC + i0 * BC) * NCB + cl;
+                    for (int i = 0; i < AI; ++i)
+                      for (int j = 0; j < BC; ++j) acc[i][j] += w * mcp[(size_t)(i * BC + j) * NCB];
+                  }
+                  if constexpr (CONST_DIFFUSION) {
+                    for (int i = 0; i < AI; ++i)
+                      for (int j = 0; j < BC; ++j) acc[i][j] += ca * kd_rt(p, la, lc, i0 + i, j);
+                  } else {
+#pragma unroll
+                    for (int n = 0; n < NDV; ++n) {
+                      const int ri = DVI[n] - i0;
+                      if (ri >= 0 && ri < AI)
+                        acc[AI == 1 ? 0 : (ri < AI ? ri : 0)][DVJ[n]] += ca * dv[n] * KDU[p][la][lc][DVK[n]][DVL[n]];
                     }
                   }
+                }
               }
             }
           }
         }
         double* blk = val + (size_t)pos * (DCG_B * DCG_B);
-        for (int i = 0; i < DCG_B; ++i)
+        for (int i = 0; i < (BC > 4 ? 1 : DCG_B); ++i) {
+          const int bi = i0 + i;
           for (int j = 0; j < DCG_B; ++j)
-            blk[i * DCG_B + j] = (i < BC && j < BC) ? acc[i < BC ? i : 0][j < BC ? j : 0] : ((diag && i == j) ? 1.0 : 0.0);
+            blk[bi * DCG_B + j] = (bi < BC && j < BC) ? acc[i < AI ? i : 0][j < BC ? j : 0] : ((diag && bi == j) ? 1.0 : 0.0);
+        }
       }
     }
     __syncthreads();
@@ -1226,13 +1236,12 @@ std::string gen_tiled(const ModelSpec& m, const Grid& g, int comp, const std::st
   for (int s : m.comp_species[comp])
     for (const auto& dt : m.species[s].diffusion) all_const = all_const && tensor_const(dt.D);
   o << "constexpr bool CONST_DIFFUSION = " << (all_const ? "true" : "false") << ";\n";
-  // values of the spatially uniform diffusion tensors D_ij
+  // nonzero entries (i, j, k, l) of the spatially uniform diffusion tensors
+  // D_ij, as a compact list: dv[n] = D_{DVI[n] DVJ[n]}[DVK[n]][DVL[n]]
   auto vars = volume_vars(m, comp, T.evol, geo);
-  o << "__device__ __forceinline__ void uniform_diffusion(double t, const double* __restrict__ prm, "
-       "double (&Dv)[BC][BC][DIM][DIM]) {\n  (void)t; (void)prm;\n";
-  o << "  for (int i = 0; i < BC; ++i) for (int j = 0; j < BC; ++j) for (int k = 0; k < DIM; ++k) "
-       "for (int l = 0; l < DIM; ++l) Dv[i][j][k][l] = 0.0;\n";
   const auto& species = m.comp_species[comp];
+  std::vector<int> di, dj, dk, dl;
+  std::vector<std::string> dexpr;
   for (size_t i = 0; i < species.size(); ++i)
     for (const auto& dt : m.species[species[i]].diffusion) {
       const int j = m.species[dt.wrt].local;
@@ -1240,12 +1249,26 @@ std::string gen_tiled(const ModelSpec& m, const Grid& g, int comp, const std::st
         for (int l = 0; l < d; ++l) {
           Ex e = dt.D.tensor ? dt.D.d[k][l] : (k == l ? dt.D.d[0][0] : nullptr);
           if (!e) continue;
-          o << "  Dv[" << i << "][" << j << "][" << k << "][" << l << "] += " << expr::emit(e, vars) << ";\n";
+          di.push_back(static_cast<int>(i));
+          dj.push_back(j);
+          dk.push_back(k);
+          dl.push_back(l);
+          std::string v = expr::emit(e, vars);
+          // unit-cell table -> cell of spacing h: |cell| / (h_k h_l)
+          if (geo) v = "(" + v + ") * prm[6] * prm[" + std::to_string(3 + k) + "] * prm[" + std::to_string(3 + l) + "]";
+          dexpr.push_back(v);
         }
     }
-  if (geo)  // unit-cell table -> cell of spacing h: |cell| / (h_k h_l)
-    o << "  for (int i = 0; i < BC; ++i) for (int j = 0; j < BC; ++j) for (int k = 0; k < DIM; ++k) "
-         "for (int l = 0; l < DIM; ++l) Dv[i][j][k][l] *= prm[6] * prm[3 + k] * prm[3 + l];\n";
+  const int ndv = static_cast<int>(di.size());
+  o << "constexpr int NDV = " << ndv << ";\n";
+  auto ints = [&](const std::vector<int>& v) { return v.empty() ? std::string("{0}") : int_init(v); };
+  o << "__device__ constexpr int DVI[NDV > 0 ? NDV : 1] = " << ints(di) << ";\n";
+  o << "__device__ constexpr int DVJ[NDV > 0 ? NDV : 1] = " << ints(dj) << ";\n";
+  o << "__device__ constexpr int DVK[NDV > 0 ? NDV : 1] = " << ints(dk) << ";\n";
+  o << "__device__ constexpr int DVL[NDV > 0 ? NDV : 1] = " << ints(dl) << ";\n";
+  o << "__device__ __forceinline__ void uniform_diffusion(double t, const double* __restrict__ prm, "
+       "double (&dv)[NDV > 0 ? NDV : 1]) {\n  (void)t; (void)prm; dv[0] = 0.0;\n";
+  for (int n = 0; n < ndv; ++n) o << "  dv[" << n << "] = " << dexpr[n] << ";\n";
   o << "}\n";
   o << "}  // namespace " << ns << "\n";
   std::string body = kTiled;

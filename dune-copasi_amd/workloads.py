@@ -280,7 +280,7 @@ outer.expression = (position_x-0.5)^2 + (position_y-0.5)^2 >= 0.09
 
 
 # ------------------------------------------------------------- serving requests
-def random_request(seed: int, precision="mixed", nl_rtol=1e-16, nl_atol=0.0, side2=(64, 512), side3=(32, 96),
+def random_request(seed: int, precision="mixed", nl_rtol=1e-20, nl_atol=0.0, side2=(64, 512), side3=(32, 96),
                    steps_range=(50, 500), dt=0.05) -> Workload:
     """One seeded random model of the serving stream (benchmark config 4):
     1 or 2 compartments (inner disk / ball of radius 0.3 and its complement)

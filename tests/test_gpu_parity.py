@@ -98,6 +98,9 @@ CASES = [
     ("ms_p1", lambda wl: wl.mitchell_schaefer(n=5, cell_type="simplex")),
     ("two_q1", lambda wl: wl.two_compartments(n=20, cell_type="cube")),
     ("two_p1", lambda wl: wl.two_compartments(n=16, cell_type="simplex")),
+    # random mass-action requests with large blocks (block rows split over lanes)
+    ("req_3d_8sp_2c", lambda wl: wl.random_request(4, side3=(5, 6), steps_range=(3, 3))),
+    ("req_2d_6sp_1c", lambda wl: wl.random_request(11, side2=(10, 12), steps_range=(3, 3))),
 ]
 
 
@@ -164,6 +167,10 @@ STEP_CASES = [
                                               lin_rtol=1e-12), 6),
     ("two_q1", lambda wl: wl.two_compartments(n=24, cell_type="cube", nl_rtol=1e-20, nl_atol=1e-26,
                                               lin_rtol=1e-12), 6),
+    ("req_3d_8sp_2c", lambda wl: wl.random_request(4, side3=(5, 6), nl_rtol=1e-20, nl_atol=1e-28,
+                                                   precision="fp64"), 3),
+    ("req_2d_6sp_1c", lambda wl: wl.random_request(11, side2=(10, 12), nl_rtol=1e-20, nl_atol=1e-28,
+                                                   precision="fp64"), 3),
 ]
 
 
