@@ -1,5 +1,6 @@
 #include "codegen.hh"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <sstream>
@@ -173,6 +174,30 @@ bool tensor_uniform(const TensorTerm& t) {
         if (v != "time" && v.rfind("$", 0) != 0) return false;
     }
   return true;
+}
+
+// (i, j) entries of the point coefficient block a compartment's point
+// function writes: storage diagonal, storage and reaction Jacobians
+std::vector<std::pair<int, int>> mc_pattern(const ModelSpec& m, int comp) {
+  std::vector<std::pair<int, int>> nz;
+  auto add = [&](int i, int j) {
+    if (std::find(nz.begin(), nz.end(), std::make_pair(i, j)) == nz.end()) nz.push_back({i, j});
+  };
+  const auto& species = m.comp_species[comp];
+  for (size_t i = 0; i < species.size(); ++i) {
+    const auto& sp = m.species[species[i]];
+    const int I = static_cast<int>(i);
+    if (sp.storage) {
+      add(I, I);
+      for (const auto& pj : sp.storage_jac)
+        if (m.species[pj.wrt].comp == comp) add(I, m.species[pj.wrt].local);
+    }
+    if (sp.reaction)
+      for (const auto& pj : sp.reaction_jac)
+        if (m.species[pj.wrt].comp == comp) add(I, m.species[pj.wrt].local);
+  }
+  std::sort(nz.begin(), nz.end());
+  return nz;
 }
 
 const char* kPrelude = R"(
@@ -672,12 +697,25 @@ std::string gen_volume(const ModelSpec& m, const Grid& g, int comp, const std::s
   }
   // quadrature point kernel
   o << "__device__ __forceinline__ void qp(int q, const double (&u)[BC], const double (&g)[BC][3], double px, double py, "
-       "double pz, double t, double cm, double ca, double (&rs)[BC], double (&rf)[BC][DIM], double (*mc)[BC], "
-       "double (*gc)[BC][DIM], double (*dc)[BC][DIM][DIM], const double* __restrict__ prm) {\n";
+       "double pz, double t, double cm, double ca, double (&rs)[BC], double (&rf)[BC][DIM], "
+    << (with_kernel ? "double (*mc)[BC], " : "double* mc, ")
+    << "double (*gc)[BC][DIM], double (*dc)[BC][DIM][DIM], const double* __restrict__ prm) {\n";
   o << "  (void)px; (void)py; (void)pz; (void)t; (void)g; (void)gc; (void)dc; (void)prm;\n";
   o << "  const double fac = FACQ(q);\n";
   o << "#pragma unroll\n  for (int i = 0; i < BC; ++i) { rs[i] = 0.0;\n#pragma unroll\n for (int k = 0; k < DIM; ++k) rf[i][k] = 0.0; }\n";
-  o << "  if (mc) {\n#pragma unroll\n  for (int i = 0; i < BC; ++i)\n#pragma unroll\n for (int j = 0; j < BC; ++j) mc[i][j] = 0.0;}\n";
+  // tiled compartments keep only the structural nonzeros of mc, in the
+  // order of mc_pattern (the tiled kernel's MCI / MCJ tables)
+  const auto mcnz = mc_pattern(m, comp);
+  auto MC = [&](int i, int j) {
+    if (with_kernel) return "mc[" + std::to_string(i) + "][" + std::to_string(j) + "]";
+    for (size_t n = 0; n < mcnz.size(); ++n)
+      if (mcnz[n].first == i && mcnz[n].second == j) return "mc[" + std::to_string(n) + "]";
+    throw Error("internal: coefficient outside the point pattern");
+  };
+  if (with_kernel)
+    o << "  if (mc) {\n#pragma unroll\n  for (int i = 0; i < BC; ++i)\n#pragma unroll\n for (int j = 0; j < BC; ++j) mc[i][j] = 0.0;}\n";
+  else if (!mcnz.empty())
+    o << "  if (mc) {\n#pragma unroll\n  for (int n = 0; n < " << mcnz.size() << "; ++n) mc[n] = 0.0;}\n";
   if (has_gc)
     o << "  if (gc) {\n#pragma unroll\n for (int i = 0; i < BC; ++i)\n#pragma unroll\n for (int j = 0; j < BC; ++j)\n#pragma unroll\n for (int k = 0; k < DIM; ++k) gc[i][j][k] = 0.0;}\n";
   if (has_dc)
@@ -689,10 +727,10 @@ std::string gen_volume(const ModelSpec& m, const Grid& g, int comp, const std::s
     if (sp.storage) {
       o << "    const double S = " << E(sp.storage) << ";\n";
       o << "    rs[" << I << "] += fac * cm * S * u[" << I << "];\n";
-      o << "    if (mc) { mc[" << I << "][" << I << "] += fac * cm * S;\n";
+      o << "    if (mc) { " << MC(i, i) << " += fac * cm * S;\n";
       for (const auto& pj : sp.storage_jac) {
         if (m.species[pj.wrt].comp != comp) continue;
-        o << "      mc[" << I << "][" << m.species[pj.wrt].local << "] += fac * cm * (" << E(pj.e) << ") * u[" << I << "];\n";
+        o << "      " << MC(i, m.species[pj.wrt].local) << " += fac * cm * (" << E(pj.e) << ") * u[" << I << "];\n";
       }
       o << "    }\n";
     }
@@ -701,7 +739,7 @@ std::string gen_volume(const ModelSpec& m, const Grid& g, int comp, const std::s
       o << "    if (mc) {\n";
       for (const auto& pj : sp.reaction_jac) {
         if (m.species[pj.wrt].comp != comp) continue;
-        o << "      mc[" << I << "][" << m.species[pj.wrt].local << "] -= fac * ca * (" << E(pj.e) << ");\n";
+        o << "      " << MC(i, m.species[pj.wrt].local) << " -= fac * ca * (" << E(pj.e) << ");\n";
       }
       o << "    }\n";
     }
@@ -901,7 +939,7 @@ constexpr int CXB = TX + 1, CYB = DIM > 1 ? TY + 1 : 1, CZB = DIM > 2 ? TZ + 1 :
 constexpr int NVB = XB * YB * ZB, NCB = CXB * CYB * CZB, NROW = TX * TY * TZ;
 constexpr int NST = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
 constexpr int OY = DIM > 1 ? 1 : 0, OZ = DIM > 2 ? 1 : 0;
-constexpr int MC_WORDS = NCB * NCLASS * NQ * BC * BC;
+constexpr int MC_WORDS = NCB * NCLASS * NQ * (NMC > 0 ? NMC : 1);
 constexpr int RL_WORDS = NCB * NCORNER * BC;
 constexpr int UNION_WORDS = MC_WORDS > RL_WORDS ? MC_WORDS : RL_WORDS;
 
@@ -935,6 +973,47 @@ __device__ __forceinline__ void residual_rt(int p, const double (&xl)[NV][BC], d
         for (int k = 0; k < DIM; ++k) s += rf[i][k] * GP(p, q, a, k);
         rl[a][i] += s;
       }
+  }
+}
+
+// element residual of class p accumulated into dst[corner * BC + i], the
+// corner values read from the staged box xs at bidx[a] (for large blocks,
+// which would otherwise hold two NV x BC arrays in registers)
+__device__ __forceinline__ void residual_lds(int p, const double* __restrict__ xs, const int (&bidx)[NV], double cx,
+                                             double cy, double cz, double t, double cm, double ca,
+                                             double* __restrict__ dst, const double* __restrict__ prm) {
+  // one point at a time, the corner values re-read from LDS for each point
+  // (the empty asm keeps the compiler from holding them in registers)
+#pragma unroll 1
+  for (int q = 0; q < NQ; ++q) {
+    __asm__ volatile("" ::: "memory");
+    double u[BC], g[BC][3];
+    for (int i = 0; i < BC; ++i) {
+      u[i] = 0.0;
+      g[i][0] = g[i][1] = g[i][2] = 0.0;
+    }
+    for (int a = 0; a < NV; ++a) {
+      const double pa = PHI[q][a], ga0 = GP(p, q, a, 0);
+      const double ga1 = DIM > 1 ? GP(p, q, a, DIM > 1 ? 1 : 0) : 0.0, ga2 = DIM > 2 ? GP(p, q, a, DIM > 2 ? 2 : 0) : 0.0;
+      for (int i = 0; i < BC; ++i) {
+        const double xv = xs[bidx[a] * BC + i];
+        u[i] += pa * xv;
+        g[i][0] += ga0 * xv;
+        if (DIM > 1) g[i][1] += ga1 * xv;
+        if (DIM > 2) g[i][2] += ga2 * xv;
+      }
+    }
+    const double px = cx + XQP(p, q, 0), py = cy + XQP(p, q, 1), pz = cz + XQP(p, q, 2);
+    double rs[BC], rf[BC][DIM];
+    qp(q, u, g, px, py, pz, t, cm, ca, rs, rf, nullptr, nullptr, nullptr, prm);
+    for (int a = 0; a < NV; ++a) {
+      double* d = dst + CORNER[p][a] * BC;
+      for (int i = 0; i < BC; ++i) {
+        double v = rs[i] * PHI[q][a];
+        for (int k = 0; k < DIM; ++k) v += rf[i][k] * GP(p, q, a, k);
+        d[i] += v;
+      }
+    }
   }
 }
 
@@ -977,40 +1056,40 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
     uniform_diffusion(t, prm, dv);
     // phase 1: Jacobian coefficients per (cell, class, point); class and point
     // are uniform per wave so the reference-element tables are scalar loads.
-    // LDS layout is cell-minor: work[((p*NQ + q)*BC*BC + i*BC + j)*NCB + cell].
+    // Only the structural nonzeros (MCI[n], MCJ[n]) of the point coefficient
+    // block are kept; LDS layout is cell-minor: work[((p*NQ + q)*NMC + n)*NCB + cell].
     for (int pq0 = wave; pq0 < NCLASS * NQ; pq0 += nwave) {
       const int pq = __builtin_amdgcn_readfirstlane(pq0);
       const int p = pq / NQ, q = pq % NQ;
       for (int cl = lane; cl < NCB; cl += 64) {
         if (!((cms[cl] >> p) & 1u)) continue;
         const int ci = cl % CXB, cj = (cl / CXB) % CYB, ck = cl / (CXB * CYB);
-        double xl[NV][BC];
+        // value and gradient at the point, straight from the staged corners
+        double u[BC], g[BC][3];
+        for (int i = 0; i < BC; ++i) {
+          u[i] = 0.0;
+          g[i][0] = g[i][1] = g[i][2] = 0.0;
+        }
         for (int a = 0; a < NV; ++a) {
           const int c = CORNER[p][a];
           const int b = (ci + (c & 1)) + XB * ((cj + ((c >> 1) & 1)) + YB * (ck + ((c >> 2) & 1)));
-          for (int i = 0; i < BC; ++i) xl[a][i] = xs[b * BC + i];
-        }
-        double u[BC], g[BC][3];
-        for (int i = 0; i < BC; ++i) {
-          double s = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
-          for (int a = 0; a < NV; ++a) {
-            s += PHI[q][a] * xl[a][i];
-            g0 += GP(p, q, a, 0) * xl[a][i];
-            if (DIM > 1) g1 += GP(p, q, a, DIM > 1 ? 1 : 0) * xl[a][i];
-            if (DIM > 2) g2 += GP(p, q, a, DIM > 2 ? 2 : 0) * xl[a][i];
+          const double pa = PHI[q][a], ga0 = GP(p, q, a, 0);
+          const double ga1 = DIM > 1 ? GP(p, q, a, DIM > 1 ? 1 : 0) : 0.0, ga2 = DIM > 2 ? GP(p, q, a, DIM > 2 ? 2 : 0) : 0.0;
+          for (int i = 0; i < BC; ++i) {
+            const double xv = xs[b * BC + i];
+            u[i] += pa * xv;
+            g[i][0] += ga0 * xv;
+            if (DIM > 1) g[i][1] += ga1 * xv;
+            if (DIM > 2) g[i][2] += ga2 * xv;
           }
-          u[i] = s;
-          g[i][0] = g0;
-          g[i][1] = g1;
-          g[i][2] = g2;
         }
         const double px = ox + (vi0 + ci) * hx + XQP(p, q, 0);
         const double py = oy + (vj0 + cj) * hy + XQP(p, q, 1);
         const double pz = oz + (vk0 + ck) * hz + XQP(p, q, 2);
-        double rs[BC], rf[BC][DIM], mc[BC][BC];
+        double rs[BC], rf[BC][DIM], mc[NMC > 0 ? NMC : 1];
         qp(q, u, g, px, py, pz, t, cm, ca, rs, rf, mc, nullptr, nullptr, prm);
-        for (int i = 0; i < BC; ++i)
-          for (int j = 0; j < BC; ++j) work[(size_t)(pq * BC * BC + i * BC + j) * NCB + cl] = mc[i][j];
+#pragma unroll
+        for (int n = 0; n < NMC; ++n) work[(size_t)(pq * NMC + n) * NCB + cl] = mc[n];
       }
     }
     __syncthreads();
@@ -1034,7 +1113,9 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
         const int pos = own_start[s] + __popc(mk & ((1u << o) - 1u));
         constexpr int AI = BC > 4 ? 1 : BC;  // block rows per lane
         double acc[AI][BC];
+#pragma unroll
         for (int i = 0; i < AI; ++i)
+#pragma unroll
           for (int j = 0; j < BC; ++j) acc[i][j] = 0.0;
         const int i0 = BC > 4 ? ib : 0;  // first block row of this lane
         if (i0 < BC) {
@@ -1056,12 +1137,17 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
                   if (!((m >> p) & 1u)) continue;
                   for (int q = 0; q < NQ; ++q) {
                     const double w = PP[q][la][lc];
-                    const double* mcp = work + (size_t)((p * NQ + q) * BC * BC + i0 * BC) * NCB + cl;
-                    for (int i = 0; i < AI; ++i)
-                      for (int j = 0; j < BC; ++j) acc[i][j] += w * mcp[(size_t)(i * BC + j) * NCB];
+                    const double* mcp = work + (size_t)((p * NQ + q) * NMC) * NCB + cl;
+#pragma unroll
+                    for (int n = 0; n < NMC; ++n) {
+                      const int ri = MCI[n] - i0;
+                      if (ri >= 0 && ri < AI) acc[AI == 1 ? 0 : (ri < AI ? ri : 0)][MCJ[n]] += w * mcp[(size_t)n * NCB];
+                    }
                   }
                   if constexpr (CONST_DIFFUSION) {
+#pragma unroll
                     for (int i = 0; i < AI; ++i)
+#pragma unroll
                       for (int j = 0; j < BC; ++j) acc[i][j] += ca * kd_rt(p, la, lc, i0 + i, j);
                   } else {
 #pragma unroll
@@ -1077,8 +1163,10 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
           }
         }
         double* blk = val + (size_t)pos * (DCG_B * DCG_B);
+#pragma unroll
         for (int i = 0; i < (BC > 4 ? 1 : DCG_B); ++i) {
           const int bi = i0 + i;
+#pragma unroll
           for (int j = 0; j < DCG_B; ++j)
             blk[bi * DCG_B + j] = (bi < BC && j < BC) ? acc[i < AI ? i : 0][j < BC ? j : 0] : ((diag && bi == j) ? 1.0 : 0.0);
         }
@@ -1097,15 +1185,24 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
       const double cx = ox + (vi0 + ci) * hx, cy = oy + (vj0 + cj) * hy, cz = oz + (vk0 + ck) * hz;
       for (int p = 0; p < NCLASS; ++p) {
         if (!((m >> p) & 1u)) continue;
-        double xl[NV][BC], rl[NV][BC];
-        for (int a = 0; a < NV; ++a) {
-          const int c = CORNER[p][a];
-          const int b = (ci + (c & 1)) + XB * ((cj + ((c >> 1) & 1)) + YB * (ck + ((c >> 2) & 1)));
-          for (int i = 0; i < BC; ++i) xl[a][i] = xs[b * BC + i];
+        if constexpr (BC > 4) {
+          int bidx[NV];
+          for (int a = 0; a < NV; ++a) {
+            const int c = CORNER[p][a];
+            bidx[a] = (ci + (c & 1)) + XB * ((cj + ((c >> 1) & 1)) + YB * (ck + ((c >> 2) & 1)));
+          }
+          residual_lds(p, xs, bidx, cx, cy, cz, t, cm, ca, dst, prm);
+        } else {
+          double xl[NV][BC], rl[NV][BC];
+          for (int a = 0; a < NV; ++a) {
+            const int c = CORNER[p][a];
+            const int b = (ci + (c & 1)) + XB * ((cj + ((c >> 1) & 1)) + YB * (ck + ((c >> 2) & 1)));
+            for (int i = 0; i < BC; ++i) xl[a][i] = xs[b * BC + i];
+          }
+          residual_rt(p, xl, cx, cy, cz, t, cm, ca, rl, prm);
+          for (int a = 0; a < NV; ++a)
+            for (int i = 0; i < BC; ++i) dst[CORNER[p][a] * BC + i] += rl[a][i];
         }
-        residual_rt(p, xl, cx, cy, cz, t, cm, ca, rl, prm);
-        for (int a = 0; a < NV; ++a)
-          for (int i = 0; i < BC; ++i) dst[CORNER[p][a] * BC + i] += rl[a][i];
       }
     }
     __syncthreads();
@@ -1178,7 +1275,7 @@ bool tile_eligible(const ModelSpec& m, int comp) {
   return true;
 }
 
-std::array<int, 3> choose_tile(const Grid& g, int bc) {
+std::array<int, 3> choose_tile(const Grid& g, int bc, int nmc) {
   // largest candidate whose LDS (staging + coefficient union) stays under 48 KiB
   std::vector<std::array<int, 3>> cand;
   if (g.dim == 3) cand = {{8, 4, 4}, {4, 4, 4}, {4, 4, 2}, {4, 2, 2}, {2, 2, 2}};
@@ -1189,7 +1286,7 @@ std::array<int, 3> choose_tile(const Grid& g, int bc) {
     long xb = c[0] + 2, yb = g.dim > 1 ? c[1] + 2 : 1, zb = g.dim > 2 ? c[2] + 2 : 1;
     long cxb = c[0] + 1, cyb = g.dim > 1 ? c[1] + 1 : 1, czb = g.dim > 2 ? c[2] + 1 : 1;
     long ncb = cxb * cyb * czb;
-    long mc = ncb * g.nclass * nq * bc * bc, rl = ncb * g.ncorner * bc;
+    long mc = ncb * g.nclass * nq * std::max(1, nmc), rl = ncb * g.ncorner * bc;
     long bytes = 8 * std::max(mc, rl) + 8 * xb * yb * zb * bc + 4 * xb * yb * zb + ncb;
     if (bytes <= 48 * 1024) return c;
   }
@@ -1216,6 +1313,17 @@ std::string gen_tiled(const ModelSpec& m, const Grid& g, int comp, const std::st
       loc.push_back(l);
     }
   o << "__device__ constexpr int LOC[NCLASS][NCORNER] = " << int_init(loc) << ";\n";
+  // structural nonzeros of the point coefficient block mc (mass and reaction
+  // Jacobian entries the point function writes)
+  const auto nz = mc_pattern(m, comp);
+  std::vector<int> mi, mj;
+  for (auto [i, j] : nz) {
+    mi.push_back(i);
+    mj.push_back(j);
+  }
+  o << "constexpr int NMC = " << nz.size() << ";\n";
+  o << "__device__ constexpr int MCI[NMC > 0 ? NMC : 1] = " << (mi.empty() ? std::string("{0}") : int_init(mi)) << ";\n";
+  o << "__device__ constexpr int MCJ[NMC > 0 ? NMC : 1] = " << (mj.empty() ? std::string("{0}") : int_init(mj)) << ";\n";
   // unit stiffness tables: sum_q FAC grad_k phi_a grad_l phi_c per class
   const int d = g.dim;
   std::vector<double> kdu;
@@ -1305,7 +1413,8 @@ GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L, bool r
     src << gen_volume(spec, L.grid, static_cast<int>(c), ns, kn, numerical, geo, !tiled);
     out.volume_kernel.push_back(tiled ? "" : kn);
     if (tiled) {
-      auto tile = choose_tile(L.grid, static_cast<int>(spec.comp_species[c].size()));
+      auto tile = choose_tile(L.grid, static_cast<int>(spec.comp_species[c].size()),
+                              static_cast<int>(mc_pattern(spec, static_cast<int>(c)).size()));
       std::string tk = "dcg_tile_" + std::to_string(c);
       src << gen_tiled(spec, L.grid, static_cast<int>(c), ns, tk, tile, geo);
       out.tile_kernel.push_back(tk);
