@@ -128,6 +128,41 @@ def cpu_baseline_batch(ws, dof_steps_budget: float):
                       f"OpenMP-coloured assembly, sequential BiCGSTAB)"}
 
 
+def plain_fp64(dca, items, steps):
+    """The plain FP64 path on the same models, one model at a time: each model
+    built alone (FP64 BiCGSTAB + block-Jacobi), one untimed step, then `steps`
+    timed steps.  items: (ini, dt, stages, total steps the workload runs).
+    Returns Newton DOF-steps/s of the whole workload at the measured per-step
+    times and the sampled DOF-steps and seconds."""
+    import torch
+
+    ov = ("model.time_step_operator.linear_solver.precision = fp64\n"
+          "model.time_step_operator.linear_solver.preconditioner.type = Jacobi\n")
+    work, secs, sampled_work, sampled_secs = 0.0, 0.0, 0.0, 0.0
+    for ini, dt, stages, total in items:
+        m = dca.Model(ini, ov, 0, runtime_parameters=True)
+        m.set_state(m.initial_state(), 0.0)
+        t = 0.0
+        if m.step(t, dt).status:
+            raise SystemExit("plain FP64 step failed")
+        t += dt
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            if m.step(t, dt).status:
+                raise SystemExit("plain FP64 step failed")
+            t += dt
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / steps
+        work += m.n_dof * stages * total
+        secs += el * total
+        sampled_work += m.n_dof * stages * steps
+        sampled_secs += el * steps
+        m.close()
+    return {"value": work / secs, "unit": "Newton DOF-steps/s", "models": len(items), "timed_steps_per_model": steps,
+            "sampled_dof_steps": sampled_work, "sampled_seconds": sampled_secs}
+
+
 def run_batch(args, dca, wl):
     """Config: 256 independent Gray-Scott instances advanced by one Batch."""
     import torch
@@ -222,6 +257,11 @@ def run_batch(args, dca, wl):
         errs[w.name] = max(e)
     cpu = None if args.no_cpu_baseline else cpu_baseline_batch(ws, args.cpu_dof_steps)
     value = args.steps * n_dof / elapsed
+    plain = None
+    if args.plain_steps > 0:
+        print("# plain FP64 path on the same models ...", file=sys.stderr, flush=True)
+        plain = plain_fp64(dca, [(w.ini, w.dt, 1, args.steps) for w in ws], args.plain_steps)
+        plain["speedup_of_this_line"] = value / plain["value"]
     out = {
         "metric": "Newton DOF-steps/s (plain FP64 vs mixed/batched), p50/p99 latency, HBM GB/s",
         "value": value,
@@ -247,6 +287,7 @@ def run_batch(args, dca, wl):
                           "steps": total_steps},
         "setup_s": setup_s,
         "device_bytes": b.device_bytes(),
+        "plain_fp64_same_models": plain,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -320,6 +361,11 @@ def run_serve(args, dca, wl):
         ref.close()
         errs[reqs[k].name] = max(e)
     srv.close()
+    plain = None
+    if args.plain_steps > 0:
+        print("# plain FP64 path on the same requests ...", file=sys.stderr, flush=True)
+        plain = plain_fp64(dca, [(r.ini, r.dt, r.stages, r.steps) for r in reqs], args.plain_steps)
+        plain["speedup_of_this_line"] = (work / makespan) / plain["value"]
     out = {
         "metric": "Newton DOF-steps/s (plain FP64 vs mixed/batched), p50/p99 latency, HBM GB/s",
         "value": work / makespan,
@@ -342,6 +388,7 @@ def run_serve(args, dca, wl):
         "makespan_s": makespan,
         "newton_dof_steps": work,
         "error_vs_fp64": {"max_rel_l2_per_species": max(errs.values()) if errs else None, "requests": errs},
+        "plain_fp64_same_requests": plain,
         "roofline": None,
         "cpu_baseline": None,
     }
@@ -367,6 +414,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=1, help="batch workload: profiled steps after timing")
     ap.add_argument("--error-members", type=int, default=3, help="batch workload: members checked against FP64")
     ap.add_argument("--cpu-dof-steps", type=float, default=4e6, help="batch workload: CPU baseline sample size")
+    ap.add_argument("--plain-steps", type=int, default=1,
+                    help="gs_batch / serve: timed steps per model of the plain FP64 path on the same models (0: skip)")
     ap.add_argument("--requests", type=int, default=16, help="serve workload: number of requests")
     ap.add_argument("--qps", type=float, default=1.0, help="serve workload: Poisson arrival rate")
     ap.add_argument("--serve-seed", type=int, default=1, help="serve workload: seed of models and arrivals")
