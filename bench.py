@@ -131,7 +131,8 @@ def cpu_baseline_batch(ws, dof_steps_budget: float):
 def plain_fp64(dca, items, steps):
     """The plain FP64 path on the same models, one model at a time: each model
     built alone (FP64 BiCGSTAB + block-Jacobi), one untimed step, then `steps`
-    timed steps.  items: (ini, dt, stages, total steps the workload runs).
+    timed steps.  items: (ini, dt, stages, total steps the workload runs,
+    initial state of a model handle -> packed state).
     Returns Newton DOF-steps/s of the whole workload at the measured per-step
     times and the sampled DOF-steps and seconds."""
     import torch
@@ -139,9 +140,9 @@ def plain_fp64(dca, items, steps):
     ov = ("model.time_step_operator.linear_solver.precision = fp64\n"
           "model.time_step_operator.linear_solver.preconditioner.type = Jacobi\n")
     work, secs, sampled_work, sampled_secs = 0.0, 0.0, 0.0, 0.0
-    for ini, dt, stages, total in items:
+    for ini, dt, stages, total, init in items:
         m = dca.Model(ini, ov, 0, runtime_parameters=True)
-        m.set_state(m.initial_state(), 0.0)
+        m.set_state(init(m), 0.0)
         t = 0.0
         if m.step(t, dt).status:
             raise SystemExit("plain FP64 step failed")
@@ -260,7 +261,7 @@ def run_batch(args, dca, wl):
     plain = None
     if args.plain_steps > 0:
         print("# plain FP64 path on the same models ...", file=sys.stderr, flush=True)
-        plain = plain_fp64(dca, [(w.ini, w.dt, 1, args.steps) for w in ws], args.plain_steps)
+        plain = plain_fp64(dca, [(w.ini, w.dt, 1, args.steps, w.initial_state) for w in ws], args.plain_steps)
         plain["speedup_of_this_line"] = value / plain["value"]
     out = {
         "metric": "Newton DOF-steps/s (plain FP64 vs mixed/batched), p50/p99 latency, HBM GB/s",
@@ -364,7 +365,8 @@ def run_serve(args, dca, wl):
     plain = None
     if args.plain_steps > 0:
         print("# plain FP64 path on the same requests ...", file=sys.stderr, flush=True)
-        plain = plain_fp64(dca, [(r.ini, r.dt, r.stages, r.steps) for r in reqs], args.plain_steps)
+        plain = plain_fp64(dca, [(r.ini, r.dt, r.stages, r.steps, lambda m: m.initial_state()) for r in reqs],
+                           args.plain_steps)
         plain["speedup_of_this_line"] = (work / makespan) / plain["value"]
     out = {
         "metric": "Newton DOF-steps/s (plain FP64 vs mixed/batched), p50/p99 latency, HBM GB/s",
