@@ -407,6 +407,8 @@ def main():
                     help="default: fp64 for single models, mixed for gs_batch and serve")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="single-model FP64 runs: skip the mixed-precision run of the same workload")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this size (counter calibration under rocprofv3)")
     ap.add_argument("--members", type=int, default=256, help="batch workload: number of models")
@@ -499,6 +501,42 @@ def main():
         probe = {"bytes_each_way": nbytes, "ms": ms, "gbs_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}
 
     value = args.steps * w.stages * m.n_dof / elapsed
+    mixed = None
+    if args.precision == "fp64" and not args.no_mixed:
+        # the FP32-Krylov / FP64-refinement form on the same workload and steps,
+        # with its error against this FP64 run
+        u64, _ = m.get_state()
+        w32, _ = make_workload(wl, args.workload, "mixed")
+        m32 = dca.Model(w32.ini, "", 0)
+        m32.set_state(u0, 0.0)
+        t32 = 0.0
+        for k in range(args.warmup):
+            if m32.step(t32, w32.dt).status:
+                raise SystemExit("mixed warmup step failed")
+            t32 += w32.dt
+        torch.cuda.synchronize()
+        s0 = time.perf_counter()
+        refs = 0
+        for k in range(args.steps):
+            st = m32.step(t32, w32.dt)
+            if st.status:
+                raise SystemExit("mixed step failed")
+            refs += st.refinement_steps
+            t32 += w32.dt
+        torch.cuda.synchronize()
+        el32 = time.perf_counter() - s0
+        u32, _ = m32.get_state()
+        errs = []
+        for c in m.compartments():
+            for sp in range(c["n_species"]):
+                sl = slice(c["dof_offset"] + sp, c["dof_offset"] + c["n_species"] * c["n_nodes"], c["n_species"])
+                errs.append(float(np.linalg.norm(u32[sl] - u64[sl]) / max(np.linalg.norm(u64[sl]), 1e-300)))
+        mixed = {"value": args.steps * w32.stages * m32.n_dof / el32, "ms_per_step": 1e3 * el32 / args.steps,
+                 "speedup_vs_fp64": (args.steps * w32.stages * m32.n_dof / el32) / value,
+                 "refinements_per_step": refs / args.steps,
+                 "error_vs_fp64_run": {"max_rel_l2_per_species": max(errs), "steps": args.warmup + args.steps}}
+        m32.close()
+        print(f"# mixed {mixed['value']:.3e} DOF-steps/s, error vs FP64 {max(errs):.2e}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu_baseline:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
@@ -524,6 +562,7 @@ def main():
         "kernels": kinds,
         "roofline": roofline,
         "spmv_autotune": {"candidates_us": tuning, "chosen": chosen},
+        "mixed_precision_same_workload": mixed,
         "copy_probe": probe,
         "cpu_baseline": cpu,
     }
