@@ -1093,11 +1093,134 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
       }
     }
     __syncthreads();
-    // phase 2: one BCSR block per (row, stencil offset), written once; the
-    // offset is uniform per wave.  Blocks up to 4 x 4: a lane per row builds
-    // the whole block; larger blocks: a lane per (row, block row), so the
-    // accumulator stays at BC registers and small tiles still fill a wave.
-    constexpr int RL = BC > 4 ? DCG_B : 1;  // lanes per row
+    if constexpr (BC <= 4) {
+    // phase 2 (blocks up to 4 x 4): SPL lanes per row, split over the last
+    // axis of the cells around the row vertex.  A lane reads each of its
+    // cells' staged point coefficients once and adds them into all of the
+    // row's stencil blocks it touches; every index into the accumulator is a
+    // compile-time constant.  Slots along the split axis are the corner bit
+    // (2 slots) instead of the offset (3 slots); the two lanes of a row then
+    // exchange the shared middle plane.  Each block is written exactly once.
+    constexpr int SPL = (DIM > 1 && NROW * 2 <= 256) ? 2 : 1;
+    constexpr int NCELL = DIM == 1 ? 2 : DIM == 2 ? 4 : 8;
+    constexpr int SA = DIM - 1;  // split axis
+    constexpr int SX = 3, SY = DIM > 1 ? ((SPL == 2 && SA == 1) ? 2 : 3) : 1, SZ = DIM > 2 ? (SPL == 2 ? 2 : 3) : 1;
+    for (int it = threadIdx.x; it < NROW * SPL; it += blockDim.x) {
+      const int lr = it / SPL, half = it % SPL;
+      const int li = lr % TX, lj = DIM > 1 ? (lr / TX) % TY : 0, lk = DIM > 2 ? lr / (TX * TY) : 0;
+      const int vb = (li + 1) + XB * ((lj + OY) + YB * (lk + OZ));
+      const int s = sl[vb];
+      if (s < 0) continue;  // both lanes of a row leave together
+      double acc[SZ][SY][SX][NZB > 0 ? NZB : 1];
+#pragma unroll
+      for (int a2 = 0; a2 < SZ; ++a2)
+#pragma unroll
+        for (int a1 = 0; a1 < SY; ++a1)
+#pragma unroll
+          for (int a0 = 0; a0 < SX; ++a0)
+#pragma unroll
+            for (int z = 0; z < (NZB > 0 ? NZB : 1); ++z) acc[a2][a1][a0][z] = 0.0;
+#pragma unroll
+      for (int e = 0; e < NCELL; ++e) {
+        const int ex = e & 1, ey = DIM > 1 ? (e >> 1) & 1 : 0, ez = DIM > 2 ? (e >> 2) & 1 : 0;
+        if (SPL == 2 && (SA == 2 ? ez : ey) != half) continue;
+        const int cl = (li + ex) + CXB * ((lj + ey) + CYB * (lk + ez));
+        const unsigned m = cms[cl];
+        if (!m) continue;
+        // corner of the row vertex in this cell
+        const int cv = (1 - ex) | (DIM > 1 ? (1 - ey) << 1 : 0) | (DIM > 2 ? (1 - ez) << 2 : 0);
+#pragma unroll
+        for (int p = 0; p < NCLASS; ++p) {
+          const int la = LOC[p][cv];
+          if (la < 0) continue;
+          if (!((m >> p) & 1u)) continue;
+          double mv[NQ][NMC > 0 ? NMC : 1];
+#pragma unroll
+          for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int n = 0; n < NMC; ++n) mv[q][n] = work[(size_t)((p * NQ + q) * NMC + n) * NCB + cl];
+#pragma unroll
+          for (int c = 0; c < NV; ++c) {
+            const int cw = CORNER[p][c];
+            const int bx = cw & 1, by = (cw >> 1) & 1, bz = (cw >> 2) & 1;
+            const int sx = bx - (cv & 1) + 1;
+            const int sy = DIM > 1 ? ((SPL == 2 && SA == 1) ? by : by - ((cv >> 1) & 1) + 1) : 0;
+            const int sz = DIM > 2 ? (SPL == 2 ? bz : bz - ((cv >> 2) & 1) + 1) : 0;
+            double* ac = acc[sz][sy][sx];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+              const double w = PP[q][la][c];
+#pragma unroll
+              for (int n = 0; n < NMC; ++n) ac[MCZ[n]] += w * mv[q][n];
+            }
+            if constexpr (CONST_DIFFUSION) {
+#pragma unroll
+              for (int n = 0; n < NKZ; ++n) ac[KZZ[n]] += ca * kd_rt(p, la, c, KZI[n], KZJ[n]);
+            } else {
+#pragma unroll
+              for (int n = 0; n < NDV; ++n) ac[DVZ[n]] += ca * dv[n] * KDU[p][la][c][DVK[n]][DVL[n]];
+            }
+          }
+        }
+      }
+      if constexpr (SPL == 2) {
+        // the middle plane: corner bit 1 on the lower lane, 0 on the upper
+#pragma unroll
+        for (int a1 = 0; a1 < (SA == 2 ? SY : 1); ++a1)
+#pragma unroll
+          for (int a0 = 0; a0 < (SA == 2 ? SX : SX); ++a0)
+#pragma unroll
+            for (int z = 0; z < (NZB > 0 ? NZB : 1); ++z) {
+              if constexpr (SA == 2) {
+                const double v0 = __shfl_xor(acc[0][a1][a0][z], 1), v1 = __shfl_xor(acc[1][a1][a0][z], 1);
+                if (half == 0) acc[1][a1][a0][z] += v0;
+                else acc[0][a1][a0][z] += v1;
+              } else {
+                const double v0 = __shfl_xor(acc[0][0][a0][z], 1), v1 = __shfl_xor(acc[0][1][a0][z], 1);
+                if (half == 0) acc[0][1][a0][z] += v0;
+                else acc[0][0][a0][z] += v1;
+              }
+            }
+      }
+      const unsigned mk = own_mask[s];
+      const int st = own_start[s];
+#pragma unroll
+      for (int a2 = 0; a2 < SZ; ++a2)
+#pragma unroll
+        for (int a1 = 0; a1 < SY; ++a1)
+#pragma unroll
+          for (int a0 = 0; a0 < SX; ++a0) {
+            // offset of this slot; on a split axis the slot is the corner bit
+            int dx = a0 - 1, dy = DIM > 1 ? a1 - 1 : 0, dz = DIM > 2 ? a2 - 1 : 0;
+            bool mine = true;
+            if constexpr (SPL == 2 && SA == 2) {
+              dz = a2 - (1 - half);
+              mine = half == 0 || a2 == 1;  // the lower lane writes the middle plane
+            }
+            if constexpr (SPL == 2 && SA == 1) {
+              dy = a1 - (1 - half);
+              mine = half == 0 || a1 == 1;
+            }
+            if (!mine) continue;
+            const int o = (dx + 1) + (DIM > 1 ? 3 * (dy + 1) : 0) + (DIM > 2 ? 9 * (dz + 1) : 0);
+            if (!((mk >> o) & 1u)) continue;
+            double* blk = val + (size_t)(st + __popc(mk & ((1u << o) - 1u))) * (DCG_B * DCG_B);
+            const bool diag = o == NST / 2;
+#pragma unroll
+            for (int i = 0; i < DCG_B; ++i)
+#pragma unroll
+              for (int j = 0; j < DCG_B; ++j) {
+                const int z = (i < BC && j < BC) ? ZIX[i < BC ? i : 0][j < BC ? j : 0] : -1;
+                blk[i * DCG_B + j] = z >= 0 ? acc[a2][a1][a0][z >= 0 ? z : 0] : ((diag && i == j && i >= BC) ? 1.0 : 0.0);
+              }
+          }
+    }
+    } else {
+    // phase 2 (blocks above 4 x 4): one BCSR block per (row, stencil offset),
+    // written once; the offset is uniform per wave and a lane builds one block
+    // row, so the accumulator stays at BC registers and small tiles still
+    // fill a wave.
+    constexpr int RL = DCG_B;  // lanes per row
     for (int o0 = wave; o0 < NST; o0 += nwave) {
       const int o = __builtin_amdgcn_readfirstlane(o0);
       const int dx = o % 3 - 1, dy = DIM > 1 ? (o / 3) % 3 - 1 : 0, dz = DIM > 2 ? o / 9 - 1 : 0;
@@ -1171,6 +1294,7 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
             blk[bi * DCG_B + j] = (bi < BC && j < BC) ? acc[i < AI ? i : 0][j < BC ? j : 0] : ((diag && bi == j) ? 1.0 : 0.0);
         }
       }
+    }
     }
     __syncthreads();
   }
@@ -1276,7 +1400,7 @@ bool tile_eligible(const ModelSpec& m, int comp) {
 }
 
 std::array<int, 3> choose_tile(const Grid& g, int bc, int nmc) {
-  // largest candidate whose LDS (staging + coefficient union) stays under 48 KiB
+  // largest candidate whose LDS (staging + coefficient union) stays under 64 KiB
   std::vector<std::array<int, 3>> cand;
   if (g.dim == 3) cand = {{8, 4, 4}, {4, 4, 4}, {4, 4, 2}, {4, 2, 2}, {2, 2, 2}};
   else if (g.dim == 2) cand = {{16, 8, 1}, {8, 8, 1}, {8, 4, 1}, {4, 4, 1}, {2, 2, 1}};
@@ -1288,7 +1412,7 @@ std::array<int, 3> choose_tile(const Grid& g, int bc, int nmc) {
     long ncb = cxb * cyb * czb;
     long mc = ncb * g.nclass * nq * std::max(1, nmc), rl = ncb * g.ncorner * bc;
     long bytes = 8 * std::max(mc, rl) + 8 * xb * yb * zb * bc + 4 * xb * yb * zb + ncb;
-    if (bytes <= 48 * 1024) return c;
+    if (bytes <= 64 * 1024) return c;
   }
   return cand.back();
 }
@@ -1374,6 +1498,39 @@ std::string gen_tiled(const ModelSpec& m, const Grid& g, int comp, const std::st
   o << "__device__ constexpr int DVJ[NDV > 0 ? NDV : 1] = " << ints(dj) << ";\n";
   o << "__device__ constexpr int DVK[NDV > 0 ? NDV : 1] = " << ints(dk) << ";\n";
   o << "__device__ constexpr int DVL[NDV > 0 ? NDV : 1] = " << ints(dl) << ";\n";
+  // entries kept per block by the row builder: the point-coefficient
+  // pattern and the diffusion pairs; their positions in the accumulator
+  {
+    std::vector<std::pair<int, int>> zb = nz;
+    auto zindex = [&](int i, int j) {
+      for (size_t z = 0; z < zb.size(); ++z)
+        if (zb[z].first == i && zb[z].second == j) return static_cast<int>(z);
+      zb.push_back({i, j});
+      return static_cast<int>(zb.size() - 1);
+    };
+    std::vector<int> mcz, dvz, kzi, kzj, kzz;
+    for (auto [i, j] : nz) mcz.push_back(zindex(i, j));
+    for (size_t n = 0; n < di.size(); ++n) dvz.push_back(zindex(di[n], dj[n]));
+    for (size_t n = 0; n < di.size(); ++n) {
+      bool seen = false;
+      for (size_t k2 = 0; k2 < kzi.size(); ++k2) seen = seen || (kzi[k2] == di[n] && kzj[k2] == dj[n]);
+      if (seen) continue;
+      kzi.push_back(di[n]);
+      kzj.push_back(dj[n]);
+      kzz.push_back(zindex(di[n], dj[n]));
+    }
+    const int bcn = static_cast<int>(species.size());
+    std::vector<int> zix(static_cast<size_t>(bcn) * bcn, -1);
+    for (size_t z = 0; z < zb.size(); ++z) zix[zb[z].first * bcn + zb[z].second] = static_cast<int>(z);
+    auto arr = [&](const std::vector<int>& v) { return v.empty() ? std::string("{0}") : int_init(v); };
+    o << "constexpr int NZB = " << zb.size() << ", NKZ = " << kzi.size() << ";\n";
+    o << "__device__ constexpr int ZIX[BC][BC] = " << arr(zix) << ";\n";
+    o << "__device__ constexpr int MCZ[NMC > 0 ? NMC : 1] = " << arr(mcz) << ";\n";
+    o << "__device__ constexpr int DVZ[NDV > 0 ? NDV : 1] = " << arr(dvz) << ";\n";
+    o << "__device__ constexpr int KZI[NKZ > 0 ? NKZ : 1] = " << arr(kzi) << ";\n";
+    o << "__device__ constexpr int KZJ[NKZ > 0 ? NKZ : 1] = " << arr(kzj) << ";\n";
+    o << "__device__ constexpr int KZZ[NKZ > 0 ? NKZ : 1] = " << arr(kzz) << ";\n";
+  }
   o << "__device__ __forceinline__ void uniform_diffusion(double t, const double* __restrict__ prm, "
        "double (&dv)[NDV > 0 ? NDV : 1]) {\n  (void)t; (void)prm; dv[0] = 0.0;\n";
   for (int n = 0; n < ndv; ++n) o << "  dv[" << n << "] = " << dexpr[n] << ";\n";
