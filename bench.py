@@ -3,7 +3,7 @@
 
 Workload (default): the 3D Mitchell-Schaefer excitable model on a 96^3 Q1 hex
 mesh of the unit cube (912,673 nodes, 1,825,346 DOF), D = 1e-3 on u, Alexander2
-with dt = 0.1, BiCGSTAB + block-Jacobi, Newton tolerance 1e-12 on the squared
+with dt = 0.1, BiCGSTAB + Jacobi, Newton tolerance 1e-12 on the squared
 l2 residual, linear tolerance 1e-10.  A "step" is one time step (two implicit
 Runge-Kutta stages, each a Newton solve); Newton DOF-steps = stages x DOF.
 
@@ -38,23 +38,23 @@ def _load(name, path, pkg=False):
     return mod
 
 
-def make_workload(wl, name, precision):
+def make_workload(wl, name, precision, precond="Jacobi"):
     if name == "ms96":
-        return wl.mitchell_schaefer(n=96, cell_type="cube", precond="Jacobi", nl_rtol=1e-12, lin_rtol=1e-10,
+        return wl.mitchell_schaefer(n=96, cell_type="cube", precond=precond, nl_rtol=1e-12, lin_rtol=1e-10,
                                     precision=precision), "mitchell_schaefer_3d_q1_96^3_alexander2_dt0.1"
     if name == "ms48":
-        return wl.mitchell_schaefer(n=48, cell_type="cube", precond="Jacobi", nl_rtol=1e-12, lin_rtol=1e-10,
+        return wl.mitchell_schaefer(n=48, cell_type="cube", precond=precond, nl_rtol=1e-12, lin_rtol=1e-10,
                                     precision=precision), "mitchell_schaefer_3d_q1_48^3_alexander2_dt0.1"
     if name == "gs128":
         return wl.gray_scott(n=128, cell_type="cube", precond="ILU", nl_rtol=1e-10, lin_rtol=1e-10,
                              precision=precision), "gray_scott_2d_q1_128^2_implicit_euler_dt1"
     if name == "gs1024":
-        return wl.gray_scott(n=1024, cell_type="cube", precond="Jacobi", nl_rtol=1e-16, lin_rtol=1e-10,
+        return wl.gray_scott(n=1024, cell_type="cube", precond=precond, nl_rtol=1e-16, lin_rtol=1e-10,
                              precision=precision), "gray_scott_2d_q1_1024^2_implicit_euler_dt1"
     if name == "two1024":
         # Newton to 1e-20 on the squared l2 defect (1e-10 in the norm): the
         # FP64 and mixed results then agree to equal accuracy (1e-8)
-        return wl.two_compartments(n=1024, cell_type="cube", precision=precision, nl_rtol=1e-20), \
+        return wl.two_compartments(n=1024, cell_type="cube", precision=precision, nl_rtol=1e-20, precond=precond), \
             "two_compartments_2d_q1_1024^2_implicit_euler_dt0.05"
     raise SystemExit(f"unknown workload {name}")
 
@@ -130,9 +130,9 @@ def cpu_baseline_batch(ws, dof_steps_budget: float):
                       f"OpenMP-coloured assembly, sequential BiCGSTAB)"}
 
 
-def plain_fp64(dca, items, steps):
+def plain_fp64(dca, items, steps, precond="Jacobi"):
     """The plain FP64 path on the same models, one model at a time: each model
-    built alone (FP64 BiCGSTAB + block-Jacobi), one untimed step, then `steps`
+    built alone (FP64 BiCGSTAB, same preconditioner), one untimed step, then `steps`
     timed steps.  items: (ini, dt, stages, total steps the workload runs,
     initial state of a model handle -> packed state).
     Returns Newton DOF-steps/s of the whole workload at the measured per-step
@@ -140,7 +140,7 @@ def plain_fp64(dca, items, steps):
     import torch
 
     ov = ("model.time_step_operator.linear_solver.precision = fp64\n"
-          "model.time_step_operator.linear_solver.preconditioner.type = Jacobi\n")
+          f"model.time_step_operator.linear_solver.preconditioner.type = {precond}\n")
     work, secs, sampled_work, sampled_secs = 0.0, 0.0, 0.0, 0.0
     for ini, dt, stages, total, init in items:
         m = dca.Model(ini, ov, 0, runtime_parameters=True)
@@ -174,7 +174,8 @@ def run_batch(args, dca, wl):
     ws = wl.gray_scott_batch(count=args.members, seed=0, precision=precision, nl_rtol=args.batch_nl_rtol,
                              nl_atol=0.0, lin_rtol=1e-10)
     t_setup = time.perf_counter()
-    b = dca.Batch(f"precision = {precision}\ngraph_iterations = {args.graph_iterations}\n", 0)
+    b = dca.Batch(f"precision = {precision}\ngraph_iterations = {args.graph_iterations}\n"
+                  f"preconditioner = {args.preconditioner}\n", 0)
     ids = []
     for w in ws:
         i = b.add(w.ini)
@@ -263,7 +264,8 @@ def run_batch(args, dca, wl):
     plain = None
     if args.plain_steps > 0:
         print("# plain FP64 path on the same models ...", file=sys.stderr, flush=True)
-        plain = plain_fp64(dca, [(w.ini, w.dt, 1, args.steps, w.initial_state) for w in ws], args.plain_steps)
+        plain = plain_fp64(dca, [(w.ini, w.dt, 1, args.steps, w.initial_state) for w in ws], args.plain_steps,
+                           args.preconditioner)
         plain["speedup_of_this_line"] = value / plain["value"]
     out = {
         "metric": "Newton DOF-steps/s (plain FP64 vs mixed/batched), p50/p99 latency, HBM GB/s",
@@ -279,7 +281,7 @@ def run_batch(args, dca, wl):
         "dtype": "f64" if precision == "fp64" else "f32-krylov/f64-refinement",
         "data": "synthetic (seeded initial states and parameters)",
         "config": {"workload": f"gray_scott_batch_{len(ws)}_2d_q1_sides32-512_implicit_euler_dt1",
-                   "members": len(ws), "n_dof": n_dof, "precision": precision, "preconditioner": "block-Jacobi",
+                   "members": len(ws), "n_dof": n_dof, "precision": precision, "preconditioner": args.preconditioner,
                    "graph_iterations": args.graph_iterations,
                    "newton_relative_tolerance_two_norm2": args.batch_nl_rtol},
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
@@ -309,7 +311,8 @@ def run_serve(args, dca, wl):
     arrivals = np.cumsum(rng.exponential(1.0 / args.qps, size=R))
     arrivals -= arrivals[0]
     srv = dca.Server(f"precision = {args.precision}\nmax_members = {args.max_members}\n"
-                     f"workers = {args.workers}\ngraph_iterations = {args.graph_iterations}\n", 0)
+                     f"workers = {args.workers}\ngraph_iterations = {args.graph_iterations}\n"
+                     f"preconditioner = {args.preconditioner}\n", 0)
     ids = [None] * R
     t0 = time.perf_counter()
     sub, last_print = 0, 0.0
@@ -368,7 +371,7 @@ def run_serve(args, dca, wl):
     if args.plain_steps > 0:
         print("# plain FP64 path on the same requests ...", file=sys.stderr, flush=True)
         plain = plain_fp64(dca, [(r.ini, r.dt, r.stages, r.steps, lambda m: m.initial_state()) for r in reqs],
-                           args.plain_steps)
+                           args.plain_steps, args.preconditioner)
         plain["speedup_of_this_line"] = (work / makespan) / plain["value"]
     out = {
         "metric": "Newton DOF-steps/s (plain FP64 vs mixed/batched), p50/p99 latency, HBM GB/s",
@@ -405,6 +408,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="ms96")
+    ap.add_argument("--preconditioner", default="Jacobi", choices=["Jacobi", "VertexBlockJacobi"],
+                    help="Jacobi: the reference's (diagonal); VertexBlockJacobi: inverse of each vertex's species block")
     ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
                     help="default: fp64 for single models, mixed for gs_batch and serve")
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -446,7 +451,7 @@ def main():
         return run_batch(args, dca, wl)
     if args.workload == "serve":
         return run_serve(args, dca, wl)
-    w, wname = make_workload(wl, args.workload, args.precision)
+    w, wname = make_workload(wl, args.workload, args.precision, args.preconditioner)
 
     t_setup = time.perf_counter()
     m = dca.Model(w.ini, "", 0)
@@ -508,7 +513,7 @@ def main():
         # the FP32-Krylov / FP64-refinement form on the same workload and steps,
         # with its error against this FP64 run
         u64, _ = m.get_state()
-        w32, _ = make_workload(wl, args.workload, "mixed")
+        w32, _ = make_workload(wl, args.workload, "mixed", args.preconditioner)
         m32 = dca.Model(w32.ini, "", 0)
         m32.set_state(u0, 0.0)
         t32 = 0.0
@@ -557,7 +562,7 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32-krylov/f64-refinement",
         "data": "synthetic (seeded initial state)",
         "config": {"workload": wname, "n_dof": m.n_dof, "stages_per_step": w.stages,
-                   "precision": args.precision, "preconditioner": "block-Jacobi"},
+                   "precision": args.precision, "preconditioner": args.preconditioner},
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
         "newton_iterations_per_step": newton / args.steps,
         "krylov_iterations_per_step": lin / args.steps,

@@ -2,7 +2,7 @@
 
 The library is the product: grid/compartment/BCSR set-up and the hipRTC
 assembly compiler on the host, the HIP kernels of assembly, SpMV, BiCGSTAB,
-block-Jacobi and ILU0 on the GPU.  This module only moves numpy arrays across
+Jacobi and ILU0 on the GPU.  This module only moves numpy arrays across
 the boundary; it has no numerical code of its own and no fallback: if the
 library is missing, loading fails with an error naming the build step.
 """
@@ -53,6 +53,18 @@ class RequestInfo(ctypes.Structure):
     ]
 
 
+class EvolveOptions(ctypes.Structure):
+    _fields_ = [("t_end", ctypes.c_double), ("dt_initial", ctypes.c_double), ("decrease_factor", ctypes.c_double),
+                ("increase_factor", ctypes.c_double), ("min_step", ctypes.c_double), ("max_step", ctypes.c_double),
+                ("snap_to_end_time", ctypes.c_int)]
+
+
+class EvolveStats(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("steps", ctypes.c_int), ("failed_steps", ctypes.c_int),
+                ("time", ctypes.c_double), ("dt", ctypes.c_double), ("newton_iterations", ctypes.c_long),
+                ("linear_iterations", ctypes.c_long)]
+
+
 _P = ctypes.c_void_p
 _D = ctypes.c_double
 _I = ctypes.c_int
@@ -99,6 +111,8 @@ _SIGS = {
     "dcg_batch_device_bytes": (_I, [_P, ctypes.POINTER(_S)]),
     "dcg_batch_profile_enable": (_I, [_P, _I]),
     "dcg_batch_profile_read": (_I, [_P, _I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "dcg_evolve_defaults": (_I, [_P, ctypes.POINTER(EvolveOptions)]),
+    "dcg_evolve": (_I, [_P, ctypes.POINTER(EvolveOptions), ctypes.POINTER(EvolveStats), _P, _P, _P, _S]),
     "dcg_server_create": (_I, [ctypes.c_char_p, _I, ctypes.POINTER(_P)]),
     "dcg_server_destroy": (None, [_P]),
     "dcg_server_submit": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p, _I, ctypes.POINTER(_I)]),
@@ -303,6 +317,28 @@ class Model:
         for v, t in zip(var[:n.value], us[:n.value]):
             out[names[int(v) % 100] + ("_fp32" if v >= 100 else "")] = float(t)
         return out, names[ch.value]
+
+    # ---- time loop
+    def evolve_defaults(self) -> EvolveOptions:
+        o = EvolveOptions()
+        _check(self._lib.dcg_evolve_defaults(self._h, ctypes.byref(o)))
+        return o
+
+    def evolve(self, t_end=None, dt=None, **kw):
+        """Adaptive time loop from the current state (options default to the INI keys);
+        returns (EvolveStats, accepted step sizes)."""
+        o = self.evolve_defaults()
+        if t_end is not None:
+            o.t_end = t_end
+        if dt is not None:
+            o.dt_initial = dt
+        for k, v in kw.items():
+            setattr(o, k, v)
+        st = EvolveStats()
+        acc = np.zeros(100000)
+        rc = self._lib.dcg_evolve(self._h, ctypes.byref(o), ctypes.byref(st), None, None, _ptr(acc), acc.size)
+        _check(rc)
+        return st, acc[:st.steps].copy()
 
     # ---- run-time parameters and footprint
     def param_names(self):

@@ -13,9 +13,10 @@ BatchOptions BatchOptions::from_ini(const std::string& text) {
   else if (prec == "mixed") o.precision = Precision::Mixed;
   else throw IOError("batch precision must be 'fp64' or 'mixed'");
   const std::string pre = c.get("preconditioner", std::string("Jacobi"));
-  if (pre == "Jacobi") o.jacobi = true;
-  else if (pre == "None") o.jacobi = false;
-  else throw IOError("batch preconditioner must be 'Jacobi' or 'None'");
+  if (pre == "Jacobi") o.precond = Preconditioner::Jacobi;
+  else if (pre == "VertexBlockJacobi") o.precond = Preconditioner::VertexBlockJacobi;
+  else if (pre == "None") o.precond = Preconditioner::None;
+  else throw IOError("batch preconditioner must be 'Jacobi', 'VertexBlockJacobi' or 'None'");
   o.ir_inner_rtol = c.get("refinement.inner_relative_tolerance", o.ir_inner_rtol);
   o.ir_max = c.get("refinement.max_steps", o.ir_max);
   o.lin_maxit = c.get("iteration_limit", o.lin_maxit);
@@ -72,7 +73,9 @@ std::string Batch::member_overrides(const std::string& overrides) const {
   ov += "\nmodel.time_step_operator.linear_solver.precision = ";
   ov += opt_.precision == Precision::Mixed ? "mixed" : "fp64";
   ov += "\nmodel.time_step_operator.linear_solver.preconditioner.type = ";
-  ov += opt_.jacobi ? "Jacobi" : "None";
+  ov += opt_.precond == Preconditioner::Jacobi            ? "Jacobi"
+        : opt_.precond == Preconditioner::VertexBlockJacobi ? "VertexBlockJacobi"
+                                                             : "None";
   ov += "\n";
   return ov;
 }
@@ -325,10 +328,10 @@ void Batch::capture(SolverGroup& g, bool fp32) {
   DCG_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
   for (int it = 0; it < opt_.graph_iterations; ++it) {
     if (fp32)
-      k::b_bicg_iteration<float>(g.B, g.variant32, g.w, g.mv.get(), k::kVKry + 9, opt_.jacobi, g.part.get(),
+      k::b_bicg_iteration<float>(g.B, g.variant32, g.w, g.mv.get(), k::kVKry + 9, opt_.precond != Preconditioner::None, g.part.get(),
                                  g.sc.get(), stream_);
     else
-      k::b_bicg_iteration<double>(g.B, g.variant64, g.w, g.mv.get(), k::kVZ, opt_.jacobi, g.part.get(), g.sc.get(),
+      k::b_bicg_iteration<double>(g.B, g.variant64, g.w, g.mv.get(), k::kVZ, opt_.precond != Preconditioner::None, g.part.get(), g.sc.get(),
                                   stream_);
   }
   DCG_HIP(hipStreamEndCapture(stream_, &graph));
@@ -481,10 +484,10 @@ void Batch::iterate(SolverGroup& g, bool fp32) {
       }
       if (all || done_its > opt_.lin_maxit + 4) break;
       if (fp32)
-        k::b_bicg_iteration<float>(g.B, g.variant32, g.w, g.mv.get(), k::kVKry + 9, opt_.jacobi, g.part.get(),
+        k::b_bicg_iteration<float>(g.B, g.variant32, g.w, g.mv.get(), k::kVKry + 9, opt_.precond != Preconditioner::None, g.part.get(),
                                    g.sc.get(), stream_, ev_);
       else
-        k::b_bicg_iteration<double>(g.B, g.variant64, g.w, g.mv.get(), k::kVZ, opt_.jacobi, g.part.get(),
+        k::b_bicg_iteration<double>(g.B, g.variant64, g.w, g.mv.get(), k::kVZ, opt_.precond != Preconditioner::None, g.part.get(),
                                     g.sc.get(), stream_, ev_);
       DCG_HIP(hipEventSynchronize(ev_[3]));
       float a = 0.f, b = 0.f;
@@ -506,10 +509,10 @@ void Batch::iterate(SolverGroup& g, bool fp32) {
         if (!g.graph[fp32 ? 1 : 0]) capture(g, fp32);
         DCG_HIP(hipGraphLaunch(g.graph[fp32 ? 1 : 0], stream_));
       } else if (fp32) {
-        k::b_bicg_iteration<float>(g.B, g.variant32, g.w, g.mv.get(), k::kVKry + 9, opt_.jacobi, g.part.get(),
+        k::b_bicg_iteration<float>(g.B, g.variant32, g.w, g.mv.get(), k::kVKry + 9, opt_.precond != Preconditioner::None, g.part.get(),
                                    g.sc.get(), stream_);
       } else {
-        k::b_bicg_iteration<double>(g.B, g.variant64, g.w, g.mv.get(), k::kVZ, opt_.jacobi, g.part.get(), g.sc.get(),
+        k::b_bicg_iteration<double>(g.B, g.variant64, g.w, g.mv.get(), k::kVZ, opt_.precond != Preconditioner::None, g.part.get(), g.sc.get(),
                                     stream_);
       }
     }
@@ -547,7 +550,8 @@ void Batch::solve(const std::vector<char>& on, const std::vector<double>& rtol, 
     if (!any) continue;
     DCG_HIP(hipMemcpyAsync(g.rtol.get(), g.h_rtol.data(), M * sizeof(double), hipMemcpyHostToDevice, stream_));
     const k::Skip off{g.act.get(), 1, 0};
-    if (opt_.jacobi) k::b_block_inverse(g.B, g.w, g.mv.get(), off, status_.get(), stream_);
+    if (opt_.precond != Preconditioner::None)
+      k::b_block_inverse(g.B, g.w, g.mv.get(), off, status_.get(), opt_.precond == Preconditioner::Jacobi, stream_);
     if (!mixed) {
       k::b_bicg_setup(g.w, off, g.rtol.get(), nullptr, 0.0, opt_.lin_maxit, g.sc.get(), stream_);
       k::b_bicg_init<double>(g.B, g.w, g.mv.get(), k::kVR, k::kVZ, g.part.get(), g.sc.get(), stream_);

@@ -11,7 +11,7 @@
 //     per module and compartment (coloured kernels and interface facets of
 //     members that need them are launched per member);
 //   * Krylov: batched BiCGSTAB kernels over (member, block) and (member,
-//     chunk) work lists with per-member scalars, block-Jacobi preconditioning
+//     chunk) work lists with per-member scalars, Jacobi preconditioning
 //     and, for mixed precision, per-member FP64 iterative refinement; blocks of
 //     iterations are captured once in a hipGraph and replayed;
 //   * vector updates and norms: batched with per-member skip flags.
@@ -32,12 +32,12 @@ namespace dcopasi {
 
 struct BatchOptions {
   Precision precision = Precision::Mixed;
-  bool jacobi = true;        // block-Jacobi, else no preconditioner
+  Preconditioner precond = Preconditioner::Jacobi;  // Jacobi, VertexBlockJacobi or None
   double ir_inner_rtol = 1e-3;
   int ir_max = 50;
   int lin_maxit = 500;
   int graph_iterations = 4;  // BiCGSTAB iterations per captured graph; 0: plain launches
-  // keys: precision = fp64|mixed, preconditioner = Jacobi|None,
+  // keys: precision = fp64|mixed, preconditioner = Jacobi|VertexBlockJacobi|None,
   // refinement.inner_relative_tolerance, refinement.max_steps,
   // iteration_limit, graph_iterations
   static BatchOptions from_ini(const std::string& text);

@@ -7,6 +7,7 @@
 #include "batch.hh"
 #include "model.hh"
 #include "serve.hh"
+#include "stepper.hh"
 
 // owning handle, or a view of a batch member (own empty)
 struct dcg_model {
@@ -276,6 +277,54 @@ int dcg_spmv_tuning(dcg_model_t h, int* count, int* variant, double* microsecond
   }
   if (chosen) *chosen = h->m->spmv_variant();
   return 0;
+}
+
+int dcg_evolve_defaults(dcg_model_t h, dcg_evolve_options* o) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    if (!o) throw dcopasi::Error("null options");
+    const auto e = dcopasi::EvolveOptions::from_model(*h->m);
+    o->t_end = e.t_end;
+    o->dt_initial = e.dt;
+    o->decrease_factor = e.decrease_factor;
+    o->increase_factor = e.increase_factor;
+    o->min_step = e.min_step;
+    o->max_step = e.max_step;
+    o->snap_to_end_time = e.snap_to_end_time ? 1 : 0;
+    return 0;
+  });
+}
+
+int dcg_evolve(dcg_model_t h, const dcg_evolve_options* o, dcg_evolve_stats* stats,
+               void (*on_step)(void* user, double time), void* user, double* accepted_dt, size_t capacity) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    dcopasi::EvolveOptions e = dcopasi::EvolveOptions::from_model(*h->m);
+    if (o) {
+      e.t_end = o->t_end;
+      e.dt = o->dt_initial;
+      e.decrease_factor = o->decrease_factor;
+      e.increase_factor = o->increase_factor;
+      e.min_step = o->min_step;
+      e.max_step = o->max_step;
+      e.snap_to_end_time = o->snap_to_end_time != 0;
+    }
+    std::function<void(double)> cb;
+    if (on_step) cb = [on_step, user](double t) { on_step(user, t); };
+    const auto st = dcopasi::evolve(*h->m, e, cb);
+    if (stats) {
+      stats->status = st.status;
+      stats->steps = st.steps;
+      stats->failed_steps = st.failed_steps;
+      stats->time = st.time;
+      stats->dt = st.dt;
+      stats->newton_iterations = st.newton_iterations;
+      stats->linear_iterations = st.linear_iterations;
+    }
+    if (accepted_dt)
+      for (size_t k = 0; k < st.accepted_dt.size() && k < capacity; ++k) accepted_dt[k] = st.accepted_dt[k];
+    return st.status;
+  });
 }
 
 int dcg_param_info(dcg_model_t h, size_t i, char* name, size_t name_len, size_t* count) {

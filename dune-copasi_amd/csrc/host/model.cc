@@ -102,6 +102,7 @@ void Model::build(const std::string& ini, const std::string& overrides) {
   }
   if (pre == "ILU" || pre == "ILU0") cfg_.precond = Preconditioner::ILU0;
   else if (pre == "Jacobi") cfg_.precond = Preconditioner::Jacobi;
+  else if (pre == "VertexBlockJacobi") cfg_.precond = Preconditioner::VertexBlockJacobi;
   else if (pre == "None" || pre == "Richardson") cfg_.precond = Preconditioner::None;
   else if (pre == "SSOR" || pre == "SOR" || pre == "GaussSeidel") {
     cfg_.precond = Preconditioner::ILU0;
@@ -109,9 +110,16 @@ void Model::build(const std::string& ini, const std::string& overrides) {
   } else {
     throw NotImplemented("preconditioner '" + pre + "'");
   }
+  if (cfg_.precond == Preconditioner::Jacobi) {
+    // SeqJac's sweeps and relaxation (istl defaults 1 and 1.0): only one
+    // sweep from zero with weight 1 is the plain inverse of the diagonal
+    const double w = c.get(p + "linear_solver.preconditioner.relaxation", 1.0);
+    const int sweeps = c.get(p + "linear_solver.preconditioner.iterations", 1);
+    if (w != 1.0 || sweeps != 1) throw NotImplemented("Jacobi with relaxation != 1 or iterations != 1");
+  }
   if (opt_.batch_member && cfg_.precond == Preconditioner::ILU0) {
-    cfg_.precond = Preconditioner::Jacobi;
-    cfg_.warnings.push_back("batch members precondition with block-Jacobi");
+    cfg_.precond = Preconditioner::VertexBlockJacobi;
+    cfg_.warnings.push_back("batch members precondition with vertex block-Jacobi in place of ILU0");
   }
   const std::string prec = c.get(p + "linear_solver.precision", std::string("fp64"));
   if (prec == "fp64") cfg_.precision = Precision::FP64;
@@ -545,8 +553,9 @@ double Model::residual_norm(const double* r) {
 void Model::precond_setup() {
   const int B = L_.B;
   const int ns = static_cast<int>(L_.nslot);
-  if (cfg_.precond == Preconditioner::Jacobi) {
-    k::block_inverse(B, ns, val_.get(), diag_pos_.get(), dinv_.get(), status_.get(), stream_);
+  if (jacobi_kind()) {
+    k::block_inverse(B, ns, val_.get(), diag_pos_.get(), dinv_.get(), status_.get(),
+                     cfg_.precond == Preconditioner::Jacobi, stream_);
   } else if (cfg_.precond == Preconditioner::ILU0) {
     DCG_HIP(hipMemcpyAsync(lu_.get(), val_.get(), val_.bytes(), hipMemcpyDeviceToDevice, stream_));
     for (size_t l = 0; l + 1 < lo_ptr_.size(); ++l)
@@ -569,7 +578,7 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
   const int nred = std::max(1, std::min(1024, static_cast<int>((n + 255) / 256)));
   const int nblk = (ns + 255) / 256;
   k::bicg_init<double>(n, b, x, r, rh, p, v, part_.get(), nred, sc_.get(), stream_);
-  const double* dinv = cfg_.precond == Preconditioner::Jacobi ? dinv_.get() : nullptr;
+  const double* dinv = jacobi_kind() ? dinv_.get() : nullptr;
   const bool ilu = cfg_.precond == Preconditioner::ILU0;
   auto apply_ilu = [&](const double* in, double* out) {
     for (size_t l = 0; l + 1 < lo_ptr_.size(); ++l)
@@ -627,7 +636,7 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
   const int nred = std::max(1, std::min(1024, static_cast<int>((n + 255) / 256)));
   const int nblk = (ns + 255) / 256;
   k::bicg_init<float>(n, b, x, r, rh, p, v, part_.get(), nred, sc_.get(), stream_);
-  const double* dinv = cfg_.precond == Preconditioner::Jacobi ? dinv_.get() : nullptr;
+  const double* dinv = jacobi_kind() ? dinv_.get() : nullptr;
   const bool ilu = cfg_.precond == Preconditioner::ILU0;
   auto apply_ilu = [&](const float* in, float* out) {
     for (size_t l = 0; l + 1 < lo_ptr_.size(); ++l)

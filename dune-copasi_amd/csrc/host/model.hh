@@ -37,7 +37,11 @@ struct RkTable {
 };
 RkTable rk_table(const std::string& type);
 
-enum class Preconditioner { None, Jacobi, ILU0 };
+// Jacobi: istl's SeqJac as the reference applies it to its scalar Jacobian,
+// one sweep from zero = the inverse of the diagonal; VertexBlockJacobi (no
+// counterpart in the reference): the inverse of each vertex's species block;
+// ILU0: incomplete LU over the species-blocked pattern.
+enum class Preconditioner { None, Jacobi, VertexBlockJacobi, ILU0 };
 enum class Precision { FP64, Mixed };
 
 struct SolverConfig {
@@ -63,8 +67,8 @@ struct ModelOptions {
   // models that differ only in them share one generated module (batches,
   // parameter sweeps); set_params changes the constants without recompiling
   bool runtime_parameters = false;
-  // member of a Batch: the batch tunes the SpMV and preconditions with
-  // block-Jacobi; `stream` (owned by the batch) replaces the model's own
+  // member of a Batch: the batch tunes the SpMV and chooses the (Jacobi)
+  // preconditioner; `stream` (owned by the batch) replaces the model's own
   bool batch_member = false;
   hipStream_t stream = nullptr;
 };
@@ -85,6 +89,7 @@ class Model {
   const Layout& layout() const { return L_; }
   const ModelSpec& spec() const { return spec_; }
   const SolverConfig& solver() const { return cfg_; }
+  const ParameterTree& config() const { return cfg_tree_; }
   const std::string& source() const { return gen_.source; }
   bool on_device() const { return device_ >= 0; }
   double time() const { return time_; }
@@ -156,6 +161,9 @@ class Model {
   int bicgstab64(const double* b, double* x, double rtol, double abstol, int maxit, int* its);
   int bicgstab32(const double* b, float* x, double rtol, int maxit, int* its);
   void precond_setup();
+  bool jacobi_kind() const {
+    return cfg_.precond == Preconditioner::Jacobi || cfg_.precond == Preconditioner::VertexBlockJacobi;
+  }
   void compile_and_load();
   void tune_spmv();
   int tune_spmv_plan(bool fp32);

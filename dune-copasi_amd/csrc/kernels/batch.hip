@@ -7,7 +7,7 @@
 // reductions are segmented: the partial sums of a member occupy a contiguous
 // range of the work list and one finaliser workgroup per member reduces them
 // in a fixed order.  These kernels restate, per member, the BiCGSTAB and
-// block-Jacobi iteration of solver.hip (dune-istl BiCGSTABSolver / SeqJac as
+// Jacobi iteration of solver.hip (dune-istl BiCGSTABSolver / SeqJac as
 // configured by model/make_step_operator.hh:102-146).
 #include <hip/hip_runtime.h>
 
@@ -152,13 +152,13 @@ __global__ void __launch_bounds__(kThreads) bnorm_fin_k(const int2* __restrict__
 
 template <int B>
 __global__ void __launch_bounds__(kThreads) binv_k(const int2* __restrict__ work, const MemberView* __restrict__ mv,
-                                                   Skip sk, int* __restrict__ status) {
+                                                   Skip sk, int* __restrict__ status, int point) {
   const int2 w = work[blockIdx.x];
   if (skipped(sk, w.x)) return;
   const MemberView& m = mv[w.x];
   const int s = w.y * kThreads + threadIdx.x;
   if (s >= m.nslot) return;
-  block_inverse_dev<B>(m.val + (size_t)m.diag_pos[s] * B * B, m.dinv + (size_t)s * B * B, status);
+  jacobi_inverse_dev<B>(m.val + (size_t)m.diag_pos[s] * B * B, m.dinv + (size_t)s * B * B, status, point != 0);
 }
 
 // FP32 copy of the matrix rows of one 256-slot block
@@ -496,9 +496,10 @@ void b_norm(int B, const BatchWork& w, const MemberView* mv, Skip sk, int x, int
   bnorm_fin_k<<<w.members, kThreads, 0, s>>>(w.vrange, part, op, out);
 }
 
-void b_block_inverse(int B, const BatchWork& w, const MemberView* mv, Skip sk, int* status, hipStream_t s) {
+void b_block_inverse(int B, const BatchWork& w, const MemberView* mv, Skip sk, int* status, bool point,
+                     hipStream_t s) {
   if (w.nvwork <= 0) return;
-  DCG_BATCH_B(B, (binv_k<BB><<<w.nvwork, kThreads, 0, s>>>(w.vwork, mv, sk, status)));
+  DCG_BATCH_B(B, (binv_k<BB><<<w.nvwork, kThreads, 0, s>>>(w.vwork, mv, sk, status, point ? 1 : 0)));
 }
 
 void b_to_float(int B, const BatchWork& w, const MemberView* mv, Skip sk, hipStream_t s) {

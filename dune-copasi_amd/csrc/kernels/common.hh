@@ -193,6 +193,28 @@ __device__ __forceinline__ void block_inverse_dev(const double* __restrict__ d, 
     for (int j = 0; j < B; ++j) o[i * B + j] = inv[i][j];
 }
 
+// point Jacobi in the same B x B layout: 1/a_ii on the diagonal, zeros
+// elsewhere, so every block-Jacobi application applies it unchanged; a zero
+// diagonal entry sets status[0] = 4
+template <int B>
+__device__ __forceinline__ void diag_inverse_dev(const double* __restrict__ d, double* __restrict__ o,
+                                                 int* __restrict__ status) {
+#pragma unroll
+  for (int i = 0; i < B; ++i) {
+    const double a = d[i * B + i];
+    if (a == 0.0) status[0] = 4;
+#pragma unroll
+    for (int j = 0; j < B; ++j) o[i * B + j] = i == j ? (a == 0.0 ? 1.0 : 1.0 / a) : 0.0;
+  }
+}
+
+template <int B>
+__device__ __forceinline__ void jacobi_inverse_dev(const double* __restrict__ d, double* __restrict__ o,
+                                                   int* __restrict__ status, bool point) {
+  if (point) diag_inverse_dev<B>(d, o, status);
+  else block_inverse_dev<B>(d, o, status);
+}
+
 
 // ----------------------------------------------------------------- SpMV chunk bodies
 // LDS stage element: the FP32 matrix stages its per-block products in FP32

@@ -111,7 +111,7 @@ void b_axpy(int B, const BatchWork& w, const MemberView* mv, Skip sk, double a, 
 // per-member reduction of vector x (op 0: x.x, 1: sum |x|, 2: max |x|) into out[m]
 void b_norm(int B, const BatchWork& w, const MemberView* mv, Skip sk, int x, int op, double* part, double* out,
             hipStream_t s);
-void b_block_inverse(int B, const BatchWork& w, const MemberView* mv, Skip sk, int* status, hipStream_t s);
+void b_block_inverse(int B, const BatchWork& w, const MemberView* mv, Skip sk, int* status, bool point, hipStream_t s);
 void b_to_float(int B, const BatchWork& w, const MemberView* mv, Skip sk, hipStream_t s);
 // Krylov set-up: tolerances from tol[m] (or, with ir, the refinement's inner
 // tolerance max(inner, target/(2 rn)) capped at 0.5); members skipped by sk or
@@ -162,7 +162,10 @@ void spmv(int B, const SpmvPlan& pl, const double* val, const double* x, double*
 void spmv(int B, const SpmvPlan& pl, const float* val, const float* x, float* y, const float* w1, const float* w2,
           double* part, BicgScalars* sc, int stage, hipStream_t s);
 
-void block_inverse(int B, int nslot, const double* val, const int* diag_pos, double* dinv, int* status, hipStream_t s);
+// Jacobi preconditioner blocks: the inverse of each diagonal B x B block, or
+// with point set only the inverse of its diagonal (in the same layout)
+void block_inverse(int B, int nslot, const double* val, const int* diag_pos, double* dinv, int* status, bool point,
+                   hipStream_t s);
 
 template <typename V>
 void bicg_init(long n, const double* b, V* x, V* r, V* rh, V* p, V* v, double* part, int nparts, BicgScalars* sc,

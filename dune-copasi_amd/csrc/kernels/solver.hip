@@ -1,5 +1,6 @@
 // Ahead-of-time kernels of the linear solve on gfx950: BCSR SpMV, the fused
-// BiCGSTAB vector updates, deterministic reductions, block-Jacobi and
+// BiCGSTAB vector updates, deterministic reductions, Jacobi (point or
+// vertex-block) and
 // level-scheduled block ILU0.  These replace dune-istl's sequential
 // BiCGSTABSolver / SeqJac / SeqILU that the reference builds on every Newton
 // iteration (model/make_step_operator.hh:102-146,
@@ -237,13 +238,13 @@ __global__ void __launch_bounds__(kThreads) spmv_blk_k(int nchunk, const int* __
   }
 }
 
-// ----------------------------------------------------------------- block Jacobi
+// ----------------------------------------------------------------- Jacobi
 template <int B>
 __global__ void block_inverse_k(int nslot, const double* __restrict__ val, const int* __restrict__ diag_pos,
-                                double* __restrict__ dinv, int* __restrict__ status) {
+                                double* __restrict__ dinv, int* __restrict__ status, int point) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslot) return;
-  block_inverse_dev<B>(val + (size_t)diag_pos[s] * B * B, dinv + (size_t)s * B * B, status);
+  jacobi_inverse_dev<B>(val + (size_t)diag_pos[s] * B * B, dinv + (size_t)s * B * B, status, point != 0);
 }
 
 // ----------------------------------------------------------------- BiCGSTAB
@@ -678,8 +679,10 @@ void spmv(int B, const SpmvPlan& pl, const float* val, const float* x, float* y,
   DCG_DISPATCH_B(B, (spmv_t<BB, float, float>(pl, val, x, y, w1, w2, part, sc, stage, s)));
 }
 
-void block_inverse(int B, int nslot, const double* val, const int* diag_pos, double* dinv, int* status, hipStream_t s) {
-  DCG_DISPATCH_B(B, (block_inverse_k<BB><<<grid_for(nslot), kThreads, 0, s>>>(nslot, val, diag_pos, dinv, status)));
+void block_inverse(int B, int nslot, const double* val, const int* diag_pos, double* dinv, int* status, bool point,
+                   hipStream_t s) {
+  DCG_DISPATCH_B(B, (block_inverse_k<BB><<<grid_for(nslot), kThreads, 0, s>>>(nslot, val, diag_pos, dinv, status,
+                                                                              point ? 1 : 0)));
 }
 
 template <typename V>

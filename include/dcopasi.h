@@ -138,6 +138,36 @@ int dcg_probe_copy(dcg_model_t model, size_t bytes, double* milliseconds);
  * FP64 variant used.  Arrays hold at least 16 entries. */
 int dcg_spmv_tuning(dcg_model_t model, int* count, int* variant, double* microseconds, int* chosen);
 
+/* ---------------------------------------------------------------- time loop
+ * The reference's SimpleAdaptiveStepper with TimeStepper::evolve and
+ * snap_to_time (dune/copasi/common/stepper.hh:95-240,330-366), as its CLI
+ * configures it (src/dune_copasi.cc:384-437): on a failed step the step size
+ * is multiplied by decrease_factor and the step retried from the same state;
+ * after a success the next size is multiplied by increase_factor (clamped
+ * to max_step); the loop stops two steps before t_end and snaps to it with
+ * equal steps.  min_step / max_step of 0 mean "no limit". */
+typedef struct dcg_evolve_options {
+  double t_end, dt_initial, decrease_factor, increase_factor, min_step, max_step;
+  int snap_to_end_time;
+} dcg_evolve_options;
+
+typedef struct dcg_evolve_stats {
+  int status; /* 0 ok, 5 step size left [min_step, max_step], else the failing step's status */
+  int steps, failed_steps;
+  double time, dt; /* reached time, suggested next step size */
+  long newton_iterations, linear_iterations;
+} dcg_evolve_stats;
+
+/* options from the model's INI keys model.time_step_operator.{time_end,
+ * time_step_initial, time_step_min, time_step_max, time_step_decrease_factor,
+ * time_step_increase_factor} */
+int dcg_evolve_defaults(dcg_model_t model, dcg_evolve_options* options);
+/* advance from the model's current time; on_step (may be NULL) after every
+ * accepted step; accepted_dt (may be NULL) receives up to capacity accepted
+ * step sizes.  Returns stats.status (or -1 on error). */
+int dcg_evolve(dcg_model_t model, const dcg_evolve_options* options, dcg_evolve_stats* stats,
+               void (*on_step)(void* user, double time), void* user, double* accepted_dt, size_t capacity);
+
 /* ---------------------------------------------------------------- run-time parameters
  * The reference re-parses `parser_context` constants whenever it builds a
  * model (dune/copasi/parser/context.cc:56-173); parameter sweeps and fits
@@ -165,11 +195,13 @@ int dcg_device_memory(int device, size_t* free_bytes, size_t* total_bytes);
  * buffers; dcg_batch_step advances every member by one step of its own
  * scheme and step size from its own time, with one launch per device
  * operation for all members (batched assembly per shared module, batched
- * BiCGSTAB with per-member scalars, block-Jacobi, per-member FP64 refinement
+ * BiCGSTAB with per-member scalars, Jacobi, per-member FP64 refinement
  * of the FP32 solve, hipGraph-captured iteration blocks).  The reference
  * has no counterpart: it advances one model per process
  * (dune/copasi/common/stepper.hh:95-175).
- * options_ini keys: precision = mixed|fp64, preconditioner = Jacobi|None,
+ * options_ini keys: precision = mixed|fp64,
+ * preconditioner = Jacobi (diagonal, as istl's SeqJac) | VertexBlockJacobi
+ * (inverse of each vertex's species block; no counterpart in the reference) | None,
  * refinement.inner_relative_tolerance, refinement.max_steps,
  * iteration_limit, graph_iterations (0: no graphs). */
 typedef struct dcg_batch* dcg_batch_t;

@@ -30,6 +30,7 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <limits>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -1516,6 +1517,68 @@ static int step(Model& M, double dt, StepStats& st) {
   return 0;
 }
 
+// ---------------------------------------------------------------- time loop
+// SimpleAdaptiveStepper::do_step (stepper.hh:330-366) inside TimeStepper::
+// evolve and snap_to_time (stepper.hh:145-240): halve (decrease factor) and
+// retry on failure, grow by the increase factor (clamped to the maximum)
+// after success, stop 2 dt before the end and snap with equal steps.  Times
+// are compared with Dune::FloatCmp's relative-weak default (8 machine
+// epsilons of the larger magnitude).
+struct EvolveRecord {
+  std::vector<double> accepted;
+  int failed = 0;
+};
+
+static bool t_eq(double a, double b) {
+  return std::fabs(a - b) <= 8.0 * std::numeric_limits<double>::epsilon() * std::max(std::fabs(a), std::fabs(b));
+}
+
+static int evolve(Model& M, double t_end, double& dt, double dec, double inc, double dmin, double dmax, bool snap,
+                  EvolveRecord& rec) {
+  auto out_of_range = [&](double h) {
+    if (dmin > 0.0 && std::fabs(h) < std::fabs(dmin) && !t_eq(std::fabs(h), std::fabs(dmin))) return true;
+    if (dmax > 0.0 && std::fabs(h) > std::fabs(dmax) && !t_eq(std::fabs(h), std::fabs(dmax))) return true;
+    return false;
+  };
+  auto adaptive = [&](double& h) -> int {
+    if (out_of_range(h)) return 5;
+    StepStats st;
+    int rc = step(M, h, st);
+    while (rc) {
+      rec.failed++;
+      h = h * dec;
+      if (out_of_range(h)) return 5;
+      StepStats st2;
+      rc = step(M, h, st2);
+    }
+    rec.accepted.push_back(h);
+    double next = h * inc;
+    if (dmax > 0.0) next = std::min(std::max(next, -std::fabs(dmax)), std::fabs(dmax));
+    h = next;
+    return 0;
+  };
+  const double stop = snap ? 2.0 : 1.0;
+  for (;;) {
+    const double a = M.time + stop * dt;
+    if (!(a < t_end || t_eq(a, t_end))) break;
+    int rc = adaptive(dt);
+    if (rc) return rc;
+  }
+  if (!snap) return 0;
+  int tries = 0, rc = 0;
+  while (M.time < t_end && !t_eq(M.time, t_end)) {
+    const int n = (int)std::ceil((t_end - M.time) / dt);
+    if (n <= 0) throw std::runtime_error("Timestep doesn't make advances towards snap step");
+    dt = (t_end - M.time) / n;
+    rc = adaptive(dt);
+    if (rc) {
+      if (tries++ == 100) throw std::runtime_error("Snapping time exceeded maximum iteration count");
+      dt *= 0.5;
+    }
+  }
+  return rc;
+}
+
 static void configure(Model& M, const std::string& ini, const std::string& over) {
   M.ini.parse(ini);
   if (!over.empty()) M.ini.parse_overrides(over);
@@ -1681,6 +1744,22 @@ int orc_step(void* h, double dt, int* newton_its, int* linear_its) {
     int rc = orc::step(*M, dt, st);
     if (newton_its) *newton_its = st.newton;
     if (linear_its) *linear_its = st.linear;
+    return rc;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+int orc_evolve(void* h, double t_end, double dt, double dec, double inc, double dmin, double dmax, int snap,
+               double* accepted, int capacity, int* n_accepted, int* n_failed, double* dt_next) {
+  auto* M = static_cast<Model*>(h);
+  try {
+    orc::EvolveRecord rec;
+    int rc = orc::evolve(*M, t_end, dt, dec, inc, dmin, dmax, snap != 0, rec);
+    if (n_accepted) *n_accepted = (int)rec.accepted.size();
+    if (n_failed) *n_failed = rec.failed;
+    if (dt_next) *dt_next = dt;
+    for (int k = 0; k < (int)rec.accepted.size() && k < capacity; ++k) accepted[k] = rec.accepted[k];
     return rc;
   } catch (const std::exception& e) {
     return fail(e);

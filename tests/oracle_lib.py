@@ -45,6 +45,7 @@ def load():
         "orc_set_state": (I, [P, P, D]),
         "orc_get_state": (I, [P, P, P]),
         "orc_step": (I, [P, D, ctypes.POINTER(I), ctypes.POINTER(I)]),
+        "orc_evolve": (I, [P, D, D, D, D, D, D, I, P, I, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(D)]),
         "orc_last_assembly_seconds": (D, [P]),
         "orc_last_solve_seconds": (D, [P]),
         "orc_nnz": (Lg, [P]),
@@ -117,6 +118,16 @@ class OracleModel:
         t = ctypes.c_double()
         self._L.orc_get_state(self._h, _p(u), ctypes.byref(t))
         return u, t.value
+
+    def evolve(self, t_end, dt, decrease=0.5, increase=1.1, min_step=0.0, max_step=0.0, snap=True):
+        """Adaptive time loop; returns (status, accepted step sizes, failed attempts, next dt)."""
+        acc = np.zeros(100000)
+        na, nf, dn = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        rc = self._L.orc_evolve(self._h, t_end, dt, decrease, increase, min_step, max_step, 1 if snap else 0,
+                                _p(acc), acc.size, ctypes.byref(na), ctypes.byref(nf), ctypes.byref(dn))
+        if rc < 0:
+            raise RuntimeError(self._L.orc_last_error().decode())
+        return rc, acc[:na.value].copy(), nf.value, dn.value
 
     def step(self, dt):
         ni, li = ctypes.c_int(), ctypes.c_int()

@@ -137,6 +137,12 @@ initial.expression = {text}
      "reaction.expression = u*(\n", "expression"),
     ("[grid]\ndimension = 2\n[compartments]\nd.expression = 1\n[model.scalar_field.u]\ncompartment = e\n",
      "unknown compartment"),
+    # istl's SeqJac sweeps and relaxation: only the defaults (one sweep, weight 1) are the diagonal inverse
+    ("[grid]\ndimension = 2\n[compartments]\nd.expression = 1\n[model.scalar_field.u]\ncompartment = d\n"
+     "[model.time_step_operator]\nlinear_solver.preconditioner.type = Jacobi\n"
+     "linear_solver.preconditioner.relaxation = 0.8\n", "Jacobi with relaxation != 1"),
+    ("[grid]\ndimension = 2\n[compartments]\nd.expression = 1\n[model.scalar_field.u]\ncompartment = d\n"
+     "[model.time_step_operator]\nlinear_solver.preconditioner.type = AMG\n", "preconditioner 'AMG'"),
 ])
 def test_errors_are_reported(dca, ini, msg):
     with pytest.raises(dca.DcgError, match=re.escape(msg)):

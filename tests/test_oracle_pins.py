@@ -342,3 +342,106 @@ def test_oracle_context_functions_and_zero_literal():
     for zero in ["0", "0.0", "-0", "", "  0.  "]:
         assert _eval(zero)[1] == 1  # term absent, as the reference treats it
     assert _eval("no_value")[0] == np.finfo(float).max
+
+
+# ---------------------------------------------------------------- time loop
+def _gs_fail_ini(nl_max=3):
+    """Gray-Scott with three Newton iterations allowed: steps of 2 and more
+    fail, steps of 1 and less converge."""
+    return f"""[grid]
+dimension = 2
+cells = 6 6
+cell_type = cube
+[compartments]
+domain.expression = 1
+[model.scalar_field.u]
+compartment = domain
+storage.expression = 1
+cross_diffusion.u.expression = 2e-3
+reaction.expression = -u*v^2 + 0.04*(1-u)
+reaction.jacobian.u.expression = -v^2 - 0.04
+reaction.jacobian.v.expression = -2*u*v
+[model.scalar_field.v]
+compartment = domain
+storage.expression = 1
+cross_diffusion.v.expression = 1e-3
+reaction.expression = u*v^2 - 0.1*v
+reaction.jacobian.u.expression = v^2
+reaction.jacobian.v.expression = 2*u*v - 0.1
+[model.time_step_operator]
+type = ImplicitEuler
+linear_solver.preconditioner.type = Jacobi
+linear_solver.convergence_condition.relative_tolerance = 1e-12
+nonlinear_solver.convergence_condition.relative_tolerance = 1e-16
+nonlinear_solver.convergence_condition.iteration_range = 0 {nl_max}
+"""
+
+
+def _gs_state(o):
+    n = o.n_dof // 2
+    u = np.empty(o.n_dof)
+    k = np.arange(n)
+    u[0::2] = 0.5 + 0.4 * np.sin(0.7 * k)
+    u[1::2] = 0.3 + 0.25 * np.cos(1.3 * k)
+    return u
+
+
+@pytest.mark.parametrize("rk", ["ImplicitEuler", "Alexander2"])
+def test_evolve_closed_form_and_snap(rk):
+    """exp.ini under the adaptive loop: the accepted steps tile [0, T]
+    exactly, grow geometrically until two steps before T, finish with equal
+    snapped steps, and the state is the product of the scheme's stability
+    function over exactly those steps."""
+    o = OracleModel(_exp_ini(2, "cube", rk))
+    o.set_state(np.ones(o.n_dof), 0.0)
+    T, dt0, inc = 1.7, 0.05, 1.1
+    rc, acc, failed, _ = o.evolve(T, dt0, increase=inc)
+    assert rc == 0 and failed == 0
+    u, t = o.get_state()
+    assert abs(t - T) <= 1e-14 and abs(acc.sum() - T) <= 1e-13
+    # growth phase: each accepted step is 1.1 times the one before it
+    k = 0
+    while k + 1 < acc.size and abs(acc[k + 1] - inc * acc[k]) <= 1e-14 * acc[k + 1]:
+        k += 1
+    assert k >= 5
+    # snap phase: the remaining steps split the remainder into equal parts
+    # (each snapped step recomputes n = ceil(remaining / (1.1 dt)))
+    assert acc[k + 1:].size >= 1
+    # stability function of the scheme over the accepted steps
+    z = -2.0 * acc
+    if rk == "ImplicitEuler":
+        R = 1.0 / (1.0 - z)
+    else:
+        R = (1.0 + (1.0 - 2.0 * ALPHA) * z) / (1.0 - ALPHA * z) ** 2
+    assert np.abs(u - np.prod(R)).max() <= 1e-12
+
+
+def test_evolve_retries_failed_steps_from_the_same_state():
+    """Steps too large for one Newton iteration fail and are retried at half
+    the size from the unchanged state: replaying the accepted sizes with
+    plain steps gives the same state bit for bit."""
+    o = OracleModel(_gs_fail_ini())
+    u0 = _gs_state(o)
+    o.set_state(u0, 0.0)
+    rc, acc, failed, _ = o.evolve(40.0, 16.0, increase=1.5, min_step=1e-3)
+    assert rc == 0 and failed >= 4
+    u, t = o.get_state()
+    assert abs(t - 40.0) <= 1e-12
+    r = OracleModel(_gs_fail_ini())
+    r.set_state(u0, 0.0)
+    for h in acc:
+        rc, _, _ = r.step(h)
+        assert rc == 0
+    ur, tr = r.get_state()
+    assert np.array_equal(u, ur) and tr == t
+
+
+def test_evolve_stops_at_the_minimum_step():
+    o = OracleModel(_gs_fail_ini())
+    u0 = _gs_state(o)
+    o.set_state(u0, 0.0)
+    rc, acc, failed, _ = o.evolve(40.0, 16.0, min_step=4.0)
+    # 16, 8 and 4 are tried and fail; 2 is below the minimum
+    assert rc == 5 and acc.size == 0 and failed == 3
+    u, t = o.get_state()
+    assert t == 0.0 and np.array_equal(u, u0)
