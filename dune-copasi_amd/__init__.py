@@ -88,6 +88,7 @@ _SIGS = {
     "dcg_generated_source": (_I, [_P, ctypes.c_char_p, _S, ctypes.POINTER(_S)]),
     "dcg_assemble": (_I, [_P, _P, _D, _D, _D, _P, _P]),
     "dcg_spmv": (_I, [_P, _P, _P, _P]),
+    "dcg_jacobian_apply": (_I, [_P, _P, _D, _D, _D, _P, _P]),
     "dcg_linear_solve": (_I, [_P, _P, _P, _P, _D, _I, ctypes.POINTER(_I)]),
     "dcg_profile_enable": (_I, [_P, _I]),
     "dcg_profile_read": (_I, [_P, _I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
@@ -125,7 +126,8 @@ _SIGS = {
 
 RUNTIME_PARAMETERS = 1
 
-PROFILE_KINDS = {"spmv64": 0, "spmv32": 1, "assembly_jacobian": 2, "assembly_residual": 3, "precond_setup": 4}
+PROFILE_KINDS = {"spmv64": 0, "spmv32": 1, "assembly_jacobian": 2, "assembly_residual": 3, "precond_setup": 4,
+                 "jacobian_apply": 5}
 
 
 def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
@@ -273,6 +275,15 @@ class Model:
         _check(self._lib.dcg_assemble(self._h, _ptr(x), t, mass_weight, stiffness_weight, _ptr(r),
                                       _ptr(J) if J is not None else None))
         return r, (J.reshape(self.n_blocks, B, B) if J is not None else None)
+
+    def jacobian_apply(self, x: np.ndarray, t: float, mass_weight: float, stiffness_weight: float,
+                       v: np.ndarray) -> np.ndarray:
+        """y = J(x) v without assembling J (the matrix-free Jacobian apply)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        y = np.zeros(self.n_dof)
+        _check(self._lib.dcg_jacobian_apply(self._h, _ptr(x), t, mass_weight, stiffness_weight, _ptr(v), _ptr(y)))
+        return y
 
     def spmv(self, J: np.ndarray, x: np.ndarray) -> np.ndarray:
         J = np.ascontiguousarray(J, dtype=np.float64)

@@ -217,6 +217,16 @@ int dcg_assemble(dcg_model_t h, const double* x, double t, double mass_weight, d
   });
 }
 
+int dcg_jacobian_apply(dcg_model_t h, const double* x, double t, double mass_weight, double stiffness_weight,
+                       const double* v, double* y) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    if (!x || !v || !y) throw dcopasi::Error("null vector");
+    h->m->jacobian_apply(x, t, mass_weight, stiffness_weight, v, y);
+    return 0;
+  });
+}
+
 int dcg_spmv(dcg_model_t h, const double* jacobian, const double* x, double* y) {
   if (check(h)) return -1;
   return guarded([&] {
@@ -371,7 +381,7 @@ int dcg_estimate_device_bytes(const char* ini_text, const char* overrides, size_
     dcopasi::Model m(ini_text ? ini_text : "", overrides ? overrides : "", -1);
     const auto& c = m.solver();
     *bytes = dcopasi::Model::estimate_device_bytes(m.layout(), m.rk().stages, c.precision == dcopasi::Precision::Mixed,
-                                                   c.precond == dcopasi::Preconditioner::ILU0);
+                                                   c.precond == dcopasi::Preconditioner::ILU0, c.matrix_free);
     return 0;
   });
 }

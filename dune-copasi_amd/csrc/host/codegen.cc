@@ -1017,6 +1017,63 @@ __device__ __forceinline__ void residual_lds(int p, const double* __restrict__ x
   }
 }
 
+// block rows i0 .. i0 + AI - 1 of the Jacobian block that couples the row
+// vertex (li, lj, lk) of the tile to its neighbour at offset (dx, dy, dz):
+// the staged point coefficients of the cells around the row vertex that
+// contain both vertices, weighted by the reference tables, plus diffusion
+template <int AI>
+__device__ __forceinline__ void block_rows(int dx, int dy, int dz, int li, int lj, int lk, int i0,
+                                           const unsigned char* __restrict__ cms, const double* __restrict__ work,
+                                           const double* __restrict__ dv, double ca, double (&acc)[AI][BC]) {
+#pragma unroll
+  for (int i = 0; i < AI; ++i)
+#pragma unroll
+    for (int j = 0; j < BC; ++j) acc[i][j] = 0.0;
+  if (i0 >= BC) return;
+  for (int ez = 0; ez < (DIM > 2 ? 2 : 1); ++ez) {
+    if (DIM > 2 && ((dz == 1 && ez == 0) || (dz == -1 && ez == 1))) continue;
+    for (int ey = 0; ey < (DIM > 1 ? 2 : 1); ++ey) {
+      if (DIM > 1 && ((dy == 1 && ey == 0) || (dy == -1 && ey == 1))) continue;
+      for (int ex = 0; ex < 2; ++ex) {
+        if ((dx == 1 && ex == 0) || (dx == -1 && ex == 1)) continue;
+        const int cl = (li + ex) + CXB * ((lj + ey) + CYB * (lk + ez));
+        const unsigned m = cms[cl];
+        if (!m) continue;
+        // corner bits of the row vertex and of its neighbour in this cell
+        const int cv = (1 - ex) | (DIM > 1 ? (1 - ey) << 1 : 0) | (DIM > 2 ? (1 - ez) << 2 : 0);
+        const int cw = (1 - ex + dx) | (DIM > 1 ? (1 - ey + dy) << 1 : 0) | (DIM > 2 ? (1 - ez + dz) << 2 : 0);
+        for (int p = 0; p < NCLASS; ++p) {
+          const int la = LOC[p][cv], lc = LOC[p][cw];
+          if (la < 0 || lc < 0) continue;
+          if (!((m >> p) & 1u)) continue;
+          for (int q = 0; q < NQ; ++q) {
+            const double w = PP[q][la][lc];
+            const double* mcp = work + (size_t)((p * NQ + q) * NMC) * NCB + cl;
+#pragma unroll
+            for (int n = 0; n < NMC; ++n) {
+              const int ri = MCI[n] - i0;
+              if (ri >= 0 && ri < AI) acc[AI == 1 ? 0 : (ri < AI ? ri : 0)][MCJ[n]] += w * mcp[(size_t)n * NCB];
+            }
+          }
+          if constexpr (CONST_DIFFUSION) {
+#pragma unroll
+            for (int i = 0; i < AI; ++i)
+#pragma unroll
+              for (int j = 0; j < BC; ++j) acc[i][j] += ca * kd_rt(p, la, lc, i0 + i, j);
+          } else {
+#pragma unroll
+            for (int n = 0; n < NDV; ++n) {
+              const int ri = DVI[n] - i0;
+              if (ri >= 0 && ri < AI)
+                acc[AI == 1 ? 0 : (ri < AI ? ri : 0)][DVJ[n]] += ca * dv[n] * KDU[p][la][lc][DVK[n]][DVL[n]];
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
 // one tile of vertices: the residual rows and/or the BCSR block rows it owns
 __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,
          const double* __restrict__ x, const double* __restrict__ base, double t, double cm, double ca,
@@ -1051,7 +1108,9 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-  if (mode & 2) {
+  if (mode & 6) {
+    // mode 2 writes the BCSR block rows; mode 4 applies them instead (the
+    // matrix-free Jacobian: r = J(x) v with the direction v passed as base)
     double dv[NDV > 0 ? NDV : 1];
     uniform_diffusion(t, prm, dv);
     // phase 1: Jacobian coefficients per (cell, class, point); class and point
@@ -1093,6 +1152,14 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
       }
     }
     __syncthreads();
+    if (mode & 4) {
+      // the linearization point is no longer needed: stage the direction
+      for (int b = threadIdx.x; b < NVB; b += blockDim.x) {
+        const int s = sl[b];
+        for (int i = 0; i < BC; ++i) xs[b * BC + i] = s >= 0 ? base[(size_t)s * DCG_B + i] : 0.0;
+      }
+      __syncthreads();
+    }
     if constexpr (BC <= 4) {
     // phase 2 (blocks up to 4 x 4): SPL lanes per row, split over the last
     // axis of the cells around the row vertex.  A lane reads each of its
@@ -1182,6 +1249,47 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
               }
             }
       }
+      if (mode & 4) {
+        // r = J v over the row's stencil: each lane its own slots, the two
+        // lanes of a row then add their sums
+        double ya[BC];
+#pragma unroll
+        for (int i = 0; i < BC; ++i) ya[i] = 0.0;
+#pragma unroll
+        for (int a2 = 0; a2 < SZ; ++a2)
+#pragma unroll
+          for (int a1 = 0; a1 < SY; ++a1)
+#pragma unroll
+            for (int a0 = 0; a0 < SX; ++a0) {
+              int dx = a0 - 1, dy = DIM > 1 ? a1 - 1 : 0, dz = DIM > 2 ? a2 - 1 : 0;
+              bool mine = true;
+              if constexpr (SPL == 2 && SA == 2) {
+                dz = a2 - (1 - half);
+                mine = half == 0 || a2 == 1;
+              }
+              if constexpr (SPL == 2 && SA == 1) {
+                dy = a1 - (1 - half);
+                mine = half == 0 || a1 == 1;
+              }
+              if (!mine) continue;
+              const int nb = vb + dx + XB * (dy + YB * dz);
+#pragma unroll
+              for (int i = 0; i < BC; ++i)
+#pragma unroll
+                for (int j = 0; j < BC; ++j) {
+                  const int z = ZIX[i][j];
+                  if (z >= 0) ya[i] += acc[a2][a1][a0][z >= 0 ? z : 0] * xs[nb * BC + j];
+                }
+            }
+        if constexpr (SPL == 2) {
+#pragma unroll
+          for (int i = 0; i < BC; ++i) ya[i] += __shfl_xor(ya[i], 1);
+        }
+        if (half == 0)
+          for (int i = 0; i < DCG_B; ++i)
+            r[(size_t)s * DCG_B + i] = i < BC ? ya[i < BC ? i : 0] : base[(size_t)s * DCG_B + i];
+        continue;
+      }
       const unsigned mk = own_mask[s];
       const int st = own_start[s];
 #pragma unroll
@@ -1221,6 +1329,41 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
     // row, so the accumulator stays at BC registers and small tiles still
     // fill a wave.
     constexpr int RL = DCG_B;  // lanes per row
+    if (mode & 4) {
+      // r = J v: a lane keeps its block rows' sums over all stencil offsets
+      // (the offset loop is uniform across the workgroup)
+      constexpr int PER = (NROW * RL + 255) / 256;
+      double ya[PER];
+#pragma unroll
+      for (int k2 = 0; k2 < PER; ++k2) ya[k2] = 0.0;
+      for (int o = 0; o < NST; ++o) {
+        const int dx = o % 3 - 1, dy = DIM > 1 ? (o / 3) % 3 - 1 : 0, dz = DIM > 2 ? o / 9 - 1 : 0;
+#pragma unroll
+        for (int k2 = 0; k2 < PER; ++k2) {
+          const int it = threadIdx.x + 256 * k2;
+          if (it >= NROW * RL) continue;
+          const int lr = it / RL, ib = it % RL;
+          const int li = lr % TX, lj = DIM > 1 ? (lr / TX) % TY : 0, lk = DIM > 2 ? lr / (TX * TY) : 0;
+          const int vb = (li + 1) + XB * ((lj + OY) + YB * (lk + OZ));
+          if (sl[vb] < 0 || ib >= BC) continue;
+          double acc[1][BC];
+          block_rows<1>(dx, dy, dz, li, lj, lk, ib, cms, work, dv, ca, acc);
+          const int nb = vb + dx + XB * (dy + YB * dz);
+#pragma unroll
+          for (int j = 0; j < BC; ++j) ya[k2] += acc[0][j] * xs[nb * BC + j];
+        }
+      }
+#pragma unroll
+      for (int k2 = 0; k2 < PER; ++k2) {
+        const int it = threadIdx.x + 256 * k2;
+        if (it >= NROW * RL) continue;
+        const int lr = it / RL, ib = it % RL;
+        const int li = lr % TX, lj = DIM > 1 ? (lr / TX) % TY : 0, lk = DIM > 2 ? lr / (TX * TY) : 0;
+        const int s = sl[(li + 1) + XB * ((lj + OY) + YB * (lk + OZ))];
+        if (s < 0) continue;
+        r[(size_t)s * DCG_B + ib] = ib < BC ? ya[k2] : base[(size_t)s * DCG_B + ib];
+      }
+    } else
     for (int o0 = wave; o0 < NST; o0 += nwave) {
       const int o = __builtin_amdgcn_readfirstlane(o0);
       const int dx = o % 3 - 1, dy = DIM > 1 ? (o / 3) % 3 - 1 : 0, dz = DIM > 2 ? o / 9 - 1 : 0;
@@ -1236,55 +1379,8 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
         const int pos = own_start[s] + __popc(mk & ((1u << o) - 1u));
         constexpr int AI = BC > 4 ? 1 : BC;  // block rows per lane
         double acc[AI][BC];
-#pragma unroll
-        for (int i = 0; i < AI; ++i)
-#pragma unroll
-          for (int j = 0; j < BC; ++j) acc[i][j] = 0.0;
         const int i0 = BC > 4 ? ib : 0;  // first block row of this lane
-        if (i0 < BC) {
-          for (int ez = 0; ez < (DIM > 2 ? 2 : 1); ++ez) {
-            if (DIM > 2 && ((dz == 1 && ez == 0) || (dz == -1 && ez == 1))) continue;
-            for (int ey = 0; ey < (DIM > 1 ? 2 : 1); ++ey) {
-              if (DIM > 1 && ((dy == 1 && ey == 0) || (dy == -1 && ey == 1))) continue;
-              for (int ex = 0; ex < 2; ++ex) {
-                if ((dx == 1 && ex == 0) || (dx == -1 && ex == 1)) continue;
-                const int cl = (li + ex) + CXB * ((lj + ey) + CYB * (lk + ez));
-                const unsigned m = cms[cl];
-                if (!m) continue;
-                // corner bits of v and w in this cell (uniform per wave)
-                const int cv = (1 - ex) | (DIM > 1 ? (1 - ey) << 1 : 0) | (DIM > 2 ? (1 - ez) << 2 : 0);
-                const int cw = (1 - ex + dx) | (DIM > 1 ? (1 - ey + dy) << 1 : 0) | (DIM > 2 ? (1 - ez + dz) << 2 : 0);
-                for (int p = 0; p < NCLASS; ++p) {
-                  const int la = LOC[p][cv], lc = LOC[p][cw];
-                  if (la < 0 || lc < 0) continue;
-                  if (!((m >> p) & 1u)) continue;
-                  for (int q = 0; q < NQ; ++q) {
-                    const double w = PP[q][la][lc];
-                    const double* mcp = work + (size_t)((p * NQ + q) * NMC) * NCB + cl;
-#pragma unroll
-                    for (int n = 0; n < NMC; ++n) {
-                      const int ri = MCI[n] - i0;
-                      if (ri >= 0 && ri < AI) acc[AI == 1 ? 0 : (ri < AI ? ri : 0)][MCJ[n]] += w * mcp[(size_t)n * NCB];
-                    }
-                  }
-                  if constexpr (CONST_DIFFUSION) {
-#pragma unroll
-                    for (int i = 0; i < AI; ++i)
-#pragma unroll
-                      for (int j = 0; j < BC; ++j) acc[i][j] += ca * kd_rt(p, la, lc, i0 + i, j);
-                  } else {
-#pragma unroll
-                    for (int n = 0; n < NDV; ++n) {
-                      const int ri = DVI[n] - i0;
-                      if (ri >= 0 && ri < AI)
-                        acc[AI == 1 ? 0 : (ri < AI ? ri : 0)][DVJ[n]] += ca * dv[n] * KDU[p][la][lc][DVK[n]][DVL[n]];
-                    }
-                  }
-                }
-              }
-            }
-          }
-        }
+        block_rows<AI>(dx, dy, dz, li, lj, lk, i0, cms, work, dv, ca, acc);
         double* blk = val + (size_t)pos * (DCG_B * DCG_B);
 #pragma unroll
         for (int i = 0; i < (BC > 4 ? 1 : DCG_B); ++i) {
@@ -1359,7 +1455,9 @@ extern "C" __global__ void __launch_bounds__(256)
          const double* __restrict__ x, const double* __restrict__ base, double t, double cm, double ca,
          double* __restrict__ r, double* __restrict__ val, const int* __restrict__ own_start,
          const unsigned* __restrict__ own_mask, int n0, int n1, int n2, double ox, double oy, double oz, double hx,
-         double hy, double hz, int mode, const double* __restrict__ prm) {
+         double hy, double hz, int mode, const double* __restrict__ prm, const int* __restrict__ stop) {
+  // stop: a Krylov solve's done flag (the matrix-free apply inside BiCGSTAB)
+  if (stop && *stop) return;
   %C%::tile_body(tiles[blockIdx.x], g2l, cmask, x, base, t, cm, ca, r, val, own_start, own_mask, n0, n1, n2, ox, oy,
                  oz, hx, hy, hz, mode, prm);
 }

@@ -1,6 +1,7 @@
 #include "model.hh"
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 
@@ -125,6 +126,22 @@ void Model::build(const std::string& ini, const std::string& overrides) {
   if (prec == "fp64") cfg_.precision = Precision::FP64;
   else if (prec == "mixed") cfg_.precision = Precision::Mixed;
   else throw IOError("linear_solver.precision must be 'fp64' or 'mixed'");
+  cfg_.matrix_free = c.get(p + "linear_solver.matrix_free", false);  // make_step_operator.hh:199
+  if (cfg_.matrix_free) {
+    // a matrix-free operator reaches only the preconditioners that need no
+    // matrix (solver/istl/factory/preconditioner.hh:95-98); the Richardson
+    // step with weight 1 is the identity
+    const std::string mp = c.get(p + "linear_solver.preconditioner.type", std::string("Richardson"));
+    if (mp != "Richardson" && mp != "None")
+      throw IOError("preconditioner '" + mp + "' needs an assembled matrix (linear_solver.matrix_free = true)");
+    if (c.get(p + "linear_solver.preconditioner.relaxation", 1.0) != 1.0)
+      throw NotImplemented("Richardson with relaxation != 1");
+    cfg_.precond = Preconditioner::None;
+    if (cfg_.precision != Precision::FP64) throw NotImplemented("mixed precision with the matrix-free Jacobian");
+    if (opt_.batch_member) throw NotImplemented("matrix-free batch members");
+    if (!can_apply_jacobian())
+      throw NotImplemented("the matrix-free Jacobian covers structured compartments without transmission or outflow terms");
+  }
   cfg_.lin_rtol = c.get(p + "linear_solver.convergence_condition.relative_tolerance", 1e-4);
   auto lr = c.get_list(p + "linear_solver.convergence_condition.iteration_range");
   if (lr.size() == 2) cfg_.lin_maxit = static_cast<int>(lr[1]);
@@ -148,6 +165,13 @@ void Model::build(const std::string& ini, const std::string& overrides) {
   time_ = c.get(p + "time_begin", 0.0);
   t_end_ = c.get(p + "time_end", 1.0);
   dt_initial_ = c.get(p + "time_step_initial", c.get(p + "time_step_max", 0.1));
+}
+
+bool Model::can_apply_jacobian() const {
+  if (!gen_.facet_kernels.empty()) return false;
+  for (size_t c = 0; c < L_.comps.size(); ++c)
+    if (L_.comps[c].b && gen_.tile_kernel[c].empty()) return false;
+  return true;
 }
 
 void Model::require_device() const {
@@ -339,8 +363,12 @@ void Model::setup_device() {
   packed_.resize(std::max<size_t>(1, static_cast<size_t>(L_.ndof)));
   stage_.resize(rk_.stages + 1);
   for (auto& s : stage_) s.resize(nv);
-  val_.resize(static_cast<size_t>(matrix_values()));
-  dinv_.resize(static_cast<size_t>(ns) * B * B);
+  if (cfg_.matrix_free) {
+    lin_x_.resize(nv);
+  } else {
+    val_.resize(static_cast<size_t>(matrix_values()));
+    dinv_.resize(static_cast<size_t>(ns) * B * B);
+  }
   for (auto& v : kv_) v.resize(nv);
   if (cfg_.precision == Precision::Mixed) {
     val32_.resize(static_cast<size_t>(matrix_values()));
@@ -400,7 +428,7 @@ void Model::setup_device() {
     prm.insert(prm.end(), spec_.param_values.begin(), spec_.param_values.end());
     prm_.upload(prm, stream_);
   }
-  if (!opt_.batch_member) tune_spmv();
+  if (!opt_.batch_member && !cfg_.matrix_free) tune_spmv();
   synchronize();
 }
 
@@ -510,8 +538,9 @@ void Model::launch_assembly(const double* x, double t, double cm, double ca, con
       const int* tiles = comp_dev_[c].tiles.get();
       const int* g2l = comp_dev_[c].g2l.get();
       const unsigned char* cmask = comp_dev_[c].cell_mask_grid.get();
+      const int* stop = nullptr;
       void* args[] = {&tiles, &g2l, &cmask, &x, &base_eff, &t, &cm, &ca, &r, &val, &own_start, &own_mask,
-                      &n0, &n1, &n2, &ox, &oy, &oz, &hx, &hy, &hz, &mode, &prm};
+                      &n0, &n1, &n2, &ox, &oy, &oz, &hx, &hy, &hz, &mode, &prm, &stop};
       if (comp_dev_[c].ntiles > 0)
         DCG_HIP(hipModuleLaunchKernel(tile_fn_[c], comp_dev_[c].ntiles, 1, 1, 256, 1, 1, 0, stream_, args, nullptr));
       continue;
@@ -537,6 +566,35 @@ void Model::launch_assembly(const double* x, double t, double cm, double ca, con
       const unsigned block = 64, grid = static_cast<unsigned>((count + block - 1) / block);
       DCG_HIP(hipModuleLaunchKernel(facet_fn_[k], grid, 1, 1, block, 1, 1, 0, stream_, args, nullptr));
     }
+  }
+}
+
+// The matrix-free Jacobian apply (local_operator.hh:510-524 with the
+// matrix-free instationary assembler, make_step_operator.hh:291-293): every
+// tile recomputes its cells' point coefficients at `lin` and applies the
+// block rows it owns to v, so y holds J(lin) v for all rows (padding species
+// rows are the identity, as in the assembled Jacobian).
+void Model::launch_apply(const double* lin, double t, double cm, double ca, const double* v, double* y,
+                         const int* stop) {
+  if (!can_apply_jacobian()) throw NotImplemented("Jacobian apply without the tiled kernel or with facet terms");
+  const Grid& g = L_.grid;
+  int n0 = static_cast<int>(g.n[0]), n1 = g.dim > 1 ? static_cast<int>(g.n[1]) : 1;
+  int n2 = g.dim > 2 ? static_cast<int>(g.n[2]) : 1;
+  double ox = g.origin[0], oy = g.origin[1], oz = g.origin[2];
+  double hx = g.h[0], hy = g.dim > 1 ? g.h[1] : 0.0, hz = g.dim > 2 ? g.h[2] : 0.0;
+  const int* own_start = own_start_.get();
+  const unsigned* own_mask = own_mask_.get();
+  const double* prm = prm_.get();
+  double* val = nullptr;
+  int mode = 4;
+  for (size_t c = 0; c < L_.comps.size(); ++c) {
+    if (!tile_fn_[c] || comp_dev_[c].ntiles <= 0) continue;
+    const int* tiles = comp_dev_[c].tiles.get();
+    const int* g2l = comp_dev_[c].g2l.get();
+    const unsigned char* cmask = comp_dev_[c].cell_mask_grid.get();
+    void* args[] = {&tiles, &g2l, &cmask, &lin, &v, &t, &cm, &ca, &y, &val, &own_start, &own_mask,
+                    &n0, &n1, &n2, &ox, &oy, &oz, &hx, &hy, &hz, &mode, &prm, &stop};
+    DCG_HIP(hipModuleLaunchKernel(tile_fn_[c], comp_dev_[c].ntiles, 1, 1, 256, 1, 1, 0, stream_, args, nullptr));
   }
 }
 
@@ -588,22 +646,33 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
       k::ilu_upper<double>(B, up_ptr_[l + 1] - up_ptr_[l], up_rows_.get() + up_ptr_[l], row_ptr_.get(), col_.get(),
                            diag_pos_.get(), lu_.get(), dinv_.get(), tmp, out, sc_.get(), stream_);
   };
+  // out = J in with the partial dots (out.w1, out.w2) finalised into `stage`
+  const int* done_flag =
+      reinterpret_cast<const int*>(reinterpret_cast<const char*>(sc_.get()) + offsetof(k::BicgScalars, done));
+  auto matvec = [&](const double* in, double* out, const double* w1, const double* w2, int stage) {
+    if (cfg_.matrix_free) {
+      prof_begin(kJacobianApply);
+      launch_apply(lin_x_.get(), lin_t_, lin_cm_, lin_ca_, in, out, done_flag);
+      prof_end(kJacobianApply, apply_bytes());
+      k::dot_parts(n, out, w1, w2, part_.get(), nred, sc_.get(), stream_);
+      k::bicg_fin(nred, part_.get(), 2, stage, sc_.get(), stream_);
+      return;
+    }
+    prof_begin(kSpmv64);
+    k::spmv(B, plan_, val_.get(), in, out, w1, w2, part_.get(), sc_.get(), stage, stream_);
+    k::bicg_fin(plan_.parts(), part_.get(), 2, stage, sc_.get(), stream_);
+    prof_end(kSpmv64, spmv_bytes(8, 8, 1));
+  };
   int done_its = 0, batch = std::max(2, std::min(last_krylov_its_, 64));
   for (;;) {
     for (int it = 0; it < batch; ++it) {
       k::bicg_p<double>(B, ns, r, p, v, ph, dinv, sc_.get(), stream_);
       if (ilu) apply_ilu(p, ph);
-      prof_begin(kSpmv64);
-      k::spmv(B, plan_, val_.get(), ph, v, rh, nullptr, part_.get(), sc_.get(), 1, stream_);
-      k::bicg_fin(plan_.parts(), part_.get(), 2, 1, sc_.get(), stream_);
-      prof_end(kSpmv64, spmv_bytes(8, 8, 1));
+      matvec(ph, v, rh, nullptr, 1);
       k::bicg_s<double>(B, ns, r, v, s, sh, dinv, sc_.get(), part_.get(), stream_);
       if (ilu) apply_ilu(s, sh);
       k::bicg_fin(nblk, part_.get(), 1, 2, sc_.get(), stream_);
-      prof_begin(kSpmv64);
-      k::spmv(B, plan_, val_.get(), sh, t, s, t, part_.get(), sc_.get(), 3, stream_);
-      k::bicg_fin(plan_.parts(), part_.get(), 2, 3, sc_.get(), stream_);
-      prof_end(kSpmv64, spmv_bytes(8, 8, 1));
+      matvec(sh, t, s, t, 3);
       k::bicg_xr<double>(B, ns, x, ph, sh, s, t, r, rh, sc_.get(), part_.get(), stream_);
       k::bicg_fin(nblk, part_.get(), 2, 4, sc_.get(), stream_);
     }
@@ -757,6 +826,14 @@ int Model::step(double t, double dt, StepStats* st) {
       prof_end(kAssemblyRes, assembly_bytes(false));
     };
     auto jacobian = [&]() {
+      if (cfg_.matrix_free) {
+        // the operator keeps its linearization point (the Newton iterate)
+        DCG_HIP(hipMemcpyAsync(lin_x_.get(), x, bytes, hipMemcpyDeviceToDevice, stream_));
+        lin_t_ = ts;
+        lin_cm_ = cm;
+        lin_ca_ = ca;
+        return;
+      }
       prof_begin(kAssemblyJac);
       launch_assembly(x, ts, cm, ca, nullptr, nullptr, val_.get(), 2);
       prof_end(kAssemblyJac, assembly_bytes(true));
@@ -983,9 +1060,19 @@ double Model::assembly_bytes(bool jacobian) const {
   return b;
 }
 
+// Algorithmic bytes of one matrix-free apply: the linearization point and v
+// read, y written, and the same per-cell tables as the assembly.
+double Model::apply_bytes() const {
+  const double N = static_cast<double>(L_.nslot), B = L_.B;
+  double cells = 0;
+  for (const auto& C : L_.comps) cells += static_cast<double>(C.cells.size());
+  return 8.0 * B * N * 3.0 + cells * (4.0 + 1.0 + 4.0 * L_.grid.ncorner);
+}
+
 // ------------------------------------------------------------------ test entry points
 void Model::assemble(const double* x_packed, double t, double cm, double ca, double* r_packed, double* jac) {
   require_device();
+  if (jac && cfg_.matrix_free) throw Error("a matrix-free model holds no assembled Jacobian");
   const long n = L_.nslot * L_.B;
   DCG_HIP(hipMemcpyAsync(packed_.get(), x_packed, L_.ndof * sizeof(double), hipMemcpyHostToDevice, stream_));
   k::unpack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), packed_.get(), xs_.get(), stream_);
@@ -997,8 +1084,22 @@ void Model::assemble(const double* x_packed, double t, double cm, double ca, dou
   synchronize();
 }
 
+void Model::jacobian_apply(const double* x_packed, double t, double cm, double ca, const double* v_packed,
+                           double* y_packed) {
+  require_device();
+  DCG_HIP(hipMemcpyAsync(packed_.get(), x_packed, L_.ndof * sizeof(double), hipMemcpyHostToDevice, stream_));
+  k::unpack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), packed_.get(), xs_.get(), stream_);
+  DCG_HIP(hipMemcpyAsync(packed_.get(), v_packed, L_.ndof * sizeof(double), hipMemcpyHostToDevice, stream_));
+  k::unpack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), packed_.get(), z_.get(), stream_);
+  launch_apply(xs_.get(), t, cm, ca, z_.get(), r_.get(), nullptr);
+  k::pack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), r_.get(), packed_.get(), stream_);
+  DCG_HIP(hipMemcpyAsync(y_packed, packed_.get(), L_.ndof * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  synchronize();
+}
+
 void Model::spmv(const double* jac, const double* x_packed, double* y_packed) {
   require_device();
+  if (cfg_.matrix_free) throw Error("a matrix-free model holds no assembled Jacobian");
   DCG_HIP(hipMemcpyAsync(val_.get(), jac, val_.bytes(), hipMemcpyHostToDevice, stream_));
   DCG_HIP(hipMemcpyAsync(packed_.get(), x_packed, L_.ndof * sizeof(double), hipMemcpyHostToDevice, stream_));
   k::unpack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), packed_.get(), xs_.get(), stream_);
@@ -1011,6 +1112,7 @@ void Model::spmv(const double* jac, const double* x_packed, double* y_packed) {
 int Model::linear_solve(const double* jac, const double* b_packed, double* x_packed, double rtol, int maxit,
                         int* its) {
   require_device();
+  if (cfg_.matrix_free) throw Error("a matrix-free model holds no assembled Jacobian");
   DCG_HIP(hipMemcpyAsync(val_.get(), jac, val_.bytes(), hipMemcpyHostToDevice, stream_));
   DCG_HIP(hipMemcpyAsync(packed_.get(), b_packed, L_.ndof * sizeof(double), hipMemcpyHostToDevice, stream_));
   k::unpack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), packed_.get(), r_.get(), stream_);
@@ -1058,10 +1160,10 @@ size_t Model::device_bytes() const {
 // Device bytes of a model with layout L, before anything is allocated: the
 // matrix (FP64, its FP32 copy when mixed, the ILU factor when used), the
 // block-diagonal inverse, the per-slot vectors and the mesh tables.
-size_t Model::estimate_device_bytes(const Layout& L, int stages, bool mixed, bool ilu) {
-  const double nval = static_cast<double>(L.nnzb()) * L.B * L.B;
+size_t Model::estimate_device_bytes(const Layout& L, int stages, bool mixed, bool ilu, bool matrix_free) {
+  const double nval = matrix_free ? 0.0 : static_cast<double>(L.nnzb()) * L.B * L.B;
   const double ns = static_cast<double>(L.nslot), nv = ns * L.B;
-  double b = 8.0 * nval * (ilu ? 2.0 : 1.0) + (mixed ? 4.0 * nval : 0.0) + 8.0 * ns * L.B * L.B;
+  double b = 8.0 * nval * (ilu ? 2.0 : 1.0) + (mixed ? 4.0 * nval : 0.0) + (matrix_free ? 8.0 * nv : 8.0 * ns * L.B * L.B);
   b += 8.0 * nv * (5 + stages + 1 + 9) + (mixed ? 4.0 * nv * 10 : 0.0) + 8.0 * L.ndof;
   b += 4.0 * (ns + 1) + 4.0 * L.nnzb() + 4.0 * ns * 4 + 8.0 * ns * 2;
   double cells = 0;

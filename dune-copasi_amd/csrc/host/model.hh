@@ -56,6 +56,9 @@ struct SolverConfig {
   bool fixed_linear_tolerance = false;  // nonlinear_solver.dx_inverse_fixed_tolerance
   double min_linear_reduction = 0.1;    // nonlinear_solver.dx_inverse_min_relative_tolerance
   double linearization_threshold = 0.0; // re-linearize when defect_k/defect_{k-1} exceeds it
+  // linear_solver.matrix_free: BiCGSTAB applies the Jacobian at the
+  // linearization point from the generated kernels, no matrix is assembled
+  bool matrix_free = false;
   double ir_inner_rtol = 1e-3;  // FP32 Krylov tolerance inside refinement
   int ir_max = 50;
   std::vector<std::string> warnings;
@@ -114,15 +117,28 @@ class Model {
   void get_params(double* values, size_t n) const;
   // bytes of device memory the model holds (after set-up) or would hold
   size_t device_bytes() const;
-  static size_t estimate_device_bytes(const Layout& L, int stages, bool mixed, bool ilu);
+  static size_t estimate_device_bytes(const Layout& L, int stages, bool mixed, bool ilu, bool matrix_free = false);
 
   // kernel-level entry points (parity tests)
   void assemble(const double* x_packed, double t, double cm, double ca, double* r_packed, double* jac);
   void spmv(const double* jac, const double* x_packed, double* y_packed);
   int linear_solve(const double* jac, const double* b_packed, double* x_packed, double rtol, int maxit, int* its);
+  // y = J(x) v without assembling J (the matrix-free Jacobian apply), for
+  // models whose compartments all use the tiled kernel and have no facets
+  void jacobian_apply(const double* x_packed, double t, double cm, double ca, const double* v_packed,
+                      double* y_packed);
+  bool can_apply_jacobian() const;
 
   // kernel timing with HIP events on the model's stream (benchmarks)
-  enum ProfKind { kSpmv64 = 0, kSpmv32 = 1, kAssemblyJac = 2, kAssemblyRes = 3, kPrecond = 4, kProfKinds = 5 };
+  enum ProfKind {
+    kSpmv64 = 0,
+    kSpmv32 = 1,
+    kAssemblyJac = 2,
+    kAssemblyRes = 3,
+    kPrecond = 4,
+    kJacobianApply = 5,
+    kProfKinds = 6
+  };
   void set_profiling(bool on);
   // launches, summed milliseconds and summed algorithmic bytes of one kind
   void profile(int kind, long* launches, double* ms, double* bytes);
@@ -153,6 +169,10 @@ class Model {
   // 1 prepare r / val, 2 volume kernels, 4 facet kernels
   void launch_assembly(const double* x, double t, double cm, double ca, const double* base, double* r, double* val,
                        int mode, bool skip_tiled = false, int phases = 7);
+  // y = J(lin) v from the tiled kernels (mode 4); stop: a done flag that
+  // turns the launches into no-ops
+  void launch_apply(const double* lin, double t, double cm, double ca, const double* v, double* y,
+                    const int* stop);
   // compartments without a batched tile kernel, or facets: per-model launches
   bool needs_own_assembly() const;
   double residual_norm(const double* r);
@@ -177,6 +197,7 @@ class Model {
 
  private:
   double assembly_bytes(bool jacobian) const;
+  double apply_bytes() const;
 
   ParameterTree cfg_tree_;
   ModelSpec spec_;
@@ -220,6 +241,9 @@ class Model {
   DeviceArray<double> weights_;  // lumped mass per slot (mass sums)
   DeviceArray<double> prm_;      // geometry words + run-time parameters
   DeviceArray<double> u_, hist_, r_, z_, xs_, packed_;
+  // matrix-free: the Newton iterate the Jacobian is applied at, and its weights
+  DeviceArray<double> lin_x_;
+  double lin_t_ = 0.0, lin_cm_ = 0.0, lin_ca_ = 0.0;
   std::vector<DeviceArray<double>> stage_;
   DeviceArray<double> val_, lu_, dinv_;
   DeviceArray<float> val32_;

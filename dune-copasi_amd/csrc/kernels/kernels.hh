@@ -162,6 +162,11 @@ void spmv(int B, const SpmvPlan& pl, const double* val, const double* x, double*
 void spmv(int B, const SpmvPlan& pl, const float* val, const float* x, float* y, const float* w1, const float* w2,
           double* part, BicgScalars* sc, int stage, hipStream_t s);
 
+// partial dots (y.w1, y.w2) per workgroup into part[2 g], part[2 g + 1] over
+// `groups` workgroups (w2 may be null); nothing once sc->done
+void dot_parts(long n, const double* y, const double* w1, const double* w2, double* part, int groups,
+               const BicgScalars* sc, hipStream_t s);
+
 // Jacobi preconditioner blocks: the inverse of each diagonal B x B block, or
 // with point set only the inverse of its diagonal (in the same layout)
 void block_inverse(int B, int nslot, const double* val, const int* diag_pos, double* dinv, int* status, bool point,

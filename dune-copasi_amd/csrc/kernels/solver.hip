@@ -303,6 +303,25 @@ __global__ void __launch_bounds__(kThreads) bicg_s_k(int nslot, const V* __restr
   if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
 }
 
+// partial (y.w1), (y.w2) per workgroup over a fixed grid (deterministic)
+__global__ void __launch_bounds__(kThreads) dot_parts_k(long n, const double* __restrict__ y,
+                                                        const double* __restrict__ w1, const double* __restrict__ w2,
+                                                        double* __restrict__ part, const BicgScalars* __restrict__ sc) {
+  if (sc && finished(sc)) return;
+  __shared__ double red[32];
+  double acc[2] = {0.0, 0.0};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double v = y[i];
+    acc[0] += v * w1[i];
+    if (w2) acc[1] += v * w2[i];
+  }
+  block_sum(acc, red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = acc[0];
+    part[2 * blockIdx.x + 1] = acc[1];
+  }
+}
+
 // x += alpha ph + omega sh ; r = s - omega t ; partial (rh.r), (r.r)
 template <int B, typename V>
 __global__ void __launch_bounds__(kThreads) bicg_xr_k(int nslot, V* __restrict__ x, const V* __restrict__ ph,
@@ -677,6 +696,11 @@ void spmv(int B, const SpmvPlan& pl, const double* val, const double* x, double*
 void spmv(int B, const SpmvPlan& pl, const float* val, const float* x, float* y, const float* w1, const float* w2,
           double* part, BicgScalars* sc, int stage, hipStream_t s) {
   DCG_DISPATCH_B(B, (spmv_t<BB, float, float>(pl, val, x, y, w1, w2, part, sc, stage, s)));
+}
+
+void dot_parts(long n, const double* y, const double* w1, const double* w2, double* part, int groups,
+               const BicgScalars* sc, hipStream_t s) {
+  dot_parts_k<<<groups, kThreads, 0, s>>>(n, y, w1, w2, part, sc);
 }
 
 void block_inverse(int B, int nslot, const double* val, const int* diag_pos, double* dinv, int* status, bool point,

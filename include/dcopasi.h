@@ -108,6 +108,14 @@ int dcg_generated_source(dcg_model_t model, char* buf, size_t len, size_t* neede
 int dcg_assemble(dcg_model_t model, const double* x, double t, double mass_weight, double stiffness_weight,
                  double* residual, double* jacobian);
 
+/* y = J(x) v without assembling J: the matrix-free Jacobian apply
+ * (localAssembleJacobianVolumeApply, local_operator.hh:510-524, through the
+ * matrix-free instationary assembler of make_step_operator.hh:291-293), the
+ * operator BiCGSTAB uses with linear_solver.matrix_free = true.  Models whose
+ * compartments are structured and carry no transmission or outflow terms. */
+int dcg_jacobian_apply(dcg_model_t model, const double* x, double t, double mass_weight, double stiffness_weight,
+                       const double* v, double* y);
+
 /* y = J x for BCSR values J on the model's pattern. */
 int dcg_spmv(dcg_model_t model, const double* jacobian, const double* x, double* y);
 
@@ -121,7 +129,7 @@ int dcg_linear_solve(dcg_model_t model, const double* jacobian, const double* b,
  * reference's Perfetto counters (dune/copasi/common/stepper.hh:100-102,
  * dune/copasi/model/make_step_operator.hh:137-139).  kind: 0 FP64 SpMV,
  * 1 FP32 SpMV, 2 Jacobian assembly pass, 3 residual assembly pass,
- * 4 preconditioner set-up.  Enabling resets the counters; `bytes` sums the
+ * 4 preconditioner set-up, 5 matrix-free Jacobian apply.  Enabling resets the counters; `bytes` sums the
  * algorithmic bytes of the timed launches. */
 int dcg_profile_enable(dcg_model_t model, int on);
 int dcg_profile_read(dcg_model_t model, int kind, long* launches, double* milliseconds, double* bytes);

@@ -1345,7 +1345,9 @@ struct Precond {
       for (long i = 0; i < A.n; ++i)
         for (long p = A.rp[i]; p < A.rp[i + 1]; ++p)
           if (A.ci[p] == i) dinv[i] = 1.0 / A.v[p];
-    } else if (t != "None") {
+    } else if (t != "None" && t != "Richardson") {
+      // Richardson: istl's SeqRichardson with its default weight 1, the
+      // identity (the preconditioner of the matrix-free solve)
       throw std::runtime_error("oracle: preconditioner '" + t + "' not supported");
     }
   }

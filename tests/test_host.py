@@ -143,6 +143,20 @@ initial.expression = {text}
      "linear_solver.preconditioner.relaxation = 0.8\n", "Jacobi with relaxation != 1"),
     ("[grid]\ndimension = 2\n[compartments]\nd.expression = 1\n[model.scalar_field.u]\ncompartment = d\n"
      "[model.time_step_operator]\nlinear_solver.preconditioner.type = AMG\n", "preconditioner 'AMG'"),
+    # a matrix-free operator has no matrix for Jacobi / ILU (preconditioner.hh:95-117)
+    ("[grid]\ndimension = 2\ncells = 4 4\ncell_type = cube\n[compartments]\nd.expression = 1\n"
+     "[model.scalar_field.u]\ncompartment = d\nstorage.expression = 1\n"
+     "[model.time_step_operator]\nlinear_solver.matrix_free = true\nlinear_solver.preconditioner.type = Jacobi\n",
+     "needs an assembled matrix"),
+    ("[grid]\ndimension = 2\ncells = 4 4\ncell_type = cube\n[compartments]\nd.expression = 1\n"
+     "[model.scalar_field.u]\ncompartment = d\nstorage.expression = 1\n"
+     "[model.time_step_operator]\nlinear_solver.matrix_free = true\nlinear_solver.precision = mixed\n",
+     "mixed precision with the matrix-free Jacobian"),
+    ("[grid]\ndimension = 2\ncells = 4 4\ncell_type = cube\n[compartments]\nd.expression = 1\n"
+     "[model.scalar_field.u]\ncompartment = d\nstorage.expression = 1\noutflow.d.expression = -u\n"
+     "outflow.d.jacobian.u.expression = -1\n"
+     "[model.time_step_operator]\nlinear_solver.matrix_free = true\n",
+     "without transmission or outflow terms"),
 ])
 def test_errors_are_reported(dca, ini, msg):
     with pytest.raises(dca.DcgError, match=re.escape(msg)):
@@ -177,6 +191,14 @@ def test_generated_kernels_compile_for_gfx950(dca, wl, make):
     # would be dead code and dominates compile time)
     assert "dcg_tile_0" in src and "dcg_volume_0" not in src
     rc, log = _hiprtc_compile(src)
+    assert rc == 0, log
+
+
+def test_large_block_kernel_source_compiles(dca, wl):
+    """8 species in 3D: the block rows are split over lanes (and the
+    matrix-free apply keeps one block row per lane)."""
+    m = dca.Model(wl.random_request(18, side3=(5, 6)).ini, "", -1)
+    rc, log = _hiprtc_compile(m.generated_source())
     assert rc == 0, log
 
 
