@@ -81,27 +81,59 @@ def committed_traffic(workload, precision, chosen):
     return float(kv["traffic_bytes"])
 
 
-def cpu_baseline(wl_obj, steps: int):
-    """Oracle on the host cores: `steps` time steps of the same workload."""
+def _host_threads():
+    """Host threads this process may use: its CPU affinity, capped by OMP_NUM_THREADS."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(env))) if env.isdigit() else n
+
+
+def _oracle_steps(o, w, steps):
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rc, _, _ = o.step(w.dt)
+        if rc:
+            raise RuntimeError(f"oracle step failed: {rc}")
+    el = time.perf_counter() - t0
+    return steps * w.stages * o.n_dof, el
+
+
+def cpu_baseline(wl_obj, steps: int, small=None):
+    """Oracle on the host cores: `steps` time steps of the same workload with
+    OpenMP-coloured assembly and a sequential BiCGSTAB (as istl's), then one
+    more step with the Krylov solve threaded too (the stronger baseline), and
+    one step of `small` (a smaller mesh of the same model, or the workload
+    itself) on one thread."""
     sys.path.insert(0, str(ROOT))
     from tests import oracle_lib  # the oracle is only ever the baseline here
 
     L = oracle_lib.load()
-    cores = os.cpu_count() or 1
+    cores = _host_threads()
     L.orc_set_threads(cores)
+    L.orc_set_krylov_threads(0)
     o = oracle_lib.OracleModel(wl_obj.ini)
     o.set_state(wl_obj.initial_state(o), 0.0)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        rc, _, _ = o.step(wl_obj.dt)
-        if rc:
-            raise RuntimeError(f"oracle step failed: {rc}")
-    el = time.perf_counter() - t0
-    dof_steps = steps * wl_obj.stages * o.n_dof
-    return {"value": dof_steps / el, "unit": "Newton DOF-steps/s", "cores": int(L.orc_max_threads()),
-            "kind": "port",
+    dof_steps, el = _oracle_steps(o, wl_obj, steps)
+    L.orc_set_krylov_threads(1)
+    t_dof, t_el = _oracle_steps(o, wl_obj, 1)
+    L.orc_set_krylov_threads(0)
+    o.close()
+    w1 = small or wl_obj
+    L.orc_set_threads(1)
+    o1 = oracle_lib.OracleModel(w1.ini)
+    o1.set_state(w1.initial_state(o1), 0.0)
+    s_dof, s_el = _oracle_steps(o1, w1, 1)
+    o1.close()
+    L.orc_set_threads(cores)
+    return {"value": dof_steps / el, "unit": "Newton DOF-steps/s", "cores": cores, "kind": "port",
             "sample": f"{steps} time step(s) of the same workload from the same initial state "
-                      f"({dof_steps} Newton DOF-steps, {el:.1f} s, OpenMP-coloured assembly, sequential BiCGSTAB)"}
+                      f"({dof_steps} Newton DOF-steps, {el:.1f} s, OpenMP-coloured assembly, sequential BiCGSTAB)",
+            "threaded_krylov": {"value": t_dof / t_el, "cores": cores,
+                                "sample": f"the next time step with SpMV, dots and updates on all threads "
+                                          f"({t_dof} Newton DOF-steps, {t_el:.1f} s)"},
+            "one_thread": {"value": s_dof / s_el, "cores": 1,
+                           "sample": f"one time step of {'the ' + str(o1.n_dof) + '-DOF mesh of the same model' if small else 'the same workload'} "
+                                     f"({s_dof} Newton DOF-steps, {s_el:.1f} s)"}}
 
 
 def cpu_baseline_batch(ws, dof_steps_budget: float):
@@ -111,7 +143,8 @@ def cpu_baseline_batch(ws, dof_steps_budget: float):
     from tests import oracle_lib  # the oracle is only ever the baseline here
 
     L = oracle_lib.load()
-    L.orc_set_threads(os.cpu_count() or 1)
+    L.orc_set_threads(_host_threads())
+    L.orc_set_krylov_threads(0)
     done, el, used = 0, 0.0, 0
     for w in ws:
         o = oracle_lib.OracleModel(w.ini)
@@ -412,7 +445,7 @@ def main():
                     help="Jacobi: the reference's (diagonal); VertexBlockJacobi: inverse of each vertex's species block")
     ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
                     help="default: fp64 for single models, mixed for gs_batch and serve")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true",
                     help="single-model FP64 runs: skip the mixed-precision run of the same workload")
@@ -547,7 +580,9 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(w, args.cpu_steps)
+        # one thread on the 48^3 mesh of the same model (a 96^3 step takes minutes on one core)
+        small = make_workload(wl, "ms48", "fp64", args.preconditioner)[0] if args.workload == "ms96" else None
+        cpu = cpu_baseline(w, args.cpu_steps, small)
     out = {
         "metric": "Newton DOF-steps/s (plain FP64 vs mixed/batched), p50/p99 latency, HBM GB/s",
         "value": value,

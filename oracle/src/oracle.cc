@@ -1273,8 +1273,14 @@ struct CSR {
   double* v;
 };
 
+// The Krylov solve is sequential, as istl's BiCGSTABSolver is; with
+// krylov_threads its matvecs, dots and vector updates run on all threads
+// (a stronger CPU baseline; dots then sum in another order).
+static bool krylov_threads = false;
+static bool par(long n) { return krylov_threads && n > 20000; }
+
 static void spmv(const CSR& A, const double* x, double* y) {
-#pragma omp parallel for schedule(static) if (A.n > 20000)
+#pragma omp parallel for schedule(static) if (par(A.n))
   for (long i = 0; i < A.n; ++i) {
     double s = 0;
     for (long p = A.rp[i]; p < A.rp[i + 1]; ++p) s += A.v[p] * x[A.ci[p]];
@@ -1284,6 +1290,7 @@ static void spmv(const CSR& A, const double* x, double* y) {
 
 static double dot(long n, const double* a, const double* b) {
   double s = 0;
+#pragma omp parallel for schedule(static) reduction(+ : s) if (par(n))
   for (long i = 0; i < n; ++i) s += a[i] * b[i];
   return s;
 }
@@ -1353,9 +1360,12 @@ struct Precond {
   }
   void apply(long n, const double* b, double* x) const {
     if (type == "ILU" || type == "BlockJacobi") ilu.apply(b, x);
-    else if (type == "Jacobi")
+    else if (type == "Jacobi") {
+#pragma omp parallel for schedule(static) if (par(n))
       for (long i = 0; i < n; ++i) x[i] = dinv[i] * b[i];
-    else std::copy(b, b + n, x);
+    } else {
+      std::copy(b, b + n, x);
+    }
   }
 };
 
@@ -1375,12 +1385,14 @@ static int bicgstab(const CSR& A, const Precond& P, const double* b, double* x, 
     double rho1 = dot(n, rh.data(), r.data());
     if (rho1 == 0.0) return 4;
     double beta = (rho1 / rho) * (alpha / omega);
+#pragma omp parallel for schedule(static) if (par(n))
     for (long i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - omega * v[i]);
     P.apply(n, p.data(), ph.data());
     spmv(A, ph.data(), v.data());
     double rv = dot(n, rh.data(), v.data());
     if (rv == 0.0) return 4;
     alpha = rho1 / rv;
+#pragma omp parallel for schedule(static) if (par(n))
     for (long i = 0; i < n; ++i) s[i] = r[i] - alpha * v[i];
     double ns = std::sqrt(dot(n, s.data(), s.data()));
     if (ns <= rtol * n0) {
@@ -1393,6 +1405,7 @@ static int bicgstab(const CSR& A, const Precond& P, const double* b, double* x, 
     spmv(A, sh.data(), t.data());
     double tt = dot(n, t.data(), t.data());
     omega = tt == 0.0 ? 0.0 : dot(n, t.data(), s.data()) / tt;
+#pragma omp parallel for schedule(static) if (par(n))
     for (long i = 0; i < n; ++i) {
       x[i] += alpha * ph[i] + omega * sh[i];
       r[i] = s[i] - omega * t[i];
@@ -1668,6 +1681,8 @@ void orc_set_threads(int n) {
   if (n > 0) omp_set_num_threads(n);
 #endif
 }
+
+void orc_set_krylov_threads(int on) { orc::krylov_threads = on != 0; }
 
 int orc_max_threads() {
 #ifdef _OPENMP

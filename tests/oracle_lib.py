@@ -30,6 +30,7 @@ def load():
         "orc_destroy": (None, [P]),
         "orc_last_error": (ctypes.c_char_p, []),
         "orc_set_threads": (None, [I]),
+        "orc_set_krylov_threads": (None, [I]),
         "orc_max_threads": (I, []),
         "orc_n_dof": (Lg, [P]),
         "orc_n_species": (I, [P]),
