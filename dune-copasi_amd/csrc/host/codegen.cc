@@ -1074,16 +1074,21 @@ __device__ __forceinline__ void block_rows(int dx, int dy, int dz, int li, int l
   }
 }
 
-// one tile of vertices: the residual rows and/or the BCSR block rows it owns
+// one tile of vertices: the residual rows and/or the BCSR block rows it owns.
+// MODE fixes the work at compile time (1 residual, 2 Jacobian, 4 apply), so
+// each entry point gets registers and LDS for its own path only; MODE 0
+// takes it from mode_rt (the batched entry point).
+template <int MODE>
 __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,
          const double* __restrict__ x, const double* __restrict__ base, double t, double cm, double ca,
          double* __restrict__ r, double* __restrict__ val, const int* __restrict__ own_start,
          const unsigned* __restrict__ own_mask, int n0, int n1, int n2, double ox, double oy, double oz, double hx,
-         double hy, double hz, int mode, const double* __restrict__ prm) {
+         double hy, double hz, int mode_rt, const double* __restrict__ prm) {
+  const int mode = MODE ? MODE : mode_rt;
   __shared__ double xs[NVB * BC];
   __shared__ int sl[NVB];
   __shared__ unsigned char cms[NCB];
-  __shared__ double work[UNION_WORDS];
+  __shared__ double work[MODE == 1 ? RL_WORDS : (MODE == 2 || MODE == 4) ? MC_WORDS : UNION_WORDS];
   const int ntx = (n0 + 1 + TX - 1) / TX;
   const int nty = DIM > 1 ? (n1 + 1 + TY - 1) / TY : 1;
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
@@ -1201,11 +1206,27 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
           const int la = LOC[p][cv];
           if (la < 0) continue;
           if (!((m >> p) & 1u)) continue;
-          double mv[NQ][NMC > 0 ? NMC : 1];
+          // point by point: the NMC coefficients of point q go into every
+          // corner's slot before the next point is read (NMC registers, not
+          // NQ x NMC); each slot still sums its points in order
 #pragma unroll
-          for (int q = 0; q < NQ; ++q)
+          for (int q = 0; q < NQ; ++q) {
+            double mq[NMC > 0 ? NMC : 1];
 #pragma unroll
-            for (int n = 0; n < NMC; ++n) mv[q][n] = work[(size_t)((p * NQ + q) * NMC + n) * NCB + cl];
+            for (int n = 0; n < NMC; ++n) mq[n] = work[(size_t)((p * NQ + q) * NMC + n) * NCB + cl];
+#pragma unroll
+            for (int c = 0; c < NV; ++c) {
+              const int cw = CORNER[p][c];
+              const int bx = cw & 1, by = (cw >> 1) & 1, bz = (cw >> 2) & 1;
+              const int sx = bx - (cv & 1) + 1;
+              const int sy = DIM > 1 ? ((SPL == 2 && SA == 1) ? by : by - ((cv >> 1) & 1) + 1) : 0;
+              const int sz = DIM > 2 ? (SPL == 2 ? bz : bz - ((cv >> 2) & 1) + 1) : 0;
+              double* ac = acc[sz][sy][sx];
+              const double w = PP[q][la][c];
+#pragma unroll
+              for (int n = 0; n < NMC; ++n) ac[MCZ[n]] += w * mq[n];
+            }
+          }
 #pragma unroll
           for (int c = 0; c < NV; ++c) {
             const int cw = CORNER[p][c];
@@ -1214,12 +1235,6 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
             const int sy = DIM > 1 ? ((SPL == 2 && SA == 1) ? by : by - ((cv >> 1) & 1) + 1) : 0;
             const int sz = DIM > 2 ? (SPL == 2 ? bz : bz - ((cv >> 2) & 1) + 1) : 0;
             double* ac = acc[sz][sy][sx];
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-              const double w = PP[q][la][c];
-#pragma unroll
-              for (int n = 0; n < NMC; ++n) ac[MCZ[n]] += w * mv[q][n];
-            }
             if constexpr (CONST_DIFFUSION) {
 #pragma unroll
               for (int n = 0; n < NKZ; ++n) ac[KZZ[n]] += ca * kd_rt(p, la, c, KZI[n], KZJ[n]);
@@ -1450,17 +1465,24 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
 }
 }  // namespace %C%
 
-extern "C" __global__ void __launch_bounds__(256)
-%KERNEL%(const int* __restrict__ tiles, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,
-         const double* __restrict__ x, const double* __restrict__ base, double t, double cm, double ca,
-         double* __restrict__ r, double* __restrict__ val, const int* __restrict__ own_start,
-         const unsigned* __restrict__ own_mask, int n0, int n1, int n2, double ox, double oy, double oz, double hx,
-         double hy, double hz, int mode, const double* __restrict__ prm, const int* __restrict__ stop) {
-  // stop: a Krylov solve's done flag (the matrix-free apply inside BiCGSTAB)
-  if (stop && *stop) return;
-  %C%::tile_body(tiles[blockIdx.x], g2l, cmask, x, base, t, cm, ca, r, val, own_start, own_mask, n0, n1, n2, ox, oy,
-                 oz, hx, hy, hz, mode, prm);
-}
+// one entry point per kind of work, same arguments: _r residual (r = base +
+// local residuals), _j Jacobian block rows into val, _a r = J(x) base
+#define DCG_TILE_ENTRY(NAME, M)                                                                                   \
+  extern "C" __global__ void __launch_bounds__(256)                                                               \
+  NAME(const int* __restrict__ tiles, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,         \
+       const double* __restrict__ x, const double* __restrict__ base, double t, double cm, double ca,              \
+       double* __restrict__ r, double* __restrict__ val, const int* __restrict__ own_start,                        \
+       const unsigned* __restrict__ own_mask, int n0, int n1, int n2, double ox, double oy, double oz, double hx,  \
+       double hy, double hz, int mode, const double* __restrict__ prm, const int* __restrict__ stop) {             \
+    /* stop: a Krylov solve's done flag (the matrix-free apply inside BiCGSTAB) */                                \
+    if (stop && *stop) return;                                                                                    \
+    %C%::tile_body<M>(tiles[blockIdx.x], g2l, cmask, x, base, t, cm, ca, r, val, own_start, own_mask, n0, n1, n2,  \
+                      ox, oy, oz, hx, hy, hz, mode, prm);                                                         \
+  }
+DCG_TILE_ENTRY(%KERNEL%_r, 1)
+DCG_TILE_ENTRY(%KERNEL%_j, 2)
+DCG_TILE_ENTRY(%KERNEL%_a, 4)
+#undef DCG_TILE_ENTRY
 )";
 
 // Batched form: one launch over (member, tile) pairs of many models that share
@@ -1476,7 +1498,7 @@ extern "C" __global__ void __launch_bounds__(256)
   const double* cf = coef + 4 * (size_t)w.x;
   if (cf[3] == 0.0) return;
   const dcg_asm_member& M = mem[w.x];
-  %C%::tile_body(w.y, M.g2l[%COMP%], M.cmask[%COMP%], M.vec[xsel], bsel >= 0 ? M.vec[bsel] : nullptr, cf[0], cf[1],
+  %C%::tile_body<0>(w.y, M.g2l[%COMP%], M.cmask[%COMP%], M.vec[xsel], bsel >= 0 ? M.vec[bsel] : nullptr, cf[0], cf[1],
                  cf[2], rsel >= 0 ? M.vec[rsel] : nullptr, (mode & 2) ? M.val : nullptr, M.own_start, M.own_mask,
                  M.n[0], M.n[1], M.n[2], M.origin[0], M.origin[1], M.origin[2], M.h[0], M.h[1], M.h[2], mode, M.prm);
 }

@@ -492,9 +492,15 @@ void Model::compile_and_load() {
   module_ = rtc_module(gen_.source, "dcopasi_assembly", device_);
   vol_fn_.assign(L_.comps.size(), nullptr);
   tile_fn_.assign(L_.comps.size(), nullptr);
+  tile_jac_fn_.assign(L_.comps.size(), nullptr);
+  tile_app_fn_.assign(L_.comps.size(), nullptr);
   for (size_t c = 0; c < L_.comps.size(); ++c) {
     if (!gen_.volume_kernel[c].empty()) vol_fn_[c] = module_->function(gen_.volume_kernel[c]);
-    if (!gen_.tile_kernel[c].empty()) tile_fn_[c] = module_->function(gen_.tile_kernel[c]);
+    if (!gen_.tile_kernel[c].empty()) {
+      tile_fn_[c] = module_->function(gen_.tile_kernel[c] + "_r");
+      tile_jac_fn_[c] = module_->function(gen_.tile_kernel[c] + "_j");
+      tile_app_fn_[c] = module_->function(gen_.tile_kernel[c] + "_a");
+    }
   }
   facet_fn_.clear();
   for (const auto& fk : gen_.facet_kernels) facet_fn_.push_back(module_->function(fk.name));
@@ -539,10 +545,16 @@ void Model::launch_assembly(const double* x, double t, double cm, double ca, con
       const int* g2l = comp_dev_[c].g2l.get();
       const unsigned char* cmask = comp_dev_[c].cell_mask_grid.get();
       const int* stop = nullptr;
-      void* args[] = {&tiles, &g2l, &cmask, &x, &base_eff, &t, &cm, &ca, &r, &val, &own_start, &own_mask,
-                      &n0, &n1, &n2, &ox, &oy, &oz, &hx, &hy, &hz, &mode, &prm, &stop};
-      if (comp_dev_[c].ntiles > 0)
-        DCG_HIP(hipModuleLaunchKernel(tile_fn_[c], comp_dev_[c].ntiles, 1, 1, 256, 1, 1, 0, stream_, args, nullptr));
+      // residual and Jacobian are separate entry points (each with its own
+      // registers and LDS); both read only x, so their order is free
+      for (int part : {1, 2}) {
+        if (!(mode & part) || comp_dev_[c].ntiles <= 0) continue;
+        int pmode = part;
+        void* args[] = {&tiles, &g2l, &cmask, &x, &base_eff, &t, &cm, &ca, &r, &val, &own_start, &own_mask,
+                        &n0, &n1, &n2, &ox, &oy, &oz, &hx, &hy, &hz, &pmode, &prm, &stop};
+        DCG_HIP(hipModuleLaunchKernel(part == 1 ? tile_fn_[c] : tile_jac_fn_[c], comp_dev_[c].ntiles, 1, 1, 256, 1, 1,
+                                      0, stream_, args, nullptr));
+      }
       continue;
     }
     const int* cells = comp_dev_[c].cells.get();
@@ -594,7 +606,7 @@ void Model::launch_apply(const double* lin, double t, double cm, double ca, cons
     const unsigned char* cmask = comp_dev_[c].cell_mask_grid.get();
     void* args[] = {&tiles, &g2l, &cmask, &lin, &v, &t, &cm, &ca, &y, &val, &own_start, &own_mask,
                     &n0, &n1, &n2, &ox, &oy, &oz, &hx, &hy, &hz, &mode, &prm, &stop};
-    DCG_HIP(hipModuleLaunchKernel(tile_fn_[c], comp_dev_[c].ntiles, 1, 1, 256, 1, 1, 0, stream_, args, nullptr));
+    DCG_HIP(hipModuleLaunchKernel(tile_app_fn_[c], comp_dev_[c].ntiles, 1, 1, 256, 1, 1, 0, stream_, args, nullptr));
   }
 }
 

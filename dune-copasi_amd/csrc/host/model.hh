@@ -211,7 +211,9 @@ class Model {
   bool own_stream_ = true;
   std::shared_ptr<RtcModule> module_;
   std::vector<hipFunction_t> vol_fn_;
-  std::vector<hipFunction_t> tile_fn_;
+  std::vector<hipFunction_t> tile_fn_;      // tiled residual (_r); set when the compartment is tiled
+  std::vector<hipFunction_t> tile_jac_fn_;  // tiled Jacobian (_j)
+  std::vector<hipFunction_t> tile_app_fn_;  // tiled matrix-free apply (_a)
   std::vector<hipFunction_t> facet_fn_;
   double time_ = 0.0, dt_initial_ = 0.0, t_end_ = 0.0;
   int last_krylov_its_ = 8;
