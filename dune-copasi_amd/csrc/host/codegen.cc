@@ -941,7 +941,10 @@ constexpr int NST = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
 constexpr int OY = DIM > 1 ? 1 : 0, OZ = DIM > 2 ? 1 : 0;
 constexpr int MC_WORDS = NCB * NCLASS * NQ * (NMC > 0 ? NMC : 1);
 constexpr int RL_WORDS = NCB * NCORNER * BC;
-constexpr int UNION_WORDS = MC_WORDS > RL_WORDS ? MC_WORDS : RL_WORDS;
+// the Jacobian paths also pass row sums between the two lanes of a row
+constexpr int XCH_WORDS = NROW * ((NZB > BC ? NZB : BC) > 0 ? (NZB > BC ? NZB : BC) : 1);
+constexpr int JW_WORDS = MC_WORDS > XCH_WORDS ? MC_WORDS : XCH_WORDS;
+constexpr int UNION_WORDS = JW_WORDS > RL_WORDS ? JW_WORDS : RL_WORDS;
 
 // element residual of class p (runtime) from corner values
 __device__ __forceinline__ void residual_rt(int p, const double (&xl)[NV][BC], double cx, double cy, double cz,
@@ -1088,7 +1091,7 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
   __shared__ double xs[NVB * BC];
   __shared__ int sl[NVB];
   __shared__ unsigned char cms[NCB];
-  __shared__ double work[MODE == 1 ? RL_WORDS : (MODE == 2 || MODE == 4) ? MC_WORDS : UNION_WORDS];
+  __shared__ double work[MODE == 1 ? RL_WORDS : (MODE == 2 || MODE == 4) ? JW_WORDS : UNION_WORDS];
   const int ntx = (n0 + 1 + TX - 1) / TX;
   const int nty = DIM > 1 ? (n1 + 1 + TY - 1) / TY : 1;
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
@@ -1172,18 +1175,17 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
     // row's stencil blocks it touches; every index into the accumulator is a
     // compile-time constant.  Slots along the split axis are the corner bit
     // (2 slots) instead of the offset (3 slots); the two lanes of a row then
-    // exchange the shared middle plane.  Each block is written exactly once.
+    // exchange the shared middle plane through LDS.  The halves are uniform
+    // per wave (lanes t and t + 64 hold the same row), so no wave runs both
+    // halves' cells.  Each block is written exactly once.
     constexpr int SPL = (DIM > 1 && NROW * 2 <= 256) ? 2 : 1;
     constexpr int NCELL = DIM == 1 ? 2 : DIM == 2 ? 4 : 8;
     constexpr int SA = DIM - 1;  // split axis
     constexpr int SX = 3, SY = DIM > 1 ? ((SPL == 2 && SA == 1) ? 2 : 3) : 1, SZ = DIM > 2 ? (SPL == 2 ? 2 : 3) : 1;
-    for (int it = threadIdx.x; it < NROW * SPL; it += blockDim.x) {
-      const int lr = it / SPL, half = it % SPL;
-      const int li = lr % TX, lj = DIM > 1 ? (lr / TX) % TY : 0, lk = DIM > 2 ? lr / (TX * TY) : 0;
-      const int vb = (li + 1) + XB * ((lj + OY) + YB * (lk + OZ));
-      const int s = sl[vb];
-      if (s < 0) continue;  // both lanes of a row leave together
-      double acc[SZ][SY][SX][NZB > 0 ? NZB : 1];
+    constexpr int NZ1 = NZB > 0 ? NZB : 1;
+    // the row's slots from this lane's cells (half: below / above the row
+    // vertex on the split axis)
+    auto accumulate = [&](int li, int lj, int lk, int half, double (&acc)[SZ][SY][SX][NZ1]) {
 #pragma unroll
       for (int a2 = 0; a2 < SZ; ++a2)
 #pragma unroll
@@ -1191,7 +1193,7 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
 #pragma unroll
           for (int a0 = 0; a0 < SX; ++a0)
 #pragma unroll
-            for (int z = 0; z < (NZB > 0 ? NZB : 1); ++z) acc[a2][a1][a0][z] = 0.0;
+            for (int z = 0; z < NZ1; ++z) acc[a2][a1][a0][z] = 0.0;
 #pragma unroll
       for (int e = 0; e < NCELL; ++e) {
         const int ex = e & 1, ey = DIM > 1 ? (e >> 1) & 1 : 0, ez = DIM > 2 ? (e >> 2) & 1 : 0;
@@ -1245,66 +1247,46 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
           }
         }
       }
-      if constexpr (SPL == 2) {
-        // the middle plane: corner bit 1 on the lower lane, 0 on the upper
-#pragma unroll
-        for (int a1 = 0; a1 < (SA == 2 ? SY : 1); ++a1)
-#pragma unroll
-          for (int a0 = 0; a0 < (SA == 2 ? SX : SX); ++a0)
-#pragma unroll
-            for (int z = 0; z < (NZB > 0 ? NZB : 1); ++z) {
-              if constexpr (SA == 2) {
-                const double v0 = __shfl_xor(acc[0][a1][a0][z], 1), v1 = __shfl_xor(acc[1][a1][a0][z], 1);
-                if (half == 0) acc[1][a1][a0][z] += v0;
-                else acc[0][a1][a0][z] += v1;
-              } else {
-                const double v0 = __shfl_xor(acc[0][0][a0][z], 1), v1 = __shfl_xor(acc[0][1][a0][z], 1);
-                if (half == 0) acc[0][1][a0][z] += v0;
-                else acc[0][0][a0][z] += v1;
-              }
-            }
+    };
+    // stencil offset of slot (a0, a1, a2) of a lane; false when the slot is
+    // the other lane's (the lower lane writes the middle plane)
+    auto slot_offset = [&](int a0, int a1, int a2, int half, int& dx, int& dy, int& dz) -> bool {
+      dx = a0 - 1;
+      dy = DIM > 1 ? a1 - 1 : 0;
+      dz = DIM > 2 ? a2 - 1 : 0;
+      if constexpr (SPL == 2 && SA == 2) {
+        dz = a2 - (1 - half);
+        return half == 0 || a2 == 1;
       }
-      if (mode & 4) {
-        // r = J v over the row's stencil: each lane its own slots, the two
-        // lanes of a row then add their sums
-        double ya[BC];
-#pragma unroll
-        for (int i = 0; i < BC; ++i) ya[i] = 0.0;
-#pragma unroll
-        for (int a2 = 0; a2 < SZ; ++a2)
-#pragma unroll
-          for (int a1 = 0; a1 < SY; ++a1)
-#pragma unroll
-            for (int a0 = 0; a0 < SX; ++a0) {
-              int dx = a0 - 1, dy = DIM > 1 ? a1 - 1 : 0, dz = DIM > 2 ? a2 - 1 : 0;
-              bool mine = true;
-              if constexpr (SPL == 2 && SA == 2) {
-                dz = a2 - (1 - half);
-                mine = half == 0 || a2 == 1;
-              }
-              if constexpr (SPL == 2 && SA == 1) {
-                dy = a1 - (1 - half);
-                mine = half == 0 || a1 == 1;
-              }
-              if (!mine) continue;
-              const int nb = vb + dx + XB * (dy + YB * dz);
-#pragma unroll
-              for (int i = 0; i < BC; ++i)
-#pragma unroll
-                for (int j = 0; j < BC; ++j) {
-                  const int z = ZIX[i][j];
-                  if (z >= 0) ya[i] += acc[a2][a1][a0][z >= 0 ? z : 0] * xs[nb * BC + j];
-                }
-            }
-        if constexpr (SPL == 2) {
-#pragma unroll
-          for (int i = 0; i < BC; ++i) ya[i] += __shfl_xor(ya[i], 1);
-        }
-        if (half == 0)
-          for (int i = 0; i < DCG_B; ++i)
-            r[(size_t)s * DCG_B + i] = i < BC ? ya[i < BC ? i : 0] : base[(size_t)s * DCG_B + i];
-        continue;
+      if constexpr (SPL == 2 && SA == 1) {
+        dy = a1 - (1 - half);
+        return half == 0 || a1 == 1;
       }
+      return true;
+    };
+    // r = J v over the row (v staged in xs), summed over this lane's slots
+    auto apply_row = [&](int vb, int half, const double (&acc)[SZ][SY][SX][NZ1], double (&ya)[BC]) {
+#pragma unroll
+      for (int i = 0; i < BC; ++i) ya[i] = 0.0;
+#pragma unroll
+      for (int a2 = 0; a2 < SZ; ++a2)
+#pragma unroll
+        for (int a1 = 0; a1 < SY; ++a1)
+#pragma unroll
+          for (int a0 = 0; a0 < SX; ++a0) {
+            int dx, dy, dz;
+            slot_offset(a0, a1, a2, half, dx, dy, dz);
+            const int nb = vb + dx + XB * (dy + YB * dz);
+#pragma unroll
+            for (int i = 0; i < BC; ++i)
+#pragma unroll
+              for (int j = 0; j < BC; ++j) {
+                const int z = ZIX[i][j];
+                if (z >= 0) ya[i] += acc[a2][a1][a0][z >= 0 ? z : 0] * xs[nb * BC + j];
+              }
+          }
+    };
+    auto write_row = [&](int s, int half, const double (&acc)[SZ][SY][SX][NZ1]) {
       const unsigned mk = own_mask[s];
       const int st = own_start[s];
 #pragma unroll
@@ -1313,18 +1295,8 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
         for (int a1 = 0; a1 < SY; ++a1)
 #pragma unroll
           for (int a0 = 0; a0 < SX; ++a0) {
-            // offset of this slot; on a split axis the slot is the corner bit
-            int dx = a0 - 1, dy = DIM > 1 ? a1 - 1 : 0, dz = DIM > 2 ? a2 - 1 : 0;
-            bool mine = true;
-            if constexpr (SPL == 2 && SA == 2) {
-              dz = a2 - (1 - half);
-              mine = half == 0 || a2 == 1;  // the lower lane writes the middle plane
-            }
-            if constexpr (SPL == 2 && SA == 1) {
-              dy = a1 - (1 - half);
-              mine = half == 0 || a1 == 1;
-            }
-            if (!mine) continue;
+            int dx, dy, dz;
+            if (!slot_offset(a0, a1, a2, half, dx, dy, dz)) continue;
             const int o = (dx + 1) + (DIM > 1 ? 3 * (dy + 1) : 0) + (DIM > 2 ? 9 * (dz + 1) : 0);
             if (!((mk >> o) & 1u)) continue;
             double* blk = val + (size_t)(st + __popc(mk & ((1u << o) - 1u))) * (DCG_B * DCG_B);
@@ -1337,6 +1309,75 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
                 blk[i * DCG_B + j] = z >= 0 ? acc[a2][a1][a0][z >= 0 ? z : 0] : ((diag && i == j && i >= BC) ? 1.0 : 0.0);
               }
           }
+    };
+    if constexpr (SPL == 2) {
+      const int tid = threadIdx.x;
+      const int half = (tid >> 6) & 1;
+      const int lr = (tid >> 7) * 64 + (tid & 63);
+      const int li = lr % TX, lj = (lr / TX) % TY, lk = DIM > 2 ? lr / (TX * TY) : 0;
+      const int vb = (li + 1) + XB * ((lj + OY) + YB * (lk + OZ));
+      const int s = lr < NROW ? sl[vb] : -1;
+      double acc[SZ][SY][SX][NZ1];
+      if (s >= 0) accumulate(li, lj, lk, half, acc);
+      if (mode & 4) {
+        // the apply is linear: each lane applies its own slots (its part of
+        // the middle plane included) and the two sums are added
+        double ya[BC];
+        if (s >= 0) apply_row(vb, half, acc, ya);
+        __syncthreads();  // every lane is done with the staged coefficients
+        if (s >= 0 && half == 1)
+          for (int i = 0; i < BC; ++i) work[lr * BC + i] = ya[i];
+        __syncthreads();
+        if (s >= 0 && half == 0)
+          for (int i = 0; i < DCG_B; ++i)
+            r[(size_t)s * DCG_B + i] = i < BC ? ya[i < BC ? i : 0] + work[lr * BC + (i < BC ? i : 0)]
+                                              : base[(size_t)s * DCG_B + i];
+      } else {
+        // the middle plane (corner bit 1 below, 0 above) in rounds of XR
+        // slots that fit the coefficient array
+        constexpr int MSL = SA == 2 ? SY * SX : SX;
+        constexpr int XR0 = JW_WORDS / (NROW * NZ1);
+        constexpr int XR = XR0 < 1 ? 1 : XR0 < MSL ? XR0 : MSL;
+#pragma unroll
+        for (int k0 = 0; k0 < MSL; k0 += XR) {
+          __syncthreads();  // the coefficients (or the last round) are read
+          if (s >= 0 && half == 1) {
+#pragma unroll
+            for (int k = k0; k < (k0 + XR < MSL ? k0 + XR : MSL); ++k)
+#pragma unroll
+              for (int z = 0; z < NZ1; ++z)
+                work[((k - k0) * NROW + lr) * NZ1 + z] = SA == 2 ? acc[0][k / SX][k % SX][z] : acc[0][0][k][z];
+          }
+          __syncthreads();
+          if (s >= 0 && half == 0) {
+#pragma unroll
+            for (int k = k0; k < (k0 + XR < MSL ? k0 + XR : MSL); ++k)
+#pragma unroll
+              for (int z = 0; z < NZ1; ++z) {
+                const double v = work[((k - k0) * NROW + lr) * NZ1 + z];
+                if constexpr (SA == 2) acc[1][k / SX][k % SX][z] += v;
+                else acc[0][1][k][z] += v;
+              }
+          }
+        }
+        if (s >= 0) write_row(s, half, acc);
+      }
+    } else {
+      for (int lr = threadIdx.x; lr < NROW; lr += blockDim.x) {
+        const int li = lr % TX, lj = DIM > 1 ? (lr / TX) % TY : 0, lk = DIM > 2 ? lr / (TX * TY) : 0;
+        const int vb = (li + 1) + XB * ((lj + OY) + YB * (lk + OZ));
+        const int s = sl[vb];
+        if (s < 0) continue;
+        double acc[SZ][SY][SX][NZ1];
+        accumulate(li, lj, lk, 0, acc);
+        if (mode & 4) {
+          double ya[BC];
+          apply_row(vb, 0, acc, ya);
+          for (int i = 0; i < DCG_B; ++i) r[(size_t)s * DCG_B + i] = i < BC ? ya[i < BC ? i : 0] : base[(size_t)s * DCG_B + i];
+        } else {
+          write_row(s, 0, acc);
+        }
+      }
     }
     } else {
     // phase 2 (blocks above 4 x 4): one BCSR block per (row, stencil offset),
