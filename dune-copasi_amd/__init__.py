@@ -89,6 +89,9 @@ _SIGS = {
     "dcg_assemble": (_I, [_P, _P, _D, _D, _D, _P, _P]),
     "dcg_spmv": (_I, [_P, _P, _P, _P]),
     "dcg_jacobian_apply": (_I, [_P, _P, _D, _D, _D, _P, _P]),
+    "dcg_reduce": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_S)]),
+    "dcg_reduce_source": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p, _S, ctypes.POINTER(_S)]),
+    "dcg_reduce_result": (_I, [_P, _S, ctypes.c_char_p, _S, ctypes.POINTER(_D), ctypes.POINTER(_I)]),
     "dcg_linear_solve": (_I, [_P, _P, _P, _P, _D, _I, ctypes.POINTER(_I)]),
     "dcg_profile_enable": (_I, [_P, _I]),
     "dcg_profile_read": (_I, [_P, _I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
@@ -260,6 +263,34 @@ class Model:
         st = StepStats()
         _check(self._lib.dcg_step(self._h, t, dt, ctypes.byref(st)))
         return st
+
+    def reduce(self, config: str = "", check: bool = True):
+        """Transform/reduce over the current state (the model's `model.reduce`
+        keys, or `config` whose top-level sections are the keys): {key: value}.
+        With check, a fired error expression raises DcgError (the reference's
+        ReductionError); flags() of the last call gives 0/1 (warn)/2 (error)."""
+        n = _S()
+        rc = _check(self._lib.dcg_reduce(self._h, config.encode() if config else None, ctypes.byref(n)))
+        out, flags = {}, {}
+        for i in range(n.value):
+            buf = ctypes.create_string_buffer(256)
+            v, f = _D(), _I()
+            _check(self._lib.dcg_reduce_result(self._h, i, buf, 256, ctypes.byref(v), ctypes.byref(f)))
+            out[buf.value.decode()] = v.value
+            flags[buf.value.decode()] = f.value
+        self.reduce_flags = flags
+        if check and rc == 1:
+            bad = [k for k, f in flags.items() if f == 2]
+            raise DcgError(f"reduction error expression fired for {bad}: {[out[k] for k in bad]}")
+        return out
+
+    def reduce_source(self, config: str = "") -> str:
+        need = _S()
+        cfg = config.encode() if config else None
+        _check(self._lib.dcg_reduce_source(self._h, cfg, None, 0, ctypes.byref(need)))
+        buf = ctypes.create_string_buffer(need.value)
+        _check(self._lib.dcg_reduce_source(self._h, cfg, buf, need.value, None))
+        return buf.value.decode()
 
     def mass(self) -> np.ndarray:
         m = np.zeros(self.n_species)

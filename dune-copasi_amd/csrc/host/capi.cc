@@ -13,6 +13,7 @@
 struct dcg_model {
   std::unique_ptr<dcopasi::Model> own;
   dcopasi::Model* m = nullptr;
+  std::vector<dcopasi::ReduceResult> reduced;  // last dcg_reduce
 };
 
 struct dcg_batch {
@@ -143,6 +144,39 @@ int dcg_mass(dcg_model_t h, double* per_species) {
   if (check(h)) return -1;
   return guarded([&] {
     h->m->mass(per_species);
+    return 0;
+  });
+}
+
+int dcg_reduce(dcg_model_t h, const char* config_ini, size_t* count) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    h->reduced = h->m->reduce(config_ini ? config_ini : "");
+    if (count) *count = h->reduced.size();
+    for (const auto& r : h->reduced)
+      if (r.flag == 2) return 1;
+    return 0;
+  });
+}
+
+int dcg_reduce_source(dcg_model_t h, const char* config_ini, char* buf, size_t len, size_t* needed) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    const std::string s = h->m->reduce_source(config_ini ? config_ini : "");
+    if (needed) *needed = s.size() + 1;
+    if (buf && len) copy_name(s, buf, len);
+    return 0;
+  });
+}
+
+int dcg_reduce_result(dcg_model_t h, size_t i, char* name, size_t name_len, double* value, int* flag) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    if (i >= h->reduced.size()) throw dcopasi::Error("no reduce result " + std::to_string(i));
+    const auto& r = h->reduced[i];
+    if (name) copy_name(r.name, name, name_len);
+    if (value) *value = r.value;
+    if (flag) *flag = r.flag;
     return 0;
   });
 }

@@ -6,6 +6,7 @@
 #include <sstream>
 
 #include "error.hh"
+#include "reduce.hh"
 
 namespace dcopasi {
 
@@ -84,9 +85,9 @@ struct ElementTables {
   double evol;
 };
 
-ElementTables element_tables(const Grid& g) {
+ElementTables element_tables(const Grid& g, const Rule* rule = nullptr) {
   ElementTables T;
-  Rule r = element_rule(g.dim, g.cube);
+  Rule r = rule ? *rule : element_rule(g.dim, g.cube);
   T.nq = static_cast<int>(r.w.size());
   T.nv = g.nvpe;
   const int d = g.dim;
@@ -1764,6 +1765,154 @@ GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L, bool r
   }
   out.source = src.str();
   return out;
+}
+
+std::string generate_reduce(const ModelSpec& m, const Layout& L, const std::vector<ReduceKey>& keys,
+                            std::vector<std::string>* kernels) {
+  const Grid& g = L.grid;
+  const Rule rule = reduce_rule(g.dim, g.cube);
+  const auto T = element_tables(unit_cell(g), &rule);
+  const int nk = static_cast<int>(keys.size());
+  std::ostringstream src;
+  src << "#define DCG_B " << L.B << "\n";
+  kernels->clear();
+  for (size_t c = 0; c < L.comps.size(); ++c) {
+    const auto& C = L.comps[c];
+    if (C.cells.empty() || nk == 0) {
+      kernels->push_back("");
+      continue;
+    }
+    const int bc = C.b;
+    const std::string ns = "red" + std::to_string(c), kn = "dcg_reduce_" + std::to_string(c);
+    auto vars = volume_vars(m, static_cast<int>(c), T.evol, true);
+    // reduce binds position, time and integration_factor only: the other
+    // point keywords keep LocalDomain's zero defaults (local_domain.hh:40-46)
+    vars["integration_factor"] = "ifac";
+    vars["entity_volume"] = "0.0";
+    vars["in_volume"] = "0.0";
+    std::ostringstream o;
+    o << "namespace " << ns << " {\n";
+    o << "constexpr int DIM = " << g.dim << ", NV = " << T.nv << ", NQR = " << T.nq << ", NCLASS = " << g.nclass
+      << ", NCORNER = " << g.ncorner << ", BC = " << (bc > 0 ? bc : 1) << ", NK = " << nk << ";\n";
+    std::vector<int> corner;
+    for (int p = 0; p < g.nclass; ++p)
+      for (int a = 0; a < 8; ++a) corner.push_back(g.corner[p][a]);
+    o << "__device__ constexpr int CORNER[NCLASS][8] = " << int_init(corner) << ";\n";
+    o << "__device__ constexpr double RPHI[NQR][NV] = " << array_init(T.phi) << ";\n";
+    o << "__device__ constexpr double RG[NCLASS][NQR][NV][DIM] = " << array_init(T.gphi) << ";\n";
+    o << "__device__ constexpr double RX[NCLASS][NQR][3] = " << array_init(T.xq) << ";\n";
+    o << "__device__ constexpr double RW[NQR] = " << array_init(T.fac) << ";\n";
+    // reductions (point value, accumulator) -> accumulator
+    for (int k = 0; k < nk; ++k) {
+      const auto& f = keys[k].reduction;
+      o << "__device__ __forceinline__ double red" << k << "(double a, double b, const double* __restrict__ prm) {\n"
+        << "  (void)prm;\n";
+      if (f.body) {
+        auto rv = vars;
+        rv[f.args[0]] = "a";
+        rv[f.args[1]] = "b";
+        o << "  return " << expr::emit(f.body, rv) << ";\n";
+      } else if (!f.args.empty()) {
+        o << "  return 0.0;\n";  // a literal zero body
+      } else {
+        o << "  return a + b;\n";
+      }
+      o << "}\n";
+    }
+    o << "__device__ __forceinline__ void evaluate(const double (&u)[BC], const double (&g)[BC][3], double px, "
+         "double py, double pz, double t, double ifac, double hv, const double* __restrict__ prm, double (&e)[NK]) {\n"
+         "  (void)u; (void)g; (void)px; (void)py; (void)pz; (void)t; (void)ifac; (void)hv; (void)prm;\n";
+    for (int k = 0; k < nk; ++k)
+      o << "  e[" << k << "] = " << (keys[k].evaluation ? expr::emit(keys[k].evaluation, vars) : std::string("0.0"))
+        << ";\n";
+    o << "}\n";
+    o << "__device__ __forceinline__ double combine(int k, double a, double b, const double* __restrict__ prm) {\n"
+         "  switch (k) {\n";
+    for (int k = 0; k < nk; ++k) o << "    case " << k << ": return red" << k << "(a, b, prm);\n";
+    o << "  }\n  return a + b;\n}\n";
+    o << "}  // namespace " << ns << "\n";
+    std::string body = R"(
+extern "C" __global__ void __launch_bounds__(256)
+%KERNEL%(const int* __restrict__ cells, const unsigned char* __restrict__ cmask, const int* __restrict__ cslots,
+         int ncell, const double* __restrict__ x, double t, int n0, int n1, double ox, double oy, double oz,
+         double hx, double hy, double hz, const double* __restrict__ prm, double* __restrict__ part) {
+  using namespace %C%;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double h[3] = {hx, hy, hz};
+  const double hv = hx * (DIM > 1 ? hy : 1.0) * (DIM > 2 ? hz : 1.0);
+  double acc[NK];
+  bool have[NK];
+#pragma unroll
+  for (int k = 0; k < NK; ++k) {
+    acc[k] = 0.0;
+    have[k] = false;
+  }
+  if (i < ncell) {
+    const int cell = cells[i];
+    const int ci = cell % n0, cj = (cell / n0) % n1, ck = cell / (n0 * n1);
+    const double cx = ox + ci * hx, cy = DIM > 1 ? oy + cj * hy : 0.0, cz = DIM > 2 ? oz + ck * hz : 0.0;
+    const unsigned m = cmask[i];
+    for (int p = 0; p < NCLASS; ++p) {
+      if (!((m >> p) & 1u)) continue;
+      double ul[NV][BC];
+      for (int a = 0; a < NV; ++a) {
+        const int s = cslots[(size_t)i * NCORNER + CORNER[p][a]];
+        for (int b = 0; b < BC; ++b) ul[a][b] = (s >= 0 && b < %NB%) ? x[(size_t)s * DCG_B + b] : 0.0;
+      }
+      for (int q = 0; q < NQR; ++q) {
+        double u[BC], gr[BC][3];
+        for (int b = 0; b < BC; ++b) {
+          u[b] = 0.0;
+          gr[b][0] = gr[b][1] = gr[b][2] = 0.0;
+          for (int a = 0; a < NV; ++a) {
+            u[b] += RPHI[q][a] * ul[a][b];
+            for (int k = 0; k < DIM; ++k) gr[b][k] += RG[p][q][a][k] / h[k] * ul[a][b];
+          }
+        }
+        const double px = cx + RX[p][q][0] * hx, py = DIM > 1 ? cy + RX[p][q][1] * hy : 0.0,
+                     pz = DIM > 2 ? cz + RX[p][q][2] * hz : 0.0;
+        double e[NK];
+        evaluate(u, gr, px, py, pz, t, RW[q] * hv, hv, prm, e);
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+          acc[k] = have[k] ? combine(k, e[k], acc[k], prm) : e[k];
+          have[k] = true;
+        }
+      }
+    }
+  }
+  // threads in a fixed tree: the higher half is the later cells
+  __shared__ double sv[256];
+  __shared__ int sh[256];
+  for (int k = 0; k < NK; ++k) {
+    sv[threadIdx.x] = acc[k];
+    sh[threadIdx.x] = have[k] ? 1 : 0;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+      if (threadIdx.x < st) {
+        const int o = threadIdx.x + st;
+        if (sh[o]) {
+          sv[threadIdx.x] = sh[threadIdx.x] ? combine(k, sv[o], sv[threadIdx.x], prm) : sv[o];
+          sh[threadIdx.x] = 1;
+        }
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      part[((size_t)blockIdx.x * NK + k) * 2] = sv[0];
+      part[((size_t)blockIdx.x * NK + k) * 2 + 1] = sh[0];
+    }
+    __syncthreads();
+  }
+}
+)";
+    replace_all(body, "%C%", ns);
+    replace_all(body, "%KERNEL%", kn);
+    replace_all(body, "%NB%", std::to_string(bc));
+    src << o.str() << body;
+    kernels->push_back(kn);
+  }
+  return src.str();
 }
 
 }  // namespace dcopasi

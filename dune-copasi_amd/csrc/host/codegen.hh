@@ -44,4 +44,13 @@ constexpr int kGeoWords = 8;
 // so models that differ only in mesh size and parameters share one source.
 GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& layout, bool runtime_geometry = false);
 
+struct ReduceKey;
+// Device source of a transform/reduce evaluation: kernel "dcg_reduce_<c>" per
+// compartment with cells ("" otherwise) folds every key's evaluation over the
+// order-4 points of its cells, one thread per cell, and writes one partial
+// (value, defined) per key and workgroup.  Unit-cell tables; the spacing,
+// origin and time come at run time.
+std::string generate_reduce(const ModelSpec& spec, const Layout& layout, const std::vector<ReduceKey>& keys,
+                            std::vector<std::string>* kernels);
+
 }  // namespace dcopasi

@@ -122,6 +122,59 @@ Rule element_rule(int dim, bool cube) {
 
 Rule facet_rule(int dim, bool cube) { return element_rule(dim - 1, cube); }
 
+Rule reduce_rule(int dim, bool cube) {
+  Rule r;
+  if (dim == 0) {
+    r.x.push_back({0, 0, 0});
+    r.w.push_back(1.0);
+    return r;
+  }
+  if (cube || dim == 1) {
+    const double s = 0.5 * std::sqrt(0.6);
+    const double x1[3] = {0.5 - s, 0.5, 0.5 + s}, w1[3] = {5.0 / 18.0, 8.0 / 18.0, 5.0 / 18.0};
+    int n = 1;
+    for (int k = 0; k < dim; ++k) n *= 3;
+    for (int i = 0; i < n; ++i) {
+      std::array<double, 3> p{0, 0, 0};
+      double w = 1.0;
+      for (int k = 0, m = i; k < dim; ++k, m /= 3) {
+        p[k] = x1[m % 3];
+        w *= w1[m % 3];
+      }
+      r.x.push_back(p);
+      r.w.push_back(w);
+    }
+    return r;
+  }
+  if (dim == 2) {
+    // two orbits of three points (area coordinates (a, a, 1 - 2a))
+    const double a[2] = {0.445948490915965, 0.091576213509771};
+    const double w[2] = {0.223381589678011 / 2.0, 0.109951743655322 / 2.0};
+    for (int o = 0; o < 2; ++o) {
+      const double b = 1.0 - 2.0 * a[o];
+      r.x.push_back({a[o], a[o], 0});
+      r.x.push_back({b, a[o], 0});
+      r.x.push_back({a[o], b, 0});
+      for (int k = 0; k < 3; ++k) r.w.push_back(w[o]);
+    }
+    return r;
+  }
+  // centroid, an orbit of 4 (a, a, a, 1 - 3a) and an orbit of 6 (b, b, c, c)
+  r.x.push_back({0.25, 0.25, 0.25});
+  r.w.push_back(-74.0 / 5625.0);
+  const double a = 1.0 / 14.0, a3 = 1.0 - 3.0 * a;
+  for (auto p : std::vector<std::array<double, 3>>{{a, a, a}, {a3, a, a}, {a, a3, a}, {a, a, a3}}) {
+    r.x.push_back(p);
+    r.w.push_back(343.0 / 45000.0);
+  }
+  const double b = 0.25 * (1.0 + std::sqrt(5.0 / 14.0)), c = 0.25 * (1.0 - std::sqrt(5.0 / 14.0));
+  for (auto p : std::vector<std::array<double, 3>>{{b, b, c}, {b, c, b}, {c, b, b}, {c, c, b}, {c, b, c}, {b, c, c}}) {
+    r.x.push_back(p);
+    r.w.push_back(56.0 / 2250.0);
+  }
+  return r;
+}
+
 // ------------------------------------------------------------------ marking
 std::vector<int8_t> mark_elements(const Grid& g, const std::vector<expr::Ex>& comp_expr) {
   if (comp_expr.size() > 120) throw NotImplemented("more than 120 compartments");

@@ -117,6 +117,10 @@ struct Rule {
   std::vector<double> w;
 };
 Rule element_rule(int dim, bool cube);
+// order-4 rule of the transform/reduce evaluation (reduce.hh:167-169): Gauss-
+// Legendre with 3 points per axis on boxes, the 6-point degree-4 rule on
+// triangles and the 11-point degree-4 rule on tetrahedra
+Rule reduce_rule(int dim, bool cube);
 Rule facet_rule(int dim, bool cube);
 
 }  // namespace dcopasi

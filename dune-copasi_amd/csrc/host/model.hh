@@ -17,6 +17,7 @@
 #include "config.hh"
 #include "device.hh"
 #include "layout.hh"
+#include "reduce.hh"
 #include "rtc.hh"
 #include "spec.hh"
 
@@ -110,6 +111,11 @@ class Model {
   void initial_state(double* packed) const;  // from initial.expression keys
   int step(double t, double dt, StepStats* st);
   void mass(double* per_species);
+  // transform/reduce over the current state: the `model.reduce` keys, or the
+  // keys of `reduce_ini` (top-level sections) when it is not empty
+  std::vector<ReduceResult> reduce(const std::string& reduce_ini);
+  // the generated source of those reduce kernels (no device needed)
+  std::string reduce_source(const std::string& reduce_ini) const;
   // run-time parameters (parser_context constants by name); only models
   // created with runtime_parameters have them
   const std::vector<std::string>& param_names() const { return spec_.params; }
@@ -197,6 +203,7 @@ class Model {
 
  private:
   double assembly_bytes(bool jacobian) const;
+  std::vector<ReduceKey> reduce_keys(const std::string& reduce_ini) const;
   double apply_bytes() const;
 
   ParameterTree cfg_tree_;

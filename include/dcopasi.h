@@ -146,6 +146,23 @@ int dcg_probe_copy(dcg_model_t model, size_t bytes, double* milliseconds);
  * FP64 variant used.  Arrays hold at least 16 entries. */
 int dcg_spmv_tuning(dcg_model_t model, int* count, int* variant, double* microseconds, int* chosen);
 
+/* ---------------------------------------------------------------- transform / reduce
+ * The reference's `model.reduce` keys (dune/copasi/model/diffusion_reaction/
+ * reduce.hh:40-285, evaluated by the CLI after every step,
+ * src/dune_copasi.cc:420-423) on the model's current state: per key,
+ * evaluation.expression at every order-4 quadrature point of every cell,
+ * folded by reduction.expression ("a, b: ...", default +) from
+ * initial.value, then transformation.expression, error.expression and
+ * warn.expression.  config_ini: NULL or "" for the model's own `model.reduce`
+ * keys, else INI text whose top-level sections are the keys.  Returns 0, or
+ * 1 when an error expression fired (the reference's ReductionError); the
+ * results stay readable either way.  flag: 0, 1 warn fired, 2 error fired. */
+int dcg_reduce(dcg_model_t model, const char* config_ini, size_t* count);
+int dcg_reduce_result(dcg_model_t model, size_t i, char* name, size_t name_len, double* value, int* flag);
+/* HIP source of the reduce kernels for config_ini (as dcg_reduce reads it);
+ * no device needed.  *needed gets the size including the terminating zero. */
+int dcg_reduce_source(dcg_model_t model, const char* config_ini, char* buf, size_t len, size_t* needed);
+
 /* ---------------------------------------------------------------- time loop
  * The reference's SimpleAdaptiveStepper with TimeStepper::evolve and
  * snap_to_time (dune/copasi/common/stepper.hh:95-240,330-366), as its CLI

@@ -1594,6 +1594,206 @@ static int evolve(Model& M, double t_end, double& dt, double dec, double inc, do
   return rc;
 }
 
+
+// ---------------------------------------------------------------- transform / reduce
+// reduce.hh:40-285 of the reference, as the CLI runs it (sequential policy,
+// model_single_compartment.impl.hh:247): per key, the evaluation at every
+// order-4 point of every element in grid order, folded from initial.value by
+// reduction (default +); that result is reduced once more with initial.value
+// (reduce.hh:212-217); then transformation, error and warn.  Only position,
+// time, integration_factor, the species and their gradients are bound
+// (reduce.hh:171-190); the other point keywords keep their zero defaults
+// (local_domain.hh:40-46).  Elements outside every compartment are skipped.
+
+// Order-4 rules: Gauss-Legendre, 3 points per axis, on boxes; on triangles the
+// 6-point rule with area coordinates (a, a, 1 - 2a); on tetrahedra the
+// 11-point rule (centroid, orbit of 4, orbit of 6).
+static QRule reduce_rule(int d, bool cube) {
+  QRule r;
+  if (cube || d == 1) {
+    const double s = 0.5 * std::sqrt(0.6);
+    const double x1[3] = {0.5 - s, 0.5, 0.5 + s}, w1[3] = {5.0 / 18.0, 8.0 / 18.0, 5.0 / 18.0};
+    int n = 1;
+    for (int k = 0; k < d; ++k) n *= 3;
+    for (int i = 0; i < n; ++i) {
+      std::array<double, 3> p{0, 0, 0};
+      double w = 1.0;
+      int m = i;
+      for (int k = 0; k < d; ++k) {
+        p[k] = x1[m % 3];
+        w *= w1[m % 3];
+        m /= 3;
+      }
+      r.x.push_back(p);
+      r.w.push_back(w);
+    }
+    return r;
+  }
+  if (d == 2) {
+    const double a[2] = {0.445948490915965, 0.091576213509771};
+    const double w[2] = {0.5 * 0.223381589678011, 0.5 * 0.109951743655322};
+    for (int o = 0; o < 2; ++o) {
+      const double b = 1.0 - 2.0 * a[o];
+      r.x.push_back({a[o], a[o], 0});
+      r.x.push_back({b, a[o], 0});
+      r.x.push_back({a[o], b, 0});
+      for (int k = 0; k < 3; ++k) r.w.push_back(w[o]);
+    }
+    return r;
+  }
+  r.x.push_back({0.25, 0.25, 0.25});
+  r.w.push_back(-74.0 / 5625.0);
+  const double a = 1.0 / 14.0, a3 = 11.0 / 14.0;
+  const double pa[4][3] = {{a, a, a}, {a3, a, a}, {a, a3, a}, {a, a, a3}};
+  for (auto& q : pa) {
+    r.x.push_back({q[0], q[1], q[2]});
+    r.w.push_back(343.0 / 45000.0);
+  }
+  const double b = 0.25 * (1.0 + std::sqrt(5.0 / 14.0)), c = 0.25 * (1.0 - std::sqrt(5.0 / 14.0));
+  const double pb[6][3] = {{b, b, c}, {b, c, b}, {c, b, b}, {c, c, b}, {c, b, c}, {b, c, c}};
+  for (auto& q : pb) {
+    r.x.push_back({q[0], q[1], q[2]});
+    r.w.push_back(56.0 / 2250.0);
+  }
+  return r;
+}
+
+// "a, b: body" with the arguments in slots 0.. of their own value array
+struct RFun {
+  bool present = false;
+  int nargs = 0;
+  Expr body;
+};
+
+static RFun reduce_function(const std::string& text, const Scope& model_scope, int nargs) {
+  RFun f;
+  f.present = true;
+  auto colon = text.find(':');
+  if (colon == std::string::npos) throw std::runtime_error("oracle: expected 'args: expression' in '" + text + "'");
+  Scope sc;
+  sc.consts = model_scope.consts;
+  sc.funcs = model_scope.funcs;
+  std::string args = text.substr(0, colon);
+  std::vector<std::string> names;
+  std::string cur;
+  for (char ch : args + ",") {
+    if (ch == ',') {
+      std::string t = trim(cur);
+      if (!t.empty()) names.push_back(t);
+      cur.clear();
+    } else {
+      cur += ch;
+    }
+  }
+  if ((int)names.size() != nargs) throw std::runtime_error("oracle: function needs " + std::to_string(nargs) + " arguments");
+  for (int i = 0; i < nargs; ++i) {
+    sc.consts.erase(names[i]);
+    sc.vars[names[i]] = i;
+  }
+  f.nargs = nargs;
+  f.body = compile(text.substr(colon + 1), sc);
+  return f;
+}
+
+static double call(const RFun& f, double a, double b = 0.0) {
+  const double v[2] = {a, b};
+  return f.body.null() ? 0.0 : f.body(v);
+}
+
+struct ReduceOut {
+  std::vector<std::string> names;
+  std::vector<double> values;
+  std::vector<int> flags;  // 0, 1 warn, 2 error
+};
+
+static int reduce(const Model& M, const Ini& cfg, const std::string& prefix, ReduceOut& out) {
+  struct Key {
+    std::string name;
+    Expr eval;
+    RFun red, tr, err, warn;
+    double init = 0;
+  };
+  std::vector<Key> keys;
+  const std::string p0 = prefix.empty() ? "" : prefix + ".";
+  for (auto& k : cfg.subkeys(prefix)) {
+    Key K;
+    K.name = k;
+    const std::string p = p0 + k + ".";
+    if (cfg.has(p + "evaluation.expression")) K.eval = compile(cfg.get(p + "evaluation.expression"), M.scope);
+    if (cfg.has(p + "reduction.expression")) K.red = reduce_function(cfg.get(p + "reduction.expression"), M.scope, 2);
+    K.init = cfg.get_double(p + "initial.value", 0.0);
+    if (cfg.has(p + "transformation.expression"))
+      K.tr = reduce_function(cfg.get(p + "transformation.expression"), M.scope, 1);
+    if (cfg.has(p + "error.expression")) K.err = reduce_function(cfg.get(p + "error.expression"), M.scope, 1);
+    if (cfg.has(p + "warn.expression")) K.warn = reduce_function(cfg.get(p + "warn.expression"), M.scope, 1);
+    keys.push_back(std::move(K));
+  }
+  const Mesh& m = M.mesh;
+  const int d = m.dim, nv = m.nvpe;
+  const QRule q = reduce_rule(d, m.cube);
+  std::vector<double> acc(keys.size());
+  for (size_t k = 0; k < keys.size(); ++k) acc[k] = keys[k].init;
+  std::vector<double> v(V_SPEC + 4 * M.spec.size() + 4, 0.0);
+  double phi[8], dref[8][3], grad[8][3];
+  for (long e = 0; e < m.nelem; ++e) {
+    const int c = M.elem_comp[e];
+    if (c < 0) continue;
+    const Compartment& C = M.comps[c];
+    Geo g = element_geo(m, e);
+    std::fill(v.begin(), v.end(), 0.0);
+    v[V_TIME] = M.time;
+    const long* conn = &m.conn[e * nv];
+    for (size_t iq = 0; iq < q.w.size(); ++iq) {
+      shape(m, q.x[iq].data(), phi, dref);
+      for (int a = 0; a < nv; ++a)
+        for (int k = 0; k < 3; ++k) {
+          double s = 0;
+          for (int l = 0; l < d; ++l) s += dref[a][l] * g.Jinv[l][k];
+          grad[a][k] = k < d ? s : 0.0;
+        }
+      for (int k = 0; k < 3; ++k) {
+        double s = g.x0[k];
+        for (int l = 0; l < d; ++l) s += g.J[k][l] * q.x[iq][l];
+        v[V_POS + k] = k < d ? s : 0.0;
+      }
+      v[V_IFAC] = q.w[iq] * g.det;
+      for (int i = 0; i < C.b; ++i) {
+        const int gs = C.species[i];
+        double val = 0, gr[3] = {0, 0, 0};
+        for (int a = 0; a < nv; ++a) {
+          const long node = C.g2l[conn[a]];
+          const double x = M.u[C.dof_off + node * C.b + i];
+          val += x * phi[a];
+          for (int k = 0; k < 3; ++k) gr[k] += x * grad[a][k];
+        }
+        v[V_SPEC + 4 * gs] = val;
+        for (int k = 0; k < 3; ++k) v[V_SPEC + 4 * gs + 1 + k] = gr[k];
+      }
+      for (size_t k = 0; k < keys.size(); ++k) {
+        const double ev = keys[k].eval.null() ? 0.0 : keys[k].eval(v.data());
+        acc[k] = keys[k].red.present ? call(keys[k].red, ev, acc[k]) : ev + acc[k];
+      }
+    }
+  }
+  int rc = 0;
+  for (size_t k = 0; k < keys.size(); ++k) {
+    const Key& K = keys[k];
+    double val = K.red.present ? call(K.red, acc[k], K.init) : acc[k] + K.init;
+    if (K.tr.present) val = call(K.tr, val);
+    int flag = 0;
+    if (K.err.present && call(K.err, val) != 0.0) {
+      flag = 2;
+      rc = 1;
+    } else if (K.warn.present && call(K.warn, val) != 0.0) {
+      flag = 1;
+    }
+    out.names.push_back(K.name);
+    out.values.push_back(val);
+    out.flags.push_back(flag);
+  }
+  return rc;
+}
+
 static void configure(Model& M, const std::string& ini, const std::string& over) {
   M.ini.parse(ini);
   if (!over.empty()) M.ini.parse_overrides(over);
@@ -1683,6 +1883,40 @@ void orc_set_threads(int n) {
 }
 
 void orc_set_krylov_threads(int on) { orc::krylov_threads = on != 0; }
+
+// transform/reduce on the current state: config NULL or "" reads the model's
+// model.reduce keys; names are written '\n'-separated.  Returns 0, 1 when an
+// error expression fired, -1 on failure.
+int orc_reduce(void* h, const char* config, char* names, int names_len, double* values, int* flags, int capacity,
+               int* count) {
+  try {
+    auto* M = static_cast<Model*>(h);
+    orc::ReduceOut out;
+    int rc;
+    if (config && *config) {
+      orc::Ini cfg;
+      cfg.parse(config);
+      rc = orc::reduce(*M, cfg, "", out);
+    } else {
+      rc = orc::reduce(*M, M->ini, "model.reduce", out);
+    }
+    std::string joined;
+    for (size_t i = 0; i < out.names.size(); ++i) {
+      if (i) joined += "\n";
+      joined += out.names[i];
+      if ((int)i < capacity) {
+        values[i] = out.values[i];
+        flags[i] = out.flags[i];
+      }
+    }
+    std::snprintf(names, names_len, "%s", joined.c_str());
+    *count = (int)out.names.size();
+    return rc;
+  } catch (const std::exception& e) {
+    fail(e);
+    return -1;
+  }
+}
 
 int orc_max_threads() {
 #ifdef _OPENMP

@@ -31,6 +31,7 @@ def load():
         "orc_last_error": (ctypes.c_char_p, []),
         "orc_set_threads": (None, [I]),
         "orc_set_krylov_threads": (None, [I]),
+        "orc_reduce": (I, [P, ctypes.c_char_p, ctypes.c_char_p, I, P, P, I, ctypes.POINTER(I)]),
         "orc_max_threads": (I, []),
         "orc_n_dof": (Lg, [P]),
         "orc_n_species": (I, [P]),
@@ -129,6 +130,19 @@ class OracleModel:
         if rc < 0:
             raise RuntimeError(self._L.orc_last_error().decode())
         return rc, acc[:na.value].copy(), nf.value, dn.value
+
+    def reduce(self, config=""):
+        """Transform/reduce on the current state: ({key: value}, {key: flag}, rc)."""
+        names = ctypes.create_string_buffer(1 << 14)
+        vals = np.zeros(256)
+        flags = np.zeros(256, dtype=np.int32)
+        n = ctypes.c_int()
+        rc = self._L.orc_reduce(self._h, config.encode() if config else None, names, len(names), _p(vals), _p(flags),
+                                256, ctypes.byref(n))
+        if rc < 0:
+            raise RuntimeError(self._L.orc_last_error().decode())
+        keys = names.value.decode().split("\n") if n.value else []
+        return ({k: vals[i] for i, k in enumerate(keys)}, {k: int(flags[i]) for i, k in enumerate(keys)}, rc)
 
     def step(self, dt):
         ni, li = ctypes.c_int(), ctypes.c_int()

@@ -262,3 +262,27 @@ def test_footprint_estimate_scales_with_mesh(dca, wl):
     assert small > 0 and 12 < big / small < 20
     fp64 = dca.estimate_device_bytes(wl.gray_scott(n=128, precision="fp64").ini)
     assert fp64 < big
+
+
+@pytest.mark.parametrize("make", [lambda wl: wl.gray_scott(n=6, cell_type="cube"),
+                                  lambda wl: wl.mitchell_schaefer(n=4, cell_type="simplex"),
+                                  lambda wl: wl.two_compartments(n=8, cell_type="cube")],
+                         ids=["gs_q1", "ms_p1", "two_q1"])
+def test_reduce_source_compiles_for_gfx950(dca, wl, make):
+    m = dca.Model(make(wl).ini, "", -1)
+    s0 = m.species()[0][0]
+    cfg = (f"[mx]\nevaluation.expression = {s0}\nreduction.expression = a, b: max(a, b)\ninitial.value = -1e100\n"
+           f"[l2]\nevaluation.expression = {s0}^2*integration_factor\ntransformation.expression = x: sqrt(x)\n"
+           "[t]\nevaluation.expression = time*position_x\n")
+    src = m.reduce_source(cfg)
+    assert "dcg_reduce_0" in src
+    rc, log = _hiprtc_compile(src)
+    assert rc == 0, log
+
+
+def test_reduce_configuration_errors(dca, wl):
+    m = dca.Model(wl.gray_scott(n=6, cell_type="cube").ini, "", -1)
+    with pytest.raises(dca.DcgError, match="exactly 2"):
+        m.reduce_source("[k]\nevaluation.expression = u\nreduction.expression = a: a\n")
+    with pytest.raises(dca.DcgError, match="exactly 1"):
+        m.reduce_source("[k]\nevaluation.expression = u\ntransformation.expression = a, b: a\n")

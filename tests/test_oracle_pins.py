@@ -17,6 +17,7 @@ stubs), so the oracle is checked against:
   * the root property: the state after a step makes the stage residual vanish.
 """
 import ctypes
+import itertools
 import math
 
 import numpy as np
@@ -445,3 +446,86 @@ def test_evolve_stops_at_the_minimum_step():
     assert rc == 5 and acc.size == 0 and failed == 3
     u, t = o.get_state()
     assert t == 0.0 and np.array_equal(u, u0)
+
+
+# ---------------------------------------------------------------- transform / reduce
+def _nodes_xyz(o, dim):
+    """Vertex coordinates of the oracle's compartment-0 nodes on the unit box
+    with 3 cells per axis (vertex id = i + 4 j + 16 k)."""
+    ids = o.compartment_nodes(0)
+    x = (ids % 4) / 3.0
+    y = ((ids // 4) % 4) / 3.0 if dim > 1 else 0 * x
+    z = (ids // 16) / 3.0 if dim > 2 else 0 * x
+    return x, y, z
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("cell_type", ["simplex", "cube"])
+def test_reduce_rule_integrates_degree_four_exactly(dim, cell_type):
+    """The order-4 rule: integration_factor times every monomial of degree
+    <= 4 sums to its integral over the unit box (1 / prod(p_k + 1)); a plain
+    evaluation counts the points (no integration factor), and the initial
+    value enters twice (fold start and final combination)."""
+    o = OracleModel(_exp_ini(dim, cell_type, "ImplicitEuler"))
+    o.set_state(np.ones(o.n_dof), 0.0)
+    exps = [e for e in itertools.product(range(5), repeat=dim) if sum(e) <= 4]
+    ax = "xyz"
+    keys = []
+    for n, e in enumerate(exps):
+        mono = "*".join(f"position_{ax[k]}^{p}" for k, p in enumerate(e) if p) or "1"
+        keys.append(f"[m{n}]\nevaluation.expression = {mono}*integration_factor\n")
+    keys.append("[count]\nevaluation.expression = 1\ninitial.value = 5\n")
+    vals, flags, rc = o.reduce("".join(keys))
+    assert rc == 0
+    for n, e in enumerate(exps):
+        exact = 1.0 / np.prod([p + 1 for p in e])
+        assert abs(vals[f"m{n}"] - exact) <= 1e-13, (e, vals[f"m{n}"], exact)
+    cells = 3 ** dim
+    per_elem = {1: 3, 2: 6, 3: 11}[dim] if cell_type == "simplex" and dim > 1 else 3 ** dim
+    elems = cells * (math.factorial(dim) if cell_type == "simplex" and dim > 1 else 1)
+    if dim == 1:
+        per_elem = 3
+    assert vals["count"] == elems * per_elem + 10
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("cell_type", ["simplex", "cube"])
+def test_reduce_keys_of_the_reference_tests(dim, cell_type):
+    """Keys as test/exp.ini and test/gauss.ini write them, on u = 1 + x + 2y
+    (exact in P1 and Q1): max / min over the points with the (init, val)
+    reduction, the L2 norm of u - (1 + x + 2y) through a transformation,
+    gradients, and error / warn expressions."""
+    o = OracleModel(_exp_ini(dim, cell_type, "ImplicitEuler"))
+    x, y, z = _nodes_xyz(o, dim)
+    o.set_state(1.0 + x + 2.0 * y, 0.0)
+    cfg = """[u_max]
+evaluation.expression = u
+reduction.expression = init, val: max(init, val)
+initial.value = -1e100
+error.expression = arg: arg > 3.5
+[u_min]
+evaluation.expression = u
+reduction.expression = init, val: min(init, val)
+initial.value = 1e100
+warn.expression = arg: arg < 1.5
+[u_error]
+evaluation.expression = (u - (1 + position_x + 2*position_y))^2 * integration_factor
+transformation.expression = arg: sqrt(arg)
+[grad]
+evaluation.expression = (grad_u_x + 10*grad_u_y) * integration_factor
+[mass]
+evaluation.expression = u * integration_factor
+"""
+    vals, flags, rc = o.reduce(cfg)
+    # the points are interior: max/min of 1 + x + 2y over them, below the corner values
+    assert 1.0 < vals["u_min"] < 1.2 and 3.8 < vals["u_max"] < 4.0
+    assert flags["u_max"] == 2 and rc == 1          # 3.5 < max fires the error
+    assert flags["u_min"] == 1                       # min < 1.5 fires the warning
+    assert vals["u_error"] <= 1e-14
+    assert abs(vals["grad"] - 21.0) <= 1e-12
+    assert abs(vals["mass"] - 2.5) <= 1e-13          # int 1 + x + 2y over the unit box
+    # the model's own model.reduce keys are read when no config is given
+    o2 = OracleModel(_exp_ini(dim, cell_type, "ImplicitEuler") + "[model.reduce]\nm.evaluation.expression = u*integration_factor\n")
+    o2.set_state(np.full(o2.n_dof, 2.0), 0.0)
+    v2, _, _ = o2.reduce()
+    assert abs(v2["m"] - 2.0) <= 1e-13
