@@ -90,6 +90,7 @@ _SIGS = {
     "dcg_spmv": (_I, [_P, _P, _P, _P]),
     "dcg_jacobian_apply": (_I, [_P, _P, _D, _D, _D, _P, _P]),
     "dcg_reduce": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_S)]),
+    "dcg_write_vtk": (_I, [_P, ctypes.c_char_p, _I]),
     "dcg_reduce_source": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p, _S, ctypes.POINTER(_S)]),
     "dcg_reduce_result": (_I, [_P, _S, ctypes.c_char_p, _S, ctypes.POINTER(_D), ctypes.POINTER(_I)]),
     "dcg_linear_solve": (_I, [_P, _P, _P, _P, _D, _I, ctypes.POINTER(_I)]),
@@ -283,6 +284,10 @@ class Model:
             bad = [k for k, f in flags.items() if f == 2]
             raise DcgError(f"reduction error expression fired for {bad}: {[out[k] for k in bad]}")
         return out
+
+    def write_vtk(self, path, append: bool = True):
+        """VTK files of the current state (one .vtu per compartment, .pvd series)."""
+        _check(self._lib.dcg_write_vtk(self._h, str(path).encode(), 1 if append else 0))
 
     def reduce_source(self, config: str = "") -> str:
         need = _S()

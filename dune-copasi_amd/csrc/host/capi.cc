@@ -159,6 +159,15 @@ int dcg_reduce(dcg_model_t h, const char* config_ini, size_t* count) {
   });
 }
 
+int dcg_write_vtk(dcg_model_t h, const char* path, int append) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    if (!path || !*path) throw dcopasi::IOError("empty output path");
+    h->m->write_vtk(path, append != 0);
+    return 0;
+  });
+}
+
 int dcg_reduce_source(dcg_model_t h, const char* config_ini, char* buf, size_t len, size_t* needed) {
   if (check(h)) return -1;
   return guarded([&] {

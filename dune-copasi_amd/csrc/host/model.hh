@@ -8,6 +8,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -116,6 +117,10 @@ class Model {
   std::vector<ReduceResult> reduce(const std::string& reduce_ini);
   // the generated source of those reduce kernels (no device needed)
   std::string reduce_source(const std::string& reduce_ini) const;
+  // VTK files of the current state under `path` (one .vtu per compartment and
+  // call, a .pvd time series per compartment; append keeps the earlier
+  // steps written to that path in the series)
+  void write_vtk(const std::string& path, bool append);
   // run-time parameters (parser_context constants by name); only models
   // created with runtime_parameters have them
   const std::vector<std::string>& param_names() const { return spec_.params; }
@@ -223,6 +228,7 @@ class Model {
   std::vector<hipFunction_t> tile_app_fn_;  // tiled matrix-free apply (_a)
   std::vector<hipFunction_t> facet_fn_;
   double time_ = 0.0, dt_initial_ = 0.0, t_end_ = 0.0;
+  std::map<std::string, std::vector<double>> vtk_steps_;  // times written per output path
   int last_krylov_its_ = 8;
 
   // device data

@@ -163,6 +163,14 @@ int dcg_reduce_result(dcg_model_t model, size_t i, char* name, size_t name_len, 
  * no device needed.  *needed gets the size including the terminating zero. */
 int dcg_reduce_source(dcg_model_t model, const char* config_ini, char* buf, size_t len, size_t* needed);
 
+/* VTK output of the current state (the reference's write_vtk,
+ * model_single_compartment.impl.hh:250-319 / model_multi_compartment.impl.hh:
+ * 219-300, used by the CLI for model.writer.vtk.path): per compartment
+ * "<path>/<stem>-<compartment>-NNNNN.vtu" (stem: last component of path)
+ * with one scalar vertex field per species, and the time series
+ * "<path>/<stem>-<compartment>.pvd"; append = 0 restarts the series. */
+int dcg_write_vtk(dcg_model_t model, const char* path, int append);
+
 /* ---------------------------------------------------------------- time loop
  * The reference's SimpleAdaptiveStepper with TimeStepper::evolve and
  * snap_to_time (dune/copasi/common/stepper.hh:95-240,330-366), as its CLI
