@@ -104,7 +104,9 @@ ElementTables element_tables(const Grid& g, const Rule* rule = nullptr) {
     for (int col = 0; col < d; ++col) {
       int cv = g.cube ? (1 << col) : g.corner[p][col + 1];
       int c0 = g.corner[p][0];
-      for (int k = 0; k < d; ++k) J[k][col] = cpos(cv, k) - cpos(c0, k);
+      // unstructured cells: the reference simplex (J = I); the element's own
+      // geometry comes at run time
+      for (int k = 0; k < d; ++k) J[k][col] = g.unstructured ? (k == col ? 1.0 : 0.0) : cpos(cv, k) - cpos(c0, k);
     }
     invert(d, J, Ji, det);
     for (int q = 0; q < T.nq; ++q) {
@@ -116,7 +118,7 @@ ElementTables element_tables(const Grid& g, const Rule* rule = nullptr) {
           T.gphi.push_back(s);
         }
       for (int k = 0; k < 3; ++k) {
-        double s = k < d ? cpos(g.corner[p][0], k) : 0.0;
+        double s = k < d && !g.unstructured ? cpos(g.corner[p][0], k) : 0.0;
         for (int l = 0; l < d && k < d; ++l) s += J[k][l] * r.x[q][l];
         T.xq.push_back(s);
       }
@@ -206,7 +208,14 @@ const char* kPrelude = R"(
 // DCG_GEO: element tables are those of the unit cell and the spacing comes at
 // run time in prm[0..7] = {h_x, h_y, h_z, 1/h_x, 1/h_y, 1/h_z, |cell|, 0}
 #define DCG_GEO %GEO%
-#if DCG_GEO
+// DCG_UNSTRUCTURED: every element carries its own geometry in eg[]: the
+// gradients of its basis functions, the points and |det J| (UG_* offsets)
+#define DCG_UNSTRUCTURED %UNS%
+#if DCG_UNSTRUCTURED
+#define GP(p, q, a, k) (eg[(a) * DIM + (k)])
+#define FACQ(q) (FAC[q] * eg[UG_DET])
+#define XQP(p, q, k) (eg[UG_POS + (q) * 3 + (k)])
+#elif DCG_GEO
 #define GP(p, q, a, k) (GPHI[p][q][a][k] * prm[3 + (k)])
 #define FACQ(q) (FAC[q] * prm[6])
 #define XQP(p, q, k) (XQ[p][q][k] * prm[k])
@@ -241,7 +250,7 @@ template <int P>
 __device__ __forceinline__ void residual(const double (&xl)[NV][BC], double cx, double cy, double cz, double t,
                                          double cm, double ca, double (&rl)[NV][BC], double (*mc)[BC][BC],
                                          double (*gc)[BC][BC][DIM], double (*dc)[BC][BC][DIM][DIM],
-                                         const double* __restrict__ prm) {
+                                         const double* __restrict__ prm, const double* __restrict__ eg) {
 #pragma unroll
   for (int a = 0; a < NV; ++a)
 #pragma unroll
@@ -268,7 +277,8 @@ __device__ __forceinline__ void residual(const double (&xl)[NV][BC], double cx, 
     const double px = cx + XQP(P, q, 0), py = cy + XQP(P, q, 1), pz = cz + XQP(P, q, 2);
     (void)px; (void)py; (void)pz;
     double rs[BC], rf[BC][DIM];
-    qp(q, u, g, px, py, pz, t, cm, ca, rs, rf, mc ? mc[q] : nullptr, gc ? gc[q] : nullptr, dc ? dc[q] : nullptr, prm);
+    qp(q, u, g, px, py, pz, t, cm, ca, rs, rf, mc ? mc[q] : nullptr, gc ? gc[q] : nullptr, dc ? dc[q] : nullptr, prm,
+       eg);
 #pragma unroll
     for (int a = 0; a < NV; ++a)
 #pragma unroll
@@ -285,7 +295,7 @@ template <int P>
 __device__ __forceinline__ void element(const double (&xc)[NCORNER][BC], const int (&sl)[NCORNER], double cx,
                                         double cy, double cz, double t, double cm, double ca, double* r,
                                         double* val, const int* own_start, const unsigned* own_mask, int mode,
-                                        const double* __restrict__ prm) {
+                                        const double* __restrict__ prm, const double* __restrict__ eg) {
   double xl[NV][BC];
 #pragma unroll
   for (int a = 0; a < NV; ++a)
@@ -294,7 +304,7 @@ __device__ __forceinline__ void element(const double (&xc)[NCORNER][BC], const i
   double rl[NV][BC];
   const bool want_j = (mode & 2) != 0;
 #if NUMERICAL_JAC
-  residual<P>(xl, cx, cy, cz, t, cm, ca, rl, nullptr, nullptr, nullptr, prm);
+  residual<P>(xl, cx, cy, cz, t, cm, ca, rl, nullptr, nullptr, nullptr, prm, eg);
   if (want_j) {
 #pragma unroll 1
     for (int c = 0; c < NV; ++c) {
@@ -308,11 +318,15 @@ __device__ __forceinline__ void element(const double (&xc)[NCORNER][BC], const i
         const double delta = FD_EPS * (1.0 + fabs(xl[c][j]));
         xp[c][j] += delta;
         double rp[NV][BC];
-        residual<P>(xp, cx, cy, cz, t, cm, ca, rp, nullptr, nullptr, nullptr, prm);
+        residual<P>(xp, cx, cy, cz, t, cm, ca, rp, nullptr, nullptr, nullptr, prm, eg);
 #pragma unroll
         for (int a = 0; a < NV; ++a) {
           const int sa = sl[CORNER[P][a]];
+#if DCG_UNSTRUCTURED
+          const int pos = own_start[a * NV + c];
+#else
           const int pos = dcg_pos(own_start, own_mask, sa, OFF[P][a][c]);
+#endif
 #pragma unroll
           for (int i = 0; i < BC; ++i)
             val[(size_t)pos * (DCG_B * DCG_B) + i * DCG_B + j] += (rp[a][i] - rl[a][i]) / delta;
@@ -339,7 +353,7 @@ __device__ __forceinline__ void element(const double (&xc)[NCORNER][BC], const i
 #else
               nullptr,
 #endif
-              prm);
+              prm, eg);
   if (want_j) {
 #pragma unroll
     for (int a = 0; a < NV; ++a) {
@@ -348,7 +362,13 @@ __device__ __forceinline__ void element(const double (&xc)[NCORNER][BC], const i
       const unsigned mk = own_mask[sa];
 #pragma unroll
       for (int c = 0; c < NV; ++c) {
+#if DCG_UNSTRUCTURED
+        const int pos = own_start[a * NV + c];
+        (void)st;
+        (void)mk;
+#else
         const int pos = st + __popc(mk & ((1u << OFF[P][a][c]) - 1u));
+#endif
         double* blk = val + (size_t)pos * (DCG_B * DCG_B);
 #pragma unroll
         for (int i = 0; i < BC; ++i)
@@ -394,7 +414,7 @@ extern "C" __global__ void __launch_bounds__(128)
          int first, int count, const double* __restrict__ x, double t, double cm, double ca,
          double* __restrict__ r, double* __restrict__ val, const int* __restrict__ own_start,
          const unsigned* __restrict__ own_mask, int n0, int n1, double ox, double oy, double oz, double hx,
-         double hy, double hz, int mode, const double* __restrict__ prm) {
+         double hy, double hz, int mode, const double* __restrict__ prm, const double* __restrict__ egeo) {
   using namespace %C%;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= count) return;
@@ -409,8 +429,17 @@ extern "C" __global__ void __launch_bounds__(128)
 #pragma unroll
     for (int i = 0; i < BC; ++i) xc[c][i] = sl[c] >= 0 ? x[(size_t)sl[c] * DCG_B + i] : 0.0;
   }
+#if DCG_UNSTRUCTURED
+  // own_start: the cells' block positions [cell][a][c]; egeo: their geometry
+  (void)cell;
+  own_start += (size_t)idx * NV * NV;
+  const double* eg = egeo + (size_t)idx * UG_WORDS;
+  const double cx = 0.0, cy = 0.0, cz = 0.0;
+#else
+  const double* eg = nullptr;
   const int ci = cell % n0, cj = (cell / n0) % n1, ck = cell / (n0 * n1);
   const double cx = ox + ci * hx, cy = oy + cj * hy, cz = oz + ck * hz;
+#endif
 %DISPATCH%
 }
 )";
@@ -587,11 +616,12 @@ void param_vars(const ModelSpec& m, std::map<std::string, std::string>& v) {
     v["$" + m.params[k]] = "prm[" + std::to_string(kGeoWords + k) + "]";
 }
 
-std::map<std::string, std::string> volume_vars(const ModelSpec& m, int comp, double evol, bool geo) {
+std::map<std::string, std::string> volume_vars(const ModelSpec& m, int comp, double evol, bool geo, bool uns = false) {
   std::map<std::string, std::string> v;
   param_vars(m, v);
   v["time"] = "t";
   v["entity_volume"] = geo ? "(" + D(evol) + " * prm[6])" : D(evol);
+  if (uns) v["entity_volume"] = "(" + D(evol) + " * eg[UG_DET])";
   v["in_volume"] = "1.0";
   v["in_boundary"] = "0.0";
   v["in_skeleton"] = "0.0";
@@ -625,9 +655,10 @@ std::string gen_volume(const ModelSpec& m, const Grid& g, int comp, const std::s
   const auto& species = m.comp_species[comp];
   const int bc = static_cast<int>(species.size());
   const int d = m.dim;
-  auto vars = volume_vars(m, comp, T.evol, geo);
+  auto vars = volume_vars(m, comp, T.evol, geo, g.unstructured);
   // numeric diffusion is folded into tables only when the spacing is known
-  auto folded = [&](const TensorTerm& t) { return !geo && tensor_const(t); };
+  // (structured cells of a fixed size)
+  auto folded = [&](const TensorTerm& t) { return !geo && !g.unstructured && tensor_const(t); };
   auto E = [&](const Ex& e) { return expr::emit(e, vars); };
 
   bool has_gc = false, has_dc = false;
@@ -643,6 +674,8 @@ std::string gen_volume(const ModelSpec& m, const Grid& g, int comp, const std::s
   o << "namespace " << ns << " {\n";
   o << "constexpr int BC = " << bc << ", NV = " << T.nv << ", NQ = " << T.nq << ", NCORNER = " << g.ncorner
     << ", NCLASS = " << g.nclass << ", DIM = " << d << ";\n";
+  // per-element geometry words of unstructured cells: basis gradients, points, |det J|
+  o << "constexpr int UG_POS = NV * DIM, UG_DET = UG_POS + NQ * 3, UG_WORDS = UG_DET + 1;\n";
   o << "constexpr double FD_EPS = " << D(m.fd_epsilon) << ";\n";
   std::vector<int> corner;
   for (int p = 0; p < g.nclass; ++p)
@@ -700,8 +733,9 @@ std::string gen_volume(const ModelSpec& m, const Grid& g, int comp, const std::s
   o << "__device__ __forceinline__ void qp(int q, const double (&u)[BC], const double (&g)[BC][3], double px, double py, "
        "double pz, double t, double cm, double ca, double (&rs)[BC], double (&rf)[BC][DIM], "
     << (with_kernel ? "double (*mc)[BC], " : "double* mc, ")
-    << "double (*gc)[BC][DIM], double (*dc)[BC][DIM][DIM], const double* __restrict__ prm) {\n";
-  o << "  (void)px; (void)py; (void)pz; (void)t; (void)g; (void)gc; (void)dc; (void)prm;\n";
+    << "double (*gc)[BC][DIM], double (*dc)[BC][DIM][DIM], const double* __restrict__ prm, "
+       "const double* __restrict__ eg) {\n";
+  o << "  (void)px; (void)py; (void)pz; (void)t; (void)g; (void)gc; (void)dc; (void)prm; (void)eg;\n";
   o << "  const double fac = FACQ(q);\n";
   o << "#pragma unroll\n  for (int i = 0; i < BC; ++i) { rs[i] = 0.0;\n#pragma unroll\n for (int k = 0; k < DIM; ++k) rf[i][k] = 0.0; }\n";
   // tiled compartments keep only the structural nonzeros of mc, in the
@@ -796,7 +830,7 @@ std::string gen_volume(const ModelSpec& m, const Grid& g, int comp, const std::s
   std::ostringstream disp;
   for (int p = 0; p < g.nclass; ++p)
     disp << "  if ((msk >> " << p << ") & 1u) " << ns << "::element<" << p
-         << ">(xc, sl, cx, cy, cz, t, cm, ca, r, val, own_start, own_mask, mode, prm);\n";
+         << ">(xc, sl, cx, cy, cz, t, cm, ca, r, val, own_start, own_mask, mode, prm, eg);\n";
   replace_all(body, "%DISPATCH%", disp.str());
   std::ostringstream full;
   full << "#define NUMERICAL_JAC " << (numerical ? 1 : 0) << "\n#define HAS_GC " << (has_gc ? 1 : 0) << "\n#define HAS_DC "
@@ -970,7 +1004,7 @@ __device__ __forceinline__ void residual_rt(int p, const double (&xl)[NV][BC], d
     }
     const double px = cx + XQP(p, q, 0), py = cy + XQP(p, q, 1), pz = cz + XQP(p, q, 2);
     double rs[BC], rf[BC][DIM];
-    qp(q, u, g, px, py, pz, t, cm, ca, rs, rf, nullptr, nullptr, nullptr, prm);
+    qp(q, u, g, px, py, pz, t, cm, ca, rs, rf, nullptr, nullptr, nullptr, prm, nullptr);
     for (int a = 0; a < NV; ++a)
       for (int i = 0; i < BC; ++i) {
         double s = rs[i] * PHI[q][a];
@@ -1009,7 +1043,7 @@ __device__ __forceinline__ void residual_lds(int p, const double* __restrict__ x
     }
     const double px = cx + XQP(p, q, 0), py = cy + XQP(p, q, 1), pz = cz + XQP(p, q, 2);
     double rs[BC], rf[BC][DIM];
-    qp(q, u, g, px, py, pz, t, cm, ca, rs, rf, nullptr, nullptr, nullptr, prm);
+    qp(q, u, g, px, py, pz, t, cm, ca, rs, rf, nullptr, nullptr, nullptr, prm, nullptr);
     for (int a = 0; a < NV; ++a) {
       double* d = dst + CORNER[p][a] * BC;
       for (int i = 0; i < BC; ++i) {
@@ -1155,7 +1189,7 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
         const double py = oy + (vj0 + cj) * hy + XQP(p, q, 1);
         const double pz = oz + (vk0 + ck) * hz + XQP(p, q, 2);
         double rs[BC], rf[BC][DIM], mc[NMC > 0 ? NMC : 1];
-        qp(q, u, g, px, py, pz, t, cm, ca, rs, rf, mc, nullptr, nullptr, prm);
+        qp(q, u, g, px, py, pz, t, cm, ca, rs, rf, mc, nullptr, nullptr, prm, nullptr);
 #pragma unroll
         for (int n = 0; n < NMC; ++n) work[(size_t)(pq * NMC + n) * NCB + cl] = mc[n];
       }
@@ -1715,6 +1749,7 @@ GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L, bool r
   std::string pre = kPrelude;
   replace_all(pre, "%B%", std::to_string(L.B));
   replace_all(pre, "%GEO%", geo ? "1" : "0");
+  replace_all(pre, "%UNS%", L.grid.unstructured ? "1" : "0");
   std::ostringstream src;
   src << pre;
   const bool numerical = spec.jacobian == JacobianType::Numerical && !spec.is_linear;
@@ -1728,7 +1763,7 @@ GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L, bool r
     }
     std::string ns = "vol" + std::to_string(c);
     std::string kn = "dcg_volume_" + std::to_string(c);
-    const bool tiled = tile_eligible(spec, static_cast<int>(c));
+    const bool tiled = !L.grid.unstructured && tile_eligible(spec, static_cast<int>(c));
     src << gen_volume(spec, L.grid, static_cast<int>(c), ns, kn, numerical, geo, !tiled);
     out.volume_kernel.push_back(tiled ? "" : kn);
     if (tiled) {
@@ -1764,6 +1799,53 @@ GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L, bool r
     out.facet_kernels.push_back(fk);
   }
   out.source = src.str();
+  return out;
+}
+
+double cell_det(const Grid& g, long cell) {
+  const int d = g.dim;
+  double x0[3], xk[3], J[3][3] = {{0}}, Ji[3][3] = {{0}}, det = 0;
+  g.vertex_pos(g.cell_vertex(cell, 0), x0);
+  for (int col = 0; col < d; ++col) {
+    g.vertex_pos(g.cell_vertex(cell, col + 1), xk);
+    for (int k = 0; k < d; ++k) J[k][col] = xk[k] - x0[k];
+  }
+  invert(d, J, Ji, det);
+  return det;
+}
+
+std::vector<double> cell_geometry(const Grid& g, const std::vector<int>& cells) {
+  const int d = g.dim, nv = g.nvpe;
+  const Rule r = element_rule(d, false);
+  const int nq = static_cast<int>(r.w.size());
+  std::vector<double> out;
+  out.reserve(cells.size() * (nv * d + nq * 3 + 1));
+  double phi[8], dref[8][3];
+  const double xi0[3] = {0, 0, 0};
+  shape(d, false, xi0, phi, dref);  // P1: constant reference gradients
+  for (int cell : cells) {
+    double x0[3], xk[3], J[3][3] = {{0}}, Ji[3][3] = {{0}}, det = 0;
+    g.vertex_pos(g.cell_vertex(cell, 0), x0);
+    for (int col = 0; col < d; ++col) {
+      g.vertex_pos(g.cell_vertex(cell, col + 1), xk);
+      for (int k = 0; k < d; ++k) J[k][col] = xk[k] - x0[k];
+    }
+    invert(d, J, Ji, det);
+    if (!(det > 0.0)) throw Error("degenerate cell " + std::to_string(cell) + " in the grid file");
+    for (int a = 0; a < nv; ++a)
+      for (int k = 0; k < d; ++k) {
+        double s = 0;
+        for (int l = 0; l < d; ++l) s += dref[a][l] * Ji[l][k];
+        out.push_back(s);
+      }
+    for (int q = 0; q < nq; ++q)
+      for (int k = 0; k < 3; ++k) {
+        double s = k < d ? x0[k] : 0.0;
+        for (int l = 0; l < d && k < d; ++l) s += J[k][l] * r.x[q][l];
+        out.push_back(s);
+      }
+    out.push_back(det);
+  }
   return out;
 }
 

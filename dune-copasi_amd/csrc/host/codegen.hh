@@ -44,6 +44,13 @@ constexpr int kGeoWords = 8;
 // so models that differ only in mesh size and parameters share one source.
 GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& layout, bool runtime_geometry = false);
 
+// Geometry words of unstructured cells, in the order of `cells`, as the
+// generated cell kernels read them: the gradients of the element's basis
+// functions [a][k], the points of the order-2 rule [q][3], |det J|.
+std::vector<double> cell_geometry(const Grid& g, const std::vector<int>& cells);
+// |det J| of one unstructured cell
+double cell_det(const Grid& g, long cell);
+
 struct ReduceKey;
 // Device source of a transform/reduce evaluation: kernel "dcg_reduce_<c>" per
 // compartment with cells ("" otherwise) folds every key's evaluation over the

@@ -1,6 +1,7 @@
 #include "layout.hh"
 
 #include <algorithm>
+#include <fstream>
 #include <cmath>
 #include <numeric>
 #include <unordered_map>
@@ -63,6 +64,97 @@ Grid make_grid(int dim, const std::vector<double>& cells, const std::vector<doub
   return g;
 }
 
+Grid read_gmsh(const std::string& path, int dim) {
+  if (dim < 1 || dim > 3) throw IOError("grid.dimension must be 1, 2 or 3");
+  std::ifstream in(path);
+  if (!in) throw IOError("cannot open grid file '" + path + "'");
+  std::string tok;
+  std::unordered_map<long, std::array<double, 3>> nodes;
+  struct Elem {
+    std::array<long, 4> v;
+    double phys;
+  };
+  std::vector<Elem> elems;
+  const int cell_type = dim == 1 ? 1 : dim == 2 ? 2 : 4;  // line, triangle, tetrahedron
+  const int nvpe = dim + 1;
+  bool have_format = false;
+  while (in >> tok) {
+    if (tok == "$MeshFormat") {
+      double version;
+      int file_type, data_size;
+      in >> version >> file_type >> data_size;
+      if (version < 2.0 || version >= 3.0) throw NotImplemented("gmsh format " + std::to_string(version) + " (2.x only)");
+      if (file_type != 0) throw NotImplemented("binary gmsh files");
+      have_format = true;
+    } else if (tok == "$Nodes") {
+      long n;
+      in >> n;
+      for (long i = 0; i < n; ++i) {
+        long id;
+        std::array<double, 3> x;
+        in >> id >> x[0] >> x[1] >> x[2];
+        nodes[id] = x;
+      }
+    } else if (tok == "$Elements") {
+      long n;
+      in >> n;
+      for (long i = 0; i < n; ++i) {
+        long id;
+        int type, ntags;
+        in >> id >> type >> ntags;
+        std::vector<long> tags(ntags);
+        for (auto& t : tags) in >> t;
+        static const int nv_of[16] = {0, 2, 3, 4, 4, 8, 6, 5, 3, 6, 9, 10, 27, 18, 14, 1};
+        if (type < 1 || type > 15) throw NotImplemented("gmsh element type " + std::to_string(type));
+        std::vector<long> v(nv_of[type]);
+        for (auto& x : v) in >> x;
+        const int edim = (type == 15) ? 0 : (type == 1 || type == 8) ? 1 : (type == 2 || type == 3 || type == 9 || type == 10) ? 2 : 3;
+        if (edim != dim) continue;  // lower-dimensional elements are boundary segments
+        if (type != cell_type) throw NotImplemented("gmsh cells of type " + std::to_string(type) + " (simplices only)");
+        Elem e;
+        e.v = {-1, -1, -1, -1};
+        for (int a = 0; a < nvpe; ++a) e.v[a] = v[a];
+        e.phys = ntags > 0 ? static_cast<double>(tags[0]) : 0.0;
+        elems.push_back(e);
+      }
+    }
+    if (!in && !in.eof()) throw IOError("malformed gmsh file '" + path + "'");
+  }
+  if (!have_format) throw IOError("'" + path + "' has no $MeshFormat section");
+  if (elems.empty()) throw IOError("'" + path + "' has no cells of dimension " + std::to_string(dim));
+  // used nodes in increasing node number
+  std::vector<long> used;
+  for (const auto& e : elems)
+    for (int a = 0; a < nvpe; ++a) used.push_back(e.v[a]);
+  std::sort(used.begin(), used.end());
+  used.erase(std::unique(used.begin(), used.end()), used.end());
+  std::unordered_map<long, int> index;
+  Grid g;
+  g.dim = dim;
+  g.cube = false;
+  g.unstructured = true;
+  g.nvpe = g.ncorner = nvpe;
+  g.nclass = 1;
+  std::array<int, 8> c{};
+  for (int a = 0; a < 8; ++a) c[a] = a;
+  g.corner.push_back(c);
+  g.nvert = static_cast<long>(used.size());
+  g.ncell = static_cast<long>(elems.size());
+  if (g.nvert >= (1L << 31) / 2) throw NotImplemented("grid too large for 32-bit slot indices");
+  g.X.resize(3 * used.size());
+  for (size_t i = 0; i < used.size(); ++i) {
+    auto it = nodes.find(used[i]);
+    if (it == nodes.end()) throw IOError("gmsh cell uses undefined node " + std::to_string(used[i]));
+    index[used[i]] = static_cast<int>(i);
+    for (int k = 0; k < 3; ++k) g.X[3 * i + k] = k < dim ? it->second[k] : 0.0;
+  }
+  for (const auto& e : elems) {
+    for (int a = 0; a < nvpe; ++a) g.conn.push_back(index[e.v[a]]);
+    g.gmsh_id.push_back(e.phys);
+  }
+  return g;
+}
+
 void Grid::cell_ijk(long cell, long ijk[3]) const {
   ijk[0] = cell % n[0];
   long r = cell / n[0];
@@ -71,12 +163,17 @@ void Grid::cell_ijk(long cell, long ijk[3]) const {
 }
 
 long Grid::cell_vertex(long cell, int c) const {
+  if (unstructured) return conn[static_cast<size_t>(cell) * nvpe + c];
   long ijk[3];
   cell_ijk(cell, ijk);
   return vertex(ijk[0] + (c & 1), ijk[1] + ((c >> 1) & 1), ijk[2] + ((c >> 2) & 1));
 }
 
 void Grid::vertex_pos(long v, double x[3]) const {
+  if (unstructured) {
+    for (int a = 0; a < 3; ++a) x[a] = a < dim ? X[3 * static_cast<size_t>(v) + a] : 0.0;
+    return;
+  }
   long i = v % (n[0] + 1);
   long r = v / (n[0] + 1);
   long j = dim > 1 ? r % (n[1] + 1) : 0;
@@ -179,6 +276,7 @@ Rule reduce_rule(int dim, bool cube) {
 std::vector<int8_t> mark_elements(const Grid& g, const std::vector<expr::Ex>& comp_expr) {
   if (comp_expr.size() > 120) throw NotImplemented("more than 120 compartments");
   std::map<std::string, int> slots = {{"position_x", 0}, {"position_y", 1}, {"position_z", 2}};
+  if (g.unstructured) slots["gmsh_id"] = 3;  // cell data of gmsh grids (make_multi_domain_grid.hh:99-101)
   std::vector<expr::Bound> ev;
   std::vector<char> null;
   for (auto& e : comp_expr) {
@@ -190,7 +288,7 @@ std::vector<int8_t> mark_elements(const Grid& g, const std::vector<expr::Ex>& co
     double cx[8][3];
     for (int c = 0; c < g.ncorner; ++c) g.vertex_pos(g.cell_vertex(cell, c), cx[c]);
     for (int cls = 0; cls < g.nclass; ++cls) {
-      double x[3] = {0, 0, 0};
+      double x[4] = {0, 0, 0, g.unstructured ? g.gmsh_id[cell] : 0.0};
       for (int a = 0; a < g.nvpe; ++a)
         for (int k = 0; k < 3; ++k) x[k] += cx[g.corner[cls][a]][k] / g.nvpe;
       int found = -1;
@@ -343,14 +441,25 @@ Layout make_layout(const Grid& g, const std::vector<int8_t>& elem_comp, std::vec
           C.g2l[v] = static_cast<int>(C.nodes.size());
           C.nodes.push_back(v);
         }
-      // colour = parity pattern of the cell index: same-colour cells share no vertex
-      std::vector<std::vector<int>> by_color(1 << d);
+      // colour = parity pattern of the cell index: same-colour cells share no
+      // vertex; unstructured cells are coloured greedily in cell order
+      std::vector<std::vector<int>> by_color(g.unstructured ? 0 : 1 << d);
+      std::vector<uint64_t> vcol(g.unstructured ? g.nvert : 0, 0);
       for (long cell = 0; cell < g.ncell; ++cell) {
         if (!mask[cell]) continue;
-        long ijk[3];
-        g.cell_ijk(cell, ijk);
         int col = 0;
-        for (int k = 0; k < d; ++k) col |= static_cast<int>(ijk[k] & 1) << k;
+        if (g.unstructured) {
+          uint64_t taken = 0;
+          for (int a = 0; a < g.nvpe; ++a) taken |= vcol[g.cell_vertex(cell, a)];
+          while (col < 64 && ((taken >> col) & 1)) ++col;
+          if (col == 64) throw Error("cell colouring overflow");
+          for (int a = 0; a < g.nvpe; ++a) vcol[g.cell_vertex(cell, a)] |= 1ull << col;
+          if (static_cast<int>(by_color.size()) <= col) by_color.resize(col + 1);
+        } else {
+          long ijk[3];
+          g.cell_ijk(cell, ijk);
+          for (int k = 0; k < d; ++k) col |= static_cast<int>(ijk[k] & 1) << k;
+        }
         by_color[col].push_back(static_cast<int>(cell));
       }
       C.color_ptr.assign(1, 0);
@@ -375,8 +484,10 @@ Layout make_layout(const Grid& g, const std::vector<int8_t>& elem_comp, std::vec
   L.slot_comp.assign(slot, -1);
   for (size_t ci = 0; ci < comps.size(); ++ci)
     for (long s = 0; s < comps[ci].nslot; ++s) L.slot_comp[comps[ci].slot_off + s] = static_cast<int>(ci);
-  // cell corner slots and own-compartment stencil masks
+  // cell corner slots and own-compartment stencil masks (unstructured: the
+  // own-compartment neighbour lists)
   L.own_mask.assign(slot, 0u);
+  std::vector<std::vector<int>> own_nb(g.unstructured ? slot : 0);
   for (auto& C : comps) {
     C.cell_slots.assign(C.cells.size() * g.ncorner, -1);
     for (size_t i = 0; i < C.cells.size(); ++i) {
@@ -390,9 +501,13 @@ Layout make_layout(const Grid& g, const std::vector<int8_t>& elem_comp, std::vec
         for (int a = 0; a < g.nvpe; ++a)
           for (int b = 0; b < g.nvpe; ++b) {
             int ca = g.corner[cls][a], cb = g.corner[cls][b];
+            int sa = C.cell_slots[i * g.ncorner + ca];
+            if (g.unstructured) {
+              own_nb[sa].push_back(C.cell_slots[i * g.ncorner + cb]);
+              continue;
+            }
             int delta[3];
             corner_delta(ca, cb, d, delta);
-            int sa = C.cell_slots[i * g.ncorner + ca];
             L.own_mask[sa] |= 1u << stencil_index(d, delta);
           }
       }
@@ -504,7 +619,14 @@ Layout make_layout(const Grid& g, const std::vector<int8_t>& elem_comp, std::vec
     auto& ex = extra[s];
     std::sort(ex.begin(), ex.end());
     ex.erase(std::unique(ex.begin(), ex.end()), ex.end());
-    nnz += static_cast<long>(ex.size()) + __builtin_popcount(L.own_mask[s]);
+    if (g.unstructured) {
+      auto& nb = own_nb[s];
+      std::sort(nb.begin(), nb.end());
+      nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+      nnz += static_cast<long>(ex.size() + nb.size());
+    } else {
+      nnz += static_cast<long>(ex.size()) + __builtin_popcount(L.own_mask[s]);
+    }
   }
   if (nnz >= (1L << 31)) throw NotImplemented("BCSR pattern exceeds 32-bit indexing");
   L.col.resize(nnz);
@@ -519,6 +641,14 @@ Layout make_layout(const Grid& g, const std::vector<int8_t>& elem_comp, std::vec
       while (e < ex.size() && ex[e] < C.slot_off) L.col[p++] = ex[e++];
       L.own_start[s] = static_cast<int>(p);
       long v = C.nodes[l];
+      if (g.unstructured) {
+        for (int t : own_nb[s]) {
+          if (t == s) L.diag_pos[s] = static_cast<int>(p);
+          L.col[p++] = t;
+        }
+        while (e < ex.size()) L.col[p++] = ex[e++];
+        continue;
+      }
       for (int o = 0; o < nstencil; ++o) {
         if (!((L.own_mask[s] >> o) & 1)) continue;
         int dx = o % 3 - 1, dy = d > 1 ? (o / 3) % 3 - 1 : 0, dz = d > 2 ? o / 9 - 1 : 0;
@@ -534,6 +664,23 @@ Layout make_layout(const Grid& g, const std::vector<int8_t>& elem_comp, std::vec
   L.row_ptr[slot] = static_cast<int>(p);
   for (long s = 0; s < slot; ++s)
     if (L.diag_pos[s] < 0) throw Error("internal: missing diagonal block");
+  if (g.unstructured) {
+    // block of every (a, c) vertex pair of every cell, for the cell kernels
+    for (auto& C : comps) {
+      const int nv = g.nvpe;
+      C.cell_pos.assign(C.cells.size() * nv * nv, -1);
+      for (size_t i = 0; i < C.cells.size(); ++i)
+        for (int a = 0; a < nv; ++a)
+          for (int c = 0; c < nv; ++c) {
+            const int sa = C.cell_slots[i * g.ncorner + a], sc = C.cell_slots[i * g.ncorner + c];
+            const int* b = L.col.data() + L.own_start[sa];
+            const int* e = b + own_nb[sa].size();
+            const int* it = std::lower_bound(b, e, sc);
+            if (it == e || *it != sc) throw Error("internal: cell block missing from pattern");
+            C.cell_pos[(i * nv + a) * nv + c] = static_cast<int>(it - L.col.data());
+          }
+    }
+  }
   L.comps = std::move(comps);
   return L;
 }

@@ -43,6 +43,12 @@ struct Grid {
   int ncorner = 2;  // vertices per cell (2^d)
   // corner[cls][a]: cell-corner index (bit k = +1 along axis k) of local vertex a
   std::vector<std::array<int, 8>> corner;
+  // unstructured simplex mesh read from `grid.path`: every cell is one
+  // element (one class, corners = its vertices in file order)
+  bool unstructured = false;
+  std::vector<double> X;        // 3 coordinates per vertex
+  std::vector<int> conn;        // nvpe vertices per cell
+  std::vector<double> gmsh_id;  // physical entity of each cell
 
   long vertex(long i, long j, long k) const { return i + (n[0] + 1) * (j + (n[1] + 1) * k); }
   long cell_vertex(long cell, int c) const;  // global vertex of a cell corner
@@ -74,6 +80,7 @@ struct Compartment {
   std::vector<uint8_t> cell_mask;       // which Kuhn classes of the cell belong here
   std::vector<int> cell_slots;          // [cell][corner] slot or -1
   std::vector<int> color_ptr;           // colour ranges into cells
+  std::vector<int> cell_pos;            // unstructured: [cell][a][c] BCSR block of (a, c)
 };
 
 struct Layout {
@@ -99,6 +106,13 @@ struct SpeciesDecl {
 // Build the grid from `grid.*` keys.
 Grid make_grid(int dim, const std::vector<double>& cells, const std::vector<double>& ext,
                const std::vector<double>& origin, int refine, bool cube);
+
+// Read a gmsh file (format 2, ASCII), as dune-grid's GmshReader does for the
+// reference (make_multi_domain_grid.hh:56-63): the elements of the grid
+// dimension (triangles in 2D, tetrahedra in 3D, lines in 1D) become cells,
+// their first tag (the physical entity) becomes the cell datum `gmsh_id`, and
+// the vertices they use are numbered in increasing node number.
+Grid read_gmsh(const std::string& path, int dim);
 
 // Assign elements to compartments by evaluating each compartment's
 // expression at element centres; returns per element (cell*nclass+cls) the

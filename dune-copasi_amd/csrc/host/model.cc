@@ -72,12 +72,18 @@ void Model::build(const std::string& ini, const std::string& overrides) {
   cfg_tree_ = ParameterTree::from_ini(ini, overrides);
   const auto& c = cfg_tree_;
   const int dim = c.get("grid.dimension", 2);
-  if (c.has_key("grid.path")) throw NotImplemented("grid files (grid.path) are not read by this build");
   spec_ = read_model(c, dim, !opt_.runtime_parameters);
-  const std::string cell = c.get("grid.cell_type", std::string("simplex"));
-  if (cell != "simplex" && cell != "cube") throw IOError("grid.cell_type must be 'simplex' or 'cube'");
-  Grid g = make_grid(dim, c.get_list("grid.cells"), c.get_list("grid.extensions"), c.get_list("grid.origin"),
-                     c.get("grid.refinement_level", 0), cell == "cube");
+  Grid g;
+  if (c.has_key("grid.path")) {
+    // unstructured simplices from a gmsh file (make_multi_domain_grid.hh:56-63)
+    if (c.get("grid.refinement_level", 0) > 0) throw NotImplemented("refinement of grids read from files");
+    g = read_gmsh(c.get("grid.path", std::string()), dim);
+  } else {
+    const std::string cell = c.get("grid.cell_type", std::string("simplex"));
+    if (cell != "simplex" && cell != "cube") throw IOError("grid.cell_type must be 'simplex' or 'cube'");
+    g = make_grid(dim, c.get_list("grid.cells"), c.get_list("grid.extensions"), c.get_list("grid.origin"),
+                  c.get("grid.refinement_level", 0), cell == "cube");
+  }
   auto elem_comp = mark_elements(g, spec_.comp_expr);
   std::vector<Compartment> comps(spec_.comp_names.size());
   for (size_t i = 0; i < comps.size(); ++i) {
@@ -242,6 +248,10 @@ void Model::setup_device() {
     comp_dev_[c].cells.upload(C.cells, stream_);
     comp_dev_[c].cell_mask.upload(C.cell_mask, stream_);
     comp_dev_[c].cell_slots.upload(C.cell_slots, stream_);
+    if (gr.unstructured && C.b) {
+      comp_dev_[c].egeo.upload(cell_geometry(gr, C.cells), stream_);
+      comp_dev_[c].cpos.upload(C.cell_pos, stream_);
+    }
     if (!C.b) continue;
     if (gen_.tile_kernel[c].empty()) {
       all_tiled_ = false;
@@ -347,11 +357,16 @@ void Model::setup_device() {
         wint[a] += r.w[q] * det * phi;
       }
     }
+    if (L_.grid.unstructured)
+      for (auto& v : wint) v /= det;  // reference-simplex integrals; each cell scales by its own |det J|
     for (const auto& C : L_.comps)
-      for (size_t i = 0; i < C.cells.size(); ++i)
+      for (size_t i = 0; i < C.cells.size(); ++i) {
+        const double s = L_.grid.unstructured ? cell_det(L_.grid, C.cells[i]) : 1.0;
         for (int cls = 0; cls < L_.grid.nclass; ++cls)
           if ((C.cell_mask[i] >> cls) & 1)
-            for (int a = 0; a < L_.grid.nvpe; ++a) w[C.cell_slots[i * L_.grid.ncorner + L_.grid.corner[cls][a]]] += wint[a];
+            for (int a = 0; a < L_.grid.nvpe; ++a)
+              w[C.cell_slots[i * L_.grid.ncorner + L_.grid.corner[cls][a]]] += s * wint[a];
+      }
     weights_.upload(w, stream_);
   }
   const size_t nv = static_cast<size_t>(ns) * B;
@@ -560,11 +575,14 @@ void Model::launch_assembly(const double* x, double t, double cm, double ca, con
     const int* cells = comp_dev_[c].cells.get();
     const unsigned char* cmask = comp_dev_[c].cell_mask.get();
     const int* cslots = comp_dev_[c].cell_slots.get();
+    // unstructured cells find their blocks in cpos and carry their geometry
+    const int* starts = g.unstructured ? comp_dev_[c].cpos.get() : own_start;
+    const double* egeo = comp_dev_[c].egeo.get();
     for (size_t col = 0; col + 1 < C.color_ptr.size(); ++col) {
       int first = C.color_ptr[col], count = C.color_ptr[col + 1] - first;
       if (count <= 0) continue;
-      void* args[] = {&cells, &cmask, &cslots, &first, &count, &x, &t, &cm, &ca, &r, &val, &own_start,
-                      &own_mask, &n0, &n1, &ox, &oy, &oz, &hx, &hy, &hz, &mode, &prm};
+      void* args[] = {&cells, &cmask, &cslots, &first, &count, &x, &t, &cm, &ca, &r, &val, &starts,
+                      &own_mask, &n0, &n1, &ox, &oy, &oz, &hx, &hy, &hz, &mode, &prm, &egeo};
       const unsigned block = 128, grid = static_cast<unsigned>((count + block - 1) / block);
       DCG_HIP(hipModuleLaunchKernel(vol_fn_[c], grid, 1, 1, block, 1, 1, 0, stream_, args, nullptr));
     }

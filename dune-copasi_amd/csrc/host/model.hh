@@ -243,6 +243,9 @@ class Model {
     DeviceArray<int> g2l, tiles;
     DeviceArray<unsigned char> cell_mask_grid;
     int ntiles = 0;
+    // unstructured cells: geometry words and block positions per cell
+    DeviceArray<double> egeo;
+    DeviceArray<int> cpos;
   };
   bool all_tiled_ = false;
   DeviceArray<int> extra_pos_;  // BCSR blocks coupling two compartments (zeroed before facets)

@@ -73,6 +73,7 @@ double reduce_call(const ReduceFunction& f, const std::vector<double>& x, const 
 }
 
 std::vector<ReduceKey> Model::reduce_keys(const std::string& reduce_ini) const {
+  if (L_.grid.unstructured) throw NotImplemented("transform/reduce on grids read from files");
   if (reduce_ini.empty()) return read_reduce(cfg_tree_, "model.reduce", spec_.ctx);
   return read_reduce(ParameterTree::from_ini(reduce_ini, ""), "", spec_.ctx);
 }

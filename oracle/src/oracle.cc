@@ -32,6 +32,7 @@
 #include <cmath>
 #include <limits>
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <memory>
 #include <numeric>
@@ -105,7 +106,75 @@ struct Mesh {
   long nvert = 0, nelem = 0;
   std::vector<double> X;    // 3 * nvert
   std::vector<long> conn;   // nvpe * nelem
+  std::vector<double> phys; // gmsh physical entity per element (grids read from files)
 };
+
+// gmsh format 2 (ASCII), read as dune-grid's GmshReader feeds the reference
+// (make_multi_domain_grid.hh:56-63): elements of the grid dimension are the
+// cells (lower-dimensional ones are boundary segments), the first tag is the
+// element's physical entity (cell datum gmsh_id), and the nodes the cells use
+// are numbered in increasing node number.
+static void read_gmsh(Mesh& m, const std::string& path) {
+  std::ifstream in(path);
+  if (!in) throw std::runtime_error("oracle: cannot open '" + path + "'");
+  const int want = m.dim == 1 ? 1 : m.dim == 2 ? 2 : 4;
+  const int nvpe = m.dim + 1;
+  std::map<long, std::array<double, 3>> node;
+  std::vector<std::vector<long>> cells;
+  std::vector<double> phys;
+  std::string w;
+  while (in >> w) {
+    if (w == "$Nodes") {
+      long n;
+      in >> n;
+      for (long i = 0; i < n; ++i) {
+        long id;
+        std::array<double, 3> x;
+        in >> id >> x[0] >> x[1] >> x[2];
+        if (!in) throw std::runtime_error("oracle: malformed $Nodes in '" + path + "'");
+        node[id] = x;
+      }
+    } else if (w == "$Elements") {
+      long n;
+      in >> n;
+      for (long i = 0; i < n; ++i) {
+        long id;
+        int type, nt;
+        in >> id >> type >> nt;
+        std::vector<long> tag(nt);
+        for (auto& t : tag) in >> t;
+        int nv = type == 15 ? 1 : type == 1 ? 2 : type == 2 ? 3 : type == 4 ? 4 : -1;
+        if (nv < 0) throw std::runtime_error("oracle: gmsh element type " + std::to_string(type) + " not restated");
+        std::vector<long> v(nv);
+        for (auto& x : v) in >> x;
+        if (!in) throw std::runtime_error("oracle: malformed $Elements in '" + path + "'");
+        if (type == want) {
+          cells.push_back(v);
+          phys.push_back(nt ? (double)tag[0] : 0.0);
+        }
+      }
+    }
+  }
+  if (cells.empty()) throw std::runtime_error("oracle: no cells of dimension " + std::to_string(m.dim) + " in '" + path + "'");
+  std::vector<long> ids;
+  for (auto& c : cells) ids.insert(ids.end(), c.begin(), c.end());
+  std::sort(ids.begin(), ids.end());
+  ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+  std::map<long, long> idx;
+  m.cube = false;
+  m.nvpe = nvpe;
+  m.nvert = (long)ids.size();
+  m.nelem = (long)cells.size();
+  m.X.assign(3 * m.nvert, 0.0);
+  for (long i = 0; i < m.nvert; ++i) {
+    idx[ids[i]] = i;
+    for (int k = 0; k < m.dim; ++k) m.X[3 * i + k] = node.at(ids[i])[k];
+  }
+  m.conn.clear();
+  for (auto& c : cells)
+    for (int a = 0; a < nvpe; ++a) m.conn.push_back(idx[c[a]]);
+  m.phys = phys;
+}
 
 static long vid(const Mesh& m, long i, long j, long k) {
   return i + (m.nc[0] + 1) * (j + (m.nc[1] + 1) * k);
@@ -579,6 +648,7 @@ static void mark_compartments(Model& M) {
     for (int a = 0; a < m.nvpe; ++a)
       for (int k = 0; k < 3; ++k) cen[k] += m.X[3 * c[a] + k] / m.nvpe;
     for (int k = 0; k < 3; ++k) v[V_POS + k] = cen[k];
+    if (!m.phys.empty()) v[V_SPEC + 4 * M.spec.size()] = m.phys[e];
     int found = -1;
     for (size_t i = 0; i < ce.size(); ++i) {
       if (!ce[i].null() && ce[i](v.data()) != 0.0) {
@@ -1814,9 +1884,15 @@ static void configure(Model& M, const std::string& ini, const std::string& over)
   std::string ct = I.get("grid.cell_type", "simplex");
   if (ct != "simplex" && ct != "cube") throw std::runtime_error("oracle: grid.cell_type must be simplex or cube");
   m.cube = (ct == "cube");
-  build_mesh(m);
+  if (I.has("grid.path")) {
+    if (lvl > 0) throw std::runtime_error("oracle: refinement of grids read from files is not restated");
+    read_gmsh(m, I.get("grid.path"));
+  } else {
+    build_mesh(m);
+  }
   build_species(M);
   build_scope(M);
+  if (!m.phys.empty()) M.scope.vars["gmsh_id"] = V_SPEC + 4 * (int)M.spec.size();
   build_equations(M);
   mark_compartments(M);
   build_layout(M);

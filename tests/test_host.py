@@ -286,3 +286,48 @@ def test_reduce_configuration_errors(dca, wl):
         m.reduce_source("[k]\nevaluation.expression = u\nreduction.expression = a: a\n")
     with pytest.raises(dca.DcgError, match="exactly 1"):
         m.reduce_source("[k]\nevaluation.expression = u\ntransformation.expression = a, b: a\n")
+
+
+# ---------------------------------------------------------------- grids read from gmsh files
+@pytest.fixture(scope="module")
+def gmsh_files(tmp_path_factory):
+    from tests import gmsh_meshes
+    d = tmp_path_factory.mktemp("gmsh")
+    gmsh_meshes.square(d / "square.msh", 8)
+    gmsh_meshes.cube(d / "cube.msh", 3)
+    return d
+
+
+@pytest.mark.parametrize("mesh,dim", [("square.msh", 2), ("cube.msh", 3)])
+@pytest.mark.parametrize("model", ["gray_scott", "two_halves"])
+def test_gmsh_layout_matches_oracle(dca, gmsh_files, mesh, dim, model):
+    """Cells of the grid dimension, nodes numbered by increasing node number,
+    compartments by physical entity (gmsh_id): the same DOF layout as the
+    oracle's independent reader; the generated cell kernels compile."""
+    from tests import unstructured_models
+    from tests.oracle_lib import OracleModel
+    ini = getattr(unstructured_models, model)(gmsh_files / mesh, dim)
+    m = dca.Model(ini, "", -1)
+    o = OracleModel(ini)
+    assert m.n_dof == o.n_dof
+    for c, (a, b) in enumerate(zip(m.compartments(), o.compartments())):
+        assert (a["n_nodes"], a["dof_offset"], a["n_species"]) == (b["n_nodes"], b["dof_offset"], b["n_species"])
+        assert np.array_equal(m.compartment_nodes(c), o.compartment_nodes(c))
+    src = m.generated_source()
+    assert "dcg_tile" not in src and "DCG_UNSTRUCTURED 1" in src
+    rc, log = _hiprtc_compile(src)
+    assert rc == 0, log
+
+
+def test_gmsh_errors(dca, gmsh_files, tmp_path):
+    from tests import unstructured_models
+    ini = unstructured_models.gray_scott(gmsh_files / "square.msh")
+    with pytest.raises(dca.DcgError, match="refinement of grids read from files"):
+        dca.Model(ini + "[grid]\nrefinement_level = 1\n", "", -1)
+    with pytest.raises(dca.DcgError, match="cannot open"):
+        dca.Model(unstructured_models.gray_scott(tmp_path / "missing.msh"), "", -1)
+    quad = tmp_path / "quad.msh"
+    quad.write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n4\n1 0 0 0\n2 1 0 0\n3 1 1 0\n4 0 1 0\n$EndNodes\n"
+                    "$Elements\n1\n1 3 2 1 1 1 2 3 4\n$EndElements\n")
+    with pytest.raises(dca.DcgError, match="simplices only"):
+        dca.Model(unstructured_models.gray_scott(quad), "", -1)
