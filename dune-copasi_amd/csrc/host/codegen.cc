@@ -1814,9 +1814,9 @@ double cell_det(const Grid& g, long cell) {
   return det;
 }
 
-std::vector<double> cell_geometry(const Grid& g, const std::vector<int>& cells) {
+std::vector<double> cell_geometry(const Grid& g, const std::vector<int>& cells, const Rule* rule) {
   const int d = g.dim, nv = g.nvpe;
-  const Rule r = element_rule(d, false);
+  const Rule r = rule ? *rule : element_rule(d, false);
   const int nq = static_cast<int>(r.w.size());
   std::vector<double> out;
   out.reserve(cells.size() * (nv * d + nq * 3 + 1));
@@ -1876,6 +1876,9 @@ std::string generate_reduce(const ModelSpec& m, const Layout& L, const std::vect
     o << "namespace " << ns << " {\n";
     o << "constexpr int DIM = " << g.dim << ", NV = " << T.nv << ", NQR = " << T.nq << ", NCLASS = " << g.nclass
       << ", NCORNER = " << g.ncorner << ", BC = " << (bc > 0 ? bc : 1) << ", NK = " << nk << ";\n";
+    // cells of grids read from files carry their geometry (cell_geometry with this rule)
+    o << "constexpr bool UNS = " << (g.unstructured ? "true" : "false")
+      << ";\nconstexpr int RG_POS = NV * DIM, RG_DET = RG_POS + NQR * 3, RG_WORDS = RG_DET + 1;\n";
     std::vector<int> corner;
     for (int p = 0; p < g.nclass; ++p)
       for (int a = 0; a < 8; ++a) corner.push_back(g.corner[p][a]);
@@ -1917,7 +1920,8 @@ std::string generate_reduce(const ModelSpec& m, const Layout& L, const std::vect
 extern "C" __global__ void __launch_bounds__(256)
 %KERNEL%(const int* __restrict__ cells, const unsigned char* __restrict__ cmask, const int* __restrict__ cslots,
          int ncell, const double* __restrict__ x, double t, int n0, int n1, double ox, double oy, double oz,
-         double hx, double hy, double hz, const double* __restrict__ prm, double* __restrict__ part) {
+         double hx, double hy, double hz, const double* __restrict__ prm, double* __restrict__ part,
+         const double* __restrict__ rgeo) {
   using namespace %C%;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const double h[3] = {hx, hy, hz};
@@ -1933,6 +1937,7 @@ extern "C" __global__ void __launch_bounds__(256)
     const int cell = cells[i];
     const int ci = cell % n0, cj = (cell / n0) % n1, ck = cell / (n0 * n1);
     const double cx = ox + ci * hx, cy = DIM > 1 ? oy + cj * hy : 0.0, cz = DIM > 2 ? oz + ck * hz : 0.0;
+    const double* eg = UNS ? rgeo + (size_t)i * RG_WORDS : nullptr;
     const unsigned m = cmask[i];
     for (int p = 0; p < NCLASS; ++p) {
       if (!((m >> p) & 1u)) continue;
@@ -1948,13 +1953,24 @@ extern "C" __global__ void __launch_bounds__(256)
           gr[b][0] = gr[b][1] = gr[b][2] = 0.0;
           for (int a = 0; a < NV; ++a) {
             u[b] += RPHI[q][a] * ul[a][b];
-            for (int k = 0; k < DIM; ++k) gr[b][k] += RG[p][q][a][k] / h[k] * ul[a][b];
+            for (int k = 0; k < DIM; ++k)
+              gr[b][k] += (UNS ? eg[a * DIM + k] : RG[p][q][a][k] / h[k]) * ul[a][b];
           }
         }
-        const double px = cx + RX[p][q][0] * hx, py = DIM > 1 ? cy + RX[p][q][1] * hy : 0.0,
-                     pz = DIM > 2 ? cz + RX[p][q][2] * hz : 0.0;
+        double px, py, pz, ifac;
+        if constexpr (UNS) {
+          px = eg[RG_POS + q * 3];
+          py = DIM > 1 ? eg[RG_POS + q * 3 + 1] : 0.0;
+          pz = DIM > 2 ? eg[RG_POS + q * 3 + 2] : 0.0;
+          ifac = RW[q] * eg[RG_DET];
+        } else {
+          px = cx + RX[p][q][0] * hx;
+          py = DIM > 1 ? cy + RX[p][q][1] * hy : 0.0;
+          pz = DIM > 2 ? cz + RX[p][q][2] * hz : 0.0;
+          ifac = RW[q] * hv;
+        }
         double e[NK];
-        evaluate(u, gr, px, py, pz, t, RW[q] * hv, hv, prm, e);
+        evaluate(u, gr, px, py, pz, t, ifac, hv, prm, e);
 #pragma unroll
         for (int k = 0; k < NK; ++k) {
           acc[k] = have[k] ? combine(k, e[k], acc[k], prm) : e[k];

@@ -47,7 +47,8 @@ GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& layout, b
 // Geometry words of unstructured cells, in the order of `cells`, as the
 // generated cell kernels read them: the gradients of the element's basis
 // functions [a][k], the points of the order-2 rule [q][3], |det J|.
-std::vector<double> cell_geometry(const Grid& g, const std::vector<int>& cells);
+// `rule`: the points to place (default: the order-2 rule of the assembly).
+std::vector<double> cell_geometry(const Grid& g, const std::vector<int>& cells, const Rule* rule = nullptr);
 // |det J| of one unstructured cell
 double cell_det(const Grid& g, long cell);
 

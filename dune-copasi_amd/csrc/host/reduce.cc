@@ -73,7 +73,6 @@ double reduce_call(const ReduceFunction& f, const std::vector<double>& x, const 
 }
 
 std::vector<ReduceKey> Model::reduce_keys(const std::string& reduce_ini) const {
-  if (L_.grid.unstructured) throw NotImplemented("transform/reduce on grids read from files");
   if (reduce_ini.empty()) return read_reduce(cfg_tree_, "model.reduce", spec_.ctx);
   return read_reduce(ParameterTree::from_ini(reduce_ini, ""), "", spec_.ctx);
 }
@@ -110,6 +109,7 @@ std::vector<ReduceResult> Model::reduce(const std::string& reduce_ini) {
   double t = time_;
   const double* x = u_.get();
   const double* prm = prm_.get();
+  std::vector<DeviceArray<double>> rgeo(L_.comps.size());
   for (size_t c = 0; c < L_.comps.size(); ++c) {
     if (kernels[c].empty() || first[c + 1] == first[c]) continue;
     hipFunction_t fn = mod->function(kernels[c]);
@@ -118,7 +118,14 @@ std::vector<ReduceResult> Model::reduce(const std::string& reduce_ini) {
     const int* cslots = comp_dev_[c].cell_slots.get();
     int ncell = static_cast<int>(L_.comps[c].cells.size());
     double* out = part.get() + static_cast<size_t>(first[c]) * nk * 2;
-    void* args[] = {&cells, &cmask, &cslots, &ncell, &x, &t, &n0, &n1, &ox, &oy, &oz, &hx, &hy, &hz, &prm, &out};
+    // cells of file grids: geometry words at the order-4 points
+    DeviceArray<double>& rg = rgeo[c];
+    if (g.unstructured) {
+      const Rule rule = reduce_rule(g.dim, false);
+      rg.upload(cell_geometry(g, L_.comps[c].cells, &rule), stream_);
+    }
+    const double* rgp = rg.get();
+    void* args[] = {&cells, &cmask, &cslots, &ncell, &x, &t, &n0, &n1, &ox, &oy, &oz, &hx, &hy, &hz, &prm, &out, &rgp};
     DCG_HIP(hipModuleLaunchKernel(fn, static_cast<unsigned>(first[c + 1] - first[c]), 1, 1, 256, 1, 1, 0, stream_,
                                   args, nullptr));
   }

@@ -88,3 +88,22 @@ def test_gmsh_vtk_cells(dca, meshes, tmp_path):
         assert np.all(arr["types"] == 10) and int(piece.get("NumberOfPoints")) == comp["n_nodes"]
         total += arr["types"].size
     assert total == 6 * 4 ** 3
+
+
+@pytest.mark.parametrize("name,model,mesh,dim", CASES, ids=[c[0] for c in CASES])
+def test_gmsh_reduce_matches_oracle(dca, meshes, name, model, mesh, dim):
+    from tests.test_gpu_reduce import _config
+    ini = _ini(meshes, model, mesh, dim)
+    gpu = dca.Model(ini, "", 0)
+    orc = OracleModel(ini)
+    u = gpu.initial_state() + 0.05 * np.random.default_rng(3).standard_normal(gpu.n_dof)
+    gpu.set_state(u, 0.4)
+    orc.set_state(u, 0.4)
+    cfg = _config([s[0] for s in gpu.species()], dim) + "[area]\nevaluation.expression = integration_factor\n"
+    got = gpu.reduce(cfg)
+    ref, _, rc = orc.reduce(cfg)
+    assert rc == 0 and list(got) == list(ref)
+    assert abs(got["area"] - 1.0) <= 1e-12  # the meshes fill the unit box
+    for k, v in ref.items():
+        tol = 1e-14 * abs(v) if k.startswith(("max", "min")) else 1e-12 * max(1.0, abs(v))
+        assert abs(got[k] - v) <= tol, (k, got[k], v)
