@@ -491,7 +491,8 @@ __device__ __forceinline__ void facet_residual(const double (&xi)[NVF][BI > 0 ? 
 extern "C" __global__ void __launch_bounds__(64)
 %KERNEL%(const int* __restrict__ fint, const double* __restrict__ fdbl, int first, int count,
          const double* __restrict__ x, double t, double ca, double* __restrict__ r, double* __restrict__ val,
-         int mode, const double* __restrict__ prm) {
+         int mode, const double* __restrict__ prm, const double* __restrict__ dir) {
+  // mode 4: r += J(x) dir over the facet blocks (the matrix-free apply)
   using namespace %F%;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= count) return;
@@ -558,6 +559,44 @@ extern "C" __global__ void __launch_bounds__(64)
 #pragma unroll
   for (int k = 0; k < NVF * NVF * BO * BO; ++k) Joo[k] = 0.0;
   facet_residual(xi, xo, fd, t, ca, ri, ro, Jii, Jio, Joi, Joo, prm);
+  if (mode & 4) {
+    double vi[NVF][BI > 0 ? BI : 1], vo[NVF][BO > 0 ? BO : 1];
+#pragma unroll
+    for (int f = 0; f < NVF; ++f) {
+#pragma unroll
+      for (int i = 0; i < BI; ++i) vi[f][i] = dir[(size_t)fi[f] * DCG_B + i];
+#pragma unroll
+      for (int i = 0; i < BO; ++i) vo[f][i] = dir[(size_t)fi[NVF + f] * DCG_B + i];
+    }
+#pragma unroll
+    for (int f = 0; f < NVF; ++f) {
+#pragma unroll
+      for (int i = 0; i < BI; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int g = 0; g < NVF; ++g) {
+#pragma unroll
+          for (int j = 0; j < BI; ++j) s += Jii[((f * NVF + g) * BI + i) * BI + j] * vi[g][j];
+#pragma unroll
+          for (int j = 0; j < BO; ++j) s += Jio[((f * NVF + g) * BI + i) * (BO > 0 ? BO : 1) + j] * vo[g][j];
+        }
+        r[(size_t)fi[f] * DCG_B + i] += s;
+      }
+#pragma unroll
+      for (int i = 0; i < BO; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int g = 0; g < NVF; ++g) {
+#pragma unroll
+          for (int j = 0; j < BI; ++j) s += Joi[((f * NVF + g) * (BO > 0 ? BO : 1) + i) * BI + j] * vi[g][j];
+#pragma unroll
+          for (int j = 0; j < BO; ++j)
+            s += Joo[((f * NVF + g) * (BO > 0 ? BO : 1) + i) * (BO > 0 ? BO : 1) + j] * vo[g][j];
+        }
+        r[(size_t)fi[NVF + f] * DCG_B + i] += s;
+      }
+    }
+  }
   if (want_j) {
     const int* pos = fi + 2 * NVF;
 #pragma unroll

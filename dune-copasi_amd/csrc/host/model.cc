@@ -146,7 +146,7 @@ void Model::build(const std::string& ini, const std::string& overrides) {
     if (cfg_.precision != Precision::FP64) throw NotImplemented("mixed precision with the matrix-free Jacobian");
     if (opt_.batch_member) throw NotImplemented("matrix-free batch members");
     if (!can_apply_jacobian())
-      throw NotImplemented("the matrix-free Jacobian covers structured compartments without transmission or outflow terms");
+      throw NotImplemented("the matrix-free Jacobian covers structured compartments with the tiled kernel");
   }
   cfg_.lin_rtol = c.get(p + "linear_solver.convergence_condition.relative_tolerance", 1e-4);
   auto lr = c.get_list(p + "linear_solver.convergence_condition.iteration_range");
@@ -174,7 +174,6 @@ void Model::build(const std::string& ini, const std::string& overrides) {
 }
 
 bool Model::can_apply_jacobian() const {
-  if (!gen_.facet_kernels.empty()) return false;
   for (size_t c = 0; c < L_.comps.size(); ++c)
     if (L_.comps[c].b && gen_.tile_kernel[c].empty()) return false;
   return true;
@@ -592,7 +591,8 @@ void Model::launch_assembly(const double* x, double t, double cm, double ca, con
     const int* fint = facet_dev_[k].fint.get();
     const double* fdbl = facet_dev_[k].fdbl.get();
     for (auto [first, count] : facet_dev_[k].color_ranges) {
-      void* args[] = {&fint, &fdbl, &first, &count, &x, &t, &ca, &r, &val, &mode, &prm};
+      const double* dir = nullptr;
+      void* args[] = {&fint, &fdbl, &first, &count, &x, &t, &ca, &r, &val, &mode, &prm, &dir};
       const unsigned block = 64, grid = static_cast<unsigned>((count + block - 1) / block);
       DCG_HIP(hipModuleLaunchKernel(facet_fn_[k], grid, 1, 1, block, 1, 1, 0, stream_, args, nullptr));
     }
@@ -606,7 +606,7 @@ void Model::launch_assembly(const double* x, double t, double cm, double ca, con
 // rows are the identity, as in the assembled Jacobian).
 void Model::launch_apply(const double* lin, double t, double cm, double ca, const double* v, double* y,
                          const int* stop) {
-  if (!can_apply_jacobian()) throw NotImplemented("Jacobian apply without the tiled kernel or with facet terms");
+  if (!can_apply_jacobian()) throw NotImplemented("Jacobian apply without the tiled kernel");
   const Grid& g = L_.grid;
   int n0 = static_cast<int>(g.n[0]), n1 = g.dim > 1 ? static_cast<int>(g.n[1]) : 1;
   int n2 = g.dim > 2 ? static_cast<int>(g.n[2]) : 1;
@@ -625,6 +625,19 @@ void Model::launch_apply(const double* lin, double t, double cm, double ca, cons
     void* args[] = {&tiles, &g2l, &cmask, &lin, &v, &t, &cm, &ca, &y, &val, &own_start, &own_mask,
                     &n0, &n1, &n2, &ox, &oy, &oz, &hx, &hy, &hz, &mode, &prm, &stop};
     DCG_HIP(hipModuleLaunchKernel(tile_app_fn_[c], comp_dev_[c].ntiles, 1, 1, 256, 1, 1, 0, stream_, args, nullptr));
+  }
+  // transmission and outflow blocks added on top, colour by colour
+  // (localAssembleJacobianSkeletonApply / BoundaryApply, local_operator.hh:1354-1452)
+  if (ca == 0.0) return;
+  for (size_t k = 0; k < facet_fn_.size(); ++k) {
+    const int* fint = facet_dev_[k].fint.get();
+    const double* fdbl = facet_dev_[k].fdbl.get();
+    for (auto [first, count] : facet_dev_[k].color_ranges) {
+      double* y_out = y;
+      void* fargs[] = {&fint, &fdbl, &first, &count, &lin, &t, &ca, &y_out, &val, &mode, &prm, &v};
+      const unsigned block = 64, grid = static_cast<unsigned>((count + block - 1) / block);
+      DCG_HIP(hipModuleLaunchKernel(facet_fn_[k], grid, 1, 1, block, 1, 1, 0, stream_, fargs, nullptr));
+    }
   }
 }
 

@@ -135,7 +135,7 @@ class Model {
   void spmv(const double* jac, const double* x_packed, double* y_packed);
   int linear_solve(const double* jac, const double* b_packed, double* x_packed, double rtol, int maxit, int* its);
   // y = J(x) v without assembling J (the matrix-free Jacobian apply), for
-  // models whose compartments all use the tiled kernel and have no facets
+  // models whose compartments all use the tiled kernel (facet terms included)
   void jacobian_apply(const double* x_packed, double t, double cm, double ca, const double* v_packed,
                       double* y_packed);
   bool can_apply_jacobian() const;

@@ -111,8 +111,10 @@ int dcg_assemble(dcg_model_t model, const double* x, double t, double mass_weigh
 /* y = J(x) v without assembling J: the matrix-free Jacobian apply
  * (localAssembleJacobianVolumeApply, local_operator.hh:510-524, through the
  * matrix-free instationary assembler of make_step_operator.hh:291-293), the
- * operator BiCGSTAB uses with linear_solver.matrix_free = true.  Models whose
- * compartments are structured and carry no transmission or outflow terms. */
+ * operator BiCGSTAB uses with linear_solver.matrix_free = true, transmission
+ * and outflow blocks included (local_operator.hh:1354-1452).  Models whose
+ * compartments all take the tiled kernel (structured grids, analytical
+ * Jacobians, no advection or gradient-dependent diffusion). */
 int dcg_jacobian_apply(dcg_model_t model, const double* x, double t, double mass_weight, double stiffness_weight,
                        const double* v, double* y);
 

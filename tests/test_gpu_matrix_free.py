@@ -51,12 +51,23 @@ def test_jacobian_apply_matches_oracle_matrix(dca, wl, name, make, cm, ca):
     assert np.abs(y - gpu.spmv(J, v)).max() <= 1e-12 * np.abs(ref).max()
 
 
-def test_jacobian_apply_rejects_transmission_terms(dca, wl):
-    w = wl.two_compartments(n=12, cell_type="cube")
+@pytest.mark.parametrize("make", [lambda wl: wl.two_compartments(n=12, cell_type="cube"),
+                                  lambda wl: wl.two_compartments(n=10, cell_type="simplex"),
+                                  lambda wl: wl.random_request(4, side3=(5, 6), steps_range=(3, 3))],
+                         ids=["two_q1", "two_p1", "req_3d_8sp_2c"])
+def test_jacobian_apply_with_transmission_terms(dca, wl, make):
+    """Interface blocks (both sides, both directions) are applied by the facet
+    kernels on top of the tiles' volume blocks."""
+    w = make(wl)
     gpu = dca.Model(w.ini, "", 0)
-    x = w.initial_state(gpu)
-    with pytest.raises(dca.DcgError, match="facet"):
-        gpu.jacobian_apply(x, 0.0, 1.0, 0.1, x)
+    orc = OracleModel(w.ini)
+    x = _perturbed_state(w, gpu)
+    v = np.random.default_rng(9).standard_normal(gpu.n_dof)
+    for cm, ca in [(0.0, 0.7), (1.0, 0.3)]:
+        y = gpu.jacobian_apply(x, 0.2, cm, ca, v)
+        _, vals = orc.assemble(x, 0.2, cm, ca)
+        ref = orc.csr(vals) @ v
+        assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
 STEP_CASES = [
@@ -65,6 +76,8 @@ STEP_CASES = [
                                               lin_rtol=1e-12), 4),
     ("req_2d_6sp_1c", lambda wl: wl.random_request(11, side2=(10, 12), nl_rtol=1e-20, nl_atol=1e-28,
                                                    precision="fp64"), 3),
+    ("two_q1", lambda wl: wl.two_compartments(n=16, cell_type="cube", nl_rtol=1e-20, nl_atol=1e-26,
+                                              lin_rtol=1e-12), 3),
 ]
 
 

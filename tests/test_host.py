@@ -153,10 +153,9 @@ initial.expression = {text}
      "[model.time_step_operator]\nlinear_solver.matrix_free = true\nlinear_solver.precision = mixed\n",
      "mixed precision with the matrix-free Jacobian"),
     ("[grid]\ndimension = 2\ncells = 4 4\ncell_type = cube\n[compartments]\nd.expression = 1\n"
-     "[model.scalar_field.u]\ncompartment = d\nstorage.expression = 1\noutflow.d.expression = -u\n"
-     "outflow.d.jacobian.u.expression = -1\n"
+     "[model.scalar_field.u]\ncompartment = d\nstorage.expression = 1\nvelocity.x.expression = 0.1\n"
      "[model.time_step_operator]\nlinear_solver.matrix_free = true\n",
-     "without transmission or outflow terms"),
+     "with the tiled kernel"),
 ])
 def test_errors_are_reported(dca, ini, msg):
     with pytest.raises(dca.DcgError, match=re.escape(msg)):
