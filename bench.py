@@ -60,7 +60,7 @@ def make_workload(wl, name, precision, precond="Jacobi"):
 
 
 TRAFFIC_FILES = {("ms96", "fp64"): "traffic_mitchell_schaefer_q1_96_fp64.txt"}
-VARIANT_KERNELS = {"lds_blocks": "spmv_blk_k", "lds_entries": "spmv_k", "rows_g8": "spmv_rows_k",
+VARIANT_KERNELS = {"points": "dcg_tile_0_p", "lds_blocks": "spmv_blk_k", "lds_entries": "spmv_k", "rows_g8": "spmv_rows_k",
                    "rows_g16": "spmv_rows_k", "rows_g32": "spmv_rows_k", "rows_g64": "spmv_rows_k"}
 
 
@@ -443,6 +443,9 @@ def main():
     ap.add_argument("--workload", default="ms96")
     ap.add_argument("--preconditioner", default="Jacobi", choices=["Jacobi", "VertexBlockJacobi"],
                     help="Jacobi: the reference's (diagonal); VertexBlockJacobi: inverse of each vertex's species block")
+    ap.add_argument("--storage", default="matrix", choices=["matrix", "points", "auto"],
+                    help="single models: the Jacobian as an assembled BCSR matrix, or as point coefficients per "
+                         "cell applied by the Krylov operator (linear_solver.storage)")
     ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
                     help="default: fp64 for single models, mixed for gs_batch and serve")
     ap.add_argument("--cpu-steps", type=int, default=1)
@@ -486,8 +489,9 @@ def main():
         return run_serve(args, dca, wl)
     w, wname = make_workload(wl, args.workload, args.precision, args.preconditioner)
 
+    storage = f"model.time_step_operator.linear_solver.storage = {args.storage}\n"
     t_setup = time.perf_counter()
-    m = dca.Model(w.ini, "", 0)
+    m = dca.Model(w.ini, storage, 0)
     u0 = w.initial_state(m)
     m.set_state(u0, 0.0)
     setup_s = time.perf_counter() - t_setup
@@ -521,17 +525,22 @@ def main():
     print(f"# timed {args.steps} steps in {elapsed:.3f}s", file=sys.stderr, flush=True)
 
     kinds = {}
-    for kname in ["spmv64", "spmv32", "assembly_jacobian", "assembly_residual", "precond_setup"]:
+    for kname in ["spmv64", "spmv32", "assembly_jacobian", "assembly_residual", "precond_setup", "point_setup",
+                  "point_apply64", "point_apply32"]:
         n, ms, b = m.profile(kname)
         if n:
             kinds[kname] = {"launches": n, "avg_us": 1e3 * ms / n, "gbs": (b / n) / (ms / n * 1e-3) / 1e9 if ms else 0.0,
                             "bytes_per_launch": b / n, "share_of_step_time": ms / (1e3 * elapsed)}
-    spmv_kind = "spmv32" if args.precision == "mixed" and "spmv32" in kinds else "spmv64"
+    # the dominant kernel: the Krylov operator (assembled SpMV or point-coefficient apply)
+    apply_kinds = [k for k in ("spmv64", "spmv32", "point_apply64", "point_apply32") if k in kinds]
+    spmv_kind = max(apply_kinds, key=lambda k: kinds[k]["share_of_step_time"]) if apply_kinds else None
     sp = kinds.get(spmv_kind)
     roofline = None
     if sp:
+        variant = "points" if spmv_kind.startswith("point") else chosen
         roofline = {"bound": "hbm", "kernel": spmv_kind, "achieved": sp["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": sp["gbs"] / HBM_PEAK_GBS, "traffic": committed_traffic(args.workload, args.precision, chosen),
+                    "frac": sp["gbs"] / HBM_PEAK_GBS,
+                    "traffic": committed_traffic(args.workload, args.precision, variant),
                     "avg_launch_us": sp["avg_us"], "algorithmic_bytes_per_launch": sp["bytes_per_launch"]}
 
     probe = None
@@ -547,7 +556,7 @@ def main():
         # with its error against this FP64 run
         u64, _ = m.get_state()
         w32, _ = make_workload(wl, args.workload, "mixed", args.preconditioner)
-        m32 = dca.Model(w32.ini, "", 0)
+        m32 = dca.Model(w32.ini, storage, 0)
         m32.set_state(u0, 0.0)
         t32 = 0.0
         for k in range(args.warmup):
@@ -597,7 +606,8 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32-krylov/f64-refinement",
         "data": "synthetic (seeded initial state)",
         "config": {"workload": wname, "n_dof": m.n_dof, "stages_per_step": w.stages,
-                   "precision": args.precision, "preconditioner": args.preconditioner},
+                   "precision": args.precision, "preconditioner": args.preconditioner,
+                   "jacobian_storage": args.storage},
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
         "newton_iterations_per_step": newton / args.steps,
         "krylov_iterations_per_step": lin / args.steps,

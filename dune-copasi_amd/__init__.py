@@ -131,7 +131,7 @@ _SIGS = {
 RUNTIME_PARAMETERS = 1
 
 PROFILE_KINDS = {"spmv64": 0, "spmv32": 1, "assembly_jacobian": 2, "assembly_residual": 3, "precond_setup": 4,
-                 "jacobian_apply": 5}
+                 "jacobian_apply": 5, "point_setup": 6, "point_apply64": 7, "point_apply32": 8}
 
 
 def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:

@@ -1152,9 +1152,9 @@ __device__ __forceinline__ void block_rows(int dx, int dy, int dz, int li, int l
 }
 
 // one tile of vertices: the residual rows and/or the BCSR block rows it owns.
-// MODE fixes the work at compile time (1 residual, 2 Jacobian, 4 apply), so
-// each entry point gets registers and LDS for its own path only; MODE 0
-// takes it from mode_rt (the batched entry point).
+// MODE fixes the work at compile time (1 residual, 2 Jacobian, 4 apply, 8
+// point coefficients), so each entry point gets registers and LDS for its own
+// path only; MODE 0 takes it from mode_rt (the batched entry point).
 template <int MODE>
 __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,
          const double* __restrict__ x, const double* __restrict__ base, double t, double cm, double ca,
@@ -1165,7 +1165,7 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
   __shared__ double xs[NVB * BC];
   __shared__ int sl[NVB];
   __shared__ unsigned char cms[NCB];
-  __shared__ double work[MODE == 1 ? RL_WORDS : (MODE == 2 || MODE == 4) ? JW_WORDS : UNION_WORDS];
+  __shared__ double work[MODE == 1 ? RL_WORDS : (MODE == 2 || MODE == 4 || MODE == 8) ? JW_WORDS : UNION_WORDS];
   const int ntx = (n0 + 1 + TX - 1) / TX;
   const int nty = DIM > 1 ? (n1 + 1 + TY - 1) / TY : 1;
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
@@ -1190,9 +1190,11 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-  if (mode & 6) {
+  if (mode & 14) {
     // mode 2 writes the BCSR block rows; mode 4 applies them instead (the
-    // matrix-free Jacobian: r = J(x) v with the direction v passed as base)
+    // matrix-free Jacobian: r = J(x) v with the direction v passed as base);
+    // mode 8 stores the point coefficients themselves (val) and the rows'
+    // diagonal blocks (r)
     double dv[NDV > 0 ? NDV : 1];
     uniform_diffusion(t, prm, dv);
     // phase 1: Jacobian coefficients per (cell, class, point); class and point
@@ -1234,6 +1236,46 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
       }
     }
     __syncthreads();
+    if constexpr (MODE == 8) {
+      // the cells this tile owns (their lowest corner is one of its rows, so
+      // every cell has exactly one owner) write their coefficients to
+      // val[((p*NQ + q)*NMC + n) * ncell + cell], cells in grid order
+      const long ncg = (long)n0 * n1 * n2;
+      for (int cl = threadIdx.x; cl < NCB; cl += blockDim.x) {
+        const unsigned m = cms[cl];
+        if (!m) continue;
+        const int ci = cl % CXB, cj = (cl / CXB) % CYB, ck = cl / (CXB * CYB);
+        if (ci < 1 || (DIM > 1 && cj < 1) || (DIM > 2 && ck < 1)) continue;
+        const long gc = (vi0 + ci) + (long)n0 * ((vj0 + cj) + (long)n1 * (vk0 + ck));
+        for (int p = 0; p < NCLASS; ++p) {
+          if (!((m >> p) & 1u)) continue;
+          for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int n = 0; n < NMC; ++n)
+              val[(size_t)((p * NQ + q) * NMC + n) * ncg + gc] = work[(size_t)((p * NQ + q) * NMC + n) * NCB + cl];
+        }
+      }
+      // diagonal block of every row (the Jacobi preconditioner's input):
+      // r[slot * B * B + i * B + j], the identity on padding species
+      constexpr int AI = BC > 4 ? 1 : BC, NR = BC / AI;
+      for (int it = threadIdx.x; it < NROW * NR; it += blockDim.x) {
+        const int lr = it / NR, i0 = (it % NR) * AI;
+        const int li = lr % TX, lj = DIM > 1 ? (lr / TX) % TY : 0, lk = DIM > 2 ? lr / (TX * TY) : 0;
+        const int s = sl[(li + 1) + XB * ((lj + OY) + YB * (lk + OZ))];
+        if (s < 0) continue;
+        double acc[AI][BC];
+        block_rows<AI>(0, 0, 0, li, lj, lk, i0, cms, work, dv, ca, acc);
+        double* blk = r + (size_t)s * (DCG_B * DCG_B);
+#pragma unroll
+        for (int i = 0; i < AI; ++i)
+#pragma unroll
+          for (int j = 0; j < DCG_B; ++j) blk[(i0 + i) * DCG_B + j] = j < BC ? acc[i][j < BC ? j : 0] : 0.0;
+        if (i0 + AI == BC)
+          for (int i = BC; i < DCG_B; ++i)
+            for (int j = 0; j < DCG_B; ++j) blk[i * DCG_B + j] = i == j ? 1.0 : 0.0;
+      }
+      return;
+    }
     if (mode & 4) {
       // the linearization point is no longer needed: stage the direction
       for (int b = threadIdx.x; b < NVB; b += blockDim.x) {
@@ -1578,6 +1620,160 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
     }
   }
 }
+
+// y = J v from stored point coefficients (mode 8 of tile_body): per cell of
+// the tile's box, v at its corners is interpolated to each point, multiplied
+// by the point's coefficient block and tested against the basis functions,
+// the uniform diffusion adds the element stiffness times v; the cells' corner
+// sums are folded onto the tile's rows as in the residual.  T is the value
+// type of pc, v and y (double, or float inside the refinement loop).
+template <typename T>
+__device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,
+                                         const T* __restrict__ pc, const T* __restrict__ v, T* __restrict__ y, double t,
+                                         double ca, int n0, int n1, int n2, const double* __restrict__ prm) {
+  __shared__ T vs[NVB * BC];
+  __shared__ int sl[NVB];
+  __shared__ unsigned char cms[NCB];
+  __shared__ T out[NCB * NCORNER * BC];
+  const int ntx = (n0 + 1 + TX - 1) / TX;
+  const int nty = DIM > 1 ? (n1 + 1 + TY - 1) / TY : 1;
+  const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+  const int vi0 = tx * TX - 1, vj0 = DIM > 1 ? ty * TY - 1 : 0, vk0 = DIM > 2 ? tz * TZ - 1 : 0;
+  for (int b = threadIdx.x; b < NVB; b += blockDim.x) {
+    const int bi = b % XB, bj = (b / XB) % YB, bk = b / (XB * YB);
+    const int gi = vi0 + bi, gj = vj0 + bj, gk = vk0 + bk;
+    int s = -1;
+    if (gi >= 0 && gi <= n0 && gj >= 0 && gj <= (DIM > 1 ? n1 : 0) && gk >= 0 && gk <= (DIM > 2 ? n2 : 0))
+      s = g2l[gi + (n0 + 1) * (gj + (n1 + 1) * gk)];
+    sl[b] = s;
+    for (int i = 0; i < BC; ++i) vs[b * BC + i] = s >= 0 ? v[(size_t)s * DCG_B + i] : T(0);
+  }
+  for (int c = threadIdx.x; c < NCB; c += blockDim.x) {
+    const int ci = c % CXB, cj = (c / CXB) % CYB, ck = c / (CXB * CYB);
+    const int gi = vi0 + ci, gj = vj0 + cj, gk = vk0 + ck;
+    unsigned char m = 0;
+    if (gi >= 0 && gi < n0 && gj >= 0 && gj < (DIM > 1 ? n1 : 1) && gk >= 0 && gk < (DIM > 2 ? n2 : 1))
+      m = cmask[gi + n0 * (gj + n1 * gk)];
+    cms[c] = m;
+  }
+  __syncthreads();
+  double dv[NDV > 0 ? NDV : 1];
+  uniform_diffusion(t, prm, dv);
+  const long ncg = (long)n0 * n1 * n2;
+  // corner values stay in registers for small blocks, in LDS otherwise
+  constexpr bool VREG = BC <= 4;
+  for (int cl = threadIdx.x; cl < NCB; cl += blockDim.x) {
+    T* dst = out + (size_t)cl * (NCORNER * BC);
+    for (int k = 0; k < NCORNER * BC; ++k) dst[k] = T(0);
+    const unsigned m = cms[cl];
+    if (!m) continue;
+    const int ci = cl % CXB, cj = (cl / CXB) % CYB, ck = cl / (CXB * CYB);
+    const long gc = (vi0 + ci) + (long)n0 * ((vj0 + cj) + (long)n1 * (vk0 + ck));
+    for (int p = 0; p < NCLASS; ++p) {
+      if (!((m >> p) & 1u)) continue;
+      int bidx[NV];
+      T vc[VREG ? NV : 1][BC];
+#pragma unroll
+      for (int a = 0; a < NV; ++a) {
+        const int c = CORNER[p][a];
+        bidx[a] = (ci + (c & 1)) + XB * ((cj + ((c >> 1) & 1)) + YB * (ck + ((c >> 2) & 1)));
+        if constexpr (VREG)
+#pragma unroll
+          for (int i = 0; i < BC; ++i) vc[a][i] = vs[bidx[a] * BC + i];
+      }
+      auto vcor = [&](int a, int i) -> T { return VREG ? vc[VREG ? a : 0][i] : vs[bidx[a] * BC + i]; };
+      // the point terms of one point into the corner sums: acc(a, i) += PHI[q][a] w[i]
+      auto point_terms = [&](int q, auto&& add) {
+        T vq[BC];
+#pragma unroll
+        for (int j = 0; j < BC; ++j) {
+          T sum = T(0);
+#pragma unroll
+          for (int a = 0; a < NV; ++a) sum += T(PHI[q][a]) * vcor(a, j);
+          vq[j] = sum;
+        }
+        const T* mp = pc + (size_t)((p * NQ + q) * NMC) * ncg + gc;
+        T w[BC];
+#pragma unroll
+        for (int i = 0; i < BC; ++i) w[i] = T(0);
+#pragma unroll
+        for (int n = 0; n < NMC; ++n) w[MCI[n]] += mp[(size_t)n * ncg] * vq[MCJ[n]];
+#pragma unroll
+        for (int a = 0; a < NV; ++a)
+#pragma unroll
+          for (int i = 0; i < BC; ++i) add(a, i, T(PHI[q][a]) * w[i]);
+      };
+      // the uniform diffusion: acc(a, i) += ca sum_c K[a][c] v_c for the element
+      // stiffness K (the table entries are immediates; ca scales the sum)
+      auto diffusion_terms = [&](auto&& add) {
+#pragma unroll
+        for (int a = 0; a < NV; ++a) {
+          if constexpr (CONST_DIFFUSION) {
+#pragma unroll
+            for (int n = 0; n < NKZ; ++n) {
+              T sum = T(0);
+#pragma unroll
+              for (int c = 0; c < NV; ++c) sum += T(kd_rt(p, a, c, KZI[n], KZJ[n])) * vcor(c, KZJ[n]);
+              add(a, KZI[n], T(ca) * sum);
+            }
+          } else {
+#pragma unroll
+            for (int n = 0; n < NDV; ++n) {
+              T sum = T(0);
+#pragma unroll
+              for (int c = 0; c < NV; ++c) sum += T(KDU[p][a][c][DVK[n]][DVL[n]]) * vcor(c, DVJ[n]);
+              add(a, DVI[n], T(ca * dv[n]) * sum);
+            }
+          }
+        }
+      };
+      if constexpr (VREG) {
+        // small blocks: the corner sums in registers
+        T o[NV][BC];
+#pragma unroll
+        for (int a = 0; a < NV; ++a)
+#pragma unroll
+          for (int i = 0; i < BC; ++i) o[a][i] = T(0);
+        auto add = [&](int a, int i, T v2) { o[a][i] += v2; };
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) point_terms(q, add);
+        diffusion_terms(add);
+#pragma unroll
+        for (int a = 0; a < NV; ++a)
+#pragma unroll
+          for (int i = 0; i < BC; ++i) dst[CORNER[p][a] * BC + i] += o[a][i];
+      } else {
+        // large blocks: the corner sums in this cell's LDS row, point by point
+        auto add = [&](int a, int i, T v2) { dst[CORNER[p][a] * BC + i] += v2; };
+#pragma unroll 1
+        for (int q = 0; q < NQ; ++q) point_terms(q, add);
+        diffusion_terms(add);
+      }
+    }
+  }
+  __syncthreads();
+  for (int it = threadIdx.x; it < NROW * DCG_B; it += blockDim.x) {
+    const int lr = it / DCG_B, i = it % DCG_B;
+    const int li = lr % TX, lj = DIM > 1 ? (lr / TX) % TY : 0, lk = DIM > 2 ? lr / (TX * TY) : 0;
+    const int vb = (li + 1) + XB * ((lj + OY) + YB * (lk + OZ));
+    const int s = sl[vb];
+    if (s < 0) continue;
+    T acc = T(0);
+    if (i < BC) {
+      for (int ez = 0; ez < (DIM > 2 ? 2 : 1); ++ez)
+        for (int ey = 0; ey < (DIM > 1 ? 2 : 1); ++ey)
+          for (int ex = 0; ex < 2; ++ex) {
+            const int cl = (li + ex) + CXB * ((lj + ey) + CYB * (lk + ez));
+            if (!cms[cl]) continue;
+            const int cv = (1 - ex) | (DIM > 1 ? (1 - ey) << 1 : 0) | (DIM > 2 ? (1 - ez) << 2 : 0);
+            acc += out[(size_t)cl * (NCORNER * BC) + cv * BC + i];
+          }
+    } else {
+      acc = v[(size_t)s * DCG_B + i];  // padding species: identity rows
+    }
+    y[(size_t)s * DCG_B + i] = acc;
+  }
+}
 }  // namespace %C%
 
 // one entry point per kind of work, same arguments: _r residual (r = base +
@@ -1598,6 +1794,33 @@ DCG_TILE_ENTRY(%KERNEL%_r, 1)
 DCG_TILE_ENTRY(%KERNEL%_j, 2)
 DCG_TILE_ENTRY(%KERNEL%_a, 4)
 #undef DCG_TILE_ENTRY
+)";
+
+// Entry points of the point-coefficient Jacobian (emitted on request): _q
+// stores the point coefficients at x into val (per cell and point) and the
+// rows' diagonal blocks into r; _p / _pf apply them, y = J v, in FP64 / FP32.
+const char* kTiledPoints = R"(
+extern "C" __global__ void __launch_bounds__(256)
+%KERNEL%_q(const int* __restrict__ tiles, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,
+           const double* __restrict__ x, const double* __restrict__ base, double t, double cm, double ca,
+           double* __restrict__ r, double* __restrict__ val, const int* __restrict__ own_start,
+           const unsigned* __restrict__ own_mask, int n0, int n1, int n2, double ox, double oy, double oz, double hx,
+           double hy, double hz, int mode, const double* __restrict__ prm, const int* __restrict__ stop) {
+  (void)stop;
+  %C%::tile_body<8>(tiles[blockIdx.x], g2l, cmask, x, base, t, cm, ca, r, val, own_start, own_mask, n0, n1, n2, ox,
+                    oy, oz, hx, hy, hz, mode, prm);
+}
+#define DCG_POINT_ENTRY(NAME, T)                                                                                  \
+  extern "C" __global__ void __launch_bounds__(256)                                                               \
+  NAME(const int* __restrict__ tiles, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,         \
+       const T* __restrict__ pc, const T* __restrict__ v, T* __restrict__ y, double t, double ca, int n0, int n1,  \
+       int n2, const double* __restrict__ prm, const int* __restrict__ stop) {                                     \
+    if (stop && *stop) return;                                                                                    \
+    %C%::pc_apply<T>(tiles[blockIdx.x], g2l, cmask, pc, v, y, t, ca, n0, n1, n2, prm);                            \
+  }
+DCG_POINT_ENTRY(%KERNEL%_p, double)
+DCG_POINT_ENTRY(%KERNEL%_pf, float)
+#undef DCG_POINT_ENTRY
 )";
 
 // Batched form: one launch over (member, tile) pairs of many models that share
@@ -1653,7 +1876,7 @@ std::array<int, 3> choose_tile(const Grid& g, int bc, int nmc) {
 }
 
 std::string gen_tiled(const ModelSpec& m, const Grid& g, int comp, const std::string& ns, const std::string& kname,
-                      std::array<int, 3> tile, bool geo) {
+                      std::array<int, 3> tile, bool geo, bool points) {
   const auto T = element_tables(geo ? unit_cell(g) : g);
   std::ostringstream o;
   o << "namespace " << ns << " {\n";
@@ -1772,6 +1995,7 @@ std::string gen_tiled(const ModelSpec& m, const Grid& g, int comp, const std::st
   o << "}\n";
   o << "}  // namespace " << ns << "\n";
   std::string body = kTiled;
+  if (points) body += kTiledPoints;
   if (geo && comp < 4) body += kTiledBatch;
   replace_all(body, "%C%", ns);
   replace_all(body, "%KERNEL%", kname);
@@ -1782,7 +2006,7 @@ std::string gen_tiled(const ModelSpec& m, const Grid& g, int comp, const std::st
 namespace {
 }  // namespace
 
-GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L, bool runtime_geometry) {
+GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L, bool runtime_geometry, bool points) {
   GeneratedSource out;
   const bool geo = runtime_geometry;
   std::string pre = kPrelude;
@@ -1798,6 +2022,7 @@ GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L, bool r
       out.tile_kernel.push_back("");
       out.tile_batch_kernel.push_back("");
       out.tile_dims.push_back({0, 0, 0});
+      out.point_words.push_back(0);
       continue;
     }
     std::string ns = "vol" + std::to_string(c);
@@ -1809,14 +2034,19 @@ GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L, bool r
       auto tile = choose_tile(L.grid, static_cast<int>(spec.comp_species[c].size()),
                               static_cast<int>(mc_pattern(spec, static_cast<int>(c)).size()));
       std::string tk = "dcg_tile_" + std::to_string(c);
-      src << gen_tiled(spec, L.grid, static_cast<int>(c), ns, tk, tile, geo);
+      src << gen_tiled(spec, L.grid, static_cast<int>(c), ns, tk, tile, geo, points);
       out.tile_kernel.push_back(tk);
       out.tile_batch_kernel.push_back(geo && c < 4 ? tk + "_batch" : "");
       out.tile_dims.push_back(tile);
+      const int nq = static_cast<int>(element_rule(L.grid.dim, L.grid.cube).w.size());
+      out.point_words.push_back(points ? L.grid.nclass * nq *
+                                             static_cast<int>(mc_pattern(spec, static_cast<int>(c)).size())
+                                       : 0);
     } else {
       out.tile_kernel.push_back("");
       out.tile_batch_kernel.push_back("");
       out.tile_dims.push_back({0, 0, 0});
+      out.point_words.push_back(0);
     }
   }
   // facet kernels for every (in, out, boundary) combination present

@@ -32,6 +32,9 @@ struct GeneratedSource {
   // batched tiled kernel per compartment (runtime geometry only, else "")
   std::vector<std::string> tile_batch_kernel;
   std::vector<std::array<int, 3>> tile_dims;
+  // doubles per grid cell of the point-coefficient Jacobian (classes x
+  // points x coefficients; 0 when the compartment has no point kernels)
+  std::vector<int> point_words;
   std::vector<FacetKernel> facet_kernels;
 };
 
@@ -42,7 +45,14 @@ constexpr int kGeoWords = 8;
 
 // runtime_geometry: element tables of the unit cell, spacing from prm[0..7],
 // so models that differ only in mesh size and parameters share one source.
-GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& layout, bool runtime_geometry = false);
+// the compartment's equations suit the tiled kernel (analytic Jacobian, no
+// velocity, diffusion uniform in space and independent of the state)
+bool tile_eligible(const ModelSpec& m, int comp);
+
+// points: also emit the point-coefficient Jacobian entry points (_q, _p, _pf)
+// of every tiled compartment
+GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& layout, bool runtime_geometry = false,
+                                  bool points = false);
 
 // Geometry words of unstructured cells, in the order of `cells`, as the
 // generated cell kernels read them: the gradients of the element's basis

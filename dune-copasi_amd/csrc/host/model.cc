@@ -92,7 +92,6 @@ void Model::build(const std::string& ini, const std::string& overrides) {
     comps[i].b = static_cast<int>(comps[i].species.size());
   }
   L_ = make_layout(g, elem_comp, std::move(comps), spec_.has_outflow_boundary, spec_.has_outflow_interface);
-  gen_ = generate_assembly(spec_, L_, opt_.runtime_parameters);
 
   // solver configuration, keys of model/make_step_operator.hh:183-288
   const std::string p = "model.time_step_operator.";
@@ -145,8 +144,29 @@ void Model::build(const std::string& ini, const std::string& overrides) {
     cfg_.precond = Preconditioner::None;
     if (cfg_.precision != Precision::FP64) throw NotImplemented("mixed precision with the matrix-free Jacobian");
     if (opt_.batch_member) throw NotImplemented("matrix-free batch members");
-    if (!can_apply_jacobian())
+    if (!all_tile_eligible())
       throw NotImplemented("the matrix-free Jacobian covers structured compartments with the tiled kernel");
+  }
+  // linear_solver.storage: how the Krylov solver holds the Jacobian.
+  // "matrix" assembles the BCSR matrix (as the reference does); "points"
+  // keeps only the coefficient blocks of every cell's quadrature points and
+  // applies them (with the uniform diffusion) to each Krylov direction, which
+  // is the same operator in fewer bytes; "auto" takes points on box grids
+  // where the model allows it.  Points need no interface or outflow terms
+  // and a preconditioner built from diagonal blocks (Jacobi, vertex-block
+  // Jacobi or none).
+  {
+    const std::string st = c.get(p + "linear_solver.storage", std::string("matrix"));
+    if (st != "matrix" && st != "points" && st != "auto")
+      throw IOError("linear_solver.storage must be 'matrix', 'points' or 'auto'");
+    std::string why;
+    if (cfg_.matrix_free) why = "the matrix-free Jacobian";
+    else if (opt_.batch_member) why = "batch members";
+    else if (cfg_.precond == Preconditioner::ILU0) why = "an ILU0 preconditioner";
+    else if (!L_.facets.empty()) why = "interface or outflow terms";
+    else if (!all_tile_eligible()) why = "compartments outside the tiled kernel";
+    if (st == "points" && !why.empty()) throw NotImplemented("linear_solver.storage = points with " + why);
+    cfg_.points = why.empty() && (st == "points" || (st == "auto" && L_.grid.cube));
   }
   cfg_.lin_rtol = c.get(p + "linear_solver.convergence_condition.relative_tolerance", 1e-4);
   auto lr = c.get_list(p + "linear_solver.convergence_condition.iteration_range");
@@ -171,6 +191,14 @@ void Model::build(const std::string& ini, const std::string& overrides) {
   time_ = c.get(p + "time_begin", 0.0);
   t_end_ = c.get(p + "time_end", 1.0);
   dt_initial_ = c.get(p + "time_step_initial", c.get(p + "time_step_max", 0.1));
+  gen_ = generate_assembly(spec_, L_, opt_.runtime_parameters, cfg_.points);
+}
+
+bool Model::all_tile_eligible() const {
+  if (L_.grid.unstructured) return false;
+  for (size_t c = 0; c < L_.comps.size(); ++c)
+    if (L_.comps[c].b && !tile_eligible(spec_, static_cast<int>(c))) return false;
+  return true;
 }
 
 bool Model::can_apply_jacobian() const {
@@ -377,15 +405,29 @@ void Model::setup_device() {
   packed_.resize(std::max<size_t>(1, static_cast<size_t>(L_.ndof)));
   stage_.resize(rk_.stages + 1);
   for (auto& s : stage_) s.resize(nv);
+  use_points_ = cfg_.points;
   if (cfg_.matrix_free) {
     lin_x_.resize(nv);
+  } else if (cfg_.points) {
+    pc_.resize(L_.comps.size());
+    if (cfg_.precision == Precision::Mixed) pc32_.resize(L_.comps.size());
+    for (size_t c = 0; c < L_.comps.size(); ++c) {
+      const size_t words = static_cast<size_t>(gen_.point_words[c]) * static_cast<size_t>(L_.grid.ncell);
+      pc_[c].resize(words);
+      if (cfg_.precision == Precision::Mixed) pc32_[c].resize(words);
+    }
+    pdiag_.resize(static_cast<size_t>(ns) * B * B);
+    dinv_.resize(static_cast<size_t>(ns) * B * B);
+    std::vector<int> id(ns);
+    for (long i = 0; i < ns; ++i) id[i] = static_cast<int>(i);
+    diag_id_.upload(id, stream_);
   } else {
     val_.resize(static_cast<size_t>(matrix_values()));
     dinv_.resize(static_cast<size_t>(ns) * B * B);
   }
   for (auto& v : kv_) v.resize(nv);
   if (cfg_.precision == Precision::Mixed) {
-    val32_.resize(static_cast<size_t>(matrix_values()));
+    if (!cfg_.points) val32_.resize(static_cast<size_t>(matrix_values()));
     for (auto& v : kf_) v.resize(nv);
   }
   const size_t nparts = std::max<size_t>(std::max<size_t>(static_cast<size_t>(plan_.nchunk), plan_.row_wgs) + 8,
@@ -442,7 +484,7 @@ void Model::setup_device() {
     prm.insert(prm.end(), spec_.param_values.begin(), spec_.param_values.end());
     prm_.upload(prm, stream_);
   }
-  if (!opt_.batch_member && !cfg_.matrix_free) tune_spmv();
+  if (!opt_.batch_member && !cfg_.matrix_free && !cfg_.points) tune_spmv();
   synchronize();
 }
 
@@ -508,12 +550,20 @@ void Model::compile_and_load() {
   tile_fn_.assign(L_.comps.size(), nullptr);
   tile_jac_fn_.assign(L_.comps.size(), nullptr);
   tile_app_fn_.assign(L_.comps.size(), nullptr);
+  pc_set_fn_.assign(L_.comps.size(), nullptr);
+  pc_app_fn_.assign(L_.comps.size(), nullptr);
+  pc_appf_fn_.assign(L_.comps.size(), nullptr);
   for (size_t c = 0; c < L_.comps.size(); ++c) {
     if (!gen_.volume_kernel[c].empty()) vol_fn_[c] = module_->function(gen_.volume_kernel[c]);
     if (!gen_.tile_kernel[c].empty()) {
       tile_fn_[c] = module_->function(gen_.tile_kernel[c] + "_r");
       tile_jac_fn_[c] = module_->function(gen_.tile_kernel[c] + "_j");
       tile_app_fn_[c] = module_->function(gen_.tile_kernel[c] + "_a");
+      if (cfg_.points) {
+        pc_set_fn_[c] = module_->function(gen_.tile_kernel[c] + "_q");
+        pc_app_fn_[c] = module_->function(gen_.tile_kernel[c] + "_p");
+        pc_appf_fn_[c] = module_->function(gen_.tile_kernel[c] + "_pf");
+      }
     }
   }
   facet_fn_.clear();
@@ -641,6 +691,66 @@ void Model::launch_apply(const double* lin, double t, double cm, double ca, cons
   }
 }
 
+// Point coefficients at x (the cells' quadrature-point blocks of the mass and
+// reaction Jacobian, weights cm and ca folded in) and the rows' diagonal
+// blocks, compartment by compartment; the uniform diffusion is applied from
+// its tables, so t and ca are kept for the applies.
+void Model::launch_points_setup(const double* x, double t, double cm, double ca) {
+  const Grid& g = L_.grid;
+  int n0 = static_cast<int>(g.n[0]), n1 = g.dim > 1 ? static_cast<int>(g.n[1]) : 1;
+  int n2 = g.dim > 2 ? static_cast<int>(g.n[2]) : 1;
+  double ox = g.origin[0], oy = g.origin[1], oz = g.origin[2];
+  double hx = g.h[0], hy = g.dim > 1 ? g.h[1] : 0.0, hz = g.dim > 2 ? g.h[2] : 0.0;
+  const int* own_start = own_start_.get();
+  const unsigned* own_mask = own_mask_.get();
+  const double* prm = prm_.get();
+  const double* base = nullptr;
+  const int* stop = nullptr;
+  double* diag = pdiag_.get();
+  int mode = 8;
+  for (size_t c = 0; c < L_.comps.size(); ++c) {
+    if (!pc_set_fn_[c] || comp_dev_[c].ntiles <= 0) continue;
+    const int* tiles = comp_dev_[c].tiles.get();
+    const int* g2l = comp_dev_[c].g2l.get();
+    const unsigned char* cmask = comp_dev_[c].cell_mask_grid.get();
+    double* pc = pc_[c].get();
+    void* args[] = {&tiles, &g2l, &cmask, &x, &base, &t, &cm, &ca, &diag, &pc, &own_start, &own_mask,
+                    &n0, &n1, &n2, &ox, &oy, &oz, &hx, &hy, &hz, &mode, &prm, &stop};
+    DCG_HIP(hipModuleLaunchKernel(pc_set_fn_[c], comp_dev_[c].ntiles, 1, 1, 256, 1, 1, 0, stream_, args, nullptr));
+  }
+  pc_t_ = t;
+  pc_ca_ = ca;
+}
+
+// y = J v from the stored point coefficients: every compartment's tiles write
+// the rows they own (padding species rows are the identity).
+void Model::launch_points_apply(const void* v, void* y, const int* stop, bool fp32) {
+  const Grid& g = L_.grid;
+  int n0 = static_cast<int>(g.n[0]), n1 = g.dim > 1 ? static_cast<int>(g.n[1]) : 1;
+  int n2 = g.dim > 2 ? static_cast<int>(g.n[2]) : 1;
+  const double* prm = prm_.get();
+  double t = pc_t_, ca = pc_ca_;
+  for (size_t c = 0; c < L_.comps.size(); ++c) {
+    if (!pc_app_fn_[c] || comp_dev_[c].ntiles <= 0) continue;
+    const int* tiles = comp_dev_[c].tiles.get();
+    const int* g2l = comp_dev_[c].g2l.get();
+    const unsigned char* cmask = comp_dev_[c].cell_mask_grid.get();
+    const void* pc = fp32 ? static_cast<const void*>(pc32_[c].get()) : static_cast<const void*>(pc_[c].get());
+    void* args[] = {&tiles, &g2l, &cmask, &pc, &v, &y, &t, &ca, &n0, &n1, &n2, &prm, &stop};
+    DCG_HIP(hipModuleLaunchKernel(fp32 ? pc_appf_fn_[c] : pc_app_fn_[c], comp_dev_[c].ntiles, 1, 1, 256, 1, 1, 0,
+                                  stream_, args, nullptr));
+  }
+}
+
+void Model::ensure_matrix() {
+  if (val_.size()) return;
+  val_.resize(static_cast<size_t>(matrix_values()));
+  if (cfg_.precision == Precision::Mixed) val32_.resize(static_cast<size_t>(matrix_values()));
+  if (!dinv_.size()) dinv_.resize(static_cast<size_t>(L_.nslot) * L_.B * L_.B);
+  if (cfg_.precond == Preconditioner::ILU0 && !lu_.size()) lu_.resize(static_cast<size_t>(matrix_values()));
+  plan32_ = plan_;
+}
+
 double Model::residual_norm(const double* r) {
   const long n = L_.nslot * L_.B;
   k::reduce(n, r, nullptr, cfg_.norm == 0 ? 0 : cfg_.norm == 1 ? 2 : 1, part_.get(), scalar_out_.get(), stream_);
@@ -654,7 +764,10 @@ double Model::residual_norm(const double* r) {
 void Model::precond_setup() {
   const int B = L_.B;
   const int ns = static_cast<int>(L_.nslot);
-  if (jacobi_kind()) {
+  if (jacobi_kind() && use_points_) {
+    k::block_inverse(B, ns, pdiag_.get(), diag_id_.get(), dinv_.get(), status_.get(),
+                     cfg_.precond == Preconditioner::Jacobi, stream_);
+  } else if (jacobi_kind()) {
     k::block_inverse(B, ns, val_.get(), diag_pos_.get(), dinv_.get(), status_.get(),
                      cfg_.precond == Preconditioner::Jacobi, stream_);
   } else if (cfg_.precond == Preconditioner::ILU0) {
@@ -697,6 +810,14 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
       prof_begin(kJacobianApply);
       launch_apply(lin_x_.get(), lin_t_, lin_cm_, lin_ca_, in, out, done_flag);
       prof_end(kJacobianApply, apply_bytes());
+      k::dot_parts(n, out, w1, w2, part_.get(), nred, sc_.get(), stream_);
+      k::bicg_fin(nred, part_.get(), 2, stage, sc_.get(), stream_);
+      return;
+    }
+    if (use_points_) {
+      prof_begin(kPointApply64);
+      launch_points_apply(in, out, done_flag, false);
+      prof_end(kPointApply64, point_apply_bytes(8));
       k::dot_parts(n, out, w1, w2, part_.get(), nred, sc_.get(), stream_);
       k::bicg_fin(nred, part_.get(), 2, stage, sc_.get(), stream_);
       return;
@@ -758,22 +879,32 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
       k::ilu_upper<float>(B, up_ptr_[l + 1] - up_ptr_[l], up_rows_.get() + up_ptr_[l], row_ptr_.get(), col_.get(),
                           diag_pos_.get(), lu_.get(), dinv_.get(), tmp, out, sc_.get(), stream_);
   };
+  const int* done_flag =
+      reinterpret_cast<const int*>(reinterpret_cast<const char*>(sc_.get()) + offsetof(k::BicgScalars, done));
+  auto matvec = [&](const float* in, float* out, const float* w1, const float* w2, int stage) {
+    if (use_points_) {
+      prof_begin(kPointApply32);
+      launch_points_apply(in, out, done_flag, true);
+      prof_end(kPointApply32, point_apply_bytes(4));
+      k::dot_parts(n, out, w1, w2, part_.get(), nred, sc_.get(), stream_);
+      k::bicg_fin(nred, part_.get(), 2, stage, sc_.get(), stream_);
+      return;
+    }
+    prof_begin(kSpmv32);
+    k::spmv(B, plan32_, val32_.get(), in, out, w1, w2, part_.get(), sc_.get(), stage, stream_);
+    k::bicg_fin(plan32_.parts(), part_.get(), 2, stage, sc_.get(), stream_);
+    prof_end(kSpmv32, spmv_bytes(4, 4, 1));
+  };
   int done_its = 0, batch = std::max(2, std::min(last_krylov_its_, 64));
   for (;;) {
     for (int it = 0; it < batch; ++it) {
       k::bicg_p<float>(B, ns, r, p, v, ph, dinv, sc_.get(), stream_);
       if (ilu) apply_ilu(p, ph);
-      prof_begin(kSpmv32);
-      k::spmv(B, plan32_, val32_.get(), ph, v, rh, nullptr, part_.get(), sc_.get(), 1, stream_);
-      k::bicg_fin(plan32_.parts(), part_.get(), 2, 1, sc_.get(), stream_);
-      prof_end(kSpmv32, spmv_bytes(4, 4, 1));
+      matvec(ph, v, rh, nullptr, 1);
       k::bicg_s<float>(B, ns, r, v, s, sh, dinv, sc_.get(), part_.get(), stream_);
       if (ilu) apply_ilu(s, sh);
       k::bicg_fin(nblk, part_.get(), 1, 2, sc_.get(), stream_);
-      prof_begin(kSpmv32);
-      k::spmv(B, plan32_, val32_.get(), sh, t, s, t, part_.get(), sc_.get(), 3, stream_);
-      k::bicg_fin(plan32_.parts(), part_.get(), 2, 3, sc_.get(), stream_);
-      prof_end(kSpmv32, spmv_bytes(4, 4, 1));
+      matvec(sh, t, s, t, 3);
       k::bicg_xr<float>(B, ns, x, ph, sh, s, t, r, rh, sc_.get(), part_.get(), stream_);
       k::bicg_fin(nblk, part_.get(), 2, 4, sc_.get(), stream_);
     }
@@ -803,7 +934,12 @@ int Model::krylov(const double* b, double* x, double rtol, int maxit, int* its, 
   *refinements = 0;
   const long n = L_.nslot * L_.B;
   if (cfg_.precision == Precision::FP64) return bicgstab64(b, x, rtol, 0.0, maxit, its);
-  k::to_float(matrix_values(), val_.get(), val32_.get(), stream_);
+  if (use_points_) {
+    for (size_t c = 0; c < pc_.size(); ++c)
+      if (pc_[c].size()) k::to_float(static_cast<long>(pc_[c].size()), pc_[c].get(), pc32_[c].get(), stream_);
+  } else {
+    k::to_float(matrix_values(), val_.get(), val32_.get(), stream_);
+  }
   double* res = kv_[8].get();  // FP64 refinement residual
   double* ax = kv_[7].get();
   float* d = kf_[9].get();
@@ -828,7 +964,8 @@ int Model::krylov(const double* b, double* x, double rtol, int maxit, int* its, 
     // x += d (FP64), res = b - A x (FP64)
     k::to_double(n, d, ax, stream_);
     k::axpy(n, 1.0, ax, x, stream_);
-    k::spmv(L_.B, plan_, val_.get(), x, ax, nullptr, nullptr, nullptr, nullptr, 0, stream_);
+    if (use_points_) launch_points_apply(x, ax, nullptr, false);
+    else k::spmv(L_.B, plan_, val_.get(), x, ax, nullptr, nullptr, nullptr, nullptr, 0, stream_);
     DCG_HIP(hipMemcpyAsync(res, b, n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
     k::axpy(n, -1.0, ax, res, stream_);
     k::reduce(n, res, nullptr, 0, part_.get(), scalar_out_.get(), stream_);
@@ -875,6 +1012,12 @@ int Model::step(double t, double dt, StepStats* st) {
         lin_t_ = ts;
         lin_cm_ = cm;
         lin_ca_ = ca;
+        return;
+      }
+      if (use_points_) {
+        prof_begin(kPointSetup);
+        launch_points_setup(x, ts, cm, ca);
+        prof_end(kPointSetup, point_setup_bytes());
         return;
       }
       prof_begin(kAssemblyJac);
@@ -1112,10 +1255,38 @@ double Model::apply_bytes() const {
   return 8.0 * B * N * 3.0 + cells * (4.0 + 1.0 + 4.0 * L_.grid.ncorner);
 }
 
+// Algorithmic bytes of one point-coefficient apply: the coefficients of every
+// cell of the compartments once, v read and y written once, the cell class
+// masks and vertex -> slot maps the tiles read.
+double Model::point_apply_bytes(int value_bytes) const {
+  const double N = static_cast<double>(L_.nslot), B = L_.B;
+  double b = 2.0 * value_bytes * B * N;
+  for (size_t c = 0; c < L_.comps.size(); ++c) {
+    if (!gen_.point_words[c]) continue;
+    const double cells = static_cast<double>(L_.comps[c].cells.size());
+    b += cells * (value_bytes * gen_.point_words[c] + 1.0) + 4.0 * L_.comps[c].nslot;
+  }
+  return b;
+}
+
+// Algorithmic bytes of one point-coefficient set-up: the state read once, the
+// coefficients and diagonal blocks written once, the same tile tables.
+double Model::point_setup_bytes() const {
+  const double N = static_cast<double>(L_.nslot), B = L_.B;
+  double b = 8.0 * B * N + 8.0 * B * B * N;
+  for (size_t c = 0; c < L_.comps.size(); ++c) {
+    if (!gen_.point_words[c]) continue;
+    const double cells = static_cast<double>(L_.comps[c].cells.size());
+    b += cells * (8.0 * gen_.point_words[c] + 1.0) + 4.0 * L_.comps[c].nslot;
+  }
+  return b;
+}
+
 // ------------------------------------------------------------------ test entry points
 void Model::assemble(const double* x_packed, double t, double cm, double ca, double* r_packed, double* jac) {
   require_device();
   if (jac && cfg_.matrix_free) throw Error("a matrix-free model holds no assembled Jacobian");
+  if (jac) ensure_matrix();
   const long n = L_.nslot * L_.B;
   DCG_HIP(hipMemcpyAsync(packed_.get(), x_packed, L_.ndof * sizeof(double), hipMemcpyHostToDevice, stream_));
   k::unpack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), packed_.get(), xs_.get(), stream_);
@@ -1134,7 +1305,13 @@ void Model::jacobian_apply(const double* x_packed, double t, double cm, double c
   k::unpack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), packed_.get(), xs_.get(), stream_);
   DCG_HIP(hipMemcpyAsync(packed_.get(), v_packed, L_.ndof * sizeof(double), hipMemcpyHostToDevice, stream_));
   k::unpack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), packed_.get(), z_.get(), stream_);
-  launch_apply(xs_.get(), t, cm, ca, z_.get(), r_.get(), nullptr);
+  if (cfg_.points) {
+    // the point-coefficient operator the solver uses
+    launch_points_setup(xs_.get(), t, cm, ca);
+    launch_points_apply(z_.get(), r_.get(), nullptr, false);
+  } else {
+    launch_apply(xs_.get(), t, cm, ca, z_.get(), r_.get(), nullptr);
+  }
   k::pack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), r_.get(), packed_.get(), stream_);
   DCG_HIP(hipMemcpyAsync(y_packed, packed_.get(), L_.ndof * sizeof(double), hipMemcpyDeviceToHost, stream_));
   synchronize();
@@ -1143,6 +1320,7 @@ void Model::jacobian_apply(const double* x_packed, double t, double cm, double c
 void Model::spmv(const double* jac, const double* x_packed, double* y_packed) {
   require_device();
   if (cfg_.matrix_free) throw Error("a matrix-free model holds no assembled Jacobian");
+  ensure_matrix();
   DCG_HIP(hipMemcpyAsync(val_.get(), jac, val_.bytes(), hipMemcpyHostToDevice, stream_));
   DCG_HIP(hipMemcpyAsync(packed_.get(), x_packed, L_.ndof * sizeof(double), hipMemcpyHostToDevice, stream_));
   k::unpack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), packed_.get(), xs_.get(), stream_);
@@ -1156,11 +1334,16 @@ int Model::linear_solve(const double* jac, const double* b_packed, double* x_pac
                         int* its) {
   require_device();
   if (cfg_.matrix_free) throw Error("a matrix-free model holds no assembled Jacobian");
+  ensure_matrix();
   DCG_HIP(hipMemcpyAsync(val_.get(), jac, val_.bytes(), hipMemcpyHostToDevice, stream_));
   DCG_HIP(hipMemcpyAsync(packed_.get(), b_packed, L_.ndof * sizeof(double), hipMemcpyHostToDevice, stream_));
   k::unpack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), packed_.get(), r_.get(), stream_);
   int refs = 0;
+  // the given matrix is the operator, whatever the model's storage
+  const bool points = use_points_;
+  use_points_ = false;
   int rc = krylov(r_.get(), z_.get(), rtol, maxit, its, &refs);
+  use_points_ = points;
   k::pack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), z_.get(), packed_.get(), stream_);
   DCG_HIP(hipMemcpyAsync(x_packed, packed_.get(), L_.ndof * sizeof(double), hipMemcpyDeviceToHost, stream_));
   synchronize();
@@ -1189,7 +1372,10 @@ size_t Model::device_bytes() const {
              chunk_row_.bytes() + own_mask_.bytes() + slot_dof_.bytes() + extra_pos_.bytes() + weights_.bytes() +
              prm_.bytes() + u_.bytes() + hist_.bytes() + r_.bytes() + z_.bytes() + xs_.bytes() + packed_.bytes() +
              val_.bytes() + lu_.bytes() + dinv_.bytes() + val32_.bytes() + part_.bytes() + scalar_out_.bytes() +
-             sc_.bytes() + status_.bytes() + lo_rows_.bytes() + up_rows_.bytes();
+             sc_.bytes() + status_.bytes() + lo_rows_.bytes() + up_rows_.bytes() + pdiag_.bytes() + diag_id_.bytes() +
+             lin_x_.bytes();
+  for (const auto& v : pc_) b += v.bytes();
+  for (const auto& v : pc32_) b += v.bytes();
   for (const auto& v : stage_) b += v.bytes();
   for (const auto& v : kv_) b += v.bytes();
   for (const auto& v : kf_) b += v.bytes();

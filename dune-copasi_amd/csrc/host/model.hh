@@ -61,6 +61,9 @@ struct SolverConfig {
   // linear_solver.matrix_free: BiCGSTAB applies the Jacobian at the
   // linearization point from the generated kernels, no matrix is assembled
   bool matrix_free = false;
+  // linear_solver.storage: the Jacobian as point coefficients per cell (the
+  // BiCGSTAB operator applies them) instead of an assembled BCSR matrix
+  bool points = false;
   double ir_inner_rtol = 1e-3;  // FP32 Krylov tolerance inside refinement
   int ir_max = 50;
   std::vector<std::string> warnings;
@@ -148,7 +151,10 @@ class Model {
     kAssemblyRes = 3,
     kPrecond = 4,
     kJacobianApply = 5,
-    kProfKinds = 6
+    kPointSetup = 6,    // point coefficients and diagonal blocks (linear_solver.storage = points)
+    kPointApply64 = 7,  // y = J v from the point coefficients, FP64
+    kPointApply32 = 8,  // the same in FP32 inside the refinement loop
+    kProfKinds = 9
   };
   void set_profiling(bool on);
   // launches, summed milliseconds and summed algorithmic bytes of one kind
@@ -186,6 +192,14 @@ class Model {
                     const int* stop);
   // compartments without a batched tile kernel, or facets: per-model launches
   bool needs_own_assembly() const;
+  bool all_tile_eligible() const;
+  // point-coefficient Jacobian: coefficients and diagonal blocks at x, and
+  // y = J v from them (v, y FP64 or FP32 as fp32 says); stop as in launch_apply
+  void launch_points_setup(const double* x, double t, double cm, double ca);
+  void launch_points_apply(const void* v, void* y, const int* stop, bool fp32);
+  // the assembled matrix for the kernel-level entry points of a model that
+  // solves with point coefficients
+  void ensure_matrix();
   double residual_norm(const double* r);
   // linear solves on the current val_
   int krylov(const double* b, double* x, double rtol, int maxit, int* its, int* refinements);
@@ -210,6 +224,8 @@ class Model {
   double assembly_bytes(bool jacobian) const;
   std::vector<ReduceKey> reduce_keys(const std::string& reduce_ini) const;
   double apply_bytes() const;
+  double point_apply_bytes(int value_bytes) const;
+  double point_setup_bytes() const;
 
   ParameterTree cfg_tree_;
   ModelSpec spec_;
@@ -226,6 +242,8 @@ class Model {
   std::vector<hipFunction_t> tile_fn_;      // tiled residual (_r); set when the compartment is tiled
   std::vector<hipFunction_t> tile_jac_fn_;  // tiled Jacobian (_j)
   std::vector<hipFunction_t> tile_app_fn_;  // tiled matrix-free apply (_a)
+  // point coefficients (_q), their FP64 (_p) and FP32 (_pf) apply
+  std::vector<hipFunction_t> pc_set_fn_, pc_app_fn_, pc_appf_fn_;
   std::vector<hipFunction_t> facet_fn_;
   double time_ = 0.0, dt_initial_ = 0.0, t_end_ = 0.0;
   std::map<std::string, std::vector<double>> vtk_steps_;  // times written per output path
@@ -262,6 +280,16 @@ class Model {
   // matrix-free: the Newton iterate the Jacobian is applied at, and its weights
   DeviceArray<double> lin_x_;
   double lin_t_ = 0.0, lin_cm_ = 0.0, lin_ca_ = 0.0;
+  // point-coefficient Jacobian: per compartment the coefficients of every grid
+  // cell ([class, point, coefficient][cell]), their FP32 copy when mixed, the
+  // rows' diagonal blocks, an identity block-position map for the block
+  // inverse, and the time and stiffness weight they were computed with
+  std::vector<DeviceArray<double>> pc_;
+  std::vector<DeviceArray<float>> pc32_;
+  DeviceArray<double> pdiag_;
+  DeviceArray<int> diag_id_;
+  double pc_t_ = 0.0, pc_ca_ = 0.0;
+  bool use_points_ = false;  // the Krylov operator is the point-coefficient one
   std::vector<DeviceArray<double>> stage_;
   DeviceArray<double> val_, lu_, dinv_;
   DeviceArray<float> val32_;

@@ -166,6 +166,8 @@ void spmv(int B, const SpmvPlan& pl, const float* val, const float* x, float* y,
 // `groups` workgroups (w2 may be null); nothing once sc->done
 void dot_parts(long n, const double* y, const double* w1, const double* w2, double* part, int groups,
                const BicgScalars* sc, hipStream_t s);
+void dot_parts(long n, const float* y, const float* w1, const float* w2, double* part, int groups,
+               const BicgScalars* sc, hipStream_t s);
 
 // Jacobi preconditioner blocks: the inverse of each diagonal B x B block, or
 // with point set only the inverse of its diagonal (in the same layout)

@@ -304,16 +304,17 @@ __global__ void __launch_bounds__(kThreads) bicg_s_k(int nslot, const V* __restr
 }
 
 // partial (y.w1), (y.w2) per workgroup over a fixed grid (deterministic)
-__global__ void __launch_bounds__(kThreads) dot_parts_k(long n, const double* __restrict__ y,
-                                                        const double* __restrict__ w1, const double* __restrict__ w2,
-                                                        double* __restrict__ part, const BicgScalars* __restrict__ sc) {
+template <typename V>
+__global__ void __launch_bounds__(kThreads) dot_parts_k(long n, const V* __restrict__ y, const V* __restrict__ w1,
+                                                        const V* __restrict__ w2, double* __restrict__ part,
+                                                        const BicgScalars* __restrict__ sc) {
   if (sc && finished(sc)) return;
   __shared__ double red[32];
   double acc[2] = {0.0, 0.0};
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const double v = y[i];
-    acc[0] += v * w1[i];
-    if (w2) acc[1] += v * w2[i];
+    acc[0] += v * static_cast<double>(w1[i]);
+    if (w2) acc[1] += v * static_cast<double>(w2[i]);
   }
   block_sum(acc, red);
   if (threadIdx.x == 0) {
@@ -700,7 +701,11 @@ void spmv(int B, const SpmvPlan& pl, const float* val, const float* x, float* y,
 
 void dot_parts(long n, const double* y, const double* w1, const double* w2, double* part, int groups,
                const BicgScalars* sc, hipStream_t s) {
-  dot_parts_k<<<groups, kThreads, 0, s>>>(n, y, w1, w2, part, sc);
+  dot_parts_k<double><<<groups, kThreads, 0, s>>>(n, y, w1, w2, part, sc);
+}
+void dot_parts(long n, const float* y, const float* w1, const float* w2, double* part, int groups,
+               const BicgScalars* sc, hipStream_t s) {
+  dot_parts_k<float><<<groups, kThreads, 0, s>>>(n, y, w1, w2, part, sc);
 }
 
 void block_inverse(int B, int nslot, const double* val, const int* diag_pos, double* dinv, int* status, bool point,

@@ -131,8 +131,10 @@ int dcg_linear_solve(dcg_model_t model, const double* jacobian, const double* b,
  * reference's Perfetto counters (dune/copasi/common/stepper.hh:100-102,
  * dune/copasi/model/make_step_operator.hh:137-139).  kind: 0 FP64 SpMV,
  * 1 FP32 SpMV, 2 Jacobian assembly pass, 3 residual assembly pass,
- * 4 preconditioner set-up, 5 matrix-free Jacobian apply.  Enabling resets the counters; `bytes` sums the
- * algorithmic bytes of the timed launches. */
+ * 4 preconditioner set-up, 5 matrix-free Jacobian apply, 6 point-coefficient
+ * set-up, 7 FP64 and 8 FP32 point-coefficient apply (linear_solver.storage =
+ * points).  Enabling resets the counters; `bytes` sums the algorithmic bytes
+ * of the timed launches. */
 int dcg_profile_enable(dcg_model_t model, int on);
 int dcg_profile_read(dcg_model_t model, int kind, long* launches, double* milliseconds, double* bytes);
 
