@@ -1237,24 +1237,12 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
     }
     __syncthreads();
     if constexpr (MODE == 8) {
-      // the cells this tile owns (their lowest corner is one of its rows, so
-      // every cell has exactly one owner) write their coefficients to
-      // val[((p*NQ + q)*NMC + n) * ncell + cell], cells in grid order
-      const long ncg = (long)n0 * n1 * n2;
-      for (int cl = threadIdx.x; cl < NCB; cl += blockDim.x) {
-        const unsigned m = cms[cl];
-        if (!m) continue;
-        const int ci = cl % CXB, cj = (cl / CXB) % CYB, ck = cl / (CXB * CYB);
-        if (ci < 1 || (DIM > 1 && cj < 1) || (DIM > 2 && ck < 1)) continue;
-        const long gc = (vi0 + ci) + (long)n0 * ((vj0 + cj) + (long)n1 * (vk0 + ck));
-        for (int p = 0; p < NCLASS; ++p) {
-          if (!((m >> p) & 1u)) continue;
-          for (int q = 0; q < NQ; ++q)
-#pragma unroll
-            for (int n = 0; n < NMC; ++n)
-              val[(size_t)((p * NQ + q) * NMC + n) * ncg + gc] = work[(size_t)((p * NQ + q) * NMC + n) * NCB + cl];
-        }
-      }
+      // the coefficients of every cell of the tile's box, in the LDS layout,
+      // into the tile's own block of val (block = position in the launch's
+      // tile list): the apply then reads one contiguous block per tile; cells
+      // outside the compartment carry values that are never read
+      if constexpr (NMC > 0)
+        for (int k = threadIdx.x; k < MC_WORDS; k += blockDim.x) val[(size_t)blockIdx.x * MC_WORDS + k] = work[k];
       // diagonal block of every row (the Jacobi preconditioner's input):
       // r[slot * B * B + i * B + j], the identity on padding species
       constexpr int AI = BC > 4 ? 1 : BC, NR = BC / AI;
@@ -1621,8 +1609,8 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
   }
 }
 
-// y = J v from stored point coefficients (mode 8 of tile_body): per cell of
-// the tile's box, v at its corners is interpolated to each point, multiplied
+// y = J v from stored point coefficients (mode 8 of tile_body, one block per
+// tile): per cell of the tile's box, v at its corners is interpolated to each point, multiplied
 // by the point's coefficient block and tested against the basis functions,
 // the uniform diffusion adds the element stiffness times v; the cells' corner
 // sums are folded onto the tile's rows as in the residual.  T is the value
@@ -1639,6 +1627,18 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
   const int nty = DIM > 1 ? (n1 + 1 + TY - 1) / TY : 1;
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
   const int vi0 = tx * TX - 1, vj0 = DIM > 1 ? ty * TY - 1 : 0, vk0 = DIM > 2 ? tz * TZ - 1 : 0;
+  // this tile's block of coefficients: [(p*NQ + q)*NMC + n][box cell]
+  const T* pcb = pc + (size_t)blockIdx.x * MC_WORDS;
+  // one cell per thread and few coefficients: the thread's coefficients are
+  // loaded before the staging barrier, so they arrive while v is staged
+  constexpr int PW = NCLASS * NQ * NMC;
+  constexpr bool PREFETCH = NMC > 0 && PW <= 32 && NCB <= 256;
+  T pre[PREFETCH ? PW : 1];
+  if constexpr (PREFETCH) {
+    if (threadIdx.x < NCB)
+#pragma unroll
+      for (int k = 0; k < PW; ++k) pre[k] = pcb[(size_t)k * NCB + threadIdx.x];
+  }
   for (int b = threadIdx.x; b < NVB; b += blockDim.x) {
     const int bi = b % XB, bj = (b / XB) % YB, bk = b / (XB * YB);
     const int gi = vi0 + bi, gj = vj0 + bj, gk = vk0 + bk;
@@ -1659,7 +1659,6 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
   __syncthreads();
   double dv[NDV > 0 ? NDV : 1];
   uniform_diffusion(t, prm, dv);
-  const long ncg = (long)n0 * n1 * n2;
   // corner values stay in registers for small blocks, in LDS otherwise
   constexpr bool VREG = BC <= 4;
   for (int cl = threadIdx.x; cl < NCB; cl += blockDim.x) {
@@ -1668,7 +1667,6 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
     const unsigned m = cms[cl];
     if (!m) continue;
     const int ci = cl % CXB, cj = (cl / CXB) % CYB, ck = cl / (CXB * CYB);
-    const long gc = (vi0 + ci) + (long)n0 * ((vj0 + cj) + (long)n1 * (vk0 + ck));
     for (int p = 0; p < NCLASS; ++p) {
       if (!((m >> p) & 1u)) continue;
       int bidx[NV];
@@ -1692,12 +1690,17 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
           for (int a = 0; a < NV; ++a) sum += T(PHI[q][a]) * vcor(a, j);
           vq[j] = sum;
         }
-        const T* mp = pc + (size_t)((p * NQ + q) * NMC) * ncg + gc;
+        const T* mp = pcb + (size_t)((p * NQ + q) * NMC) * NCB + cl;
         T w[BC];
 #pragma unroll
         for (int i = 0; i < BC; ++i) w[i] = T(0);
 #pragma unroll
-        for (int n = 0; n < NMC; ++n) w[MCI[n]] += mp[(size_t)n * ncg] * vq[MCJ[n]];
+        for (int n = 0; n < NMC; ++n) {
+          T m;
+          if constexpr (PREFETCH) m = pre[PREFETCH ? (p * NQ + q) * NMC + n : 0];
+          else m = mp[(size_t)n * NCB];
+          w[MCI[n]] += m * vq[MCJ[n]];
+        }
 #pragma unroll
         for (int a = 0; a < NV; ++a)
 #pragma unroll

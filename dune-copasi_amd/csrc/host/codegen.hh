@@ -32,8 +32,9 @@ struct GeneratedSource {
   // batched tiled kernel per compartment (runtime geometry only, else "")
   std::vector<std::string> tile_batch_kernel;
   std::vector<std::array<int, 3>> tile_dims;
-  // doubles per grid cell of the point-coefficient Jacobian (classes x
-  // points x coefficients; 0 when the compartment has no point kernels)
+  // doubles per cell of the point-coefficient Jacobian (classes x points x
+  // coefficients; 0 when the compartment has no point kernels); each tile
+  // keeps the cells of its box: (tile_dims + 1 per axis) cells per tile
   std::vector<int> point_words;
   std::vector<FacetKernel> facet_kernels;
 };

@@ -412,7 +412,11 @@ void Model::setup_device() {
     pc_.resize(L_.comps.size());
     if (cfg_.precision == Precision::Mixed) pc32_.resize(L_.comps.size());
     for (size_t c = 0; c < L_.comps.size(); ++c) {
-      const size_t words = static_cast<size_t>(gen_.point_words[c]) * static_cast<size_t>(L_.grid.ncell);
+      // one block per tile of the cells of its box (halo cells included)
+      const auto td = gen_.tile_dims[c];
+      const size_t box = static_cast<size_t>(td[0] + 1) * (L_.grid.dim > 1 ? td[1] + 1 : 1) *
+                         (L_.grid.dim > 2 ? td[2] + 1 : 1);
+      const size_t words = static_cast<size_t>(gen_.point_words[c]) * box * std::max(0, comp_dev_[c].ntiles);
       pc_[c].resize(words);
       if (cfg_.precision == Precision::Mixed) pc32_[c].resize(words);
     }
