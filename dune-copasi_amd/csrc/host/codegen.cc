@@ -1614,11 +1614,16 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
 // by the point's coefficient block and tested against the basis functions,
 // the uniform diffusion adds the element stiffness times v; the cells' corner
 // sums are folded onto the tile's rows as in the residual.  T is the value
-// type of pc, v and y (double, or float inside the refinement loop).
+// type of pc, v and y (double, or float inside the refinement loop).  g2l
+// null: the compartment's slots are the grid's vertices in order.  With part,
+// the workgroup also writes its rows' partial dots (y.w1, y.w2) to
+// part[2 blockIdx.x + 0/1] (w2 may be null), as the BiCGSTAB finaliser reads.
 template <typename T>
 __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,
                                          const T* __restrict__ pc, const T* __restrict__ v, T* __restrict__ y, double t,
-                                         double ca, int n0, int n1, int n2, const double* __restrict__ prm) {
+                                         double ca, int n0, int n1, int n2, const double* __restrict__ prm,
+                                         const T* __restrict__ w1, const T* __restrict__ w2,
+                                         double* __restrict__ part) {
   __shared__ T vs[NVB * BC];
   __shared__ int sl[NVB];
   __shared__ unsigned char cms[NCB];
@@ -1643,8 +1648,10 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
     const int bi = b % XB, bj = (b / XB) % YB, bk = b / (XB * YB);
     const int gi = vi0 + bi, gj = vj0 + bj, gk = vk0 + bk;
     int s = -1;
-    if (gi >= 0 && gi <= n0 && gj >= 0 && gj <= (DIM > 1 ? n1 : 0) && gk >= 0 && gk <= (DIM > 2 ? n2 : 0))
-      s = g2l[gi + (n0 + 1) * (gj + (n1 + 1) * gk)];
+    if (gi >= 0 && gi <= n0 && gj >= 0 && gj <= (DIM > 1 ? n1 : 0) && gk >= 0 && gk <= (DIM > 2 ? n2 : 0)) {
+      const int vid = gi + (n0 + 1) * (gj + (n1 + 1) * gk);
+      s = g2l ? g2l[vid] : vid;
+    }
     sl[b] = s;
     for (int i = 0; i < BC; ++i) vs[b * BC + i] = s >= 0 ? v[(size_t)s * DCG_B + i] : T(0);
   }
@@ -1755,6 +1762,7 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
     }
   }
   __syncthreads();
+  double d1 = 0.0, d2 = 0.0;
   for (int it = threadIdx.x; it < NROW * DCG_B; it += blockDim.x) {
     const int lr = it / DCG_B, i = it % DCG_B;
     const int li = lr % TX, lj = DIM > 1 ? (lr / TX) % TY : 0, lk = DIM > 2 ? lr / (TX * TY) : 0;
@@ -1775,6 +1783,33 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
       acc = v[(size_t)s * DCG_B + i];  // padding species: identity rows
     }
     y[(size_t)s * DCG_B + i] = acc;
+    if (part) {
+      d1 += (double)acc * (double)w1[(size_t)s * DCG_B + i];
+      if (w2) d2 += (double)acc * (double)w2[(size_t)s * DCG_B + i];
+    }
+  }
+  if (part) {
+    // workgroup sums of the two partial dots (fixed order)
+    __shared__ double red[8];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      d1 += __shfl_down(d1, o, 64);
+      d2 += __shfl_down(d2, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      red[2 * (threadIdx.x >> 6)] = d1;
+      red[2 * (threadIdx.x >> 6) + 1] = d2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double a = 0.0, b = 0.0;
+      for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+        a += red[2 * k];
+        b += red[2 * k + 1];
+      }
+      part[2 * blockIdx.x] = a;
+      part[2 * blockIdx.x + 1] = b;
+    }
   }
 }
 }  // namespace %C%
@@ -1817,9 +1852,10 @@ extern "C" __global__ void __launch_bounds__(256)
   extern "C" __global__ void __launch_bounds__(256)                                                               \
   NAME(const int* __restrict__ tiles, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,         \
        const T* __restrict__ pc, const T* __restrict__ v, T* __restrict__ y, double t, double ca, int n0, int n1,  \
-       int n2, const double* __restrict__ prm, const int* __restrict__ stop) {                                     \
+       int n2, const double* __restrict__ prm, const int* __restrict__ stop, const T* __restrict__ w1,            \
+       const T* __restrict__ w2, double* __restrict__ part) {                                                     \
     if (stop && *stop) return;                                                                                    \
-    %C%::pc_apply<T>(tiles[blockIdx.x], g2l, cmask, pc, v, y, t, ca, n0, n1, n2, prm);                            \
+    %C%::pc_apply<T>(tiles[blockIdx.x], g2l, cmask, pc, v, y, t, ca, n0, n1, n2, prm, w1, w2, part);              \
   }
 DCG_POINT_ENTRY(%KERNEL%_p, double)
 DCG_POINT_ENTRY(%KERNEL%_pf, float)

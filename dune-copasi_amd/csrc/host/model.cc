@@ -303,6 +303,11 @@ void Model::setup_device() {
     for (long t = 0; t < ntx * nty * ntz; ++t)
       if (used[t]) tiles.push_back(static_cast<int>(t));
     comp_dev_[c].g2l.upload(g2l, stream_);
+    {
+      bool id = C.slot_off == 0 && C.nslot == gr.nvert;
+      for (long l = 0; id && l < C.nslot; ++l) id = C.nodes[l] == l;
+      comp_dev_[c].g2l_identity = id;
+    }
     comp_dev_[c].cell_mask_grid.upload(cm, stream_);
     comp_dev_[c].tiles.upload(tiles, stream_);
     comp_dev_[c].ntiles = static_cast<int>(tiles.size());
@@ -436,7 +441,9 @@ void Model::setup_device() {
   }
   const size_t nparts = std::max<size_t>(std::max<size_t>(static_cast<size_t>(plan_.nchunk), plan_.row_wgs) + 8,
                                          static_cast<size_t>((ns + 255) / 256) + 8);
-  part_.resize(std::max<size_t>(2 * nparts, 8 * 1024 + 64));
+  size_t ntiles_all = 0;
+  for (const auto& cd : comp_dev_) ntiles_all += static_cast<size_t>(std::max(0, cd.ntiles));
+  part_.resize(std::max<size_t>(std::max<size_t>(2 * nparts, 2 * ntiles_all + 16), 8 * 1024 + 64));
   scalar_out_.resize(16);
   sc_.resize(1);
   status_.resize(4);
@@ -727,23 +734,29 @@ void Model::launch_points_setup(const double* x, double t, double cm, double ca)
 }
 
 // y = J v from the stored point coefficients: every compartment's tiles write
-// the rows they own (padding species rows are the identity).
-void Model::launch_points_apply(const void* v, void* y, const int* stop, bool fp32) {
+// the rows they own (padding species rows are the identity) and, with part,
+// one pair of partial dots each.
+int Model::launch_points_apply(const void* v, void* y, const int* stop, bool fp32, const void* w1, const void* w2,
+                               double* part) {
   const Grid& g = L_.grid;
   int n0 = static_cast<int>(g.n[0]), n1 = g.dim > 1 ? static_cast<int>(g.n[1]) : 1;
   int n2 = g.dim > 2 ? static_cast<int>(g.n[2]) : 1;
   const double* prm = prm_.get();
   double t = pc_t_, ca = pc_ca_;
+  int nparts = 0;
   for (size_t c = 0; c < L_.comps.size(); ++c) {
     if (!pc_app_fn_[c] || comp_dev_[c].ntiles <= 0) continue;
     const int* tiles = comp_dev_[c].tiles.get();
-    const int* g2l = comp_dev_[c].g2l.get();
+    const int* g2l = comp_dev_[c].g2l_identity ? nullptr : comp_dev_[c].g2l.get();
     const unsigned char* cmask = comp_dev_[c].cell_mask_grid.get();
     const void* pc = fp32 ? static_cast<const void*>(pc32_[c].get()) : static_cast<const void*>(pc_[c].get());
-    void* args[] = {&tiles, &g2l, &cmask, &pc, &v, &y, &t, &ca, &n0, &n1, &n2, &prm, &stop};
+    double* pp = part ? part + 2 * static_cast<size_t>(nparts) : nullptr;
+    void* args[] = {&tiles, &g2l, &cmask, &pc, &v, &y, &t, &ca, &n0, &n1, &n2, &prm, &stop, &w1, &w2, &pp};
     DCG_HIP(hipModuleLaunchKernel(fp32 ? pc_appf_fn_[c] : pc_app_fn_[c], comp_dev_[c].ntiles, 1, 1, 256, 1, 1, 0,
                                   stream_, args, nullptr));
+    nparts += comp_dev_[c].ntiles;
   }
+  return nparts;
 }
 
 void Model::ensure_matrix() {
@@ -820,10 +833,9 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
     }
     if (use_points_) {
       prof_begin(kPointApply64);
-      launch_points_apply(in, out, done_flag, false);
+      const int np = launch_points_apply(in, out, done_flag, false, w1, w2, part_.get());
       prof_end(kPointApply64, point_apply_bytes(8));
-      k::dot_parts(n, out, w1, w2, part_.get(), nred, sc_.get(), stream_);
-      k::bicg_fin(nred, part_.get(), 2, stage, sc_.get(), stream_);
+      k::bicg_fin(np, part_.get(), 2, stage, sc_.get(), stream_);
       return;
     }
     prof_begin(kSpmv64);
@@ -888,10 +900,9 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
   auto matvec = [&](const float* in, float* out, const float* w1, const float* w2, int stage) {
     if (use_points_) {
       prof_begin(kPointApply32);
-      launch_points_apply(in, out, done_flag, true);
+      const int np = launch_points_apply(in, out, done_flag, true, w1, w2, part_.get());
       prof_end(kPointApply32, point_apply_bytes(4));
-      k::dot_parts(n, out, w1, w2, part_.get(), nred, sc_.get(), stream_);
-      k::bicg_fin(nred, part_.get(), 2, stage, sc_.get(), stream_);
+      k::bicg_fin(np, part_.get(), 2, stage, sc_.get(), stream_);
       return;
     }
     prof_begin(kSpmv32);

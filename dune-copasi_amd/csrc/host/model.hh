@@ -196,7 +196,10 @@ class Model {
   // point-coefficient Jacobian: coefficients and diagonal blocks at x, and
   // y = J v from them (v, y FP64 or FP32 as fp32 says); stop as in launch_apply
   void launch_points_setup(const double* x, double t, double cm, double ca);
-  void launch_points_apply(const void* v, void* y, const int* stop, bool fp32);
+  // with part: also the partial dots (y.w1, y.w2) per tile into part; returns
+  // the number of partial pairs written
+  int launch_points_apply(const void* v, void* y, const int* stop, bool fp32, const void* w1 = nullptr,
+                          const void* w2 = nullptr, double* part = nullptr);
   // the assembled matrix for the kernel-level entry points of a model that
   // solves with point coefficients
   void ensure_matrix();
@@ -261,6 +264,7 @@ class Model {
     DeviceArray<int> g2l, tiles;
     DeviceArray<unsigned char> cell_mask_grid;
     int ntiles = 0;
+    bool g2l_identity = false;  // the compartment's slots are the grid's vertices in order
     // unstructured cells: geometry words and block positions per cell
     DeviceArray<double> egeo;
     DeviceArray<int> cpos;
