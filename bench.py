@@ -443,7 +443,7 @@ def main():
     ap.add_argument("--workload", default="ms96")
     ap.add_argument("--preconditioner", default="Jacobi", choices=["Jacobi", "VertexBlockJacobi"],
                     help="Jacobi: the reference's (diagonal); VertexBlockJacobi: inverse of each vertex's species block")
-    ap.add_argument("--storage", default="matrix", choices=["matrix", "points", "auto"],
+    ap.add_argument("--storage", default="auto", choices=["matrix", "points", "auto"],
                     help="single models: the Jacobian as an assembled BCSR matrix, or as point coefficients per "
                          "cell applied by the Krylov operator (linear_solver.storage)")
     ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],

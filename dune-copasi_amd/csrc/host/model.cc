@@ -156,7 +156,7 @@ void Model::build(const std::string& ini, const std::string& overrides) {
   // and a preconditioner built from diagonal blocks (Jacobi, vertex-block
   // Jacobi or none).
   {
-    const std::string st = c.get(p + "linear_solver.storage", std::string("matrix"));
+    const std::string st = c.get(p + "linear_solver.storage", std::string("auto"));
     if (st != "matrix" && st != "points" && st != "auto")
       throw IOError("linear_solver.storage must be 'matrix', 'points' or 'auto'");
     std::string why;
