@@ -1624,10 +1624,16 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
                                          double ca, int n0, int n1, int n2, const double* __restrict__ prm,
                                          const T* __restrict__ w1, const T* __restrict__ w2,
                                          double* __restrict__ part) {
+  // corner values stay in registers for small blocks, in LDS otherwise; with
+  // small blocks and one cell per thread the corner sums go straight onto
+  // the vertices (SCATTER), otherwise through a per-cell corner array
+  constexpr bool VREG = BC <= 4;
+  constexpr bool SCATTER = VREG && NCB <= 256;
   __shared__ T vs[NVB * BC];
   __shared__ int sl[NVB];
   __shared__ unsigned char cms[NCB];
-  __shared__ T out[NCB * NCORNER * BC];
+  __shared__ T out[SCATTER ? 1 : NCB * NCORNER * BC];
+  __shared__ T accv[SCATTER ? NVB * BC : 1];
   const int ntx = (n0 + 1 + TX - 1) / TX;
   const int nty = DIM > 1 ? (n1 + 1 + TY - 1) / TY : 1;
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
@@ -1654,6 +1660,8 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
     }
     sl[b] = s;
     for (int i = 0; i < BC; ++i) vs[b * BC + i] = s >= 0 ? v[(size_t)s * DCG_B + i] : T(0);
+    if constexpr (SCATTER)
+      for (int i = 0; i < BC; ++i) accv[b * BC + i] = T(0);
   }
   for (int c = threadIdx.x; c < NCB; c += blockDim.x) {
     const int ci = c % CXB, cj = (c / CXB) % CYB, ck = c / (CXB * CYB);
@@ -1666,14 +1674,10 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
   __syncthreads();
   double dv[NDV > 0 ? NDV : 1];
   uniform_diffusion(t, prm, dv);
-  // corner values stay in registers for small blocks, in LDS otherwise
-  constexpr bool VREG = BC <= 4;
-  for (int cl = threadIdx.x; cl < NCB; cl += blockDim.x) {
-    T* dst = out + (size_t)cl * (NCORNER * BC);
-    for (int k = 0; k < NCORNER * BC; ++k) dst[k] = T(0);
-    const unsigned m = cms[cl];
-    if (!m) continue;
+  // the corner sums of box cell cl over its classes, handed to put(corner, i, value)
+  auto cell_corner_sums = [&](int cl, unsigned m, auto&& put) {
     const int ci = cl % CXB, cj = (cl / CXB) % CYB, ck = cl / (CXB * CYB);
+#pragma unroll
     for (int p = 0; p < NCLASS; ++p) {
       if (!((m >> p) & 1u)) continue;
       int bidx[NV];
@@ -1703,10 +1707,10 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
         for (int i = 0; i < BC; ++i) w[i] = T(0);
 #pragma unroll
         for (int n = 0; n < NMC; ++n) {
-          T m;
-          if constexpr (PREFETCH) m = pre[PREFETCH ? (p * NQ + q) * NMC + n : 0];
-          else m = mp[(size_t)n * NCB];
-          w[MCI[n]] += m * vq[MCJ[n]];
+          T mv;
+          if constexpr (PREFETCH) mv = pre[PREFETCH ? (p * NQ + q) * NMC + n : 0];
+          else mv = mp[(size_t)n * NCB];
+          w[MCI[n]] += mv * vq[MCJ[n]];
         }
 #pragma unroll
         for (int a = 0; a < NV; ++a)
@@ -1738,7 +1742,7 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
         }
       };
       if constexpr (VREG) {
-        // small blocks: the corner sums in registers
+        // small blocks: the class's corner sums in registers
         T o[NV][BC];
 #pragma unroll
         for (int a = 0; a < NV; ++a)
@@ -1751,14 +1755,46 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
 #pragma unroll
         for (int a = 0; a < NV; ++a)
 #pragma unroll
-          for (int i = 0; i < BC; ++i) dst[CORNER[p][a] * BC + i] += o[a][i];
+          for (int i = 0; i < BC; ++i) put(CORNER[p][a], i, o[a][i]);
       } else {
-        // large blocks: the corner sums in this cell's LDS row, point by point
-        auto add = [&](int a, int i, T v2) { dst[CORNER[p][a] * BC + i] += v2; };
+        // large blocks: straight into the sums, point by point
+        auto add = [&](int a, int i, T v2) { put(CORNER[p][a], i, v2); };
 #pragma unroll 1
         for (int q = 0; q < NQ; ++q) point_terms(q, add);
         diffusion_terms(add);
       }
+    }
+  };
+  if constexpr (SCATTER) {
+    // one cell per thread; its corner sums are added to the vertices corner
+    // by corner: for one corner the cells' vertices are distinct, so no two
+    // threads add to the same vertex between two barriers, and every vertex
+    // sums its cells in corner order
+    const int cl = threadIdx.x;
+    const bool has = cl < NCB && cms[cl < NCB ? cl : 0];
+    T oc[NCORNER][BC];
+#pragma unroll
+    for (int c = 0; c < NCORNER; ++c)
+#pragma unroll
+      for (int i = 0; i < BC; ++i) oc[c][i] = T(0);
+    if (has) cell_corner_sums(cl, cms[cl], [&](int c, int i, T v2) { oc[c][i] += v2; });
+    const int ci = cl % CXB, cj = (cl / CXB) % CYB, ck = cl / (CXB * CYB);
+#pragma unroll
+    for (int c = 0; c < NCORNER; ++c) {
+      if (c) __syncthreads();
+      if (has) {
+        const int b = (ci + (c & 1)) + XB * ((cj + ((c >> 1) & 1)) + YB * (ck + ((c >> 2) & 1)));
+#pragma unroll
+        for (int i = 0; i < BC; ++i) accv[b * BC + i] += oc[c][i];
+      }
+    }
+  } else {
+    for (int cl = threadIdx.x; cl < NCB; cl += blockDim.x) {
+      T* dst = out + (size_t)cl * (NCORNER * BC);
+      for (int k = 0; k < NCORNER * BC; ++k) dst[k] = T(0);
+      const unsigned m = cms[cl];
+      if (!m) continue;
+      cell_corner_sums(cl, m, [&](int c, int i, T v2) { dst[c * BC + i] += v2; });
     }
   }
   __syncthreads();
@@ -1770,14 +1806,16 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
     const int s = sl[vb];
     if (s < 0) continue;
     T acc = T(0);
-    if (i < BC) {
+    if (i < BC && SCATTER) {
+      acc = accv[(SCATTER ? vb : 0) * BC + i];
+    } else if (i < BC) {
       for (int ez = 0; ez < (DIM > 2 ? 2 : 1); ++ez)
         for (int ey = 0; ey < (DIM > 1 ? 2 : 1); ++ey)
           for (int ex = 0; ex < 2; ++ex) {
             const int cl = (li + ex) + CXB * ((lj + ey) + CYB * (lk + ez));
             if (!cms[cl]) continue;
             const int cv = (1 - ex) | (DIM > 1 ? (1 - ey) << 1 : 0) | (DIM > 2 ? (1 - ez) << 2 : 0);
-            acc += out[(size_t)cl * (NCORNER * BC) + cv * BC + i];
+            acc += out[(SCATTER ? 0 : (size_t)cl * (NCORNER * BC) + cv * BC) + i];
           }
     } else {
       acc = v[(size_t)s * DCG_B + i];  // padding species: identity rows
