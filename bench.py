@@ -443,6 +443,8 @@ def main():
     ap.add_argument("--workload", default="ms96")
     ap.add_argument("--preconditioner", default="Jacobi", choices=["Jacobi", "VertexBlockJacobi"],
                     help="Jacobi: the reference's (diagonal); VertexBlockJacobi: inverse of each vertex's species block")
+    ap.add_argument("--profile-steps-single", type=int, default=2,
+                    help="single models: steps after the timed ones with per-kernel HIP-event timing")
     ap.add_argument("--storage", default="auto", choices=["matrix", "points", "auto"],
                     help="single models: the Jacobian as an assembled BCSR matrix, or as point coefficients per "
                          "cell applied by the Krylov operator (linear_solver.storage)")
@@ -507,7 +509,6 @@ def main():
         t += w.dt
     print(f"# warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
 
-    m.profile_enable(True)
     torch.cuda.synchronize()
     lat, newton, lin = [], 0, 0
     t0 = time.perf_counter()
@@ -524,13 +525,27 @@ def main():
     elapsed = time.perf_counter() - t0
     print(f"# timed {args.steps} steps in {elapsed:.3f}s", file=sys.stderr, flush=True)
 
+    u_timed, _ = m.get_state()  # the state the mixed run is compared with
+    # per-kernel HIP-event timing over further steps (events around every
+    # launch; these steps are not part of the timed region)
+    m.profile_enable(True)
+    torch.cuda.synchronize()
+    p0 = time.perf_counter()
+    for k in range(args.profile_steps_single):
+        st = m.step(t, w.dt)
+        if st.status:
+            raise SystemExit(f"profiled step {k} failed with status {st.status}")
+        t += w.dt
+    torch.cuda.synchronize()
+    prof_elapsed = time.perf_counter() - p0
+
     kinds = {}
     for kname in ["spmv64", "spmv32", "assembly_jacobian", "assembly_residual", "precond_setup", "point_setup",
                   "point_apply64", "point_apply32"]:
         n, ms, b = m.profile(kname)
         if n:
             kinds[kname] = {"launches": n, "avg_us": 1e3 * ms / n, "gbs": (b / n) / (ms / n * 1e-3) / 1e9 if ms else 0.0,
-                            "bytes_per_launch": b / n, "share_of_step_time": ms / (1e3 * elapsed)}
+                            "bytes_per_launch": b / n, "share_of_step_time": ms / (1e3 * prof_elapsed)}
     # the dominant kernel: the Krylov operator (assembled SpMV or point-coefficient apply)
     apply_kinds = [k for k in ("spmv64", "spmv32", "point_apply64", "point_apply32") if k in kinds]
     spmv_kind = max(apply_kinds, key=lambda k: kinds[k]["share_of_step_time"]) if apply_kinds else None
@@ -554,7 +569,7 @@ def main():
     if args.precision == "fp64" and not args.no_mixed:
         # the FP32-Krylov / FP64-refinement form on the same workload and steps,
         # with its error against this FP64 run
-        u64, _ = m.get_state()
+        u64 = u_timed
         w32, _ = make_workload(wl, args.workload, "mixed", args.preconditioner)
         m32 = dca.Model(w32.ini, storage, 0)
         m32.set_state(u0, 0.0)

@@ -1614,16 +1614,20 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
 // by the point's coefficient block and tested against the basis functions,
 // the uniform diffusion adds the element stiffness times v; the cells' corner
 // sums are folded onto the tile's rows as in the residual.  T is the value
-// type of pc, v and y (double, or float inside the refinement loop).  g2l
-// null: the compartment's slots are the grid's vertices in order.  With part,
+// type of pc, v and y (double, or float inside the refinement loop).  tca:
+// the time and stiffness weight of the linearisation, written to device
+// memory by the set-up, so an apply's arguments are the same for every
+// linearisation.  g2l null: the
+// compartment's slots are the grid's vertices in order.  With part,
 // the workgroup also writes its rows' partial dots (y.w1, y.w2) to
 // part[2 blockIdx.x + 0/1] (w2 may be null), as the BiCGSTAB finaliser reads.
 template <typename T>
 __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,
-                                         const T* __restrict__ pc, const T* __restrict__ v, T* __restrict__ y, double t,
-                                         double ca, int n0, int n1, int n2, const double* __restrict__ prm,
-                                         const T* __restrict__ w1, const T* __restrict__ w2,
-                                         double* __restrict__ part) {
+                                         const T* __restrict__ pc, const T* __restrict__ v, T* __restrict__ y,
+                                         const double* __restrict__ tca, int n0, int n1, int n2,
+                                         const double* __restrict__ prm, const T* __restrict__ w1,
+                                         const T* __restrict__ w2, double* __restrict__ part) {
+  const double t = tca[0], ca = tca[1];
   // corner values stay in registers for small blocks, in LDS otherwise; with
   // small blocks and one cell per thread the corner sums go straight onto
   // the vertices (SCATTER), otherwise through a per-cell corner array
@@ -1640,16 +1644,6 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
   const int vi0 = tx * TX - 1, vj0 = DIM > 1 ? ty * TY - 1 : 0, vk0 = DIM > 2 ? tz * TZ - 1 : 0;
   // this tile's block of coefficients: [(p*NQ + q)*NMC + n][box cell]
   const T* pcb = pc + (size_t)blockIdx.x * MC_WORDS;
-  // one cell per thread and few coefficients: the thread's coefficients are
-  // loaded before the staging barrier, so they arrive while v is staged
-  constexpr int PW = NCLASS * NQ * NMC;
-  constexpr bool PREFETCH = NMC > 0 && PW <= 32 && NCB <= 256;
-  T pre[PREFETCH ? PW : 1];
-  if constexpr (PREFETCH) {
-    if (threadIdx.x < NCB)
-#pragma unroll
-      for (int k = 0; k < PW; ++k) pre[k] = pcb[(size_t)k * NCB + threadIdx.x];
-  }
   for (int b = threadIdx.x; b < NVB; b += blockDim.x) {
     const int bi = b % XB, bj = (b / XB) % YB, bk = b / (XB * YB);
     const int gi = vi0 + bi, gj = vj0 + bj, gk = vk0 + bk;
@@ -1706,12 +1700,7 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
 #pragma unroll
         for (int i = 0; i < BC; ++i) w[i] = T(0);
 #pragma unroll
-        for (int n = 0; n < NMC; ++n) {
-          T mv;
-          if constexpr (PREFETCH) mv = pre[PREFETCH ? (p * NQ + q) * NMC + n : 0];
-          else mv = mp[(size_t)n * NCB];
-          w[MCI[n]] += mv * vq[MCJ[n]];
-        }
+        for (int n = 0; n < NMC; ++n) w[MCI[n]] += mp[(size_t)n * NCB] * vq[MCJ[n]];
 #pragma unroll
         for (int a = 0; a < NV; ++a)
 #pragma unroll
@@ -1889,11 +1878,11 @@ extern "C" __global__ void __launch_bounds__(256)
 #define DCG_POINT_ENTRY(NAME, T)                                                                                  \
   extern "C" __global__ void __launch_bounds__(256)                                                               \
   NAME(const int* __restrict__ tiles, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,         \
-       const T* __restrict__ pc, const T* __restrict__ v, T* __restrict__ y, double t, double ca, int n0, int n1,  \
-       int n2, const double* __restrict__ prm, const int* __restrict__ stop, const T* __restrict__ w1,            \
-       const T* __restrict__ w2, double* __restrict__ part) {                                                     \
+       const T* __restrict__ pc, const T* __restrict__ v, T* __restrict__ y, const double* __restrict__ tca,       \
+       int n0, int n1, int n2, const double* __restrict__ prm, const int* __restrict__ stop,                      \
+       const T* __restrict__ w1, const T* __restrict__ w2, double* __restrict__ part) {                           \
     if (stop && *stop) return;                                                                                    \
-    %C%::pc_apply<T>(tiles[blockIdx.x], g2l, cmask, pc, v, y, t, ca, n0, n1, n2, prm, w1, w2, part);              \
+    %C%::pc_apply<T>(tiles[blockIdx.x], g2l, cmask, pc, v, y, tca, n0, n1, n2, prm, w1, w2, part);                \
   }
 DCG_POINT_ENTRY(%KERNEL%_p, double)
 DCG_POINT_ENTRY(%KERNEL%_pf, float)

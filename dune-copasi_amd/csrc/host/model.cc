@@ -60,12 +60,10 @@ void Model::attach(int device, hipStream_t stream) {
 }
 
 Model::~Model() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto& p : prof_data_)
     for (auto e : p.pool) (void)hipEventDestroy(e);
-  if (stream_) {
-    (void)hipStreamSynchronize(stream_);
-    if (own_stream_) (void)hipStreamDestroy(stream_);
-  }
+  if (stream_ && own_stream_) (void)hipStreamDestroy(stream_);
 }
 
 void Model::build(const std::string& ini, const std::string& overrides) {
@@ -426,6 +424,7 @@ void Model::setup_device() {
       if (cfg_.precision == Precision::Mixed) pc32_[c].resize(words);
     }
     pdiag_.resize(static_cast<size_t>(ns) * B * B);
+    pc_tca_.resize(2);
     dinv_.resize(static_cast<size_t>(ns) * B * B);
     std::vector<int> id(ns);
     for (long i = 0; i < ns; ++i) id[i] = static_cast<int>(i);
@@ -729,8 +728,8 @@ void Model::launch_points_setup(const double* x, double t, double cm, double ca)
                     &n0, &n1, &n2, &ox, &oy, &oz, &hx, &hy, &hz, &mode, &prm, &stop};
     DCG_HIP(hipModuleLaunchKernel(pc_set_fn_[c], comp_dev_[c].ntiles, 1, 1, 256, 1, 1, 0, stream_, args, nullptr));
   }
-  pc_t_ = t;
-  pc_ca_ = ca;
+  k::fill(pc_tca_.get(), t, 1, stream_);
+  k::fill(pc_tca_.get() + 1, ca, 1, stream_);
 }
 
 // y = J v from the stored point coefficients: every compartment's tiles write
@@ -742,7 +741,7 @@ int Model::launch_points_apply(const void* v, void* y, const int* stop, bool fp3
   int n0 = static_cast<int>(g.n[0]), n1 = g.dim > 1 ? static_cast<int>(g.n[1]) : 1;
   int n2 = g.dim > 2 ? static_cast<int>(g.n[2]) : 1;
   const double* prm = prm_.get();
-  double t = pc_t_, ca = pc_ca_;
+  const double* tca = pc_tca_.get();
   int nparts = 0;
   for (size_t c = 0; c < L_.comps.size(); ++c) {
     if (!pc_app_fn_[c] || comp_dev_[c].ntiles <= 0) continue;
@@ -751,7 +750,7 @@ int Model::launch_points_apply(const void* v, void* y, const int* stop, bool fp3
     const unsigned char* cmask = comp_dev_[c].cell_mask_grid.get();
     const void* pc = fp32 ? static_cast<const void*>(pc32_[c].get()) : static_cast<const void*>(pc_[c].get());
     double* pp = part ? part + 2 * static_cast<size_t>(nparts) : nullptr;
-    void* args[] = {&tiles, &g2l, &cmask, &pc, &v, &y, &t, &ca, &n0, &n1, &n2, &prm, &stop, &w1, &w2, &pp};
+    void* args[] = {&tiles, &g2l, &cmask, &pc, &v, &y, &tca, &n0, &n1, &n2, &prm, &stop, &w1, &w2, &pp};
     DCG_HIP(hipModuleLaunchKernel(fp32 ? pc_appf_fn_[c] : pc_app_fn_[c], comp_dev_[c].ntiles, 1, 1, 256, 1, 1, 0,
                                   stream_, args, nullptr));
     nparts += comp_dev_[c].ntiles;
@@ -843,19 +842,20 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
     k::bicg_fin(plan_.parts(), part_.get(), 2, stage, sc_.get(), stream_);
     prof_end(kSpmv64, spmv_bytes(8, 8, 1));
   };
+  auto iteration = [&]() {
+    k::bicg_p<double>(B, ns, r, p, v, ph, dinv, sc_.get(), stream_);
+    if (ilu) apply_ilu(p, ph);
+    matvec(ph, v, rh, nullptr, 1);
+    k::bicg_s<double>(B, ns, r, v, s, sh, dinv, sc_.get(), part_.get(), stream_);
+    if (ilu) apply_ilu(s, sh);
+    k::bicg_fin(nblk, part_.get(), 1, 2, sc_.get(), stream_);
+    matvec(sh, t, s, t, 3);
+    k::bicg_xr<double>(B, ns, x, ph, sh, s, t, r, rh, sc_.get(), part_.get(), stream_);
+    k::bicg_fin(nblk, part_.get(), 2, 4, sc_.get(), stream_);
+  };
   int done_its = 0, batch = std::max(2, std::min(last_krylov_its_, 64));
   for (;;) {
-    for (int it = 0; it < batch; ++it) {
-      k::bicg_p<double>(B, ns, r, p, v, ph, dinv, sc_.get(), stream_);
-      if (ilu) apply_ilu(p, ph);
-      matvec(ph, v, rh, nullptr, 1);
-      k::bicg_s<double>(B, ns, r, v, s, sh, dinv, sc_.get(), part_.get(), stream_);
-      if (ilu) apply_ilu(s, sh);
-      k::bicg_fin(nblk, part_.get(), 1, 2, sc_.get(), stream_);
-      matvec(sh, t, s, t, 3);
-      k::bicg_xr<double>(B, ns, x, ph, sh, s, t, r, rh, sc_.get(), part_.get(), stream_);
-      k::bicg_fin(nblk, part_.get(), 2, 4, sc_.get(), stream_);
-    }
+    for (int it = 0; it < batch; ++it) iteration();
     done_its += batch;
     DCG_HIP(hipMemcpyAsync(&host, sc_.get(), sizeof host, hipMemcpyDeviceToHost, stream_));
     synchronize();
@@ -910,19 +910,20 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
     k::bicg_fin(plan32_.parts(), part_.get(), 2, stage, sc_.get(), stream_);
     prof_end(kSpmv32, spmv_bytes(4, 4, 1));
   };
+  auto iteration = [&]() {
+    k::bicg_p<float>(B, ns, r, p, v, ph, dinv, sc_.get(), stream_);
+    if (ilu) apply_ilu(p, ph);
+    matvec(ph, v, rh, nullptr, 1);
+    k::bicg_s<float>(B, ns, r, v, s, sh, dinv, sc_.get(), part_.get(), stream_);
+    if (ilu) apply_ilu(s, sh);
+    k::bicg_fin(nblk, part_.get(), 1, 2, sc_.get(), stream_);
+    matvec(sh, t, s, t, 3);
+    k::bicg_xr<float>(B, ns, x, ph, sh, s, t, r, rh, sc_.get(), part_.get(), stream_);
+    k::bicg_fin(nblk, part_.get(), 2, 4, sc_.get(), stream_);
+  };
   int done_its = 0, batch = std::max(2, std::min(last_krylov_its_, 64));
   for (;;) {
-    for (int it = 0; it < batch; ++it) {
-      k::bicg_p<float>(B, ns, r, p, v, ph, dinv, sc_.get(), stream_);
-      if (ilu) apply_ilu(p, ph);
-      matvec(ph, v, rh, nullptr, 1);
-      k::bicg_s<float>(B, ns, r, v, s, sh, dinv, sc_.get(), part_.get(), stream_);
-      if (ilu) apply_ilu(s, sh);
-      k::bicg_fin(nblk, part_.get(), 1, 2, sc_.get(), stream_);
-      matvec(sh, t, s, t, 3);
-      k::bicg_xr<float>(B, ns, x, ph, sh, s, t, r, rh, sc_.get(), part_.get(), stream_);
-      k::bicg_fin(nblk, part_.get(), 2, 4, sc_.get(), stream_);
-    }
+    for (int it = 0; it < batch; ++it) iteration();
     done_its += batch;
     DCG_HIP(hipMemcpyAsync(&host, sc_.get(), sizeof host, hipMemcpyDeviceToHost, stream_));
     synchronize();

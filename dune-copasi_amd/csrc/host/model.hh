@@ -292,7 +292,7 @@ class Model {
   std::vector<DeviceArray<float>> pc32_;
   DeviceArray<double> pdiag_;
   DeviceArray<int> diag_id_;
-  double pc_t_ = 0.0, pc_ca_ = 0.0;
+  DeviceArray<double> pc_tca_;  // (t, ca) of the point coefficients, read by the applies
   bool use_points_ = false;  // the Krylov operator is the point-coefficient one
   std::vector<DeviceArray<double>> stage_;
   DeviceArray<double> val_, lu_, dinv_;
