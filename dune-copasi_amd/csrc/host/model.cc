@@ -150,7 +150,7 @@ void Model::build(const std::string& ini, const std::string& overrides) {
   // keeps only the coefficient blocks of every cell's quadrature points and
   // applies them (with the uniform diffusion) to each Krylov direction, which
   // is the same operator in fewer bytes; "auto" takes points on box grids
-  // where the model allows it.  Points need no interface or outflow terms
+  // with blocks of up to 4 species where the model allows it.  Points need no interface or outflow terms
   // and a preconditioner built from diagonal blocks (Jacobi, vertex-block
   // Jacobi or none).
   {
@@ -164,7 +164,11 @@ void Model::build(const std::string& ini, const std::string& overrides) {
     else if (!L_.facets.empty()) why = "interface or outflow terms";
     else if (!all_tile_eligible()) why = "compartments outside the tiled kernel";
     if (st == "points" && !why.empty()) throw NotImplemented("linear_solver.storage = points with " + why);
-    cfg_.points = why.empty() && (st == "points" || (st == "auto" && L_.grid.cube));
+    // auto: box cells, and blocks of up to 4 species (larger blocks keep their
+    // corner sums in LDS, where the assembled SpMV is still faster)
+    int bmax = 0;
+    for (const auto& C : L_.comps) bmax = std::max(bmax, C.b);
+    cfg_.points = why.empty() && (st == "points" || (st == "auto" && L_.grid.cube && bmax <= 4));
   }
   cfg_.lin_rtol = c.get(p + "linear_solver.convergence_condition.relative_tolerance", 1e-4);
   auto lr = c.get_list(p + "linear_solver.convergence_condition.iteration_range");
