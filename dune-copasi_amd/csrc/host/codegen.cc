@@ -486,14 +486,18 @@ __device__ __forceinline__ void facet_residual(const double (&xi)[NVF][BI > 0 ? 
 %QPBODY%
   }
 }
-}  // namespace %F%
 
-extern "C" __global__ void __launch_bounds__(64)
-%KERNEL%(const int* __restrict__ fint, const double* __restrict__ fdbl, int first, int count,
-         const double* __restrict__ x, double t, double ca, double* __restrict__ r, double* __restrict__ val,
-         int mode, const double* __restrict__ prm, const double* __restrict__ dir) {
-  // mode 4: r += J(x) dir over the facet blocks (the matrix-free apply)
-  using namespace %F%;
+// mode 1: r += facet residual; 2: facet Jacobian blocks into val (BCSR
+// positions); 4: r += J(x) dir over the facet blocks (the Jacobian apply);
+// 8: the diagonal blocks of J(x) (Jii and Joo with f = g) added to
+// val[slot * B * B], the rows' diagonal blocks of the point-coefficient
+// Jacobian.  V is the type of r and dir (FP32 for the apply inside the
+// refinement loop).
+template <typename V>
+__device__ __forceinline__ void facet_body(const int* __restrict__ fint, const double* __restrict__ fdbl, int first,
+                                           int count, const double* __restrict__ x, double t, double ca,
+                                           V* __restrict__ r, double* __restrict__ val, int mode,
+                                           const double* __restrict__ prm, const V* __restrict__ dir) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= count) return;
   const int id = first + tid;
@@ -597,6 +601,24 @@ extern "C" __global__ void __launch_bounds__(64)
       }
     }
   }
+  if (mode & 8) {
+#pragma unroll
+    for (int f = 0; f < NVF; ++f) {
+      double* d0 = val + (size_t)fi[f] * (DCG_B * DCG_B);
+#pragma unroll
+      for (int i = 0; i < BI; ++i)
+#pragma unroll
+        for (int j = 0; j < BI; ++j) d0[i * DCG_B + j] += Jii[((f * NVF + f) * BI + i) * BI + j];
+      if (BO > 0) {
+        double* d3 = val + (size_t)fi[NVF + f] * (DCG_B * DCG_B);
+#pragma unroll
+        for (int i = 0; i < BO; ++i)
+#pragma unroll
+          for (int j = 0; j < BO; ++j)
+            d3[i * DCG_B + j] += Joo[((f * NVF + f) * (BO > 0 ? BO : 1) + i) * (BO > 0 ? BO : 1) + j];
+      }
+    }
+  }
   if (want_j) {
     const int* pos = fi + 2 * NVF;
 #pragma unroll
@@ -637,6 +659,21 @@ extern "C" __global__ void __launch_bounds__(64)
       for (int i = 0; i < BO; ++i) r[(size_t)fi[NVF + f] * DCG_B + i] += ro[f][i];
     }
   }
+}
+}  // namespace %F%
+
+extern "C" __global__ void __launch_bounds__(64)
+%KERNEL%(const int* __restrict__ fint, const double* __restrict__ fdbl, int first, int count,
+         const double* __restrict__ x, double t, double ca, double* __restrict__ r, double* __restrict__ val,
+         int mode, const double* __restrict__ prm, const double* __restrict__ dir) {
+  %F%::facet_body<double>(fint, fdbl, first, count, x, t, ca, r, val, mode, prm, dir);
+}
+// the FP32 apply (mode 4) of the refinement loop
+extern "C" __global__ void __launch_bounds__(64)
+%KERNEL%_f(const int* __restrict__ fint, const double* __restrict__ fdbl, int first, int count,
+           const double* __restrict__ x, double t, double ca, float* __restrict__ r, double* __restrict__ val,
+           int mode, const double* __restrict__ prm, const float* __restrict__ dir) {
+  %F%::facet_body<float>(fint, fdbl, first, count, x, t, ca, r, val, mode, prm, dir);
 }
 )";
 

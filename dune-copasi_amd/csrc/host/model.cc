@@ -150,9 +150,10 @@ void Model::build(const std::string& ini, const std::string& overrides) {
   // keeps only the coefficient blocks of every cell's quadrature points and
   // applies them (with the uniform diffusion) to each Krylov direction, which
   // is the same operator in fewer bytes; "auto" takes points on box grids
-  // with blocks of up to 4 species where the model allows it.  Points need no interface or outflow terms
-  // and a preconditioner built from diagonal blocks (Jacobi, vertex-block
-  // Jacobi or none).
+  // with blocks of up to 4 species where the model allows it.  Points need
+  // compartments on the tiled kernel and a preconditioner built from diagonal
+  // blocks (Jacobi, vertex-block Jacobi or none); interface and outflow terms
+  // are applied from the linearisation point by the facet kernels.
   {
     const std::string st = c.get(p + "linear_solver.storage", std::string("auto"));
     if (st != "matrix" && st != "points" && st != "auto")
@@ -161,7 +162,6 @@ void Model::build(const std::string& ini, const std::string& overrides) {
     if (cfg_.matrix_free) why = "the matrix-free Jacobian";
     else if (opt_.batch_member) why = "batch members";
     else if (cfg_.precond == Preconditioner::ILU0) why = "an ILU0 preconditioner";
-    else if (!L_.facets.empty()) why = "interface or outflow terms";
     else if (!all_tile_eligible()) why = "compartments outside the tiled kernel";
     if (st == "points" && !why.empty()) throw NotImplemented("linear_solver.storage = points with " + why);
     // auto: box cells, and blocks of up to 4 species (larger blocks keep their
@@ -429,6 +429,7 @@ void Model::setup_device() {
     }
     pdiag_.resize(static_cast<size_t>(ns) * B * B);
     pc_tca_.resize(2);
+    if (!L_.facets.empty()) lin_x_.resize(nv);  // the facet terms apply from the linearisation point
     dinv_.resize(static_cast<size_t>(ns) * B * B);
     std::vector<int> id(ns);
     for (long i = 0; i < ns; ++i) id[i] = static_cast<int>(i);
@@ -581,7 +582,11 @@ void Model::compile_and_load() {
     }
   }
   facet_fn_.clear();
-  for (const auto& fk : gen_.facet_kernels) facet_fn_.push_back(module_->function(fk.name));
+  facetf_fn_.clear();
+  for (const auto& fk : gen_.facet_kernels) {
+    facet_fn_.push_back(module_->function(fk.name));
+    facetf_fn_.push_back(cfg_.points ? module_->function(fk.name + "_f") : nullptr);
+  }
 }
 
 // ------------------------------------------------------------------ assembly
@@ -734,11 +739,33 @@ void Model::launch_points_setup(const double* x, double t, double cm, double ca)
   }
   k::fill(pc_tca_.get(), t, 1, stream_);
   k::fill(pc_tca_.get() + 1, ca, 1, stream_);
+  if (facet_fn_.empty()) return;
+  // interface and outflow terms: their diagonal blocks join the rows' ones,
+  // and the applies evaluate them at the linearisation point (a copy of x)
+  DCG_HIP(hipMemcpyAsync(lin_x_.get(), x, lin_x_.bytes(), hipMemcpyDeviceToDevice, stream_));
+  lin_t_ = t;
+  lin_ca_ = ca;
+  if (ca == 0.0) return;  // they belong to the stiffness operator
+  const double* lin = lin_x_.get();
+  double* r = nullptr;
+  const double* dir = nullptr;
+  int fmode = 8;
+  for (size_t k2 = 0; k2 < facet_fn_.size(); ++k2) {
+    const int* fint = facet_dev_[k2].fint.get();
+    const double* fdbl = facet_dev_[k2].fdbl.get();
+    for (auto [first, count] : facet_dev_[k2].color_ranges) {
+      void* fargs[] = {&fint, &fdbl, &first, &count, &lin, &t, &ca, &r, &diag, &fmode, &prm, &dir};
+      const unsigned block = 64, grid = static_cast<unsigned>((count + block - 1) / block);
+      DCG_HIP(hipModuleLaunchKernel(facet_fn_[k2], grid, 1, 1, block, 1, 1, 0, stream_, fargs, nullptr));
+    }
+  }
 }
 
 // y = J v from the stored point coefficients: every compartment's tiles write
 // the rows they own (padding species rows are the identity) and, with part,
-// one pair of partial dots each.
+// one pair of partial dots each; interface and outflow terms are then added
+// by the facet kernels from the linearisation point, and no partial dots are
+// written (the returned count is 0: the caller forms the dots).
 int Model::launch_points_apply(const void* v, void* y, const int* stop, bool fp32, const void* w1, const void* w2,
                                double* part) {
   const Grid& g = L_.grid;
@@ -753,13 +780,30 @@ int Model::launch_points_apply(const void* v, void* y, const int* stop, bool fp3
     const int* g2l = comp_dev_[c].g2l_identity ? nullptr : comp_dev_[c].g2l.get();
     const unsigned char* cmask = comp_dev_[c].cell_mask_grid.get();
     const void* pc = fp32 ? static_cast<const void*>(pc32_[c].get()) : static_cast<const void*>(pc_[c].get());
-    double* pp = part ? part + 2 * static_cast<size_t>(nparts) : nullptr;
+    double* pp = part && facet_fn_.empty() ? part + 2 * static_cast<size_t>(nparts) : nullptr;
     void* args[] = {&tiles, &g2l, &cmask, &pc, &v, &y, &tca, &n0, &n1, &n2, &prm, &stop, &w1, &w2, &pp};
     DCG_HIP(hipModuleLaunchKernel(fp32 ? pc_appf_fn_[c] : pc_app_fn_[c], comp_dev_[c].ntiles, 1, 1, 256, 1, 1, 0,
                                   stream_, args, nullptr));
     nparts += comp_dev_[c].ntiles;
   }
-  return nparts;
+  if (facet_fn_.empty()) return nparts;
+  if (lin_ca_ != 0.0) {
+    const double* lin = lin_x_.get();
+    double t2 = lin_t_, ca2 = lin_ca_;
+    double* val = nullptr;
+    int fmode = 4;
+    for (size_t k2 = 0; k2 < facet_fn_.size(); ++k2) {
+      const int* fint = facet_dev_[k2].fint.get();
+      const double* fdbl = facet_dev_[k2].fdbl.get();
+      for (auto [first, count] : facet_dev_[k2].color_ranges) {
+        void* fargs[] = {&fint, &fdbl, &first, &count, &lin, &t2, &ca2, &y, &val, &fmode, &prm, &v};
+        const unsigned block = 64, grid = static_cast<unsigned>((count + block - 1) / block);
+        DCG_HIP(hipModuleLaunchKernel(fp32 ? facetf_fn_[k2] : facet_fn_[k2], grid, 1, 1, block, 1, 1, 0, stream_,
+                                      fargs, nullptr));
+      }
+    }
+  }
+  return 0;
 }
 
 void Model::ensure_matrix() {
@@ -838,7 +882,8 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
       prof_begin(kPointApply64);
       const int np = launch_points_apply(in, out, done_flag, false, w1, w2, part_.get());
       prof_end(kPointApply64, point_apply_bytes(8));
-      k::bicg_fin(np, part_.get(), 2, stage, sc_.get(), stream_);
+      if (!np) k::dot_parts(n, out, w1, w2, part_.get(), nred, sc_.get(), stream_);
+      k::bicg_fin(np ? np : nred, part_.get(), 2, stage, sc_.get(), stream_);
       return;
     }
     prof_begin(kSpmv64);
@@ -906,7 +951,8 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
       prof_begin(kPointApply32);
       const int np = launch_points_apply(in, out, done_flag, true, w1, w2, part_.get());
       prof_end(kPointApply32, point_apply_bytes(4));
-      k::bicg_fin(np, part_.get(), 2, stage, sc_.get(), stream_);
+      if (!np) k::dot_parts(n, out, w1, w2, part_.get(), nred, sc_.get(), stream_);
+      k::bicg_fin(np ? np : nred, part_.get(), 2, stage, sc_.get(), stream_);
       return;
     }
     prof_begin(kSpmv32);

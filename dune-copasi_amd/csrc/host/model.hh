@@ -248,6 +248,7 @@ class Model {
   // point coefficients (_q), their FP64 (_p) and FP32 (_pf) apply
   std::vector<hipFunction_t> pc_set_fn_, pc_app_fn_, pc_appf_fn_;
   std::vector<hipFunction_t> facet_fn_;
+  std::vector<hipFunction_t> facetf_fn_;  // FP32 apply entries (point storage)
   double time_ = 0.0, dt_initial_ = 0.0, t_end_ = 0.0;
   std::map<std::string, std::vector<double>> vtk_steps_;  // times written per output path
   int last_krylov_its_ = 8;

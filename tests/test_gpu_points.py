@@ -9,7 +9,7 @@ operator of the assembled Jacobian with its sums in another order.  Checks:
 * y = J(x) v against the oracle's assembled Jacobian times v and against the
   device's own assembled Jacobian (1e-12 relative: summation order only), on
   box and simplex cells, 2D and 3D, blocks of 2 to 8 species, two
-  compartments without interface terms;
+  compartments with and without interface terms;
 * time steps against the oracle (the parity bar) with the same Newton
   iterations and BiCGSTAB iterations within a few percent (summation order),
   for Jacobi and vertex-block Jacobi, FP64 and FP32 Krylov inside FP64
@@ -41,6 +41,10 @@ APPLY_CASES = [
     ("ms_q1_odd", lambda wl: wl.mitchell_schaefer(n=11, cell_type="cube")),
     ("ms_p1", lambda wl: wl.mitchell_schaefer(n=5, cell_type="simplex")),
     ("two_q1_no_interface", lambda wl: _no_interface(wl.two_compartments(n=14, cell_type="cube"))),
+    # interface (transmission) terms applied by the facet kernels
+    ("two_q1", lambda wl: wl.two_compartments(n=14, cell_type="cube")),
+    ("two_p1", lambda wl: wl.two_compartments(n=10, cell_type="simplex")),
+    ("req_3d_8sp_2c", lambda wl: wl.random_request(4, side3=(5, 6), steps_range=(3, 3))),
     # blocks above 4 x 4 (corner values read from LDS)
     ("req_2d_6sp_1c", lambda wl: wl.random_request(11, side2=(10, 12), steps_range=(3, 3))),
     ("req_3d_8sp_1c", lambda wl: wl.random_request(18, side3=(5, 6), steps_range=(3, 3))),
@@ -104,6 +108,8 @@ STEP_CASES = [
     ("gs_p1", lambda wl: wl.gray_scott(n=20, cell_type="simplex", nl_rtol=1e-20, nl_atol=1e-26, lin_rtol=1e-12), 3),
     ("two_q1_no_interface", lambda wl: _no_interface(wl.two_compartments(n=16, cell_type="cube", nl_rtol=1e-20,
                                                                          nl_atol=1e-26, lin_rtol=1e-12)), 3),
+    ("two_q1", lambda wl: wl.two_compartments(n=16, cell_type="cube", nl_rtol=1e-20, nl_atol=1e-26,
+                                              lin_rtol=1e-12), 3),
     ("req_3d_8sp_1c", lambda wl: wl.random_request(18, side3=(6, 7), nl_rtol=1e-20, nl_atol=1e-28,
                                                    precision="fp64"), 3),
 ]
@@ -121,10 +127,15 @@ def test_point_steps_match_oracle(dca, wl, name, make, steps, precond):
     _step_pair(dca, w, pre.format(precond), steps, pre.format("Jacobi"))
 
 
-def test_point_steps_mixed_precision(dca, wl):
-    """FP32 point-coefficient applies inside FP64 refinement: the states meet
-    the parity bar against the oracle's FP64 steps."""
-    w = wl.mitchell_schaefer(n=10, cell_type="cube", nl_rtol=1e-20, nl_atol=1e-26, lin_rtol=1e-12)
+@pytest.mark.parametrize("make", [
+    lambda wl: wl.mitchell_schaefer(n=10, cell_type="cube", nl_rtol=1e-20, nl_atol=1e-26, lin_rtol=1e-12),
+    lambda wl: wl.two_compartments(n=16, cell_type="cube", nl_rtol=1e-20, nl_atol=1e-26, lin_rtol=1e-12),
+], ids=["ms_q1", "two_q1"])
+def test_point_steps_mixed_precision(dca, wl, make):
+    """FP32 point-coefficient applies (with FP32 facet applies for interface
+    terms) inside FP64 refinement: the states meet the parity bar against the
+    oracle's FP64 steps."""
+    w = make(wl)
     mix = "model.time_step_operator.linear_solver.precision = mixed\n"
     gpu = dca.Model(w.ini, PTS + mix, 0)
     orc = OracleModel(w.ini)

@@ -156,8 +156,8 @@ initial.expression = {text}
      "[model.scalar_field.u]\ncompartment = d\nstorage.expression = 1\nvelocity.x.expression = 0.1\n"
      "[model.time_step_operator]\nlinear_solver.matrix_free = true\n",
      "with the tiled kernel"),
-    # point-coefficient storage: a preconditioner from diagonal blocks, no
-    # interface / outflow terms, compartments of the tiled kernel
+    # point-coefficient storage: a preconditioner from diagonal blocks,
+    # compartments of the tiled kernel
     ("[grid]\ndimension = 2\ncells = 4 4\ncell_type = cube\n[compartments]\nd.expression = 1\n"
      "[model.scalar_field.u]\ncompartment = d\nstorage.expression = 1\n"
      "[model.time_step_operator]\nlinear_solver.storage = points\nlinear_solver.preconditioner.type = ILU\n",
@@ -166,10 +166,6 @@ initial.expression = {text}
      "[model.scalar_field.u]\ncompartment = d\nstorage.expression = 1\nvelocity.x.expression = 0.1\n"
      "[model.time_step_operator]\nlinear_solver.storage = points\nlinear_solver.preconditioner.type = Jacobi\n",
      "storage = points with compartments outside the tiled kernel"),
-    ("[grid]\ndimension = 2\ncells = 4 4\ncell_type = cube\n[compartments]\nd.expression = 1\n"
-     "[model.scalar_field.u]\ncompartment = d\nstorage.expression = 1\noutflow.d.expression = u\n"
-     "[model.time_step_operator]\nlinear_solver.storage = points\nlinear_solver.preconditioner.type = Jacobi\n",
-     "storage = points with interface or outflow terms"),
     ("[grid]\ndimension = 2\ncells = 4 4\ncell_type = cube\n[compartments]\nd.expression = 1\n"
      "[model.scalar_field.u]\ncompartment = d\nstorage.expression = 1\n"
      "[model.time_step_operator]\nlinear_solver.storage = csr\n",
