@@ -13,27 +13,61 @@ namespace dcopasi {
 // 408-443); coefficients are the textbook ones of each scheme.
 RkTable rk_table(const std::string& type) {
   RkTable t;
-  if (type == "ImplicitEuler") {
-    t.stages = 1;
-    t.a[0][0] = -1.0;
-    t.a[0][1] = 1.0;
-    t.b[0][1] = 1.0;
-    t.d[1] = 1.0;
+  // rows of a and b for stage r (1-based) from lists; d from a list
+  auto set = [&](int stages, std::vector<std::vector<double>> a, std::vector<std::vector<double>> b,
+                 std::vector<double> d) {
+    t.stages = stages;
+    for (int r = 0; r < stages; ++r)
+      for (int j = 0; j <= stages; ++j) {
+        t.a[r][j] = a[r][j];
+        t.b[r][j] = b[r][j];
+      }
+    for (int j = 0; j <= stages; ++j) t.d[j] = d[j];
+  };
+  if (type == "ExplicitEuler") {
+    set(1, {{-1.0, 1.0}}, {{1.0, 0.0}}, {0.0, 1.0});
+  } else if (type == "ImplicitEuler") {
+    set(1, {{-1.0, 1.0}}, {{0.0, 1.0}}, {0.0, 1.0});
+  } else if (type == "Heun") {
+    // explicit trapezoidal rule in Shu-Osher form: u1 = u0 + dt L(u0),
+    // u2 = (u0 + u1 + dt L(u1)) / 2
+    set(2, {{-1.0, 1.0, 0.0}, {-0.5, -0.5, 1.0}}, {{1.0, 0.0, 0.0}, {0.0, 0.5, 0.0}}, {0.0, 1.0, 1.0});
+  } else if (type == "Shu3") {
+    // Shu and Osher (1988) third-order strong-stability-preserving scheme
+    set(3, {{-1.0, 1.0, 0.0, 0.0}, {-0.75, -0.25, 1.0, 0.0}, {-1.0 / 3.0, 0.0, -2.0 / 3.0, 1.0}},
+        {{1.0, 0.0, 0.0, 0.0}, {0.0, 0.25, 0.0, 0.0}, {0.0, 0.0, 2.0 / 3.0, 0.0}}, {0.0, 1.0, 0.5, 1.0});
+  } else if (type == "RungeKutta4") {
+    // the classical fourth-order scheme, stage values u_r = u0 + c_{r+1} dt k_r
+    set(4,
+        {{-1.0, 1.0, 0.0, 0.0, 0.0}, {-1.0, 0.0, 1.0, 0.0, 0.0}, {-1.0, 0.0, 0.0, 1.0, 0.0},
+         {-1.0, 0.0, 0.0, 0.0, 1.0}},
+        {{0.5, 0.0, 0.0, 0.0, 0.0}, {0.0, 0.5, 0.0, 0.0, 0.0}, {0.0, 0.0, 1.0, 0.0, 0.0},
+         {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0, 0.0}},
+        {0.0, 0.5, 0.5, 1.0, 1.0});
   } else if (type == "Alexander2") {
     // Alexander (1977): 2-stage, L-stable SDIRK, alpha = 1 - sqrt(2)/2
     const double al = 1.0 - std::sqrt(2.0) / 2.0;
-    t.stages = 2;
-    t.a[0][0] = -1.0;
-    t.a[0][1] = 1.0;
-    t.b[0][1] = al;
-    t.a[1][0] = -1.0;
-    t.a[1][2] = 1.0;
-    t.b[1][1] = 1.0 - al;
-    t.b[1][2] = al;
-    t.d[1] = al;
-    t.d[2] = 1.0;
+    set(2, {{-1.0, 1.0, 0.0}, {-1.0, 0.0, 1.0}}, {{0.0, al, 0.0}, {0.0, 1.0 - al, al}}, {0.0, al, 1.0});
+  } else if (type == "FractionalStepTheta") {
+    // Glowinski's fractional-step theta scheme: theta = 1 - sqrt(2)/2, steps
+    // theta, 1 - 2 theta, theta with implicit weights alpha, beta, alpha
+    const double th = 1.0 - std::sqrt(2.0) / 2.0, thp = 1.0 - 2.0 * th;
+    const double al = 2.0 - std::sqrt(2.0), be = 1.0 - al;
+    set(3, {{-1.0, 1.0, 0.0, 0.0}, {0.0, -1.0, 1.0, 0.0}, {0.0, 0.0, -1.0, 1.0}},
+        {{be * th, al * th, 0.0, 0.0}, {0.0, al * thp, be * thp, 0.0}, {0.0, 0.0, be * th, al * th}},
+        {0.0, th, 1.0 - th, 1.0});
+  } else if (type == "Alexander3") {
+    // Alexander (1977): 3-stage, third-order, L-stable SDIRK; alpha is the
+    // root of x^3 - 3x^2 + 3x/2 - 1/6 in (1/6, 1/2)
+    double al = 0.4358665215;
+    for (int it = 0; it < 8; ++it)
+      al -= (((al - 3.0) * al + 1.5) * al - 1.0 / 6.0) / ((3.0 * al - 6.0) * al + 1.5);
+    const double tau2 = (1.0 + al) / 2.0;
+    const double b1 = -(6.0 * al * al - 16.0 * al + 1.0) / 4.0, b2 = (6.0 * al * al - 20.0 * al + 5.0) / 4.0;
+    set(3, {{-1.0, 1.0, 0.0, 0.0}, {-1.0, 0.0, 1.0, 0.0}, {-1.0, 0.0, 0.0, 1.0}},
+        {{0.0, al, 0.0, 0.0}, {0.0, tau2 - al, al, 0.0}, {0.0, b1, b2, al}}, {0.0, al, tau2, 1.0});
   } else {
-    throw NotImplemented("time stepping type '" + type + "' is not available on the device path");
+    throw IOError("Not known time stepping type '" + type + "'");
   }
   return t;
 }

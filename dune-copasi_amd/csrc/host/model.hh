@@ -35,7 +35,7 @@ struct StepStats {
 
 struct RkTable {
   int stages = 1;
-  double a[3][4] = {{0}}, b[3][4] = {{0}}, d[4] = {0};
+  double a[4][5] = {{0}}, b[4][5] = {{0}}, d[5] = {0};
 };
 RkTable rk_table(const std::string& type);
 

@@ -390,19 +390,74 @@ struct RK {
 };
 
 static RK rk_table(const std::string& type) {
+  // PDELab one-step form of each scheme (the tables of
+  // dune/pdelab/instationary/onestepparameter.hh the reference selects in
+  // model/make_step_operator.hh:408-443): stage r solves
+  //   sum_j a[r][j] M(u_j) + dt b[r][j] A(u_j, t + d[j] dt) = 0
   RK r;
-  if (type == "ImplicitEuler") {
+  const double s2 = std::sqrt(2.0);
+  if (type == "ExplicitEuler") {
+    r.stages = 1;
+    r.a = {{-1.0, 1.0}};
+    r.b = {{1.0, 0.0}};
+    r.d = {0.0, 1.0};
+  } else if (type == "ImplicitEuler") {
     r.stages = 1;
     r.a = {{-1.0, 1.0}};
     r.b = {{0.0, 1.0}};
     r.d = {0.0, 1.0};
+  } else if (type == "Heun") {
+    r.stages = 2;
+    r.a = {{-1.0, 1.0, 0.0}, {-0.5, -0.5, 1.0}};
+    r.b = {{1.0, 0.0, 0.0}, {0.0, 0.5, 0.0}};
+    r.d = {0.0, 1.0, 1.0};
+  } else if (type == "Shu3") {
+    r.stages = 3;
+    r.a = {{-1.0, 1.0, 0.0, 0.0}, {-3.0 / 4.0, -1.0 / 4.0, 1.0, 0.0}, {-1.0 / 3.0, 0.0, -2.0 / 3.0, 1.0}};
+    r.b = {{1.0, 0.0, 0.0, 0.0}, {0.0, 1.0 / 4.0, 0.0, 0.0}, {0.0, 0.0, 2.0 / 3.0, 0.0}};
+    r.d = {0.0, 1.0, 0.5, 1.0};
+  } else if (type == "RungeKutta4") {
+    r.stages = 4;
+    r.a.assign(4, std::vector<double>(5, 0.0));
+    r.b.assign(4, std::vector<double>(5, 0.0));
+    for (int k = 0; k < 4; ++k) {
+      r.a[k][0] = -1.0;
+      r.a[k][k + 1] = 1.0;
+    }
+    r.b[0][0] = 0.5;
+    r.b[1][1] = 0.5;
+    r.b[2][2] = 1.0;
+    r.b[3] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0, 0.0};
+    r.d = {0.0, 0.5, 0.5, 1.0, 1.0};
   } else if (type == "Alexander2") {
     // Alexander (1977) 2-stage L-stable SDIRK, alpha = 1 - sqrt(2)/2
-    double al = 1.0 - std::sqrt(2.0) / 2.0;
+    double al = 1.0 - s2 / 2.0;
     r.stages = 2;
     r.a = {{-1.0, 1.0, 0.0}, {-1.0, 0.0, 1.0}};
     r.b = {{0.0, al, 0.0}, {0.0, 1.0 - al, al}};
     r.d = {0.0, al, 1.0};
+  } else if (type == "FractionalStepTheta") {
+    // theta-steps theta, 1 - 2 theta, theta; implicit weights alpha, beta, alpha
+    const double th = 1.0 - s2 / 2.0, thp = 1.0 - 2.0 * th, al = (1.0 - 2.0 * th) / (1.0 - th), be = 1.0 - al;
+    r.stages = 3;
+    r.a = {{-1.0, 1.0, 0.0, 0.0}, {0.0, -1.0, 1.0, 0.0}, {0.0, 0.0, -1.0, 1.0}};
+    r.b = {{be * th, al * th, 0.0, 0.0}, {0.0, al * thp, be * thp, 0.0}, {0.0, 0.0, be * th, al * th}};
+    r.d = {0.0, th, 1.0 - th, 1.0};
+  } else if (type == "Alexander3") {
+    // 3-stage L-stable SDIRK of order 3; alpha solves
+    // 6 x^3 - 18 x^2 + 9 x - 1 = 0 on (0.4, 0.5), here by bisection
+    double lo = 0.4, hi = 0.5;
+    for (int it = 0; it < 200; ++it) {
+      const double mid = 0.5 * (lo + hi);
+      const double f = ((6.0 * mid - 18.0) * mid + 9.0) * mid - 1.0;
+      (f > 0.0 ? lo : hi) = mid;
+    }
+    const double al = 0.5 * (lo + hi), tau = 0.5 * (1.0 + al);
+    const double b1 = -(6.0 * al * al - 16.0 * al + 1.0) / 4.0, b2 = (6.0 * al * al - 20.0 * al + 5.0) / 4.0;
+    r.stages = 3;
+    r.a = {{-1.0, 1.0, 0.0, 0.0}, {-1.0, 0.0, 1.0, 0.0}, {-1.0, 0.0, 0.0, 1.0}};
+    r.b = {{0.0, al, 0.0, 0.0}, {0.0, tau - al, al, 0.0}, {0.0, b1, b2, al}};
+    r.d = {0.0, al, tau, 1.0};
   } else {
     throw std::runtime_error("oracle: time stepping type '" + type + "' not supported");
   }

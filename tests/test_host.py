@@ -176,6 +176,20 @@ def test_errors_are_reported(dca, ini, msg):
         dca.Model(ini, "", -1)
 
 
+@pytest.mark.parametrize("rk", ["ExplicitEuler", "ImplicitEuler", "Heun", "Shu3", "RungeKutta4", "Alexander2",
+                                "FractionalStepTheta", "Alexander3"])
+def test_time_stepping_types(dca, wl, rk):
+    """Every type of model/make_step_operator.hh:422-439 builds; others are an IOError."""
+    w = wl.gray_scott(n=4, cell_type="cube")
+    dca.Model(w.ini, f"model.time_step_operator.type = {rk}\n", -1).close()
+
+
+def test_unknown_time_stepping_type(dca, wl):
+    w = wl.gray_scott(n=4, cell_type="cube")
+    with pytest.raises(dca.DcgError, match="Not known time stepping type 'Euler'"):
+        dca.Model(w.ini, "model.time_step_operator.type = Euler\n", -1)
+
+
 def test_overrides_apply_last(dca, wl):
     w = wl.gray_scott(n=8, cell_type="cube")
     m = dca.Model(w.ini, "grid.cells=4 4\n", -1)

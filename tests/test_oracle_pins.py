@@ -3,8 +3,10 @@
 The reference ships no golden vectors for this path (its test/data is Git-LFS
 stubs), so the oracle is checked against:
   * closed forms of the reference's own test/exp.ini (u' = -2u): implicit
-    Euler gives exactly (1 + 2 dt)^-n at every node, Alexander2 its stability
-    function, on P1 and Q1 meshes in 1D/2D/3D;
+    Euler gives exactly (1 + 2 dt)^-n at every node, Alexander2, the explicit
+    schemes and the fractional-step theta scheme their stability functions,
+    on P1 and Q1 meshes in 1D/2D/3D; every scheme reaches its order of
+    accuracy, and the L-stable ones damp a huge step to zero;
   * textbook element matrices (P1 triangle/tetrahedron and Q1 square mass and
     stiffness), assembled by numpy on 1-cell and 2x2-cell meshes;
   * invariants: stiffness rows sum to zero, mass entries sum to |Omega|,
@@ -69,6 +71,62 @@ def test_exp_ini_closed_form(dim, cell_type, rk, D):
     R = 1.0 / (1.0 - z) if rk == "ImplicitEuler" else (1 + (1 - 2 * ALPHA) * z) / (1 - ALPHA * z) ** 2
     assert abs(t - n * dt) < 1e-12
     np.testing.assert_allclose(u, R ** n, rtol=1e-12, atol=1e-14)
+
+
+# Stability functions of the explicit schemes (the Taylor polynomial of
+# exp(z) of the scheme's order) and of the fractional-step theta scheme
+# (product of its three theta-steps).
+_TH = 1.0 - math.sqrt(2.0) / 2.0
+_FS_A = (1.0 - 2.0 * _TH) / (1.0 - _TH)
+_R = {
+    "ExplicitEuler": lambda z: 1 + z,
+    "Heun": lambda z: 1 + z + z * z / 2,
+    "Shu3": lambda z: 1 + z + z * z / 2 + z ** 3 / 6,
+    "RungeKutta4": lambda z: 1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24,
+    "FractionalStepTheta": lambda z: ((1 + (1 - _FS_A) * _TH * z) / (1 - _FS_A * _TH * z)) ** 2
+    * (1 + _FS_A * (1 - 2 * _TH) * z) / (1 - (1 - _FS_A) * (1 - 2 * _TH) * z),
+}
+
+
+@pytest.mark.parametrize("rk", sorted(_R))
+@pytest.mark.parametrize("dim,cell_type", [(1, "cube"), (2, "simplex"), (3, "cube")])
+def test_exp_ini_stability_functions(rk, dim, cell_type):
+    """exp.ini (u' = -2u, every node alone): n steps multiply by R(z)^n.  The
+    explicit schemes run without diffusion (with it, dt = 0.1 on this mesh is
+    beyond their stability limit and rounding noise grows)."""
+    o = OracleModel(_exp_ini(dim, cell_type, rk, D=0.3 if rk == "FractionalStepTheta" else 0.0))
+    o.set_state(np.ones(o.n_dof), 0.0)
+    dt, n = 0.1, 6
+    for _ in range(n):
+        rc, _, _ = o.step(dt)
+        assert rc == 0
+    u, t = o.get_state()
+    assert abs(t - n * dt) < 1e-12
+    np.testing.assert_allclose(u, _R[rk](-2.0 * dt) ** n, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("rk,order", [("ExplicitEuler", 1), ("ImplicitEuler", 1), ("Heun", 2), ("Alexander2", 2),
+                                      ("FractionalStepTheta", 2), ("Shu3", 3), ("Alexander3", 3),
+                                      ("RungeKutta4", 4)])
+def test_schemes_reach_their_order(rk, order):
+    """The error at t = 0.4 against exp(-2t) falls by 2^order when dt halves:
+    a wrong coefficient (order conditions) or a wrong Alexander3 alpha shows
+    here.  Alexander3 is also stiffly accurate: one huge step gives ~0."""
+    errs = []
+    for dt in (0.05, 0.025, 0.0125):
+        o = OracleModel(_exp_ini(2, "cube", rk))
+        o.set_state(np.ones(o.n_dof), 0.0)
+        for _ in range(int(round(0.4 / dt))):
+            assert o.step(dt)[0] == 0
+        u, t = o.get_state()
+        errs.append(abs(u[0] - math.exp(-2.0 * t)))
+    rates = [math.log2(errs[i] / errs[i + 1]) for i in range(2)]
+    assert all(abs(r - order) < 0.2 for r in rates), (rk, errs, rates)
+    if rk in ("Alexander2", "Alexander3", "ImplicitEuler"):
+        o = OracleModel(_exp_ini(2, "cube", rk))
+        o.set_state(np.ones(o.n_dof), 0.0)
+        assert o.step(1e6)[0] == 0
+        assert np.abs(o.get_state()[0]).max() < 1e-5
 
 
 def _diffusion_ini(dim, cell_type, cells, D=1.0, ext=None):
