@@ -1,6 +1,7 @@
 #include "layout.hh"
 
 #include <algorithm>
+#include <map>
 #include <fstream>
 #include <cmath>
 #include <numeric>
@@ -153,6 +154,75 @@ Grid read_gmsh(const std::string& path, int dim) {
     g.gmsh_id.push_back(e.phys);
   }
   return g;
+}
+
+void refine_simplices(Grid& g, int levels) {
+  if (!g.unstructured) throw Error("refine_simplices: grid is not unstructured");
+  const int nv = g.nvpe;
+  // local edges in creation order
+  static const int E1[1][2] = {{0, 1}};
+  static const int E2[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+  static const int E3[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+  const int ne = g.dim == 1 ? 1 : g.dim == 2 ? 3 : 6;
+  const int(*E)[2] = g.dim == 1 ? E1 : g.dim == 2 ? E2 : E3;
+  for (int level = 0; level < levels; ++level) {
+    std::map<std::pair<int, int>, int> mid;
+    std::vector<int> conn;
+    std::vector<double> gid;
+    conn.reserve(g.conn.size() * (g.dim == 3 ? 8 : g.dim == 2 ? 4 : 2));
+    for (long c = 0; c < g.ncell; ++c) {
+      const int* v = &g.conn[static_cast<size_t>(c) * nv];
+      int m[6];
+      for (int e = 0; e < ne; ++e) {
+        const int a = v[E[e][0]], b = v[E[e][1]];
+        const auto key = std::make_pair(std::min(a, b), std::max(a, b));
+        auto it = mid.find(key);
+        if (it == mid.end()) {
+          const int id = static_cast<int>(g.X.size() / 3);
+          for (int k = 0; k < 3; ++k) g.X.push_back(0.5 * (g.X[3 * a + k] + g.X[3 * b + k]));
+          it = mid.emplace(key, id).first;
+        }
+        m[e] = it->second;
+      }
+      auto child = [&](std::initializer_list<int> vs) {
+        for (int x : vs) conn.push_back(x);
+        gid.push_back(g.gmsh_id[c]);
+      };
+      if (g.dim == 1) {
+        child({v[0], m[0]});
+        child({m[0], v[1]});
+      } else if (g.dim == 2) {
+        const int m01 = m[0], m02 = m[1], m12 = m[2];
+        child({v[0], m01, m02});
+        child({m01, v[1], m12});
+        child({m02, m12, v[2]});
+        child({m01, m12, m02});
+      } else {
+        const int m01 = m[0], m02 = m[1], m03 = m[2], m12 = m[3], m13 = m[4], m23 = m[5];
+        child({v[0], m01, m02, m03});
+        child({m01, v[1], m12, m13});
+        child({m02, m12, v[2], m23});
+        child({m03, m13, m23, v[3]});
+        auto d2 = [&](int a, int b) {
+          double s = 0.0;
+          for (int k = 0; k < 3; ++k) s += (g.X[3 * a + k] - g.X[3 * b + k]) * (g.X[3 * a + k] - g.X[3 * b + k]);
+          return s;
+        };
+        // the octahedron around each diagonal: its other four midpoints as a ring
+        const int diag[3][2] = {{m01, m23}, {m02, m13}, {m03, m12}};
+        const int ring[3][4] = {{m02, m03, m13, m12}, {m01, m03, m23, m12}, {m01, m02, m23, m13}};
+        int best = 0;
+        for (int k = 1; k < 3; ++k)
+          if (d2(diag[k][0], diag[k][1]) < d2(diag[best][0], diag[best][1])) best = k;
+        for (int k = 0; k < 4; ++k) child({diag[best][0], diag[best][1], ring[best][k], ring[best][(k + 1) % 4]});
+      }
+    }
+    g.conn.swap(conn);
+    g.gmsh_id.swap(gid);
+    g.nvert = static_cast<long>(g.X.size() / 3);
+    g.ncell = static_cast<long>(g.gmsh_id.size());
+    if (g.nvert >= (1L << 31) / 2) throw NotImplemented("grid too large for 32-bit slot indices");
+  }
 }
 
 void Grid::cell_ijk(long cell, long ijk[3]) const {

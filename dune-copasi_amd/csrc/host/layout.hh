@@ -114,6 +114,17 @@ Grid make_grid(int dim, const std::vector<double>& cells, const std::vector<doub
 // the vertices they use are numbered in increasing node number.
 Grid read_gmsh(const std::string& path, int dim);
 
+// `levels` rounds of uniform red refinement of an unstructured simplex grid
+// (grid.refinement_level on a grid read from a file; the reference calls
+// globalRefine on its UGGrid, make_multi_domain_grid.hh:96-100).  Every edge
+// gets its midpoint; a line splits in 2, a triangle in 4 (three corners and
+// the middle one), a tetrahedron in 8 (four corners and the inner octahedron
+// cut along its shortest diagonal, the first of m01-m23, m02-m13, m03-m12 on a
+// tie).  Children keep the parent's gmsh_id.  Vertices keep their indices and
+// the midpoints follow in the order they are first met (cells in order, edges
+// 01 02 03 12 13 23); each cell's children replace it in place.
+void refine_simplices(Grid& g, int levels);
+
 // Assign elements to compartments by evaluating each compartment's
 // expression at element centres; returns per element (cell*nclass+cls) the
 // compartment or -1.  Overlapping compartments are rejected.

@@ -108,8 +108,10 @@ void Model::build(const std::string& ini, const std::string& overrides) {
   Grid g;
   if (c.has_key("grid.path")) {
     // unstructured simplices from a gmsh file (make_multi_domain_grid.hh:56-63)
-    if (c.get("grid.refinement_level", 0) > 0) throw NotImplemented("refinement of grids read from files");
     g = read_gmsh(c.get("grid.path", std::string()), dim);
+    const int levels = c.get("grid.refinement_level", 0);
+    if (levels < 0) throw IOError("grid.refinement_level must not be negative");
+    refine_simplices(g, levels);
   } else {
     const std::string cell = c.get("grid.cell_type", std::string("simplex"));
     if (cell != "simplex" && cell != "cube") throw IOError("grid.cell_type must be 'simplex' or 'cube'");

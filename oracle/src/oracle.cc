@@ -176,6 +176,78 @@ static void read_gmsh(Mesh& m, const std::string& path) {
   m.phys = phys;
 }
 
+// Uniform red refinement of a mesh read from a file, `lvl` times (the
+// reference's UGGrid globalRefine, make_multi_domain_grid.hh:96-100): edge
+// midpoints appended in first-met order (elements in order, local edges
+// 01 02 03 12 13 23), children in place of their parent: lines 2, triangles
+// 4 (corner children, then the middle), tetrahedra 8 (corner children, then
+// the octahedron split along its shortest diagonal, first of m01-m23,
+// m02-m13, m03-m12 on ties); children keep the parent's physical entity.
+static void refine_file_mesh(Mesh& m, int lvl) {
+  const int d = m.dim, nv = m.nvpe;
+  std::vector<std::pair<int, int>> edges;
+  for (int a = 0; a < nv; ++a)
+    for (int b = a + 1; b < nv; ++b) edges.push_back({a, b});
+  for (int l = 0; l < lvl; ++l) {
+    std::map<std::pair<long, long>, long> midpoint;
+    std::vector<long> conn;
+    std::vector<double> phys;
+    for (long e = 0; e < m.nelem; ++e) {
+      std::vector<long> v(m.conn.begin() + e * nv, m.conn.begin() + (e + 1) * nv);
+      // M[a][b]: vertex a for a == b, else the midpoint of edge ab
+      long M[4][4];
+      for (int a = 0; a < nv; ++a) M[a][a] = v[a];
+      for (auto [a, b] : edges) {
+        std::pair<long, long> key{std::min(v[a], v[b]), std::max(v[a], v[b])};
+        auto it = midpoint.find(key);
+        long id;
+        if (it != midpoint.end()) {
+          id = it->second;
+        } else {
+          id = (long)(m.X.size() / 3);
+          for (int k = 0; k < 3; ++k) m.X.push_back((m.X[3 * v[a] + k] + m.X[3 * v[b] + k]) / 2.0);
+          midpoint[key] = id;
+        }
+        M[a][b] = M[b][a] = id;
+      }
+      std::vector<std::vector<long>> kids;
+      for (int a = 0; a < nv; ++a) {
+        // corner child at vertex a: a and the midpoints of its edges, in the parent's order
+        std::vector<long> c(nv);
+        for (int b = 0; b < nv; ++b) c[b] = M[a][b];
+        kids.push_back(c);
+      }
+      if (d == 2) kids.push_back({M[0][1], M[1][2], M[0][2]});
+      if (d == 3) {
+        auto dist2 = [&](long p, long q) {
+          double s = 0;
+          for (int k = 0; k < 3; ++k) s += (m.X[3 * p + k] - m.X[3 * q + k]) * (m.X[3 * p + k] - m.X[3 * q + k]);
+          return s;
+        };
+        // diagonals as vertex pairs (ab, cd) with {a, b, c, d} = {0, 1, 2, 3}
+        const int dg[3][4] = {{0, 1, 2, 3}, {0, 2, 1, 3}, {0, 3, 1, 2}};
+        int k0 = 0;
+        for (int k = 1; k < 3; ++k)
+          if (dist2(M[dg[k][0]][dg[k][1]], M[dg[k][2]][dg[k][3]]) < dist2(M[dg[k0][0]][dg[k0][1]], M[dg[k0][2]][dg[k0][3]]))
+            k0 = k;
+        const int a = dg[k0][0], b = dg[k0][1], c = dg[k0][2], e2 = dg[k0][3];
+        const long p = M[a][b], q = M[c][e2];
+        // the other four midpoints, each sharing one index with its neighbour
+        const long ring[4] = {M[a][c], M[a][e2], M[b][e2], M[b][c]};
+        for (int k = 0; k < 4; ++k) kids.push_back({p, q, ring[k], ring[(k + 1) % 4]});
+      }
+      for (auto& c : kids) {
+        conn.insert(conn.end(), c.begin(), c.end());
+        phys.push_back(m.phys[e]);
+      }
+    }
+    m.conn.swap(conn);
+    m.phys.swap(phys);
+    m.nvert = (long)(m.X.size() / 3);
+    m.nelem = (long)m.phys.size();
+  }
+}
+
 static long vid(const Mesh& m, long i, long j, long k) {
   return i + (m.nc[0] + 1) * (j + (m.nc[1] + 1) * k);
 }
@@ -1940,8 +2012,8 @@ static void configure(Model& M, const std::string& ini, const std::string& over)
   if (ct != "simplex" && ct != "cube") throw std::runtime_error("oracle: grid.cell_type must be simplex or cube");
   m.cube = (ct == "cube");
   if (I.has("grid.path")) {
-    if (lvl > 0) throw std::runtime_error("oracle: refinement of grids read from files is not restated");
     read_gmsh(m, I.get("grid.path"));
+    refine_file_mesh(m, lvl);
   } else {
     build_mesh(m);
   }

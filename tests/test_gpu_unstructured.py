@@ -1,4 +1,4 @@
-"""Grids read from gmsh files (unstructured triangles and tetrahedra) on the device
+"""Grids read from gmsh files (unstructured triangles and tetrahedra, also refined) on the device
 against the oracle: residual and Jacobian assembly to 1e-12, time-step
 sequences to the parity bar, mass sums, and VTK output of simplex cells."""
 import base64
@@ -23,18 +23,23 @@ def meshes(tmp_path_factory):
 
 
 CASES = [("gs_tri", "gray_scott", "square.msh", 2), ("two_tri", "two_halves", "square.msh", 2),
-         ("gs_tet", "gray_scott", "cube.msh", 3), ("two_tet", "two_halves", "cube.msh", 3)]
+         ("gs_tet", "gray_scott", "cube.msh", 3), ("two_tet", "two_halves", "cube.msh", 3),
+         # grid.refinement_level on the file grids (red refinement)
+         ("two_tri_r2", "two_halves", "square.msh", 2), ("gs_tet_r1", "gray_scott", "cube.msh", 3)]
 
 
-def _ini(meshes, model, mesh, dim):
-    return getattr(unstructured_models, model)(meshes / mesh, dim)
+def _ini(meshes, model, mesh, dim, name=""):
+    ini = getattr(unstructured_models, model)(meshes / mesh, dim)
+    if "_r" in name:
+        ini += f"[grid]\nrefinement_level = {name.rsplit('_r', 1)[1]}\n"
+    return ini
 
 
 @pytest.mark.parametrize("name,model,mesh,dim", CASES, ids=[c[0] for c in CASES])
 @pytest.mark.parametrize("cm,ca", [(1.0, 0.0), (0.0, 0.7), (1.0, 0.3)])
 def test_gmsh_assembly_matches_oracle(dca, meshes, name, model, mesh, dim, cm, ca):
     print(f"case: gmsh assembly {name} cm={cm} ca={ca}", flush=True)
-    ini = _ini(meshes, model, mesh, dim)
+    ini = _ini(meshes, model, mesh, dim, name)
     gpu = dca.Model(ini, "", 0)
     orc = OracleModel(ini)
     x = gpu.initial_state() + 0.05 * np.random.default_rng(5).standard_normal(gpu.n_dof)
@@ -49,7 +54,7 @@ def test_gmsh_assembly_matches_oracle(dca, meshes, name, model, mesh, dim, cm, c
 @pytest.mark.parametrize("name,model,mesh,dim", CASES, ids=[c[0] for c in CASES])
 def test_gmsh_time_steps_match_oracle(dca, meshes, name, model, mesh, dim):
     print(f"case: gmsh steps {name}", flush=True)
-    ini = _ini(meshes, model, mesh, dim)
+    ini = _ini(meshes, model, mesh, dim, name)
     gpu = dca.Model(ini, "", 0)
     orc = OracleModel(ini)
     u0 = gpu.initial_state()
@@ -93,7 +98,7 @@ def test_gmsh_vtk_cells(dca, meshes, tmp_path):
 @pytest.mark.parametrize("name,model,mesh,dim", CASES, ids=[c[0] for c in CASES])
 def test_gmsh_reduce_matches_oracle(dca, meshes, name, model, mesh, dim):
     from tests.test_gpu_reduce import _config
-    ini = _ini(meshes, model, mesh, dim)
+    ini = _ini(meshes, model, mesh, dim, name)
     gpu = dca.Model(ini, "", 0)
     orc = OracleModel(ini)
     u = gpu.initial_state() + 0.05 * np.random.default_rng(3).standard_normal(gpu.n_dof)

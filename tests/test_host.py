@@ -346,11 +346,36 @@ def test_gmsh_layout_matches_oracle(dca, gmsh_files, mesh, dim, model):
     assert rc == 0, log
 
 
+@pytest.mark.parametrize("mesh,dim,levels", [("square.msh", 2, 1), ("square.msh", 2, 2), ("cube.msh", 3, 1),
+                                             ("cube.msh", 3, 2)])
+def test_gmsh_refinement(dca, gmsh_files, mesh, dim, levels):
+    """grid.refinement_level on a file grid: red refinement keeps the
+    measure of every compartment, adds vertices, and gives the same layout as
+    the oracle's own refinement."""
+    from tests import unstructured_models
+    from tests.oracle_lib import OracleModel
+    base = unstructured_models.two_halves(gmsh_files / mesh, dim)
+    ini = base + f"[grid]\nrefinement_level = {levels}\n"
+    coarse = dca.Model(base, "", -1)
+    m = dca.Model(ini, "", -1)
+    o = OracleModel(ini)
+    assert m.n_dof == o.n_dof
+    for c, (a, b) in enumerate(zip(m.compartments(), o.compartments())):
+        assert (a["n_nodes"], a["dof_offset"], a["n_species"]) == (b["n_nodes"], b["dof_offset"], b["n_species"])
+        assert np.array_equal(m.compartment_nodes(c), o.compartment_nodes(c))
+    # more vertices, and each compartment keeps its measure (the integral of 1)
+    oc = OracleModel(base)
+    for c in range(len(m.compartments())):
+        assert len(coarse.compartment_nodes(c)) < len(m.compartment_nodes(c))
+    np.testing.assert_allclose(o.mass(np.ones(o.n_dof)), oc.mass(np.ones(oc.n_dof)), rtol=1e-12)
+    assert abs(sum(o.mass(np.ones(o.n_dof))) / 2 - 1.0 / 2) < 1e-12  # each species sums over its half; halves fill the unit box
+
+
 def test_gmsh_errors(dca, gmsh_files, tmp_path):
     from tests import unstructured_models
     ini = unstructured_models.gray_scott(gmsh_files / "square.msh")
-    with pytest.raises(dca.DcgError, match="refinement of grids read from files"):
-        dca.Model(ini + "[grid]\nrefinement_level = 1\n", "", -1)
+    with pytest.raises(dca.DcgError, match="refinement_level must not be negative"):
+        dca.Model(ini + "[grid]\nrefinement_level = -1\n", "", -1)
     with pytest.raises(dca.DcgError, match="cannot open"):
         dca.Model(unstructured_models.gray_scott(tmp_path / "missing.msh"), "", -1)
     quad = tmp_path / "quad.msh"
