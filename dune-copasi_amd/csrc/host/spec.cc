@@ -1,5 +1,8 @@
 #include "spec.hh"
 
+#include <algorithm>
+#include <cstdio>
+
 #include "error.hh"
 
 namespace dcopasi {
@@ -8,7 +11,7 @@ static const char* kAxis = "xyz";
 
 static expr::Context read_context(const ParameterTree& cfg) {
   expr::Context ctx;
-  // parser_context.<name>.type = constant | function (context.cc:56-173)
+  // parser_context.<name>.type = constant | function | interpolation (context.cc:56-173)
   for (const auto& name : cfg.sub_keys("parser_context")) {
     const std::string p = "parser_context." + name;
     const std::string type = cfg.get(p + ".type", std::string());
@@ -30,6 +33,30 @@ static expr::Context read_context(const ParameterTree& cfg) {
         start = comma + 1;
       }
       f.body = text.substr(colon + 1);
+      ctx.functions[name] = f;
+    } else if (type == "interpolation") {
+      // context.cc:72-97: linear between the (domain, range) points, the end
+      // values outside; inlined as a one-argument function of nested ifs
+      const std::vector<double> dom = cfg.get_list(p + ".domain"), rng = cfg.get_list(p + ".range");
+      if (!std::is_sorted(dom.begin(), dom.end())) throw IOError("The interpolation domain must be sorted");
+      if (dom.size() < 2 || dom.size() != rng.size())
+        throw IOError("Interpolation range and domain must have at least two points and be the same size");
+      auto num = [](double v) {
+        char b[40];
+        std::snprintf(b, sizeof b, "(%.17g)", v);
+        return std::string(b);
+      };
+      std::string body = num(rng.back());
+      for (size_t i = dom.size(); i-- > 1;) {
+        if (dom[i] == dom[i - 1]) continue;  // an empty interval
+        body = "if(x <= " + num(dom[i]) + ", " + num(rng[i - 1]) + " + ((x - " + num(dom[i - 1]) + ") / (" +
+               num(dom[i]) + " - " + num(dom[i - 1]) + ")) * (" + num(rng[i]) + " - " + num(rng[i - 1]) + "), " +
+               body + ")";
+      }
+      body = "if(x <= " + num(dom[0]) + ", " + num(rng[0]) + ", " + body + ")";
+      expr::Function f;
+      f.params = {"x"};
+      f.body = body;
       ctx.functions[name] = f;
     } else if (!type.empty()) {
       throw NotImplemented("parser_context type '" + type + "' (" + name + ")");

@@ -15,6 +15,7 @@
 //
 // Test infrastructure only.
 #pragma once
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <functional>
@@ -30,7 +31,13 @@ namespace orc {
 
 enum class Op {
   Num, Var, Neg, Not, Add, Sub, Mul, Div, Pow, Mod,
-  Lt, Le, Gt, Ge, Eq, Ne, And, Or, Tern, Call
+  Lt, Le, Gt, Ge, Eq, Ne, And, Or, Tern, Call, Table
+};
+
+// parser_context type "interpolation": points (domain sorted) of a piecewise
+// linear function, constant beyond the ends (context.cc:72-97)
+struct Table {
+  std::vector<double> x, y;
 };
 
 struct Node {
@@ -38,6 +45,7 @@ struct Node {
   double num = 0;
   int var = -1;
   std::string fname;
+  std::shared_ptr<const Table> table;  // Op::Table
   std::vector<std::shared_ptr<Node>> kids;
 };
 using NodeP = std::shared_ptr<Node>;
@@ -53,6 +61,7 @@ struct Scope {
   std::map<std::string, int> vars;
   std::map<std::string, double> consts;
   std::map<std::string, UserFunction> funcs;
+  std::map<std::string, std::shared_ptr<const Table>> tables;
 };
 
 class Parser {
@@ -255,6 +264,13 @@ class Parser {
     fail("unknown symbol '" + id + "'");
   }
   NodeP call(const std::string& f, std::vector<NodeP> a) {
+    auto tb = scope_.tables.find(f);
+    if (tb != scope_.tables.end()) {
+      if (a.size() != 1) fail("wrong argument count for '" + f + "'");
+      auto n = mk(Op::Table, std::move(a));
+      n->table = tb->second;
+      return n;
+    }
     auto uf = scope_.funcs.find(f);
     if (uf != scope_.funcs.end()) {
       if (uf->second.args.size() != a.size()) fail("wrong argument count for '" + f + "'");
@@ -301,6 +317,16 @@ inline double eval(const Node* n, const double* v) {
     case Op::And: return (eval(n->kids[0].get(), v) != 0.0 && eval(n->kids[1].get(), v) != 0.0) ? 1.0 : 0.0;
     case Op::Or: return (eval(n->kids[0].get(), v) != 0.0 || eval(n->kids[1].get(), v) != 0.0) ? 1.0 : 0.0;
     case Op::Tern: return eval(n->kids[0].get(), v) != 0.0 ? eval(n->kids[1].get(), v) : eval(n->kids[2].get(), v);
+    case Op::Table: {
+      // first point not below the argument; the end values outside
+      const Table& T = *n->table;
+      const double a = eval(n->kids[0].get(), v);
+      const size_t i = std::lower_bound(T.x.begin(), T.x.end(), a) - T.x.begin();
+      if (i == 0) return T.y.front();
+      if (i == T.x.size()) return T.y.back();
+      const double w = (a - T.x[i - 1]) / (T.x[i] - T.x[i - 1]);
+      return T.y[i - 1] + w * (T.y[i] - T.y[i - 1]);
+    }
     case Op::Call: {
       const std::string& f = n->fname;
       double a = eval(n->kids[0].get(), v);

@@ -607,6 +607,18 @@ static void build_species(Model& M) {
   for (auto& C : M.comps) C.b = (int)C.species.size();
 }
 
+// parser_context.<name>.{domain,range} of an interpolation function
+// (context.cc:72-97), checked as the reference checks them
+static std::shared_ptr<const Table> read_table(const Ini& I, const std::string& p) {
+  auto t = std::make_shared<Table>();
+  t->x = I.get_doubles(p + ".domain");
+  t->y = I.get_doubles(p + ".range");
+  if (!std::is_sorted(t->x.begin(), t->x.end())) throw std::runtime_error("The interpolation domain must be sorted");
+  if (t->x.size() < 2 || t->x.size() != t->y.size())
+    throw std::runtime_error("Interpolation range and domain must have at least two points and be the same size");
+  return t;
+}
+
 static void build_scope(Model& M) {
   const Ini& I = M.ini;
   Scope& sc = M.scope;
@@ -659,6 +671,8 @@ static void build_scope(Model& M) {
       }
       f.body = e.substr(colon + 1);
       sc.funcs[k] = f;
+    } else if (type == "interpolation") {
+      sc.tables[k] = read_table(I, p);
     } else if (!type.empty()) {
       throw std::runtime_error("oracle: parser_context type '" + type + "' not supported");
     }
@@ -2317,6 +2331,7 @@ extern "C" int orc_eval(const char* context, const char* text, const char* const
         f.body = e.substr(colon + 1);
         sc.funcs[k] = f;
       }
+      if (type == "interpolation") sc.tables[k] = orc::read_table(M.ini, p);
     }
     orc::Expr e = orc::compile(text, sc);
     *out = e.null() ? 0.0 : e(values);

@@ -13,11 +13,13 @@ from tests.oracle_lib import OracleModel, bcsr_to_dense_csr, rel_l2_per_species
 pytestmark = pytest.mark.gpu
 
 def _fallback_model(kind, cell_type):
-    """Gray-Scott variants that need the coloured assembly kernel."""
+    """Gray-Scott variants that need the coloured assembly kernel (kind "" is
+    the plain model, for the tiled kernel)."""
     extra_u = {"adv": "velocity.x.expression = 0.02\nvelocity.y.expression = -0.01*v\n"
                       "velocity.jacobian.v.y.expression = -0.01\n",
                "nld": "",
-               "num": ""}[kind]
+               "num": "",
+               "": ""}[kind]
     du = ("cross_diffusion.u.expression = 1e-3*(1+u^2)\ncross_diffusion.u.jacobian.u.expression = 2e-3*u\n"
           if kind == "nld" else "cross_diffusion.u.expression = 2e-3\n")
     jt = "[model]\njacobian.type = numerical\n" if kind == "num" else ""
@@ -89,6 +91,33 @@ def test_coloured_kernel_models(dca, name, make):
         t += w.dt
         err = max(rel_l2_per_species(gpu.compartments(), gpu.get_state()[0], orc.get_state()[0]))
         assert err <= 1e-8, (k, err)
+
+
+def test_interpolation_context_on_device(dca):
+    """A reaction rate from a parser_context interpolation table (a jump in
+    the middle, constant beyond the ends): device assembly and steps = oracle."""
+    ini = _fallback_model("", "cube").replace(
+        "reaction.expression = -u*v^2 + 0.04*(1-u)\nreaction.jacobian.u.expression = -v^2 - 0.04\n",
+        "reaction.expression = -u*v^2 + k(position_x)*(1-u)\nreaction.jacobian.u.expression = -v^2 - k(position_x)\n")
+    ini = ini.replace("[compartments]", "[parser_context.k]\ntype = interpolation\ndomain = 0.2 0.45 0.45 0.7\n"
+                      "range = 0.02 0.06 0.01 0.08\n[compartments]")
+    assert "k(position_x)" in ini
+    w = _Inline(ini)
+    gpu = dca.Model(w.ini, "", 0)
+    orc = OracleModel(w.ini)
+    x = _perturbed_state(w, gpu)
+    r_g, J_g = gpu.assemble(x, 0.1, 0.8, 0.6)
+    r_o, v_o = orc.assemble(x, 0.1, 0.8, 0.6)
+    assert np.abs(r_g - r_o).max() <= 1e-12 * np.abs(r_o).max()
+    assert np.abs(bcsr_to_dense_csr(gpu, J_g).toarray() - orc.csr(v_o).toarray()).max() <= 1e-12 * np.abs(v_o).max()
+    u0 = w.initial_state(gpu)
+    gpu.set_state(u0, 0.0)
+    orc.set_state(u0, 0.0)
+    t = 0.0
+    for k in range(3):
+        assert gpu.step(t, w.dt).status == 0 and orc.step(w.dt)[0] == 0
+        t += w.dt
+        assert max(rel_l2_per_species(gpu.compartments(), gpu.get_state()[0], orc.get_state()[0])) <= 1e-8, k
 
 
 CASES = [

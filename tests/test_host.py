@@ -190,6 +190,59 @@ def test_unknown_time_stepping_type(dca, wl):
         dca.Model(w.ini, "model.time_step_operator.type = Euler\n", -1)
 
 
+_INTERP = """[grid]
+dimension = 2
+cells = 8 4
+[parser_context.f]
+type = interpolation
+domain = 0.1 0.3 0.3 0.8
+range = 0.5 -0.25 2.0 1.0
+[compartments]
+domain.expression = 1
+[model.scalar_field.u]
+compartment = domain
+initial.expression = f(position_x) + 10*f(position_y - 0.2)
+"""
+
+
+def _interp_ref(x):
+    """The reference's interpolation (context.cc:72-97) by hand: first domain
+    point >= x; the end values outside."""
+    d, r = [0.1, 0.3, 0.3, 0.8], [0.5, -0.25, 2.0, 1.0]
+    i = next((k for k, v in enumerate(d) if v >= x), len(d))
+    if i == 0:
+        return r[0]
+    if i == len(d):
+        return r[-1]
+    return r[i - 1] + (x - d[i - 1]) / (d[i] - d[i - 1]) * (r[i] - r[i - 1])
+
+
+def test_interpolation_context(dca):
+    """parser_context type = interpolation through initial conditions: nodes
+    below, on and between the points and beyond the last (the duplicated 0.3
+    is a jump); np.interp agrees away from the jump."""
+    m = dca.Model(_INTERP, "", -1)
+    u = m.initial_state()
+    ids = m.compartment_nodes(0)
+    xs, ys = (ids % 9) / 8.0, (ids // 9) / 4.0
+    ref = np.array([_interp_ref(x) + 10 * _interp_ref(y - 0.2) for x, y in zip(xs, ys)])
+    np.testing.assert_allclose(u, ref, rtol=1e-15, atol=1e-15)
+    away = np.abs(xs - 0.3) > 1e-9
+    np.testing.assert_allclose(u[away] - 10 * np.array([_interp_ref(y - 0.2) for y in ys[away]]),
+                               np.interp(xs[away], [0.1, 0.3, 0.8], [0.5, -0.25, 1.0]) * (xs[away] < 0.3)
+                               + np.interp(xs[away], [0.3, 0.8], [2.0, 1.0]) * (xs[away] > 0.3), rtol=1e-14, atol=1e-15)
+
+
+@pytest.mark.parametrize("dom,rng,msg", [("0 2 1", "1 2 3", "domain must be sorted"),
+                                         ("0 1", "1 2 3", "at least two points and be the same size"),
+                                         ("0", "1", "at least two points and be the same size")])
+def test_interpolation_context_errors(dca, dom, rng, msg):
+    ini = _INTERP.replace("domain = 0.1 0.3 0.3 0.8", f"domain = {dom}").replace("range = 0.5 -0.25 2.0 1.0",
+                                                                                 f"range = {rng}")
+    with pytest.raises(dca.DcgError, match=msg):
+        dca.Model(ini, "", -1)
+
+
 def test_overrides_apply_last(dca, wl):
     w = wl.gray_scott(n=8, cell_type="cube")
     m = dca.Model(w.ini, "grid.cells=4 4\n", -1)

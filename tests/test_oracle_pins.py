@@ -129,6 +129,38 @@ def test_schemes_reach_their_order(rk, order):
         assert np.abs(o.get_state()[0]).max() < 1e-5
 
 
+def test_interpolation_context_values():
+    """parser_context type = interpolation in the oracle: a reaction f(u) on
+    a constant state c gives residual rows -f(c) times the rows of the mass
+    matrix, so f(c) is read back at values below, on, between (both sides of
+    the jump at the duplicated 0.3) and above the points; the values are the
+    reference's rule (context.cc:72-97) worked by hand."""
+    base = """[grid]
+dimension = 2
+cells = 3 3
+cell_type = cube
+[parser_context.f]
+type = interpolation
+domain = 0.1 0.3 0.3 0.8
+range = 0.5 -0.25 2.0 1.0
+[compartments]
+domain.expression = 1
+[model.scalar_field.u]
+compartment = domain
+storage.expression = 1
+reaction.expression = f(u)
+reaction.jacobian.u.expression = 0
+"""
+    o = OracleModel(base)
+    mass, _ = o.assemble(np.ones(o.n_dof), 0.0, 1.0, 0.0, jacobian=False)
+    # (not at 0.3 itself: the points' values of a constant state round to
+    # either side of the jump)
+    cases = {-1.0: 0.5, 0.1: 0.5, 0.2: 0.125, 0.29: -0.2125, 0.31: 1.98, 0.55: 1.5, 0.8: 1.0, 5.0: 1.0}
+    for c, fc in cases.items():
+        r, _ = o.assemble(np.full(o.n_dof, c), 0.0, 0.0, 1.0, jacobian=False)
+        np.testing.assert_allclose(r, -fc * mass, rtol=1e-9, atol=1e-15)
+
+
 def _diffusion_ini(dim, cell_type, cells, D=1.0, ext=None):
     ext = ext or " ".join(["1"] * dim)
     return f"""[grid]
