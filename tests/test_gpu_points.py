@@ -177,3 +177,28 @@ def test_point_model_holds_no_matrix(dca, wl):
     assert rc == 0
     A = bcsr_to_dense_csr(pts, J)
     assert np.linalg.norm(A @ sol - b) <= 1e-10 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 5, 7, 8, 9, 12, 21, 33])
+def test_random_requests_with_default_storage(dca, wl, seed):
+    """Random mass-action requests of the serving stream (1-2 compartments,
+    2-8 species, transmission between compartments) with the default storage:
+    point coefficients where `auto` takes them, the matrix elsewhere; two
+    steps against the oracle to the parity bar."""
+    w = wl.random_request(seed, precision="fp64", nl_rtol=1e-20, nl_atol=1e-28, side2=(10, 14), side3=(5, 7),
+                          steps_range=(2, 2))
+    print(f"case: random request {seed}", flush=True)
+    gpu = dca.Model(w.ini, "", 0)
+    orc = OracleModel(w.ini)
+    u0 = w.initial_state(gpu)
+    gpu.set_state(u0, 0.0)
+    orc.set_state(u0, 0.0)
+    t = 0.0
+    for k in range(2):
+        st = gpu.step(t, w.dt)
+        rc, newton, _ = orc.step(w.dt)
+        assert st.status == 0 and rc == 0 and st.newton_iterations == newton, (k, st.status, rc)
+        t += w.dt
+        ug, _ = gpu.get_state()
+        uo, _ = orc.get_state()
+        assert max(rel_l2_per_species(gpu.compartments(), ug, uo)) <= 1e-8, k
