@@ -1188,6 +1188,199 @@ __device__ __forceinline__ void block_rows(int dx, int dy, int dz, int li, int l
   }
 }
 
+// The rows of one tile from point coefficients pcb[((p*NQ + q)*NMC + n)*NCB +
+// box cell] (a tile's block in HBM, or the set-up's staged LDS array) and the
+// direction staged in vs over the vertex box (slots sl, class masks cms): per
+// cell, the direction at its corners interpolated to each point, multiplied
+// by the point's block and tested with the basis functions, plus the uniform
+// diffusion's element stiffness times the corner values; the cells' corner
+// sums folded onto the tile's rows.  y gets the rows (padding species rows
+// are the identity, from v); with part, the workgroup's partial dots (y.w1,
+// y.w2) go to part[2 blockIdx.x + 0/1].  LDS scratch: accv (zeroed by the
+// caller) when SCATTER, else out.
+template <typename T, typename C>
+__device__ __forceinline__ void cells_apply(const C* __restrict__ pcb, const T* vs, const int* sl,
+                                            const unsigned char* cms, T* out, T* accv, const T* __restrict__ v,
+                                            T* __restrict__ y, double ca, const double (&dv)[NDV > 0 ? NDV : 1],
+                                            const T* __restrict__ w1, const T* __restrict__ w2,
+                                            double* __restrict__ part) {
+  // corner values stay in registers for small blocks, in LDS otherwise; with
+  // small blocks and one cell per thread the corner sums go straight onto
+  // the vertices (SCATTER), otherwise through a per-cell corner array
+  constexpr bool VREG = BC <= 4;
+  constexpr bool SCATTER = VREG && NCB <= 256;
+  // the corner sums of box cell cl over its classes, handed to put(corner, i, value)
+  auto cell_corner_sums = [&](int cl, unsigned m, auto&& put) {
+    const int ci = cl % CXB, cj = (cl / CXB) % CYB, ck = cl / (CXB * CYB);
+#pragma unroll
+    for (int p = 0; p < NCLASS; ++p) {
+      if (!((m >> p) & 1u)) continue;
+      int bidx[NV];
+      T vc[VREG ? NV : 1][BC];
+#pragma unroll
+      for (int a = 0; a < NV; ++a) {
+        const int c = CORNER[p][a];
+        bidx[a] = (ci + (c & 1)) + XB * ((cj + ((c >> 1) & 1)) + YB * (ck + ((c >> 2) & 1)));
+        if constexpr (VREG)
+#pragma unroll
+          for (int i = 0; i < BC; ++i) vc[a][i] = vs[bidx[a] * BC + i];
+      }
+      auto vcor = [&](int a, int i) -> T { return VREG ? vc[VREG ? a : 0][i] : vs[bidx[a] * BC + i]; };
+      // the point terms of one point into the corner sums: acc(a, i) += PHI[q][a] w[i]
+      auto point_terms = [&](int q, auto&& add) {
+        T vq[BC];
+#pragma unroll
+        for (int j = 0; j < BC; ++j) {
+          T sum = T(0);
+#pragma unroll
+          for (int a = 0; a < NV; ++a) sum += T(PHI[q][a]) * vcor(a, j);
+          vq[j] = sum;
+        }
+        const C* mp = pcb + (size_t)((p * NQ + q) * NMC) * NCB + cl;
+        T w[BC];
+#pragma unroll
+        for (int i = 0; i < BC; ++i) w[i] = T(0);
+#pragma unroll
+        for (int n = 0; n < NMC; ++n) w[MCI[n]] += T(mp[(size_t)n * NCB]) * vq[MCJ[n]];
+#pragma unroll
+        for (int a = 0; a < NV; ++a)
+#pragma unroll
+          for (int i = 0; i < BC; ++i) add(a, i, T(PHI[q][a]) * w[i]);
+      };
+      // the uniform diffusion: acc(a, i) += ca sum_c K[a][c] v_c for the element
+      // stiffness K (the table entries are immediates; ca scales the sum)
+      auto diffusion_terms = [&](auto&& add) {
+#pragma unroll
+        for (int a = 0; a < NV; ++a) {
+          if constexpr (CONST_DIFFUSION) {
+#pragma unroll
+            for (int n = 0; n < NKZ; ++n) {
+              T sum = T(0);
+#pragma unroll
+              for (int c = 0; c < NV; ++c) sum += T(kd_rt(p, a, c, KZI[n], KZJ[n])) * vcor(c, KZJ[n]);
+              add(a, KZI[n], T(ca) * sum);
+            }
+          } else {
+#pragma unroll
+            for (int n = 0; n < NDV; ++n) {
+              T sum = T(0);
+#pragma unroll
+              for (int c = 0; c < NV; ++c) sum += T(KDU[p][a][c][DVK[n]][DVL[n]]) * vcor(c, DVJ[n]);
+              add(a, DVI[n], T(ca * dv[n]) * sum);
+            }
+          }
+        }
+      };
+      if constexpr (VREG) {
+        // small blocks: the class's corner sums in registers
+        T o[NV][BC];
+#pragma unroll
+        for (int a = 0; a < NV; ++a)
+#pragma unroll
+          for (int i = 0; i < BC; ++i) o[a][i] = T(0);
+        auto add = [&](int a, int i, T v2) { o[a][i] += v2; };
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) point_terms(q, add);
+        diffusion_terms(add);
+#pragma unroll
+        for (int a = 0; a < NV; ++a)
+#pragma unroll
+          for (int i = 0; i < BC; ++i) put(CORNER[p][a], i, o[a][i]);
+      } else {
+        // large blocks: straight into the sums, point by point
+        auto add = [&](int a, int i, T v2) { put(CORNER[p][a], i, v2); };
+#pragma unroll 1
+        for (int q = 0; q < NQ; ++q) point_terms(q, add);
+        diffusion_terms(add);
+      }
+    }
+  };
+  if constexpr (SCATTER) {
+    // one cell per thread; its corner sums are added to the vertices corner
+    // by corner: for one corner the cells' vertices are distinct, so no two
+    // threads add to the same vertex between two barriers, and every vertex
+    // sums its cells in corner order
+    const int cl = threadIdx.x;
+    const bool has = cl < NCB && cms[cl < NCB ? cl : 0];
+    T oc[NCORNER][BC];
+#pragma unroll
+    for (int c = 0; c < NCORNER; ++c)
+#pragma unroll
+      for (int i = 0; i < BC; ++i) oc[c][i] = T(0);
+    if (has) cell_corner_sums(cl, cms[cl], [&](int c, int i, T v2) { oc[c][i] += v2; });
+    const int ci = cl % CXB, cj = (cl / CXB) % CYB, ck = cl / (CXB * CYB);
+#pragma unroll
+    for (int c = 0; c < NCORNER; ++c) {
+      if (c) __syncthreads();
+      if (has) {
+        const int b = (ci + (c & 1)) + XB * ((cj + ((c >> 1) & 1)) + YB * (ck + ((c >> 2) & 1)));
+#pragma unroll
+        for (int i = 0; i < BC; ++i) accv[b * BC + i] += oc[c][i];
+      }
+    }
+  } else {
+    for (int cl = threadIdx.x; cl < NCB; cl += blockDim.x) {
+      T* dst = out + (size_t)cl * (NCORNER * BC);
+      for (int k = 0; k < NCORNER * BC; ++k) dst[k] = T(0);
+      const unsigned m = cms[cl];
+      if (!m) continue;
+      cell_corner_sums(cl, m, [&](int c, int i, T v2) { dst[c * BC + i] += v2; });
+    }
+  }
+  __syncthreads();
+  double d1 = 0.0, d2 = 0.0;
+  for (int it = threadIdx.x; it < NROW * DCG_B; it += blockDim.x) {
+    const int lr = it / DCG_B, i = it % DCG_B;
+    const int li = lr % TX, lj = DIM > 1 ? (lr / TX) % TY : 0, lk = DIM > 2 ? lr / (TX * TY) : 0;
+    const int vb = (li + 1) + XB * ((lj + OY) + YB * (lk + OZ));
+    const int s = sl[vb];
+    if (s < 0) continue;
+    T acc = T(0);
+    if (i < BC && SCATTER) {
+      acc = accv[(SCATTER ? vb : 0) * BC + i];
+    } else if (i < BC) {
+      for (int ez = 0; ez < (DIM > 2 ? 2 : 1); ++ez)
+        for (int ey = 0; ey < (DIM > 1 ? 2 : 1); ++ey)
+          for (int ex = 0; ex < 2; ++ex) {
+            const int cl = (li + ex) + CXB * ((lj + ey) + CYB * (lk + ez));
+            if (!cms[cl]) continue;
+            const int cv = (1 - ex) | (DIM > 1 ? (1 - ey) << 1 : 0) | (DIM > 2 ? (1 - ez) << 2 : 0);
+            acc += out[(SCATTER ? 0 : (size_t)cl * (NCORNER * BC) + cv * BC) + i];
+          }
+    } else {
+      acc = v[(size_t)s * DCG_B + i];  // padding species: identity rows
+    }
+    y[(size_t)s * DCG_B + i] = acc;
+    if (part) {
+      d1 += (double)acc * (double)w1[(size_t)s * DCG_B + i];
+      if (w2) d2 += (double)acc * (double)w2[(size_t)s * DCG_B + i];
+    }
+  }
+  if (part) {
+    // workgroup sums of the two partial dots (fixed order)
+    __shared__ double red[8];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      d1 += __shfl_down(d1, o, 64);
+      d2 += __shfl_down(d2, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      red[2 * (threadIdx.x >> 6)] = d1;
+      red[2 * (threadIdx.x >> 6) + 1] = d2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double a = 0.0, b = 0.0;
+      for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+        a += red[2 * k];
+        b += red[2 * k + 1];
+      }
+      part[2 * blockIdx.x] = a;
+      part[2 * blockIdx.x + 1] = b;
+    }
+  }
+}
+
 // one tile of vertices: the residual rows and/or the BCSR block rows it owns.
 // MODE fixes the work at compile time (1 residual, 2 Jacobian, 4 apply, 8
 // point coefficients), so each entry point gets registers and LDS for its own
@@ -1203,6 +1396,10 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
   __shared__ int sl[NVB];
   __shared__ unsigned char cms[NCB];
   __shared__ double work[MODE == 1 ? RL_WORDS : (MODE == 2 || MODE == 4 || MODE == 8) ? JW_WORDS : UNION_WORDS];
+  // mode 4 with small blocks: the staged coefficients are applied cell by
+  // cell (cells_apply), whose corner sums gather on the vertex box here
+  constexpr bool CELL_APPLY = MODE == 4 && BC <= 4 && NCB <= 256;
+  __shared__ double acc4[CELL_APPLY ? NVB * BC : 1];
   const int ntx = (n0 + 1 + TX - 1) / TX;
   const int nty = DIM > 1 ? (n1 + 1 + TY - 1) / TY : 1;
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
@@ -1306,8 +1503,14 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
       for (int b = threadIdx.x; b < NVB; b += blockDim.x) {
         const int s = sl[b];
         for (int i = 0; i < BC; ++i) xs[b * BC + i] = s >= 0 ? base[(size_t)s * DCG_B + i] : 0.0;
+        if constexpr (CELL_APPLY)
+          for (int i = 0; i < BC; ++i) acc4[b * BC + i] = 0.0;
       }
       __syncthreads();
+      if constexpr (CELL_APPLY) {
+        cells_apply<double, double>(work, xs, sl, cms, nullptr, acc4, base, r, ca, dv, nullptr, nullptr, nullptr);
+        return;
+      }
     }
     if constexpr (BC <= 4) {
     // phase 2 (blocks up to 4 x 4): SPL lanes per row, split over the last
@@ -1705,176 +1908,7 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
   __syncthreads();
   double dv[NDV > 0 ? NDV : 1];
   uniform_diffusion(t, prm, dv);
-  // the corner sums of box cell cl over its classes, handed to put(corner, i, value)
-  auto cell_corner_sums = [&](int cl, unsigned m, auto&& put) {
-    const int ci = cl % CXB, cj = (cl / CXB) % CYB, ck = cl / (CXB * CYB);
-#pragma unroll
-    for (int p = 0; p < NCLASS; ++p) {
-      if (!((m >> p) & 1u)) continue;
-      int bidx[NV];
-      T vc[VREG ? NV : 1][BC];
-#pragma unroll
-      for (int a = 0; a < NV; ++a) {
-        const int c = CORNER[p][a];
-        bidx[a] = (ci + (c & 1)) + XB * ((cj + ((c >> 1) & 1)) + YB * (ck + ((c >> 2) & 1)));
-        if constexpr (VREG)
-#pragma unroll
-          for (int i = 0; i < BC; ++i) vc[a][i] = vs[bidx[a] * BC + i];
-      }
-      auto vcor = [&](int a, int i) -> T { return VREG ? vc[VREG ? a : 0][i] : vs[bidx[a] * BC + i]; };
-      // the point terms of one point into the corner sums: acc(a, i) += PHI[q][a] w[i]
-      auto point_terms = [&](int q, auto&& add) {
-        T vq[BC];
-#pragma unroll
-        for (int j = 0; j < BC; ++j) {
-          T sum = T(0);
-#pragma unroll
-          for (int a = 0; a < NV; ++a) sum += T(PHI[q][a]) * vcor(a, j);
-          vq[j] = sum;
-        }
-        const T* mp = pcb + (size_t)((p * NQ + q) * NMC) * NCB + cl;
-        T w[BC];
-#pragma unroll
-        for (int i = 0; i < BC; ++i) w[i] = T(0);
-#pragma unroll
-        for (int n = 0; n < NMC; ++n) w[MCI[n]] += mp[(size_t)n * NCB] * vq[MCJ[n]];
-#pragma unroll
-        for (int a = 0; a < NV; ++a)
-#pragma unroll
-          for (int i = 0; i < BC; ++i) add(a, i, T(PHI[q][a]) * w[i]);
-      };
-      // the uniform diffusion: acc(a, i) += ca sum_c K[a][c] v_c for the element
-      // stiffness K (the table entries are immediates; ca scales the sum)
-      auto diffusion_terms = [&](auto&& add) {
-#pragma unroll
-        for (int a = 0; a < NV; ++a) {
-          if constexpr (CONST_DIFFUSION) {
-#pragma unroll
-            for (int n = 0; n < NKZ; ++n) {
-              T sum = T(0);
-#pragma unroll
-              for (int c = 0; c < NV; ++c) sum += T(kd_rt(p, a, c, KZI[n], KZJ[n])) * vcor(c, KZJ[n]);
-              add(a, KZI[n], T(ca) * sum);
-            }
-          } else {
-#pragma unroll
-            for (int n = 0; n < NDV; ++n) {
-              T sum = T(0);
-#pragma unroll
-              for (int c = 0; c < NV; ++c) sum += T(KDU[p][a][c][DVK[n]][DVL[n]]) * vcor(c, DVJ[n]);
-              add(a, DVI[n], T(ca * dv[n]) * sum);
-            }
-          }
-        }
-      };
-      if constexpr (VREG) {
-        // small blocks: the class's corner sums in registers
-        T o[NV][BC];
-#pragma unroll
-        for (int a = 0; a < NV; ++a)
-#pragma unroll
-          for (int i = 0; i < BC; ++i) o[a][i] = T(0);
-        auto add = [&](int a, int i, T v2) { o[a][i] += v2; };
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) point_terms(q, add);
-        diffusion_terms(add);
-#pragma unroll
-        for (int a = 0; a < NV; ++a)
-#pragma unroll
-          for (int i = 0; i < BC; ++i) put(CORNER[p][a], i, o[a][i]);
-      } else {
-        // large blocks: straight into the sums, point by point
-        auto add = [&](int a, int i, T v2) { put(CORNER[p][a], i, v2); };
-#pragma unroll 1
-        for (int q = 0; q < NQ; ++q) point_terms(q, add);
-        diffusion_terms(add);
-      }
-    }
-  };
-  if constexpr (SCATTER) {
-    // one cell per thread; its corner sums are added to the vertices corner
-    // by corner: for one corner the cells' vertices are distinct, so no two
-    // threads add to the same vertex between two barriers, and every vertex
-    // sums its cells in corner order
-    const int cl = threadIdx.x;
-    const bool has = cl < NCB && cms[cl < NCB ? cl : 0];
-    T oc[NCORNER][BC];
-#pragma unroll
-    for (int c = 0; c < NCORNER; ++c)
-#pragma unroll
-      for (int i = 0; i < BC; ++i) oc[c][i] = T(0);
-    if (has) cell_corner_sums(cl, cms[cl], [&](int c, int i, T v2) { oc[c][i] += v2; });
-    const int ci = cl % CXB, cj = (cl / CXB) % CYB, ck = cl / (CXB * CYB);
-#pragma unroll
-    for (int c = 0; c < NCORNER; ++c) {
-      if (c) __syncthreads();
-      if (has) {
-        const int b = (ci + (c & 1)) + XB * ((cj + ((c >> 1) & 1)) + YB * (ck + ((c >> 2) & 1)));
-#pragma unroll
-        for (int i = 0; i < BC; ++i) accv[b * BC + i] += oc[c][i];
-      }
-    }
-  } else {
-    for (int cl = threadIdx.x; cl < NCB; cl += blockDim.x) {
-      T* dst = out + (size_t)cl * (NCORNER * BC);
-      for (int k = 0; k < NCORNER * BC; ++k) dst[k] = T(0);
-      const unsigned m = cms[cl];
-      if (!m) continue;
-      cell_corner_sums(cl, m, [&](int c, int i, T v2) { dst[c * BC + i] += v2; });
-    }
-  }
-  __syncthreads();
-  double d1 = 0.0, d2 = 0.0;
-  for (int it = threadIdx.x; it < NROW * DCG_B; it += blockDim.x) {
-    const int lr = it / DCG_B, i = it % DCG_B;
-    const int li = lr % TX, lj = DIM > 1 ? (lr / TX) % TY : 0, lk = DIM > 2 ? lr / (TX * TY) : 0;
-    const int vb = (li + 1) + XB * ((lj + OY) + YB * (lk + OZ));
-    const int s = sl[vb];
-    if (s < 0) continue;
-    T acc = T(0);
-    if (i < BC && SCATTER) {
-      acc = accv[(SCATTER ? vb : 0) * BC + i];
-    } else if (i < BC) {
-      for (int ez = 0; ez < (DIM > 2 ? 2 : 1); ++ez)
-        for (int ey = 0; ey < (DIM > 1 ? 2 : 1); ++ey)
-          for (int ex = 0; ex < 2; ++ex) {
-            const int cl = (li + ex) + CXB * ((lj + ey) + CYB * (lk + ez));
-            if (!cms[cl]) continue;
-            const int cv = (1 - ex) | (DIM > 1 ? (1 - ey) << 1 : 0) | (DIM > 2 ? (1 - ez) << 2 : 0);
-            acc += out[(SCATTER ? 0 : (size_t)cl * (NCORNER * BC) + cv * BC) + i];
-          }
-    } else {
-      acc = v[(size_t)s * DCG_B + i];  // padding species: identity rows
-    }
-    y[(size_t)s * DCG_B + i] = acc;
-    if (part) {
-      d1 += (double)acc * (double)w1[(size_t)s * DCG_B + i];
-      if (w2) d2 += (double)acc * (double)w2[(size_t)s * DCG_B + i];
-    }
-  }
-  if (part) {
-    // workgroup sums of the two partial dots (fixed order)
-    __shared__ double red[8];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      d1 += __shfl_down(d1, o, 64);
-      d2 += __shfl_down(d2, o, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-      red[2 * (threadIdx.x >> 6)] = d1;
-      red[2 * (threadIdx.x >> 6) + 1] = d2;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double a = 0.0, b = 0.0;
-      for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
-        a += red[2 * k];
-        b += red[2 * k + 1];
-      }
-      part[2 * blockIdx.x] = a;
-      part[2 * blockIdx.x + 1] = b;
-    }
-  }
+  cells_apply<T, T>(pcb, vs, sl, cms, out, accv, v, y, ca, dv, w1, w2, part);
 }
 }  // namespace %C%
 
