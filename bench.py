@@ -628,7 +628,8 @@ def main():
         "krylov_iterations_per_step": lin / args.steps,
         "kernels": kinds,
         "roofline": roofline,
-        "spmv_autotune": {"candidates_us": tuning, "chosen": chosen},
+        # the SpMV variants exist only for an assembled matrix
+        "spmv_autotune": {"candidates_us": tuning, "chosen": chosen} if "spmv64" in kinds or "spmv32" in kinds else None,
         "mixed_precision_same_workload": mixed,
         "copy_probe": probe,
         "cpu_baseline": cpu,
