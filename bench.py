@@ -445,6 +445,9 @@ def main():
                     help="Jacobi: the reference's (diagonal); VertexBlockJacobi: inverse of each vertex's species block")
     ap.add_argument("--profile-steps-single", type=int, default=2,
                     help="single models: steps after the timed ones with per-kernel HIP-event timing")
+    ap.add_argument("--readback-iterations", type=int, default=0,
+                    help="single-model workloads: BiCGSTAB iterations between read-backs (0: adaptive; 1 launches "
+                         "none past convergence, so a profiler's per-kernel averages count only working launches)")
     ap.add_argument("--storage", default="auto", choices=["matrix", "points", "auto"],
                     help="single models: the Jacobian as an assembled BCSR matrix, or as point coefficients per "
                          "cell applied by the Krylov operator (linear_solver.storage)")
@@ -492,6 +495,8 @@ def main():
     w, wname = make_workload(wl, args.workload, args.precision, args.preconditioner)
 
     storage = f"model.time_step_operator.linear_solver.storage = {args.storage}\n"
+    if args.readback_iterations:
+        storage += f"model.time_step_operator.linear_solver.readback_iterations = {args.readback_iterations}\n"
     t_setup = time.perf_counter()
     m = dca.Model(w.ini, storage, 0)
     u0 = w.initial_state(m)
@@ -622,7 +627,7 @@ def main():
         "data": "synthetic (seeded initial state)",
         "config": {"workload": wname, "n_dof": m.n_dof, "stages_per_step": w.stages,
                    "precision": args.precision, "preconditioner": args.preconditioner,
-                   "jacobian_storage": args.storage},
+                   "jacobian_storage": args.storage, "readback_iterations": args.readback_iterations},
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
         "newton_iterations_per_step": newton / args.steps,
         "krylov_iterations_per_step": lin / args.steps,

@@ -1051,6 +1051,10 @@ constexpr int NVB = XB * YB * ZB, NCB = CXB * CYB * CZB, NROW = TX * TY * TZ;
 constexpr int NST = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
 constexpr int OY = DIM > 1 ? 1 : 0, OZ = DIM > 2 ? 1 : 0;
 constexpr int MC_WORDS = NCB * NCLASS * NQ * (NMC > 0 ? NMC : 1);
+// the cells a tile owns (their lowest corner is one of its vertices): each
+// cell's stored point coefficients live once, in its owner's block
+constexpr int NOWN = TX * (DIM > 1 ? TY : 1) * (DIM > 2 ? TZ : 1);
+constexpr int OWN_WORDS = NCLASS * NQ * (NMC > 0 ? NMC : 1) * NOWN;
 constexpr int RL_WORDS = NCB * NCORNER * BC;
 // the Jacobian paths also pass row sums between the two lanes of a row
 constexpr int XCH_WORDS = NROW * ((NZB > BC ? NZB : BC) > 0 ? (NZB > BC ? NZB : BC) : 1);
@@ -1188,18 +1192,18 @@ __device__ __forceinline__ void block_rows(int dx, int dy, int dz, int li, int l
   }
 }
 
-// The rows of one tile from point coefficients pcb[((p*NQ + q)*NMC + n)*NCB +
-// box cell] (a tile's block in HBM, or the set-up's staged LDS array) and the
+// The rows of one tile from point coefficients cellp(box cell)[((p*NQ + q)*NMC +
+// n)*WS] (the owner tile's block in HBM, or the set-up's staged LDS array) and the
 // direction staged in vs over the vertex box (slots sl, class masks cms): per
 // cell, the direction at its corners interpolated to each point, multiplied
 // by the point's block and tested with the basis functions, plus the uniform
 // diffusion's element stiffness times the corner values; the cells' corner
 // sums folded onto the tile's rows.  y gets the rows (padding species rows
 // are the identity, from v); with part, the workgroup's partial dots (y.w1,
-// y.w2) go to part[2 blockIdx.x + 0/1].  LDS scratch: accv (zeroed by the
+// y.w2) go to part[0] and part[1].  LDS scratch: accv (zeroed by the
 // caller) when SCATTER, else out.
-template <typename T, typename C>
-__device__ __forceinline__ void cells_apply(const C* __restrict__ pcb, const T* vs, const int* sl,
+template <typename T, typename C, int WS, typename CP>
+__device__ __forceinline__ void cells_apply(CP cellp, const T* vs, const int* sl,
                                             const unsigned char* cms, T* out, T* accv, const T* __restrict__ v,
                                             T* __restrict__ y, double ca, const double (&dv)[NDV > 0 ? NDV : 1],
                                             const T* __restrict__ w1, const T* __restrict__ w2,
@@ -1212,6 +1216,7 @@ __device__ __forceinline__ void cells_apply(const C* __restrict__ pcb, const T* 
   // the corner sums of box cell cl over its classes, handed to put(corner, i, value)
   auto cell_corner_sums = [&](int cl, unsigned m, auto&& put) {
     const int ci = cl % CXB, cj = (cl / CXB) % CYB, ck = cl / (CXB * CYB);
+    const C* __restrict__ cb = cellp(cl);
 #pragma unroll
     for (int p = 0; p < NCLASS; ++p) {
       if (!((m >> p) & 1u)) continue;
@@ -1236,12 +1241,12 @@ __device__ __forceinline__ void cells_apply(const C* __restrict__ pcb, const T* 
           for (int a = 0; a < NV; ++a) sum += T(PHI[q][a]) * vcor(a, j);
           vq[j] = sum;
         }
-        const C* mp = pcb + (size_t)((p * NQ + q) * NMC) * NCB + cl;
+        const C* mp = cb + (size_t)((p * NQ + q) * NMC) * WS;
         T w[BC];
 #pragma unroll
         for (int i = 0; i < BC; ++i) w[i] = T(0);
 #pragma unroll
-        for (int n = 0; n < NMC; ++n) w[MCI[n]] += T(mp[(size_t)n * NCB]) * vq[MCJ[n]];
+        for (int n = 0; n < NMC; ++n) w[MCI[n]] += T(mp[(size_t)n * WS]) * vq[MCJ[n]];
 #pragma unroll
         for (int a = 0; a < NV; ++a)
 #pragma unroll
@@ -1375,8 +1380,8 @@ __device__ __forceinline__ void cells_apply(const C* __restrict__ pcb, const T* 
         a += red[2 * k];
         b += red[2 * k + 1];
       }
-      part[2 * blockIdx.x] = a;
-      part[2 * blockIdx.x + 1] = b;
+      part[0] = a;
+      part[1] = b;
     }
   }
 }
@@ -1471,12 +1476,16 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
     }
     __syncthreads();
     if constexpr (MODE == 8) {
-      // the coefficients of every cell of the tile's box, in the LDS layout,
-      // into the tile's own block of val (block = position in the launch's
-      // tile list): the apply then reads one contiguous block per tile; cells
-      // outside the compartment carry values that are never read
+      // the coefficients of the tile's owned cells into its block of val
+      // (block = position in the launch's tile list), word-major:
+      // val[block][(p*NQ + q)*NMC + n][owned cell]; owned cells outside the
+      // grid or the compartment carry values that are never read
       if constexpr (NMC > 0)
-        for (int k = threadIdx.x; k < MC_WORDS; k += blockDim.x) val[(size_t)blockIdx.x * MC_WORDS + k] = work[k];
+        for (int k = threadIdx.x; k < OWN_WORDS; k += blockDim.x) {
+          const int w = k / NOWN, o = k % NOWN;
+          const int oi = o % TX, oj = DIM > 1 ? (o / TX) % TY : 0, ok = DIM > 2 ? o / (TX * TY) : 0;
+          val[(size_t)blockIdx.x * OWN_WORDS + k] = work[(size_t)w * NCB + (oi + 1) + CXB * ((oj + OY) + CYB * (ok + OZ))];
+        }
       // diagonal block of every row (the Jacobi preconditioner's input):
       // r[slot * B * B + i * B + j], the identity on padding species
       constexpr int AI = BC > 4 ? 1 : BC, NR = BC / AI;
@@ -1508,7 +1517,8 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
       }
       __syncthreads();
       if constexpr (CELL_APPLY) {
-        cells_apply<double, double>(work, xs, sl, cms, nullptr, acc4, base, r, ca, dv, nullptr, nullptr, nullptr);
+        cells_apply<double, double, NCB>([&](int cl) { return (const double*)work + cl; }, xs, sl, cms, nullptr, acc4,
+                                         base, r, ca, dv, nullptr, nullptr, nullptr);
         return;
       }
     }
@@ -1849,8 +1859,11 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
   }
 }
 
-// y = J v from stored point coefficients (mode 8 of tile_body, one block per
-// tile): per cell of the tile's box, v at its corners is interpolated to each point, multiplied
+// y = J v from stored point coefficients (mode 8 of tile_body: each cell's
+// coefficients once, in the block of the tile that owns it; a box cell of
+// another tile is read from that tile's block, found through tpos, the
+// position of every tile of the grid in the launch's tile list):
+// per cell of the tile's box, v at its corners is interpolated to each point, multiplied
 // by the point's coefficient block and tested against the basis functions,
 // the uniform diffusion adds the element stiffness times v; the cells' corner
 // sums are folded onto the tile's rows as in the residual.  T is the value
@@ -1860,10 +1873,11 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
 // linearisation.  g2l null: the
 // compartment's slots are the grid's vertices in order.  With part,
 // the workgroup also writes its rows' partial dots (y.w1, y.w2) to
-// part[2 blockIdx.x + 0/1] (w2 may be null), as the BiCGSTAB finaliser reads.
+// part[0] and part[1] (w2 may be null), as the BiCGSTAB finaliser reads.
 template <typename T>
 __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,
-                                         const T* __restrict__ pc, const T* __restrict__ v, T* __restrict__ y,
+                                         const int* __restrict__ tpos, const T* __restrict__ pc,
+                                         const T* __restrict__ v, T* __restrict__ y,
                                          const double* __restrict__ tca, int n0, int n1, int n2,
                                          const double* __restrict__ prm, const T* __restrict__ w1,
                                          const T* __restrict__ w2, double* __restrict__ part) {
@@ -1876,14 +1890,13 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
   __shared__ T vs[NVB * BC];
   __shared__ int sl[NVB];
   __shared__ unsigned char cms[NCB];
+  __shared__ size_t coff[NCB];
   __shared__ T out[SCATTER ? 1 : NCB * NCORNER * BC];
   __shared__ T accv[SCATTER ? NVB * BC : 1];
   const int ntx = (n0 + 1 + TX - 1) / TX;
   const int nty = DIM > 1 ? (n1 + 1 + TY - 1) / TY : 1;
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
   const int vi0 = tx * TX - 1, vj0 = DIM > 1 ? ty * TY - 1 : 0, vk0 = DIM > 2 ? tz * TZ - 1 : 0;
-  // this tile's block of coefficients: [(p*NQ + q)*NMC + n][box cell]
-  const T* pcb = pc + (size_t)blockIdx.x * MC_WORDS;
   for (int b = threadIdx.x; b < NVB; b += blockDim.x) {
     const int bi = b % XB, bj = (b / XB) % YB, bk = b / (XB * YB);
     const int gi = vi0 + bi, gj = vj0 + bj, gk = vk0 + bk;
@@ -1901,14 +1914,27 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
     const int ci = c % CXB, cj = (c / CXB) % CYB, ck = c / (CXB * CYB);
     const int gi = vi0 + ci, gj = vj0 + cj, gk = vk0 + ck;
     unsigned char m = 0;
-    if (gi >= 0 && gi < n0 && gj >= 0 && gj < (DIM > 1 ? n1 : 1) && gk >= 0 && gk < (DIM > 2 ? n2 : 1))
+    size_t off = 0;
+    if (gi >= 0 && gi < n0 && gj >= 0 && gj < (DIM > 1 ? n1 : 1) && gk >= 0 && gk < (DIM > 2 ? n2 : 1)) {
       m = cmask[gi + n0 * (gj + n1 * gk)];
+      if (m) {
+        // the cell's coefficients: its owner tile's block (tpos null: the
+        // launch's tile list is every tile in order), word-minor by owned cell
+        const int ot = gi / TX + ntx * ((DIM > 1 ? gj / TY : 0) + nty * (DIM > 2 ? gk / TZ : 0));
+        const int pos = tpos ? tpos[ot] : ot;
+        const int loc = gi % TX + TX * ((DIM > 1 ? gj % TY : 0) + (DIM > 1 ? TY : 1) * (DIM > 2 ? gk % TZ : 0));
+        if (pos >= 0) off = (size_t)pos * OWN_WORDS + loc;
+        else m = 0;  // (the owner holds the cell's lowest vertex, so it is listed)
+      }
+    }
     cms[c] = m;
+    coff[c] = off;
   }
   __syncthreads();
   double dv[NDV > 0 ? NDV : 1];
   uniform_diffusion(t, prm, dv);
-  cells_apply<T, T>(pcb, vs, sl, cms, out, accv, v, y, ca, dv, w1, w2, part);
+  cells_apply<T, T, NOWN>([&](int cl) { return pc + coff[cl]; }, vs, sl, cms, out, accv, v, y, ca, dv, w1, w2,
+                         part);
 }
 }  // namespace %C%
 
@@ -1946,14 +1972,22 @@ extern "C" __global__ void __launch_bounds__(256)
   %C%::tile_body<8>(tiles[blockIdx.x], g2l, cmask, x, base, t, cm, ca, r, val, own_start, own_mask, n0, n1, n2, ox,
                     oy, oz, hx, hy, hz, mode, prm);
 }
+// The grid is a multiple of 8 workgroups; workgroup b takes list position
+// (b % 8) * (grid / 8) + b / 8, so each XCD (workgroups are dealt to the 8
+// XCDs in turn) walks one contiguous run of tiles and finds the neighbour
+// tiles' coefficients in its own L2.  Position c writes part[2c], part[2c+1].
 #define DCG_POINT_ENTRY(NAME, T)                                                                                  \
   extern "C" __global__ void __launch_bounds__(256)                                                               \
-  NAME(const int* __restrict__ tiles, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,         \
-       const T* __restrict__ pc, const T* __restrict__ v, T* __restrict__ y, const double* __restrict__ tca,       \
-       int n0, int n1, int n2, const double* __restrict__ prm, const int* __restrict__ stop,                      \
-       const T* __restrict__ w1, const T* __restrict__ w2, double* __restrict__ part) {                           \
+  NAME(const int* __restrict__ tiles, int ntile, const int* __restrict__ g2l,                                     \
+       const unsigned char* __restrict__ cmask, const int* __restrict__ tpos, const T* __restrict__ pc,            \
+       const T* __restrict__ v, T* __restrict__ y, const double* __restrict__ tca, int n0, int n1, int n2,         \
+       const double* __restrict__ prm, const int* __restrict__ stop, const T* __restrict__ w1,                     \
+       const T* __restrict__ w2, double* __restrict__ part) {                                                     \
     if (stop && *stop) return;                                                                                    \
-    %C%::pc_apply<T>(tiles[blockIdx.x], g2l, cmask, pc, v, y, tca, n0, n1, n2, prm, w1, w2, part);                \
+    const int c = (int)(blockIdx.x % 8u) * (int)(gridDim.x / 8u) + (int)(blockIdx.x / 8u);                       \
+    if (c >= ntile) return;                                                                                       \
+    %C%::pc_apply<T>(tiles[c], g2l, cmask, tpos, pc, v, y, tca, n0, n1, n2, prm, w1, w2,                          \
+                     part ? part + 2 * (size_t)c : nullptr);                                                      \
   }
 DCG_POINT_ENTRY(%KERNEL%_p, double)
 DCG_POINT_ENTRY(%KERNEL%_pf, float)

@@ -206,6 +206,8 @@ void Model::build(const std::string& ini, const std::string& overrides) {
     for (const auto& C : L_.comps) bmax = std::max(bmax, C.b);
     cfg_.points = why.empty() && (st == "points" || (st == "auto" && L_.grid.cube && bmax <= 4));
   }
+  cfg_.readback_iterations = c.get(p + "linear_solver.readback_iterations", 0);
+  if (cfg_.readback_iterations < 0) throw IOError("linear_solver.readback_iterations must be >= 0");
   cfg_.lin_rtol = c.get(p + "linear_solver.convergence_condition.relative_tolerance", 1e-4);
   auto lr = c.get_list(p + "linear_solver.convergence_condition.iteration_range");
   if (lr.size() == 2) cfg_.lin_maxit = static_cast<int>(lr[1]);
@@ -348,6 +350,11 @@ void Model::setup_device() {
     }
     comp_dev_[c].cell_mask_grid.upload(cm, stream_);
     comp_dev_[c].tiles.upload(tiles, stream_);
+    if (static_cast<long>(tiles.size()) != ntx * nty * ntz) {
+      std::vector<int> tpos(ntx * nty * ntz, -1);
+      for (size_t i = 0; i < tiles.size(); ++i) tpos[tiles[i]] = static_cast<int>(i);
+      comp_dev_[c].tpos.upload(tpos, stream_);
+    }
     comp_dev_[c].ntiles = static_cast<int>(tiles.size());
   }
   {
@@ -455,11 +462,11 @@ void Model::setup_device() {
     pc_.resize(L_.comps.size());
     if (cfg_.precision == Precision::Mixed) pc32_.resize(L_.comps.size());
     for (size_t c = 0; c < L_.comps.size(); ++c) {
-      // one block per tile of the cells of its box (halo cells included)
+      // one block per tile of the cells it owns (lowest corner among its
+      // vertices): every cell once
       const auto td = gen_.tile_dims[c];
-      const size_t box = static_cast<size_t>(td[0] + 1) * (L_.grid.dim > 1 ? td[1] + 1 : 1) *
-                         (L_.grid.dim > 2 ? td[2] + 1 : 1);
-      const size_t words = static_cast<size_t>(gen_.point_words[c]) * box * std::max(0, comp_dev_[c].ntiles);
+      const size_t own = static_cast<size_t>(td[0]) * (L_.grid.dim > 1 ? td[1] : 1) * (L_.grid.dim > 2 ? td[2] : 1);
+      const size_t words = static_cast<size_t>(gen_.point_words[c]) * own * std::max(0, comp_dev_[c].ntiles);
       pc_[c].resize(words);
       if (cfg_.precision == Precision::Mixed) pc32_[c].resize(words);
     }
@@ -815,11 +822,14 @@ int Model::launch_points_apply(const void* v, void* y, const int* stop, bool fp3
     const int* tiles = comp_dev_[c].tiles.get();
     const int* g2l = comp_dev_[c].g2l_identity ? nullptr : comp_dev_[c].g2l.get();
     const unsigned char* cmask = comp_dev_[c].cell_mask_grid.get();
+    const int* tpos = comp_dev_[c].tpos.size() ? comp_dev_[c].tpos.get() : nullptr;
     const void* pc = fp32 ? static_cast<const void*>(pc32_[c].get()) : static_cast<const void*>(pc_[c].get());
     double* pp = part && facet_fn_.empty() ? part + 2 * static_cast<size_t>(nparts) : nullptr;
-    void* args[] = {&tiles, &g2l, &cmask, &pc, &v, &y, &tca, &n0, &n1, &n2, &prm, &stop, &w1, &w2, &pp};
-    DCG_HIP(hipModuleLaunchKernel(fp32 ? pc_appf_fn_[c] : pc_app_fn_[c], comp_dev_[c].ntiles, 1, 1, 256, 1, 1, 0,
-                                  stream_, args, nullptr));
+    int ntile = comp_dev_[c].ntiles;
+    void* args[] = {&tiles, &ntile, &g2l, &cmask, &tpos, &pc, &v, &y, &tca, &n0, &n1, &n2, &prm, &stop, &w1, &w2, &pp};
+    const unsigned grid = static_cast<unsigned>((ntile + 7) / 8 * 8);  // the kernel maps workgroups to XCD runs
+    DCG_HIP(hipModuleLaunchKernel(fp32 ? pc_appf_fn_[c] : pc_app_fn_[c], grid, 1, 1, 256, 1, 1, 0, stream_, args,
+                                  nullptr));
     nparts += comp_dev_[c].ntiles;
   }
   if (facet_fn_.empty()) return nparts;
@@ -927,6 +937,7 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
     k::bicg_fin(plan_.parts(), part_.get(), 2, stage, sc_.get(), stream_);
     prof_end(kSpmv64, spmv_bytes(8, 8, 1));
   };
+  const int akind = cfg_.matrix_free ? kJacobianApply : use_points_ ? kPointApply64 : kSpmv64;
   auto iteration = [&]() {
     k::bicg_p<double>(B, ns, r, p, v, ph, dinv, sc_.get(), stream_);
     if (ilu) apply_ilu(p, ph);
@@ -938,7 +949,13 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
     k::bicg_xr<double>(B, ns, x, ph, sh, s, t, r, rh, sc_.get(), part_.get(), stream_);
     k::bicg_fin(nblk, part_.get(), 2, 4, sc_.get(), stream_);
   };
-  int done_its = 0, batch = std::max(2, std::min(last_krylov_its_, 64));
+  // iterations run in batches between read-backs of the scalars: the first
+  // as many as this solve took in the last time step (or the last solve)
+  const int slot = solve_index_++;
+  const int guess = slot < static_cast<int>(its_hint_.size()) ? its_hint_[slot] : last_krylov_its_;
+  const size_t mark = prof_mark(akind);
+  const int fixed = cfg_.readback_iterations;
+  int done_its = 0, batch = fixed ? fixed : std::max(2, std::min(guess, 64));
   for (;;) {
     for (int it = 0; it < batch; ++it) iteration();
     done_its += batch;
@@ -949,10 +966,14 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
       host.done = 2;
       break;
     }
-    batch = 4;
+    batch = fixed ? fixed : 4;
   }
+  // both operator applies of every iteration ran (a breakdown keeps them all)
+  if (host.done == 1 || host.done == 2) prof_keep(akind, mark, 2L * host.iter);
   *its = host.iter;
   last_krylov_its_ = std::max(2, host.iter);
+  if (slot >= static_cast<int>(its_hint_.size())) its_hint_.resize(slot + 1, 0);
+  its_hint_[slot] = last_krylov_its_;
   return host.done == 1 ? 0 : host.done;
 }
 
@@ -996,6 +1017,7 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
     k::bicg_fin(plan32_.parts(), part_.get(), 2, stage, sc_.get(), stream_);
     prof_end(kSpmv32, spmv_bytes(4, 4, 1));
   };
+  const int akind = use_points_ ? kPointApply32 : kSpmv32;
   auto iteration = [&]() {
     k::bicg_p<float>(B, ns, r, p, v, ph, dinv, sc_.get(), stream_);
     if (ilu) apply_ilu(p, ph);
@@ -1007,7 +1029,13 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
     k::bicg_xr<float>(B, ns, x, ph, sh, s, t, r, rh, sc_.get(), part_.get(), stream_);
     k::bicg_fin(nblk, part_.get(), 2, 4, sc_.get(), stream_);
   };
-  int done_its = 0, batch = std::max(2, std::min(last_krylov_its_, 64));
+  // iterations run in batches between read-backs of the scalars: the first
+  // as many as this solve took in the last time step (or the last solve)
+  const int slot = solve_index_++;
+  const int guess = slot < static_cast<int>(its_hint_.size()) ? its_hint_[slot] : last_krylov_its_;
+  const size_t mark = prof_mark(akind);
+  const int fixed = cfg_.readback_iterations;
+  int done_its = 0, batch = fixed ? fixed : std::max(2, std::min(guess, 64));
   for (;;) {
     for (int it = 0; it < batch; ++it) iteration();
     done_its += batch;
@@ -1018,10 +1046,14 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
       host.done = 2;
       break;
     }
-    batch = 4;
+    batch = fixed ? fixed : 4;
   }
+  // both operator applies of every iteration ran (a breakdown keeps them all)
+  if (host.done == 1 || host.done == 2) prof_keep(akind, mark, 2L * host.iter);
   *its = host.iter;
   last_krylov_its_ = std::max(2, host.iter);
+  if (slot >= static_cast<int>(its_hint_.size())) its_hint_.resize(slot + 1, 0);
+  its_hint_[slot] = last_krylov_its_;
   return host.done == 1 ? 0 : host.done;
 }
 
@@ -1087,6 +1119,7 @@ int Model::step(double t, double dt, StepStats* st) {
   StepStats local;
   StepStats& S = st ? *st : local;
   S = StepStats{};
+  solve_index_ = 0;
   const long n = L_.nslot * L_.B;
   const size_t bytes = static_cast<size_t>(n) * sizeof(double);
   DCG_HIP(hipMemcpyAsync(stage_[0].get(), u_.get(), bytes, hipMemcpyDeviceToDevice, stream_));
@@ -1279,6 +1312,17 @@ void Model::prof_end(int kind, double bytes) {
   DCG_HIP(hipEventRecord(p.pool[p.used + 1], stream_));
   p.used += 2;
   p.pending_bytes.push_back(bytes);
+}
+
+size_t Model::prof_mark(int kind) const { return prof_ ? prof_data_[kind].used / 2 : 0; }
+
+void Model::prof_keep(int kind, size_t mark, long keep) {
+  if (!prof_) return;
+  Prof& p = prof_data_[kind];
+  const size_t n = mark + static_cast<size_t>(std::max(0L, keep));
+  if (p.used / 2 <= n) return;
+  p.used = 2 * n;
+  p.pending_bytes.resize(n);
 }
 
 void Model::prof_collect() {

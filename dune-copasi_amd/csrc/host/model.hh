@@ -64,6 +64,10 @@ struct SolverConfig {
   // linear_solver.storage: the Jacobian as point coefficients per cell (the
   // BiCGSTAB operator applies them) instead of an assembled BCSR matrix
   bool points = false;
+  // linear_solver.readback_iterations: BiCGSTAB iterations launched between
+  // read-backs of its scalars (0: as many as the same solve took in the last
+  // time step, then 4); 1 launches none past convergence (for profilers)
+  int readback_iterations = 0;
   double ir_inner_rtol = 1e-3;  // FP32 Krylov tolerance inside refinement
   int ir_max = 50;
   std::vector<std::string> warnings;
@@ -218,6 +222,12 @@ class Model {
   void prof_begin(int kind);
   void prof_end(int kind, double bytes);
   void prof_collect();
+  // launches of `kind` recorded so far (not yet collected), and dropping the
+  // records after the first `keep` of them from `mark` on: a Krylov solve
+  // launches its iterations in batches, and the operator launches after its
+  // convergence return at once, so they are not launches of the kernel's work
+  size_t prof_mark(int kind) const;
+  void prof_keep(int kind, size_t mark, long keep);
 
  public:
   // algorithmic bytes of one SpMV on this model's matrix
@@ -252,6 +262,10 @@ class Model {
   double time_ = 0.0, dt_initial_ = 0.0, t_end_ = 0.0;
   std::map<std::string, std::vector<double>> vtk_steps_;  // times written per output path
   int last_krylov_its_ = 8;
+  // iterations of the n-th Krylov solve of the last time step (the first
+  // batch of the n-th solve of the next one), and the solve counter
+  std::vector<int> its_hint_;
+  int solve_index_ = 0;
 
   // device data
   DeviceArray<int> row_ptr_, col_, own_start_, diag_pos_, slot_b_, chunk_row_;
@@ -263,6 +277,9 @@ class Model {
     DeviceArray<unsigned char> cell_mask;
     // tiled kernel: vertex -> slot, cell -> class mask over the whole grid, tile list
     DeviceArray<int> g2l, tiles;
+    // position of every tile of the grid in `tiles` (-1: not listed); empty
+    // when `tiles` is every tile in order
+    DeviceArray<int> tpos;
     DeviceArray<unsigned char> cell_mask_grid;
     int ntiles = 0;
     bool g2l_identity = false;  // the compartment's slots are the grid's vertices in order

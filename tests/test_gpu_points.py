@@ -202,3 +202,28 @@ def test_random_requests_with_default_storage(dca, wl, seed):
         ug, _ = gpu.get_state()
         uo, _ = orc.get_state()
         assert max(rel_l2_per_species(gpu.compartments(), ug, uo)) <= 1e-8, k
+
+
+def test_readback_batches_change_nothing_but_launches(dca, wl):
+    """BiCGSTAB runs its iterations in batches between read-backs; applies
+    launched past convergence return at once and are not counted as applies.
+    Read-backs after every iteration (readback_iterations = 1) reach bit-equal
+    states with equal iterations, and both count exactly two applies per
+    BiCGSTAB iteration."""
+    w = wl.mitchell_schaefer(n=10, cell_type="cube")
+    runs = []
+    for extra in ("", "model.time_step_operator.linear_solver.readback_iterations = 1\n"):
+        m = dca.Model(w.ini, PTS + extra, 0)
+        m.set_state(w.initial_state(m), 0.0)
+        m.profile_enable(True)
+        t, lin, newton = 0.0, 0, 0
+        for _ in range(3):
+            st = m.step(t, w.dt)
+            assert st.status == 0
+            lin += st.linear_iterations
+            newton += st.newton_iterations
+            t += w.dt
+        assert m.profile("point_apply64")[0] == 2 * lin
+        runs.append((m.get_state()[0], lin, newton))
+    assert runs[0][1:] == runs[1][1:]
+    assert np.array_equal(runs[0][0], runs[1][0])

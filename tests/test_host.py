@@ -170,6 +170,10 @@ initial.expression = {text}
      "[model.scalar_field.u]\ncompartment = d\nstorage.expression = 1\n"
      "[model.time_step_operator]\nlinear_solver.storage = csr\n",
      "linear_solver.storage must be"),
+    ("[grid]\ndimension = 2\ncells = 4 4\ncell_type = cube\n[compartments]\nd.expression = 1\n"
+     "[model.scalar_field.u]\ncompartment = d\nstorage.expression = 1\n"
+     "[model.time_step_operator]\nlinear_solver.readback_iterations = -1\n",
+     "linear_solver.readback_iterations must be >= 0"),
 ])
 def test_errors_are_reported(dca, ini, msg):
     with pytest.raises(dca.DcgError, match=re.escape(msg)):
