@@ -467,8 +467,15 @@ void Batch::iterate(SolverGroup& g, bool fp32) {
   const size_t M = g.mem.size();
   std::vector<k::BicgScalars> sc(M);
   int done_its = 0;
-  int blocks = std::max(1, (g.est_its + K - 1) / K);
+  const int slot = g.call++;
+  const int est = slot < static_cast<int>(g.hints.size()) ? g.hints[slot] : g.est_its;
+  int blocks = std::max(1, (est + K - 1) / K);
   int most = 0;
+  auto remember = [&]() {
+    g.est_its = std::max(2, most);
+    if (slot >= static_cast<int>(g.hints.size())) g.hints.resize(slot + 1, 0);
+    g.hints[slot] = g.est_its;
+  };
   if (prof_) {
     // one iteration at a time, the SpMVs bracketed by events; the bytes are
     // those of the members still iterating
@@ -500,7 +507,7 @@ void Batch::iterate(SolverGroup& g, bool fp32) {
       ++done_its;
     }
     for (const auto& s2 : sc) most = std::max(most, s2.iter);
-    g.est_its = std::max(2, most);
+    remember();
     return;
   }
   for (;;) {
@@ -529,7 +536,7 @@ void Batch::iterate(SolverGroup& g, bool fp32) {
     if (done_its > opt_.lin_maxit + 4) break;  // unfinished members report "not converged"
     blocks = 1;
   }
-  g.est_its = std::max(2, most);
+  remember();
 }
 
 void Batch::solve(const std::vector<char>& on, const std::vector<double>& rtol, std::vector<int>& status,
@@ -611,6 +618,7 @@ void Batch::step(std::vector<StepStats>* stats) {
     if (stats) stats->clear();
     return;
   }
+  for (auto& g : sgroups_) g.call = 0;
   // per-member coefficients of every assembly call of this step
   coef_.assign(N, {});
   int max_stages = 0;

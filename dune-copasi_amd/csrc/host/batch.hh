@@ -111,6 +111,10 @@ class Batch {
     int variant64 = 0, variant32 = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // FP64, FP32 iteration blocks
     int est_its = 8;
+    // most iterations of any member in the n-th solve of the last step (the
+    // first block count of the n-th solve of this one), and the solve counter
+    std::vector<int> hints;
+    int call = 0;
   };
   struct AsmGroup {
     std::shared_ptr<RtcModule> module;
