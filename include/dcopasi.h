@@ -134,7 +134,9 @@ int dcg_linear_solve(dcg_model_t model, const double* jacobian, const double* b,
  * 4 preconditioner set-up, 5 matrix-free Jacobian apply, 6 point-coefficient
  * set-up, 7 FP64 and 8 FP32 point-coefficient apply (linear_solver.storage =
  * points).  Enabling resets the counters; `bytes` sums the algorithmic bytes
- * of the timed launches. */
+ * of the timed launches.  Krylov operator launches that follow a BiCGSTAB
+ * solve's convergence (the rest of a batch of iterations; they return at
+ * once) are not counted, so averages are over working launches. */
 int dcg_profile_enable(dcg_model_t model, int on);
 int dcg_profile_read(dcg_model_t model, int kind, long* launches, double* milliseconds, double* bytes);
 
