@@ -62,6 +62,7 @@ static double apply_fn(const std::string& f, const std::vector<double>& a) {
   if (f == "floor") return std::floor(x);
   if (f == "ceil") return std::ceil(x);
   if (f == "round") return std::round(x);
+  if (f == "trunc") return std::trunc(x);
   if (f == "sign") return double((x > 0) - (x < 0));
   throw IOError("unknown function '" + f + "'");
 }
@@ -430,7 +431,13 @@ Ex derivative(const Ex& e, const std::string& x) {
       }
       break;
     }
-    case Kind::Mod: r = D(A[0]); break;
+    case Kind::Mod:
+      // fmod(a, b) = a - trunc(a / b) b: the quotient is piecewise constant
+      if (depends_on(A[1], x))
+        r = make(Kind::Sub, {D(A[0]), make(Kind::Mul, {call("trunc", {make(Kind::Div, {A[0], A[1]})}), D(A[1])})});
+      else
+        r = D(A[0]);
+      break;
     case Kind::Cond: r = make(Kind::Cond, {A[0], D(A[1]), D(A[2])}); break;
     case Kind::Lt: case Kind::Le: case Kind::Gt: case Kind::Ge: case Kind::Eq: case Kind::Ne:
     case Kind::And: case Kind::Or: case Kind::Not:
@@ -454,7 +461,7 @@ Ex derivative(const Ex& e, const std::string& x) {
       else if (f == "log2") r = make(Kind::Div, {da, make(Kind::Mul, {a, C(std::log(2.0))})});
       else if (f == "sqrt") r = make(Kind::Div, {da, make(Kind::Mul, {C(2.0), e})});
       else if (f == "abs") r = make(Kind::Mul, {call("sign", {a}), da});
-      else if (f == "floor" || f == "ceil" || f == "round" || f == "sign") return nullptr;
+      else if (f == "floor" || f == "ceil" || f == "round" || f == "sign" || f == "trunc") return nullptr;
       else if (f == "pow") r = derivative(make(Kind::Pow, {A[0], A[1]}), x);
       else if (f == "atan2") {
         // d atan2(y, x) = (x dy - y dx) / (x^2 + y^2)
@@ -553,7 +560,7 @@ enum Code : int { PUSH_CONST = 1000, PUSH_SLOT, OP_KIND, OP_CALL };
 const std::vector<std::string>& fn_names() {
   static const std::vector<std::string> n = {"sin", "cos", "tan", "asin", "acos", "atan", "sinh", "cosh",
                                              "tanh", "exp", "log", "log10", "log2", "sqrt", "abs", "floor",
-                                             "ceil", "round", "sign", "pow", "atan2", "min", "max"};
+                                             "ceil", "round", "sign", "pow", "atan2", "min", "max", "trunc"};
   return n;
 }
 void compile_node(const Ex& e, const std::map<std::string, int>& slots, Bound& b) {

@@ -972,8 +972,10 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
   if (host.done == 1 || host.done == 2) prof_keep(akind, mark, 2L * host.iter);
   *its = host.iter;
   last_krylov_its_ = std::max(2, host.iter);
-  if (slot >= static_cast<int>(its_hint_.size())) its_hint_.resize(slot + 1, 0);
-  its_hint_[slot] = last_krylov_its_;
+  if (slot < kMaxSolveHints) {
+    if (slot >= static_cast<int>(its_hint_.size())) its_hint_.resize(slot + 1, 0);
+    its_hint_[slot] = last_krylov_its_;
+  }
   return host.done == 1 ? 0 : host.done;
 }
 
@@ -1052,8 +1054,10 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
   if (host.done == 1 || host.done == 2) prof_keep(akind, mark, 2L * host.iter);
   *its = host.iter;
   last_krylov_its_ = std::max(2, host.iter);
-  if (slot >= static_cast<int>(its_hint_.size())) its_hint_.resize(slot + 1, 0);
-  its_hint_[slot] = last_krylov_its_;
+  if (slot < kMaxSolveHints) {
+    if (slot >= static_cast<int>(its_hint_.size())) its_hint_.resize(slot + 1, 0);
+    its_hint_[slot] = last_krylov_its_;
+  }
   return host.done == 1 ? 0 : host.done;
 }
 
@@ -1485,11 +1489,20 @@ int Model::linear_solve(const double* jac, const double* b_packed, double* x_pac
   DCG_HIP(hipMemcpyAsync(packed_.get(), b_packed, L_.ndof * sizeof(double), hipMemcpyHostToDevice, stream_));
   k::unpack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), packed_.get(), r_.get(), stream_);
   int refs = 0;
-  // the given matrix is the operator, whatever the model's storage
-  const bool points = use_points_;
+  // the given matrix is the operator, whatever the model's storage; a solve
+  // outside a time step takes no place in the per-step iteration hints
+  struct Restore {
+    Model* m;
+    bool points;
+    int index;
+    ~Restore() {
+      m->use_points_ = points;
+      m->solve_index_ = index;
+    }
+  } restore{this, use_points_, solve_index_};
   use_points_ = false;
+  solve_index_ = kMaxSolveHints;
   int rc = krylov(r_.get(), z_.get(), rtol, maxit, its, &refs);
-  use_points_ = points;
   k::pack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), z_.get(), packed_.get(), stream_);
   DCG_HIP(hipMemcpyAsync(x_packed, packed_.get(), L_.ndof * sizeof(double), hipMemcpyDeviceToHost, stream_));
   synchronize();

@@ -266,6 +266,9 @@ class Model {
   // batch of the n-th solve of the next one), and the solve counter
   std::vector<int> its_hint_;
   int solve_index_ = 0;
+  // solves per time step that keep a hint (later ones, and solves outside a
+  // step, start from the last solve's count)
+  static constexpr int kMaxSolveHints = 4096;
 
   // device data
   DeviceArray<int> row_ptr_, col_, own_start_, diag_pos_, slot_b_, chunk_row_;

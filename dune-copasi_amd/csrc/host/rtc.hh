@@ -1,5 +1,8 @@
 // Runtime compilation of a model's generated assembly kernels with hipRTC,
-// with an on-disk cache of code objects keyed by a hash of source + options.
+// with an on-disk cache of code objects keyed by a hash of the source, the
+// options and the hipRTC / HIP versions; cache files carry a size and a
+// checksum, and a file that fails them or that the runtime will not load is
+// compiled again.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -9,8 +12,9 @@
 
 namespace dcopasi {
 
-// Compile `source` for gfx950; returns the code object bytes.
-std::vector<char> rtc_compile(const std::string& source, const std::string& name);
+// Compile `source` for gfx950 (or take it from the cache when use_cache and a
+// valid file exists); returns the code object bytes and refreshes the cache.
+std::vector<char> rtc_compile(const std::string& source, const std::string& name, bool use_cache = true);
 
 class RtcModule {
  public:
@@ -19,6 +23,7 @@ class RtcModule {
   RtcModule& operator=(const RtcModule&) = delete;
   ~RtcModule();
   void load(const std::vector<char>& code);
+  bool try_load(const std::vector<char>& code);  // false (nothing loaded) when the runtime refuses it
   hipFunction_t function(const std::string& name) const;
   bool loaded() const { return mod_ != nullptr; }
 

@@ -103,6 +103,11 @@ void Server::worker() {
   }
 }
 
+// Only the thread that calls tick() touches the batch; everything other
+// threads read (the request table, infos, the member map, byte and member
+// counts) changes under mu_, so submit, result, forget and stats may run
+// concurrently with a tick.  Device work (adding, stepping, reading back,
+// removing members) happens outside the lock.
 int Server::tick() {
   if (batch_->size() == 0) {
     std::unique_lock<std::mutex> lock(mu_);
@@ -112,13 +117,14 @@ int Server::tick() {
   // admission in arrival order while the footprint fits and a slot is free
   for (;;) {
     Request* r = nullptr;
+    std::unique_ptr<Model> m;
     {
       std::lock_guard<std::mutex> lock(mu_);
       if (ready_.empty()) break;
-      Request* front = req_[ready_.front()].get();
-      if (static_cast<int>(batch_->size()) >= opt_.max_members) break;
+      Request* front = req_.at(ready_.front()).get();
+      if (resident_ >= opt_.max_members) break;
       if (used_ + front->estimate > opt_.hbm_budget) {
-        if (batch_->size() > 0) break;  // wait for memory to free up
+        if (resident_ > 0) break;  // wait for memory to free up
         if (front->estimate > opt_.hbm_budget) {
           front->info.status = -1;
           front->info.error = "request does not fit the HBM budget";
@@ -129,26 +135,28 @@ int Server::tick() {
         }
       }
       r = front;
+      m = std::move(r->model);
       ready_.pop_front();
     }
+    int member = -1;
     try {
-      std::unique_ptr<Model> m = std::move(r->model);
       Model* mp = m.get();
-      r->member = batch_->add(std::move(m));
+      member = batch_->add(std::move(m));
       std::vector<double> u0(mp->n_dof());
       mp->initial_state(u0.data());
       mp->set_state(u0.data(), mp->time());
-      r->info.device_bytes = mp->device_bytes();
-      used_ += r->info.device_bytes;
-      by_member_[r->member] = r->id;
+      const size_t bytes = mp->device_bytes();
       std::lock_guard<std::mutex> lock(mu_);
+      r->member = member;
+      r->info.device_bytes = bytes;
+      used_ += bytes;
+      ++resident_;
+      by_member_[member] = r->id;
       r->info.t_admitted = now();
     } catch (const std::exception& e) {
-      if (r->member >= 0) {
-        batch_->remove(r->member);
-        r->member = -1;
-      }
+      if (member >= 0) batch_->remove(member);
       std::lock_guard<std::mutex> lock(mu_);
+      r->member = -1;
       r->info.status = -1;
       r->info.error = e.what();
       r->info.finished = true;
@@ -158,32 +166,37 @@ int Server::tick() {
   if (batch_->size() > 0) {
     std::vector<StepStats> st;
     batch_->step(&st);
-    ++ticks_;
     const auto ids = batch_->ids();
     std::vector<int> retire;
-    for (size_t k = 0; k < ids.size(); ++k) {
-      Request* r = req_[by_member_[ids[k]]].get();
-      if (st[k].status) {
-        r->info.status = st[k].status;
-        retire.push_back(ids[k]);
-        continue;
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      ++ticks_;
+      for (size_t k = 0; k < ids.size(); ++k) {
+        Request* r = req_.at(by_member_.at(ids[k])).get();
+        if (st[k].status) {
+          r->info.status = st[k].status;
+          retire.push_back(ids[k]);
+          continue;
+        }
+        r->info.steps_done += 1;
+        r->info.newton_iterations += st[k].newton_iterations;
+        r->info.linear_iterations += st[k].linear_iterations;
+        if (r->info.steps_done >= r->steps) retire.push_back(ids[k]);
       }
-      r->info.steps_done += 1;
-      r->info.newton_iterations += st[k].newton_iterations;
-      r->info.linear_iterations += st[k].linear_iterations;
-      if (r->info.steps_done >= r->steps) retire.push_back(ids[k]);
     }
     for (int mid : retire) {
-      Request* r = req_[by_member_[mid]].get();
       Model& m = batch_->member(mid);
-      r->state.resize(m.n_dof());
+      std::vector<double> state(m.n_dof());
       double t = 0;
-      m.get_state(r->state.data(), &t);
-      used_ -= std::min(used_, r->info.device_bytes);
+      m.get_state(state.data(), &t);
       batch_->remove(mid);
+      std::lock_guard<std::mutex> lock(mu_);
+      Request* r = req_.at(by_member_.at(mid)).get();
+      r->state = std::move(state);
+      used_ -= std::min(used_, r->info.device_bytes);
+      --resident_;
       by_member_.erase(mid);
       r->member = -1;
-      std::lock_guard<std::mutex> lock(mu_);
       r->info.finished = true;
       r->info.t_finished = now();
     }
@@ -218,7 +231,7 @@ Server::Stats Server::stats() {
   Stats s;
   s.queued = static_cast<int>(todo_.size());
   s.ready = static_cast<int>(ready_.size());
-  s.resident = static_cast<int>(batch_->size());
+  s.resident = resident_;
   for (const auto& [id, r] : req_)
     if (r->info.finished) ++s.finished;
   s.used_bytes = used_;

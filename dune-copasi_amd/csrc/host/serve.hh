@@ -54,6 +54,8 @@ class Server {
   int submit(const std::string& ini, const std::string& overrides, int steps);
   // one scheduling round: admit what fits, one step of every resident
   // request, retire finished ones; returns the requests not yet finished.
+  // One thread drives tick(); submit, result, forget and stats may be called
+  // from other threads meanwhile.
   // With nothing resident and nothing built yet it first waits (at most
   // 100 ms) for a worker to finish a request.
   int tick();
@@ -98,6 +100,7 @@ class Server {
   int next_ = 0;
   int building_ = 0;  // taken by a worker, not yet ready
   size_t used_ = 0;
+  int resident_ = 0;  // requests in the batch (read by stats() on any thread)
   long ticks_ = 0;
 };
 

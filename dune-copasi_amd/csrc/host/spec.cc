@@ -151,7 +151,7 @@ ModelSpec read_model(const ParameterTree& cfg, int dim, bool fold_constants) {
   if (jt == "analytical") m.jacobian = JacobianType::Analytical;
   else if (jt == "numerical") m.jacobian = JacobianType::Numerical;
   else if (jt == "symbolic") m.jacobian = JacobianType::Symbolic;
-  else throw IOError("The option 'model.jacobian.type' must be either 'analytical' or 'numerical'");
+  else throw IOError("The option 'model.jacobian.type' must be either 'analytical', 'numerical' or 'symbolic'");
   m.fd_epsilon = cfg.get("model.jacobian.epsilon", 1e-7);
   m.is_linear = cfg.get("model.is_linear", false);
   if (m.is_linear) m.jacobian = JacobianType::Analytical;  // local_operator.hh:252-253

@@ -290,7 +290,10 @@ typedef struct dcg_request_info {
 int dcg_server_create(const char* options_ini, int device, dcg_server_t* out);
 void dcg_server_destroy(dcg_server_t server);
 int dcg_server_submit(dcg_server_t server, const char* ini_text, const char* overrides, int steps, int* id);
-/* admit, advance every resident request one step, retire; *open = requests not finished */
+/* admit, advance every resident request one step, retire; *open = requests
+ * not finished.  One thread drives dcg_server_tick; dcg_server_submit,
+ * dcg_server_result, dcg_server_forget and dcg_server_stats may be called on
+ * other threads at the same time. */
 int dcg_server_tick(dcg_server_t server, int* open);
 /* info always; state (n == info.n_dof) once finished; returns 1 while unfinished */
 int dcg_server_result(dcg_server_t server, int id, double* state, size_t n, dcg_request_info* info);

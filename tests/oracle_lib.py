@@ -213,3 +213,22 @@ def rel_l2_per_species(model_info, a, b, weights=None):
             den = np.linalg.norm(sb)
             out.append(np.linalg.norm(sa - sb) / (den if den > 0 else 1.0))
     return out
+
+
+def rel_mass_l2_per_species(oracle_model, a, b, t=0.0):
+    """max over species of ||a-b||_M / ||b||_M, M the oracle's FP64 mass
+    matrix restricted to the species (the mass operator's Jacobian at b, which
+    is the consistent mass matrix times the storage coefficient)."""
+    _, vals = oracle_model.assemble(b, t, 1.0, 0.0)
+    M = oracle_model.csr(vals)
+    out = []
+    for comp in oracle_model.compartments():
+        n, off, bsp = comp["n_nodes"], comp["dof_offset"], comp["n_species"]
+        for i in range(bsp):
+            idx = off + i + bsp * np.arange(n)
+            Ms = M[idx][:, idx]
+            e = a[idx] - b[idx]
+            den = float(b[idx] @ (Ms @ b[idx]))
+            num = float(e @ (Ms @ e))
+            out.append(np.sqrt(max(num, 0.0) / den) if den > 0 else np.sqrt(max(num, 0.0)))
+    return out

@@ -1,17 +1,16 @@
 """Batches and run-time parameters on the device, through the C ABI.
 
 Every batch member must give the result the same model gives when it is
-stepped alone: members are compared with single FP64 models (folded
-constants, compile-time geometry) after the same steps, to 1e-8 relative L2
-per species -- the parity bar of the plain path, which the single model meets
-against the oracle (test_gpu_parity.py).  Batches are exercised with uneven
+stepped alone: members are compared with the CPU oracle and with single FP64
+device models (folded constants, compile-time geometry) after the same steps,
+to 1e-8 relative L2 per species (the parity bar).  Batches are exercised with uneven
 mesh sizes, different parameters, block sizes 2 and 4, one- and two-stage
 schemes, interface facets, and members joining and leaving between steps.
 """
 import numpy as np
 import pytest
 
-from tests.oracle_lib import rel_l2_per_species
+from tests.oracle_lib import OracleModel, rel_l2_per_species, rel_mass_l2_per_species
 
 pytestmark = pytest.mark.gpu
 
@@ -162,4 +161,50 @@ def test_batch_members_join_and_leave(dca, wl):
         assert abs(t - tref) < 1e-12
         err = max(rel_l2_per_species(info, u, ref))
         assert err < TOL, (w.name, err)
+    b.close()
+
+
+def _oracle_steps(w, steps, dt=None):
+    orc = OracleModel(w.ini)
+    orc.set_state(w.initial_state(orc), 0.0)
+    for _ in range(steps):
+        assert orc.step(dt or w.dt)[0] == 0
+    return orc
+
+
+def test_batch_members_match_oracle(dca, wl):
+    """Every member of a mixed-precision batch against the CPU oracle stepped
+    alone (the reference's Newton per model, make_step_operator.hh:246-288):
+    uneven Gray-Scott members up to side 128, a Mitchell-Schaefer member
+    (Alexander2, block size 2 in 3D) and a two-compartment member with
+    transmission (block size 4), in the nodal and the mass-matrix norm."""
+    print("case: batch members vs oracle", flush=True)
+    steps = 3
+    ws = wl.gray_scott_batch(count=4, seed=5, side_min=16, side_max=64, precision="mixed", steps=steps, **NL)
+    ws.append(wl.gray_scott(n=128, seed=9, precond="Jacobi", precision="mixed", steps=steps, **NL))
+    ws.append(wl.mitchell_schaefer(n=8, dt=0.5, **NL))
+    ws.append(wl.two_compartments(n=24, dt=0.05, **NL))
+    b = dca.Batch("precision = mixed\n", 0)
+    ids = [b.add(w.ini) for w in ws]
+    for i, w in zip(ids, ws):
+        b.set_time_step(i, w.dt)
+        mv = b.member(i)
+        mv.set_state(w.initial_state(mv), 0.0)
+        mv.close()
+    for s in range(steps):
+        st = b.step()
+        assert [x.status for x in st] == [0] * len(ws), s
+    for i, w in zip(ids, ws):
+        mv = b.member(i)
+        u, t = mv.get_state()
+        info = mv.compartments()
+        mv.close()
+        orc = _oracle_steps(w, steps)
+        uo, to = orc.get_state()
+        assert abs(t - to) < 1e-12
+        err = max(rel_l2_per_species(info, u, uo))
+        errm = max(rel_mass_l2_per_species(orc, u, uo))
+        print(f"{w.name}: vs oracle rel L2 {err:.3e} (mass norm {errm:.3e})", flush=True)
+        assert err < TOL and errm < TOL, (w.name, err, errm)
+        orc.close()
     b.close()

@@ -223,3 +223,25 @@ def test_time_steps_match_oracle(dca, wl, name, make, steps):
         err = max(rel_l2_per_species(gpu.compartments(), ug, uo))
         assert err <= 1e-8, (k, err)
         assert abs(tg - t) < 1e-12
+
+
+@pytest.mark.parametrize("dim,cell_type,n", [(2, "cube", 33), (2, "simplex", 27), (3, "cube", 10)])
+def test_transmission_residual_is_flux_times_interface_measure(dca, dim, cell_type, n):
+    """The device's facet residual against the closed form that pins the
+    oracle's: per species, the sum of the transmission residual at a constant
+    state is +-k_t (D - A) |Gamma_h| (local_operator.hh:918-927,956-963)."""
+    from tests.test_oracle_pins import _BALL, _DISK, _interface_measure, _transmission_ini
+
+    kt = 0.5
+    gpu = dca.Model(_transmission_ini(dim, cell_type, n, kt, *(_DISK if dim == 2 else _BALL)), "", 0)
+    gamma = _interface_measure(dim, cell_type, n)
+    cs = gpu.compartments()
+    x = np.empty(gpu.n_dof)
+    x[cs[0]["dof_offset"]: cs[0]["dof_offset"] + cs[0]["n_nodes"]] = 0.3
+    x[cs[1]["dof_offset"]: cs[1]["dof_offset"] + cs[1]["n_nodes"]] = 0.8
+    r, _ = gpu.assemble(x, 0.0, 0.0, 1.0, jacobian=False)
+    s_in = r[cs[0]["dof_offset"]: cs[0]["dof_offset"] + cs[0]["n_nodes"]].sum()
+    s_out = r[cs[1]["dof_offset"]: cs[1]["dof_offset"] + cs[1]["n_nodes"]].sum()
+    flux = kt * (0.3 - 0.8)
+    assert s_in == pytest.approx(flux * gamma, rel=1e-12)
+    assert s_out == pytest.approx(-flux * gamma, rel=1e-12)

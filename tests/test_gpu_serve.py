@@ -5,8 +5,9 @@ meshes, 2-8 species, one or two compartments with a transmission flux,
 random mass-action networks, implicit Euler and Alexander2.  Checked:
   * a request's result is the same alone and in a mix where other requests
     arrive, run beside it and leave (the serving correctness condition);
-  * every served (mixed-precision, batched) result matches a single FP64
-    model converged to round-off, to 1e-8 relative L2 per species;
+  * every served (mixed-precision, batched) result matches the CPU oracle
+    and a single FP64 device model, both converged to round-off, to 1e-8
+    relative L2 per species;
   * admission by HBM footprint: with a budget that holds one request at a
     time everything still finishes, one after the other.
 """
@@ -15,7 +16,7 @@ import time
 import numpy as np
 import pytest
 
-from tests.oracle_lib import rel_l2_per_species
+from tests.oracle_lib import OracleModel, rel_l2_per_species, rel_mass_l2_per_species
 
 pytestmark = pytest.mark.gpu
 
@@ -119,3 +120,43 @@ def test_admission_by_footprint(dca, wl):
                 continue
             assert a.device_bytes + est[j] <= budget, (i, j)
     srv.close()
+
+
+def test_served_results_match_oracle(dca, wl):
+    """Every served result of a mix of requests (mixed precision, batched,
+    arriving while others run; one and two compartments with a transmission
+    flux, 2D and 3D, both schemes) against the CPU oracle stepping the same
+    request alone, in the nodal and the mass-matrix norm."""
+    print("case: served vs oracle", flush=True)
+    seeds = MIX + [2, 5]
+    reqs = [wl.random_request(s, **SMALL) for s in seeds]
+    assert any(len(_info(dca, r.ini)) == 2 for r in reqs)
+    srv = dca.Server("max_members = 8\nworkers = 3\n", 0)
+    ids = [srv.submit(r.ini, r.steps) for r in reqs[:4]]
+    for _ in range(2):
+        srv.tick()
+    ids += [srv.submit(r.ini, r.steps) for r in reqs[4:]]
+    _drain(srv)
+    for rid, r in zip(ids, reqs):
+        u, info = srv.result(rid)
+        assert info.status == 0, (r.name, info.status)
+        orc = OracleModel(r.ini)
+        orc.set_state(orc_initial(dca, r.ini), 0.0)
+        for _ in range(r.steps):
+            assert orc.step(r.dt)[0] == 0
+        uo, _ = orc.get_state()
+        err = max(rel_l2_per_species(orc.compartments(), u, uo))
+        errm = max(rel_mass_l2_per_species(orc, u, uo))
+        print(f"{r.name}: served vs oracle rel L2 {err:.3e} (mass norm {errm:.3e})", flush=True)
+        assert err < 1e-8 and errm < 1e-8, (r.name, err, errm)
+        orc.close()
+    srv.close()
+
+
+def orc_initial(dca, ini):
+    """The request's initial state from its initial.expression keys (the
+    server starts every request from them)."""
+    m = dca.Model(ini, "", -1)
+    u = m.initial_state()
+    m.close()
+    return u

@@ -132,7 +132,7 @@ initial.expression = {text}
 @pytest.mark.parametrize("ini,msg", [
     ("[grid]\ndimension = 2\n[compartments]\nd.expression = 1\n", "Basis has dimension 0"),
     ("[grid]\ndimension = 2\n[compartments]\nd.expression = 1\n[model.scalar_field.u]\ncompartment = d\n"
-     "[model]\njacobian.type = foo\n", "must be either 'analytical' or 'numerical'"),
+     "[model]\njacobian.type = foo\n", "must be either 'analytical', 'numerical' or 'symbolic'"),
     ("[grid]\ndimension = 2\n[compartments]\nd.expression = 1\n[model.scalar_field.u]\ncompartment = d\n"
      "reaction.expression = u*(\n", "expression"),
     ("[grid]\ndimension = 2\n[compartments]\nd.expression = 1\n[model.scalar_field.u]\ncompartment = e\n",

@@ -619,3 +619,349 @@ evaluation.expression = u * integration_factor
     o2.set_state(np.full(o2.n_dof, 2.0), 0.0)
     v2, _, _ = o2.reduce()
     assert abs(v2["m"] - 2.0) <= 1e-13
+
+
+# ---------------------------------------------------------------- transmission and outflow residual
+# The facet terms (local_operator.hh:918-927,956-963) are pinned by what the
+# weak form fixes independently of the code: with T = k_t (D_in - A_out) on the
+# inside species and -T on the outside one (the sign convention of
+# test/two_disks.ini:24-35), the residual summed over a species' nodes is
+# T * |Gamma_h| for a constant state (the hat functions are a partition of
+# unity on every facet), where |Gamma_h| is the measure of the facets between
+# cells marked inside and outside, counted here from the cell-centre rule on
+# its own; along a straight interface the nodal residual is the 1D P1 mass
+# matrix applied to the interface values; the total of both species is
+# conserved by a time step, and D falls while D > A.
+def _transmission_ini(dim, cell_type, n, kt, comp_in, comp_out, rk="ImplicitEuler"):
+    cells = " ".join([str(n)] * dim)
+    return f"""[grid]
+dimension = {dim}
+cells = {cells}
+extensions = {" ".join(["1"] * dim)}
+cell_type = {cell_type}
+[compartments]
+inner.expression = {comp_in}
+outer.expression = {comp_out}
+[model.scalar_field.D]
+compartment = inner
+storage.expression = 1
+outflow.outer.expression = {kt}*(D - A)
+outflow.outer.jacobian.D.expression = {kt}
+outflow.outer.jacobian.A.expression = {-kt}
+[model.scalar_field.A]
+compartment = outer
+storage.expression = 1
+outflow.inner.expression = -{kt}*(D - A)
+outflow.inner.jacobian.D.expression = {-kt}
+outflow.inner.jacobian.A.expression = {kt}
+[model.time_step_operator]
+type = {rk}
+linear_solver.convergence_condition.relative_tolerance = 1e-14
+nonlinear_solver.convergence_condition.relative_tolerance = 1e-26
+nonlinear_solver.convergence_condition.absolute_tolerance = 1e-30
+"""
+
+
+_DISK = ("(position_x-0.5)^2 + (position_y-0.5)^2 < 0.09", "(position_x-0.5)^2 + (position_y-0.5)^2 >= 0.09")
+_BALL = ("(position_x-0.5)^2 + (position_y-0.5)^2 + (position_z-0.5)^2 < 0.09",
+         "(position_x-0.5)^2 + (position_y-0.5)^2 + (position_z-0.5)^2 >= 0.09")
+
+
+def _interface_measure(dim, cell_type, n):
+    """|Gamma_h|: measure of the facets between cells whose centres lie inside
+    and outside the disk / ball of radius 0.3 about the domain centre."""
+    h = 1.0 / n
+    inside = lambda c: float(np.sum((np.asarray(c) - 0.5) ** 2)) < 0.09
+    if cell_type == "cube":
+        idx = np.indices([n] * dim).reshape(dim, -1).T
+        mark = {tuple(i): inside((i + 0.5) * h) for i in idx}
+        total = 0.0
+        for i in idx:
+            for ax in range(dim):
+                j = i.copy()
+                j[ax] += 1
+                if j[ax] < n and mark[tuple(i)] != mark[tuple(j)]:
+                    total += h ** (dim - 1)
+        return total
+    assert dim == 2
+    # Kuhn triangles of each cell: (0,0)-(1,0)-(1,1) and (0,0)-(0,1)-(1,1)
+    edges = {}
+    for i in range(n):
+        for j in range(n):
+            for tri in (((0, 0), (1, 0), (1, 1)), ((0, 0), (0, 1), (1, 1))):
+                pts = [(i + a, j + b) for a, b in tri]
+                c = np.mean(np.array(pts, dtype=float), axis=0) * h
+                m = inside(c)
+                for a, b in ((0, 1), (1, 2), (0, 2)):
+                    e = tuple(sorted((pts[a], pts[b])))
+                    edges.setdefault(e, []).append(m)
+    total = 0.0
+    for (p, q), marks in edges.items():
+        if len(marks) == 2 and marks[0] != marks[1]:
+            total += h * math.hypot(q[0] - p[0], q[1] - p[1])
+    return total
+
+
+def _inner_measure(dim, cell_type, n):
+    """Measure of the cells whose centres lie in the disk / ball of radius 0.3."""
+    h = 1.0 / n
+    centres = [np.array(c) + 0.5 for c in np.ndindex(*([n] * dim))]
+    if cell_type == "cube":
+        return sum(h ** dim for c in centres if np.sum((c * h - 0.5) ** 2) < 0.09)
+    total = 0.0
+    for i, j in np.ndindex(n, n):
+        for c in ((i + 2 / 3, j + 1 / 3), (i + 1 / 3, j + 2 / 3)):
+            if np.sum((np.array(c) * h - 0.5) ** 2) < 0.09:
+                total += 0.5 * h * h
+    return total
+
+
+def _vertex_xyz(o):
+    """Coordinates of every grid vertex (rows = vertex ids)."""
+    L = load()
+    n = L.orc_n_vertices(o._h)
+    xyz = np.zeros((n, 3))
+    L.orc_vertex_coords(o._h, ctypes.c_void_p(xyz.ctypes.data))
+    return xyz
+
+
+def _species_sums(o, r):
+    sums = []
+    for c in o.compartments():
+        b, off, nn = c["n_species"], c["dof_offset"], c["n_nodes"]
+        sums.append(r[off: off + b * nn].reshape(nn, b).sum(axis=0))
+    return np.concatenate(sums)
+
+
+@pytest.mark.parametrize("dim,cell_type,n", [(2, "cube", 20), (2, "cube", 33), (2, "simplex", 20), (2, "simplex", 27),
+                                             (3, "cube", 10)])
+def test_transmission_residual_is_flux_times_interface_measure(dim, cell_type, n):
+    kt = 0.5
+    comp = _DISK if dim == 2 else _BALL
+    o = OracleModel(_transmission_ini(dim, cell_type, n, kt, *comp))
+    gamma = _interface_measure(dim, cell_type, n)
+    assert gamma > 0.5  # about the circumference 1.885 (2D) / area 1.131 (3D) of the exact interface
+    for d_val, a_val in [(1.0, 0.0), (0.3, 0.8), (2.0, 2.0)]:
+        x = np.empty(o.n_dof)
+        cs = o.compartments()
+        x[cs[0]["dof_offset"]: cs[0]["dof_offset"] + cs[0]["n_nodes"]] = d_val
+        x[cs[1]["dof_offset"]: cs[1]["dof_offset"] + cs[1]["n_nodes"]] = a_val
+        r, _ = o.assemble(x, 0.0, 0.0, 1.0, jacobian=False)
+        s = _species_sums(o, r)
+        flux = kt * (d_val - a_val)
+        np.testing.assert_allclose(s[0], flux * gamma, rtol=1e-12, atol=1e-14)
+        np.testing.assert_allclose(s[1], -flux * gamma, rtol=1e-12, atol=1e-14)
+        # the mass operator carries no facet term: its sums are the values
+        # times the measure of each compartment's cells
+        rm, _ = o.assemble(x, 0.0, 1.0, 0.0, jacobian=False)
+        vol_in = _inner_measure(dim, cell_type, n)
+        np.testing.assert_allclose(_species_sums(o, rm), [d_val * vol_in, a_val * (1.0 - vol_in)], rtol=1e-12)
+
+
+@pytest.mark.parametrize("cell_type", ["cube", "simplex"])
+def test_transmission_residual_along_a_straight_interface(cell_type):
+    # inner = x < 0.5 on an 8 x 8 grid: the interface is the line x = 0.5; with
+    # D = 1 + y and A = y^2 the nodal residual of D at (0.5, y_j) is the
+    # integral of k_t (D_h - A_h) phi_j over the line, D_h and A_h the linear
+    # interpolants on each segment (a degree-2 integrand, so any rule of
+    # order 2 gives it exactly; it is evaluated here with 2-point Gauss)
+    n, kt = 8, 0.5
+    o = OracleModel(_transmission_ini(2, cell_type, n, kt, "position_x < 0.5", "position_x >= 0.5"))
+    h = 1.0 / n
+    cs = o.compartments()
+    xyz = _vertex_xyz(o)
+    x = np.zeros(o.n_dof)
+    for c, comp in enumerate(cs):
+        ids = o.compartment_nodes(c)
+        y = xyz[ids, 1]
+        x[comp["dof_offset"]: comp["dof_offset"] + comp["n_nodes"]] = (1.0 + y) if c == 0 else y ** 2
+    r, _ = o.assemble(x, 0.0, 0.0, 1.0, jacobian=False)
+    g = np.array([0.5 - 0.5 / math.sqrt(3.0), 0.5 + 0.5 / math.sqrt(3.0)])
+    for c, comp in enumerate(cs):
+        ids = o.compartment_nodes(c)
+        rc = r[comp["dof_offset"]: comp["dof_offset"] + comp["n_nodes"]]
+        on = np.abs(xyz[ids, 0] - 0.5) < 1e-12
+        np.testing.assert_allclose(rc[~on], 0.0, atol=1e-15)
+        sign = 1.0 if c == 0 else -1.0
+        for node, yj in zip(np.nonzero(on)[0], xyz[ids[on], 1]):
+            ref = 0.0
+            for lo in (yj - h, yj):
+                if lo < -1e-12 or lo + h > 1 + 1e-12:
+                    continue
+                for gq in g:
+                    yq = lo + gq * h
+                    phi = 1.0 - abs(yq - yj) / h
+                    # D and A are interpolated linearly on the segment
+                    d_h = 1.0 + yq
+                    a_h = np.interp(yq, [lo, lo + h], [lo ** 2, (lo + h) ** 2])
+                    ref += 0.5 * h * kt * (d_h - a_h) * phi
+            assert rc[node] == pytest.approx(sign * ref, rel=1e-12, abs=1e-15)
+
+
+@pytest.mark.parametrize("cell_type", ["cube", "simplex"])
+def test_transmission_conserves_and_relaxes(cell_type):
+    kt = 0.5
+    o = OracleModel(_transmission_ini(2, cell_type, 16, kt, *_DISK))
+    cs = o.compartments()
+    x = np.empty(o.n_dof)
+    x[cs[0]["dof_offset"]: cs[0]["dof_offset"] + cs[0]["n_nodes"]] = 1.0
+    x[cs[1]["dof_offset"]: cs[1]["dof_offset"] + cs[1]["n_nodes"]] = 0.0
+    o.set_state(x, 0.0)
+    m0 = o.mass(x)
+    total0 = m0.sum()
+    prev = m0
+    for _ in range(6):
+        assert o.step(0.2)[0] == 0
+        u, _ = o.get_state()
+        m = o.mass(u)
+        assert m.sum() == pytest.approx(total0, rel=1e-12)
+        assert m[0] < prev[0] and m[1] > prev[1]  # D flows out while D > A
+        prev = m
+
+
+def _outflow_ini(dim, cell_type, n, k):
+    cells = " ".join([str(n)] * dim)
+    return f"""[grid]
+dimension = {dim}
+cells = {cells}
+extensions = {" ".join(["1"] * dim)}
+cell_type = {cell_type}
+[compartments]
+domain.expression = 1
+[model.scalar_field.u]
+compartment = domain
+storage.expression = 1
+outflow.domain.expression = {k}*u
+outflow.domain.jacobian.u.expression = {k}
+[model.time_step_operator]
+type = ImplicitEuler
+linear_solver.convergence_condition.relative_tolerance = 1e-14
+nonlinear_solver.convergence_condition.relative_tolerance = 1e-26
+nonlinear_solver.convergence_condition.absolute_tolerance = 1e-30
+"""
+
+
+@pytest.mark.parametrize("dim,cell_type", [(2, "cube"), (2, "simplex"), (3, "cube"), (3, "simplex")])
+def test_boundary_outflow_is_flux_times_boundary_measure(dim, cell_type):
+    # outflow.<own compartment> on the domain boundary (local_operator.hh:850-852):
+    # sum of the residual = k u |dOmega| for a constant u, zero at interior nodes,
+    # and a corner node of the unit square gets k u h (two half edges)
+    k, n, u = 0.7, 5, 1.3
+    o = OracleModel(_outflow_ini(dim, cell_type, n, k))
+    x = np.full(o.n_dof, u)
+    r, _ = o.assemble(x, 0.0, 0.0, 1.0, jacobian=False)
+    assert r.sum() == pytest.approx(k * u * 2 * dim, rel=1e-12)
+    xyz = _vertex_xyz(o)[o.compartment_nodes(0), :dim]
+    interior = np.all((xyz > 1e-12) & (xyz < 1 - 1e-12), axis=1)
+    np.testing.assert_allclose(r[interior], 0.0, atol=1e-15)  # round-off of the basis on far facets
+    if dim == 2:
+        corner = np.all(np.abs(xyz) < 1e-12, axis=1)
+        assert r[corner][0] == pytest.approx(k * u / n, rel=1e-12)
+        # outflow of u = 1 decays the total as d/dt int u = -k int_dOmega u
+        o.set_state(np.ones(o.n_dof), 0.0)
+        assert o.step(0.1)[0] == 0
+        u1, _ = o.get_state()
+        r1, _ = o.assemble(u1, 0.1, 0.0, 1.0, jacobian=False)
+        assert o.mass(u1).sum() == pytest.approx(o.mass(np.ones(o.n_dof)).sum() - 0.1 * r1.sum(), rel=1e-12)
+
+
+# ---------------------------------------------------------------- linear defect correction
+# model.is_linear = true replaces Newton by one residual, one Jacobian and one
+# linear solve per stage (make_step_operator.hh:210-245).  For a linear model
+# that is the exact stage solution up to the linear tolerance, so the closed
+# forms of test/exp.ini hold unchanged, and the heat kernel of test/gauss.ini
+# keeps the reference's own error bound (its u_error key, limit 0.5).
+@pytest.mark.parametrize("cell_type", ["simplex", "cube"])
+@pytest.mark.parametrize("rk", ["ImplicitEuler", "Alexander2"])
+def test_linear_defect_correction_closed_form(cell_type, rk):
+    o = OracleModel(_exp_ini(2, cell_type, rk, D=0.3), "model.is_linear = true")
+    o.set_state(np.ones(o.n_dof), 0.0)
+    dt, n = 0.1, 6
+    for _ in range(n):
+        rc, newton, _ = o.step(dt)
+        assert rc == 0
+        assert newton == (1 if rk == "ImplicitEuler" else 2)  # one correction per stage
+    u, _ = o.get_state()
+    z = -2.0 * dt
+    R = 1.0 / (1.0 - z) if rk == "ImplicitEuler" else (1 + (1 - 2 * ALPHA) * z) / (1 - ALPHA * z) ** 2
+    np.testing.assert_allclose(u, R ** n, rtol=1e-12, atol=1e-14)
+
+
+def heat_kernel_ini(cell_type="simplex", level=5):
+    """The model of the reference's test/gauss.ini (heat kernel of a point
+    source from t = 1, diffusion 0.005, on [-1, 1]^2, model.is_linear = true,
+    with its max/min/error reduce keys), with a tight linear tolerance."""
+    return f"""[grid]
+origin = -1 -1
+extensions = 2 2
+refinement_level = {level}
+cell_type = {cell_type}
+[compartments.domain]
+type = expression
+expression = 1
+[parser_context.diffusion]
+type = constant
+value = 0.005
+[parser_context.gauss]
+type = function
+expression = x, y, z, t: exp(-(x^2+y^2+z^2)/(4*t*diffusion)) / (4*3.14159265359*t*diffusion)
+[model]
+is_linear = true
+[model.scalar_field.u]
+compartment = domain
+storage.expression = 1
+cross_diffusion.u.expression = diffusion
+initial.expression = gauss(position_x, position_y, position_z, time)
+[model.time_step_operator]
+time_begin = 1
+time_end = 1.2
+linear_solver.convergence_condition.relative_tolerance = 1e-13
+linear_solver.convergence_condition.iteration_range = 1 5000
+nonlinear_solver.convergence_condition.relative_tolerance = 1e-26
+nonlinear_solver.convergence_condition.absolute_tolerance = 1e-30
+[model.reduce]
+u_max.evaluation.expression = u
+u_max.reduction.expression = init, val: max(init, val)
+u_max.initial.value = -1e100
+u_max.error.expression = arg: arg > 1/(4*3.14159265359*diffusion)
+u_min.evaluation.expression = u
+u_min.reduction.expression = init, val: min(init, val)
+u_min.initial.value = 1e100
+u_min.warn.expression = arg: arg < 0
+u_min.error.expression = arg: arg < -1e-2
+u_error.evaluation.expression = (u - gauss(position_x, position_y, position_z, time))^2 * integration_factor
+u_error.transformation.expression = arg: sqrt(arg)
+u_error.error.expression = arg: arg > 0.50
+"""
+
+
+def _heat_initial(o, level=5):
+    n = 2 ** level
+    xyz = _vertex_xyz(o)[o.compartment_nodes(0)]
+    assert xyz[:, 0].min() == -1.0 and xyz[:, 0].max() == 1.0 and o.n_dof == (n + 1) ** 2
+    r2 = (xyz ** 2).sum(axis=1)
+    return np.exp(-r2 / (4 * 1.0 * 0.005)) / (4 * 3.14159265359 * 1.0 * 0.005)
+
+
+@pytest.mark.parametrize("cell_type", ["simplex", "cube"])
+def test_heat_kernel_linear_path_keeps_the_reference_bound(cell_type):
+    o_lin = OracleModel(heat_kernel_ini(cell_type))
+    o_new = OracleModel(heat_kernel_ini(cell_type), "model.is_linear = false")
+    u0 = _heat_initial(o_lin)
+    for o in (o_lin, o_new):
+        o.set_state(u0, 1.0)
+    vals, flags, rc = o_lin.reduce()
+    assert rc == 0 and vals["u_error"] < 0.5
+    for _ in range(4):
+        rc, newton, _ = o_lin.step(0.05)
+        assert rc == 0 and newton == 2  # Alexander2: one correction per stage
+        assert o_new.step(0.05)[0] == 0
+    ul, t = o_lin.get_state()
+    un, _ = o_new.get_state()
+    assert t == pytest.approx(1.2)
+    # the linear path and Newton solve the same linear stage equations
+    assert np.linalg.norm(ul - un) <= 1e-9 * np.linalg.norm(un)
+    vals, flags, rc = o_lin.reduce()
+    assert rc == 0 and 0.0 < vals["u_error"] < 0.5 and flags["u_max"] == 0
+    # the heat kernel decays: its peak 1/(4 pi D t) falls from t = 1 to 1.2
+    assert vals["u_max"] < 1.0 / (4 * math.pi * 0.005)
