@@ -35,10 +35,28 @@ OUTFLOW = {"D": ("0.5*(D^2 - A) + 0.1*D*exp(-A)", {"D": "D + 0.1*exp(-A)", "A": 
            "A": ("-0.5*(D^2 - A) - 0.1*D*exp(-A)", {"D": "-D - 0.1*exp(-A)", "A": "0.5 + 0.1*D*exp(-A)"})}
 
 
-def rules_ini(cell_type="cube", n=6, symbolic=True, with_diffusion=False):
+# the discontinuous terms (floor, fmod) and their partials: dropped for time
+# steps, where Newton needs a residual without jumps
+_JUMPS = {"u": (" + (u*v) % 0.3", {"u": " + v", "v": " + u"}), "v": (" + 3*floor(4*u)", {})}
+
+
+def _reaction(s, smooth):
+    r, rj = REACTIONS[s]
+    if not smooth:
+        return r, rj
+    term, parts = _JUMPS[s]
+    assert r.endswith(term)
+    rj = {w: (e[: -len(parts[w])] if w in parts else e) for w, e in rj.items()}
+    for w, tail in parts.items():
+        assert REACTIONS[s][1][w].endswith(tail)
+    return r[: -len(term)], rj
+
+
+def rules_ini(cell_type="cube", n=6, symbolic=True, with_diffusion=False, smooth=False):
     """Two species u, v with REACTIONS (and STORAGE on u); with_diffusion adds
     the state-dependent DIFFUSION of u.  symbolic: no jacobian keys and
-    model.jacobian.type = symbolic; else the hand-written keys."""
+    model.jacobian.type = symbolic; else the hand-written keys.  smooth drops
+    the floor and fmod terms (for time steps)."""
     lines = [f"[grid]\ndimension = 2\ncells = {n} {n}\ncell_type = {cell_type}\n[compartments]\ndomain.expression = 1\n"]
     if symbolic:
         lines.append("[model]\njacobian.type = symbolic\n")
@@ -55,7 +73,7 @@ def rules_ini(cell_type="cube", n=6, symbolic=True, with_diffusion=False):
                 sec += [f"cross_diffusion.{s}.jacobian.{w}.expression = {e}" for w, e in dj.items()]
         else:
             sec.append(f"cross_diffusion.{s}.expression = {'2e-3' if s == 'u' else '1e-3'}")
-        r, rj = REACTIONS[s]
+        r, rj = _reaction(s, smooth)
         sec.append(f"reaction.expression = {r}")
         if not symbolic:
             sec += [f"reaction.jacobian.{w}.expression = {e}" for w, e in rj.items()]

@@ -16,11 +16,11 @@ from tests.oracle_lib import OracleModel, bcsr_to_dense_csr, rel_l2_per_species
 pytestmark = pytest.mark.gpu
 
 
-def _uv_state(model, seed):
+def _uv_state(model, seed, u_range=(0.6, 1.4), v_range=(0.25, 0.75)):
     rng = np.random.default_rng(seed)
     u = np.empty(model.n_dof)
-    u[0::2] = rng.uniform(0.6, 1.4, model.n_dof // 2)
-    u[1::2] = rng.uniform(0.25, 0.75, model.n_dof // 2)
+    u[0::2] = rng.uniform(*u_range, model.n_dof // 2)
+    u[1::2] = rng.uniform(*v_range, model.n_dof // 2)
     return u
 
 
@@ -44,9 +44,13 @@ def test_symbolic_rules_match_explicit_keys(dca, cell_type, with_diffusion):
     keys = sc.rules_ini(cell_type, symbolic=False, with_diffusion=with_diffusion)
     probe = dca.Model(sym, "", -1)
     x = _uv_state(probe, 3)
-    probe.close()
+    _compare(dca, sym, keys, x)
+    # time steps of the model without its jumps (floor, fmod), from a state
+    # on one side of its conditional (u > v), so Newton meets a smooth residual
+    sym = sc.rules_ini(cell_type, symbolic=True, with_diffusion=with_diffusion, smooth=True)
+    keys = sc.rules_ini(cell_type, symbolic=False, with_diffusion=with_diffusion, smooth=True)
+    x = _uv_state(probe, 3, (0.9, 1.4), (0.25, 0.6))
     gpu, orc = _compare(dca, sym, keys, x)
-    # and time steps from there
     gpu.set_state(x, 0.0)
     orc.set_state(x, 0.0)
     t = 0.0
@@ -55,6 +59,7 @@ def test_symbolic_rules_match_explicit_keys(dca, cell_type, with_diffusion):
         assert orc.step(0.01)[0] == 0
         t += 0.01
         assert max(rel_l2_per_species(gpu.compartments(), gpu.get_state()[0], orc.get_state()[0])) <= 1e-8, k
+    probe.close()
 
 
 @pytest.mark.parametrize("cell_type", ["cube", "simplex"])
