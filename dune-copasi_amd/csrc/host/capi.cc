@@ -289,6 +289,19 @@ int dcg_linear_solve(dcg_model_t h, const double* jacobian, const double* b, dou
   });
 }
 
+static_assert(DCG_NOALLOC_WORKSPACE_BYTES == dcopasi::k::kNoallocScratchBytes, "work space size of the header");
+
+int dcg_linear_solve_noalloc(dcg_model_t h, const double* jacobian, const double* b, double* x, double rtol, int maxit,
+                             int* iterations) {
+  if (check(h)) return -1;
+  return guarded([&] {
+    int its = 0;
+    const int rc = h->m->linear_solve_noalloc(jacobian, b, x, rtol, maxit, &its);
+    if (iterations) *iterations = its;
+    return rc;
+  });
+}
+
 int dcg_profile_enable(dcg_model_t h, int on) {
   if (check(h)) return -1;
   return guarded([&] {

@@ -871,83 +871,97 @@ double Model::residual_norm(const double* r) {
 }
 
 // ------------------------------------------------------------------ Krylov
-void Model::precond_setup() {
+Model::KrylovWork Model::own_work() {
+  KrylovWork w;
+  for (int i = 0; i < 9; ++i) w.kv[i] = kv_[i].get();
+  w.part = part_.get();
+  w.sc = sc_.get();
+  w.val = val_.get();
+  w.dinv = dinv_.get();
+  w.lu = lu_.get();
+  w.status = status_.get();
+  return w;
+}
+
+void Model::precond_setup(const KrylovWork& w) {
   const int B = L_.B;
   const int ns = static_cast<int>(L_.nslot);
   if (jacobi_kind() && use_points_) {
-    k::block_inverse(B, ns, pdiag_.get(), diag_id_.get(), dinv_.get(), status_.get(),
-                     cfg_.precond == Preconditioner::Jacobi, stream_);
+    k::block_inverse(B, ns, pdiag_.get(), diag_id_.get(), w.dinv, w.status, cfg_.precond == Preconditioner::Jacobi,
+                     stream_);
   } else if (jacobi_kind()) {
-    k::block_inverse(B, ns, val_.get(), diag_pos_.get(), dinv_.get(), status_.get(),
-                     cfg_.precond == Preconditioner::Jacobi, stream_);
+    k::block_inverse(B, ns, w.val, diag_pos_.get(), w.dinv, w.status, cfg_.precond == Preconditioner::Jacobi, stream_);
   } else if (cfg_.precond == Preconditioner::ILU0) {
-    DCG_HIP(hipMemcpyAsync(lu_.get(), val_.get(), val_.bytes(), hipMemcpyDeviceToDevice, stream_));
+    DCG_HIP(hipMemcpyAsync(w.lu, w.val, matrix_values() * sizeof(double), hipMemcpyDeviceToDevice, stream_));
     for (size_t l = 0; l + 1 < lo_ptr_.size(); ++l)
       k::ilu_factor(B, lo_ptr_[l + 1] - lo_ptr_[l], lo_rows_.get() + lo_ptr_[l], row_ptr_.get(), col_.get(),
-                    diag_pos_.get(), lu_.get(), dinv_.get(), status_.get(), stream_);
+                    diag_pos_.get(), w.lu, w.dinv, w.status, stream_);
   }
 }
 
-int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, int maxit, int* its) {
+int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, int maxit, int* its,
+                      const KrylovWork& w) {
   const int B = L_.B;
   const int ns = static_cast<int>(L_.nslot);
   const long n = static_cast<long>(ns) * B;
-  double *r = kv_[0].get(), *rh = kv_[1].get(), *p = kv_[2].get(), *v = kv_[3].get(), *ph = kv_[4].get(),
-         *s = kv_[5].get(), *sh = kv_[6].get(), *t = kv_[7].get(), *tmp = kv_[8].get();
+  double *r = w.kv[0], *rh = w.kv[1], *p = w.kv[2], *v = w.kv[3], *ph = w.kv[4], *s = w.kv[5], *sh = w.kv[6],
+         *t = w.kv[7], *tmp = w.kv[8];
+  k::BicgScalars* sc = w.sc;
+  double* part = w.part;
   k::BicgScalars host{};
   host.tol = rtol;
   host.abstol = abstol;
   host.maxit = maxit;
-  DCG_HIP(hipMemcpyAsync(sc_.get(), &host, sizeof host, hipMemcpyHostToDevice, stream_));
+  DCG_HIP(hipMemcpyAsync(sc, &host, sizeof host, hipMemcpyHostToDevice, stream_));
   const int nred = std::max(1, std::min(1024, static_cast<int>((n + 255) / 256)));
   const int nblk = (ns + 255) / 256;
-  k::bicg_init<double>(n, b, x, r, rh, p, v, part_.get(), nred, sc_.get(), stream_);
-  const double* dinv = jacobi_kind() ? dinv_.get() : nullptr;
+  k::bicg_init<double>(n, b, x, r, rh, p, v, part, nred, sc, stream_);
+  const double* dinv = jacobi_kind() ? w.dinv : nullptr;
   const bool ilu = cfg_.precond == Preconditioner::ILU0;
   auto apply_ilu = [&](const double* in, double* out) {
     for (size_t l = 0; l + 1 < lo_ptr_.size(); ++l)
       k::ilu_lower<double>(B, lo_ptr_[l + 1] - lo_ptr_[l], lo_rows_.get() + lo_ptr_[l], row_ptr_.get(), col_.get(),
-                           diag_pos_.get(), lu_.get(), in, tmp, sc_.get(), stream_);
+                           diag_pos_.get(), w.lu, in, tmp, sc, stream_);
     for (size_t l = 0; l + 1 < up_ptr_.size(); ++l)
       k::ilu_upper<double>(B, up_ptr_[l + 1] - up_ptr_[l], up_rows_.get() + up_ptr_[l], row_ptr_.get(), col_.get(),
-                           diag_pos_.get(), lu_.get(), dinv_.get(), tmp, out, sc_.get(), stream_);
+                           diag_pos_.get(), w.lu, w.dinv, tmp, out, sc, stream_);
   };
   // out = J in with the partial dots (out.w1, out.w2) finalised into `stage`
   const int* done_flag =
-      reinterpret_cast<const int*>(reinterpret_cast<const char*>(sc_.get()) + offsetof(k::BicgScalars, done));
+      reinterpret_cast<const int*>(reinterpret_cast<const char*>(sc) + offsetof(k::BicgScalars, done));
   auto matvec = [&](const double* in, double* out, const double* w1, const double* w2, int stage) {
     if (cfg_.matrix_free) {
       prof_begin(kJacobianApply);
       launch_apply(lin_x_.get(), lin_t_, lin_cm_, lin_ca_, in, out, done_flag);
       prof_end(kJacobianApply, apply_bytes());
-      k::dot_parts(n, out, w1, w2, part_.get(), nred, sc_.get(), stream_);
-      k::bicg_fin(nred, part_.get(), 2, stage, sc_.get(), stream_);
+      k::dot_parts(n, out, w1, w2, part, nred, sc, stream_);
+      k::bicg_fin(nred, part, 2, stage, sc, stream_);
       return;
     }
     if (use_points_) {
       prof_begin(kPointApply64);
-      const int np = launch_points_apply(in, out, done_flag, false, w1, w2, part_.get());
+      const int np = launch_points_apply(in, out, done_flag, false, w1, w2, part);
       prof_end(kPointApply64, point_apply_bytes(8));
-      if (!np) k::dot_parts(n, out, w1, w2, part_.get(), nred, sc_.get(), stream_);
-      k::bicg_fin(np ? np : nred, part_.get(), 2, stage, sc_.get(), stream_);
+      if (!np) k::dot_parts(n, out, w1, w2, part, nred, sc, stream_);
+      k::bicg_fin(np ? np : nred, part, 2, stage, sc, stream_);
       return;
     }
     prof_begin(kSpmv64);
-    k::spmv(B, plan_, val_.get(), in, out, w1, w2, part_.get(), sc_.get(), stage, stream_);
-    k::bicg_fin(plan_.parts(), part_.get(), 2, stage, sc_.get(), stream_);
+    k::spmv(B, plan_, w.val, in, out, w1, w2, part, sc, stage, stream_);
+    k::bicg_fin(plan_.parts(), part, 2, stage, sc, stream_);
     prof_end(kSpmv64, spmv_bytes(8, 8, 1));
   };
   const int akind = cfg_.matrix_free ? kJacobianApply : use_points_ ? kPointApply64 : kSpmv64;
   auto iteration = [&]() {
-    k::bicg_p<double>(B, ns, r, p, v, ph, dinv, sc_.get(), stream_);
+    k::bicg_p<double>(B, ns, r, p, v, ph, dinv, sc, stream_);
     if (ilu) apply_ilu(p, ph);
     matvec(ph, v, rh, nullptr, 1);
-    k::bicg_s<double>(B, ns, r, v, s, sh, dinv, sc_.get(), part_.get(), stream_);
+    k::bicg_s<double>(B, ns, r, v, s, sh, dinv, sc, part, stream_);
     if (ilu) apply_ilu(s, sh);
-    k::bicg_fin(nblk, part_.get(), 1, 2, sc_.get(), stream_);
+    k::bicg_fin(nblk, part, 1, 2, sc, stream_);
     matvec(sh, t, s, t, 3);
-    k::bicg_xr<double>(B, ns, x, ph, sh, s, t, r, rh, sc_.get(), part_.get(), stream_);
-    k::bicg_fin(nblk, part_.get(), 2, 4, sc_.get(), stream_);
+    k::bicg_xr<double>(B, ns, x, ph, sh, s, t, r, rh, sc, part, stream_);
+    k::bicg_fin(nblk, part, 2, 4, sc, stream_);
   };
   // iterations run in batches between read-backs of the scalars: the first
   // as many as this solve took in the last time step (or the last solve)
@@ -959,7 +973,7 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
   for (;;) {
     for (int it = 0; it < batch; ++it) iteration();
     done_its += batch;
-    DCG_HIP(hipMemcpyAsync(&host, sc_.get(), sizeof host, hipMemcpyDeviceToHost, stream_));
+    DCG_HIP(hipMemcpyAsync(&host, sc, sizeof host, hipMemcpyDeviceToHost, stream_));
     synchronize();
     if (host.done) break;
     if (done_its > maxit + 4) {
@@ -1065,13 +1079,14 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
 // BiCGSTAB.  Mixed: FP32 BiCGSTAB on an FP32 copy of the matrix inside FP64
 // iterative refinement; the refinement residual b - A x is FP64.
 int Model::krylov(const double* b, double* x, double rtol, int maxit, int* its, int* refinements) {
+  const KrylovWork w = own_work();
   prof_begin(kPrecond);
-  precond_setup();
+  precond_setup(w);
   prof_end(kPrecond, 0.0);
   *its = 0;
   *refinements = 0;
   const long n = L_.nslot * L_.B;
-  if (cfg_.precision == Precision::FP64) return bicgstab64(b, x, rtol, 0.0, maxit, its);
+  if (cfg_.precision == Precision::FP64) return bicgstab64(b, x, rtol, 0.0, maxit, its, w);
   if (use_points_) {
     for (size_t c = 0; c < pc_.size(); ++c)
       if (pc_[c].size()) k::to_float(static_cast<long>(pc_[c].size()), pc_[c].get(), pc32_[c].get(), stream_);
@@ -1505,6 +1520,71 @@ int Model::linear_solve(const double* jac, const double* b_packed, double* x_pac
   int rc = krylov(r_.get(), z_.get(), rtol, maxit, its, &refs);
   k::pack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), z_.get(), packed_.get(), stream_);
   DCG_HIP(hipMemcpyAsync(x_packed, packed_.get(), L_.ndof * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  return rc;
+}
+
+// dcg_linear_solve without device allocations: J, b and x are the caller's
+// device buffers (BCSR values, packed state layout), every other buffer the
+// FP64 solve needs (Krylov vectors, the unpacked right-hand side and
+// solution, preconditioner, partial sums, solver scalars) is carved out of
+// the library's fixed work space on this device.  Calls that overlap on one
+// device share that space and race.
+int Model::linear_solve_noalloc(const double* jac, const double* b_packed, double* x_packed, double rtol, int maxit,
+                                int* its) {
+  require_device();
+  if (cfg_.matrix_free) throw Error("a matrix-free model holds no assembled Jacobian");
+  if (cfg_.precision != Precision::FP64)
+    throw NotImplemented("the allocation-free linear solve runs FP64 models only (linear_solver.precision = fp64)");
+  if (!jac || !b_packed || !x_packed) throw Error("null device buffer");
+  const size_t nv = static_cast<size_t>(L_.nslot) * L_.B;
+  const size_t align = 32;  // doubles: every region starts on 256 bytes
+  auto round = [&](size_t w) { return (w + align - 1) / align * align; };
+  const bool ilu = cfg_.precond == Preconditioner::ILU0;
+  const size_t sc_words = round((sizeof(k::BicgScalars) + sizeof(double) - 1) / sizeof(double));
+  const size_t words = 11 * round(nv) + round(static_cast<size_t>(L_.nslot) * L_.B * L_.B) +
+                       (ilu ? round(static_cast<size_t>(matrix_values())) : 0) + round(part_.size()) + sc_words +
+                       round(2);
+  const size_t limit = k::kNoallocScratchBytes / sizeof(double);
+  if (words > limit)
+    throw Error("the allocation-free linear solve needs " + std::to_string(words * sizeof(double)) +
+                " bytes of work space for this model; the fixed work space holds " +
+                std::to_string(k::kNoallocScratchBytes) + " bytes");
+  double* base = k::noalloc_scratch(device_);
+  size_t off = 0;
+  auto take = [&](size_t w) {
+    double* q = base + off;
+    off += round(w);
+    return q;
+  };
+  KrylovWork w;
+  for (auto& v : w.kv) v = take(nv);
+  double* rhs = take(nv);
+  double* sol = take(nv);
+  w.dinv = take(static_cast<size_t>(L_.nslot) * L_.B * L_.B);
+  w.lu = ilu ? take(static_cast<size_t>(matrix_values())) : nullptr;
+  w.part = take(part_.size());
+  w.sc = reinterpret_cast<k::BicgScalars*>(base + off);
+  off += sc_words;
+  w.status = reinterpret_cast<int*>(take(2));
+  w.val = jac;
+  // the given matrix is the operator; the solve keeps no per-step hint
+  struct Restore {
+    Model* m;
+    bool points;
+    int index;
+    ~Restore() {
+      m->use_points_ = points;
+      m->solve_index_ = index;
+    }
+  } restore{this, use_points_, solve_index_};
+  use_points_ = false;
+  solve_index_ = kMaxSolveHints;
+  k::fill(w.status, 0, 4, stream_);
+  k::unpack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), b_packed, rhs, stream_);
+  precond_setup(w);
+  int rc = bicgstab64(rhs, sol, rtol, 0.0, maxit, its, w);
+  k::pack_state(L_.nslot, L_.B, slot_b_.get(), slot_dof_.get(), sol, x_packed, stream_);
   synchronize();
   return rc;
 }

@@ -141,6 +141,10 @@ class Model {
   void assemble(const double* x_packed, double t, double cm, double ca, double* r_packed, double* jac);
   void spmv(const double* jac, const double* x_packed, double* y_packed);
   int linear_solve(const double* jac, const double* b_packed, double* x_packed, double rtol, int maxit, int* its);
+  // the same FP64 solve on the caller's device buffers with no device
+  // allocation: work space from the library's fixed per-device array
+  int linear_solve_noalloc(const double* jac_dev, const double* b_dev, double* x_dev, double rtol, int maxit,
+                           int* its);
   // y = J(x) v without assembling J (the matrix-free Jacobian apply), for
   // models whose compartments all use the tiled kernel (facet terms included)
   void jacobian_apply(const double* x_packed, double t, double cm, double ca, const double* v_packed,
@@ -210,9 +214,21 @@ class Model {
   double residual_norm(const double* r);
   // linear solves on the current val_
   int krylov(const double* b, double* x, double rtol, int maxit, int* its, int* refinements);
-  int bicgstab64(const double* b, double* x, double rtol, double abstol, int maxit, int* its);
+  // device buffers one FP64 Krylov solve works in: the model's own, or a
+  // carved-out part of the library's fixed work space (linear_solve_noalloc)
+  struct KrylovWork {
+    double* kv[9] = {};  // r rh p v ph s sh t tmp
+    double* part = nullptr;
+    k::BicgScalars* sc = nullptr;
+    const double* val = nullptr;  // the assembled operator
+    double* dinv = nullptr;
+    double* lu = nullptr;
+    int* status = nullptr;
+  };
+  KrylovWork own_work();
+  int bicgstab64(const double* b, double* x, double rtol, double abstol, int maxit, int* its, const KrylovWork& w);
   int bicgstab32(const double* b, float* x, double rtol, int maxit, int* its);
-  void precond_setup();
+  void precond_setup(const KrylovWork& w);
   bool jacobi_kind() const {
     return cfg_.precond == Preconditioner::Jacobi || cfg_.precond == Preconditioner::VertexBlockJacobi;
   }

@@ -187,6 +187,12 @@ template <typename V>
 void bicg_xr(int B, int nslot, V* x, const V* ph, const V* sh, const V* sv, const V* t, V* r, const V* rh,
              const BicgScalars* sc, double* part, hipStream_t s);
 
+// The fixed device work space of the allocation-free entry points: its size,
+// and its device address on `device` (looked up once per device, with that
+// device current; every call on a device shares the one array).
+constexpr size_t kNoallocScratchBytes = size_t(64) << 20;
+double* noalloc_scratch(int device);
+
 void ilu_factor(int B, int nrows, const int* rows, const int* rp, const int* col, const int* diag_pos, double* lu,
                 double* dinv, int* status, hipStream_t s);
 template <typename V>

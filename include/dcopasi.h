@@ -126,6 +126,21 @@ int dcg_spmv(dcg_model_t model, const double* jacobian, const double* x, double*
 int dcg_linear_solve(dcg_model_t model, const double* jacobian, const double* b, double* x, double rtol, int maxit,
                      int* iterations);
 
+/* dcg_linear_solve with no device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  jacobian (BCSR values on the
+ * model's pattern), b and x (packed state layout) are DEVICE pointers the
+ * caller owns, ready before the call (synchronise the stream that wrote them);
+ * the call returns with x written.  Every other buffer of the FP64 BiCGSTAB
+ * solve with the model's preconditioner lives in one fixed device array of
+ * DCG_NOALLOC_WORKSPACE_BYTES per GPU, held until the process ends; a model
+ * whose work space exceeds it, or that solves in mixed precision, is
+ * rejected with an error.  Calls that overlap on one GPU share that array
+ * and race: serialise them. */
+#define DCG_NOALLOC_WORKSPACE_BYTES (64u << 20)
+int dcg_linear_solve_noalloc(dcg_model_t model, const double* jacobian, const double* b, double* x, double rtol,
+                             int maxit, int* iterations);
+
 /* ---- measurement ----
  * HIP-event timing on the model's own stream, the device counterpart of the
  * reference's Perfetto counters (dune/copasi/common/stepper.hh:100-102,
