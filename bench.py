@@ -113,6 +113,7 @@ def cpu_baseline(wl_obj, steps: int, small=None):
     L.orc_set_krylov_threads(0)
     o = oracle_lib.OracleModel(wl_obj.ini)
     o.set_state(wl_obj.initial_state(o), 0.0)
+    _oracle_steps(o, wl_obj, 1)  # the first step (the stimulus transient) is a warm-up
     dof_steps, el = _oracle_steps(o, wl_obj, steps)
     L.orc_set_krylov_threads(1)
     t_dof, t_el = _oracle_steps(o, wl_obj, 1)
@@ -126,8 +127,9 @@ def cpu_baseline(wl_obj, steps: int, small=None):
     o1.close()
     L.orc_set_threads(cores)
     return {"value": dof_steps / el, "unit": "Newton DOF-steps/s", "cores": cores, "kind": "port",
-            "sample": f"{steps} time step(s) of the same workload from the same initial state "
-                      f"({dof_steps} Newton DOF-steps, {el:.1f} s, OpenMP-coloured assembly, sequential BiCGSTAB)",
+            "sample": f"time step(s) 2..{steps + 1} of the same workload from the same initial state, step 1 "
+                      f"untimed ({dof_steps} Newton DOF-steps, {el:.1f} s, OpenMP-coloured assembly, "
+                      f"sequential BiCGSTAB)",
             "threaded_krylov": {"value": t_dof / t_el, "cores": cores,
                                 "sample": f"the next time step with SpMV, dots and updates on all threads "
                                           f"({t_dof} Newton DOF-steps, {t_el:.1f} s)"},
@@ -163,19 +165,25 @@ def cpu_baseline_batch(ws, dof_steps_budget: float):
                       f"OpenMP-coloured assembly, sequential BiCGSTAB)"}
 
 
-def plain_fp64(dca, items, steps, precond="Jacobi"):
+def plain_fp64(dca, items, steps, precond="Jacobi", stride=1):
     """The plain FP64 path on the same models, one model at a time: each model
     built alone (FP64 BiCGSTAB, same preconditioner), one untimed step, then `steps`
     timed steps.  items: (ini, dt, stages, total steps the workload runs,
-    initial state of a model handle -> packed state).
+    initial state of a model handle -> packed state, size key).  stride > 1
+    times every stride-th model in order of size (a stratified sample: each
+    timed model stands for the `stride` models around it in size).
     Returns Newton DOF-steps/s of the whole workload at the measured per-step
     times and the sampled DOF-steps and seconds."""
     import torch
 
     ov = ("model.time_step_operator.linear_solver.precision = fp64\n"
           f"model.time_step_operator.linear_solver.preconditioner.type = {precond}\n")
+    order = sorted(range(len(items)), key=lambda j: items[j][5])
+    strata = [order[i:i + stride] for i in range(0, len(order), stride)]
     work, secs, sampled_work, sampled_secs = 0.0, 0.0, 0.0, 0.0
-    for ini, dt, stages, total, init in items:
+    for stratum in strata:
+        ini, dt, stages, total, init, _ = items[stratum[len(stratum) // 2]]
+        weight = len(stratum)
         m = dca.Model(ini, ov, 0, runtime_parameters=True)
         m.set_state(init(m), 0.0)
         t = 0.0
@@ -190,20 +198,21 @@ def plain_fp64(dca, items, steps, precond="Jacobi"):
             t += dt
         torch.cuda.synchronize()
         el = (time.perf_counter() - t0) / steps
-        work += m.n_dof * stages * total
-        secs += el * total
+        work += weight * m.n_dof * stages * total
+        secs += weight * el * total
         sampled_work += m.n_dof * stages * steps
         sampled_secs += el * steps
         m.close()
-    return {"value": work / secs, "unit": "Newton DOF-steps/s", "models": len(items), "timed_steps_per_model": steps,
-            "sampled_dof_steps": sampled_work, "sampled_seconds": sampled_secs}
+    return {"value": work / secs, "unit": "Newton DOF-steps/s", "models": len(items), "timed_models": len(strata),
+            "timed_steps_per_model": steps, "sampled_dof_steps": sampled_work, "sampled_seconds": sampled_secs}
 
 
-def run_batch(args, dca, wl):
-    """Config: 256 independent Gray-Scott instances advanced by one Batch."""
+def batch_measure(args, dca, wl, precision, warmup, steps, plain_stride, cpu=True):
+    """Config: 256 independent Gray-Scott instances advanced by one Batch.
+    Returns the measurement as a dict (the workload's own JSON line, or an
+    object of the default line)."""
     import torch
 
-    precision = args.precision
     ws = wl.gray_scott_batch(count=args.members, seed=0, precision=precision, nl_rtol=args.batch_nl_rtol,
                              nl_atol=0.0, lin_rtol=1e-10)
     t_setup = time.perf_counter()
@@ -232,13 +241,13 @@ def run_batch(args, dca, wl):
             raise SystemExit(f"batch step {k}: {len(bad)} members failed (status {bad[:5]})")
         return st
 
-    for k in range(args.warmup):
+    for k in range(warmup):
         one_step(k)
-    print(f"# warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+    print(f"# batch warmup done ({warmup} steps)", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     lat, newton, lin, refs = [], 0, 0, 0
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         s0 = time.perf_counter()
         st = one_step(k)
         lat.append(time.perf_counter() - s0)
@@ -247,7 +256,7 @@ def run_batch(args, dca, wl):
         refs += max(x.refinement_steps for x in st)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    print(f"# timed {args.steps} steps in {elapsed:.3f}s", file=sys.stderr, flush=True)
+    print(f"# batch: timed {steps} steps in {elapsed:.3f}s", file=sys.stderr, flush=True)
     # kernel roofline: profiled steps after the timed region (plain launches,
     # HIP events around every batched SpMV on the batch stream)
     b.profile_enable(True)
@@ -262,7 +271,7 @@ def run_batch(args, dca, wl):
         roofline = {"bound": "hbm", "kernel": f"batched {kind} (bspmv_k)", "achieved": gbs, "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None, "avg_launch_us": 1e3 * ms / n,
                     "algorithmic_bytes_per_launch": by / n, "launches": n}
-    total_steps = args.warmup + args.steps + args.profile_steps
+    total_steps = warmup + steps + args.profile_steps
     # error of sampled members against single FP64 models converged to round-off
     sample = sorted(range(len(ws)), key=lambda j: ws[j].cells[0])
     sample = sorted(set([sample[0], sample[len(sample) // 2], sample[-1]]))[: args.error_members]
@@ -292,22 +301,22 @@ def run_batch(args, dca, wl):
                 bb = r[c["dof_offset"] + sp: c["dof_offset"] + c["n_species"] * c["n_nodes"]: c["n_species"]]
                 e.append(float(np.linalg.norm(a - bb) / max(np.linalg.norm(bb), 1e-300)))
         errs[w.name] = max(e)
-    cpu = None if args.no_cpu_baseline else cpu_baseline_batch(ws, args.cpu_dof_steps)
-    value = args.steps * n_dof / elapsed
+    cpu_b = cpu_baseline_batch(ws, args.cpu_dof_steps) if cpu and not args.no_cpu_baseline else None
+    value = steps * n_dof / elapsed
     plain = None
     if args.plain_steps > 0:
         print("# plain FP64 path on the same models ...", file=sys.stderr, flush=True)
-        plain = plain_fp64(dca, [(w.ini, w.dt, 1, args.steps, w.initial_state) for w in ws], args.plain_steps,
-                           args.preconditioner)
+        plain = plain_fp64(dca, [(w.ini, w.dt, 1, steps, w.initial_state, w.cells[0]) for w in ws],
+                           args.plain_steps, args.preconditioner, plain_stride)
         plain["speedup_of_this_line"] = value / plain["value"]
     out = {
         "metric": "Newton DOF-steps/s (plain FP64 vs mixed/batched), p50/p99 latency, HBM GB/s",
         "value": value,
         "unit": "Newton DOF-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": 1e3 * elapsed / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -318,32 +327,36 @@ def run_batch(args, dca, wl):
                    "graph_iterations": args.graph_iterations,
                    "newton_relative_tolerance_two_norm2": args.batch_nl_rtol},
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
-        "newton_iterations_per_step_max_member": newton / args.steps,
-        "krylov_iterations_per_step_max_member": lin / args.steps,
-        "refinements_per_step_max_member": refs / args.steps,
+        "newton_iterations_per_step_max_member": newton / steps,
+        "krylov_iterations_per_step_max_member": lin / steps,
+        "refinements_per_step_max_member": refs / steps,
         "error_vs_fp64": {"max_rel_l2_per_species": max(errs.values()) if errs else None, "members": errs,
                           "steps": total_steps},
         "setup_s": setup_s,
         "device_bytes": b.device_bytes(),
         "plain_fp64_same_models": plain,
         "roofline": roofline,
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu_b,
     }
     b.close()
-    print(json.dumps(out))
+    return out
 
 
-def run_serve(args, dca, wl):
-    """Config: a Poisson stream of seeded random models served by one Server."""
+def run_batch(args, dca, wl):
+    print(json.dumps(batch_measure(args, dca, wl, args.precision, args.warmup, args.steps, args.plain_stride)))
+
+
+def serve_measure(args, dca, wl, precision, R, qps, min_steps, max_steps):
+    """Config: a Poisson stream of seeded random models served by one Server.
+    Returns the measurement as a dict."""
     import torch
 
-    R = args.requests
-    reqs = [wl.random_request(args.serve_seed * 100000 + i, precision=args.precision,
-                              steps_range=(args.min_steps, args.max_steps)) for i in range(R)]
+    reqs = [wl.random_request(args.serve_seed * 100000 + i, precision=precision,
+                              steps_range=(min_steps, max_steps)) for i in range(R)]
     rng = np.random.Generator(np.random.PCG64(args.serve_seed))
-    arrivals = np.cumsum(rng.exponential(1.0 / args.qps, size=R))
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, size=R))
     arrivals -= arrivals[0]
-    srv = dca.Server(f"precision = {args.precision}\nmax_members = {args.max_members}\n"
+    srv = dca.Server(f"precision = {precision}\nmax_members = {args.max_members}\n"
                      f"workers = {args.workers}\ngraph_iterations = {args.graph_iterations}\n"
                      f"preconditioner = {args.preconditioner}\n", 0)
     ids = [None] * R
@@ -403,8 +416,8 @@ def run_serve(args, dca, wl):
     plain = None
     if args.plain_steps > 0:
         print("# plain FP64 path on the same requests ...", file=sys.stderr, flush=True)
-        plain = plain_fp64(dca, [(r.ini, r.dt, r.stages, r.steps, lambda m: m.initial_state()) for r in reqs],
-                           args.plain_steps, args.preconditioner)
+        plain = plain_fp64(dca, [(r.ini, r.dt, r.stages, r.steps, lambda m: m.initial_state(), k)
+                                 for k, r in enumerate(reqs)], args.plain_steps, args.preconditioner)
         plain["speedup_of_this_line"] = (work / makespan) / plain["value"]
     out = {
         "metric": "Newton DOF-steps/s (plain FP64 vs mixed/batched), p50/p99 latency, HBM GB/s",
@@ -417,11 +430,11 @@ def run_serve(args, dca, wl):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32-krylov/f64-refinement",
+        "dtype": "f64" if precision == "fp64" else "f32-krylov/f64-refinement",
         "data": "synthetic (seeded random models and Poisson arrivals)",
         "config": {"workload": "serving_poisson_random_mass_action_2d_64-512_3d_32-96",
-                   "requests": R, "qps": args.qps, "steps_range": [args.min_steps, args.max_steps],
-                   "precision": args.precision, "max_members": args.max_members, "workers": args.workers},
+                   "requests": R, "qps": qps, "steps_range": [min_steps, max_steps],
+                   "precision": precision, "max_members": args.max_members, "workers": args.workers},
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                       "max": float(lat.max())},
         "queue_s": {"p50": float(np.percentile(queue, 50)), "p99": float(np.percentile(queue, 99))},
@@ -432,7 +445,12 @@ def run_serve(args, dca, wl):
         "roofline": None,
         "cpu_baseline": None,
     }
-    print(json.dumps(out))
+    return out
+
+
+def run_serve(args, dca, wl):
+    print(json.dumps(serve_measure(args, dca, wl, args.precision, args.requests, args.qps, args.min_steps,
+                                   args.max_steps)))
 
 
 def main():
@@ -468,6 +486,18 @@ def main():
     ap.add_argument("--cpu-dof-steps", type=float, default=4e6, help="batch workload: CPU baseline sample size")
     ap.add_argument("--plain-steps", type=int, default=1,
                     help="gs_batch / serve: timed steps per model of the plain FP64 path on the same models (0: skip)")
+    ap.add_argument("--plain-stride", type=int, default=1,
+                    help="gs_batch: time the plain FP64 path on every n-th model by size (stratified sample)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="default line: skip the batched-models object (config 3 sample)")
+    ap.add_argument("--no-serve", action="store_true", help="default line: skip the serving object (config 4 sample)")
+    ap.add_argument("--line-batch-steps", type=int, default=3, help="default line: timed batch steps")
+    ap.add_argument("--line-plain-stride", type=int, default=4,
+                    help="default line: stratified sample of the plain FP64 path beside the batch")
+    ap.add_argument("--line-requests", type=int, default=12, help="default line: served requests")
+    ap.add_argument("--line-qps", type=float, default=0.5, help="default line: serving arrival rate")
+    ap.add_argument("--line-min-steps", type=int, default=20, help="default line: fewest steps of a served request")
+    ap.add_argument("--line-max-steps", type=int, default=60, help="default line: most steps of a served request")
     ap.add_argument("--requests", type=int, default=16, help="serve workload: number of requests")
     ap.add_argument("--qps", type=float, default=1.0, help="serve workload: Poisson arrival rate")
     ap.add_argument("--serve-seed", type=int, default=1, help="serve workload: seed of models and arrivals")
@@ -612,6 +642,23 @@ def main():
         # one thread on the 48^3 mesh of the same model (a 96^3 step takes minutes on one core)
         small = make_workload(wl, "ms48", "fp64", args.preconditioner)[0] if args.workload == "ms96" else None
         cpu = cpu_baseline(w, args.cpu_steps, small)
+    n_dof = m.n_dof
+    m.close()
+    batch_obj = serve_obj = None
+    if args.workload == "ms96" and not args.no_batch:
+        # the north-star forms on their own workloads: the 256-member batch
+        # (mixed precision, beside the plain FP64 path on the same models)
+        print("# batch of 256 Gray-Scott models ...", file=sys.stderr, flush=True)
+        batch_obj = batch_measure(args, dca, wl, "mixed", 1, args.line_batch_steps, args.line_plain_stride, cpu=False)
+        batch_obj = {k: v for k, v in batch_obj.items() if k not in ("metric", "n_gpus", "higher_is_better", "scaling",
+                                                                    "vs_baseline", "data")}
+    if args.workload == "ms96" and not args.no_serve:
+        print("# serving sample ...", file=sys.stderr, flush=True)
+        serve_obj = serve_measure(args, dca, wl, "mixed", args.line_requests, args.line_qps, args.line_min_steps,
+                                  args.line_max_steps)
+        serve_obj = {k: v for k, v in serve_obj.items() if k not in ("metric", "n_gpus", "higher_is_better", "scaling",
+                                                                    "vs_baseline", "data", "roofline",
+                                                                    "cpu_baseline", "warmup")}
     out = {
         "metric": "Newton DOF-steps/s (plain FP64 vs mixed/batched), p50/p99 latency, HBM GB/s",
         "value": value,
@@ -625,7 +672,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32-krylov/f64-refinement",
         "data": "synthetic (seeded initial state)",
-        "config": {"workload": wname, "n_dof": m.n_dof, "stages_per_step": w.stages,
+        "config": {"workload": wname, "n_dof": n_dof, "stages_per_step": w.stages,
                    "precision": args.precision, "preconditioner": args.preconditioner,
                    "jacobian_storage": args.storage, "readback_iterations": args.readback_iterations},
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
@@ -638,6 +685,8 @@ def main():
         "mixed_precision_same_workload": mixed,
         "copy_probe": probe,
         "cpu_baseline": cpu,
+        "batch_same_models": batch_obj,
+        "serving": serve_obj,
     }
     print(json.dumps(out))
 

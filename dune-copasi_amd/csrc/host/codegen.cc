@@ -1193,7 +1193,7 @@ __device__ __forceinline__ void block_rows(int dx, int dy, int dz, int li, int l
 }
 
 // The rows of one tile from point coefficients cellp(box cell)[((p*NQ + q)*NMC +
-// n)*WS] (the owner tile's block in HBM, or the set-up's staged LDS array) and the
+// n)*WS] (the grid-wide array in HBM, or the set-up's staged LDS array) and the
 // direction staged in vs over the vertex box (slots sl, class masks cms): per
 // cell, the direction at its corners interpolated to each point, multiplied
 // by the point's block and tested with the basis functions, plus the uniform
@@ -1202,8 +1202,8 @@ __device__ __forceinline__ void block_rows(int dx, int dy, int dz, int li, int l
 // are the identity, from v); with part, the workgroup's partial dots (y.w1,
 // y.w2) go to part[0] and part[1].  LDS scratch: accv (zeroed by the
 // caller) when SCATTER, else out.
-template <typename T, typename C, int WS, typename CP>
-__device__ __forceinline__ void cells_apply(CP cellp, const T* vs, const int* sl,
+template <typename T, typename C, typename CP>
+__device__ __forceinline__ void cells_apply(CP cellp, size_t WS, const T* vs, const int* sl,
                                             const unsigned char* cms, T* out, T* accv, const T* __restrict__ v,
                                             T* __restrict__ y, double ca, const double (&dv)[NDV > 0 ? NDV : 1],
                                             const T* __restrict__ w1, const T* __restrict__ w2,
@@ -1476,16 +1476,21 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
     }
     __syncthreads();
     if constexpr (MODE == 8) {
-      // the coefficients of the tile's owned cells into its block of val
-      // (block = position in the launch's tile list), word-major:
-      // val[block][(p*NQ + q)*NMC + n][owned cell]; owned cells outside the
-      // grid or the compartment carry values that are never read
-      if constexpr (NMC > 0)
+      // the coefficients of the tile's owned cells (lowest corner among its
+      // vertices) into the grid-wide array, word-major:
+      // val[((p*NQ + q)*NMC + n) * cells of the grid + cell]; cells outside
+      // the compartment carry values that are never read
+      if constexpr (NMC > 0) {
+        const size_t ncg = (size_t)n0 * (size_t)n1 * (size_t)n2;
         for (int k = threadIdx.x; k < OWN_WORDS; k += blockDim.x) {
           const int w = k / NOWN, o = k % NOWN;
           const int oi = o % TX, oj = DIM > 1 ? (o / TX) % TY : 0, ok = DIM > 2 ? o / (TX * TY) : 0;
-          val[(size_t)blockIdx.x * OWN_WORDS + k] = work[(size_t)w * NCB + (oi + 1) + CXB * ((oj + OY) + CYB * (ok + OZ))];
+          const int gi = vi0 + 1 + oi, gj = vj0 + OY + oj, gk = vk0 + OZ + ok;
+          if (gi >= n0 || (DIM > 1 && gj >= n1) || (DIM > 2 && gk >= n2)) continue;
+          val[(size_t)w * ncg + (size_t)gi + (size_t)n0 * ((size_t)gj + (size_t)n1 * gk)] =
+              work[(size_t)w * NCB + (oi + 1) + CXB * ((oj + OY) + CYB * (ok + OZ))];
         }
+      }
       // diagonal block of every row (the Jacobi preconditioner's input):
       // r[slot * B * B + i * B + j], the identity on padding species
       constexpr int AI = BC > 4 ? 1 : BC, NR = BC / AI;
@@ -1517,8 +1522,8 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
       }
       __syncthreads();
       if constexpr (CELL_APPLY) {
-        cells_apply<double, double, NCB>([&](int cl) { return (const double*)work + cl; }, xs, sl, cms, nullptr, acc4,
-                                         base, r, ca, dv, nullptr, nullptr, nullptr);
+        cells_apply<double, double>([&](int cl) { return (const double*)work + cl; }, (size_t)NCB, xs, sl, cms,
+                                    nullptr, acc4, base, r, ca, dv, nullptr, nullptr, nullptr);
         return;
       }
     }
@@ -1860,24 +1865,21 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
 }
 
 // y = J v from stored point coefficients (mode 8 of tile_body: each cell's
-// coefficients once, in the block of the tile that owns it; a box cell of
-// another tile is read from that tile's block, found through tpos, the
-// position of every tile of the grid in the launch's tile list):
-// per cell of the tile's box, v at its corners is interpolated to each point, multiplied
-// by the point's coefficient block and tested against the basis functions,
-// the uniform diffusion adds the element stiffness times v; the cells' corner
-// sums are folded onto the tile's rows as in the residual.  T is the value
-// type of pc, v and y (double, or float inside the refinement loop).  tca:
-// the time and stiffness weight of the linearisation, written to device
-// memory by the set-up, so an apply's arguments are the same for every
-// linearisation.  g2l null: the
-// compartment's slots are the grid's vertices in order.  With part,
-// the workgroup also writes its rows' partial dots (y.w1, y.w2) to
-// part[0] and part[1] (w2 may be null), as the BiCGSTAB finaliser reads.
+// coefficients once, in the grid-wide word-major array pc[word * cells +
+// cell]): per cell of the tile's box, v at its corners is interpolated to
+// each point, multiplied by the point's coefficient block and tested against
+// the basis functions, the uniform diffusion adds the element stiffness
+// times v; the cells' corner sums are folded onto the tile's rows as in the
+// residual.  T is the value type of pc, v and y (double, or float inside the
+// refinement loop).  tca: the time and stiffness weight of the
+// linearisation, written to device memory by the set-up, so an apply's
+// arguments are the same for every linearisation.  g2l null: the
+// compartment's slots are the grid's vertices in order.  With part, the
+// workgroup also writes its rows' partial dots (y.w1, y.w2) to part[0] and
+// part[1] (w2 may be null), as the BiCGSTAB finaliser reads.
 template <typename T>
 __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,
-                                         const int* __restrict__ tpos, const T* __restrict__ pc,
-                                         const T* __restrict__ v, T* __restrict__ y,
+                                         const T* __restrict__ pc, const T* __restrict__ v, T* __restrict__ y,
                                          const double* __restrict__ tca, int n0, int n1, int n2,
                                          const double* __restrict__ prm, const T* __restrict__ w1,
                                          const T* __restrict__ w2, double* __restrict__ part) {
@@ -1916,16 +1918,8 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
     unsigned char m = 0;
     size_t off = 0;
     if (gi >= 0 && gi < n0 && gj >= 0 && gj < (DIM > 1 ? n1 : 1) && gk >= 0 && gk < (DIM > 2 ? n2 : 1)) {
-      m = cmask[gi + n0 * (gj + n1 * gk)];
-      if (m) {
-        // the cell's coefficients: its owner tile's block (tpos null: the
-        // launch's tile list is every tile in order), word-minor by owned cell
-        const int ot = gi / TX + ntx * ((DIM > 1 ? gj / TY : 0) + nty * (DIM > 2 ? gk / TZ : 0));
-        const int pos = tpos ? tpos[ot] : ot;
-        const int loc = gi % TX + TX * ((DIM > 1 ? gj % TY : 0) + (DIM > 1 ? TY : 1) * (DIM > 2 ? gk % TZ : 0));
-        if (pos >= 0) off = (size_t)pos * OWN_WORDS + loc;
-        else m = 0;  // (the owner holds the cell's lowest vertex, so it is listed)
-      }
+      off = (size_t)gi + (size_t)n0 * ((size_t)gj + (size_t)n1 * gk);
+      m = cmask[off];
     }
     cms[c] = m;
     coff[c] = off;
@@ -1933,9 +1927,10 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
   __syncthreads();
   double dv[NDV > 0 ? NDV : 1];
   uniform_diffusion(t, prm, dv);
-  cells_apply<T, T, NOWN>([&](int cl) { return pc + coff[cl]; }, vs, sl, cms, out, accv, v, y, ca, dv, w1, w2,
-                         part);
+  cells_apply<T, T>([&](int cl) { return pc + coff[cl]; }, (size_t)n0 * (size_t)n1 * (size_t)n2, vs, sl, cms, out,
+                    accv, v, y, ca, dv, w1, w2, part);
 }
+
 }  // namespace %C%
 
 // one entry point per kind of work, same arguments: _r residual (r = base +
@@ -1979,14 +1974,14 @@ extern "C" __global__ void __launch_bounds__(256)
 #define DCG_POINT_ENTRY(NAME, T)                                                                                  \
   extern "C" __global__ void __launch_bounds__(256)                                                               \
   NAME(const int* __restrict__ tiles, int ntile, const int* __restrict__ g2l,                                     \
-       const unsigned char* __restrict__ cmask, const int* __restrict__ tpos, const T* __restrict__ pc,            \
-       const T* __restrict__ v, T* __restrict__ y, const double* __restrict__ tca, int n0, int n1, int n2,         \
-       const double* __restrict__ prm, const int* __restrict__ stop, const T* __restrict__ w1,                     \
-       const T* __restrict__ w2, double* __restrict__ part) {                                                     \
+       const unsigned char* __restrict__ cmask, const T* __restrict__ pc, const T* __restrict__ v,                 \
+       T* __restrict__ y, const double* __restrict__ tca, int n0, int n1, int n2, const double* __restrict__ prm,  \
+       const int* __restrict__ stop, const T* __restrict__ w1, const T* __restrict__ w2,                          \
+       double* __restrict__ part) {                                                                               \
     if (stop && *stop) return;                                                                                    \
     const int c = (int)(blockIdx.x % 8u) * (int)(gridDim.x / 8u) + (int)(blockIdx.x / 8u);                       \
     if (c >= ntile) return;                                                                                       \
-    %C%::pc_apply<T>(tiles[c], g2l, cmask, tpos, pc, v, y, tca, n0, n1, n2, prm, w1, w2,                          \
+    %C%::pc_apply<T>(tiles[c], g2l, cmask, pc, v, y, tca, n0, n1, n2, prm, w1, w2,                                \
                      part ? part + 2 * (size_t)c : nullptr);                                                      \
   }
 DCG_POINT_ENTRY(%KERNEL%_p, double)

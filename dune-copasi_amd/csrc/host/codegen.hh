@@ -32,9 +32,9 @@ struct GeneratedSource {
   // batched tiled kernel per compartment (runtime geometry only, else "")
   std::vector<std::string> tile_batch_kernel;
   std::vector<std::array<int, 3>> tile_dims;
-  // doubles per cell of the point-coefficient Jacobian (classes x points x
-  // coefficients; 0 when the compartment has no point kernels); each tile
-  // keeps the cells of its box: (tile_dims + 1 per axis) cells per tile
+  // words per cell of the point-coefficient Jacobian (classes x points x
+  // coefficients; 0 when the compartment has no point kernels), stored for
+  // every cell of the grid, word-major: pc[word * cells + cell]
   std::vector<int> point_words;
   std::vector<FacetKernel> facet_kernels;
 };

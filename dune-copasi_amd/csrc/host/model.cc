@@ -462,11 +462,10 @@ void Model::setup_device() {
     pc_.resize(L_.comps.size());
     if (cfg_.precision == Precision::Mixed) pc32_.resize(L_.comps.size());
     for (size_t c = 0; c < L_.comps.size(); ++c) {
-      // one block per tile of the cells it owns (lowest corner among its
-      // vertices): every cell once
-      const auto td = gen_.tile_dims[c];
-      const size_t own = static_cast<size_t>(td[0]) * (L_.grid.dim > 1 ? td[1] : 1) * (L_.grid.dim > 2 ? td[2] : 1);
-      const size_t words = static_cast<size_t>(gen_.point_words[c]) * own * std::max(0, comp_dev_[c].ntiles);
+      // every cell of the grid, word-major (cells outside the compartment
+      // are never read)
+      if (!gen_.point_words[c] || comp_dev_[c].ntiles <= 0) continue;
+      const size_t words = static_cast<size_t>(gen_.point_words[c]) * static_cast<size_t>(L_.grid.ncell);
       pc_[c].resize(words);
       if (cfg_.precision == Precision::Mixed) pc32_[c].resize(words);
     }
@@ -526,6 +525,14 @@ void Model::setup_device() {
     up_ptr_ = cu;
     for (long i = 0; i < ns; ++i) rows_u[cu[lev[i]]++] = static_cast<int>(i);
     up_rows_.upload(rows_u, stream_);
+    // narrow levels (structured grids: a few hundred rows per anti-diagonal)
+    // sweep every level in one workgroup instead of one launch per level
+    int widest = 0;
+    for (size_t l = 0; l + 1 < lo_ptr_.size(); ++l) widest = std::max(widest, lo_ptr_[l + 1] - lo_ptr_[l]);
+    for (size_t l = 0; l + 1 < up_ptr_.size(); ++l) widest = std::max(widest, up_ptr_[l + 1] - up_ptr_[l]);
+    ilu_sweep_ = widest <= kIluSweepWidth;
+    lo_lev_.upload(lo_ptr_, stream_);
+    up_lev_.upload(up_ptr_, stream_);
     lu_.resize(static_cast<size_t>(matrix_values()));
   }
   {
@@ -819,14 +826,13 @@ int Model::launch_points_apply(const void* v, void* y, const int* stop, bool fp3
   int nparts = 0;
   for (size_t c = 0; c < L_.comps.size(); ++c) {
     if (!pc_app_fn_[c] || comp_dev_[c].ntiles <= 0) continue;
-    const int* tiles = comp_dev_[c].tiles.get();
     const int* g2l = comp_dev_[c].g2l_identity ? nullptr : comp_dev_[c].g2l.get();
     const unsigned char* cmask = comp_dev_[c].cell_mask_grid.get();
-    const int* tpos = comp_dev_[c].tpos.size() ? comp_dev_[c].tpos.get() : nullptr;
     const void* pc = fp32 ? static_cast<const void*>(pc32_[c].get()) : static_cast<const void*>(pc_[c].get());
     double* pp = part && facet_fn_.empty() ? part + 2 * static_cast<size_t>(nparts) : nullptr;
+    const int* tiles = comp_dev_[c].tiles.get();
     int ntile = comp_dev_[c].ntiles;
-    void* args[] = {&tiles, &ntile, &g2l, &cmask, &tpos, &pc, &v, &y, &tca, &n0, &n1, &n2, &prm, &stop, &w1, &w2, &pp};
+    void* args[] = {&tiles, &ntile, &g2l, &cmask, &pc, &v, &y, &tca, &n0, &n1, &n2, &prm, &stop, &w1, &w2, &pp};
     const unsigned grid = static_cast<unsigned>((ntile + 7) / 8 * 8);  // the kernel maps workgroups to XCD runs
     DCG_HIP(hipModuleLaunchKernel(fp32 ? pc_appf_fn_[c] : pc_app_fn_[c], grid, 1, 1, 256, 1, 1, 0, stream_, args,
                                   nullptr));
@@ -893,6 +899,11 @@ void Model::precond_setup(const KrylovWork& w) {
     k::block_inverse(B, ns, w.val, diag_pos_.get(), w.dinv, w.status, cfg_.precond == Preconditioner::Jacobi, stream_);
   } else if (cfg_.precond == Preconditioner::ILU0) {
     DCG_HIP(hipMemcpyAsync(w.lu, w.val, matrix_values() * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    if (ilu_sweep_) {
+      k::ilu_factor_sweep(B, static_cast<int>(lo_ptr_.size()) - 1, lo_lev_.get(), lo_rows_.get(), row_ptr_.get(),
+                          col_.get(), diag_pos_.get(), w.lu, w.dinv, w.status, stream_);
+      return;
+    }
     for (size_t l = 0; l + 1 < lo_ptr_.size(); ++l)
       k::ilu_factor(B, lo_ptr_[l + 1] - lo_ptr_[l], lo_rows_.get() + lo_ptr_[l], row_ptr_.get(), col_.get(),
                     diag_pos_.get(), w.lu, w.dinv, w.status, stream_);
@@ -919,6 +930,12 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
   const double* dinv = jacobi_kind() ? w.dinv : nullptr;
   const bool ilu = cfg_.precond == Preconditioner::ILU0;
   auto apply_ilu = [&](const double* in, double* out) {
+    if (ilu_sweep_) {
+      k::ilu_apply_sweep<double>(B, static_cast<int>(lo_ptr_.size()) - 1, lo_lev_.get(), lo_rows_.get(),
+                                 static_cast<int>(up_ptr_.size()) - 1, up_lev_.get(), up_rows_.get(), row_ptr_.get(),
+                                 col_.get(), diag_pos_.get(), w.lu, w.dinv, in, tmp, out, sc, stream_);
+      return;
+    }
     for (size_t l = 0; l + 1 < lo_ptr_.size(); ++l)
       k::ilu_lower<double>(B, lo_ptr_[l + 1] - lo_ptr_[l], lo_rows_.get() + lo_ptr_[l], row_ptr_.get(), col_.get(),
                            diag_pos_.get(), w.lu, in, tmp, sc, stream_);
@@ -1010,6 +1027,12 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
   const double* dinv = jacobi_kind() ? dinv_.get() : nullptr;
   const bool ilu = cfg_.precond == Preconditioner::ILU0;
   auto apply_ilu = [&](const float* in, float* out) {
+    if (ilu_sweep_) {
+      k::ilu_apply_sweep<float>(B, static_cast<int>(lo_ptr_.size()) - 1, lo_lev_.get(), lo_rows_.get(),
+                                static_cast<int>(up_ptr_.size()) - 1, up_lev_.get(), up_rows_.get(), row_ptr_.get(),
+                                col_.get(), diag_pos_.get(), lu_.get(), dinv_.get(), in, tmp, out, sc_.get(), stream_);
+      return;
+    }
     for (size_t l = 0; l + 1 < lo_ptr_.size(); ++l)
       k::ilu_lower<float>(B, lo_ptr_[l + 1] - lo_ptr_[l], lo_rows_.get() + lo_ptr_[l], row_ptr_.get(), col_.get(),
                           diag_pos_.get(), lu_.get(), in, tmp, sc_.get(), stream_);

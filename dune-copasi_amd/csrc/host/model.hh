@@ -342,7 +342,11 @@ class Model {
   k::SpmvPlan plan_, plan32_;
   // ILU level schedules (host copies of the level pointers, device row lists)
   std::vector<int> lo_ptr_, up_ptr_;
-  DeviceArray<int> lo_rows_, up_rows_;
+  DeviceArray<int> lo_rows_, up_rows_, lo_lev_, up_lev_;
+  // every level fits one workgroup in a few passes: one-workgroup sweeps
+  // over all levels instead of one launch per level
+  bool ilu_sweep_ = false;
+  static constexpr int kIluSweepWidth = 4096;
   bool ilu_ready_ = false;
   std::vector<std::pair<int, double>> tuning_;
   // profiling

@@ -195,6 +195,14 @@ double* noalloc_scratch(int device);
 
 void ilu_factor(int B, int nrows, const int* rows, const int* rp, const int* col, const int* diag_pos, double* lu,
                 double* dinv, int* status, hipStream_t s);
+// every level of the factorisation / of both triangular solves in one
+// workgroup (lev: level boundaries into rows, as the per-level launches use)
+void ilu_factor_sweep(int B, int nlev, const int* lev, const int* rows, const int* rp, const int* col,
+                      const int* diag_pos, double* lu, double* dinv, int* status, hipStream_t s);
+template <typename V>
+void ilu_apply_sweep(int B, int nlo, const int* lo_lev, const int* lo_rows, int nup, const int* up_lev,
+                     const int* up_rows, const int* rp, const int* col, const int* diag_pos, const double* lu,
+                     const double* dinv, const V* in, V* tmp, V* out, const BicgScalars* sc, hipStream_t s);
 template <typename V>
 void ilu_lower(int B, int nrows, const int* rows, const int* rp, const int* col, const int* diag_pos,
                const double* lu, const V* b, V* y, const BicgScalars* sc, hipStream_t s);

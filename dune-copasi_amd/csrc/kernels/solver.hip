@@ -397,12 +397,9 @@ __device__ __forceinline__ void mat_mul_sub(double* __restrict__ c, const double
 // Factor the rows of one level: L_ik <- A_ik U_kk^-1, A_ij -= L_ik U_kj,
 // U_ii^-1 stored in dinv.  lu holds a copy of the matrix values.
 template <int B>
-__global__ void ilu_factor_k(int nrows, const int* __restrict__ rows, const int* __restrict__ rp,
-                             const int* __restrict__ col, const int* __restrict__ diag_pos, double* __restrict__ lu,
-                             double* __restrict__ dinv, int* __restrict__ status) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nrows) return;
-  const int i = rows[t];
+__device__ __forceinline__ void ilu_factor_row(int i, const int* __restrict__ rp, const int* __restrict__ col,
+                                               const int* __restrict__ diag_pos, double* __restrict__ lu,
+                                               double* __restrict__ dinv, int* __restrict__ status) {
   for (int p = rp[i]; p < diag_pos[i]; ++p) {
     const int k = col[p];
     double* lik = lu + (size_t)p * B * B;
@@ -487,16 +484,38 @@ __global__ void ilu_factor_k(int nrows, const int* __restrict__ rows, const int*
     for (int c = 0; c < B; ++c) o[r * B + c] = inv[r][c];
 }
 
-// forward solve (unit lower): y_i = b_i - sum_{k<i} L_ik y_k
-template <int B, typename V>
-__global__ void ilu_lower_k(int nrows, const int* __restrict__ rows, const int* __restrict__ rp,
-                            const int* __restrict__ col, const int* __restrict__ diag_pos,
-                            const double* __restrict__ lu, const V* __restrict__ b, V* __restrict__ y,
-                            const BicgScalars* __restrict__ sc) {
-  if (sc && finished(sc)) return;
+template <int B>
+__global__ void ilu_factor_k(int nrows, const int* __restrict__ rows, const int* __restrict__ rp,
+                             const int* __restrict__ col, const int* __restrict__ diag_pos, double* __restrict__ lu,
+                             double* __restrict__ dinv, int* __restrict__ status) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nrows) return;
-  const int i = rows[t];
+  ilu_factor_row<B>(rows[t], rp, col, diag_pos, lu, dinv, status);
+}
+
+// All levels in one workgroup: the rows of a level (rows[lev[l] .. lev[l+1]))
+// in parallel, a barrier between levels.  A row reads only rows of earlier
+// levels, which the barrier makes visible; for level widths a workgroup
+// covers in a few passes this replaces one launch per level.
+template <int B>
+__global__ void __launch_bounds__(256) ilu_factor_sweep_k(int nlev, const int* __restrict__ lev,
+                                                          const int* __restrict__ rows, const int* __restrict__ rp,
+                                                          const int* __restrict__ col,
+                                                          const int* __restrict__ diag_pos, double* __restrict__ lu,
+                                                          double* __restrict__ dinv, int* __restrict__ status) {
+  for (int l = 0; l < nlev; ++l) {
+    const int e = lev[l + 1];
+    for (int t = lev[l] + (int)threadIdx.x; t < e; t += blockDim.x)
+      ilu_factor_row<B>(rows[t], rp, col, diag_pos, lu, dinv, status);
+    __syncthreads();
+  }
+}
+
+// forward solve (unit lower): y_i = b_i - sum_{k<i} L_ik y_k
+template <int B, typename V>
+__device__ __forceinline__ void ilu_lower_row(int i, const int* __restrict__ rp, const int* __restrict__ col,
+                                              const int* __restrict__ diag_pos, const double* __restrict__ lu,
+                                              const V* __restrict__ b, V* __restrict__ y) {
   double s[B];
 #pragma unroll
   for (int a = 0; a < B; ++a) s[a] = (double)b[(size_t)i * B + a];
@@ -514,14 +533,10 @@ __global__ void ilu_lower_k(int nrows, const int* __restrict__ rows, const int* 
 
 // backward solve: x_i = U_ii^-1 (y_i - sum_{k>i} U_ik x_k)
 template <int B, typename V>
-__global__ void ilu_upper_k(int nrows, const int* __restrict__ rows, const int* __restrict__ rp,
-                            const int* __restrict__ col, const int* __restrict__ diag_pos,
-                            const double* __restrict__ lu, const double* __restrict__ dinv, const V* __restrict__ y,
-                            V* __restrict__ x, const BicgScalars* __restrict__ sc) {
-  if (sc && finished(sc)) return;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nrows) return;
-  const int i = rows[t];
+__device__ __forceinline__ void ilu_upper_row(int i, const int* __restrict__ rp, const int* __restrict__ col,
+                                              const int* __restrict__ diag_pos, const double* __restrict__ lu,
+                                              const double* __restrict__ dinv, const V* __restrict__ y,
+                                              V* __restrict__ x) {
   double s[B];
 #pragma unroll
   for (int a = 0; a < B; ++a) s[a] = (double)y[(size_t)i * B + a];
@@ -540,6 +555,56 @@ __global__ void ilu_upper_k(int nrows, const int* __restrict__ rows, const int* 
 #pragma unroll
     for (int c = 0; c < B; ++c) v += d[a * B + c] * s[c];
     x[(size_t)i * B + a] = (V)v;
+  }
+}
+
+template <int B, typename V>
+__global__ void ilu_lower_k(int nrows, const int* __restrict__ rows, const int* __restrict__ rp,
+                            const int* __restrict__ col, const int* __restrict__ diag_pos,
+                            const double* __restrict__ lu, const V* __restrict__ b, V* __restrict__ y,
+                            const BicgScalars* __restrict__ sc) {
+  if (sc && finished(sc)) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows) return;
+  ilu_lower_row<B, V>(rows[t], rp, col, diag_pos, lu, b, y);
+}
+
+template <int B, typename V>
+__global__ void ilu_upper_k(int nrows, const int* __restrict__ rows, const int* __restrict__ rp,
+                            const int* __restrict__ col, const int* __restrict__ diag_pos,
+                            const double* __restrict__ lu, const double* __restrict__ dinv, const V* __restrict__ y,
+                            V* __restrict__ x, const BicgScalars* __restrict__ sc) {
+  if (sc && finished(sc)) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows) return;
+  ilu_upper_row<B, V>(rows[t], rp, col, diag_pos, lu, dinv, y, x);
+}
+
+// both triangular solves, every level, in one workgroup (barriers between
+// levels and between the two sweeps): out = U^-1 L^-1 in, tmp holds L^-1 in
+template <int B, typename V>
+__global__ void __launch_bounds__(256) ilu_apply_sweep_k(int nlo, const int* __restrict__ lo_lev,
+                                                         const int* __restrict__ lo_rows, int nup,
+                                                         const int* __restrict__ up_lev,
+                                                         const int* __restrict__ up_rows, const int* __restrict__ rp,
+                                                         const int* __restrict__ col,
+                                                         const int* __restrict__ diag_pos,
+                                                         const double* __restrict__ lu,
+                                                         const double* __restrict__ dinv, const V* __restrict__ in,
+                                                         V* __restrict__ tmp, V* __restrict__ out,
+                                                         const BicgScalars* __restrict__ sc) {
+  if (sc && finished(sc)) return;
+  for (int l = 0; l < nlo; ++l) {
+    const int e = lo_lev[l + 1];
+    for (int t = lo_lev[l] + (int)threadIdx.x; t < e; t += blockDim.x)
+      ilu_lower_row<B, V>(lo_rows[t], rp, col, diag_pos, lu, in, tmp);
+    __syncthreads();
+  }
+  for (int l = 0; l < nup; ++l) {
+    const int e = up_lev[l + 1];
+    for (int t = up_lev[l] + (int)threadIdx.x; t < e; t += blockDim.x)
+      ilu_upper_row<B, V>(up_rows[t], rp, col, diag_pos, lu, dinv, tmp, out);
+    __syncthreads();
   }
 }
 
@@ -764,6 +829,26 @@ void ilu_factor(int B, int nrows, const int* rows, const int* rp, const int* col
   if (nrows <= 0) return;
   DCG_DISPATCH_B(B, (ilu_factor_k<BB><<<grid_for(nrows, 64), 64, 0, s>>>(nrows, rows, rp, col, diag_pos, lu, dinv, status)));
 }
+
+void ilu_factor_sweep(int B, int nlev, const int* lev, const int* rows, const int* rp, const int* col,
+                      const int* diag_pos, double* lu, double* dinv, int* status, hipStream_t s) {
+  if (nlev <= 0) return;
+  DCG_DISPATCH_B(B, (ilu_factor_sweep_k<BB><<<1, 256, 0, s>>>(nlev, lev, rows, rp, col, diag_pos, lu, dinv, status)));
+}
+
+template <typename V>
+void ilu_apply_sweep(int B, int nlo, const int* lo_lev, const int* lo_rows, int nup, const int* up_lev,
+                     const int* up_rows, const int* rp, const int* col, const int* diag_pos, const double* lu,
+                     const double* dinv, const V* in, V* tmp, V* out, const BicgScalars* sc, hipStream_t s) {
+  DCG_DISPATCH_B(B, (ilu_apply_sweep_k<BB, V><<<1, 256, 0, s>>>(nlo, lo_lev, lo_rows, nup, up_lev, up_rows, rp, col,
+                                                                diag_pos, lu, dinv, in, tmp, out, sc)));
+}
+template void ilu_apply_sweep<double>(int, int, const int*, const int*, int, const int*, const int*, const int*,
+                                      const int*, const int*, const double*, const double*, const double*, double*,
+                                      double*, const BicgScalars*, hipStream_t);
+template void ilu_apply_sweep<float>(int, int, const int*, const int*, int, const int*, const int*, const int*,
+                                     const int*, const int*, const double*, const double*, const float*, float*,
+                                     float*, const BicgScalars*, hipStream_t);
 
 template <typename V>
 void ilu_lower(int B, int nrows, const int* rows, const int* rp, const int* col, const int* diag_pos,
