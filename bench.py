@@ -268,7 +268,9 @@ def batch_measure(args, dca, wl, precision, warmup, steps, plain_stride, cpu=Tru
     roofline = None
     if n:
         gbs = (by / n) / (ms / n * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "kernel": f"batched {kind} (bspmv_k)", "achieved": gbs, "peak": HBM_PEAK_GBS,
+        roofline = {"bound": "hbm", "kernel": f"batched {'FP32' if kind == 'spmv32' else 'FP64'} Krylov operator "
+                                              f"(point apply dcg_tile_0_pfbatch / BCSR bspmv_k)",
+                    "achieved": gbs, "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None, "avg_launch_us": 1e3 * ms / n,
                     "algorithmic_bytes_per_launch": by / n, "launches": n}
     total_steps = warmup + steps + args.profile_steps

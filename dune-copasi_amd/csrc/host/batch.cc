@@ -1,6 +1,7 @@
 #include "batch.hh"
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 
 namespace dcopasi {
@@ -163,16 +164,18 @@ void Batch::rebuild() {
   destroy_graphs();
   sgroups_.clear();
   agroups_.clear();
-  // solver groups by block size
-  std::map<int, int> by_b;
+  // solver groups by block size and Jacobian storage
+  std::map<std::pair<int, bool>, int> by_b;
   for (size_t i = 0; i < members_.size(); ++i) {
     Entry& e = members_[i];
     const int B = e.m->layout().B;
-    auto it = by_b.find(B);
+    const bool pts = e.m->point_storage();
+    auto it = by_b.find({B, pts});
     if (it == by_b.end()) {
-      it = by_b.emplace(B, static_cast<int>(sgroups_.size())).first;
+      it = by_b.emplace(std::make_pair(B, pts), static_cast<int>(sgroups_.size())).first;
       sgroups_.emplace_back();
       sgroups_.back().B = B;
+      sgroups_.back().points = pts;
     }
     e.sg = it->second;
     e.si = static_cast<int>(sgroups_[e.sg].mem.size());
@@ -202,6 +205,7 @@ void Batch::rebuild() {
     }
     g.h_act.assign(M, 0);
     g.h_rtol.assign(M, 0.0);
+    if (g.points) cwork.clear();  // no matrix chunks: the point apply is the operator
     g.mv.upload(mv, stream_);
     g.vwork.upload(vwork, stream_);
     g.vrange.upload(vrange, stream_);
@@ -220,6 +224,83 @@ void Batch::rebuild() {
     g.w.ncwork = static_cast<int>(cwork.size());
     g.w.crange = g.crange.get();
     g.w.members = M;
+    if (g.points) {
+      // per member: its compartment's tables, coefficients and vectors; per
+      // generated module: one launch over its members' (member, tile) items
+      std::vector<k::PcMember> pcm(M);
+      std::vector<int2> prange(M);
+      std::map<const RtcModule*, size_t> by_mod;
+      std::vector<std::vector<int2>> items;
+      std::vector<std::vector<int>> owners;
+      for (int j = 0; j < M; ++j) {
+        Model& m = *members_[g.mem[j]].m;
+        size_t c0 = 0;
+        while (c0 < m.pc_.size() && !m.pc_[c0].size()) ++c0;
+        if (c0 >= m.pc_.size()) throw Error("internal: point member without coefficients");
+        k::PcMember& pm = pcm[j];
+        pm.g2l = m.comp_dev_[c0].g2l_identity ? nullptr : m.comp_dev_[c0].g2l.get();
+        pm.cmask = m.comp_dev_[c0].cell_mask_grid.get();
+        pm.pc = m.pc_[c0].get();
+        pm.pc32 = c0 < m.pc32_.size() ? m.pc32_[c0].get() : nullptr;
+        pm.tca = m.pc_tca_.get();
+        pm.prm = m.prm_.get();
+        const Grid& gr = m.L_.grid;
+        pm.n[0] = static_cast<int>(gr.n[0]);
+        pm.n[1] = gr.dim > 1 ? static_cast<int>(gr.n[1]) : 1;
+        pm.n[2] = gr.dim > 2 ? static_cast<int>(gr.n[2]) : 1;
+        for (int v2 = 0; v2 < k::kBatchVec; ++v2) pm.v[v2] = mv[j].v[v2];
+        for (int v2 = 0; v2 < k::kBatchVecF; ++v2) pm.f[v2] = mv[j].f[v2];
+        pm.done = reinterpret_cast<const int*>(reinterpret_cast<const char*>(g.sc.get() + j) +
+                                               offsetof(k::BicgScalars, done));
+        pm.irdone = reinterpret_cast<const int*>(reinterpret_cast<const char*>(g.ir.get() + j) +
+                                                 offsetof(k::IrState, done));
+        const RtcModule* key = m.module_.get();
+        auto it = by_mod.find(key);
+        if (it == by_mod.end()) {
+          it = by_mod.emplace(key, g.plaunch.size()).first;
+          g.plaunch.emplace_back();
+          auto& pl = g.plaunch.back();
+          const std::string name = m.gen_.tile_kernel[c0];
+          pl.f64 = m.module_->function(name + "_pbatch");
+          if (opt_.precision == Precision::Mixed) pl.f32 = m.module_->function(name + "_pfbatch");
+          items.emplace_back();
+          owners.emplace_back();
+        }
+        std::vector<int> tiles(m.comp_dev_[c0].ntiles);
+        if (!tiles.empty())
+          DCG_HIP(hipMemcpy(tiles.data(), m.comp_dev_[c0].tiles.get(), tiles.size() * sizeof(int),
+                            hipMemcpyDeviceToHost));
+        prange[j] = make_int2(static_cast<int>(items[it->second].size()), 0);  // launch-local for now
+        for (int t : tiles) items[it->second].push_back(make_int2(j, t));
+        prange[j].y = static_cast<int>(items[it->second].size());
+        owners[it->second].push_back(j);
+      }
+      int off = 0;
+      for (size_t l = 0; l < g.plaunch.size(); ++l) {
+        auto& pl = g.plaunch[l];
+        pl.work.upload(items[l], stream_);
+        pl.nwork = static_cast<int>(items[l].size());
+        pl.off = off;
+        for (int j : owners[l]) {
+          prange[j].x += off;
+          prange[j].y += off;
+        }
+        off += pl.nwork;
+      }
+      g.nitems = off;
+      g.pcm.upload(pcm, stream_);
+      g.prange.upload(prange, stream_);
+      g.part.resize(2 * std::max<size_t>(std::max(vwork.size(), static_cast<size_t>(g.nitems)), 1) + 64);
+      const int gi = static_cast<int>(&g - sgroups_.data());
+      g.op64.range = g.prange.get();
+      g.op64.apply = [this, gi](int x, int y, int w1, int w2, double* part, int skip) {
+        point_apply(gi, false, x, y, w1, w2, part, skip);
+      };
+      g.op32.range = g.prange.get();
+      g.op32.apply = [this, gi](int x, int y, int w1, int w2, double* part, int skip) {
+        point_apply(gi, true, x, y, w1, w2, part, skip);
+      };
+    }
     tune(g);
   }
   // assembly groups by shared module
@@ -236,8 +317,13 @@ void Batch::rebuild() {
       a.module = e.m->module_;
       const auto& names = e.m->gen_.tile_batch_kernel;
       a.fn.assign(names.size(), nullptr);
+      a.qfn.assign(names.size(), nullptr);
+      a.points = e.m->point_storage();
       for (size_t c = 0; c < names.size(); ++c)
-        if (!names[c].empty()) a.fn[c] = a.module->function(names[c]);
+        if (!names[c].empty()) {
+          a.fn[c] = a.module->function(names[c]);
+          if (a.points) a.qfn[c] = a.module->function(e.m->gen_.tile_kernel[c] + "_qbatch");
+        }
       a.work.resize(names.size());
       a.nwork.assign(names.size(), 0);
     }
@@ -264,6 +350,12 @@ void Batch::rebuild() {
       for (int k2 = 0; k2 < 3; ++k2) {
         am[j].origin[k2] = gr.origin[k2];
         am[j].h[k2] = k2 < gr.dim ? gr.h[k2] : 0.0;
+      }
+      if (a.points) {
+        for (size_t c = 0; c < m.pc_.size() && c < static_cast<size_t>(k::kAsmComps); ++c)
+          am[j].pc[c] = m.pc_[c].size() ? m.pc_[c].get() : nullptr;
+        am[j].pdiag = m.pdiag_.get();
+        am[j].tca = m.pc_tca_.get();
       }
       for (size_t c = 0; c < a.fn.size(); ++c) {
         if (!a.fn[c] || c >= static_cast<size_t>(k::kAsmComps)) continue;
@@ -294,6 +386,7 @@ void Batch::rebuild() {
 // Times the LDS-staged SpMV variants (entry and block form) on the group's
 // batched work list and keeps the faster one per precision.
 void Batch::tune(SolverGroup& g) {
+  if (g.points) return;  // the point apply has no SpMV variants
   const k::Skip none{};
   // zeroed operands: x = ph, y = v of the Krylov set
   k::b_copy(g.B, g.w, g.mv.get(), none, k::kVKry + 4, -1, stream_);
@@ -329,10 +422,10 @@ void Batch::capture(SolverGroup& g, bool fp32) {
   for (int it = 0; it < opt_.graph_iterations; ++it) {
     if (fp32)
       k::b_bicg_iteration<float>(g.B, g.variant32, g.w, g.mv.get(), k::kVKry + 9, opt_.precond != Preconditioner::None, g.part.get(),
-                                 g.sc.get(), stream_);
+                                 g.sc.get(), stream_, nullptr, g.points ? &g.op32 : nullptr);
     else
       k::b_bicg_iteration<double>(g.B, g.variant64, g.w, g.mv.get(), k::kVZ, opt_.precond != Preconditioner::None, g.part.get(), g.sc.get(),
-                                  stream_);
+                                  stream_, nullptr, g.points ? &g.op64 : nullptr);
   }
   DCG_HIP(hipStreamEndCapture(stream_, &graph));
   hipGraphExec_t exec = nullptr;
@@ -367,6 +460,28 @@ std::vector<std::array<int, 3>> Batch::spmv_choice() {
   std::vector<std::array<int, 3>> v;
   for (const auto& g : sgroups_) v.push_back({g.B, g.variant64, g.variant32});
   return v;
+}
+
+// y = A x over the point members of group gi: one launch per generated module
+// (grid a multiple of 8; the kernel deals items to the XCDs in contiguous
+// runs).  FP32 solves index the members' FP32 vectors (selector - kVKry).
+void Batch::point_apply(int gi, bool fp32, int x, int y, int w1, int w2, double* part, int skip) {
+  SolverGroup& g = sgroups_[gi];
+  if (fp32) {
+    x -= k::kVKry;
+    y -= k::kVKry;
+    if (w1 >= 0) w1 -= k::kVKry;
+    if (w2 >= 0) w2 -= k::kVKry;
+  }
+  const k::PcMember* mem = g.pcm.get();
+  for (auto& pl : g.plaunch) {
+    if (pl.nwork <= 0) continue;
+    const int2* work = pl.work.get();
+    int nwork = pl.nwork, off = pl.off, xs = x, ys = y, a1 = w1, a2 = w2, sk = skip;
+    void* args[] = {&work, &nwork, &off, &mem, &xs, &ys, &a1, &a2, &sk, &part};
+    const unsigned grid = static_cast<unsigned>((pl.nwork + 7) / 8 * 8);
+    DCG_HIP(hipModuleLaunchKernel(fp32 ? pl.f32 : pl.f64, grid, 1, 1, 256, 1, 1, 0, stream_, args, nullptr));
+  }
 }
 
 // ------------------------------------------------------------------ batched operations
@@ -406,6 +521,13 @@ void Batch::assemble(int call, const std::vector<char>& on, int xsel, int bsel, 
       const int2* work = a.work[c].get();
       const k::AsmMember* mem = a.am.get();
       int xs = xsel, bs = bsel, rs = rsel, md = mode;
+      if (a.points && mode == 2) {
+        // the point coefficients and diagonal blocks replace the matrix
+        void* qargs[] = {&work, &mem, &coef, &act, &xs};
+        DCG_HIP(hipModuleLaunchKernel(a.qfn[c], static_cast<unsigned>(a.nwork[c]), 1, 1, 256, 1, 1, 0, stream_,
+                                      qargs, nullptr));
+        continue;
+      }
       void* args[] = {&work, &mem, &coef, &act, &xs, &bs, &rs, &md};
       DCG_HIP(hipModuleLaunchKernel(a.fn[c], static_cast<unsigned>(a.nwork[c]), 1, 1, 256, 1, 1, 0, stream_, args,
                                     nullptr));
@@ -487,15 +609,16 @@ void Batch::iterate(SolverGroup& g, bool fp32) {
       for (size_t j = 0; j < M; ++j) {
         if (sc[j].done) continue;
         all = false;
-        bytes += members_[g.mem[j]].m->spmv_bytes(fp32 ? 4 : 8, fp32 ? 4 : 8, 1);
+        const Model& mj = *members_[g.mem[j]].m;
+        bytes += mj.point_storage() ? mj.point_apply_bytes(fp32 ? 4 : 8) : mj.spmv_bytes(fp32 ? 4 : 8, fp32 ? 4 : 8, 1);
       }
       if (all || done_its > opt_.lin_maxit + 4) break;
       if (fp32)
         k::b_bicg_iteration<float>(g.B, g.variant32, g.w, g.mv.get(), k::kVKry + 9, opt_.precond != Preconditioner::None, g.part.get(),
-                                   g.sc.get(), stream_, ev_);
+                                   g.sc.get(), stream_, ev_, g.points ? &g.op32 : nullptr);
       else
         k::b_bicg_iteration<double>(g.B, g.variant64, g.w, g.mv.get(), k::kVZ, opt_.precond != Preconditioner::None, g.part.get(),
-                                    g.sc.get(), stream_, ev_);
+                                    g.sc.get(), stream_, ev_, g.points ? &g.op64 : nullptr);
       DCG_HIP(hipEventSynchronize(ev_[3]));
       float a = 0.f, b = 0.f;
       DCG_HIP(hipEventElapsedTime(&a, ev_[0], ev_[1]));
@@ -517,10 +640,10 @@ void Batch::iterate(SolverGroup& g, bool fp32) {
         DCG_HIP(hipGraphLaunch(g.graph[fp32 ? 1 : 0], stream_));
       } else if (fp32) {
         k::b_bicg_iteration<float>(g.B, g.variant32, g.w, g.mv.get(), k::kVKry + 9, opt_.precond != Preconditioner::None, g.part.get(),
-                                   g.sc.get(), stream_);
+                                   g.sc.get(), stream_, nullptr, g.points ? &g.op32 : nullptr);
       } else {
         k::b_bicg_iteration<double>(g.B, g.variant64, g.w, g.mv.get(), k::kVZ, opt_.precond != Preconditioner::None, g.part.get(), g.sc.get(),
-                                    stream_);
+                                    stream_, nullptr, g.points ? &g.op64 : nullptr);
       }
     }
     done_its += blocks * K;
@@ -588,7 +711,8 @@ void Batch::solve(const std::vector<char>& on, const std::vector<double>& rtol, 
       k::b_bicg_init<float>(g.B, g.w, g.mv.get(), k::kVKry + 8, k::kVKry + 9, g.part.get(), g.sc.get(), stream_);
       iterate(g, true);
       k::b_ir_update(g.B, g.w, g.mv.get(), g.ir.get(), k::kVZ, stream_);
-      k::b_spmv64(g.B, g.variant64, g.w, g.mv.get(), ir_done, k::kVZ, k::kVKry + 7, stream_);
+      if (g.points) g.op64.apply(k::kVZ, k::kVKry + 7, -1, -1, nullptr, 1);
+      else k::b_spmv64(g.B, g.variant64, g.w, g.mv.get(), ir_done, k::kVZ, k::kVKry + 7, stream_);
       k::b_ir_residual(g.B, g.w, g.mv.get(), g.ir.get(), k::kVR, k::kVKry + 7, k::kVKry + 8, g.part.get(),
                        g.sums.get(), stream_);
       k::b_ir_check(g.w, g.sums.get(), g.sc.get(), g.ir.get(), stream_);

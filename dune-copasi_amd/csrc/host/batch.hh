@@ -96,6 +96,9 @@ class Batch {
   };
   struct SolverGroup {
     int B = 0;
+    // members hold point coefficients (the Krylov operator is the batched
+    // point apply) instead of assembled matrices
+    bool points = false;
     std::vector<int> mem;  // indices into members_
     DeviceArray<k::MemberView> mv;
     DeviceArray<int2> vwork, vrange, crange;
@@ -115,10 +118,27 @@ class Batch {
     // first block count of the n-th solve of this one), and the solve counter
     std::vector<int> hints;
     int call = 0;
+    // point storage: one launch per generated module over its members'
+    // (member, tile) items; item c of launch l writes partial pair off + c,
+    // prange[j] = member j's [first, last) pairs
+    struct PointLaunch {
+      hipFunction_t f64 = nullptr, f32 = nullptr;
+      DeviceArray<int2> work;
+      int nwork = 0, off = 0;
+    };
+    std::vector<PointLaunch> plaunch;
+    DeviceArray<k::PcMember> pcm;
+    DeviceArray<int2> prange;
+    int nitems = 0;
+    k::BatchOperator op64, op32;
   };
   struct AsmGroup {
     std::shared_ptr<RtcModule> module;
     std::vector<hipFunction_t> fn;  // batched tile kernel per compartment (or null)
+    // point storage: the batched point set-up per compartment replaces the
+    // Jacobian pass
+    std::vector<hipFunction_t> qfn;
+    bool points = false;
     std::vector<int> mem;
     DeviceArray<k::AsmMember> am;
     std::vector<DeviceArray<int2>> work;
@@ -145,6 +165,8 @@ class Batch {
   void solve(const std::vector<char>& on, const std::vector<double>& rtol, std::vector<int>& status,
              std::vector<int>& its, std::vector<int>& refs);
   void iterate(SolverGroup& g, bool fp32);
+  // the batched point apply of group gi (selectors as the SpMV takes them)
+  void point_apply(int gi, bool fp32, int x, int y, int w1, int w2, double* part, int skip);
 
   int device_;
   BatchOptions opt_;

@@ -1,5 +1,7 @@
 #include "codegen.hh"
 
+#include "../kernels/kernels.hh"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -236,7 +238,26 @@ struct dcg_asm_member {
   int n[4];
   double origin[3];
   double h[3];
+  double* pc[4];
+  double* pdiag;
+  double* tca;
 };
+// descriptor of one batch member for the batched point-coefficient apply
+struct dcg_pc_member {
+  const int* g2l;
+  const unsigned char* cmask;
+  const double* pc;
+  const float* pc32;
+  const double* tca;
+  const double* prm;
+  int n[4];
+  double* v[17];
+  float* f[10];
+  const int* done;
+  const int* irdone;
+};
+static_assert(sizeof(dcg_asm_member) == %ASM_BYTES%, "batch member descriptor layout");
+static_assert(sizeof(dcg_pc_member) == %PCM_BYTES%, "point member descriptor layout");
 __device__ __forceinline__ double dcg_sign(double v) { return (double)((v > 0.0) - (v < 0.0)); }
 __device__ __forceinline__ int dcg_pos(const int* own_start, const unsigned* own_mask, int s, int o) {
   return own_start[s] + __popc(own_mask[s] & ((1u << o) - 1u));
@@ -1989,6 +2010,49 @@ DCG_POINT_ENTRY(%KERNEL%_pf, float)
 #undef DCG_POINT_ENTRY
 )";
 
+// Batched point-coefficient Jacobian (members with point storage): _qbatch
+// stores every listed member's coefficients and diagonal blocks at the
+// member's x (as _q does) and records its (t, ca) for the applies; _pbatch /
+// _pfbatch apply them over (member, tile) work items with the member's
+// vectors by selector, writing item c's partial dots to part[2 (off + c)],
+// and skip members whose solve (skip 0) or refinement (skip 1) is done.
+const char* kTiledPointBatch = R"(
+extern "C" __global__ void __launch_bounds__(256)
+%KERNEL%_qbatch(const int2* __restrict__ work, const dcg_asm_member* __restrict__ mem,
+                const double* __restrict__ coef, const int* __restrict__ act, int xsel) {
+  const int2 w = work[blockIdx.x];
+  if (act && !act[w.x]) return;
+  const double* cf = coef + 4 * (size_t)w.x;
+  if (cf[3] == 0.0) return;
+  const dcg_asm_member& M = mem[w.x];
+  if (threadIdx.x == 0) {
+    // every workgroup of the member writes the same two values
+    M.tca[0] = cf[0];
+    M.tca[1] = cf[2];
+  }
+  %C%::tile_body<8>(w.y, M.g2l[%COMP%], M.cmask[%COMP%], M.vec[xsel], nullptr, cf[0], cf[1], cf[2], M.pdiag,
+                    M.pc[%COMP%], M.own_start, M.own_mask, M.n[0], M.n[1], M.n[2], M.origin[0], M.origin[1],
+                    M.origin[2], M.h[0], M.h[1], M.h[2], 8, M.prm);
+}
+#define DCG_PBATCH_ENTRY(NAME, T, VEC, PC)                                                                        \
+  extern "C" __global__ void __launch_bounds__(256)                                                               \
+  NAME(const int2* __restrict__ work, int nwork, int part_off, const dcg_pc_member* __restrict__ mem, int xsel,   \
+       int ysel, int w1sel, int w2sel, int skip, double* __restrict__ part) {                                      \
+    const int c = (int)(blockIdx.x % 8u) * (int)(gridDim.x / 8u) + (int)(blockIdx.x / 8u);                       \
+    if (c >= nwork) return;                                                                                       \
+    const int2 w = work[c];                                                                                       \
+    const dcg_pc_member& M = mem[w.x];                                                                            \
+    if (skip == 0 && *M.done) return;                                                                             \
+    if (skip == 1 && *M.irdone) return;                                                                           \
+    %C%::pc_apply<T>(w.y, M.g2l, M.cmask, M.PC, M.VEC[xsel], M.VEC[ysel], M.tca, M.n[0], M.n[1], M.n[2], M.prm,   \
+                     w1sel >= 0 ? M.VEC[w1sel] : nullptr, w2sel >= 0 ? M.VEC[w2sel] : nullptr,                    \
+                     part ? part + 2 * (size_t)(part_off + c) : nullptr);                                         \
+  }
+DCG_PBATCH_ENTRY(%KERNEL%_pbatch, double, v, pc)
+DCG_PBATCH_ENTRY(%KERNEL%_pfbatch, float, f, pc32)
+#undef DCG_PBATCH_ENTRY
+)";
+
 // Batched form: one launch over (member, tile) pairs of many models that share
 // this source; each member's tables, spacing, parameters and vectors come from
 // its descriptor, its time and mass / stiffness weights from coef[4 m .. 4 m + 3]
@@ -2163,6 +2227,7 @@ std::string gen_tiled(const ModelSpec& m, const Grid& g, int comp, const std::st
   std::string body = kTiled;
   if (points) body += kTiledPoints;
   if (geo && comp < 4) body += kTiledBatch;
+  if (geo && comp < 4 && points) body += kTiledPointBatch;
   replace_all(body, "%C%", ns);
   replace_all(body, "%KERNEL%", kname);
   replace_all(body, "%COMP%", std::to_string(comp));
@@ -2179,6 +2244,8 @@ GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L, bool r
   replace_all(pre, "%B%", std::to_string(L.B));
   replace_all(pre, "%GEO%", geo ? "1" : "0");
   replace_all(pre, "%UNS%", L.grid.unstructured ? "1" : "0");
+  replace_all(pre, "%ASM_BYTES%", std::to_string(sizeof(k::AsmMember)));
+  replace_all(pre, "%PCM_BYTES%", std::to_string(sizeof(k::PcMember)));
   std::ostringstream src;
   src << pre;
   const bool numerical = spec.jacobian == JacobianType::Numerical && !spec.is_linear;

@@ -195,8 +195,11 @@ void Model::build(const std::string& ini, const std::string& overrides) {
     if (st != "matrix" && st != "points" && st != "auto")
       throw IOError("linear_solver.storage must be 'matrix', 'points' or 'auto'");
     std::string why;
+    int with_species = 0;
+    for (const auto& C : L_.comps) with_species += C.b > 0 ? 1 : 0;
     if (cfg_.matrix_free) why = "the matrix-free Jacobian";
-    else if (opt_.batch_member) why = "batch members";
+    else if (opt_.batch_member && (with_species != 1 || !L_.facets.empty()))
+      why = "batch members with interface or outflow terms or several compartments";
     else if (cfg_.precond == Preconditioner::ILU0) why = "an ILU0 preconditioner";
     else if (!all_tile_eligible()) why = "compartments outside the tiled kernel";
     if (st == "points" && !why.empty()) throw NotImplemented("linear_solver.storage = points with " + why);
@@ -1674,6 +1677,19 @@ k::MemberView Model::view() const {
   v.val = val_.get();
   v.val32 = val32_.get();
   v.dinv = dinv_.get();
+  if (cfg_.points) {
+    // the block inverse reads the rows' diagonal blocks through an identity
+    // position map; the FP32 copy converts the point coefficients
+    v.val = pdiag_.get();
+    v.diag_pos = diag_id_.get();
+    v.nchunk = 0;
+    for (size_t c = 0; c < pc_.size(); ++c)
+      if (pc_[c].size()) {
+        v.pc = pc_[c].get();
+        v.pc32 = c < pc32_.size() ? pc32_[c].get() : nullptr;
+        v.npc = static_cast<long>(pc_[c].size());
+      }
+  }
   v.v[k::kVU] = u_.get();
   v.v[k::kVHist] = hist_.get();
   v.v[k::kVR] = r_.get();

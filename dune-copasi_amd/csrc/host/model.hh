@@ -253,8 +253,15 @@ class Model {
   double assembly_bytes(bool jacobian) const;
   std::vector<ReduceKey> reduce_keys(const std::string& reduce_ini) const;
   double apply_bytes() const;
+
+ public:
+  // algorithmic bytes of one point-coefficient apply / set-up
   double point_apply_bytes(int value_bytes) const;
   double point_setup_bytes() const;
+  // the Krylov operator is the point-coefficient one
+  bool point_storage() const { return cfg_.points; }
+
+ private:
 
   ParameterTree cfg_tree_;
   ModelSpec spec_;

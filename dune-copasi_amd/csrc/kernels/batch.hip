@@ -168,6 +168,13 @@ __global__ void __launch_bounds__(kThreads) btof_k(const int2* __restrict__ work
   const int2 w = work[blockIdx.x];
   if (skipped(sk, w.x)) return;
   const MemberView& m = mv[w.x];
+  if (m.npc) {
+    // point storage: the member's coefficients in equal shares per block
+    const long nblk = (m.nslot + kThreads - 1) / kThreads, share = (m.npc + nblk - 1) / nblk;
+    const long v0 = (long)w.y * share, v1 = min(m.npc, v0 + share);
+    for (long i = v0 + threadIdx.x; i < v1; i += kThreads) m.pc32[i] = (float)m.pc[i];
+    return;
+  }
   const int s0 = w.y * kThreads, s1 = min(m.nslot, s0 + kThreads);
   const long v0 = (long)m.rp[s0] * B * B, v1 = (long)m.rp[s1] * B * B;
   for (long i = v0 + threadIdx.x; i < v1; i += kThreads) m.val32[i] = (float)m.val[i];
@@ -526,7 +533,7 @@ template void b_bicg_init<float>(int, const BatchWork&, const MemberView*, int, 
 
 template <typename V>
 void b_bicg_iteration(int B, int variant, const BatchWork& w, const MemberView* mv, int x, bool jacobi,
-                      double* part, BicgScalars* sc, hipStream_t st, hipEvent_t* ev) {
+                      double* part, BicgScalars* sc, hipStream_t st, hipEvent_t* ev, const BatchOperator* op) {
   if (w.nvwork <= 0) return;
   using T = V;  // the matrix copy has the precision of the Krylov vectors
   const int jac = jacobi ? 1 : 0;
@@ -534,26 +541,35 @@ void b_bicg_iteration(int B, int variant, const BatchWork& w, const MemberView* 
   auto mark = [&](int i) {
     if (ev) (void)hipEventRecord(ev[i], st);
   };
+  // the operator: the members' BCSR matrices, or the given one (point storage)
+  const int2* orange = op ? op->range : w.crange;
+  auto matvec = [&](int xs, int ys, int w1, int w2) {
+    if (op) {
+      op->apply(xs, ys, w1, w2, part, 0);
+      return;
+    }
+    DCG_BATCH_B(B, (bspmv_launch<BB, T, V>(variant, w, mv, none, sc, xs, ys, w1, w2, part, st)));
+  };
   DCG_BATCH_B(B, {
     bp_k<BB, V><<<w.nvwork, kThreads, 0, st>>>(w.vwork, mv, jac, sc);
     mark(0);
-    bspmv_launch<BB, T, V>(variant, w, mv, none, sc, kVKry + 4, kVKry + 3, kVKry + 1, -1, part, st);
+    matvec(kVKry + 4, kVKry + 3, kVKry + 1, -1);
     mark(1);
-    bfin_k<<<w.members, kThreads, 0, st>>>(w.crange, part, 2, 1, sc);
+    bfin_k<<<w.members, kThreads, 0, st>>>(orange, part, 2, 1, sc);
     bs_k<BB, V><<<w.nvwork, kThreads, 0, st>>>(w.vwork, mv, jac, sc, part);
     bfin_k<<<w.members, kThreads, 0, st>>>(w.vrange, part, 1, 2, sc);
     mark(2);
-    bspmv_launch<BB, T, V>(variant, w, mv, none, sc, kVKry + 6, kVKry + 7, kVKry + 5, kVKry + 7, part, st);
+    matvec(kVKry + 6, kVKry + 7, kVKry + 5, kVKry + 7);
     mark(3);
-    bfin_k<<<w.members, kThreads, 0, st>>>(w.crange, part, 2, 3, sc);
+    bfin_k<<<w.members, kThreads, 0, st>>>(orange, part, 2, 3, sc);
     bxr_k<BB, V><<<w.nvwork, kThreads, 0, st>>>(w.vwork, mv, x, sc, part);
     bfin_k<<<w.members, kThreads, 0, st>>>(w.vrange, part, 2, 4, sc);
   });
 }
 template void b_bicg_iteration<double>(int, int, const BatchWork&, const MemberView*, int, bool, double*,
-                                       BicgScalars*, hipStream_t, hipEvent_t*);
+                                       BicgScalars*, hipStream_t, hipEvent_t*, const BatchOperator*);
 template void b_bicg_iteration<float>(int, int, const BatchWork&, const MemberView*, int, bool, double*,
-                                      BicgScalars*, hipStream_t, hipEvent_t*);
+                                      BicgScalars*, hipStream_t, hipEvent_t*, const BatchOperator*);
 
 void b_spmv64(int B, int variant, const BatchWork& w, const MemberView* mv, Skip sk, int x, int y, hipStream_t st) {
   DCG_BATCH_B(B, (bspmv_launch<BB, double, double>(variant, w, mv, sk, nullptr, x, y, -1, -1, nullptr, st)));

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
+
 namespace dcopasi::k {
 
 // LDS stage of the SpMV: block-row entries (blocks x B) per workgroup chunk.
@@ -55,12 +57,46 @@ struct MemberView {
   const int* col = nullptr;
   const int* diag_pos = nullptr;
   const int* chunk_row = nullptr;
-  double* val = nullptr;
+  double* val = nullptr;  // BCSR values, or the rows' diagonal blocks (point storage, diag_pos the identity)
   float* val32 = nullptr;
   double* dinv = nullptr;
   double* v[kBatchVec] = {};
   float* f[kBatchVecF] = {};
   BicgScalars* sc = nullptr;
+  // point storage: the coefficients, their FP32 copy and their count (0:
+  // the member holds an assembled matrix)
+  const double* pc = nullptr;
+  float* pc32 = nullptr;
+  long npc = 0;
+};
+
+// One batch member as the batched point-coefficient apply sees it (host twin
+// of the generated dcg_pc_member, same field order and types): its
+// compartment's tables and coefficients, the linearisation's time and
+// stiffness weight, its vectors by selector and the flags that end its work.
+struct PcMember {
+  const int* g2l = nullptr;  // null: the slots are the grid's vertices in order
+  const unsigned char* cmask = nullptr;
+  const double* pc = nullptr;
+  const float* pc32 = nullptr;
+  const double* tca = nullptr;
+  const double* prm = nullptr;
+  int n[4] = {1, 1, 1, 0};
+  double* v[kBatchVec] = {};
+  float* f[kBatchVecF] = {};
+  const int* done = nullptr;    // BicgScalars::done of the member's Krylov solve
+  const int* irdone = nullptr;  // IrState::done of its refinement
+};
+
+// The operator of a batched Krylov solve when it is not the members' BCSR
+// matrices: apply(x, y, w1, w2, part, skip) computes y = A x for vector
+// selectors (FP32 solves: kVKry + i for the FP32 vector i) with the partial
+// dots (y.w1, y.w2) per work item into part (w1 < 0: none), skipping members
+// whose solve is done (skip 0), whose refinement is done (1) or none (2);
+// range[m] = [first, last) partial pairs of member m.
+struct BatchOperator {
+  std::function<void(int x, int y, int w1, int w2, double* part, int skip)> apply;
+  const int2* range = nullptr;
 };
 
 // per-member state of FP64 iterative refinement around the FP32 Krylov solve
@@ -104,6 +140,11 @@ struct AsmMember {
   int n[4] = {1, 1, 1, 0};
   double origin[3] = {0, 0, 0};
   double h[3] = {0, 0, 0};
+  // point storage: coefficients per compartment, the rows' diagonal blocks,
+  // and where the set-up leaves the linearisation's (time, stiffness weight)
+  double* pc[kAsmComps] = {};
+  double* pdiag = nullptr;
+  double* tca = nullptr;
 };
 
 void b_copy(int B, const BatchWork& w, const MemberView* mv, Skip sk, int dst, int src, hipStream_t s);
@@ -126,7 +167,8 @@ void b_bicg_init(int B, const BatchWork& w, const MemberView* mv, int b, int x, 
 // ev (optional, 4 events): recorded before and after each of the two SpMVs
 template <typename V>
 void b_bicg_iteration(int B, int variant, const BatchWork& w, const MemberView* mv, int x, bool jacobi,
-                      double* part, BicgScalars* sc, hipStream_t st, hipEvent_t* ev = nullptr);
+                      double* part, BicgScalars* sc, hipStream_t st, hipEvent_t* ev = nullptr,
+                      const BatchOperator* op = nullptr);
 // y = A x on the FP64 matrix (selectors), members skipped by sk
 void b_spmv64(int B, int variant, const BatchWork& w, const MemberView* mv, Skip sk, int x, int y, hipStream_t st);
 // y = A x on the FP32 matrix copy, FP32 Krylov vectors (selectors kVKry + i)
