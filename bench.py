@@ -60,11 +60,11 @@ def make_workload(wl, name, precision, precond="Jacobi"):
 
 
 TRAFFIC_FILES = {("ms96", "fp64"): "traffic_mitchell_schaefer_q1_96_fp64.txt"}
-VARIANT_KERNELS = {"points": "dcg_tile_0_p", "lds_blocks": "spmv_blk_k", "lds_entries": "spmv_k", "rows_g8": "spmv_rows_k",
+VARIANT_KERNELS = {"lds_blocks": "spmv_blk_k", "lds_entries": "spmv_k", "rows_g8": "spmv_rows_k",
                    "rows_g16": "spmv_rows_k", "rows_g32": "spmv_rows_k", "rows_g64": "spmv_rows_k"}
 
 
-def committed_traffic(workload, precision, chosen):
+def committed_traffic(workload, precision, chosen_kernel):
     """HBM bytes per launch of the SpMV from the committed counter summary
     (rocprofv3 FETCH_SIZE/WRITE_SIZE passes calibrated on a plain copy), when
     it covers the kernel variant this run uses; None otherwise."""
@@ -76,7 +76,7 @@ def committed_traffic(workload, precision, chosen):
         if "=" in line and not line.startswith("#"):
             k, v = line.split("=", 1)
             kv[k.strip()] = v.strip()
-    if not kv.get("kernel", "").startswith(VARIANT_KERNELS.get(chosen, "?")):
+    if kv.get("kernel", "") != chosen_kernel:
         return None
     return float(kv["traffic_bytes"])
 
@@ -217,7 +217,8 @@ def batch_measure(args, dca, wl, precision, warmup, steps, plain_stride, cpu=Tru
                              nl_atol=0.0, lin_rtol=1e-10)
     t_setup = time.perf_counter()
     b = dca.Batch(f"precision = {precision}\ngraph_iterations = {args.graph_iterations}\n"
-                  f"preconditioner = {args.preconditioner}\n", 0)
+                  f"preconditioner = {args.preconditioner}\n"
+                  f"refinement.inner_relative_tolerance = {args.ir_inner:g}\n", 0)
     ids = []
     for w in ws:
         i = b.add(w.ini)
@@ -348,16 +349,36 @@ def run_batch(args, dca, wl):
     print(json.dumps(batch_measure(args, dca, wl, args.precision, args.warmup, args.steps, args.plain_stride)))
 
 
-def serve_measure(args, dca, wl, precision, R, qps, min_steps, max_steps):
-    """Config: a Poisson stream of seeded random models served by one Server.
+def serve_measure(args, dca, wl, precision, R, qps, min_steps, max_steps, plain_steps=None, error_members=None):
+    """Config: a Poisson stream of seeded random models served by one Server
+    (qps <= 0: every request arrives at once, the saturated server).
     Returns the measurement as a dict."""
     import torch
 
+    plain_steps = args.plain_steps if plain_steps is None else plain_steps
+    error_members = args.error_members if error_members is None else error_members
     reqs = [wl.random_request(args.serve_seed * 100000 + i, precision=precision,
                               steps_range=(min_steps, max_steps)) for i in range(R)]
     rng = np.random.Generator(np.random.PCG64(args.serve_seed))
-    arrivals = np.cumsum(rng.exponential(1.0 / qps, size=R))
-    arrivals -= arrivals[0]
+    if qps > 0:
+        arrivals = np.cumsum(rng.exponential(1.0 / qps, size=R))
+        arrivals -= arrivals[0]
+    else:
+        arrivals = np.zeros(R)
+    # the compiled kernels of every request's model are cached on disk (a
+    # parameter sweep or fit sends the same model structure again and again):
+    # one untimed pass of one step per request builds and compiles them, so
+    # the timed stream measures serving, not first-time hipRTC compilation
+    t_warm = time.perf_counter()
+    warm = dca.Server(f"precision = {precision}\nmax_members = {args.max_members}\nworkers = {args.workers}\n"
+                      f"preconditioner = {args.preconditioner}\n", 0)
+    wids = [warm.submit(r.ini, 1) for r in reqs]
+    while warm.tick() > 0:
+        pass
+    if any(warm.result(i)[1].status for i in wids):
+        raise SystemExit("warm-up request failed")
+    warm.close()
+    warm_s = time.perf_counter() - t_warm
     srv = dca.Server(f"precision = {precision}\nmax_members = {args.max_members}\n"
                      f"workers = {args.workers}\ngraph_iterations = {args.graph_iterations}\n"
                      f"preconditioner = {args.preconditioner}\n", 0)
@@ -397,7 +418,7 @@ def serve_measure(args, dca, wl, precision, R, qps, min_steps, max_steps):
              "model.time_step_operator.nonlinear_solver.convergence_condition.absolute_tolerance = 1e-28\n")
     order = sorted(range(R), key=lambda k: infos[k].n_dof * infos[k].steps)
     errs = {}
-    for k in order[: args.error_members]:
+    for k in order[: error_members]:
         u, _ = srv.result(ids[k])
         ref = dca.Model(reqs[k].ini, tight, 0)
         ref.set_state(ref.initial_state(), 0.0)
@@ -416,10 +437,10 @@ def serve_measure(args, dca, wl, precision, R, qps, min_steps, max_steps):
         errs[reqs[k].name] = max(e)
     srv.close()
     plain = None
-    if args.plain_steps > 0:
+    if plain_steps > 0:
         print("# plain FP64 path on the same requests ...", file=sys.stderr, flush=True)
         plain = plain_fp64(dca, [(r.ini, r.dt, r.stages, r.steps, lambda m: m.initial_state(), k)
-                                 for k, r in enumerate(reqs)], args.plain_steps, args.preconditioner)
+                                 for k, r in enumerate(reqs)], plain_steps, args.preconditioner)
         plain["speedup_of_this_line"] = (work / makespan) / plain["value"]
     out = {
         "metric": "Newton DOF-steps/s (plain FP64 vs mixed/batched), p50/p99 latency, HBM GB/s",
@@ -440,6 +461,7 @@ def serve_measure(args, dca, wl, precision, R, qps, min_steps, max_steps):
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                       "max": float(lat.max())},
         "queue_s": {"p50": float(np.percentile(queue, 50)), "p99": float(np.percentile(queue, 99))},
+        "compile_warmup_s": warm_s,
         "makespan_s": makespan,
         "newton_dof_steps": work,
         "error_vs_fp64": {"max_rel_l2_per_species": max(errs.values()) if errs else None, "requests": errs},
@@ -473,12 +495,16 @@ def main():
                          "cell applied by the Krylov operator (linear_solver.storage)")
     ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
                     help="default: fp64 for single models, mixed for gs_batch and serve")
+    ap.add_argument("--point-tile", default="auto",
+                    help="point storage: the apply's tile shape, auto (timed at set-up) or a candidate index")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true",
                     help="single-model FP64 runs: skip the mixed-precision run of the same workload")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this size (counter calibration under rocprofv3)")
+    ap.add_argument("--ir-inner", type=float, default=1e-3,
+                    help="mixed precision: relative tolerance of each FP32 Krylov solve inside the FP64 refinement")
     ap.add_argument("--members", type=int, default=256, help="batch workload: number of models")
     ap.add_argument("--graph-iterations", type=int, default=4, help="batch workload: iterations per hipGraph")
     ap.add_argument("--batch-nl-rtol", type=float, default=1e-16,
@@ -527,6 +553,8 @@ def main():
     w, wname = make_workload(wl, args.workload, args.precision, args.preconditioner)
 
     storage = f"model.time_step_operator.linear_solver.storage = {args.storage}\n"
+    if args.point_tile != "auto":
+        storage += f"model.time_step_operator.linear_solver.point_tile = {args.point_tile}\n"
     if args.readback_iterations:
         storage += f"model.time_step_operator.linear_solver.readback_iterations = {args.readback_iterations}\n"
     t_setup = time.perf_counter()
@@ -535,8 +563,9 @@ def main():
     m.set_state(u0, 0.0)
     setup_s = time.perf_counter() - t_setup
     tuning, chosen = m.spmv_tuning()
+    ptuning = m.point_tuning()
     print(f"# setup {setup_s:.1f}s, {m.n_dof} DOF, {m.n_blocks} blocks of {m.block_size}x{m.block_size}; "
-          f"SpMV autotune {tuning} -> {chosen}", file=sys.stderr, flush=True)
+          f"SpMV autotune {tuning} -> {chosen}; point-apply tiles {ptuning}", file=sys.stderr, flush=True)
 
     t = 0.0
     for k in range(args.warmup):
@@ -589,10 +618,16 @@ def main():
     sp = kinds.get(spmv_kind)
     roofline = None
     if sp:
-        variant = "points" if spmv_kind.startswith("point") else chosen
-        roofline = {"bound": "hbm", "kernel": spmv_kind, "achieved": sp["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": sp["gbs"] / HBM_PEAK_GBS,
-                    "traffic": committed_traffic(args.workload, args.precision, variant),
+        if spmv_kind.startswith("point"):
+            # the generated apply of compartment 0 with the tile shape in use
+            shapes = [ch for c, _, _, ch in ptuning if c == 0]
+            k = shapes.index(True) if True in shapes else 0
+            kernel = f"dcg_tile_0_p{'f' if spmv_kind.endswith('32') else ''}{k}"
+        else:
+            kernel = VARIANT_KERNELS.get(chosen, "?")
+        roofline = {"bound": "hbm", "kernel": spmv_kind, "kernel_name": kernel, "achieved": sp["gbs"],
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sp["gbs"] / HBM_PEAK_GBS,
+                    "traffic": committed_traffic(args.workload, args.precision, kernel),
                     "avg_launch_us": sp["avg_us"], "algorithmic_bytes_per_launch": sp["bytes_per_launch"]}
 
     probe = None
@@ -608,7 +643,8 @@ def main():
         # with its error against this FP64 run
         u64 = u_timed
         w32, _ = make_workload(wl, args.workload, "mixed", args.preconditioner)
-        m32 = dca.Model(w32.ini, storage, 0)
+        m32 = dca.Model(w32.ini, storage + "model.time_step_operator.linear_solver.refinement.inner_relative_tolerance = "
+                        f"{args.ir_inner:g}\n", 0)
         m32.set_state(u0, 0.0)
         t32 = 0.0
         for k in range(args.warmup):
@@ -655,12 +691,19 @@ def main():
         batch_obj = {k: v for k, v in batch_obj.items() if k not in ("metric", "n_gpus", "higher_is_better", "scaling",
                                                                     "vs_baseline", "data")}
     if args.workload == "ms96" and not args.no_serve:
-        print("# serving sample ...", file=sys.stderr, flush=True)
-        serve_obj = serve_measure(args, dca, wl, "mixed", args.line_requests, args.line_qps, args.line_min_steps,
-                                  args.line_max_steps)
-        serve_obj = {k: v for k, v in serve_obj.items() if k not in ("metric", "n_gpus", "higher_is_better", "scaling",
-                                                                    "vs_baseline", "data", "roofline",
-                                                                    "cpu_baseline", "warmup")}
+        # the same requests twice: all at once (the saturated server's
+        # throughput beside the plain FP64 path), then as a Poisson stream at
+        # a rate the server sustains (latency of service rather than queueing)
+        drop = ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline", "data", "roofline", "cpu_baseline",
+                "warmup")
+        print("# serving sample, all requests at once ...", file=sys.stderr, flush=True)
+        sat = serve_measure(args, dca, wl, "mixed", args.line_requests, 0.0, args.line_min_steps, args.line_max_steps)
+        print(f"# serving sample at {args.line_qps} requests/s ...", file=sys.stderr, flush=True)
+        paced = serve_measure(args, dca, wl, "mixed", args.line_requests, args.line_qps, args.line_min_steps,
+                              args.line_max_steps, plain_steps=0, error_members=0)
+        serve_obj = {"saturated": {k: v for k, v in sat.items() if k not in drop},
+                     "paced": {k: v for k, v in paced.items() if k not in drop + ("error_vs_fp64",
+                                                                                 "plain_fp64_same_requests")}}
     out = {
         "metric": "Newton DOF-steps/s (plain FP64 vs mixed/batched), p50/p99 latency, HBM GB/s",
         "value": value,
@@ -684,6 +727,9 @@ def main():
         "roofline": roofline,
         # the SpMV variants exist only for an assembled matrix
         "spmv_autotune": {"candidates_us": tuning, "chosen": chosen} if "spmv64" in kinds or "spmv32" in kinds else None,
+        # the point apply's tile shapes (cells per workgroup) timed at set-up
+        "point_tile_autotune": [{"compartment": c, "tile": list(tl), "us": round(us, 2), "chosen": ch}
+                                for c, tl, us, ch in ptuning] or None,
         "mixed_precision_same_workload": mixed,
         "copy_probe": probe,
         "cpu_baseline": cpu,

@@ -97,6 +97,7 @@ _SIGS = {
     "dcg_profile_enable": (_I, [_P, _I]),
     "dcg_profile_read": (_I, [_P, _I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "dcg_spmv_tuning": (_I, [_P, ctypes.POINTER(_I), _P, _P, ctypes.POINTER(_I)]),
+    "dcg_point_tuning": (_I, [_P, ctypes.POINTER(_I), _P, _P, _P, _P]),
     "dcg_probe_copy": (_I, [_P, _S, ctypes.POINTER(_D)]),
     "dcg_model_create_ex": (_I, [ctypes.c_char_p, ctypes.c_char_p, _I, ctypes.c_uint, ctypes.POINTER(_P)]),
     "dcg_param_info": (_I, [_P, _S, ctypes.c_char_p, _S, ctypes.POINTER(_S)]),
@@ -364,6 +365,17 @@ class Model:
         for v, t in zip(var[:n.value], us[:n.value]):
             out[names[int(v) % 100] + ("_fp32" if v >= 100 else "")] = float(t)
         return out, names[ch.value]
+
+    def point_tuning(self):
+        """[(compartment, (tx, ty, tz), microseconds, chosen)] of the point-apply tile autotuner."""
+        n = _I()
+        comp = np.zeros(64, dtype=np.int32)
+        tile = np.zeros(192, dtype=np.int32)
+        us = np.zeros(64)
+        ch = np.zeros(64, dtype=np.int32)
+        _check(self._lib.dcg_point_tuning(self._h, ctypes.byref(n), _ptr(comp), _ptr(tile), _ptr(us), _ptr(ch)))
+        return [(int(comp[k]), tuple(int(x) for x in tile[3 * k:3 * k + 3]), float(us[k]), bool(ch[k]))
+                for k in range(n.value)]
 
     # ---- time loop
     def evolve_defaults(self) -> EvolveOptions:

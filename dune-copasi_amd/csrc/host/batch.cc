@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cstddef>
+#include <cstdlib>
 #include <cmath>
 
 namespace dcopasi {
@@ -22,6 +23,15 @@ BatchOptions BatchOptions::from_ini(const std::string& text) {
   o.ir_max = c.get("refinement.max_steps", o.ir_max);
   o.lin_maxit = c.get("iteration_limit", o.lin_maxit);
   o.graph_iterations = c.get("graph_iterations", o.graph_iterations);
+  {
+    const std::string pt = c.get("point_tile", std::string("auto"));
+    if (pt != "auto") {
+      char* end = nullptr;
+      const long k2 = std::strtol(pt.c_str(), &end, 10);
+      if (pt.empty() || *end || k2 < 0 || k2 > 15) throw IOError("point_tile must be 'auto' or a candidate index in [0, 15]");
+      o.point_tile = static_cast<int>(k2);
+    }
+  }
   if (o.graph_iterations < 0 || o.graph_iterations > 64) throw IOError("graph_iterations must be in [0, 64]");
   return o;
 }
@@ -230,7 +240,10 @@ void Batch::rebuild() {
       std::vector<k::PcMember> pcm(M);
       std::vector<int2> prange(M);
       std::map<const RtcModule*, size_t> by_mod;
-      std::vector<std::vector<int2>> items;
+      // per launch and candidate tile shape: the (member, tile) items and
+      // each owner's launch-local [first, last)
+      std::vector<std::vector<std::vector<int2>>> items;
+      std::vector<std::vector<std::vector<int2>>> ranges;
       std::vector<std::vector<int>> owners;
       for (int j = 0; j < M; ++j) {
         Model& m = *members_[g.mem[j]].m;
@@ -255,40 +268,96 @@ void Batch::rebuild() {
         pm.irdone = reinterpret_cast<const int*>(reinterpret_cast<const char*>(g.ir.get() + j) +
                                                  offsetof(k::IrState, done));
         const RtcModule* key = m.module_.get();
+        const size_t nk = m.gen_.apply_tiles[c0].size();
         auto it = by_mod.find(key);
         if (it == by_mod.end()) {
           it = by_mod.emplace(key, g.plaunch.size()).first;
           g.plaunch.emplace_back();
           auto& pl = g.plaunch.back();
           const std::string name = m.gen_.tile_kernel[c0];
-          pl.f64 = m.module_->function(name + "_pbatch");
-          if (opt_.precision == Precision::Mixed) pl.f32 = m.module_->function(name + "_pfbatch");
-          items.emplace_back();
+          for (size_t k2 = 0; k2 < nk; ++k2) {
+            const std::string id = std::to_string(k2);
+            pl.f64s.push_back(m.module_->function(name + "_pbatch" + id));
+            pl.f32s.push_back(opt_.precision == Precision::Mixed ? m.module_->function(name + "_pfbatch" + id)
+                                                                 : nullptr);
+            pl.tiles.push_back(m.gen_.apply_tiles[c0][k2]);
+            pl.nts.push_back(apply_tile_threads(m.L_.grid.dim, m.gen_.apply_tiles[c0][k2], m.L_.comps[c0].b));
+          }
+          items.emplace_back(nk);
+          ranges.emplace_back(nk);
           owners.emplace_back();
         }
-        std::vector<int> tiles(m.comp_dev_[c0].ntiles);
-        if (!tiles.empty())
-          DCG_HIP(hipMemcpy(tiles.data(), m.comp_dev_[c0].tiles.get(), tiles.size() * sizeof(int),
-                            hipMemcpyDeviceToHost));
-        prange[j] = make_int2(static_cast<int>(items[it->second].size()), 0);  // launch-local for now
-        for (int t : tiles) items[it->second].push_back(make_int2(j, t));
-        prange[j].y = static_cast<int>(items[it->second].size());
-        owners[it->second].push_back(j);
+        const size_t l = it->second;
+        for (size_t k2 = 0; k2 < nk; ++k2) {
+          const auto& cd = m.comp_dev_[c0];
+          std::vector<int> tiles(cd.natiles[k2]);
+          if (!tiles.empty())
+            DCG_HIP(hipMemcpy(tiles.data(), cd.atiles[k2].get(), tiles.size() * sizeof(int), hipMemcpyDeviceToHost));
+          auto& it2 = items[l][k2];
+          int2 r = make_int2(static_cast<int>(it2.size()), 0);
+          for (int t : tiles) it2.push_back(make_int2(j, t));
+          r.y = static_cast<int>(it2.size());
+          ranges[l][k2].push_back(r);
+        }
+        owners[l].push_back(j);
       }
+      g.pcm.upload(pcm, stream_);
+      // the tile shape of each launch: the candidate with the shortest
+      // batched apply over the launch's members (no skips, no dot products;
+      // the candidates run in turn over several rounds)
+      hipEvent_t e0, e1;
+      DCG_HIP(hipEventCreate(&e0));
+      DCG_HIP(hipEventCreate(&e1));
       int off = 0;
       for (size_t l = 0; l < g.plaunch.size(); ++l) {
         auto& pl = g.plaunch[l];
-        pl.work.upload(items[l], stream_);
-        pl.nwork = static_cast<int>(items[l].size());
+        const size_t nk = pl.f64s.size();
+        size_t best = 0;
+        if (opt_.point_tile >= 0) {
+          best = static_cast<size_t>(opt_.point_tile) < nk ? static_cast<size_t>(opt_.point_tile) : 0;
+        } else if (nk > 1) {
+          std::vector<DeviceArray<int2>> dw(nk);
+          for (size_t k2 = 0; k2 < nk; ++k2) dw[k2].upload(items[l][k2], stream_);
+          std::vector<double> ms(nk, 0.0);
+          const k::PcMember* mem = g.pcm.get();
+          for (int round = 0; round < 4; ++round)
+            for (size_t k2 = 0; k2 < nk; ++k2) {
+              const int2* work = dw[k2].get();
+              int nwork = static_cast<int>(items[l][k2].size()), o0 = 0, xs = k::kVKry, ys = k::kVKry + 8, a1 = -1,
+                  a2 = -1, sk = -1;
+              double* part = nullptr;
+              void* args[] = {&work, &nwork, &o0, &mem, &xs, &ys, &a1, &a2, &sk, &part};
+              const unsigned grid = static_cast<unsigned>((nwork + 7) / 8 * 8);
+              if (nwork <= 0) continue;
+              DCG_HIP(hipEventRecord(e0, stream_));
+              for (int rep = 0; rep < 2; ++rep)
+                DCG_HIP(hipModuleLaunchKernel(pl.f64s[k2], grid, 1, 1, static_cast<unsigned>(pl.nts[k2]), 1, 1, 0,
+                                              stream_, args, nullptr));
+              DCG_HIP(hipEventRecord(e1, stream_));
+              DCG_HIP(hipEventSynchronize(e1));
+              float t = 0.f;
+              DCG_HIP(hipEventElapsedTime(&t, e0, e1));
+              if (round) ms[k2] += t;
+            }
+          for (size_t k2 = 1; k2 < nk; ++k2)
+            if (ms[k2] < ms[best]) best = k2;
+        }
+        pl.f64 = pl.f64s[best];
+        pl.f32 = pl.f32s[best];
+        pl.tile = pl.tiles[best];
+        pl.nt = pl.nts[best];
+        pl.work.upload(items[l][best], stream_);
+        pl.nwork = static_cast<int>(items[l][best].size());
         pl.off = off;
-        for (int j : owners[l]) {
-          prange[j].x += off;
-          prange[j].y += off;
+        for (size_t o = 0; o < owners[l].size(); ++o) {
+          const int j = owners[l][o];
+          prange[j] = make_int2(ranges[l][best][o].x + off, ranges[l][best][o].y + off);
         }
         off += pl.nwork;
       }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
       g.nitems = off;
-      g.pcm.upload(pcm, stream_);
       g.prange.upload(prange, stream_);
       g.part.resize(2 * std::max<size_t>(std::max(vwork.size(), static_cast<size_t>(g.nitems)), 1) + 64);
       const int gi = static_cast<int>(&g - sgroups_.data());
@@ -462,6 +531,14 @@ std::vector<std::array<int, 3>> Batch::spmv_choice() {
   return v;
 }
 
+std::vector<std::array<int, 3>> Batch::point_tiles() {
+  if (dirty_) rebuild();
+  std::vector<std::array<int, 3>> v;
+  for (const auto& g : sgroups_)
+    for (const auto& pl : g.plaunch) v.push_back(pl.tile);
+  return v;
+}
+
 // y = A x over the point members of group gi: one launch per generated module
 // (grid a multiple of 8; the kernel deals items to the XCDs in contiguous
 // runs).  FP32 solves index the members' FP32 vectors (selector - kVKry).
@@ -480,7 +557,8 @@ void Batch::point_apply(int gi, bool fp32, int x, int y, int w1, int w2, double*
     int nwork = pl.nwork, off = pl.off, xs = x, ys = y, a1 = w1, a2 = w2, sk = skip;
     void* args[] = {&work, &nwork, &off, &mem, &xs, &ys, &a1, &a2, &sk, &part};
     const unsigned grid = static_cast<unsigned>((pl.nwork + 7) / 8 * 8);
-    DCG_HIP(hipModuleLaunchKernel(fp32 ? pl.f32 : pl.f64, grid, 1, 1, 256, 1, 1, 0, stream_, args, nullptr));
+    DCG_HIP(hipModuleLaunchKernel(fp32 ? pl.f32 : pl.f64, grid, 1, 1, static_cast<unsigned>(pl.nt), 1, 1, 0, stream_,
+                                  args, nullptr));
   }
 }
 

@@ -20,6 +20,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <map>
 #include <memory>
 #include <string>
@@ -37,9 +38,10 @@ struct BatchOptions {
   int ir_max = 50;
   int lin_maxit = 500;
   int graph_iterations = 4;  // BiCGSTAB iterations per captured graph; 0: plain launches
+  int point_tile = -1;       // point-apply tile shape: -1 timed per launch, else the candidate index
   // keys: precision = fp64|mixed, preconditioner = Jacobi|VertexBlockJacobi|None,
   // refinement.inner_relative_tolerance, refinement.max_steps,
-  // iteration_limit, graph_iterations
+  // iteration_limit, graph_iterations, point_tile (auto or an index)
   static BatchOptions from_ini(const std::string& text);
 };
 
@@ -76,6 +78,8 @@ class Batch {
   const BatchOptions& options() const { return opt_; }
   // SpMV variant chosen for each block-size group: (B, FP64 variant, FP32 variant)
   std::vector<std::array<int, 3>> spmv_choice();
+  // point-apply tile shape chosen for each launch of each point group
+  std::vector<std::array<int, 3>> point_tiles();
 
   // Kernel timing: with profiling on, BiCGSTAB iterations run as plain
   // launches with HIP events around every batched SpMV (kind 0: FP64, 1:
@@ -122,7 +126,14 @@ class Batch {
     // (member, tile) items; item c of launch l writes partial pair off + c,
     // prange[j] = member j's [first, last) pairs
     struct PointLaunch {
+      // the kernels and tile shape in use, chosen by timing among the
+      // module's candidate tile shapes
       hipFunction_t f64 = nullptr, f32 = nullptr;
+      std::array<int, 3> tile{};
+      int nt = 256;  // workgroup size
+      std::vector<hipFunction_t> f64s, f32s;
+      std::vector<std::array<int, 3>> tiles;
+      std::vector<int> nts;
       DeviceArray<int2> work;
       int nwork = 0, off = 0;
     };

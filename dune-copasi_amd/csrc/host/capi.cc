@@ -345,6 +345,21 @@ int dcg_spmv_tuning(dcg_model_t h, int* count, int* variant, double* microsecond
   return 0;
 }
 
+int dcg_point_tuning(dcg_model_t h, int* count, int* comp, int* tile, double* microseconds, int* chosen) {
+  if (check(h)) return -1;
+  const auto& t = h->m->point_tuning();
+  const int n = static_cast<int>(std::min<size_t>(t.size(), 64));
+  if (count) *count = n;
+  for (int k = 0; k < n; ++k) {
+    if (comp) comp[k] = t[k].comp;
+    if (tile)
+      for (int d = 0; d < 3; ++d) tile[3 * k + d] = t[k].tile[d];
+    if (microseconds) microseconds[k] = t[k].us;
+    if (chosen) chosen[k] = h->m->point_tile(t[k].comp) == t[k].tile ? 1 : 0;
+  }
+  return 0;
+}
+
 int dcg_evolve_defaults(dcg_model_t h, dcg_evolve_options* o) {
   if (check(h)) return -1;
   return guarded([&] {

@@ -1213,6 +1213,19 @@ __device__ __forceinline__ void block_rows(int dx, int dy, int dz, int li, int l
   }
 }
 
+// The box geometry of one tile of TX x TY x TZ row vertices: the vertex box
+// around it (XB x YB x ZB) and the cells touching its rows (CXB x CYB x CZB).
+// The matrix-free apply uses the assembly tile; the point apply has its own,
+// chosen per model by timing the candidates.
+template <int X, int Y, int Z, int THREADS = 256>
+struct ApplyTile {
+  static constexpr int NT = THREADS;  // workgroup size
+  static constexpr int TX = X, TY = DIM > 1 ? Y : 1, TZ = DIM > 2 ? Z : 1;
+  static constexpr int XB = TX + 2, YB = DIM > 1 ? TY + 2 : 1, ZB = DIM > 2 ? TZ + 2 : 1;
+  static constexpr int CXB = TX + 1, CYB = DIM > 1 ? TY + 1 : 1, CZB = DIM > 2 ? TZ + 1 : 1;
+  static constexpr int NVB = XB * YB * ZB, NCB = CXB * CYB * CZB, NROW = TX * TY * TZ;
+};
+%APPLY_SHAPES%
 // The rows of one tile from point coefficients cellp(box cell)[((p*NQ + q)*NMC +
 // n)*WS] (the grid-wide array in HBM, or the set-up's staged LDS array) and the
 // direction staged in vs over the vertex box (slots sl, class masks cms): per
@@ -1223,17 +1236,19 @@ __device__ __forceinline__ void block_rows(int dx, int dy, int dz, int li, int l
 // are the identity, from v); with part, the workgroup's partial dots (y.w1,
 // y.w2) go to part[0] and part[1].  LDS scratch: accv (zeroed by the
 // caller) when SCATTER, else out.
-template <typename T, typename C, typename CP>
+template <typename TL, typename T, typename C, typename CP>
 __device__ __forceinline__ void cells_apply(CP cellp, size_t WS, const T* vs, const int* sl,
                                             const unsigned char* cms, T* out, T* accv, const T* __restrict__ v,
                                             T* __restrict__ y, double ca, const double (&dv)[NDV > 0 ? NDV : 1],
                                             const T* __restrict__ w1, const T* __restrict__ w2,
                                             double* __restrict__ part) {
+  constexpr int TX = TL::TX, TY = TL::TY, XB = TL::XB, YB = TL::YB, CXB = TL::CXB, CYB = TL::CYB;
+  constexpr int NCB = TL::NCB, NROW = TL::NROW;
   // corner values stay in registers for small blocks, in LDS otherwise; with
   // small blocks and one cell per thread the corner sums go straight onto
   // the vertices (SCATTER), otherwise through a per-cell corner array
   constexpr bool VREG = BC <= 4;
-  constexpr bool SCATTER = VREG && NCB <= 256;
+  constexpr bool SCATTER = VREG && NCB <= TL::NT;
   // the corner sums of box cell cl over its classes, handed to put(corner, i, value)
   auto cell_corner_sums = [&](int cl, unsigned m, auto&& put) {
     const int ci = cl % CXB, cj = (cl / CXB) % CYB, ck = cl / (CXB * CYB);
@@ -1384,7 +1399,7 @@ __device__ __forceinline__ void cells_apply(CP cellp, size_t WS, const T* vs, co
   }
   if (part) {
     // workgroup sums of the two partial dots (fixed order)
-    __shared__ double red[8];
+    __shared__ double red[32];  // two sums per wave, up to 16 waves
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       d1 += __shfl_down(d1, o, 64);
@@ -1543,8 +1558,9 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
       }
       __syncthreads();
       if constexpr (CELL_APPLY) {
-        cells_apply<double, double>([&](int cl) { return (const double*)work + cl; }, (size_t)NCB, xs, sl, cms,
-                                    nullptr, acc4, base, r, ca, dv, nullptr, nullptr, nullptr);
+        cells_apply<ApplyTile<TX, TY, TZ>, double, double>([&](int cl) { return (const double*)work + cl; },
+                                                           (size_t)NCB, xs, sl, cms, nullptr, acc4, base, r, ca, dv,
+                                                           nullptr, nullptr, nullptr);
         return;
       }
     }
@@ -1898,18 +1914,20 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
 // compartment's slots are the grid's vertices in order.  With part, the
 // workgroup also writes its rows' partial dots (y.w1, y.w2) to part[0] and
 // part[1] (w2 may be null), as the BiCGSTAB finaliser reads.
-template <typename T>
+template <typename TL, typename T>
 __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,
                                          const T* __restrict__ pc, const T* __restrict__ v, T* __restrict__ y,
                                          const double* __restrict__ tca, int n0, int n1, int n2,
                                          const double* __restrict__ prm, const T* __restrict__ w1,
                                          const T* __restrict__ w2, double* __restrict__ part) {
+  constexpr int TX = TL::TX, TY = TL::TY, TZ = TL::TZ, XB = TL::XB, YB = TL::YB, CXB = TL::CXB, CYB = TL::CYB;
+  constexpr int NVB = TL::NVB, NCB = TL::NCB;
   const double t = tca[0], ca = tca[1];
   // corner values stay in registers for small blocks, in LDS otherwise; with
   // small blocks and one cell per thread the corner sums go straight onto
   // the vertices (SCATTER), otherwise through a per-cell corner array
   constexpr bool VREG = BC <= 4;
-  constexpr bool SCATTER = VREG && NCB <= 256;
+  constexpr bool SCATTER = VREG && NCB <= TL::NT;
   __shared__ T vs[NVB * BC];
   __shared__ int sl[NVB];
   __shared__ unsigned char cms[NCB];
@@ -1948,8 +1966,8 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
   __syncthreads();
   double dv[NDV > 0 ? NDV : 1];
   uniform_diffusion(t, prm, dv);
-  cells_apply<T, T>([&](int cl) { return pc + coff[cl]; }, (size_t)n0 * (size_t)n1 * (size_t)n2, vs, sl, cms, out,
-                    accv, v, y, ca, dv, w1, w2, part);
+  cells_apply<TL, T, T>([&](int cl) { return pc + coff[cl]; }, (size_t)n0 * (size_t)n1 * (size_t)n2, vs, sl, cms,
+                        out, accv, v, y, ca, dv, w1, w2, part);
 }
 
 }  // namespace %C%
@@ -1992,8 +2010,8 @@ extern "C" __global__ void __launch_bounds__(256)
 // (b % 8) * (grid / 8) + b / 8, so each XCD (workgroups are dealt to the 8
 // XCDs in turn) walks one contiguous run of tiles and finds the neighbour
 // tiles' coefficients in its own L2.  Position c writes part[2c], part[2c+1].
-#define DCG_POINT_ENTRY(NAME, T)                                                                                  \
-  extern "C" __global__ void __launch_bounds__(256)                                                               \
+#define DCG_POINT_ENTRY(NAME, T, TL)                                                                              \
+  extern "C" __global__ void __launch_bounds__(TL::NT)                                                            \
   NAME(const int* __restrict__ tiles, int ntile, const int* __restrict__ g2l,                                     \
        const unsigned char* __restrict__ cmask, const T* __restrict__ pc, const T* __restrict__ v,                 \
        T* __restrict__ y, const double* __restrict__ tca, int n0, int n1, int n2, const double* __restrict__ prm,  \
@@ -2002,11 +2020,10 @@ extern "C" __global__ void __launch_bounds__(256)
     if (stop && *stop) return;                                                                                    \
     const int c = (int)(blockIdx.x % 8u) * (int)(gridDim.x / 8u) + (int)(blockIdx.x / 8u);                       \
     if (c >= ntile) return;                                                                                       \
-    %C%::pc_apply<T>(tiles[c], g2l, cmask, pc, v, y, tca, n0, n1, n2, prm, w1, w2,                                \
-                     part ? part + 2 * (size_t)c : nullptr);                                                      \
+    %C%::pc_apply<TL, T>(tiles[c], g2l, cmask, pc, v, y, tca, n0, n1, n2, prm, w1, w2,                            \
+                         part ? part + 2 * (size_t)c : nullptr);                                                  \
   }
-DCG_POINT_ENTRY(%KERNEL%_p, double)
-DCG_POINT_ENTRY(%KERNEL%_pf, float)
+%POINT_ENTRIES%
 #undef DCG_POINT_ENTRY
 )";
 
@@ -2034,8 +2051,8 @@ extern "C" __global__ void __launch_bounds__(256)
                     M.pc[%COMP%], M.own_start, M.own_mask, M.n[0], M.n[1], M.n[2], M.origin[0], M.origin[1],
                     M.origin[2], M.h[0], M.h[1], M.h[2], 8, M.prm);
 }
-#define DCG_PBATCH_ENTRY(NAME, T, VEC, PC)                                                                        \
-  extern "C" __global__ void __launch_bounds__(256)                                                               \
+#define DCG_PBATCH_ENTRY(NAME, T, VEC, PC, TL)                                                                    \
+  extern "C" __global__ void __launch_bounds__(TL::NT)                                                            \
   NAME(const int2* __restrict__ work, int nwork, int part_off, const dcg_pc_member* __restrict__ mem, int xsel,   \
        int ysel, int w1sel, int w2sel, int skip, double* __restrict__ part) {                                      \
     const int c = (int)(blockIdx.x % 8u) * (int)(gridDim.x / 8u) + (int)(blockIdx.x / 8u);                       \
@@ -2044,12 +2061,12 @@ extern "C" __global__ void __launch_bounds__(256)
     const dcg_pc_member& M = mem[w.x];                                                                            \
     if (skip == 0 && *M.done) return;                                                                             \
     if (skip == 1 && *M.irdone) return;                                                                           \
-    %C%::pc_apply<T>(w.y, M.g2l, M.cmask, M.PC, M.VEC[xsel], M.VEC[ysel], M.tca, M.n[0], M.n[1], M.n[2], M.prm,   \
+    %C%::pc_apply<TL, T>(w.y, M.g2l, M.cmask, M.PC, M.VEC[xsel], M.VEC[ysel], M.tca, M.n[0], M.n[1], M.n[2],      \
+                         M.prm,                                                                                   \
                      w1sel >= 0 ? M.VEC[w1sel] : nullptr, w2sel >= 0 ? M.VEC[w2sel] : nullptr,                    \
                      part ? part + 2 * (size_t)(part_off + c) : nullptr);                                         \
   }
-DCG_PBATCH_ENTRY(%KERNEL%_pbatch, double, v, pc)
-DCG_PBATCH_ENTRY(%KERNEL%_pfbatch, float, f, pc32)
+%PBATCH_ENTRIES%
 #undef DCG_PBATCH_ENTRY
 )";
 
@@ -2073,6 +2090,30 @@ extern "C" __global__ void __launch_bounds__(256)
 )";
 
 }  // namespace
+
+int apply_tile_threads(int dim, std::array<int, 3> t, int species) {
+  // one cell of the cell box per thread (corner sums scattered in LDS) for
+  // blocks of up to 4 species; larger blocks loop over the cells
+  int ncb = t[0] + 1;
+  if (dim > 1) ncb *= t[1] + 1;
+  if (dim > 2) ncb *= t[2] + 1;
+  if (species > 4 || ncb <= 256 || ncb > 1024) return 256;
+  return ncb <= 512 ? 512 : 1024;
+}
+
+std::vector<std::array<int, 3>> apply_tile_candidates(const Grid& g, int species, std::array<int, 3> setup_tile) {
+  std::vector<std::array<int, 3>> out{setup_tile};
+  if (species > 4) return out;  // corner values in LDS: the assembly tile only
+  // cell boxes of 256, 512 and 1024 cells around the rows (one cell per
+  // thread): squarer boxes re-read fewer halo cells, slabs along x give
+  // longer contiguous runs
+  std::vector<std::array<int, 3>> cand;
+  if (g.dim == 2) cand = {{15, 15, 1}, {31, 7, 1}, {16, 8, 1}, {31, 15, 1}, {31, 31, 1}};
+  else if (g.dim == 3) cand = {{8, 4, 4}, {7, 7, 3}, {6, 5, 5}, {15, 3, 3}, {7, 7, 7}, {15, 7, 3}, {15, 7, 7}};
+  for (auto c : cand)
+    if (c != setup_tile) out.push_back(c);
+  return out;
+}
 
 bool tile_eligible(const ModelSpec& m, int comp) {
   if (m.jacobian == JacobianType::Numerical && !m.is_linear) return false;
@@ -2106,7 +2147,8 @@ std::array<int, 3> choose_tile(const Grid& g, int bc, int nmc) {
 }
 
 std::string gen_tiled(const ModelSpec& m, const Grid& g, int comp, const std::string& ns, const std::string& kname,
-                      std::array<int, 3> tile, bool geo, bool points) {
+                      std::array<int, 3> tile, bool geo, bool points,
+                      const std::vector<std::array<int, 3>>& apply_tiles) {
   const auto T = element_tables(geo ? unit_cell(g) : g);
   std::ostringstream o;
   o << "namespace " << ns << " {\n";
@@ -2228,14 +2270,34 @@ std::string gen_tiled(const ModelSpec& m, const Grid& g, int comp, const std::st
   if (points) body += kTiledPoints;
   if (geo && comp < 4) body += kTiledBatch;
   if (geo && comp < 4 && points) body += kTiledPointBatch;
+  {
+    // one point apply (and batched point apply) per candidate tile shape k:
+    // %KERNEL%_p<k>, _pf<k>, _pbatch<k>, _pfbatch<k>
+    std::string pe, pb;
+    for (size_t k2 = 0; k2 < apply_tiles.size(); ++k2) {
+      const std::string id = std::to_string(k2);
+      const std::string tl = ns + "::ApplyShape" + id;
+      pe += "DCG_POINT_ENTRY(%KERNEL%_p" + id + ", double, " + tl + ")\n";
+      pe += "DCG_POINT_ENTRY(%KERNEL%_pf" + id + ", float, " + tl + ")\n";
+      pb += "DCG_PBATCH_ENTRY(%KERNEL%_pbatch" + id + ", double, v, pc, " + tl + ")\n";
+      pb += "DCG_PBATCH_ENTRY(%KERNEL%_pfbatch" + id + ", float, f, pc32, " + tl + ")\n";
+    }
+    // the candidate apply shapes as plain names (macro arguments hold no commas)
+    std::string as;
+    for (size_t k2 = 0; k2 < apply_tiles.size(); ++k2)
+      as += "using ApplyShape" + std::to_string(k2) + " = ApplyTile<" + std::to_string(apply_tiles[k2][0]) + ", " +
+            std::to_string(apply_tiles[k2][1]) + ", " + std::to_string(apply_tiles[k2][2]) + ", " +
+            std::to_string(apply_tile_threads(g.dim, apply_tiles[k2], static_cast<int>(m.comp_species[comp].size()))) +
+            ">;\n";
+    replace_all(body, "%APPLY_SHAPES%", as);
+    replace_all(body, "%POINT_ENTRIES%", pe);
+    replace_all(body, "%PBATCH_ENTRIES%", pb);
+  }
   replace_all(body, "%C%", ns);
   replace_all(body, "%KERNEL%", kname);
   replace_all(body, "%COMP%", std::to_string(comp));
   return o.str() + body;
 }
-
-namespace {
-}  // namespace
 
 GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L, bool runtime_geometry, bool points) {
   GeneratedSource out;
@@ -2253,6 +2315,7 @@ GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L, bool r
     if (spec.comp_species[c].empty()) {
       out.volume_kernel.push_back("");
       out.tile_kernel.push_back("");
+      out.apply_tiles.push_back({});
       out.tile_batch_kernel.push_back("");
       out.tile_dims.push_back({0, 0, 0});
       out.point_words.push_back(0);
@@ -2267,8 +2330,11 @@ GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L, bool r
       auto tile = choose_tile(L.grid, static_cast<int>(spec.comp_species[c].size()),
                               static_cast<int>(mc_pattern(spec, static_cast<int>(c)).size()));
       std::string tk = "dcg_tile_" + std::to_string(c);
-      src << gen_tiled(spec, L.grid, static_cast<int>(c), ns, tk, tile, geo, points);
+      const auto shapes = points ? apply_tile_candidates(L.grid, static_cast<int>(spec.comp_species[c].size()), tile)
+                                 : std::vector<std::array<int, 3>>{};
+      src << gen_tiled(spec, L.grid, static_cast<int>(c), ns, tk, tile, geo, points, shapes);
       out.tile_kernel.push_back(tk);
+      out.apply_tiles.push_back(shapes);
       out.tile_batch_kernel.push_back(geo && c < 4 ? tk + "_batch" : "");
       out.tile_dims.push_back(tile);
       const int nq = static_cast<int>(element_rule(L.grid.dim, L.grid.cube).w.size());
@@ -2277,6 +2343,7 @@ GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L, bool r
                                        : 0);
     } else {
       out.tile_kernel.push_back("");
+      out.apply_tiles.push_back({});
       out.tile_batch_kernel.push_back("");
       out.tile_dims.push_back({0, 0, 0});
       out.point_words.push_back(0);

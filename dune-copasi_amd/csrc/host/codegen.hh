@@ -36,6 +36,10 @@ struct GeneratedSource {
   // coefficients; 0 when the compartment has no point kernels), stored for
   // every cell of the grid, word-major: pc[word * cells + cell]
   std::vector<int> point_words;
+  // candidate tile shapes of the point apply per compartment (entry points
+  // <tile_kernel>_p<k>, _pf<k> and the batched _pbatch<k>, _pfbatch<k>);
+  // the first is the assembly tile
+  std::vector<std::vector<std::array<int, 3>>> apply_tiles;
   std::vector<FacetKernel> facet_kernels;
 };
 
@@ -49,6 +53,10 @@ constexpr int kGeoWords = 8;
 // the compartment's equations suit the tiled kernel (analytic Jacobian, no
 // velocity, diffusion uniform in space and independent of the state)
 bool tile_eligible(const ModelSpec& m, int comp);
+// tile shapes the point apply is generated for (the set-up tile first)
+std::vector<std::array<int, 3>> apply_tile_candidates(const Grid& g, int species, std::array<int, 3> setup_tile);
+// workgroup size of the point apply with tile t (cells per workgroup along x, y, z)
+int apply_tile_threads(int dim, std::array<int, 3> t, int species);
 
 // points: also emit the point-coefficient Jacobian entry points (_q, _p, _pf)
 // of every tiled compartment

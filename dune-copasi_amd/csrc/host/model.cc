@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cstddef>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 
@@ -79,6 +80,7 @@ Model::Model(const std::string& ini, const std::string& overrides, int device, c
   if (device_ >= 0) {
     setup_device();
     compile_and_load();
+    tune_points();
   }
 }
 
@@ -91,6 +93,7 @@ void Model::attach(int device, hipStream_t stream) {
   opt_.stream = stream;
   setup_device();
   compile_and_load();
+  tune_points();
 }
 
 Model::~Model() {
@@ -208,6 +211,16 @@ void Model::build(const std::string& ini, const std::string& overrides) {
     int bmax = 0;
     for (const auto& C : L_.comps) bmax = std::max(bmax, C.b);
     cfg_.points = why.empty() && (st == "points" || (st == "auto" && L_.grid.cube && bmax <= 4));
+  }
+  {
+    const std::string pt = c.get(p + "linear_solver.point_tile", std::string("auto"));
+    if (pt != "auto") {
+      char* end = nullptr;
+      const long k2 = std::strtol(pt.c_str(), &end, 10);
+      if (pt.empty() || *end || k2 < 0 || k2 > 15)
+        throw IOError("linear_solver.point_tile must be 'auto' or a candidate index in [0, 15]");
+      cfg_.point_tile = static_cast<int>(k2);
+    }
   }
   cfg_.readback_iterations = c.get(p + "linear_solver.readback_iterations", 0);
   if (cfg_.readback_iterations < 0) throw IOError("linear_solver.readback_iterations must be >= 0");
@@ -331,20 +344,32 @@ void Model::setup_device() {
     for (long l = 0; l < C.nslot; ++l) g2l[C.nodes[l]] = static_cast<int>(C.slot_off + l);
     std::vector<unsigned char> cm(gr.ncell, 0);
     for (size_t i = 0; i < C.cells.size(); ++i) cm[C.cells[i]] = C.cell_mask[i];
-    const auto td = gen_.tile_dims[c];
-    const long ntx = (gr.n[0] + 1 + td[0] - 1) / td[0];
-    const long nty = gr.dim > 1 ? (gr.n[1] + 1 + td[1] - 1) / td[1] : 1;
-    const long ntz = gr.dim > 2 ? (gr.n[2] + 1 + td[2] - 1) / td[2] : 1;
-    std::vector<char> used(ntx * nty * ntz, 0);
-    for (long l = 0; l < C.nslot; ++l) {
-      long v = C.nodes[l];
-      long i = v % (gr.n[0] + 1), rest = v / (gr.n[0] + 1);
-      long j = gr.dim > 1 ? rest % (gr.n[1] + 1) : 0, k = gr.dim > 2 ? rest / (gr.n[1] + 1) : 0;
-      used[(i / td[0]) + ntx * ((j / td[1]) + nty * (k / td[2]))] = 1;
+    // the tiles of shape td holding at least one of the compartment's vertices
+    auto tile_list = [&](std::array<int, 3> td) {
+      const long ntx = (gr.n[0] + 1 + td[0] - 1) / td[0];
+      const long nty = gr.dim > 1 ? (gr.n[1] + 1 + td[1] - 1) / td[1] : 1;
+      const long ntz = gr.dim > 2 ? (gr.n[2] + 1 + td[2] - 1) / td[2] : 1;
+      std::vector<char> used(ntx * nty * ntz, 0);
+      for (long l = 0; l < C.nslot; ++l) {
+        long v = C.nodes[l];
+        long i = v % (gr.n[0] + 1), rest = v / (gr.n[0] + 1);
+        long j = gr.dim > 1 ? rest % (gr.n[1] + 1) : 0, k = gr.dim > 2 ? rest / (gr.n[1] + 1) : 0;
+        used[(i / td[0]) + ntx * ((gr.dim > 1 ? j / td[1] : 0) + nty * (gr.dim > 2 ? k / td[2] : 0))] = 1;
+      }
+      std::vector<int> out;
+      for (long t = 0; t < ntx * nty * ntz; ++t)
+        if (used[t]) out.push_back(static_cast<int>(t));
+      return out;
+    };
+    std::vector<int> tiles = tile_list(gen_.tile_dims[c]);
+    const auto& shapes = gen_.apply_tiles[c];
+    comp_dev_[c].atiles.resize(shapes.size());
+    comp_dev_[c].natiles.assign(shapes.size(), 0);
+    for (size_t k2 = 0; k2 < shapes.size(); ++k2) {
+      auto at = tile_list(shapes[k2]);
+      comp_dev_[c].natiles[k2] = static_cast<int>(at.size());
+      comp_dev_[c].atiles[k2].upload(at, stream_);
     }
-    std::vector<int> tiles;
-    for (long t = 0; t < ntx * nty * ntz; ++t)
-      if (used[t]) tiles.push_back(static_cast<int>(t));
     comp_dev_[c].g2l.upload(g2l, stream_);
     {
       bool id = C.slot_off == 0 && C.nslot == gr.nvert;
@@ -353,11 +378,6 @@ void Model::setup_device() {
     }
     comp_dev_[c].cell_mask_grid.upload(cm, stream_);
     comp_dev_[c].tiles.upload(tiles, stream_);
-    if (static_cast<long>(tiles.size()) != ntx * nty * ntz) {
-      std::vector<int> tpos(ntx * nty * ntz, -1);
-      for (size_t i = 0; i < tiles.size(); ++i) tpos[tiles[i]] = static_cast<int>(i);
-      comp_dev_[c].tpos.upload(tpos, stream_);
-    }
     comp_dev_[c].ntiles = static_cast<int>(tiles.size());
   }
   {
@@ -491,7 +511,11 @@ void Model::setup_device() {
   const size_t nparts = std::max<size_t>(std::max<size_t>(static_cast<size_t>(plan_.nchunk), plan_.row_wgs) + 8,
                                          static_cast<size_t>((ns + 255) / 256) + 8);
   size_t ntiles_all = 0;
-  for (const auto& cd : comp_dev_) ntiles_all += static_cast<size_t>(std::max(0, cd.ntiles));
+  for (const auto& cd : comp_dev_) {
+    int most = std::max(0, cd.ntiles);
+    for (int n2 : cd.natiles) most = std::max(most, n2);
+    ntiles_all += static_cast<size_t>(most);
+  }
   part_.resize(std::max<size_t>(std::max<size_t>(2 * nparts, 2 * ntiles_all + 16), 8 * 1024 + 64));
   scalar_out_.resize(16);
   sc_.resize(1);
@@ -612,6 +636,57 @@ int Model::tune_spmv_plan(bool fp32) {
   return best_v;
 }
 
+// Times the point apply with every candidate tile shape on the model's own
+// grid (zero coefficients and vectors: the work does not depend on the
+// values; the candidates run in turn, several rounds, so caches favour none)
+// and keeps the fastest per compartment.
+void Model::tune_points() {
+  if (!cfg_.points || opt_.batch_member) return;
+  if (cfg_.point_tile >= 0) {
+    for (size_t c = 0; c < L_.comps.size(); ++c)
+      apply_shape_[c] = cfg_.point_tile < static_cast<int>(pc_app_fns_[c].size()) ? cfg_.point_tile : 0;
+    return;
+  }
+  bool any = false;
+  for (const auto& f : pc_app_fns_) any = any || f.size() > 1;
+  if (!any) return;
+  for (auto& a : pc_)
+    if (a.size()) k::fill(a.get(), 0.0, static_cast<long>(a.size()), stream_);
+  k::fill(kv_[0].get(), 0.0, L_.nslot * L_.B, stream_);
+  k::fill(pc_tca_.get(), 0.0, 2, stream_);
+  hipEvent_t e0, e1;
+  DCG_HIP(hipEventCreate(&e0));
+  DCG_HIP(hipEventCreate(&e1));
+  for (size_t c = 0; c < L_.comps.size(); ++c) {
+    const size_t nk = pc_app_fns_[c].size();
+    if (nk < 2) continue;
+    std::vector<double> ms(nk, 0.0);
+    for (int round = 0; round < 4; ++round)
+      for (size_t k2 = 0; k2 < nk; ++k2) {
+        apply_shape_[c] = static_cast<int>(k2);
+        launch_points_apply(kv_[0].get(), kv_[1].get(), nullptr, false);
+        DCG_HIP(hipEventRecord(e0, stream_));
+        launch_points_apply(kv_[0].get(), kv_[1].get(), nullptr, false);
+        launch_points_apply(kv_[0].get(), kv_[1].get(), nullptr, false);
+        DCG_HIP(hipEventRecord(e1, stream_));
+        DCG_HIP(hipEventSynchronize(e1));
+        float t = 0.f;
+        DCG_HIP(hipEventElapsedTime(&t, e0, e1));
+        if (round) ms[k2] += t;  // the first round warms up
+      }
+    size_t best = 0;
+    for (size_t k2 = 0; k2 < nk; ++k2) {
+      const auto& a = gen_.apply_tiles[c][k2];
+      point_tuning_.push_back({static_cast<int>(c), a, 1e3 * ms[k2] / 6.0});
+      if (ms[k2] < ms[best]) best = k2;
+    }
+    apply_shape_[c] = static_cast<int>(best);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  synchronize();
+}
+
 void Model::compile_and_load() {
   module_ = rtc_module(gen_.source, "dcopasi_assembly", device_);
   vol_fn_.assign(L_.comps.size(), nullptr);
@@ -621,6 +696,9 @@ void Model::compile_and_load() {
   pc_set_fn_.assign(L_.comps.size(), nullptr);
   pc_app_fn_.assign(L_.comps.size(), nullptr);
   pc_appf_fn_.assign(L_.comps.size(), nullptr);
+  pc_app_fns_.assign(L_.comps.size(), {});
+  pc_appf_fns_.assign(L_.comps.size(), {});
+  apply_shape_.assign(L_.comps.size(), 0);
   for (size_t c = 0; c < L_.comps.size(); ++c) {
     if (!gen_.volume_kernel[c].empty()) vol_fn_[c] = module_->function(gen_.volume_kernel[c]);
     if (!gen_.tile_kernel[c].empty()) {
@@ -629,8 +707,12 @@ void Model::compile_and_load() {
       tile_app_fn_[c] = module_->function(gen_.tile_kernel[c] + "_a");
       if (cfg_.points) {
         pc_set_fn_[c] = module_->function(gen_.tile_kernel[c] + "_q");
-        pc_app_fn_[c] = module_->function(gen_.tile_kernel[c] + "_p");
-        pc_appf_fn_[c] = module_->function(gen_.tile_kernel[c] + "_pf");
+        for (size_t k2 = 0; k2 < gen_.apply_tiles[c].size(); ++k2) {
+          pc_app_fns_[c].push_back(module_->function(gen_.tile_kernel[c] + "_p" + std::to_string(k2)));
+          pc_appf_fns_[c].push_back(module_->function(gen_.tile_kernel[c] + "_pf" + std::to_string(k2)));
+        }
+        pc_app_fn_[c] = pc_app_fns_[c].front();
+        pc_appf_fn_[c] = pc_appf_fns_[c].front();
       }
     }
   }
@@ -833,13 +915,17 @@ int Model::launch_points_apply(const void* v, void* y, const int* stop, bool fp3
     const unsigned char* cmask = comp_dev_[c].cell_mask_grid.get();
     const void* pc = fp32 ? static_cast<const void*>(pc32_[c].get()) : static_cast<const void*>(pc_[c].get());
     double* pp = part && facet_fn_.empty() ? part + 2 * static_cast<size_t>(nparts) : nullptr;
-    const int* tiles = comp_dev_[c].tiles.get();
-    int ntile = comp_dev_[c].ntiles;
+    // the apply's own tile shape (chosen at set-up) and its tile list
+    const int k2 = apply_shape_[c];
+    const int* tiles = comp_dev_[c].atiles[k2].get();
+    int ntile = comp_dev_[c].natiles[k2];
     void* args[] = {&tiles, &ntile, &g2l, &cmask, &pc, &v, &y, &tca, &n0, &n1, &n2, &prm, &stop, &w1, &w2, &pp};
     const unsigned grid = static_cast<unsigned>((ntile + 7) / 8 * 8);  // the kernel maps workgroups to XCD runs
-    DCG_HIP(hipModuleLaunchKernel(fp32 ? pc_appf_fn_[c] : pc_app_fn_[c], grid, 1, 1, 256, 1, 1, 0, stream_, args,
-                                  nullptr));
-    nparts += comp_dev_[c].ntiles;
+    const unsigned nt = static_cast<unsigned>(
+        apply_tile_threads(L_.grid.dim, gen_.apply_tiles[c][k2], L_.comps[c].b));
+    DCG_HIP(hipModuleLaunchKernel(fp32 ? pc_appf_fns_[c][k2] : pc_app_fns_[c][k2], grid, 1, 1, nt, 1, 1, 0, stream_,
+                                  args, nullptr));
+    nparts += ntile;
   }
   if (facet_fn_.empty()) return nparts;
   if (lin_ca_ != 0.0) {

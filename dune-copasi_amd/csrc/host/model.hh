@@ -8,6 +8,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <map>
 #include <memory>
 #include <string>
@@ -68,6 +69,10 @@ struct SolverConfig {
   // read-backs of its scalars (0: as many as the same solve took in the last
   // time step, then 4); 1 launches none past convergence (for profilers)
   int readback_iterations = 0;
+  // linear_solver.point_tile: the point apply's tile shape, "auto" (timed at
+  // set-up) or the index of a candidate shape (compartments with fewer
+  // candidates keep their first)
+  int point_tile = -1;
   double ir_inner_rtol = 1e-3;  // FP32 Krylov tolerance inside refinement
   int ir_max = 50;
   std::vector<std::string> warnings;
@@ -170,6 +175,14 @@ class Model {
   // device copy of `bytes` through a plain kernel; returns its milliseconds
   double probe_copy(long bytes);
 
+  // point-apply autotuning result: (compartment, tile shape, microseconds) per candidate
+  struct PointTuning {
+    int comp;
+    std::array<int, 3> tile;
+    double us;
+  };
+  const std::vector<PointTuning>& point_tuning() const { return point_tuning_; }
+  std::array<int, 3> point_tile(int c) const { return gen_.apply_tiles[c][apply_shape_[c]]; }
   // SpMV autotuning result: (variant, microseconds) per candidate; chosen variant
   const std::vector<std::pair<int, double>>& spmv_tuning() const { return tuning_; }
   int spmv_variant() const { return plan_.variant; }
@@ -208,6 +221,8 @@ class Model {
   // the number of partial pairs written
   int launch_points_apply(const void* v, void* y, const int* stop, bool fp32, const void* w1 = nullptr,
                           const void* w2 = nullptr, double* part = nullptr);
+  // set-up timing of the point apply's candidate tile shapes
+  void tune_points();
   // the assembled matrix for the kernel-level entry points of a model that
   // solves with point coefficients
   void ensure_matrix();
@@ -280,6 +295,10 @@ class Model {
   std::vector<hipFunction_t> tile_app_fn_;  // tiled matrix-free apply (_a)
   // point coefficients (_q), their FP64 (_p) and FP32 (_pf) apply
   std::vector<hipFunction_t> pc_set_fn_, pc_app_fn_, pc_appf_fn_;
+  // the point apply per candidate tile shape, and the shape in use, per compartment
+  std::vector<std::vector<hipFunction_t>> pc_app_fns_, pc_appf_fns_;
+  std::vector<int> apply_shape_;
+  std::vector<PointTuning> point_tuning_;
   std::vector<hipFunction_t> facet_fn_;
   std::vector<hipFunction_t> facetf_fn_;  // FP32 apply entries (point storage)
   double time_ = 0.0, dt_initial_ = 0.0, t_end_ = 0.0;
@@ -303,9 +322,9 @@ class Model {
     DeviceArray<unsigned char> cell_mask;
     // tiled kernel: vertex -> slot, cell -> class mask over the whole grid, tile list
     DeviceArray<int> g2l, tiles;
-    // position of every tile of the grid in `tiles` (-1: not listed); empty
-    // when `tiles` is every tile in order
-    DeviceArray<int> tpos;
+    // the point apply's candidate tile shapes: tile list and count per shape
+    std::vector<DeviceArray<int>> atiles;
+    std::vector<int> natiles;
     DeviceArray<unsigned char> cell_mask_grid;
     int ntiles = 0;
     bool g2l_identity = false;  // the compartment's slots are the grid's vertices in order

@@ -167,6 +167,14 @@ int dcg_probe_copy(dcg_model_t model, size_t bytes, double* milliseconds);
  * FP64 variant used.  Arrays hold at least 16 entries. */
 int dcg_spmv_tuning(dcg_model_t model, int* count, int* variant, double* microseconds, int* chosen);
 
+/* Tile shapes the set-up autotuner timed for the point-coefficient apply
+ * (linear_solver.storage = points): for each candidate k < *count, its
+ * compartment comp[k], its cell box tile[3k..3k+2] (cells per workgroup along
+ * x, y, z) and the microseconds of one apply; chosen[k] = 1 for the shape in
+ * use.  No candidates when the model has no point storage or a single shape.
+ * Arrays hold at least 64 entries (tile: 192). */
+int dcg_point_tuning(dcg_model_t model, int* count, int* comp, int* tile, double* microseconds, int* chosen);
+
 /* ---------------------------------------------------------------- transform / reduce
  * The reference's `model.reduce` keys (dune/copasi/model/diffusion_reaction/
  * reduce.hh:40-285, evaluated by the CLI after every step,

@@ -70,12 +70,16 @@ def test_runtime_parameters_match_folded(dca, wl):
     assert max(rel_l2_per_species(ref.compartments(), b, a)) < 1e-10
 
 
-@pytest.mark.parametrize("precision,graphs", [("mixed", 4), ("fp64", 4), ("mixed", 0)])
-def test_batch_gray_scott_matches_single(dca, wl, precision, graphs):
-    print(f"case: batch gray_scott x6 precision={precision} graphs={graphs}", flush=True)
+# point_tile: the batched point apply's tile shape, timed ("auto") or forced
+# to each candidate of the 2D Gray-Scott module (the first is the assembly tile)
+@pytest.mark.parametrize("precision,graphs,tile", [("mixed", 4, "auto"), ("fp64", 4, "auto"), ("mixed", 0, "auto"),
+                                                   ("mixed", 4, "1"), ("mixed", 4, "2"), ("fp64", 4, "3"),
+                                                   ("mixed", 4, "4"), ("fp64", 4, "5")])
+def test_batch_gray_scott_matches_single(dca, wl, precision, graphs, tile):
+    print(f"case: batch gray_scott x6 precision={precision} graphs={graphs} point_tile={tile}", flush=True)
     steps = 3
     ws = wl.gray_scott_batch(count=6, seed=3, side_min=8, side_max=40, precision=precision, steps=steps, **NL)
-    b = dca.Batch(f"precision = {precision}\ngraph_iterations = {graphs}\n", 0)
+    b = dca.Batch(f"precision = {precision}\ngraph_iterations = {graphs}\npoint_tile = {tile}\n", 0)
     ids = [b.add(w.ini) for w in ws]
     assert b.ids() == ids
     for i, w in zip(ids, ws):
@@ -97,6 +101,34 @@ def test_batch_gray_scott_matches_single(dca, wl, precision, graphs):
         err = max(rel_l2_per_species(info, u, ref))
         assert err < TOL, (w.name, err)
     assert b.device_bytes() > 0
+    b.close()
+
+
+@pytest.mark.parametrize("tile", [str(k) for k in range(8)])
+def test_batch_3d_point_tiles_match_single(dca, wl, tile):
+    """3D Mitchell-Schaefer members of uneven sides (Alexander2, block size 2)
+    with the batched point apply forced to each candidate tile shape, against
+    each model stepped alone with the assembled FP64 solve."""
+    print(f"case: batch mitchell_schaefer 3D point_tile={tile}", flush=True)
+    steps = 2
+    ws = [wl.mitchell_schaefer(n=n, dt=0.5, **NL) for n in (5, 9, 13)]
+    b = dca.Batch(f"precision = mixed\npoint_tile = {tile}\n", 0)
+    ids = [b.add(w.ini) for w in ws]
+    for i, w in zip(ids, ws):
+        b.set_time_step(i, w.dt)
+        mv = b.member(i)
+        mv.set_state(w.initial_state(mv), 0.0)
+        mv.close()
+    for s in range(steps):
+        st = b.step()
+        assert [x.status for x in st] == [0] * len(ws), s
+    for i, w in zip(ids, ws):
+        mv = b.member(i)
+        u, _ = mv.get_state()
+        mv.close()
+        ref, _, info = _solo(dca, w, steps, _solo_fp64())
+        err = max(rel_l2_per_species(info, u, ref))
+        assert err < TOL, (w.name, err)
     b.close()
 
 

@@ -76,6 +76,41 @@ def test_point_apply_matches_assembled(dca, wl, name, make, cm, ca):
     assert np.abs(y - bcsr_to_dense_csr(gpu, J) @ v).max() <= 1e-12 * scale
 
 
+TILE_CASES = [
+    ("gs_q1_2d", lambda wl: wl.gray_scott(n=37, cell_type="cube")),
+    ("ms_q1_3d", lambda wl: wl.mitchell_schaefer(n=11, cell_type="cube")),
+    ("two_q1_2d", lambda wl: wl.two_compartments(n=23, cell_type="cube")),
+]
+
+
+@pytest.mark.parametrize("name,make", TILE_CASES, ids=[c[0] for c in TILE_CASES])
+@pytest.mark.parametrize("tile", [str(k) for k in range(8)])
+def test_point_apply_every_tile_shape(dca, wl, name, make, tile):
+    """The point apply forced to each candidate tile shape (sides that no
+    shape divides, so edge tiles are partial) against the oracle's assembled
+    Jacobian; with "auto" the set-up autotuner reports one timed entry per
+    candidate and exactly one chosen shape per compartment."""
+    print(f"case: point apply {name} point_tile={tile}", flush=True)
+    w = make(wl)
+    gpu = dca.Model(w.ini, PTS + f"model.time_step_operator.linear_solver.point_tile = {tile}\n", 0)
+    orc = OracleModel(w.ini)
+    x = _perturbed_state(w, gpu)
+    v = np.random.default_rng(11).standard_normal(gpu.n_dof)
+    y = gpu.jacobian_apply(x, 0.3, 1.0, 0.4, v)
+    _, vals = orc.assemble(x, 0.3, 1.0, 0.4)
+    ref = orc.csr(vals) @ v
+    assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert gpu.point_tuning() == []  # a forced shape times nothing
+    if tile == "0":
+        auto = dca.Model(w.ini, PTS, 0)
+        tuning = auto.point_tuning()
+        comps = sorted({c for c, _, _, _ in tuning})
+        assert comps and all(sum(ch for c2, _, _, ch in tuning if c2 == c) == 1 for c in comps)
+        assert all(us > 0 for _, _, us, _ in tuning)
+        y2 = auto.jacobian_apply(x, 0.3, 1.0, 0.4, v)
+        assert np.abs(y2 - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
 def _step_pair(dca, w, overrides, steps, oracle_overrides=None):
     pts = dca.Model(w.ini, PTS + overrides, 0)
     mat = dca.Model(w.ini, overrides, 0)
@@ -209,11 +244,13 @@ def test_readback_batches_change_nothing_but_launches(dca, wl):
     launched past convergence return at once and are not counted as applies.
     Read-backs after every iteration (readback_iterations = 1) reach bit-equal
     states with equal iterations, and both count exactly two applies per
-    BiCGSTAB iteration."""
+    BiCGSTAB iteration.  (Both use the same tile shape: the shape sets the
+    order of the partial dot sums, so shapes timed independently could differ
+    in the last bits.)"""
     w = wl.mitchell_schaefer(n=10, cell_type="cube")
     runs = []
     for extra in ("", "model.time_step_operator.linear_solver.readback_iterations = 1\n"):
-        m = dca.Model(w.ini, PTS + extra, 0)
+        m = dca.Model(w.ini, PTS + extra + "model.time_step_operator.linear_solver.point_tile = 1\n", 0)
         m.set_state(w.initial_state(m), 0.0)
         m.profile_enable(True)
         t, lin, newton = 0.0, 0, 0
