@@ -637,23 +637,47 @@ int Model::tune_spmv_plan(bool fp32) {
 }
 
 // Times the point apply with every candidate tile shape on the model's own
-// grid (zero coefficients and vectors: the work does not depend on the
-// values; the candidates run in turn, several rounds, so caches favour none)
-// and keeps the fastest per compartment.
+// grid, as the Krylov solver launches it (with its two partial dots), and
+// keeps the fastest per compartment.  Coefficients and vectors are zero (the
+// work does not depend on the values).  Before each timed launch a copy
+// through a buffer larger than the chip's last-level cache evicts the
+// coefficients, as the solver's vector updates do between two applies.
 void Model::tune_points() {
   if (!cfg_.points || opt_.batch_member) return;
+  point_tuning_.clear();
   if (cfg_.point_tile >= 0) {
-    for (size_t c = 0; c < L_.comps.size(); ++c)
-      apply_shape_[c] = cfg_.point_tile < static_cast<int>(pc_app_fns_[c].size()) ? cfg_.point_tile : 0;
+    for (size_t c = 0; c < L_.comps.size(); ++c) {
+      const int nk = static_cast<int>(pc_app_fns_[c].size());
+      apply_shape_[c] = cfg_.point_tile < nk ? cfg_.point_tile : 0;
+      for (int k2 = 0; k2 < nk; ++k2) point_tuning_.push_back({static_cast<int>(c), gen_.apply_tiles[c][k2], 0.0});
+    }
     return;
   }
   bool any = false;
   for (const auto& f : pc_app_fns_) any = any || f.size() > 1;
   if (!any) return;
+  size_t pc_bytes = 0;
   for (auto& a : pc_)
-    if (a.size()) k::fill(a.get(), 0.0, static_cast<long>(a.size()), stream_);
-  k::fill(kv_[0].get(), 0.0, L_.nslot * L_.B, stream_);
+    if (a.size()) {
+      k::fill(a.get(), 0.0, static_cast<long>(a.size()), stream_);
+      pc_bytes += a.size() * sizeof(double);
+    }
+  for (int i = 0; i < 4; ++i) k::fill(kv_[i].get(), 0.0, L_.nslot * L_.B, stream_);
   k::fill(pc_tca_.get(), 0.0, 2, stream_);
+  // the eviction copy: only where the coefficients could stay cached
+  constexpr long kFlushBytes = 384L << 20;
+  DeviceArray<char> fa, fb;
+  if (pc_bytes > (8u << 20)) {
+    try {
+      fa.resize(kFlushBytes);
+      fb.resize(kFlushBytes);
+      DCG_HIP(hipMemsetAsync(fa.get(), 0, kFlushBytes, stream_));
+    } catch (const std::exception&) {
+      (void)hipGetLastError();  // no room: time without the eviction
+      fa = DeviceArray<char>();
+      fb = DeviceArray<char>();
+    }
+  }
   hipEvent_t e0, e1;
   DCG_HIP(hipEventCreate(&e0));
   DCG_HIP(hipEventCreate(&e1));
@@ -661,13 +685,13 @@ void Model::tune_points() {
     const size_t nk = pc_app_fns_[c].size();
     if (nk < 2) continue;
     std::vector<double> ms(nk, 0.0);
-    for (int round = 0; round < 4; ++round)
+    constexpr int kRounds = 4;
+    for (int round = 0; round < kRounds; ++round)
       for (size_t k2 = 0; k2 < nk; ++k2) {
         apply_shape_[c] = static_cast<int>(k2);
-        launch_points_apply(kv_[0].get(), kv_[1].get(), nullptr, false);
+        if (fa.size()) k::probe_copy(fa.get(), fb.get(), kFlushBytes, stream_);
         DCG_HIP(hipEventRecord(e0, stream_));
-        launch_points_apply(kv_[0].get(), kv_[1].get(), nullptr, false);
-        launch_points_apply(kv_[0].get(), kv_[1].get(), nullptr, false);
+        launch_points_apply(kv_[0].get(), kv_[1].get(), nullptr, false, kv_[2].get(), kv_[3].get(), part_.get());
         DCG_HIP(hipEventRecord(e1, stream_));
         DCG_HIP(hipEventSynchronize(e1));
         float t = 0.f;
@@ -676,8 +700,7 @@ void Model::tune_points() {
       }
     size_t best = 0;
     for (size_t k2 = 0; k2 < nk; ++k2) {
-      const auto& a = gen_.apply_tiles[c][k2];
-      point_tuning_.push_back({static_cast<int>(c), a, 1e3 * ms[k2] / 6.0});
+      point_tuning_.push_back({static_cast<int>(c), gen_.apply_tiles[c][k2], 1e3 * ms[k2] / (kRounds - 1)});
       if (ms[k2] < ms[best]) best = k2;
     }
     apply_shape_[c] = static_cast<int>(best);

@@ -171,7 +171,9 @@ int dcg_spmv_tuning(dcg_model_t model, int* count, int* variant, double* microse
  * (linear_solver.storage = points): for each candidate k < *count, its
  * compartment comp[k], its cell box tile[3k..3k+2] (cells per workgroup along
  * x, y, z) and the microseconds of one apply; chosen[k] = 1 for the shape in
- * use.  No candidates when the model has no point storage or a single shape.
+ * use.  No candidates when the model has no point storage or a single shape;
+ * a shape fixed by linear_solver.point_tile lists the candidates with zero
+ * times.
  * Arrays hold at least 64 entries (tile: 192). */
 int dcg_point_tuning(dcg_model_t model, int* count, int* comp, int* tile, double* microseconds, int* chosen);
 

@@ -100,7 +100,14 @@ def test_point_apply_every_tile_shape(dca, wl, name, make, tile):
     _, vals = orc.assemble(x, 0.3, 1.0, 0.4)
     ref = orc.csr(vals) @ v
     assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max()
-    assert gpu.point_tuning() == []  # a forced shape times nothing
+    # a forced shape times nothing and is the one in use in every compartment
+    forced = gpu.point_tuning()
+    assert forced and all(us == 0.0 for _, _, us, _ in forced)
+    for c in sorted({c for c, _, _, _ in forced}):
+        flags = [ch for c2, _, _, ch in forced if c2 == c]
+        assert flags.count(True) == 1
+        if int(tile) < len(flags):
+            assert flags[int(tile)]
     if tile == "0":
         auto = dca.Model(w.ini, PTS, 0)
         tuning = auto.point_tuning()
