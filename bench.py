@@ -90,20 +90,23 @@ def _host_threads():
 
 def _oracle_steps(o, w, steps):
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for k in range(steps):
         rc, _, _ = o.step(w.dt)
         if rc:
             raise RuntimeError(f"oracle step failed: {rc}")
+        print(f"# oracle step {k + 1}/{steps} done at {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     el = time.perf_counter() - t0
     return steps * w.stages * o.n_dof, el
 
 
-def cpu_baseline(wl_obj, steps: int, small=None):
+def cpu_baseline(wl_obj, steps: int, small=None, gpu_state_after=None):
     """Oracle on the host cores: `steps` time steps of the same workload with
     OpenMP-coloured assembly and a sequential BiCGSTAB (as istl's), then one
     more step with the Krylov solve threaded too (the stronger baseline), and
     one step of `small` (a smaller mesh of the same model, or the workload
-    itself) on one thread."""
+    itself) on one thread.  With gpu_state_after(k) (the device's state after
+    k steps of the workload from its initial state), the oracle's state after
+    its first 1 + `steps` steps is the parity check of the full-size run."""
     sys.path.insert(0, str(ROOT))
     from tests import oracle_lib  # the oracle is only ever the baseline here
 
@@ -115,6 +118,13 @@ def cpu_baseline(wl_obj, steps: int, small=None):
     o.set_state(wl_obj.initial_state(o), 0.0)
     _oracle_steps(o, wl_obj, 1)  # the first step (the stimulus transient) is a warm-up
     dof_steps, el = _oracle_steps(o, wl_obj, steps)
+    parity = None
+    if gpu_state_after is not None:
+        u_orc, _ = o.get_state()
+        u_gpu, info = gpu_state_after(1 + steps)
+        parity = {"steps": 1 + steps,
+                  "max_rel_l2_per_species": max(oracle_lib.rel_l2_per_species(info, u_gpu, u_orc)),
+                  "max_rel_mass_l2_per_species": max(oracle_lib.rel_mass_l2_per_species(o, u_gpu, u_orc))}
     L.orc_set_krylov_threads(1)
     t_dof, t_el = _oracle_steps(o, wl_obj, 1)
     L.orc_set_krylov_threads(0)
@@ -133,6 +143,7 @@ def cpu_baseline(wl_obj, steps: int, small=None):
             "threaded_krylov": {"value": t_dof / t_el, "cores": cores,
                                 "sample": f"the next time step with SpMV, dots and updates on all threads "
                                           f"({t_dof} Newton DOF-steps, {t_el:.1f} s)"},
+            "device_vs_oracle": parity,
             "one_thread": {"value": s_dof / s_el, "cores": 1,
                            "sample": f"one time step of {'the ' + str(o1.n_dof) + '-DOF mesh of the same model' if small else 'the same workload'} "
                                      f"({s_dof} Newton DOF-steps, {s_el:.1f} s)"}}
@@ -679,7 +690,19 @@ def main():
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         # one thread on the 48^3 mesh of the same model (a 96^3 step takes minutes on one core)
         small = make_workload(wl, "ms48", "fp64", args.preconditioner)[0] if args.workload == "ms96" else None
-        cpu = cpu_baseline(w, args.cpu_steps, small)
+
+        def gpu_state_after(k):
+            # the device's state after k steps from the workload's initial state
+            m.set_state(u0, 0.0)
+            tk = 0.0
+            for _ in range(k):
+                st = m.step(tk, w.dt)
+                if st.status:
+                    raise SystemExit(f"parity step failed with status {st.status}")
+                tk += w.dt
+            return m.get_state()[0], m.compartments()
+
+        cpu = cpu_baseline(w, args.cpu_steps, small, gpu_state_after)
     n_dof = m.n_dof
     m.close()
     batch_obj = serve_obj = None

@@ -560,6 +560,7 @@ void Model::setup_device() {
     ilu_sweep_ = widest <= kIluSweepWidth;
     lo_lev_.upload(lo_ptr_, stream_);
     up_lev_.upload(up_ptr_, stream_);
+    if (ilu_sweep_) build_ilu_sweep_tables(rows, rows_u);
     lu_.resize(static_cast<size_t>(matrix_values()));
   }
   {
@@ -1001,6 +1002,224 @@ Model::KrylovWork Model::own_work() {
   return w;
 }
 
+// Tables of the one-workgroup ILU0 sweeps from the level schedules (see
+// k::IluSweep): per row position its off-diagonal blocks and the LDS ring
+// slot of each block's column; the levels in chunks for the staged apply;
+// per lower block the factorisation's update pairs, in the order a merge of
+// the two sorted rows visits them; and the LDS sizes that decide which
+// kernels run (the plain sweeps where the stages do not fit).
+void Model::build_ilu_sweep_tables(const std::vector<int>& lrows, const std::vector<int>& urows) {
+  const long ns = L_.nslot;
+  const int B = L_.B;
+  const auto& rp = L_.row_ptr;
+  const auto& col = L_.col;
+  const auto& dp = L_.diag_pos;
+  k::IluSweep t;
+  t.nlo = static_cast<int>(lo_ptr_.size()) - 1;
+  t.nup = static_cast<int>(up_ptr_.size()) - 1;
+  t.lo_lev = lo_lev_.get();
+  t.lo_rows = lo_rows_.get();
+  t.up_lev = up_lev_.get();
+  t.up_rows = up_rows_.get();
+  int width = 1;
+  for (size_t l = 0; l + 1 < lo_ptr_.size(); ++l) width = std::max(width, lo_ptr_[l + 1] - lo_ptr_[l]);
+  for (size_t l = 0; l + 1 < up_ptr_.size(); ++l) width = std::max(width, up_ptr_[l + 1] - up_ptr_[l]);
+  // level and position of every row in each sweep's order
+  auto positions = [&](const std::vector<int>& ptr, const std::vector<int>& rows, std::vector<int>& lev,
+                       std::vector<int>& pos) {
+    lev.assign(ns, 0);
+    pos.assign(ns, 0);
+    for (size_t l = 0; l + 1 < ptr.size(); ++l)
+      for (int r = ptr[l]; r < ptr[l + 1]; ++r) {
+        lev[rows[r]] = static_cast<int>(l);
+        pos[rows[r]] = r;
+      }
+  };
+  std::vector<int> llev, lpos, ulev, upos;
+  positions(lo_ptr_, lrows, llev, lpos);
+  positions(up_ptr_, urows, ulev, upos);
+  int dist = 1, maxrow = 1, maxlower = 0;
+  for (long i = 0; i < ns; ++i) {
+    for (int p2 = rp[i]; p2 < dp[i]; ++p2) dist = std::max(dist, llev[i] - llev[col[p2]]);
+    for (int p2 = dp[i] + 1; p2 < rp[i + 1]; ++p2) dist = std::max(dist, ulev[i] - ulev[col[p2]]);
+    maxrow = std::max(maxrow, rp[i + 1] - rp[i]);
+    maxlower = std::max(maxlower, dp[i] - rp[i]);
+  }
+  std::vector<int> uptr(static_cast<size_t>(L_.nnzb()) + 1, 0);
+  std::vector<int2> mq;
+  int maxpairs = 0;
+  for (long i = 0; i < ns; ++i) {
+    const size_t row_first = mq.size();
+    for (int p2 = rp[i]; p2 < dp[i]; ++p2) {
+      const int k2 = col[p2];
+      int q = dp[k2] + 1, qe = rp[k2 + 1], m = p2 + 1, me = rp[i + 1];
+      while (q < qe && m < me) {
+        if (col[q] == col[m]) {
+          mq.push_back(make_int2(m, q));
+          ++q;
+          ++m;
+        } else if (col[q] < col[m]) {
+          ++q;
+        } else {
+          ++m;
+        }
+      }
+      uptr[p2 + 1] = static_cast<int>(mq.size());
+    }
+    for (int p2 = dp[i]; p2 < rp[i + 1]; ++p2) uptr[p2 + 1] = static_cast<int>(mq.size());
+    maxpairs = std::max(maxpairs, static_cast<int>(mq.size() - row_first));
+  }
+  if (mq.empty()) mq.push_back(make_int2(0, 0));  // a device array of at least one entry
+  // the staged apply: the largest chunk length whose stage and ring fit LDS
+  auto entries = [&](int ring, const std::vector<int>& ptr, const std::vector<int>& rows,
+                     const std::vector<int>& lev, const std::vector<int>& pos, bool lower, std::vector<int>& ent,
+                     std::vector<int>& blk, std::vector<int>& src) {
+    ent.assign(ns + 1, 0);
+    blk.clear();
+    src.clear();
+    for (long r = 0; r < ns; ++r) {
+      const int i = rows[r];
+      const int b0 = lower ? rp[i] : dp[i] + 1, e0 = lower ? dp[i] : rp[i + 1];
+      for (int p2 = b0; p2 < e0; ++p2) {
+        const int k2 = col[p2];
+        blk.push_back(p2);
+        src.push_back((lev[k2] % ring) * width + (pos[k2] - ptr[lev[k2]]));
+      }
+      ent[r + 1] = static_cast<int>(blk.size());
+    }
+  };
+  std::vector<int> lent, lblk, lsrc, uent, ublk, usrc;
+  entries(1, lo_ptr_, lrows, llev, lpos, true, lent, lblk, lsrc);  // entry offsets (slots redone below)
+  entries(1, up_ptr_, urows, ulev, upos, false, uent, ublk, usrc);
+  auto chunks = [&](int clen, const std::vector<int>& ptr, const std::vector<int>& ent, std::vector<int>& ch,
+                    int& me, int& mr) {
+    const int nl = static_cast<int>(ptr.size()) - 1;
+    ch.clear();
+    for (int l = 0; l < nl; l += clen) ch.push_back(l);
+    ch.push_back(nl);
+    for (size_t c = 0; c + 1 < ch.size(); ++c) {
+      const int r0 = ptr[ch[c]], r1 = ptr[ch[c + 1]];
+      mr = std::max(mr, r1 - r0);
+      me = std::max(me, ent[r1] - ent[r0]);
+    }
+  };
+  std::vector<int> lch, uch;
+  t.chunked = false;
+  t.width = width;
+  // the apply's stage may use all of a workgroup's LDS (160 KiB on gfx950)
+  size_t apply_lds = kIluLdsBytes;
+  {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device_) == hipSuccess && v > 0)
+      apply_lds = std::max(apply_lds, std::min<size_t>(static_cast<size_t>(v), 160 * 1024));
+  }
+  for (int clen = 32; clen >= 1 && !t.chunked; --clen) {
+    int me = 0, mr = 0;
+    chunks(clen, lo_ptr_, lent, lch, me, mr);
+    chunks(clen, up_ptr_, uent, uch, me, mr);
+    k::IluSweep c2 = t;
+    c2.stage_ent = std::max(me, 1);
+    c2.stage_rows = std::max(mr, 1);
+    c2.stage_levels = clen;
+    c2.ring = std::max(dist + 1, clen);
+    if (k::ilu_chunk_lds_bytes(c2, B, static_cast<int>(sizeof(double))) <= apply_lds) {
+      t = c2;
+      t.chunked = true;
+    }
+  }
+  if (t.chunked) {
+    entries(t.ring, lo_ptr_, lrows, llev, lpos, true, lent, lblk, lsrc);
+    entries(t.ring, up_ptr_, urows, ulev, upos, false, uent, ublk, usrc);
+    // replay both sweeps on the host: every ring read must find its
+    // column's row, written at an earlier level and not yet overwritten
+    std::vector<int> held(static_cast<size_t>(t.ring) * width, -1);
+    auto replay = [&](const std::vector<int>& ptr, const std::vector<int>& rows, const std::vector<int>& ent,
+                      const std::vector<int>& blk, const std::vector<int>& src) {
+      std::fill(held.begin(), held.end(), -1);
+      for (size_t l = 0; l + 1 < ptr.size(); ++l) {
+        for (int r = ptr[l]; r < ptr[l + 1]; ++r)
+          for (int e = ent[r]; e < ent[r + 1]; ++e)
+            if (src[e] < 0 || src[e] >= t.ring * width || held[src[e]] != col[blk[e]])
+              throw Error("internal: ILU0 sweep ring table");
+        for (int r = ptr[l]; r < ptr[l + 1]; ++r)
+          held[(static_cast<int>(l) % t.ring) * width + (r - ptr[l])] = rows[r];
+      }
+    };
+    replay(lo_ptr_, lrows, lent, lblk, lsrc);
+    replay(up_ptr_, urows, uent, ublk, usrc);
+    lo_ent_.upload(lent, stream_);
+    lo_blk_.upload(lblk.empty() ? std::vector<int>{0} : lblk, stream_);
+    lo_src_.upload(lsrc.empty() ? std::vector<int>{0} : lsrc, stream_);
+    up_ent_.upload(uent, stream_);
+    up_blk_.upload(ublk.empty() ? std::vector<int>{0} : ublk, stream_);
+    up_src_.upload(usrc.empty() ? std::vector<int>{0} : usrc, stream_);
+    lo_chunk_.upload(lch, stream_);
+    up_chunk_.upload(uch, stream_);
+    t.nlo_chunk = static_cast<int>(lch.size()) - 1;
+    t.nup_chunk = static_cast<int>(uch.size()) - 1;
+    t.lo_chunk = lo_chunk_.get();
+    t.up_chunk = up_chunk_.get();
+    t.lo_ent = lo_ent_.get();
+    t.lo_blk = lo_blk_.get();
+    t.lo_src = lo_src_.get();
+    t.up_ent = up_ent_.get();
+    t.up_blk = up_blk_.get();
+    t.up_src = up_src_.get();
+  }
+  // the inbox factorisation: small blocks and rows; every operand of a row
+  // at a position fixed by the row's place in level order (k::IluSweep)
+  if (B <= 2 && maxlower <= k::kIluRegLower && maxpairs <= k::kIluRegPairs && maxrow <= k::kIluRegRow) {
+    constexpr int ML = k::kIluRegLower, MU = k::kIluRegPairs, MP = k::kIluRegPush, IN = ML + MU;
+    const int BB = B * B;
+    const size_t N = static_cast<size_t>(ns);
+    std::vector<int4> frow(ns);
+    std::vector<int> fub(N * (ML + 1), 0), fm(N * MU, 0);
+    std::vector<std::vector<int2>> pushes(ns);  // by pushing row
+    for (long r = 0; r < ns; ++r) {
+      const int i = lrows[r];
+      const int p0 = rp[i], nl = dp[i] - rp[i], u0 = uptr[p0], nu = uptr[dp[i]] - u0;
+      frow[r] = make_int4(i, p0, rp[i + 1] - p0, nl);
+      for (int a = 0; a <= ML; ++a) fub[r * (ML + 1) + a] = a <= nl ? uptr[p0 + a] - u0 : nu;
+      for (int a = 0; a < nl; ++a) {
+        const int k2 = col[p0 + a];
+        pushes[k2].push_back(make_int2(-1, static_cast<int>((r * IN + a) * BB)));
+        for (int u = uptr[p0 + a] - u0; u < uptr[p0 + a + 1] - u0; ++u) {
+          const int2 pq = mq[u0 + u];
+          fm[r * MU + u] = pq.x - p0;
+          pushes[k2].push_back(make_int2(pq.y - rp[k2], static_cast<int>((r * IN + ML + u) * BB)));
+        }
+      }
+    }
+    size_t maxpush = 0;
+    for (const auto& v : pushes) maxpush = std::max(maxpush, v.size());
+    const size_t rowbytes = static_cast<size_t>(k::kIluRegRow) * BB * sizeof(double);
+    int nt = std::min(256, (width + 63) / 64 * 64);
+    while (nt >= 64 && nt * rowbytes > kIluLdsBytes) nt -= 64;
+    const size_t inbox_doubles = N * IN * BB;
+    if (maxpush <= static_cast<size_t>(MP) && nt >= 64 && inbox_doubles < (1u << 31)) {
+      std::vector<int2> fpush(N * MP, make_int2(0, -1));
+      for (long i = 0; i < ns; ++i)
+        for (size_t j = 0; j < pushes[i].size(); ++j) fpush[static_cast<size_t>(lpos[i]) * MP + j] = pushes[i][j];
+      ilu_frow_.upload(frow, stream_);
+      ilu_fub_.upload(fub, stream_);
+      ilu_fm_.upload(fm, stream_);
+      ilu_fpush_.upload(fpush, stream_);
+      ilu_packed_.resize(N * k::kIluRegRow * BB);
+      ilu_inbox_.resize(inbox_doubles);
+      t.inbox_path = true;
+      t.factor_threads = nt;
+      t.nrows = static_cast<int>(ns);
+      t.frow = ilu_frow_.get();
+      t.fub = ilu_fub_.get();
+      t.fm = ilu_fm_.get();
+      t.fpush = ilu_fpush_.get();
+      t.packed = ilu_packed_.get();
+      t.inbox = ilu_inbox_.get();
+    }
+  }
+  ilu_tab_ = t;
+}
+
 void Model::precond_setup(const KrylovWork& w) {
   const int B = L_.B;
   const int ns = static_cast<int>(L_.nslot);
@@ -1012,8 +1231,7 @@ void Model::precond_setup(const KrylovWork& w) {
   } else if (cfg_.precond == Preconditioner::ILU0) {
     DCG_HIP(hipMemcpyAsync(w.lu, w.val, matrix_values() * sizeof(double), hipMemcpyDeviceToDevice, stream_));
     if (ilu_sweep_) {
-      k::ilu_factor_sweep(B, static_cast<int>(lo_ptr_.size()) - 1, lo_lev_.get(), lo_rows_.get(), row_ptr_.get(),
-                          col_.get(), diag_pos_.get(), w.lu, w.dinv, w.status, stream_);
+      k::ilu_factor_sweep(B, ilu_tab_, row_ptr_.get(), col_.get(), diag_pos_.get(), w.lu, w.dinv, w.status, stream_);
       return;
     }
     for (size_t l = 0; l + 1 < lo_ptr_.size(); ++l)
@@ -1043,9 +1261,8 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
   const bool ilu = cfg_.precond == Preconditioner::ILU0;
   auto apply_ilu = [&](const double* in, double* out) {
     if (ilu_sweep_) {
-      k::ilu_apply_sweep<double>(B, static_cast<int>(lo_ptr_.size()) - 1, lo_lev_.get(), lo_rows_.get(),
-                                 static_cast<int>(up_ptr_.size()) - 1, up_lev_.get(), up_rows_.get(), row_ptr_.get(),
-                                 col_.get(), diag_pos_.get(), w.lu, w.dinv, in, tmp, out, sc, stream_);
+      k::ilu_apply_sweep<double>(B, ilu_tab_, row_ptr_.get(), col_.get(), diag_pos_.get(), w.lu, w.dinv, in, tmp,
+                                 out, sc, stream_);
       return;
     }
     for (size_t l = 0; l + 1 < lo_ptr_.size(); ++l)
@@ -1140,9 +1357,8 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
   const bool ilu = cfg_.precond == Preconditioner::ILU0;
   auto apply_ilu = [&](const float* in, float* out) {
     if (ilu_sweep_) {
-      k::ilu_apply_sweep<float>(B, static_cast<int>(lo_ptr_.size()) - 1, lo_lev_.get(), lo_rows_.get(),
-                                static_cast<int>(up_ptr_.size()) - 1, up_lev_.get(), up_rows_.get(), row_ptr_.get(),
-                                col_.get(), diag_pos_.get(), lu_.get(), dinv_.get(), in, tmp, out, sc_.get(), stream_);
+      k::ilu_apply_sweep<float>(B, ilu_tab_, row_ptr_.get(), col_.get(), diag_pos_.get(), lu_.get(), dinv_.get(), in,
+                                tmp, out, sc_.get(), stream_);
       return;
     }
     for (size_t l = 0; l + 1 < lo_ptr_.size(); ++l)
@@ -1747,7 +1963,10 @@ size_t Model::device_bytes() const {
              prm_.bytes() + u_.bytes() + hist_.bytes() + r_.bytes() + z_.bytes() + xs_.bytes() + packed_.bytes() +
              val_.bytes() + lu_.bytes() + dinv_.bytes() + val32_.bytes() + part_.bytes() + scalar_out_.bytes() +
              sc_.bytes() + status_.bytes() + lo_rows_.bytes() + up_rows_.bytes() + pdiag_.bytes() + diag_id_.bytes() +
-             lin_x_.bytes();
+             lin_x_.bytes() + lo_ent_.bytes() + lo_blk_.bytes() + lo_src_.bytes() + up_ent_.bytes() + up_blk_.bytes() +
+             up_src_.bytes() + upd_ptr_.bytes() + upd_mq_.bytes() + lo_chunk_.bytes() + up_chunk_.bytes() +
+             ilu_frow_.bytes() + ilu_fub_.bytes() + ilu_fm_.bytes() + ilu_fpush_.bytes() + ilu_packed_.bytes() +
+             ilu_inbox_.bytes();
   for (const auto& v : pc_) b += v.bytes();
   for (const auto& v : pc32_) b += v.bytes();
   for (const auto& v : stage_) b += v.bytes();

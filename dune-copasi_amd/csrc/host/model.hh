@@ -373,6 +373,16 @@ class Model {
   // over all levels instead of one launch per level
   bool ilu_sweep_ = false;
   static constexpr int kIluSweepWidth = 4096;
+  // the one-workgroup sweeps' tables (entry lists in level order, LDS ring
+  // slots, the factorisation's update pairs) and their device view
+  DeviceArray<int> lo_ent_, lo_blk_, lo_src_, up_ent_, up_blk_, up_src_, upd_ptr_, lo_chunk_, up_chunk_;
+  DeviceArray<int2> upd_mq_, ilu_fpush_;
+  DeviceArray<int4> ilu_frow_;
+  DeviceArray<int> ilu_fub_, ilu_fm_;
+  DeviceArray<double> ilu_packed_, ilu_inbox_;
+  k::IluSweep ilu_tab_;
+  static constexpr size_t kIluLdsBytes = 64 * 1024;  // dynamic LDS of the sweeps
+  void build_ilu_sweep_tables(const std::vector<int>& lrows, const std::vector<int>& urows);
   bool ilu_ready_ = false;
   std::vector<std::pair<int, double>> tuning_;
   // profiling

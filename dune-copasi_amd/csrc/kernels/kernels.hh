@@ -237,14 +237,50 @@ double* noalloc_scratch(int device);
 
 void ilu_factor(int B, int nrows, const int* rows, const int* rp, const int* col, const int* diag_pos, double* lu,
                 double* dinv, int* status, hipStream_t s);
-// every level of the factorisation / of both triangular solves in one
-// workgroup (lev: level boundaries into rows, as the per-level launches use)
-void ilu_factor_sweep(int B, int nlev, const int* lev, const int* rows, const int* rp, const int* col,
-                      const int* diag_pos, double* lu, double* dinv, int* status, hipStream_t s);
+// Tables of the single-workgroup ILU0 sweeps (every level in one workgroup,
+// barriers between levels), built at set-up from the level schedules:
+//   lo_lev / up_lev: level boundaries into lo_rows / up_rows (row positions);
+//   lo_ent[r] .. lo_ent[r+1]: the strictly lower blocks of the row at
+//   position r: block lo_blk[e] of the matrix and the LDS ring slot lo_src[e]
+//   of its column's result; up_*: the same for the strictly upper blocks;
+//   lo_chunk / up_chunk: the levels in chunks whose blocks and rows fit the
+//   stage (at most stage_ent blocks, stage_rows rows, stage_levels levels);
+//   ring, width: ring levels and the widest level (chunked: the apply runs
+//   ilu_apply_chunk_k, else the plain one-workgroup sweep);
+//   upd_ptr[p] .. upd_ptr[p+1]: for lower block p of row i (column k), the
+//   factorisation's updates (m, q) = (block of row i, block of row k's upper
+//   part) with equal columns, in column order;
+//   inbox (blocks of at most 2 species and rows within the kIluReg* bounds):
+//   the factorisation reads each row's blocks from `packed` and its
+//   operands from its `inbox` slots, both at fixed positions in level order,
+//   with the tables frow (row, first block, blocks, lower blocks), fub, fm
+//   and fpush (see ilu_factor_inbox_k); factor_threads threads.
+constexpr int kIluRegLower = 4, kIluRegPairs = 16, kIluRegRow = 9, kIluRegPush = 24;
+struct IluSweep {
+  int nlo = 0, nup = 0;
+  const int *lo_lev = nullptr, *lo_rows = nullptr, *up_lev = nullptr, *up_rows = nullptr;
+  bool chunked = false;
+  int nlo_chunk = 0, nup_chunk = 0, stage_ent = 0, stage_rows = 0, stage_levels = 0, ring = 0, width = 0;
+  const int *lo_chunk = nullptr, *lo_ent = nullptr, *lo_blk = nullptr, *lo_src = nullptr;
+  const int *up_chunk = nullptr, *up_ent = nullptr, *up_blk = nullptr, *up_src = nullptr;
+  int factor_threads = 0, maxrow = 0;
+  const int* upd_ptr = nullptr;
+  const int2* upd_mq = nullptr;
+  bool inbox_path = false;
+  int nrows = 0;
+  const int4* frow = nullptr;
+  const int *fub = nullptr, *fm = nullptr;
+  const int2* fpush = nullptr;
+  double *packed = nullptr, *inbox = nullptr;
+};
+// every level of the factorisation / of both triangular solves in one workgroup
+void ilu_factor_sweep(int B, const IluSweep& t, const int* rp, const int* col, const int* diag_pos, double* lu,
+                      double* dinv, int* status, hipStream_t s);
 template <typename V>
-void ilu_apply_sweep(int B, int nlo, const int* lo_lev, const int* lo_rows, int nup, const int* up_lev,
-                     const int* up_rows, const int* rp, const int* col, const int* diag_pos, const double* lu,
+void ilu_apply_sweep(int B, const IluSweep& t, const int* rp, const int* col, const int* diag_pos, const double* lu,
                      const double* dinv, const V* in, V* tmp, V* out, const BicgScalars* sc, hipStream_t s);
+// LDS bytes of the chunked apply's stage and ring for values of vbytes bytes
+size_t ilu_chunk_lds_bytes(const IluSweep& t, int B, int vbytes);
 template <typename V>
 void ilu_lower(int B, int nrows, const int* rows, const int* rp, const int* col, const int* diag_pos,
                const double* lu, const V* b, V* y, const BicgScalars* sc, hipStream_t s);

@@ -14,6 +14,7 @@
 
 #include "kernels.hh"
 #include "common.hh"
+#include "../host/error.hh"
 
 namespace dcopasi::k {
 
@@ -394,47 +395,11 @@ __device__ __forceinline__ void mat_mul_sub(double* __restrict__ c, const double
     }
 }
 
-// Factor the rows of one level: L_ik <- A_ik U_kk^-1, A_ij -= L_ik U_kj,
-// U_ii^-1 stored in dinv.  lu holds a copy of the matrix values.
+// dinv_i = (the updated diagonal block of row i)^-1, Gauss-Jordan with
+// partial pivoting; a zero pivot sets status 4
 template <int B>
-__device__ __forceinline__ void ilu_factor_row(int i, const int* __restrict__ rp, const int* __restrict__ col,
-                                               const int* __restrict__ diag_pos, double* __restrict__ lu,
-                                               double* __restrict__ dinv, int* __restrict__ status) {
-  for (int p = rp[i]; p < diag_pos[i]; ++p) {
-    const int k = col[p];
-    double* lik = lu + (size_t)p * B * B;
-    double tmp[B * B];
-    const double* dk = dinv + (size_t)k * B * B;
-#pragma unroll
-    for (int a = 0; a < B; ++a)
-#pragma unroll
-      for (int b = 0; b < B; ++b) {
-        double s = 0.0;
-#pragma unroll
-        for (int c = 0; c < B; ++c) s += lik[a * B + c] * dk[c * B + b];
-        tmp[a * B + b] = s;
-      }
-#pragma unroll
-    for (int a = 0; a < B * B; ++a) lik[a] = tmp[a];
-    // row i entries right of column k against row k's upper part
-    int q = diag_pos[k] + 1, qe = rp[k + 1];
-    int m = p + 1, me = rp[i + 1];
-    while (q < qe && m < me) {
-      const int cq = col[q], cm = col[m];
-      if (cq == cm) {
-        mat_mul_sub<B>(lu + (size_t)m * B * B, lik, lu + (size_t)q * B * B);
-        ++q;
-        ++m;
-      } else if (cq < cm) {
-        ++q;
-      } else {
-        ++m;
-      }
-    }
-  }
-  // invert the (updated) diagonal block
+__device__ __forceinline__ void ilu_invert_block(const double* d, double* __restrict__ o, int* __restrict__ status) {
   double a[B][B], inv[B][B];
-  const double* d = lu + (size_t)diag_pos[i] * B * B;
 #pragma unroll
   for (int r = 0; r < B; ++r)
 #pragma unroll
@@ -477,11 +442,57 @@ __device__ __forceinline__ void ilu_factor_row(int i, const int* __restrict__ rp
       }
     }
   }
-  double* o = dinv + (size_t)i * B * B;
 #pragma unroll
   for (int r = 0; r < B; ++r)
 #pragma unroll
     for (int c = 0; c < B; ++c) o[r * B + c] = inv[r][c];
+}
+
+template <int B>
+__device__ __forceinline__ void ilu_invert_diag(int i, const int* __restrict__ diag_pos, const double* __restrict__ lu,
+                                                double* __restrict__ dinv, int* __restrict__ status) {
+  ilu_invert_block<B>(lu + (size_t)diag_pos[i] * B * B, dinv + (size_t)i * B * B, status);
+}
+
+// Factor the rows of one level: L_ik <- A_ik U_kk^-1, A_ij -= L_ik U_kj,
+// U_ii^-1 stored in dinv.  lu holds a copy of the matrix values.
+template <int B>
+__device__ __forceinline__ void ilu_factor_row(int i, const int* __restrict__ rp, const int* __restrict__ col,
+                                               const int* __restrict__ diag_pos, double* __restrict__ lu,
+                                               double* __restrict__ dinv, int* __restrict__ status) {
+  for (int p = rp[i]; p < diag_pos[i]; ++p) {
+    const int k = col[p];
+    double* lik = lu + (size_t)p * B * B;
+    double tmp[B * B];
+    const double* dk = dinv + (size_t)k * B * B;
+#pragma unroll
+    for (int a = 0; a < B; ++a)
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < B; ++c) s += lik[a * B + c] * dk[c * B + b];
+        tmp[a * B + b] = s;
+      }
+#pragma unroll
+    for (int a = 0; a < B * B; ++a) lik[a] = tmp[a];
+    // row i entries right of column k against row k's upper part
+    int q = diag_pos[k] + 1, qe = rp[k + 1];
+    int m = p + 1, me = rp[i + 1];
+    while (q < qe && m < me) {
+      const int cq = col[q], cm = col[m];
+      if (cq == cm) {
+        mat_mul_sub<B>(lu + (size_t)m * B * B, lik, lu + (size_t)q * B * B);
+        ++q;
+        ++m;
+      } else if (cq < cm) {
+        ++q;
+      } else {
+        ++m;
+      }
+    }
+  }
+  ilu_invert_diag<B>(i, diag_pos, lu, dinv, status);
 }
 
 template <int B>
@@ -507,6 +518,102 @@ __global__ void __launch_bounds__(256) ilu_factor_sweep_k(int nlev, const int* _
     const int e = lev[l + 1];
     for (int t = lev[l] + (int)threadIdx.x; t < e; t += blockDim.x)
       ilu_factor_row<B>(rows[t], rp, col, diag_pos, lu, dinv, status);
+    __syncthreads();
+  }
+}
+
+// The same sweep for rows of small blocks (B <= 2) with at most
+// kIluRegLower lower blocks, kIluRegPairs update pairs, kIluRegRow blocks
+// and kIluRegPush pushes, every operand at an address fixed by the row's
+// position r in level order:
+//   packed[r]: the row's blocks (gathered by ilu_pack_k before the sweep);
+//   inbox[r]: slot a < nl: dinv_k of lower block a; slot kIluRegLower + u:
+//   U_kq of update pair u, written there ("pushed") by row k when it was
+//   factored;
+//   fub[r], fm[r]: the pair ranges of the lower blocks and the pairs' row
+//   blocks m (local index); fpush[r]: (source, destination) of this row's
+//   pushes (source -1: its dinv, else one of its blocks).
+// All of a row's loads are issued together and it computes in LDS (its
+// row) and registers (its operands): no load waits for a store of the row.
+// The arithmetic is that of ilu_factor_row: lower blocks in column order,
+// each followed by its pairs in merge order.
+template <int B>
+__global__ void ilu_pack_k(int ns, IluSweep t, const double* __restrict__ val, double* __restrict__ packed) {
+  constexpr int BB = B * B;
+  const long g = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long per = (long)kIluRegRow * BB;
+  if (g >= (long)ns * per) return;
+  const int r = (int)(g / per), b = (int)(g % per);
+  const int4 f = t.frow[r];
+  if (b < f.z * BB) packed[g] = val[(size_t)f.y * BB + b];
+}
+
+extern __shared__ double ilu_row_lds[];
+template <int B>
+__global__ void __launch_bounds__(256) ilu_factor_inbox_k(IluSweep t, const double* __restrict__ packed,
+                                                          double* __restrict__ inbox, double* __restrict__ lu,
+                                                          double* __restrict__ dinv, int* __restrict__ status) {
+  constexpr int BB = B * B, ML = kIluRegLower, MU = kIluRegPairs, MR = kIluRegRow, MP = kIluRegPush;
+  constexpr int IN = ML + MU;
+  double* my = ilu_row_lds + (size_t)threadIdx.x * MR * BB;  // the thread's row
+  for (int l = 0; l < t.nlo; ++l) {
+    const int e = t.lo_lev[l + 1];
+    for (int r = t.lo_lev[l] + (int)threadIdx.x; r < e; r += blockDim.x) {
+      const int4 f = t.frow[r];  // row, first block, blocks, lower blocks
+      const int nl = f.w;
+      // every load of the row first: its blocks, its inbox, its tables
+      double rv[MR * BB], in[IN][BB];
+      int ub[ML + 1], fm[MU];
+      int2 push[MP];
+#pragma unroll
+      for (int j = 0; j < MR * BB; ++j) rv[j] = j < f.z * BB ? packed[(size_t)r * MR * BB + j] : 0.0;
+#pragma unroll
+      for (int j = 0; j < IN; ++j)
+#pragma unroll
+        for (int c = 0; c < BB; ++c) in[j][c] = inbox[((size_t)r * IN + j) * BB + c];
+#pragma unroll
+      for (int a = 0; a <= ML; ++a) ub[a] = t.fub[(size_t)r * (ML + 1) + a];
+#pragma unroll
+      for (int u = 0; u < MU; ++u) fm[u] = t.fm[(size_t)r * MU + u];
+#pragma unroll
+      for (int j = 0; j < MP; ++j) push[j] = t.fpush[(size_t)r * MP + j];
+#pragma unroll
+      for (int j = 0; j < MR * BB; ++j) my[j] = rv[j];
+#pragma unroll
+      for (int a = 0; a < ML; ++a) {
+        if (a < nl) {
+          double* lik = my + a * BB;
+          double tmp[BB];
+#pragma unroll
+          for (int x = 0; x < B; ++x)
+#pragma unroll
+            for (int y = 0; y < B; ++y) {
+              double sum = 0.0;
+#pragma unroll
+              for (int c = 0; c < B; ++c) sum += lik[x * B + c] * in[a][c * B + y];
+              tmp[x * B + y] = sum;
+            }
+#pragma unroll
+          for (int c = 0; c < BB; ++c) lik[c] = tmp[c];
+#pragma unroll
+          for (int u = 0; u < MU; ++u)
+            if (u >= ub[a] && u < ub[a + 1]) mat_mul_sub<B>(my + fm[u] * BB, tmp, in[ML + u]);
+        }
+      }
+      double inv[BB];
+      ilu_invert_block<B>(my + nl * BB, inv, status);
+      // stores: the row's blocks, its inverse, its pushes, the apply's copies
+      for (int j = 0; j < f.z * BB; ++j) lu[(size_t)f.y * BB + j] = my[j];
+#pragma unroll
+      for (int c = 0; c < BB; ++c) dinv[(size_t)f.x * BB + c] = inv[c];
+#pragma unroll
+      for (int k = 0; k < MP; ++k)
+        if (push[k].y >= 0) {
+          const double* v = push[k].x < 0 ? inv : my + push[k].x * BB;
+#pragma unroll
+          for (int c = 0; c < BB; ++c) inbox[(size_t)push[k].y + c] = v[c];
+        }
+    }
     __syncthreads();
   }
 }
@@ -606,6 +713,173 @@ __global__ void __launch_bounds__(256) ilu_apply_sweep_k(int nlo, const int* __r
       ilu_upper_row<B, V>(up_rows[t], rp, col, diag_pos, lu, dinv, tmp, out);
     __syncthreads();
   }
+}
+
+// One sweep of ilu_apply_chunk_k: the levels in chunks (chunk[c] ..
+// chunk[c+1]).  A chunk's blocks, source slots, row ids, entry offsets,
+// right-hand sides and (backward sweep) diagonal inverses are first gathered
+// into LDS with every load in flight at once; its levels then run from LDS
+// alone, each row's result going to the ring slot (l mod ring) * width + r -
+// lev[l]; at the chunk's end the results are written to `res`.  The sums
+// run in the blocks' column order, as ilu_lower_row / ilu_upper_row sum them.
+// Byte offsets of the chunk stage in LDS (kernel and launcher): blocks,
+// diagonal inverses, right-hand sides (doubles), then source slots, entry
+// offsets, row ids and level bounds (ints), then the ring (8-byte aligned).
+struct IluChunkLayout {
+  size_t d, in, src, ent, row, lev, ring, total;
+  __host__ __device__ IluChunkLayout(const IluSweep& t, int B, int vbytes) {
+    const size_t BB = (size_t)B * B;
+    d = 8 * (size_t)t.stage_ent * BB;
+    in = d + 8 * (size_t)t.stage_rows * BB;
+    src = in + 8 * (size_t)t.stage_rows * B;
+    ent = src + 4 * (size_t)t.stage_ent;
+    row = ent + 4 * ((size_t)t.stage_rows + 1);
+    lev = row + 4 * (size_t)t.stage_rows;
+    ring = (lev + 4 * ((size_t)t.stage_levels + 1) + 7) & ~(size_t)7;
+    total = ring + (size_t)t.ring * t.width * B * vbytes;
+  }
+};
+
+template <int B, typename V, bool UPPER>
+__device__ __forceinline__ void ilu_chunk_sweep(const IluSweep& t, int nchunk, const int* __restrict__ chunk,
+                                                const int* __restrict__ lev, const int* __restrict__ rows,
+                                                const int* __restrict__ ent, const int* __restrict__ blk,
+                                                const int* __restrict__ src, const double* __restrict__ lu,
+                                                const double* __restrict__ dinv, const V* __restrict__ rhs,
+                                                V* __restrict__ res, char* lds) {
+  constexpr int BB = B * B;
+  const IluChunkLayout lay(t, B, (int)sizeof(V));
+  double* sL = reinterpret_cast<double*>(lds);                  // stage_ent * BB
+  double* sD = reinterpret_cast<double*>(lds + lay.d);          // stage_rows * BB (backward sweep)
+  double* sIn = reinterpret_cast<double*>(lds + lay.in);        // stage_rows * B
+  int* sSrc = reinterpret_cast<int*>(lds + lay.src);            // stage_ent
+  int* sEnt = reinterpret_cast<int*>(lds + lay.ent);            // stage_rows + 1
+  int* sRow = reinterpret_cast<int*>(lds + lay.row);            // stage_rows
+  int* sLev = reinterpret_cast<int*>(lds + lay.lev);            // stage_levels + 1
+  V* ring = reinterpret_cast<V*>(lds + lay.ring);               // ring * width * B
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int c = 0; c < nchunk; ++c) {
+    const int L0 = chunk[c], L1 = chunk[c + 1];
+    const int r0 = lev[L0], r1 = lev[L1];
+    const int e0 = ent[r0], e1 = ent[r1];
+    // the stage: KE entries (KR rows) per thread per round, every round's
+    // index loads, then its value loads, issued together before any store
+    constexpr int KE = BB <= 4 ? 8 : BB <= 16 ? 4 : 1, KR = 4;
+    for (int l = L0 + tid; l <= L1; l += nt) sLev[l - L0] = lev[l];
+    for (int r = r0 + tid; r <= r1; r += nt) sEnt[r - r0] = ent[r] - e0;
+    for (int rb = r0; rb < r1; rb += nt * KR) {
+      int ri[KR];
+#pragma unroll
+      for (int k = 0; k < KR; ++k) {
+        const int r = rb + tid + k * nt;
+        ri[k] = r < r1 ? rows[r] : -1;
+      }
+      double vin[KR][B], vd[KR][UPPER ? BB : 1];
+#pragma unroll
+      for (int k = 0; k < KR; ++k)
+        if (ri[k] >= 0) {
+#pragma unroll
+          for (int a = 0; a < B; ++a) vin[k][a] = (double)rhs[(size_t)ri[k] * B + a];
+          if constexpr (UPPER)
+#pragma unroll
+            for (int a = 0; a < BB; ++a) vd[k][a] = dinv[(size_t)ri[k] * BB + a];
+        }
+#pragma unroll
+      for (int k = 0; k < KR; ++k)
+        if (ri[k] >= 0) {
+          const int rr = rb + tid + k * nt - r0;
+          sRow[rr] = ri[k];
+#pragma unroll
+          for (int a = 0; a < B; ++a) sIn[rr * B + a] = vin[k][a];
+          if constexpr (UPPER)
+#pragma unroll
+            for (int a = 0; a < BB; ++a) sD[rr * BB + a] = vd[k][a];
+        }
+    }
+    for (int eb = e0; eb < e1; eb += nt * KE) {
+      int bi[KE], si[KE];
+#pragma unroll
+      for (int k = 0; k < KE; ++k) {
+        const int e = eb + tid + k * nt;
+        bi[k] = e < e1 ? blk[e] : -1;
+        si[k] = e < e1 ? src[e] : 0;
+      }
+      double v[KE][BB];
+#pragma unroll
+      for (int k = 0; k < KE; ++k)
+        if (bi[k] >= 0)
+#pragma unroll
+          for (int a = 0; a < BB; ++a) v[k][a] = lu[(size_t)bi[k] * BB + a];
+#pragma unroll
+      for (int k = 0; k < KE; ++k)
+        if (bi[k] >= 0) {
+          const int ee = eb + tid + k * nt - e0;
+          sSrc[ee] = si[k];
+#pragma unroll
+          for (int a = 0; a < BB; ++a) sL[(size_t)ee * BB + a] = v[k][a];
+        }
+    }
+    __syncthreads();
+    for (int l = L0; l < L1; ++l) {
+      const int b0 = sLev[l - L0], b1 = sLev[l + 1 - L0];
+      const int base = (l % t.ring) * t.width - b0;
+      for (int r = b0 + tid; r < b1; r += nt) {
+        const int rr = r - r0;
+        double sum[B];
+#pragma unroll
+        for (int a = 0; a < B; ++a) sum[a] = sIn[rr * B + a];
+        const int ee = sEnt[rr + 1];
+        for (int e = sEnt[rr]; e < ee; ++e) {
+          const double* L = sL + (size_t)e * BB;
+          const V* yk = ring + (size_t)sSrc[e] * B;
+#pragma unroll
+          for (int a = 0; a < B; ++a)
+#pragma unroll
+            for (int cc = 0; cc < B; ++cc) sum[a] -= L[a * B + cc] * (double)yk[cc];
+        }
+        V* o = ring + (size_t)(base + r) * B;
+        if constexpr (UPPER) {
+          const double* d = sD + (size_t)rr * BB;
+#pragma unroll
+          for (int a = 0; a < B; ++a) {
+            double v = 0.0;
+#pragma unroll
+            for (int cc = 0; cc < B; ++cc) v += d[a * B + cc] * sum[cc];
+            o[a] = (V)v;
+          }
+        } else {
+#pragma unroll
+          for (int a = 0; a < B; ++a) o[a] = (V)sum[a];
+        }
+      }
+      __syncthreads();
+    }
+    for (int l = L0; l < L1; ++l) {
+      const int b0 = sLev[l - L0], b1 = sLev[l + 1 - L0];
+      const int base = (l % t.ring) * t.width - b0;
+      for (int r = b0 + tid; r < b1; r += nt) {
+        const V* o = ring + (size_t)(base + r) * B;
+#pragma unroll
+        for (int a = 0; a < B; ++a) res[(size_t)sRow[r - r0] * B + a] = o[a];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Both triangular solves of ILU0 in one workgroup, the levels in chunks
+// staged through LDS (ilu_chunk_sweep): out = U^-1 L^-1 in, tmp = L^-1 in.
+extern __shared__ char ilu_chunk_lds[];
+template <int B, typename V>
+__global__ void __launch_bounds__(256) ilu_apply_chunk_k(IluSweep t, const double* __restrict__ lu,
+                                                         const double* __restrict__ dinv, const V* __restrict__ in,
+                                                         V* __restrict__ tmp, V* __restrict__ out,
+                                                         const BicgScalars* __restrict__ sc) {
+  if (sc && finished(sc)) return;
+  ilu_chunk_sweep<B, V, false>(t, t.nlo_chunk, t.lo_chunk, t.lo_lev, t.lo_rows, t.lo_ent, t.lo_blk, t.lo_src, lu,
+                               nullptr, in, tmp, ilu_chunk_lds);
+  ilu_chunk_sweep<B, V, true>(t, t.nup_chunk, t.up_chunk, t.up_lev, t.up_rows, t.up_ent, t.up_blk, t.up_src, lu,
+                              dinv, tmp, out, ilu_chunk_lds);
 }
 
 // ----------------------------------------------------------------- state I/O helpers
@@ -830,25 +1104,58 @@ void ilu_factor(int B, int nrows, const int* rows, const int* rp, const int* col
   DCG_DISPATCH_B(B, (ilu_factor_k<BB><<<grid_for(nrows, 64), 64, 0, s>>>(nrows, rows, rp, col, diag_pos, lu, dinv, status)));
 }
 
-void ilu_factor_sweep(int B, int nlev, const int* lev, const int* rows, const int* rp, const int* col,
-                      const int* diag_pos, double* lu, double* dinv, int* status, hipStream_t s) {
-  if (nlev <= 0) return;
-  DCG_DISPATCH_B(B, (ilu_factor_sweep_k<BB><<<1, 256, 0, s>>>(nlev, lev, rows, rp, col, diag_pos, lu, dinv, status)));
+void ilu_factor_sweep(int B, const IluSweep& t, const int* rp, const int* col, const int* diag_pos, double* lu,
+                      double* dinv, int* status, hipStream_t s) {
+  if (t.nlo <= 0) return;
+  if (t.inbox_path && B <= 2) {
+    const long n = (long)t.nrows * kIluRegRow * B * B;
+    if (B == 1) {
+      ilu_pack_k<1><<<grid_for(n, kThreads), kThreads, 0, s>>>(t.nrows, t, lu, t.packed);
+      ilu_factor_inbox_k<1><<<1, t.factor_threads, (size_t)t.factor_threads * kIluRegRow * sizeof(double), s>>>(
+          t, t.packed, t.inbox, lu, dinv, status);
+    } else {
+      ilu_pack_k<2><<<grid_for(n, kThreads), kThreads, 0, s>>>(t.nrows, t, lu, t.packed);
+      ilu_factor_inbox_k<2><<<1, t.factor_threads, (size_t)t.factor_threads * kIluRegRow * 4 * sizeof(double), s>>>(
+          t, t.packed, t.inbox, lu, dinv, status);
+    }
+    return;
+  }
+  DCG_DISPATCH_B(B, (ilu_factor_sweep_k<BB><<<1, 256, 0, s>>>(t.nlo, t.lo_lev, t.lo_rows, rp, col, diag_pos, lu,
+                                                              dinv, status)));
+}
+
+// the chunked apply with `lds` bytes of dynamic LDS; above 64 KiB the
+// kernel's limit is raised first (once per instantiation and size)
+template <int BB, typename V>
+void launch_ilu_chunk(size_t lds, const IluSweep& t, const double* lu, const double* dinv, const V* in, V* tmp, V* out,
+                      const BicgScalars* sc, hipStream_t s) {
+  static size_t raised = 64 * 1024;
+  if (lds > raised) {
+    DCG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&ilu_apply_chunk_k<BB, V>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    raised = lds;
+  }
+  ilu_apply_chunk_k<BB, V><<<1, 256, lds, s>>>(t, lu, dinv, in, tmp, out, sc);
 }
 
 template <typename V>
-void ilu_apply_sweep(int B, int nlo, const int* lo_lev, const int* lo_rows, int nup, const int* up_lev,
-                     const int* up_rows, const int* rp, const int* col, const int* diag_pos, const double* lu,
+void ilu_apply_sweep(int B, const IluSweep& t, const int* rp, const int* col, const int* diag_pos, const double* lu,
                      const double* dinv, const V* in, V* tmp, V* out, const BicgScalars* sc, hipStream_t s) {
-  DCG_DISPATCH_B(B, (ilu_apply_sweep_k<BB, V><<<1, 256, 0, s>>>(nlo, lo_lev, lo_rows, nup, up_lev, up_rows, rp, col,
-                                                                diag_pos, lu, dinv, in, tmp, out, sc)));
+  if (t.chunked) {
+    const size_t lds = IluChunkLayout(t, B, static_cast<int>(sizeof(V))).total;
+    DCG_DISPATCH_B(B, (launch_ilu_chunk<BB, V>(lds, t, lu, dinv, in, tmp, out, sc, s)));
+    return;
+  }
+  DCG_DISPATCH_B(B, (ilu_apply_sweep_k<BB, V><<<1, 256, 0, s>>>(t.nlo, t.lo_lev, t.lo_rows, t.nup, t.up_lev,
+                                                                t.up_rows, rp, col, diag_pos, lu, dinv, in, tmp, out,
+                                                                sc)));
 }
-template void ilu_apply_sweep<double>(int, int, const int*, const int*, int, const int*, const int*, const int*,
-                                      const int*, const int*, const double*, const double*, const double*, double*,
-                                      double*, const BicgScalars*, hipStream_t);
-template void ilu_apply_sweep<float>(int, int, const int*, const int*, int, const int*, const int*, const int*,
-                                     const int*, const int*, const double*, const double*, const float*, float*,
-                                     float*, const BicgScalars*, hipStream_t);
+size_t ilu_chunk_lds_bytes(const IluSweep& t, int B, int vbytes) { return IluChunkLayout(t, B, vbytes).total; }
+
+template void ilu_apply_sweep<double>(int, const IluSweep&, const int*, const int*, const int*, const double*,
+                                      const double*, const double*, double*, double*, const BicgScalars*, hipStream_t);
+template void ilu_apply_sweep<float>(int, const IluSweep&, const int*, const int*, const int*, const double*,
+                                     const double*, const float*, float*, float*, const BicgScalars*, hipStream_t);
 
 template <typename V>
 void ilu_lower(int B, int nrows, const int* rows, const int* rp, const int* col, const int* diag_pos,

@@ -226,9 +226,13 @@ def rel_mass_l2_per_species(oracle_model, a, b, t=0.0):
         n, off, bsp = comp["n_nodes"], comp["dof_offset"], comp["n_species"]
         for i in range(bsp):
             idx = off + i + bsp * np.arange(n)
-            Ms = M[idx][:, idx]
-            e = a[idx] - b[idx]
-            den = float(b[idx] @ (Ms @ b[idx]))
-            num = float(e @ (Ms @ e))
+            # x^T M x of a vector that is zero off the species is the species
+            # block's quadratic form (no submatrix is formed)
+            e = np.zeros(len(a))
+            e[idx] = a[idx] - b[idx]
+            bs = np.zeros(len(b))
+            bs[idx] = b[idx]
+            den = float(bs @ (M @ bs))
+            num = float(e @ (M @ e))
             out.append(np.sqrt(max(num, 0.0) / den) if den > 0 else np.sqrt(max(num, 0.0)))
     return out
