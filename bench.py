@@ -229,7 +229,8 @@ def batch_measure(args, dca, wl, precision, warmup, steps, plain_stride, cpu=Tru
     t_setup = time.perf_counter()
     b = dca.Batch(f"precision = {precision}\ngraph_iterations = {args.graph_iterations}\n"
                   f"preconditioner = {args.preconditioner}\n"
-                  f"refinement.inner_relative_tolerance = {args.ir_inner:g}\n", 0)
+                  f"refinement.inner_relative_tolerance = {args.ir_inner:g}\n"
+                  f"point_tile = {args.point_tile}\n", 0)
     ids = []
     for w in ws:
         i = b.add(w.ini)
