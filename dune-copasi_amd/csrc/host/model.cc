@@ -516,7 +516,10 @@ void Model::setup_device() {
     for (int n2 : cd.natiles) most = std::max(most, n2);
     ntiles_all += static_cast<size_t>(most);
   }
-  part_.resize(std::max<size_t>(std::max<size_t>(2 * nparts, 2 * ntiles_all + 16), 8 * 1024 + 64));
+  // operator partial dots first, then the s-update's partial norms (their own
+  // region: they are finalised after the next operator launch)
+  spart_off_ = std::max<size_t>(std::max<size_t>(2 * nparts, 2 * ntiles_all + 16), 8 * 1024 + 64);
+  part_.resize(spart_off_ + static_cast<size_t>((L_.nslot + 255) / 256) + 64);
   scalar_out_.resize(16);
   sc_.resize(1);
   status_.resize(4);
@@ -994,6 +997,7 @@ Model::KrylovWork Model::own_work() {
   KrylovWork w;
   for (int i = 0; i < 9; ++i) w.kv[i] = kv_[i].get();
   w.part = part_.get();
+  w.spart = part_.get() + spart_off_;
   w.sc = sc_.get();
   w.val = val_.get();
   w.dinv = dinv_.get();
@@ -1275,13 +1279,20 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
   // out = J in with the partial dots (out.w1, out.w2) finalised into `stage`
   const int* done_flag =
       reinterpret_cast<const int*>(reinterpret_cast<const char*>(sc) + offsetof(k::BicgScalars, done));
+  // the partial dots' finaliser; stage 3 first finalises the s-update's
+  // partial norms (stage 2, deferred past the second operator launch)
+  const int nblk_s = (ns + 255) / 256;
+  auto finalise = [&](int nparts, int stage) {
+    if (stage == 3) k::bicg_fin23(nblk_s, w.spart, nparts, part, sc, stream_);
+    else k::bicg_fin(nparts, part, 2, stage, sc, stream_);
+  };
   auto matvec = [&](const double* in, double* out, const double* w1, const double* w2, int stage) {
     if (cfg_.matrix_free) {
       prof_begin(kJacobianApply);
       launch_apply(lin_x_.get(), lin_t_, lin_cm_, lin_ca_, in, out, done_flag);
       prof_end(kJacobianApply, apply_bytes());
       k::dot_parts(n, out, w1, w2, part, nred, sc, stream_);
-      k::bicg_fin(nred, part, 2, stage, sc, stream_);
+      finalise(nred, stage);
       return;
     }
     if (use_points_) {
@@ -1289,12 +1300,12 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
       const int np = launch_points_apply(in, out, done_flag, false, w1, w2, part);
       prof_end(kPointApply64, point_apply_bytes(8));
       if (!np) k::dot_parts(n, out, w1, w2, part, nred, sc, stream_);
-      k::bicg_fin(np ? np : nred, part, 2, stage, sc, stream_);
+      finalise(np ? np : nred, stage);
       return;
     }
     prof_begin(kSpmv64);
     k::spmv(B, plan_, w.val, in, out, w1, w2, part, sc, stage, stream_);
-    k::bicg_fin(plan_.parts(), part, 2, stage, sc, stream_);
+    finalise(plan_.parts(), stage);
     prof_end(kSpmv64, spmv_bytes(8, 8, 1));
   };
   const int akind = cfg_.matrix_free ? kJacobianApply : use_points_ ? kPointApply64 : kSpmv64;
@@ -1302,9 +1313,8 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
     k::bicg_p<double>(B, ns, r, p, v, ph, dinv, sc, stream_);
     if (ilu) apply_ilu(p, ph);
     matvec(ph, v, rh, nullptr, 1);
-    k::bicg_s<double>(B, ns, r, v, s, sh, dinv, sc, part, stream_);
+    k::bicg_s<double>(B, ns, r, v, s, sh, dinv, sc, w.spart, stream_);
     if (ilu) apply_ilu(s, sh);
-    k::bicg_fin(nblk, part, 1, 2, sc, stream_);
     matvec(sh, t, s, t, 3);
     k::bicg_xr<double>(B, ns, x, ph, sh, s, t, r, rh, sc, part, stream_);
     k::bicg_fin(nblk, part, 2, 4, sc, stream_);
@@ -1370,18 +1380,24 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
   };
   const int* done_flag =
       reinterpret_cast<const int*>(reinterpret_cast<const char*>(sc_.get()) + offsetof(k::BicgScalars, done));
+  double* spart = part_.get() + spart_off_;
+  const int nblk_s = (ns + 255) / 256;
+  auto finalise = [&](int nparts, int stage) {
+    if (stage == 3) k::bicg_fin23(nblk_s, spart, nparts, part_.get(), sc_.get(), stream_);
+    else k::bicg_fin(nparts, part_.get(), 2, stage, sc_.get(), stream_);
+  };
   auto matvec = [&](const float* in, float* out, const float* w1, const float* w2, int stage) {
     if (use_points_) {
       prof_begin(kPointApply32);
       const int np = launch_points_apply(in, out, done_flag, true, w1, w2, part_.get());
       prof_end(kPointApply32, point_apply_bytes(4));
       if (!np) k::dot_parts(n, out, w1, w2, part_.get(), nred, sc_.get(), stream_);
-      k::bicg_fin(np ? np : nred, part_.get(), 2, stage, sc_.get(), stream_);
+      finalise(np ? np : nred, stage);
       return;
     }
     prof_begin(kSpmv32);
     k::spmv(B, plan32_, val32_.get(), in, out, w1, w2, part_.get(), sc_.get(), stage, stream_);
-    k::bicg_fin(plan32_.parts(), part_.get(), 2, stage, sc_.get(), stream_);
+    finalise(plan32_.parts(), stage);
     prof_end(kSpmv32, spmv_bytes(4, 4, 1));
   };
   const int akind = use_points_ ? kPointApply32 : kSpmv32;
@@ -1389,9 +1405,8 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
     k::bicg_p<float>(B, ns, r, p, v, ph, dinv, sc_.get(), stream_);
     if (ilu) apply_ilu(p, ph);
     matvec(ph, v, rh, nullptr, 1);
-    k::bicg_s<float>(B, ns, r, v, s, sh, dinv, sc_.get(), part_.get(), stream_);
+    k::bicg_s<float>(B, ns, r, v, s, sh, dinv, sc_.get(), spart, stream_);
     if (ilu) apply_ilu(s, sh);
-    k::bicg_fin(nblk, part_.get(), 1, 2, sc_.get(), stream_);
     matvec(sh, t, s, t, 3);
     k::bicg_xr<float>(B, ns, x, ph, sh, s, t, r, rh, sc_.get(), part_.get(), stream_);
     k::bicg_fin(nblk, part_.get(), 2, 4, sc_.get(), stream_);
@@ -1915,6 +1930,7 @@ int Model::linear_solve_noalloc(const double* jac, const double* b_packed, doubl
   w.dinv = take(static_cast<size_t>(L_.nslot) * L_.B * L_.B);
   w.lu = ilu ? take(static_cast<size_t>(matrix_values())) : nullptr;
   w.part = take(part_.size());
+  w.spart = w.part + spart_off_;
   w.sc = reinterpret_cast<k::BicgScalars*>(base + off);
   off += sc_words;
   w.status = reinterpret_cast<int*>(take(2));

@@ -234,6 +234,7 @@ class Model {
   struct KrylovWork {
     double* kv[9] = {};  // r rh p v ph s sh t tmp
     double* part = nullptr;
+    double* spart = nullptr;  // the s-update's partial norms
     k::BicgScalars* sc = nullptr;
     const double* val = nullptr;  // the assembled operator
     double* dinv = nullptr;
@@ -363,6 +364,7 @@ class Model {
   DeviceArray<double> kv_[9];  // r rh p v ph s sh t tmp
   DeviceArray<float> kf_[10];  // same in FP32 + x
   DeviceArray<double> part_, scalar_out_;
+  size_t spart_off_ = 0;  // offset of the s-update's partial norms in part_
   DeviceArray<k::BicgScalars> sc_;
   DeviceArray<int> status_;
   k::SpmvPlan plan_, plan32_;

@@ -297,6 +297,9 @@ void weighted_species_sum(long first, long n, int B, const double* w, const doub
                           hipStream_t s);
 // reduce per-workgroup partials (stride 1 or 2) and update the scalars of `stage`
 void bicg_fin(int nparts, const double* part, int stride, int stage, BicgScalars* sc, hipStream_t s);
+// stages 2 and 3 of one iteration in one launch: na partial norms of the
+// s-update (stride 1), then nb partial dot pairs of the second operator
+void bicg_fin23(int na, const double* pa, int nb, const double* pb, BicgScalars* sc, hipStream_t s);
 void to_float(long n, const double* a, float* b, hipStream_t s);
 void to_double(long n, const float* a, double* b, hipStream_t s);
 

@@ -118,6 +118,35 @@ __global__ void __launch_bounds__(1024) bicg_fin_k(int nparts, const double* __r
   if (threadIdx.x == 0) bicg_update(sc, stage, v[0], v[1]);
 }
 
+// Stages 2 and 3 in one launch: the s-update's partial norms (stride 1,
+// na of them) and then the second operator's partial dots (stride 2, nb
+// pairs), each summed as bicg_fin_k sums it.  Deferring the half-step test
+// past the second operator launch costs that launch once per solve (in the
+// iteration that converges at the half step) and saves a launch in every
+// iteration.
+__global__ void __launch_bounds__(1024) bicg_fin23_k(int na, const double* __restrict__ pa, int nb,
+                                                     const double* __restrict__ pb, BicgScalars* __restrict__ sc) {
+  if (sc->done) return;
+  __shared__ double red[32];
+  double v[2] = {0.0, 0.0};
+  for (int i = threadIdx.x; i < na; i += blockDim.x) v[0] += pa[i];
+  block_sum(v, red);
+  __shared__ double a;
+  if (threadIdx.x == 0) {
+    a = v[0];
+    bicg_update(sc, 2, a, v[1]);
+  }
+  __syncthreads();
+  if (sc->done) return;
+  double u[2] = {0.0, 0.0};
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    u[0] += pb[2 * (size_t)i];
+    u[1] += pb[2 * (size_t)i + 1];
+  }
+  block_sum(u, red);
+  if (threadIdx.x == 0) bicg_update(sc, 3, u[0], u[1]);
+}
+
 // ----------------------------------------------------------------- SpMV
 // y = A x over BCSR, one workgroup per chunk of rows whose block-row entries
 // (blocks x B) fit the LDS stage (body in spmv_entries_chunk).  Optionally
@@ -1096,6 +1125,10 @@ template void bicg_xr<float>(int, int, float*, const float*, const float*, const
 
 void bicg_fin(int nparts, const double* part, int stride, int stage, BicgScalars* sc, hipStream_t s) {
   bicg_fin_k<<<1, 1024, 0, s>>>(nparts, part, stride, stage, sc);
+}
+
+void bicg_fin23(int na, const double* pa, int nb, const double* pb, BicgScalars* sc, hipStream_t s) {
+  bicg_fin23_k<<<1, 1024, 0, s>>>(na, pa, nb, pb, sc);
 }
 
 void ilu_factor(int B, int nrows, const int* rows, const int* rp, const int* col, const int* diag_pos, double* lu,
