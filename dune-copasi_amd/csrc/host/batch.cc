@@ -490,10 +490,10 @@ void Batch::capture(SolverGroup& g, bool fp32) {
   DCG_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
   for (int it = 0; it < opt_.graph_iterations; ++it) {
     if (fp32)
-      k::b_bicg_iteration<float>(g.B, g.variant32, g.w, g.mv.get(), k::kVKry + 9, opt_.precond != Preconditioner::None, g.part.get(),
+      k::b_bicg_iteration<float>(g.B, g.variant32, g.w, g.mv.get(), k::kVKry + 9, jacobi_kind(), g.part.get(),
                                  g.sc.get(), stream_, nullptr, g.points ? &g.op32 : nullptr);
     else
-      k::b_bicg_iteration<double>(g.B, g.variant64, g.w, g.mv.get(), k::kVZ, opt_.precond != Preconditioner::None, g.part.get(), g.sc.get(),
+      k::b_bicg_iteration<double>(g.B, g.variant64, g.w, g.mv.get(), k::kVZ, jacobi_kind(), g.part.get(), g.sc.get(),
                                   stream_, nullptr, g.points ? &g.op64 : nullptr);
   }
   DCG_HIP(hipStreamEndCapture(stream_, &graph));
@@ -529,6 +529,12 @@ std::vector<std::array<int, 3>> Batch::spmv_choice() {
   std::vector<std::array<int, 3>> v;
   for (const auto& g : sgroups_) v.push_back({g.B, g.variant64, g.variant32});
   return v;
+}
+
+// the preconditioner as the batched updates take it: 0 none, 1 vertex
+// blocks, 2 the scalar diagonal (B inverse values per slot)
+int Batch::jacobi_kind() const {
+  return opt_.precond == Preconditioner::None ? 0 : opt_.precond == Preconditioner::Jacobi ? 2 : 1;
 }
 
 std::vector<std::array<int, 3>> Batch::point_tiles() {
@@ -692,10 +698,10 @@ void Batch::iterate(SolverGroup& g, bool fp32) {
       }
       if (all || done_its > opt_.lin_maxit + 4) break;
       if (fp32)
-        k::b_bicg_iteration<float>(g.B, g.variant32, g.w, g.mv.get(), k::kVKry + 9, opt_.precond != Preconditioner::None, g.part.get(),
+        k::b_bicg_iteration<float>(g.B, g.variant32, g.w, g.mv.get(), k::kVKry + 9, jacobi_kind(), g.part.get(),
                                    g.sc.get(), stream_, ev_, g.points ? &g.op32 : nullptr);
       else
-        k::b_bicg_iteration<double>(g.B, g.variant64, g.w, g.mv.get(), k::kVZ, opt_.precond != Preconditioner::None, g.part.get(),
+        k::b_bicg_iteration<double>(g.B, g.variant64, g.w, g.mv.get(), k::kVZ, jacobi_kind(), g.part.get(),
                                     g.sc.get(), stream_, ev_, g.points ? &g.op64 : nullptr);
       DCG_HIP(hipEventSynchronize(ev_[3]));
       float a = 0.f, b = 0.f;
@@ -717,10 +723,10 @@ void Batch::iterate(SolverGroup& g, bool fp32) {
         if (!g.graph[fp32 ? 1 : 0]) capture(g, fp32);
         DCG_HIP(hipGraphLaunch(g.graph[fp32 ? 1 : 0], stream_));
       } else if (fp32) {
-        k::b_bicg_iteration<float>(g.B, g.variant32, g.w, g.mv.get(), k::kVKry + 9, opt_.precond != Preconditioner::None, g.part.get(),
+        k::b_bicg_iteration<float>(g.B, g.variant32, g.w, g.mv.get(), k::kVKry + 9, jacobi_kind(), g.part.get(),
                                    g.sc.get(), stream_, nullptr, g.points ? &g.op32 : nullptr);
       } else {
-        k::b_bicg_iteration<double>(g.B, g.variant64, g.w, g.mv.get(), k::kVZ, opt_.precond != Preconditioner::None, g.part.get(), g.sc.get(),
+        k::b_bicg_iteration<double>(g.B, g.variant64, g.w, g.mv.get(), k::kVZ, jacobi_kind(), g.part.get(), g.sc.get(),
                                     stream_, nullptr, g.points ? &g.op64 : nullptr);
       }
     }

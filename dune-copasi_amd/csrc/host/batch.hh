@@ -164,6 +164,7 @@ class Batch {
   static int stage_call(int s) { return 3 + (s - 1); }
 
   void rebuild();
+  int jacobi_kind() const;
   void destroy_graphs();
   void tune(SolverGroup& g);
   void capture(SolverGroup& g, bool fp32);

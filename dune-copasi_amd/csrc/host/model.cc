@@ -1262,6 +1262,8 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
   const int nblk = (ns + 255) / 256;
   k::bicg_init<double>(n, b, x, r, rh, p, v, part, nred, sc, stream_);
   const double* dinv = jacobi_kind() ? w.dinv : nullptr;
+  // the reference's Jacobi inverts the scalar diagonal: B values per slot
+  const bool point_jacobi = cfg_.precond == Preconditioner::Jacobi;
   const bool ilu = cfg_.precond == Preconditioner::ILU0;
   auto apply_ilu = [&](const double* in, double* out) {
     if (ilu_sweep_) {
@@ -1310,10 +1312,10 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
   };
   const int akind = cfg_.matrix_free ? kJacobianApply : use_points_ ? kPointApply64 : kSpmv64;
   auto iteration = [&]() {
-    k::bicg_p<double>(B, ns, r, p, v, ph, dinv, sc, stream_);
+    k::bicg_p<double>(B, ns, r, p, v, ph, dinv, point_jacobi, sc, stream_);
     if (ilu) apply_ilu(p, ph);
     matvec(ph, v, rh, nullptr, 1);
-    k::bicg_s<double>(B, ns, r, v, s, sh, dinv, sc, w.spart, stream_);
+    k::bicg_s<double>(B, ns, r, v, s, sh, dinv, point_jacobi, sc, w.spart, stream_);
     if (ilu) apply_ilu(s, sh);
     matvec(sh, t, s, t, 3);
     k::bicg_xr<double>(B, ns, x, ph, sh, s, t, r, rh, sc, part, stream_);
@@ -1364,6 +1366,7 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
   const int nblk = (ns + 255) / 256;
   k::bicg_init<float>(n, b, x, r, rh, p, v, part_.get(), nred, sc_.get(), stream_);
   const double* dinv = jacobi_kind() ? dinv_.get() : nullptr;
+  const bool point_jacobi = cfg_.precond == Preconditioner::Jacobi;
   const bool ilu = cfg_.precond == Preconditioner::ILU0;
   auto apply_ilu = [&](const float* in, float* out) {
     if (ilu_sweep_) {
@@ -1402,10 +1405,10 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
   };
   const int akind = use_points_ ? kPointApply32 : kSpmv32;
   auto iteration = [&]() {
-    k::bicg_p<float>(B, ns, r, p, v, ph, dinv, sc_.get(), stream_);
+    k::bicg_p<float>(B, ns, r, p, v, ph, dinv, point_jacobi, sc_.get(), stream_);
     if (ilu) apply_ilu(p, ph);
     matvec(ph, v, rh, nullptr, 1);
-    k::bicg_s<float>(B, ns, r, v, s, sh, dinv, sc_.get(), spart, stream_);
+    k::bicg_s<float>(B, ns, r, v, s, sh, dinv, point_jacobi, sc_.get(), spart, stream_);
     if (ilu) apply_ilu(s, sh);
     matvec(sh, t, s, t, 3);
     k::bicg_xr<float>(B, ns, x, ph, sh, s, t, r, rh, sc_.get(), part_.get(), stream_);

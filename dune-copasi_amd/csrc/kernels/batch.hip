@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(kThreads) binv_k(const int2* __restrict__ work
   const MemberView& m = mv[w.x];
   const int s = w.y * kThreads + threadIdx.x;
   if (s >= m.nslot) return;
-  jacobi_inverse_dev<B>(m.val + (size_t)m.diag_pos[s] * B * B, m.dinv + (size_t)s * B * B, status, point != 0);
+  jacobi_inverse_dev<B>(m.val + (size_t)m.diag_pos[s] * B * B, m.dinv, s, status, point != 0);
 }
 
 // FP32 copy of the matrix rows of one 256-slot block
@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(kThreads) bp_k(const int2* __restrict__ work, 
     pv[i] = (V)((double)r[k] + beta * ((double)p[k] - omega * (double)v[k]));
     p[k] = pv[i];
   }
-  if (jacobi) dinv_apply<B>(m.dinv, s, pv, ph + (size_t)s * B);
+  if (jacobi) dinv_apply<B>(m.dinv, s, pv, ph + (size_t)s * B, jacobi == 2);
   else
 #pragma unroll
     for (int i = 0; i < B; ++i) ph[(size_t)s * B + i] = pv[i];
@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(kThreads) bs_k(const int2* __restrict__ work, 
       sv[k] = t[i];
       acc[0] += (double)t[i] * (double)t[i];
     }
-    if (jacobi) dinv_apply<B>(m.dinv, s, t, sh + (size_t)s * B);
+    if (jacobi) dinv_apply<B>(m.dinv, s, t, sh + (size_t)s * B, jacobi == 2);
     else
 #pragma unroll
       for (int i = 0; i < B; ++i) sh[(size_t)s * B + i] = t[i];
@@ -532,11 +532,11 @@ template void b_bicg_init<float>(int, const BatchWork&, const MemberView*, int, 
                                  hipStream_t);
 
 template <typename V>
-void b_bicg_iteration(int B, int variant, const BatchWork& w, const MemberView* mv, int x, bool jacobi,
+void b_bicg_iteration(int B, int variant, const BatchWork& w, const MemberView* mv, int x, int jacobi,
                       double* part, BicgScalars* sc, hipStream_t st, hipEvent_t* ev, const BatchOperator* op) {
   if (w.nvwork <= 0) return;
   using T = V;  // the matrix copy has the precision of the Krylov vectors
-  const int jac = jacobi ? 1 : 0;
+  const int jac = jacobi;  // 0 none, 1 vertex blocks, 2 the scalar diagonal
   const Skip none{};
   auto mark = [&](int i) {
     if (ev) (void)hipEventRecord(ev[i], st);
@@ -566,9 +566,9 @@ void b_bicg_iteration(int B, int variant, const BatchWork& w, const MemberView* 
     bfin_k<<<w.members, kThreads, 0, st>>>(w.vrange, part, 2, 4, sc);
   });
 }
-template void b_bicg_iteration<double>(int, int, const BatchWork&, const MemberView*, int, bool, double*,
+template void b_bicg_iteration<double>(int, int, const BatchWork&, const MemberView*, int, int, double*,
                                        BicgScalars*, hipStream_t, hipEvent_t*, const BatchOperator*);
-template void b_bicg_iteration<float>(int, int, const BatchWork&, const MemberView*, int, bool, double*,
+template void b_bicg_iteration<float>(int, int, const BatchWork&, const MemberView*, int, int, double*,
                                       BicgScalars*, hipStream_t, hipEvent_t*, const BatchOperator*);
 
 void b_spmv64(int B, int variant, const BatchWork& w, const MemberView* mv, Skip sk, int x, int y, hipStream_t st) {

@@ -121,8 +121,15 @@ __device__ void bicg_update(BicgScalars* sc, int stage, double v0, double v1) {
   }
 }
 
+// out = M^-1 in at slot s: a diagonal inverse (diag: B values per slot, the
+// reference's point Jacobi) or a B x B block per slot
 template <int B, typename V>
-__device__ __forceinline__ void dinv_apply(const double* __restrict__ dinv, int s, const V* in, V* out) {
+__device__ __forceinline__ void dinv_apply(const double* __restrict__ dinv, int s, const V* in, V* out, bool diag) {
+  if (diag) {
+#pragma unroll
+    for (int i = 0; i < B; ++i) out[i] = (V)(dinv[(size_t)s * B + i] * (double)in[i]);
+    return;
+  }
   const double* d = dinv + (size_t)s * B * B;
   double t[B];
 #pragma unroll
@@ -203,16 +210,17 @@ __device__ __forceinline__ void diag_inverse_dev(const double* __restrict__ d, d
   for (int i = 0; i < B; ++i) {
     const double a = d[i * B + i];
     if (a == 0.0) status[0] = 4;
-#pragma unroll
-    for (int j = 0; j < B; ++j) o[i * B + j] = i == j ? (a == 0.0 ? 1.0 : 1.0 / a) : 0.0;
+    o[i] = a == 0.0 ? 1.0 : 1.0 / a;
   }
 }
 
+// the Jacobi inverse of slot s's diagonal block d into dinv: B values per
+// slot (point: the inverse of the scalar diagonal) or a B x B block
 template <int B>
-__device__ __forceinline__ void jacobi_inverse_dev(const double* __restrict__ d, double* __restrict__ o,
+__device__ __forceinline__ void jacobi_inverse_dev(const double* __restrict__ d, double* __restrict__ dinv, int s,
                                                    int* __restrict__ status, bool point) {
-  if (point) diag_inverse_dev<B>(d, o, status);
-  else block_inverse_dev<B>(d, o, status);
+  if (point) diag_inverse_dev<B>(d, dinv + (size_t)s * B, status);
+  else block_inverse_dev<B>(d, dinv + (size_t)s * B * B, status);
 }
 
 

@@ -166,7 +166,7 @@ void b_bicg_init(int B, const BatchWork& w, const MemberView* mv, int b, int x, 
 // one BiCGSTAB iteration on every unfinished member (jacobi: apply dinv)
 // ev (optional, 4 events): recorded before and after each of the two SpMVs
 template <typename V>
-void b_bicg_iteration(int B, int variant, const BatchWork& w, const MemberView* mv, int x, bool jacobi,
+void b_bicg_iteration(int B, int variant, const BatchWork& w, const MemberView* mv, int x, int jacobi,
                       double* part, BicgScalars* sc, hipStream_t st, hipEvent_t* ev = nullptr,
                       const BatchOperator* op = nullptr);
 // y = A x on the FP64 matrix (selectors), members skipped by sk
@@ -220,10 +220,10 @@ template <typename V>
 void bicg_init(long n, const double* b, V* x, V* r, V* rh, V* p, V* v, double* part, int nparts, BicgScalars* sc,
                hipStream_t s);
 template <typename V>
-void bicg_p(int B, int nslot, const V* r, V* p, const V* v, V* ph, const double* dinv, const BicgScalars* sc,
+void bicg_p(int B, int nslot, const V* r, V* p, const V* v, V* ph, const double* dinv, bool diag, const BicgScalars* sc,
             hipStream_t s);
 template <typename V>
-void bicg_s(int B, int nslot, const V* r, const V* v, V* sv, V* sh, const double* dinv, const BicgScalars* sc,
+void bicg_s(int B, int nslot, const V* r, const V* v, V* sv, V* sh, const double* dinv, bool diag, const BicgScalars* sc,
             double* part, hipStream_t s);
 template <typename V>
 void bicg_xr(int B, int nslot, V* x, const V* ph, const V* sh, const V* sv, const V* t, V* r, const V* rh,

@@ -274,7 +274,7 @@ __global__ void block_inverse_k(int nslot, const double* __restrict__ val, const
                                 double* __restrict__ dinv, int* __restrict__ status, int point) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslot) return;
-  jacobi_inverse_dev<B>(val + (size_t)diag_pos[s] * B * B, dinv + (size_t)s * B * B, status, point != 0);
+  jacobi_inverse_dev<B>(val + (size_t)diag_pos[s] * B * B, dinv, s, status, point != 0);
 }
 
 // ----------------------------------------------------------------- BiCGSTAB
@@ -285,7 +285,8 @@ __device__ __forceinline__ bool finished(const BicgScalars* sc) { return sc->don
 // p = r + beta (p - omega v); ph = M^-1 p (Jacobi fused, else copy)
 template <int B, typename V>
 __global__ void bicg_p_k(int nslot, const V* __restrict__ r, V* __restrict__ p, const V* __restrict__ v,
-                         V* __restrict__ ph, const double* __restrict__ dinv, const BicgScalars* __restrict__ sc) {
+                         V* __restrict__ ph, const double* __restrict__ dinv, int diag,
+                         const BicgScalars* __restrict__ sc) {
   if (finished(sc)) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslot) return;
@@ -298,7 +299,7 @@ __global__ void bicg_p_k(int nslot, const V* __restrict__ r, V* __restrict__ p, 
     pv[i] = (V)nv;
     p[k] = pv[i];
   }
-  if (dinv) dinv_apply<B>(dinv, s, pv, ph + (size_t)s * B);
+  if (dinv) dinv_apply<B>(dinv, s, pv, ph + (size_t)s * B, diag != 0);
   else
 #pragma unroll
     for (int i = 0; i < B; ++i) ph[(size_t)s * B + i] = pv[i];
@@ -308,7 +309,7 @@ __global__ void bicg_p_k(int nslot, const V* __restrict__ r, V* __restrict__ p, 
 template <int B, typename V>
 __global__ void __launch_bounds__(kThreads) bicg_s_k(int nslot, const V* __restrict__ r, const V* __restrict__ v,
                                                      V* __restrict__ sv, V* __restrict__ sh,
-                                                     const double* __restrict__ dinv,
+                                                     const double* __restrict__ dinv, int diag,
                                                      const BicgScalars* __restrict__ sc, double* __restrict__ part) {
   if (finished(sc)) return;
   __shared__ double red[16];
@@ -324,7 +325,7 @@ __global__ void __launch_bounds__(kThreads) bicg_s_k(int nslot, const V* __restr
       sv[k] = t[i];
       acc[0] += (double)t[i] * (double)t[i];
     }
-    if (dinv) dinv_apply<B>(dinv, s, t, sh + (size_t)s * B);
+    if (dinv) dinv_apply<B>(dinv, s, t, sh + (size_t)s * B, diag != 0);
     else
 #pragma unroll
       for (int i = 0; i < B; ++i) sh[(size_t)s * B + i] = t[i];
@@ -1094,24 +1095,24 @@ template void bicg_init<float>(long, const double*, float*, float*, float*, floa
                                BicgScalars*, hipStream_t);
 
 template <typename V>
-void bicg_p(int B, int nslot, const V* r, V* p, const V* v, V* ph, const double* dinv, const BicgScalars* sc,
+void bicg_p(int B, int nslot, const V* r, V* p, const V* v, V* ph, const double* dinv, bool diag, const BicgScalars* sc,
             hipStream_t s) {
-  DCG_DISPATCH_B(B, (bicg_p_k<BB, V><<<grid_for(nslot), kThreads, 0, s>>>(nslot, r, p, v, ph, dinv, sc)));
+  DCG_DISPATCH_B(B, (bicg_p_k<BB, V><<<grid_for(nslot), kThreads, 0, s>>>(nslot, r, p, v, ph, dinv, diag ? 1 : 0, sc)));
 }
-template void bicg_p<double>(int, int, const double*, double*, const double*, double*, const double*,
+template void bicg_p<double>(int, int, const double*, double*, const double*, double*, const double*, bool,
                              const BicgScalars*, hipStream_t);
-template void bicg_p<float>(int, int, const float*, float*, const float*, float*, const double*, const BicgScalars*,
-                            hipStream_t);
+template void bicg_p<float>(int, int, const float*, float*, const float*, float*, const double*, bool,
+                            const BicgScalars*, hipStream_t);
 
 template <typename V>
-void bicg_s(int B, int nslot, const V* r, const V* v, V* sv, V* sh, const double* dinv, const BicgScalars* sc,
+void bicg_s(int B, int nslot, const V* r, const V* v, V* sv, V* sh, const double* dinv, bool diag, const BicgScalars* sc,
             double* part, hipStream_t s) {
-  DCG_DISPATCH_B(B, (bicg_s_k<BB, V><<<grid_for(nslot), kThreads, 0, s>>>(nslot, r, v, sv, sh, dinv, sc, part)));
+  DCG_DISPATCH_B(B, (bicg_s_k<BB, V><<<grid_for(nslot), kThreads, 0, s>>>(nslot, r, v, sv, sh, dinv, diag ? 1 : 0, sc, part)));
 }
-template void bicg_s<double>(int, int, const double*, const double*, double*, double*, const double*,
+template void bicg_s<double>(int, int, const double*, const double*, double*, double*, const double*, bool,
                              const BicgScalars*, double*, hipStream_t);
-template void bicg_s<float>(int, int, const float*, const float*, float*, float*, const double*, const BicgScalars*,
-                            double*, hipStream_t);
+template void bicg_s<float>(int, int, const float*, const float*, float*, float*, const double*, bool,
+                            const BicgScalars*, double*, hipStream_t);
 
 template <typename V>
 void bicg_xr(int B, int nslot, V* x, const V* ph, const V* sh, const V* sv, const V* t, V* r, const V* rh,
