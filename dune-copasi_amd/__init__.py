@@ -367,14 +367,22 @@ class Model:
         return out, names[ch.value]
 
     def point_tuning(self):
-        """[(compartment, (tx, ty, tz), microseconds, chosen)] of the point-apply tile autotuner."""
+        """[(compartment, (tx, ty, tz), microseconds, chosen)] of the point-apply tile autotuner;
+        point_tuning_waves() gives each candidate's register bound (waves per SIMD)."""
+        return [(c, t[:3], us, ch) for c, t, us, ch in self._point_tuning_raw()]
+
+    def point_tuning_waves(self):
+        """Waves per SIMD each point-apply candidate's registers are bounded for (1: none)."""
+        return [t[3] for _, t, _, _ in self._point_tuning_raw()]
+
+    def _point_tuning_raw(self):
         n = _I()
         comp = np.zeros(64, dtype=np.int32)
-        tile = np.zeros(192, dtype=np.int32)
+        tile = np.zeros(256, dtype=np.int32)
         us = np.zeros(64)
         ch = np.zeros(64, dtype=np.int32)
         _check(self._lib.dcg_point_tuning(self._h, ctypes.byref(n), _ptr(comp), _ptr(tile), _ptr(us), _ptr(ch)))
-        return [(int(comp[k]), tuple(int(x) for x in tile[3 * k:3 * k + 3]), float(us[k]), bool(ch[k]))
+        return [(int(comp[k]), tuple(int(x) for x in tile[4 * k:4 * k + 4]), float(us[k]), bool(ch[k]))
                 for k in range(n.value)]
 
     # ---- time loop

@@ -352,10 +352,12 @@ int dcg_point_tuning(dcg_model_t h, int* count, int* comp, int* tile, double* mi
   if (count) *count = n;
   for (int k = 0; k < n; ++k) {
     if (comp) comp[k] = t[k].comp;
-    if (tile)
-      for (int d = 0; d < 3; ++d) tile[3 * k + d] = t[k].tile[d];
+    if (tile) {
+      for (int d = 0; d < 3; ++d) tile[4 * k + d] = t[k].tile[d];
+      tile[4 * k + 3] = t[k].waves;
+    }
     if (microseconds) microseconds[k] = t[k].us;
-    if (chosen) chosen[k] = h->m->point_tile(t[k].comp) == t[k].tile ? 1 : 0;
+    if (chosen) chosen[k] = t[k].chosen ? 1 : 0;
   }
   return 0;
 }

@@ -1217,9 +1217,10 @@ __device__ __forceinline__ void block_rows(int dx, int dy, int dz, int li, int l
 // around it (XB x YB x ZB) and the cells touching its rows (CXB x CYB x CZB).
 // The matrix-free apply uses the assembly tile; the point apply has its own,
 // chosen per model by timing the candidates.
-template <int X, int Y, int Z, int THREADS = 256>
+template <int X, int Y, int Z, int THREADS = 256, int WAVES = 1>
 struct ApplyTile {
   static constexpr int NT = THREADS;  // workgroup size
+  static constexpr int MINW = WAVES;  // waves per SIMD the registers are bounded for
   static constexpr int TX = X, TY = DIM > 1 ? Y : 1, TZ = DIM > 2 ? Z : 1;
   static constexpr int XB = TX + 2, YB = DIM > 1 ? TY + 2 : 1, ZB = DIM > 2 ? TZ + 2 : 1;
   static constexpr int CXB = TX + 1, CYB = DIM > 1 ? TY + 1 : 1, CZB = DIM > 2 ? TZ + 1 : 1;
@@ -2011,7 +2012,7 @@ extern "C" __global__ void __launch_bounds__(256)
 // XCDs in turn) walks one contiguous run of tiles and finds the neighbour
 // tiles' coefficients in its own L2.  Position c writes part[2c], part[2c+1].
 #define DCG_POINT_ENTRY(NAME, T, TL)                                                                              \
-  extern "C" __global__ void __launch_bounds__(TL::NT)                                                            \
+  extern "C" __global__ void __launch_bounds__(TL::NT, TL::MINW)                                                  \
   NAME(const int* __restrict__ tiles, int ntile, const int* __restrict__ g2l,                                     \
        const unsigned char* __restrict__ cmask, const T* __restrict__ pc, const T* __restrict__ v,                 \
        T* __restrict__ y, const double* __restrict__ tca, int n0, int n1, int n2, const double* __restrict__ prm,  \
@@ -2052,7 +2053,7 @@ extern "C" __global__ void __launch_bounds__(256)
                     M.origin[2], M.h[0], M.h[1], M.h[2], 8, M.prm);
 }
 #define DCG_PBATCH_ENTRY(NAME, T, VEC, PC, TL)                                                                    \
-  extern "C" __global__ void __launch_bounds__(TL::NT)                                                            \
+  extern "C" __global__ void __launch_bounds__(TL::NT, TL::MINW)                                                  \
   NAME(const int2* __restrict__ work, int nwork, int part_off, const dcg_pc_member* __restrict__ mem, int xsel,   \
        int ysel, int w1sel, int w2sel, int skip, double* __restrict__ part) {                                      \
     const int c = (int)(blockIdx.x % 8u) * (int)(gridDim.x / 8u) + (int)(blockIdx.x / 8u);                       \
@@ -2101,17 +2102,30 @@ int apply_tile_threads(int dim, std::array<int, 3> t, int species) {
   return ncb <= 512 ? 512 : 1024;
 }
 
-std::vector<std::array<int, 3>> apply_tile_candidates(const Grid& g, int species, std::array<int, 3> setup_tile) {
+std::vector<std::array<int, 3>> apply_tile_candidates(const Grid& g, int species, std::array<int, 3> setup_tile,
+                                                      std::vector<int>* waves) {
   std::vector<std::array<int, 3>> out{setup_tile};
+  if (waves) waves->assign(1, 1);
   if (species > 4) return out;  // corner values in LDS: the assembly tile only
   // cell boxes of 256, 512 and 1024 cells around the rows (one cell per
   // thread): squarer boxes re-read fewer halo cells, slabs along x give
-  // longer contiguous runs
+  // longer contiguous runs; a few shapes again with registers bounded for
+  // more waves per SIMD (more workgroups resident to overlap one's barriers
+  // with another's loads)
   std::vector<std::array<int, 3>> cand;
-  if (g.dim == 2) cand = {{15, 15, 1}, {31, 7, 1}, {16, 8, 1}, {31, 15, 1}, {31, 31, 1}};
-  else if (g.dim == 3) cand = {{8, 4, 4}, {7, 7, 3}, {6, 5, 5}, {15, 3, 3}, {7, 7, 7}, {15, 7, 3}, {15, 7, 7}};
-  for (auto c : cand)
-    if (c != setup_tile) out.push_back(c);
+  std::vector<int> wv;
+  if (g.dim == 2) {
+    cand = {{15, 15, 1}, {31, 7, 1}, {16, 8, 1}, {31, 15, 1}, {31, 31, 1}, {15, 15, 1}, {31, 15, 1}};
+    wv = {1, 1, 1, 1, 1, 8, 8};
+  } else if (g.dim == 3) {
+    cand = {{8, 4, 4}, {7, 7, 3}, {6, 5, 5}, {15, 3, 3}, {7, 7, 7}, {15, 7, 3}, {15, 7, 7}, {7, 7, 7}, {8, 4, 4}};
+    wv = {1, 1, 1, 1, 1, 1, 1, 6, 6};  // (8 waves spill in 3D)
+  }
+  for (size_t i = 0; i < cand.size(); ++i)
+    if (cand[i] != setup_tile || wv[i] != 1) {
+      out.push_back(cand[i]);
+      if (waves) waves->push_back(wv[i]);
+    }
   return out;
 }
 
@@ -2148,7 +2162,7 @@ std::array<int, 3> choose_tile(const Grid& g, int bc, int nmc) {
 
 std::string gen_tiled(const ModelSpec& m, const Grid& g, int comp, const std::string& ns, const std::string& kname,
                       std::array<int, 3> tile, bool geo, bool points,
-                      const std::vector<std::array<int, 3>>& apply_tiles) {
+                      const std::vector<std::array<int, 3>>& apply_tiles, const std::vector<int>& apply_waves) {
   const auto T = element_tables(geo ? unit_cell(g) : g);
   std::ostringstream o;
   o << "namespace " << ns << " {\n";
@@ -2288,7 +2302,7 @@ std::string gen_tiled(const ModelSpec& m, const Grid& g, int comp, const std::st
       as += "using ApplyShape" + std::to_string(k2) + " = ApplyTile<" + std::to_string(apply_tiles[k2][0]) + ", " +
             std::to_string(apply_tiles[k2][1]) + ", " + std::to_string(apply_tiles[k2][2]) + ", " +
             std::to_string(apply_tile_threads(g.dim, apply_tiles[k2], static_cast<int>(m.comp_species[comp].size()))) +
-            ">;\n";
+            ", " + std::to_string(k2 < apply_waves.size() ? apply_waves[k2] : 1) + ">;\n";
     replace_all(body, "%APPLY_SHAPES%", as);
     replace_all(body, "%POINT_ENTRIES%", pe);
     replace_all(body, "%PBATCH_ENTRIES%", pb);
@@ -2316,6 +2330,7 @@ GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L, bool r
       out.volume_kernel.push_back("");
       out.tile_kernel.push_back("");
       out.apply_tiles.push_back({});
+      out.apply_waves.push_back({});
       out.tile_batch_kernel.push_back("");
       out.tile_dims.push_back({0, 0, 0});
       out.point_words.push_back(0);
@@ -2330,11 +2345,14 @@ GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L, bool r
       auto tile = choose_tile(L.grid, static_cast<int>(spec.comp_species[c].size()),
                               static_cast<int>(mc_pattern(spec, static_cast<int>(c)).size()));
       std::string tk = "dcg_tile_" + std::to_string(c);
-      const auto shapes = points ? apply_tile_candidates(L.grid, static_cast<int>(spec.comp_species[c].size()), tile)
+      std::vector<int> waves;
+      const auto shapes = points ? apply_tile_candidates(L.grid, static_cast<int>(spec.comp_species[c].size()), tile,
+                                                         &waves)
                                  : std::vector<std::array<int, 3>>{};
-      src << gen_tiled(spec, L.grid, static_cast<int>(c), ns, tk, tile, geo, points, shapes);
+      src << gen_tiled(spec, L.grid, static_cast<int>(c), ns, tk, tile, geo, points, shapes, waves);
       out.tile_kernel.push_back(tk);
       out.apply_tiles.push_back(shapes);
+      out.apply_waves.push_back(waves);
       out.tile_batch_kernel.push_back(geo && c < 4 ? tk + "_batch" : "");
       out.tile_dims.push_back(tile);
       const int nq = static_cast<int>(element_rule(L.grid.dim, L.grid.cube).w.size());
@@ -2344,6 +2362,7 @@ GeneratedSource generate_assembly(const ModelSpec& spec, const Layout& L, bool r
     } else {
       out.tile_kernel.push_back("");
       out.apply_tiles.push_back({});
+      out.apply_waves.push_back({});
       out.tile_batch_kernel.push_back("");
       out.tile_dims.push_back({0, 0, 0});
       out.point_words.push_back(0);

@@ -40,6 +40,8 @@ struct GeneratedSource {
   // <tile_kernel>_p<k>, _pf<k> and the batched _pbatch<k>, _pfbatch<k>);
   // the first is the assembly tile
   std::vector<std::vector<std::array<int, 3>>> apply_tiles;
+  // per compartment and candidate: the waves per SIMD its registers are bounded for (1: none)
+  std::vector<std::vector<int>> apply_waves;
   std::vector<FacetKernel> facet_kernels;
 };
 
@@ -54,7 +56,8 @@ constexpr int kGeoWords = 8;
 // velocity, diffusion uniform in space and independent of the state)
 bool tile_eligible(const ModelSpec& m, int comp);
 // tile shapes the point apply is generated for (the set-up tile first)
-std::vector<std::array<int, 3>> apply_tile_candidates(const Grid& g, int species, std::array<int, 3> setup_tile);
+std::vector<std::array<int, 3>> apply_tile_candidates(const Grid& g, int species, std::array<int, 3> setup_tile,
+                                                      std::vector<int>* waves = nullptr);
 // workgroup size of the point apply with tile t (cells per workgroup along x, y, z)
 int apply_tile_threads(int dim, std::array<int, 3> t, int species);
 

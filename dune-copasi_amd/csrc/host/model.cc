@@ -653,7 +653,9 @@ void Model::tune_points() {
     for (size_t c = 0; c < L_.comps.size(); ++c) {
       const int nk = static_cast<int>(pc_app_fns_[c].size());
       apply_shape_[c] = cfg_.point_tile < nk ? cfg_.point_tile : 0;
-      for (int k2 = 0; k2 < nk; ++k2) point_tuning_.push_back({static_cast<int>(c), gen_.apply_tiles[c][k2], 0.0});
+      for (int k2 = 0; k2 < nk; ++k2)
+        point_tuning_.push_back({static_cast<int>(c), gen_.apply_tiles[c][k2], gen_.apply_waves[c][k2], 0.0,
+                                 k2 == apply_shape_[c]});
     }
     return;
   }
@@ -703,10 +705,11 @@ void Model::tune_points() {
         if (round) ms[k2] += t;  // the first round warms up
       }
     size_t best = 0;
-    for (size_t k2 = 0; k2 < nk; ++k2) {
-      point_tuning_.push_back({static_cast<int>(c), gen_.apply_tiles[c][k2], 1e3 * ms[k2] / (kRounds - 1)});
+    for (size_t k2 = 1; k2 < nk; ++k2)
       if (ms[k2] < ms[best]) best = k2;
-    }
+    for (size_t k2 = 0; k2 < nk; ++k2)
+      point_tuning_.push_back({static_cast<int>(c), gen_.apply_tiles[c][k2], gen_.apply_waves[c][k2],
+                               1e3 * ms[k2] / (kRounds - 1), k2 == best});
     apply_shape_[c] = static_cast<int>(best);
   }
   (void)hipEventDestroy(e0);

@@ -179,10 +179,11 @@ class Model {
   struct PointTuning {
     int comp;
     std::array<int, 3> tile;
+    int waves;  // waves per SIMD the candidate's registers are bounded for (1: none)
     double us;
+    bool chosen;
   };
   const std::vector<PointTuning>& point_tuning() const { return point_tuning_; }
-  std::array<int, 3> point_tile(int c) const { return gen_.apply_tiles[c][apply_shape_[c]]; }
   // SpMV autotuning result: (variant, microseconds) per candidate; chosen variant
   const std::vector<std::pair<int, double>>& spmv_tuning() const { return tuning_; }
   int spmv_variant() const { return plan_.variant; }

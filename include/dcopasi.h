@@ -169,12 +169,12 @@ int dcg_spmv_tuning(dcg_model_t model, int* count, int* variant, double* microse
 
 /* Tile shapes the set-up autotuner timed for the point-coefficient apply
  * (linear_solver.storage = points): for each candidate k < *count, its
- * compartment comp[k], its cell box tile[3k..3k+2] (cells per workgroup along
- * x, y, z) and the microseconds of one apply; chosen[k] = 1 for the shape in
+ * compartment comp[k], tile[4k..4k+2] (row vertices per workgroup along x,
+ * y, z), tile[4k+3] (the waves per SIMD its registers are bounded for; 1:
+ * none) and the microseconds of one apply; chosen[k] = 1 for the candidate in
  * use.  No candidates when the model has no point storage or a single shape;
  * a shape fixed by linear_solver.point_tile lists the candidates with zero
- * times.
- * Arrays hold at least 64 entries (tile: 192). */
+ * times.  Arrays hold at least 64 entries (tile: 256). */
 int dcg_point_tuning(dcg_model_t model, int* count, int* comp, int* tile, double* microseconds, int* chosen);
 
 /* ---------------------------------------------------------------- transform / reduce

@@ -104,7 +104,7 @@ def test_batch_gray_scott_matches_single(dca, wl, precision, graphs, tile):
     b.close()
 
 
-@pytest.mark.parametrize("tile", [str(k) for k in range(8)])
+@pytest.mark.parametrize("tile", [str(k) for k in range(10)])
 def test_batch_3d_point_tiles_match_single(dca, wl, tile):
     """3D Mitchell-Schaefer members of uneven sides (Alexander2, block size 2)
     with the batched point apply forced to each candidate tile shape, against

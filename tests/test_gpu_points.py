@@ -84,7 +84,7 @@ TILE_CASES = [
 
 
 @pytest.mark.parametrize("name,make", TILE_CASES, ids=[c[0] for c in TILE_CASES])
-@pytest.mark.parametrize("tile", [str(k) for k in range(8)])
+@pytest.mark.parametrize("tile", [str(k) for k in range(10)])
 def test_point_apply_every_tile_shape(dca, wl, name, make, tile):
     """The point apply forced to each candidate tile shape (sides that no
     shape divides, so edge tiles are partial) against the oracle's assembled
