@@ -230,6 +230,7 @@ def batch_measure(args, dca, wl, precision, warmup, steps, plain_stride, cpu=Tru
     b = dca.Batch(f"precision = {precision}\ngraph_iterations = {args.graph_iterations}\n"
                   f"preconditioner = {args.preconditioner}\n"
                   f"refinement.inner_relative_tolerance = {args.ir_inner:g}\n"
+                  f"refinement.tolerance_floor = {args.ir_floor:g}\n"
                   f"point_tile = {args.point_tile}\n", 0)
     ids = []
     for w in ws:
@@ -339,6 +340,7 @@ def batch_measure(args, dca, wl, precision, warmup, steps, plain_stride, cpu=Tru
         "data": "synthetic (seeded initial states and parameters)",
         "config": {"workload": f"gray_scott_batch_{len(ws)}_2d_q1_sides32-512_implicit_euler_dt1",
                    "members": len(ws), "n_dof": n_dof, "precision": precision, "preconditioner": args.preconditioner,
+                   "refinement_tolerance_floor": args.ir_floor if precision == "mixed" else None,
                    "graph_iterations": args.graph_iterations,
                    "newton_relative_tolerance_two_norm2": args.batch_nl_rtol},
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
@@ -393,7 +395,8 @@ def serve_measure(args, dca, wl, precision, R, qps, min_steps, max_steps, plain_
     warm_s = time.perf_counter() - t_warm
     srv = dca.Server(f"precision = {precision}\nmax_members = {args.max_members}\n"
                      f"workers = {args.workers}\ngraph_iterations = {args.graph_iterations}\n"
-                     f"preconditioner = {args.preconditioner}\n", 0)
+                     f"preconditioner = {args.preconditioner}\n"
+                     f"refinement.tolerance_floor = {args.ir_floor:g}\n", 0)
     ids = [None] * R
     t0 = time.perf_counter()
     sub, last_print = 0, 0.0
@@ -469,6 +472,7 @@ def serve_measure(args, dca, wl, precision, R, qps, min_steps, max_steps, plain_
         "data": "synthetic (seeded random models and Poisson arrivals)",
         "config": {"workload": "serving_poisson_random_mass_action_2d_64-512_3d_32-96",
                    "requests": R, "qps": qps, "steps_range": [min_steps, max_steps],
+                   "refinement_tolerance_floor": args.ir_floor if precision == "mixed" else None,
                    "precision": precision, "max_members": args.max_members, "workers": args.workers},
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                       "max": float(lat.max())},
@@ -509,6 +513,9 @@ def main():
                     help="default: fp64 for single models, mixed for gs_batch and serve")
     ap.add_argument("--point-tile", default="auto",
                     help="point storage: the apply's tile shape, auto (timed at set-up) or a candidate index")
+    ap.add_argument("--ir-floor", type=float, default=1e-4,
+                    help="mixed precision: lowest relative tolerance a Newton correction is solved to (0: as requested);"
+                         " Newton's FP64 defect test still decides convergence")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true",
@@ -656,7 +663,8 @@ def main():
         u64 = u_timed
         w32, _ = make_workload(wl, args.workload, "mixed", args.preconditioner)
         m32 = dca.Model(w32.ini, storage + "model.time_step_operator.linear_solver.refinement.inner_relative_tolerance = "
-                        f"{args.ir_inner:g}\n", 0)
+                        f"{args.ir_inner:g}\nmodel.time_step_operator.linear_solver.refinement.tolerance_floor = "
+                        f"{args.ir_floor:g}\n", 0)
         m32.set_state(u0, 0.0)
         t32 = 0.0
         for k in range(args.warmup):
@@ -681,6 +689,7 @@ def main():
                 sl = slice(c["dof_offset"] + sp, c["dof_offset"] + c["n_species"] * c["n_nodes"], c["n_species"])
                 errs.append(float(np.linalg.norm(u32[sl] - u64[sl]) / max(np.linalg.norm(u64[sl]), 1e-300)))
         mixed = {"value": args.steps * w32.stages * m32.n_dof / el32, "ms_per_step": 1e3 * el32 / args.steps,
+                 "refinement_tolerance_floor": args.ir_floor,
                  "speedup_vs_fp64": (args.steps * w32.stages * m32.n_dof / el32) / value,
                  "refinements_per_step": refs / args.steps,
                  "error_vs_fp64_run": {"max_rel_l2_per_species": max(errs), "steps": args.warmup + args.steps}}

@@ -20,6 +20,8 @@ BatchOptions BatchOptions::from_ini(const std::string& text) {
   else if (pre == "None") o.precond = Preconditioner::None;
   else throw IOError("batch preconditioner must be 'Jacobi', 'VertexBlockJacobi' or 'None'");
   o.ir_inner_rtol = c.get("refinement.inner_relative_tolerance", o.ir_inner_rtol);
+  o.ir_floor = c.get("refinement.tolerance_floor", o.ir_floor);
+  if (o.ir_floor < 0.0 || o.ir_floor >= 1.0) throw IOError("refinement.tolerance_floor must be in [0, 1)");
   o.ir_max = c.get("refinement.max_steps", o.ir_max);
   o.lin_maxit = c.get("iteration_limit", o.lin_maxit);
   o.graph_iterations = c.get("graph_iterations", o.graph_iterations);
@@ -758,7 +760,7 @@ void Batch::solve(const std::vector<char>& on, const std::vector<double>& rtol, 
     const size_t M = g.mem.size();
     bool any = false;
     for (size_t j = 0; j < M; ++j) {
-      g.h_rtol[j] = rtol[g.mem[j]];
+      g.h_rtol[j] = mixed ? std::max(rtol[g.mem[j]], opt_.ir_floor) : rtol[g.mem[j]];
       any = any || on[g.mem[j]];
     }
     if (!any) continue;

@@ -35,12 +35,13 @@ struct BatchOptions {
   Precision precision = Precision::Mixed;
   Preconditioner precond = Preconditioner::Jacobi;  // Jacobi, VertexBlockJacobi or None
   double ir_inner_rtol = 1e-3;
+  double ir_floor = 0.0;  // mixed: lowest relative tolerance a correction is solved to (0: as requested)
   int ir_max = 50;
   int lin_maxit = 500;
   int graph_iterations = 4;  // BiCGSTAB iterations per captured graph; 0: plain launches
   int point_tile = -1;       // point-apply tile shape: -1 timed per launch, else the candidate index
   // keys: precision = fp64|mixed, preconditioner = Jacobi|VertexBlockJacobi|None,
-  // refinement.inner_relative_tolerance, refinement.max_steps,
+  // refinement.inner_relative_tolerance, refinement.tolerance_floor, refinement.max_steps,
   // iteration_limit, graph_iterations, point_tile (auto or an index)
   static BatchOptions from_ini(const std::string& text);
 };

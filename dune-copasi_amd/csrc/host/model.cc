@@ -229,6 +229,9 @@ void Model::build(const std::string& ini, const std::string& overrides) {
   if (lr.size() == 2) cfg_.lin_maxit = static_cast<int>(lr[1]);
   cfg_.ir_inner_rtol = c.get(p + "linear_solver.refinement.inner_relative_tolerance", 1e-3);
   cfg_.ir_max = c.get(p + "linear_solver.refinement.max_steps", 50);
+  cfg_.ir_floor = c.get(p + "linear_solver.refinement.tolerance_floor", 0.0);
+  if (cfg_.ir_floor < 0.0 || cfg_.ir_floor >= 1.0)
+    throw IOError("linear_solver.refinement.tolerance_floor must be in [0, 1)");
   cfg_.nl_rtol = c.get(p + "nonlinear_solver.convergence_condition.relative_tolerance", 1e-4);
   cfg_.nl_atol = c.get(p + "nonlinear_solver.convergence_condition.absolute_tolerance", 0.0);
   auto nr = c.get_list(p + "nonlinear_solver.convergence_condition.iteration_range");
@@ -1474,7 +1477,7 @@ int Model::krylov(const double* b, double* x, double rtol, int maxit, int* its, 
   double bn2 = 0;
   DCG_HIP(hipMemcpyAsync(&bn2, scalar_out_.get(), sizeof(double), hipMemcpyDeviceToHost, stream_));
   synchronize();
-  const double target = rtol * std::sqrt(bn2);
+  const double target = std::max(rtol, cfg_.ir_floor) * std::sqrt(bn2);
   if (bn2 == 0.0) return 0;
   double rn = std::sqrt(bn2);
   for (int k2 = 0; k2 < cfg_.ir_max; ++k2) {

@@ -75,6 +75,10 @@ struct SolverConfig {
   int point_tile = -1;
   double ir_inner_rtol = 1e-3;  // FP32 Krylov tolerance inside refinement
   int ir_max = 50;
+  // linear_solver.refinement.tolerance_floor: mixed precision solves each
+  // Newton correction to max(requested, floor) (0: the requested tolerance);
+  // Newton's own FP64 defect test still decides convergence
+  double ir_floor = 0.0;
   std::vector<std::string> warnings;
 };
 

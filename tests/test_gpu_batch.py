@@ -204,7 +204,8 @@ def _oracle_steps(w, steps, dt=None):
     return orc
 
 
-def test_batch_members_match_oracle(dca, wl):
+@pytest.mark.parametrize("floor", [0.0, 1e-4])
+def test_batch_members_match_oracle(dca, wl, floor):
     """Every member of a mixed-precision batch against the CPU oracle stepped
     alone (the reference's Newton per model, make_step_operator.hh:246-288):
     uneven Gray-Scott members up to side 128, a Mitchell-Schaefer member
@@ -216,7 +217,7 @@ def test_batch_members_match_oracle(dca, wl):
     ws.append(wl.gray_scott(n=128, seed=9, precond="Jacobi", precision="mixed", steps=steps, **NL))
     ws.append(wl.mitchell_schaefer(n=8, dt=0.5, **NL))
     ws.append(wl.two_compartments(n=24, dt=0.05, **NL))
-    b = dca.Batch("precision = mixed\n", 0)
+    b = dca.Batch(f"precision = mixed\nrefinement.tolerance_floor = {floor}\n", 0)
     ids = [b.add(w.ini) for w in ws]
     for i, w in zip(ids, ws):
         b.set_time_step(i, w.dt)

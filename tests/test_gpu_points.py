@@ -173,12 +173,15 @@ def test_point_steps_match_oracle(dca, wl, name, make, steps, precond):
     lambda wl: wl.mitchell_schaefer(n=10, cell_type="cube", nl_rtol=1e-20, nl_atol=1e-26, lin_rtol=1e-12),
     lambda wl: wl.two_compartments(n=16, cell_type="cube", nl_rtol=1e-20, nl_atol=1e-26, lin_rtol=1e-12),
 ], ids=["ms_q1", "two_q1"])
-def test_point_steps_mixed_precision(dca, wl, make):
+@pytest.mark.parametrize("floor", [0.0, 1e-4, 1e-3])
+def test_point_steps_mixed_precision(dca, wl, make, floor):
     """FP32 point-coefficient applies (with FP32 facet applies for interface
     terms) inside FP64 refinement: the states meet the parity bar against the
-    oracle's FP64 steps."""
+    oracle's FP64 steps, also when each Newton correction is solved only to
+    the tolerance floor (Newton's FP64 defect test decides convergence)."""
     w = make(wl)
-    mix = "model.time_step_operator.linear_solver.precision = mixed\n"
+    mix = ("model.time_step_operator.linear_solver.precision = mixed\n"
+           f"model.time_step_operator.linear_solver.refinement.tolerance_floor = {floor}\n")
     gpu = dca.Model(w.ini, PTS + mix, 0)
     orc = OracleModel(w.ini)
     u0 = w.initial_state(gpu)

@@ -178,6 +178,10 @@ initial.expression = {text}
      "[model.scalar_field.u]\ncompartment = d\nstorage.expression = 1\n"
      "[model.time_step_operator]\nlinear_solver.point_tile = widest\n",
      "linear_solver.point_tile must be 'auto' or a candidate index"),
+    ("[grid]\ndimension = 2\ncells = 4 4\ncell_type = cube\n[compartments]\nd.expression = 1\n"
+     "[model.scalar_field.u]\ncompartment = d\nstorage.expression = 1\n"
+     "[model.time_step_operator]\nlinear_solver.refinement.tolerance_floor = 1\n",
+     "linear_solver.refinement.tolerance_floor must be in [0, 1)"),
 ])
 def test_errors_are_reported(dca, ini, msg):
     with pytest.raises(dca.DcgError, match=re.escape(msg)):
