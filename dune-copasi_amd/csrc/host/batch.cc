@@ -285,6 +285,7 @@ void Batch::rebuild() {
             pl.tiles.push_back(m.gen_.apply_tiles[c0][k2]);
             pl.nts.push_back(apply_tile_threads(m.L_.grid.dim, m.gen_.apply_tiles[c0][k2], m.L_.comps[c0].b));
           }
+          pl.module = m.module_;
           items.emplace_back(nk);
           ranges.emplace_back(nk);
           owners.emplace_back();
@@ -315,8 +316,18 @@ void Batch::rebuild() {
         auto& pl = g.plaunch[l];
         const size_t nk = pl.f64s.size();
         size_t best = 0;
+        // a module timed at an earlier rebuild keeps its choice (members join
+        // and leave between steps; timing again each time would cost more
+        // than it finds)
+        auto known = tuned_.find(pl.module.get());
+        if (known != tuned_.end() && (known->second.first.expired() || known->second.second >= nk)) {
+          tuned_.erase(known);
+          known = tuned_.end();
+        }
         if (opt_.point_tile >= 0) {
           best = static_cast<size_t>(opt_.point_tile) < nk ? static_cast<size_t>(opt_.point_tile) : 0;
+        } else if (known != tuned_.end()) {
+          best = known->second.second;
         } else if (nk > 1) {
           std::vector<DeviceArray<int2>> dw(nk);
           for (size_t k2 = 0; k2 < nk; ++k2) dw[k2].upload(items[l][k2], stream_);
@@ -343,6 +354,7 @@ void Batch::rebuild() {
             }
           for (size_t k2 = 1; k2 < nk; ++k2)
             if (ms[k2] < ms[best]) best = k2;
+          tuned_[pl.module.get()] = {pl.module, best};
         }
         pl.f64 = pl.f64s[best];
         pl.f32 = pl.f32s[best];

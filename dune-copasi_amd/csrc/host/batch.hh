@@ -130,6 +130,7 @@ class Batch {
       // the kernels and tile shape in use, chosen by timing among the
       // module's candidate tile shapes
       hipFunction_t f64 = nullptr, f32 = nullptr;
+      std::shared_ptr<RtcModule> module;
       std::array<int, 3> tile{};
       int nt = 256;  // workgroup size
       std::vector<hipFunction_t> f64s, f32s;
@@ -194,6 +195,8 @@ class Batch {
   int next_id_ = 0;
   bool dirty_ = true;
   std::vector<SolverGroup> sgroups_;
+  // point-apply candidate chosen per generated module, kept across rebuilds
+  std::map<const RtcModule*, std::pair<std::weak_ptr<RtcModule>, size_t>> tuned_;
   std::vector<AsmGroup> agroups_;
   DeviceArray<int> status_;
   // (t, cm, ca, on) per member for each call of the current step

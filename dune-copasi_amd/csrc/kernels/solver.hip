@@ -12,6 +12,8 @@
 // results do not depend on scheduling.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "kernels.hh"
 #include "common.hh"
 #include "../host/error.hh"
@@ -1163,11 +1165,13 @@ void ilu_factor_sweep(int B, const IluSweep& t, const int* rp, const int* col, c
 template <int BB, typename V>
 void launch_ilu_chunk(size_t lds, const IluSweep& t, const double* lu, const double* dinv, const V* in, V* tmp, V* out,
                       const BicgScalars* sc, hipStream_t s) {
-  static size_t raised = 64 * 1024;
-  if (lds > raised) {
+  static std::atomic<size_t> raised{64 * 1024};
+  if (lds > raised.load()) {
     DCG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&ilu_apply_chunk_k<BB, V>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    raised = lds;
+    size_t cur = raised.load();
+    while (lds > cur && !raised.compare_exchange_weak(cur, lds)) {
+    }
   }
   ilu_apply_chunk_k<BB, V><<<1, 256, lds, s>>>(t, lu, dinv, in, tmp, out, sc);
 }
