@@ -1015,9 +1015,10 @@ Model::KrylovWork Model::own_work() {
 // Tables of the one-workgroup ILU0 sweeps from the level schedules (see
 // k::IluSweep): per row position its off-diagonal blocks and the LDS ring
 // slot of each block's column; the levels in chunks for the staged apply;
-// per lower block the factorisation's update pairs, in the order a merge of
-// the two sorted rows visits them; and the LDS sizes that decide which
-// kernels run (the plain sweeps where the stages do not fit).
+// the factorisation's update pairs per lower block (in the order a merge of
+// the two sorted rows visits them) laid out as each row's inbox slots and
+// pushes; and the LDS sizes that decide which kernels run (the plain sweeps
+// where the stages do not fit).
 void Model::build_ilu_sweep_tables(const std::vector<int>& lrows, const std::vector<int>& urows) {
   const long ns = L_.nslot;
   const int B = L_.B;
@@ -1079,7 +1080,6 @@ void Model::build_ilu_sweep_tables(const std::vector<int>& lrows, const std::vec
     for (int p2 = dp[i]; p2 < rp[i + 1]; ++p2) uptr[p2 + 1] = static_cast<int>(mq.size());
     maxpairs = std::max(maxpairs, static_cast<int>(mq.size() - row_first));
   }
-  if (mq.empty()) mq.push_back(make_int2(0, 0));  // a device array of at least one entry
   // the staged apply: the largest chunk length whose stage and ring fit LDS
   auto entries = [&](int ring, const std::vector<int>& ptr, const std::vector<int>& rows,
                      const std::vector<int>& lev, const std::vector<int>& pos, bool lower, std::vector<int>& ent,
@@ -1989,7 +1989,7 @@ size_t Model::device_bytes() const {
              val_.bytes() + lu_.bytes() + dinv_.bytes() + val32_.bytes() + part_.bytes() + scalar_out_.bytes() +
              sc_.bytes() + status_.bytes() + lo_rows_.bytes() + up_rows_.bytes() + pdiag_.bytes() + diag_id_.bytes() +
              lin_x_.bytes() + lo_ent_.bytes() + lo_blk_.bytes() + lo_src_.bytes() + up_ent_.bytes() + up_blk_.bytes() +
-             up_src_.bytes() + upd_ptr_.bytes() + upd_mq_.bytes() + lo_chunk_.bytes() + up_chunk_.bytes() +
+             up_src_.bytes() + lo_chunk_.bytes() + up_chunk_.bytes() +
              ilu_frow_.bytes() + ilu_fub_.bytes() + ilu_fm_.bytes() + ilu_fpush_.bytes() + ilu_packed_.bytes() +
              ilu_inbox_.bytes();
   for (const auto& v : pc_) b += v.bytes();

@@ -381,9 +381,9 @@ class Model {
   bool ilu_sweep_ = false;
   static constexpr int kIluSweepWidth = 4096;
   // the one-workgroup sweeps' tables (entry lists in level order, LDS ring
-  // slots, the factorisation's update pairs) and their device view
-  DeviceArray<int> lo_ent_, lo_blk_, lo_src_, up_ent_, up_blk_, up_src_, upd_ptr_, lo_chunk_, up_chunk_;
-  DeviceArray<int2> upd_mq_, ilu_fpush_;
+  // slots, the factorisation's operand positions) and their device view
+  DeviceArray<int> lo_ent_, lo_blk_, lo_src_, up_ent_, up_blk_, up_src_, lo_chunk_, up_chunk_;
+  DeviceArray<int2> ilu_fpush_;
   DeviceArray<int4> ilu_frow_;
   DeviceArray<int> ilu_fub_, ilu_fm_;
   DeviceArray<double> ilu_packed_, ilu_inbox_;

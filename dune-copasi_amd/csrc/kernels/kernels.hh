@@ -247,9 +247,6 @@ void ilu_factor(int B, int nrows, const int* rows, const int* rp, const int* col
 //   stage (at most stage_ent blocks, stage_rows rows, stage_levels levels);
 //   ring, width: ring levels and the widest level (chunked: the apply runs
 //   ilu_apply_chunk_k, else the plain one-workgroup sweep);
-//   upd_ptr[p] .. upd_ptr[p+1]: for lower block p of row i (column k), the
-//   factorisation's updates (m, q) = (block of row i, block of row k's upper
-//   part) with equal columns, in column order;
 //   inbox (blocks of at most 2 species and rows within the kIluReg* bounds):
 //   the factorisation reads each row's blocks from `packed` and its
 //   operands from its `inbox` slots, both at fixed positions in level order,
@@ -263,9 +260,7 @@ struct IluSweep {
   int nlo_chunk = 0, nup_chunk = 0, stage_ent = 0, stage_rows = 0, stage_levels = 0, ring = 0, width = 0;
   const int *lo_chunk = nullptr, *lo_ent = nullptr, *lo_blk = nullptr, *lo_src = nullptr;
   const int *up_chunk = nullptr, *up_ent = nullptr, *up_blk = nullptr, *up_src = nullptr;
-  int factor_threads = 0, maxrow = 0;
-  const int* upd_ptr = nullptr;
-  const int2* upd_mq = nullptr;
+  int factor_threads = 0;
   bool inbox_path = false;
   int nrows = 0;
   const int4* frow = nullptr;
