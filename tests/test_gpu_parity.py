@@ -8,7 +8,7 @@ per species (the parity bar), linear solves to their tolerance.
 import numpy as np
 import pytest
 
-from tests.oracle_lib import OracleModel, bcsr_to_dense_csr, rel_l2_per_species
+from tests.oracle_lib import OracleModel, bcsr_to_dense_csr, rel_l2_per_species, rel_mass_l2_per_species
 
 pytestmark = pytest.mark.gpu
 
@@ -223,6 +223,9 @@ def test_time_steps_match_oracle(dca, wl, name, make, steps):
         err = max(rel_l2_per_species(gpu.compartments(), ug, uo))
         assert err <= 1e-8, (k, err)
         assert abs(tg - t) < 1e-12
+    # the survey's accuracy measure: per species, relative, in the mass-matrix norm
+    errm = max(rel_mass_l2_per_species(orc, ug, uo, t))
+    assert errm <= 1e-8, errm
 
 
 @pytest.mark.parametrize("dim,cell_type,n", [(2, "cube", 33), (2, "simplex", 27), (3, "cube", 10)])
