@@ -1183,13 +1183,15 @@ void Model::build_ilu_sweep_tables(const std::vector<int>& lrows, const std::vec
     const int BB = B * B;
     const size_t N = static_cast<size_t>(ns);
     std::vector<int4> frow(ns);
-    std::vector<int> fub(N * (ML + 1), 0), fm(N * MU, 0);
+    constexpr int FS = k::kIluFubStride;
+    static_assert(FS >= ML + 1 && FS % 4 == 0 && MU % 4 == 0 && MP % 2 == 0, "16-byte table rows");
+    std::vector<int> fub(N * FS, 0), fm(N * MU, 0);
     std::vector<std::vector<int2>> pushes(ns);  // by pushing row
     for (long r = 0; r < ns; ++r) {
       const int i = lrows[r];
       const int p0 = rp[i], nl = dp[i] - rp[i], u0 = uptr[p0], nu = uptr[dp[i]] - u0;
       frow[r] = make_int4(i, p0, rp[i + 1] - p0, nl);
-      for (int a = 0; a <= ML; ++a) fub[r * (ML + 1) + a] = a <= nl ? uptr[p0 + a] - u0 : nu;
+      for (int a = 0; a <= ML; ++a) fub[r * FS + a] = a <= nl ? uptr[p0 + a] - u0 : nu;
       for (int a = 0; a < nl; ++a) {
         const int k2 = col[p0 + a];
         pushes[k2].push_back(make_int2(-1, static_cast<int>((r * IN + a) * BB)));

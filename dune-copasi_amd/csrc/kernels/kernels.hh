@@ -253,6 +253,7 @@ void ilu_factor(int B, int nrows, const int* rows, const int* rp, const int* col
 //   with the tables frow (row, first block, blocks, lower blocks), fub, fm
 //   and fpush (see ilu_factor_inbox_k); factor_threads threads.
 constexpr int kIluRegLower = 4, kIluRegPairs = 16, kIluRegRow = 9, kIluRegPush = 24;
+constexpr int kIluFubStride = 8;  // ints per row of fub (kIluRegLower + 1, padded to 16-byte loads)
 struct IluSweep {
   int nlo = 0, nup = 0;
   const int *lo_lev = nullptr, *lo_rows = nullptr, *up_lev = nullptr, *up_rows = nullptr;

@@ -594,21 +594,64 @@ __global__ void __launch_bounds__(256) ilu_factor_inbox_k(IluSweep t, const doub
       const int4 f = t.frow[r];  // row, first block, blocks, lower blocks
       const int nl = f.w;
       // every load of the row first: its blocks, its inbox, its tables
+      // (16-byte loads where the layout allows: a single workgroup moves
+      // every byte of the sweep through one CU, so the instruction count of
+      // these per-lane gathers bounds a level)
       double rv[MR * BB], in[IN][BB];
-      int ub[ML + 1], fm[MU];
+      int ub[kIluFubStride], fm[MU];
       int2 push[MP];
+      if constexpr (BB % 2 == 0) {
+        const double2* pr = reinterpret_cast<const double2*>(packed + (size_t)r * MR * BB);
 #pragma unroll
-      for (int j = 0; j < MR * BB; ++j) rv[j] = j < f.z * BB ? packed[(size_t)r * MR * BB + j] : 0.0;
+        for (int j = 0; j < MR * BB / 2; ++j) {
+          const double2 v = 2 * j < f.z * BB ? pr[j] : make_double2(0.0, 0.0);
+          rv[2 * j] = v.x;
+          rv[2 * j + 1] = v.y;
+        }
+        const double2* pi = reinterpret_cast<const double2*>(inbox + (size_t)r * IN * BB);
 #pragma unroll
-      for (int j = 0; j < IN; ++j)
+        for (int j = 0; j < IN; ++j)
 #pragma unroll
-        for (int c = 0; c < BB; ++c) in[j][c] = inbox[((size_t)r * IN + j) * BB + c];
+          for (int c = 0; c < BB / 2; ++c) {
+            const double2 v = pi[j * BB / 2 + c];
+            in[j][2 * c] = v.x;
+            in[j][2 * c + 1] = v.y;
+          }
+      } else {
 #pragma unroll
-      for (int a = 0; a <= ML; ++a) ub[a] = t.fub[(size_t)r * (ML + 1) + a];
+        for (int j = 0; j < MR * BB; ++j) rv[j] = j < f.z * BB ? packed[(size_t)r * MR * BB + j] : 0.0;
 #pragma unroll
-      for (int u = 0; u < MU; ++u) fm[u] = t.fm[(size_t)r * MU + u];
+        for (int j = 0; j < IN; ++j)
 #pragma unroll
-      for (int j = 0; j < MP; ++j) push[j] = t.fpush[(size_t)r * MP + j];
+          for (int c = 0; c < BB; ++c) in[j][c] = inbox[((size_t)r * IN + j) * BB + c];
+      }
+      {
+        const int4* pu = reinterpret_cast<const int4*>(t.fub + (size_t)r * kIluFubStride);
+#pragma unroll
+        for (int a = 0; a < kIluFubStride / 4; ++a) {
+          const int4 v = pu[a];
+          ub[4 * a] = v.x;
+          ub[4 * a + 1] = v.y;
+          ub[4 * a + 2] = v.z;
+          ub[4 * a + 3] = v.w;
+        }
+        const int4* pm = reinterpret_cast<const int4*>(t.fm + (size_t)r * MU);
+#pragma unroll
+        for (int u = 0; u < MU / 4; ++u) {
+          const int4 v = pm[u];
+          fm[4 * u] = v.x;
+          fm[4 * u + 1] = v.y;
+          fm[4 * u + 2] = v.z;
+          fm[4 * u + 3] = v.w;
+        }
+        const int4* pp = reinterpret_cast<const int4*>(t.fpush + (size_t)r * MP);
+#pragma unroll
+        for (int j = 0; j < MP / 2; ++j) {
+          const int4 v = pp[j];
+          push[2 * j] = make_int2(v.x, v.y);
+          push[2 * j + 1] = make_int2(v.z, v.w);
+        }
+      }
 #pragma unroll
       for (int j = 0; j < MR * BB; ++j) my[j] = rv[j];
 #pragma unroll
@@ -634,17 +677,33 @@ __global__ void __launch_bounds__(256) ilu_factor_inbox_k(IluSweep t, const doub
       }
       double inv[BB];
       ilu_invert_block<B>(my + nl * BB, inv, status);
-      // stores: the row's blocks, its inverse, its pushes, the apply's copies
-      for (int j = 0; j < f.z * BB; ++j) lu[(size_t)f.y * BB + j] = my[j];
+      // stores: the row's blocks, its inverse, its pushes
+      if constexpr (BB % 2 == 0) {
+        double2* lr = reinterpret_cast<double2*>(lu + (size_t)f.y * BB);
+        for (int j = 0; j < f.z * BB / 2; ++j) lr[j] = make_double2(my[2 * j], my[2 * j + 1]);
+        double2* di = reinterpret_cast<double2*>(dinv + (size_t)f.x * BB);
 #pragma unroll
-      for (int c = 0; c < BB; ++c) dinv[(size_t)f.x * BB + c] = inv[c];
+        for (int c = 0; c < BB / 2; ++c) di[c] = make_double2(inv[2 * c], inv[2 * c + 1]);
 #pragma unroll
-      for (int k = 0; k < MP; ++k)
-        if (push[k].y >= 0) {
-          const double* v = push[k].x < 0 ? inv : my + push[k].x * BB;
+        for (int k = 0; k < MP; ++k)
+          if (push[k].y >= 0) {
+            const double* v = push[k].x < 0 ? inv : my + push[k].x * BB;
+            double2* d = reinterpret_cast<double2*>(inbox + (size_t)push[k].y);
 #pragma unroll
-          for (int c = 0; c < BB; ++c) inbox[(size_t)push[k].y + c] = v[c];
-        }
+            for (int c = 0; c < BB / 2; ++c) d[c] = make_double2(v[2 * c], v[2 * c + 1]);
+          }
+      } else {
+        for (int j = 0; j < f.z * BB; ++j) lu[(size_t)f.y * BB + j] = my[j];
+#pragma unroll
+        for (int c = 0; c < BB; ++c) dinv[(size_t)f.x * BB + c] = inv[c];
+#pragma unroll
+        for (int k = 0; k < MP; ++k)
+          if (push[k].y >= 0) {
+            const double* v = push[k].x < 0 ? inv : my + push[k].x * BB;
+#pragma unroll
+            for (int c = 0; c < BB; ++c) inbox[(size_t)push[k].y + c] = v[c];
+          }
+      }
     }
     __syncthreads();
   }
