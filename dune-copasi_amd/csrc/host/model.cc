@@ -1267,7 +1267,6 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
   host.maxit = maxit;
   DCG_HIP(hipMemcpyAsync(sc, &host, sizeof host, hipMemcpyHostToDevice, stream_));
   const int nred = std::max(1, std::min(1024, static_cast<int>((n + 255) / 256)));
-  const int nblk = (ns + 255) / 256;
   k::bicg_init<double>(n, b, x, r, rh, p, v, part, nred, sc, stream_);
   const double* dinv = jacobi_kind() ? w.dinv : nullptr;
   // the reference's Jacobi inverts the scalar diagonal: B values per slot
@@ -1326,8 +1325,8 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
     k::bicg_s<double>(B, ns, r, v, s, sh, dinv, point_jacobi, sc, w.spart, stream_);
     if (ilu) apply_ilu(s, sh);
     matvec(sh, t, s, t, 3);
-    k::bicg_xr<double>(B, ns, x, ph, sh, s, t, r, rh, sc, part, stream_);
-    k::bicg_fin(nblk, part, 2, 4, sc, stream_);
+    const int nxr = k::bicg_xr<double>(B, ns, x, ph, sh, s, t, r, rh, sc, part, stream_);
+    k::bicg_fin(nxr, part, 2, 4, sc, stream_);
   };
   // iterations run in batches between read-backs of the scalars: the first
   // as many as this solve took in the last time step (or the last solve)
@@ -1371,7 +1370,6 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
   host.maxit = maxit;
   DCG_HIP(hipMemcpyAsync(sc_.get(), &host, sizeof host, hipMemcpyHostToDevice, stream_));
   const int nred = std::max(1, std::min(1024, static_cast<int>((n + 255) / 256)));
-  const int nblk = (ns + 255) / 256;
   k::bicg_init<float>(n, b, x, r, rh, p, v, part_.get(), nred, sc_.get(), stream_);
   const double* dinv = jacobi_kind() ? dinv_.get() : nullptr;
   const bool point_jacobi = cfg_.precond == Preconditioner::Jacobi;
@@ -1419,8 +1417,8 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
     k::bicg_s<float>(B, ns, r, v, s, sh, dinv, point_jacobi, sc_.get(), spart, stream_);
     if (ilu) apply_ilu(s, sh);
     matvec(sh, t, s, t, 3);
-    k::bicg_xr<float>(B, ns, x, ph, sh, s, t, r, rh, sc_.get(), part_.get(), stream_);
-    k::bicg_fin(nblk, part_.get(), 2, 4, sc_.get(), stream_);
+    const int nxr = k::bicg_xr<float>(B, ns, x, ph, sh, s, t, r, rh, sc_.get(), part_.get(), stream_);
+    k::bicg_fin(nxr, part_.get(), 2, 4, sc_.get(), stream_);
   };
   // iterations run in batches between read-backs of the scalars: the first
   // as many as this solve took in the last time step (or the last solve)

@@ -356,7 +356,11 @@ __global__ void __launch_bounds__(kThreads) dot_parts_k(long n, const V* __restr
   }
 }
 
-// x += alpha ph + omega sh ; r = s - omega t ; partial (rh.r), (r.r)
+// x += alpha ph + omega sh ; r = s - omega t ; partial (rh.r), (r.r).
+// Each thread updates kXrSlots slots (kThreads apart, so every pass of the
+// workgroup is contiguous): more bytes in flight per thread and a quarter of
+// the workgroup reductions of one slot per thread.
+constexpr int kXrSlots = 4;
 template <int B, typename V>
 __global__ void __launch_bounds__(kThreads) bicg_xr_k(int nslot, V* __restrict__ x, const V* __restrict__ ph,
                                                       const V* __restrict__ sh, const V* __restrict__ sv,
@@ -365,11 +369,13 @@ __global__ void __launch_bounds__(kThreads) bicg_xr_k(int nslot, V* __restrict__
                                                       double* __restrict__ part) {
   if (finished(sc)) return;
   __shared__ double red[32];
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[2] = {0.0, 0.0};
   const bool half = sc->half != 0;
-  if (s < nslot) {
-    const double alpha = sc->alpha, omega = sc->omega;
+  const double alpha = sc->alpha, omega = sc->omega;
+#pragma unroll
+  for (int u = 0; u < kXrSlots; ++u) {
+    const int s = (blockIdx.x * kXrSlots + u) * blockDim.x + threadIdx.x;
+    if (s >= nslot) break;
 #pragma unroll
     for (int i = 0; i < B; ++i) {
       const size_t k = (size_t)s * B + i;
@@ -1176,14 +1182,17 @@ template void bicg_s<float>(int, int, const float*, const float*, float*, float*
                             const BicgScalars*, double*, hipStream_t);
 
 template <typename V>
-void bicg_xr(int B, int nslot, V* x, const V* ph, const V* sh, const V* sv, const V* t, V* r, const V* rh,
-             const BicgScalars* sc, double* part, hipStream_t s) {
-  DCG_DISPATCH_B(B, (bicg_xr_k<BB, V><<<grid_for(nslot), kThreads, 0, s>>>(nslot, x, ph, sh, sv, t, r, rh, sc, part)));
+int bicg_xr(int B, int nslot, V* x, const V* ph, const V* sh, const V* sv, const V* t, V* r, const V* rh,
+            const BicgScalars* sc, double* part, hipStream_t s) {
+  const int grid = (nslot + kThreads * kXrSlots - 1) / (kThreads * kXrSlots);
+  if (grid > 0)
+    DCG_DISPATCH_B(B, (bicg_xr_k<BB, V><<<grid, kThreads, 0, s>>>(nslot, x, ph, sh, sv, t, r, rh, sc, part)));
+  return grid;
 }
-template void bicg_xr<double>(int, int, double*, const double*, const double*, const double*, const double*,
-                              double*, const double*, const BicgScalars*, double*, hipStream_t);
-template void bicg_xr<float>(int, int, float*, const float*, const float*, const float*, const float*, float*,
-                             const float*, const BicgScalars*, double*, hipStream_t);
+template int bicg_xr<double>(int, int, double*, const double*, const double*, const double*, const double*, double*,
+                             const double*, const BicgScalars*, double*, hipStream_t);
+template int bicg_xr<float>(int, int, float*, const float*, const float*, const float*, const float*, float*,
+                            const float*, const BicgScalars*, double*, hipStream_t);
 
 void bicg_fin(int nparts, const double* part, int stride, int stage, BicgScalars* sc, hipStream_t s) {
   bicg_fin_k<<<1, 1024, 0, s>>>(nparts, part, stride, stage, sc);
