@@ -536,8 +536,8 @@ def main():
     ap.add_argument("--plain-stride", type=int, default=1,
                     help="gs_batch: time the plain FP64 path on every n-th model by size (stratified sample)")
     ap.add_argument("--no-batch", action="store_true",
-                    help="default line: skip the batched-models object (config 3 sample)")
-    ap.add_argument("--no-serve", action="store_true", help="default line: skip the serving object (config 4 sample)")
+                    help="default line: skip the batched-models object (256 Gray-Scott models)")
+    ap.add_argument("--no-serve", action="store_true", help="default line: skip the serving object (12 random requests)")
     ap.add_argument("--line-batch-steps", type=int, default=3, help="default line: timed batch steps")
     ap.add_argument("--line-plain-stride", type=int, default=4,
                     help="default line: stratified sample of the plain FP64 path beside the batch")

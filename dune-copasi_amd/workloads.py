@@ -89,7 +89,7 @@ def _solver_block(rk, precond, nl_rtol, nl_atol, lin_rtol, precision="fp64", max
 def gray_scott(n=128, cell_type="cube", seed=0, F=0.04, k=0.06, Du=2e-5, Dv=1e-5, precond="ILU",
                nl_rtol=1e-10, nl_atol=0.0, lin_rtol=1e-10, precision="fp64", steps=200, dt=1.0,
                constants=False) -> Workload:
-    """Gray-Scott on [0,1]^2, zero flux, implicit Euler (benchmark config 0 at n = 128).
+    """Gray-Scott on [0,1]^2, zero flux, implicit Euler (the ILU0 benchmark case at n = 128).
 
     constants=True writes F, k, Du, Dv as parser_context constants (the form a
     parameter sweep uses); the model is the same."""
@@ -143,7 +143,7 @@ reaction.jacobian.v.expression = 2*u*v - ({sF}+{sk})
 
 def gray_scott_batch(count=256, seed=0, side_min=32, side_max=512, precision="mixed", precond="Jacobi",
                      nl_rtol=1e-10, nl_atol=0.0, lin_rtol=1e-10, steps=100, dt=1.0, cell_type="cube"):
-    """Independent Gray-Scott instances (benchmark config 3 at count = 256):
+    """Independent Gray-Scott instances (the batch benchmark at count = 256):
     sides log-uniform in [side_min, side_max], F ~ U[0.01, 0.08],
     k ~ U[0.04, 0.07], D_u / D_v ~ U[1.5, 3] with D_v = 1e-5, initial state as
     in gray_scott seeded per instance, implicit Euler."""
@@ -165,7 +165,7 @@ def gray_scott_batch(count=256, seed=0, side_min=32, side_max=512, precision="mi
 def mitchell_schaefer(n=96, cell_type="cube", seed=0, D=1e-3, tau_in=0.3, tau_out=6.0, tau_open=120.0,
                       tau_close=150.0, u_gate=0.13, precond="Jacobi", nl_rtol=1e-12, nl_atol=0.0,
                       lin_rtol=1e-10, precision="fp64", steps=500, dt=0.1) -> Workload:
-    """Two-species excitable model on [0,1]^3, Alexander2 (benchmark config 1 at n = 96)."""
+    """Two-species excitable model on [0,1]^3, Alexander2 (the headline benchmark at n = 96)."""
     ini = f"""[grid]
 dimension = 3
 cells = {n} {n} {n}
@@ -222,7 +222,7 @@ reaction.jacobian.h.expression = (u < u_gate) ? -1/tau_open : -1/tau_close
 def two_compartments(n=1024, cell_type="cube", seed=0, kt=0.5, precond="Jacobi", nl_rtol=1e-12, nl_atol=0.0,
                      lin_rtol=1e-10, precision="fp64", steps=400, dt=0.05) -> Workload:
     """Inner disk r < 0.3 and its complement, 4 species each, A+B<->C, C->D inside,
-    D->A outside, transmission k_t (D_in - A_out) (benchmark config 2 at n = 1024)."""
+    D->A outside, transmission k_t (D_in - A_out) (the two-compartment benchmark at n = 1024)."""
     ds = np.exp(np.log(1e-4) + (np.log(1e-2) - np.log(1e-4)) * uniform01(seed, 99, np.arange(8)))
     names_in = ["A_in", "B_in", "C_in", "D_in"]
     names_out = ["A_out", "B_out", "C_out", "D_out"]
@@ -282,7 +282,7 @@ outer.expression = (position_x-0.5)^2 + (position_y-0.5)^2 >= 0.09
 # ------------------------------------------------------------- serving requests
 def random_request(seed: int, precision="mixed", nl_rtol=1e-20, nl_atol=0.0, side2=(64, 512), side3=(32, 96),
                    steps_range=(50, 500), dt=0.05) -> Workload:
-    """One seeded random model of the serving stream (benchmark config 4):
+    """One seeded random model of the serving benchmark's request stream:
     1 or 2 compartments (inner disk / ball of radius 0.3 and its complement)
     on a 2D Q1 mesh with side log-uniform in side2 or a 3D Q1 mesh with side
     log-uniform in side3; 2-8 species; a random mass-action network of 2-12
