@@ -584,9 +584,12 @@ void Batch::point_apply(int gi, bool fp32, int x, int y, int w1, int w2, double*
 
 // ------------------------------------------------------------------ batched operations
 void Batch::push_flags(const std::vector<char>& res, const std::vector<char>& jac) {
+  // flags go to the device only when they change (no kernel writes them)
   for (auto& g : sgroups_) {
     for (size_t j = 0; j < g.mem.size(); ++j) g.h_act[j] = res[g.mem[j]] ? 1 : 0;
+    if (g.h_act == g.dev_act) continue;
     DCG_HIP(hipMemcpyAsync(g.act.get(), g.h_act.data(), g.h_act.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
+    g.dev_act = g.h_act;
   }
   for (auto& a : agroups_) {
     const size_t M = a.mem.size();
@@ -594,7 +597,9 @@ void Batch::push_flags(const std::vector<char>& res, const std::vector<char>& ja
       a.h_act[j] = res[a.mem[j]] ? 1 : 0;
       a.h_act[M + j] = jac[a.mem[j]] ? 1 : 0;
     }
+    if (a.h_act == a.dev_act) continue;
     DCG_HIP(hipMemcpyAsync(a.act.get(), a.h_act.data(), a.h_act.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
+    a.dev_act = a.h_act;
   }
 }
 
@@ -776,7 +781,10 @@ void Batch::solve(const std::vector<char>& on, const std::vector<double>& rtol, 
       any = any || on[g.mem[j]];
     }
     if (!any) continue;
-    DCG_HIP(hipMemcpyAsync(g.rtol.get(), g.h_rtol.data(), M * sizeof(double), hipMemcpyHostToDevice, stream_));
+    if (g.h_rtol != g.dev_rtol) {
+      DCG_HIP(hipMemcpyAsync(g.rtol.get(), g.h_rtol.data(), M * sizeof(double), hipMemcpyHostToDevice, stream_));
+      g.dev_rtol = g.h_rtol;
+    }
     const k::Skip off{g.act.get(), 1, 0};
     if (opt_.precond != Preconditioner::None)
       k::b_block_inverse(g.B, g.w, g.mv.get(), off, status_.get(), opt_.precond == Preconditioner::Jacobi, stream_);

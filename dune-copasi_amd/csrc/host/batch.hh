@@ -114,8 +114,8 @@ class Batch {
     DeviceArray<int> act;
     DeviceArray<double> rtol, sums;
     DeviceArray<double> part;
-    std::vector<int> h_act;
-    std::vector<double> h_rtol;
+    std::vector<int> h_act, dev_act;  // host flags, and what the device holds
+    std::vector<double> h_rtol, dev_rtol;
     int variant64 = 0, variant32 = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // FP64, FP32 iteration blocks
     int est_its = 8;
@@ -158,7 +158,7 @@ class Batch {
     std::vector<int> nwork;
     DeviceArray<double> coef;  // [call][member][4]
     DeviceArray<int> act;      // [2][member]: residual, Jacobian
-    std::vector<int> h_act;
+    std::vector<int> h_act, dev_act;
     std::vector<double> h_coef;
   };
   static constexpr int kCalls = 5;  // hist(1,0), hist(2,0), hist(2,1), stage 1, stage 2
