@@ -1341,9 +1341,20 @@ __device__ __forceinline__ void cells_apply(CP cellp, size_t WS, const T* vs, co
     // one cell per thread; its corner sums are added to the vertices corner
     // by corner: for one corner the cells' vertices are distinct, so no two
     // threads add to the same vertex between two barriers, and every vertex
-    // sums its cells in corner order
+    // sums its cells in a fixed order.  When a box row of cells lies within
+    // one wave (CXB divides 64), the x neighbour's corner sums arrive by lane
+    // shuffle instead: lane ci holds vertex x = ci's sums of cells ci-1 and
+    // ci (own first), which halves the rounds; when a whole xy layer lies
+    // within one wave the y neighbour's arrive the same way (CXB lanes away)
+    // and the rounds halve again.  Only the tile's own rows are read back,
+    // so the halo vertices (x = 0 and x = CXB, likewise in y) need no sums.
+    constexpr bool CMB_X = DIM > 1 && 64 % CXB == 0;
+    constexpr bool CMB_Y = CMB_X && 64 % (CXB * CYB) == 0;
+    constexpr int NFOLD = CMB_Y ? 4 : CMB_X ? 2 : 1;  // corners folded into one round
+    constexpr int NR = NCORNER / NFOLD;             // rounds
     const int cl = threadIdx.x;
-    const bool has = cl < NCB && cms[cl < NCB ? cl : 0];
+    const bool in = cl < NCB;
+    const bool has = in && cms[in ? cl : 0];
     T oc[NCORNER][BC];
 #pragma unroll
     for (int c = 0; c < NCORNER; ++c)
@@ -1351,13 +1362,40 @@ __device__ __forceinline__ void cells_apply(CP cellp, size_t WS, const T* vs, co
       for (int i = 0; i < BC; ++i) oc[c][i] = T(0);
     if (has) cell_corner_sums(cl, cms[cl], [&](int c, int i, T v2) { oc[c][i] += v2; });
     const int ci = cl % CXB, cj = (cl / CXB) % CYB, ck = cl / (CXB * CYB);
+    // rs[r]: the sums of round r's vertex; round r takes corner r * NFOLD
+    T rs[NR][BC];
+    if constexpr (CMB_X) {
+      T fx[NCORNER / 2][BC];
 #pragma unroll
-    for (int c = 0; c < NCORNER; ++c) {
-      if (c) __syncthreads();
-      if (has) {
+      for (int h = 0; h < NCORNER / 2; ++h)
+#pragma unroll
+        for (int i = 0; i < BC; ++i) fx[h][i] = oc[2 * h][i] + __shfl_up(oc[2 * h + 1][i], 1, 64);
+      if constexpr (CMB_Y) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+          for (int i = 0; i < BC; ++i) rs[r][i] = fx[2 * r][i] + __shfl_up(fx[2 * r + 1][i], CXB, 64);
+      } else {
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+          for (int i = 0; i < BC; ++i) rs[r][i] = fx[r][i];
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+#pragma unroll
+        for (int i = 0; i < BC; ++i) rs[r][i] = oc[r][i];
+    }
+    const bool add = CMB_Y ? in && ci > 0 && cj > 0 : CMB_X ? in && ci > 0 : has;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      if (r) __syncthreads();
+      if (add) {
+        const int c = r * NFOLD;
         const int b = (ci + (c & 1)) + XB * ((cj + ((c >> 1) & 1)) + YB * (ck + ((c >> 2) & 1)));
 #pragma unroll
-        for (int i = 0; i < BC; ++i) accv[b * BC + i] += oc[c][i];
+        for (int i = 0; i < BC; ++i) accv[b * BC + i] += rs[r][i];
       }
     }
   } else {
