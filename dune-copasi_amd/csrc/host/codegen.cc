@@ -1227,9 +1227,9 @@ struct ApplyTile {
   static constexpr int NVB = XB * YB * ZB, NCB = CXB * CYB * CZB, NROW = TX * TY * TZ;
 };
 %APPLY_SHAPES%
-// The rows of one tile from point coefficients cellp(box cell)[((p*NQ + q)*NMC +
-// n)*WS] (the grid-wide array in HBM, or the set-up's staged LDS array) and the
-// direction staged in vs over the vertex box (slots sl, class masks cms): per
+// The rows of one tile from point coefficients coef(box cell, (p*NQ + q)*NMC +
+// n) (read from the grid-wide array in HBM, or the set-up's staged LDS array)
+// and the direction staged in vs over the vertex box (slots sl, class masks cms): per
 // cell, the direction at its corners interpolated to each point, multiplied
 // by the point's block and tested with the basis functions, plus the uniform
 // diffusion's element stiffness times the corner values; the cells' corner
@@ -1237,8 +1237,8 @@ struct ApplyTile {
 // are the identity, from v); with part, the workgroup's partial dots (y.w1,
 // y.w2) go to part[0] and part[1].  LDS scratch: accv (zeroed by the
 // caller) when SCATTER, else out.
-template <typename TL, typename T, typename C, typename CP>
-__device__ __forceinline__ void cells_apply(CP cellp, size_t WS, const T* vs, const int* sl,
+template <typename TL, typename T, typename CF>
+__device__ __forceinline__ void cells_apply(CF coef, const T* vs, const int* sl,
                                             const unsigned char* cms, T* out, T* accv, const T* __restrict__ v,
                                             T* __restrict__ y, double ca, const double (&dv)[NDV > 0 ? NDV : 1],
                                             const T* __restrict__ w1, const T* __restrict__ w2,
@@ -1253,7 +1253,6 @@ __device__ __forceinline__ void cells_apply(CP cellp, size_t WS, const T* vs, co
   // the corner sums of box cell cl over its classes, handed to put(corner, i, value)
   auto cell_corner_sums = [&](int cl, unsigned m, auto&& put) {
     const int ci = cl % CXB, cj = (cl / CXB) % CYB, ck = cl / (CXB * CYB);
-    const C* __restrict__ cb = cellp(cl);
 #pragma unroll
     for (int p = 0; p < NCLASS; ++p) {
       if (!((m >> p) & 1u)) continue;
@@ -1278,12 +1277,12 @@ __device__ __forceinline__ void cells_apply(CP cellp, size_t WS, const T* vs, co
           for (int a = 0; a < NV; ++a) sum += T(PHI[q][a]) * vcor(a, j);
           vq[j] = sum;
         }
-        const C* mp = cb + (size_t)((p * NQ + q) * NMC) * WS;
+        const int w0 = (p * NQ + q) * NMC;
         T w[BC];
 #pragma unroll
         for (int i = 0; i < BC; ++i) w[i] = T(0);
 #pragma unroll
-        for (int n = 0; n < NMC; ++n) w[MCI[n]] += T(mp[(size_t)n * WS]) * vq[MCJ[n]];
+        for (int n = 0; n < NMC; ++n) w[MCI[n]] += T(coef(cl, w0 + n)) * vq[MCJ[n]];
 #pragma unroll
         for (int a = 0; a < NV; ++a)
 #pragma unroll
@@ -1432,8 +1431,9 @@ __device__ __forceinline__ void cells_apply(CP cellp, size_t WS, const T* vs, co
     }
     y[(size_t)s * DCG_B + i] = acc;
     if (part) {
-      d1 += (double)acc * (double)w1[(size_t)s * DCG_B + i];
-      if (w2) d2 += (double)acc * (double)w2[(size_t)s * DCG_B + i];
+      // a dot operand may be y itself (t.t of BiCGSTAB): then it is acc
+      d1 += (double)acc * (double)(w1 == y ? acc : w1[(size_t)s * DCG_B + i]);
+      if (w2) d2 += (double)acc * (double)(w2 == y ? acc : w2[(size_t)s * DCG_B + i]);
     }
   }
   if (part) {
@@ -1597,9 +1597,9 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
       }
       __syncthreads();
       if constexpr (CELL_APPLY) {
-        cells_apply<ApplyTile<TX, TY, TZ>, double, double>([&](int cl) { return (const double*)work + cl; },
-                                                           (size_t)NCB, xs, sl, cms, nullptr, acc4, base, r, ca, dv,
-                                                           nullptr, nullptr, nullptr);
+        cells_apply<ApplyTile<TX, TY, TZ>, double>(
+            [&](int cl, int wd) { return ((const double*)work)[(size_t)wd * NCB + cl]; }, xs, sl, cms, nullptr, acc4,
+            base, r, ca, dv, nullptr, nullptr, nullptr);
         return;
       }
     }
@@ -1977,6 +1977,7 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
   const int nty = DIM > 1 ? (n1 + 1 + TY - 1) / TY : 1;
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
   const int vi0 = tx * TX - 1, vj0 = DIM > 1 ? ty * TY - 1 : 0, vk0 = DIM > 2 ? tz * TZ - 1 : 0;
+  const size_t WS = (size_t)n0 * (size_t)n1 * (size_t)n2;
   for (int b = threadIdx.x; b < NVB; b += blockDim.x) {
     const int bi = b % XB, bj = (b / XB) % YB, bk = b / (XB * YB);
     const int gi = vi0 + bi, gj = vj0 + bj, gk = vk0 + bk;
@@ -2005,8 +2006,9 @@ __device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, 
   __syncthreads();
   double dv[NDV > 0 ? NDV : 1];
   uniform_diffusion(t, prm, dv);
-  cells_apply<TL, T, T>([&](int cl) { return pc + coff[cl]; }, (size_t)n0 * (size_t)n1 * (size_t)n2, vs, sl, cms,
-                        out, accv, v, y, ca, dv, w1, w2, part);
+  cells_apply<TL, T>(
+      [&](int cl, int wd) -> T { return pc[coff[cl] + (size_t)wd * WS]; },
+      vs, sl, cms, out, accv, v, y, ca, dv, w1, w2, part);
 }
 
 }  // namespace %C%

@@ -1049,12 +1049,13 @@ void Model::build_ilu_sweep_tables(const std::vector<int>& lrows, const std::vec
   std::vector<int> llev, lpos, ulev, upos;
   positions(lo_ptr_, lrows, llev, lpos);
   positions(up_ptr_, urows, ulev, upos);
-  int dist = 1, maxrow = 1, maxlower = 0;
+  int dist = 1, maxrow = 1, maxlower = 0, maxupper = 0;
   for (long i = 0; i < ns; ++i) {
     for (int p2 = rp[i]; p2 < dp[i]; ++p2) dist = std::max(dist, llev[i] - llev[col[p2]]);
     for (int p2 = dp[i] + 1; p2 < rp[i + 1]; ++p2) dist = std::max(dist, ulev[i] - ulev[col[p2]]);
     maxrow = std::max(maxrow, rp[i + 1] - rp[i]);
     maxlower = std::max(maxlower, dp[i] - rp[i]);
+    maxupper = std::max(maxupper, rp[i + 1] - dp[i] - 1);
   }
   std::vector<int> uptr(static_cast<size_t>(L_.nnzb()) + 1, 0);
   std::vector<int2> mq;
@@ -1123,6 +1124,16 @@ void Model::build_ilu_sweep_tables(const std::vector<int>& lrows, const std::vec
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device_) == hipSuccess && v > 0)
       apply_lds = std::max(apply_lds, std::min<size_t>(static_cast<size_t>(v), 160 * 1024));
   }
+  // the apply's register path: every row's blocks in registers (at most 32
+  // values), one row per thread per level, and the stage padded to K places
+  // per row
+  t.reg_k = 0;
+  if (width <= k::kIluThreads)
+    for (int K : {4, 8, 16})
+      if (std::max(maxlower, maxupper) <= K && K * B * B <= 32) {
+        t.reg_k = K;
+        break;
+      }
   for (int clen = 32; clen >= 1 && !t.chunked; --clen) {
     int me = 0, mr = 0;
     chunks(clen, lo_ptr_, lent, lch, me, mr);
@@ -1137,6 +1148,7 @@ void Model::build_ilu_sweep_tables(const std::vector<int>& lrows, const std::vec
       t.chunked = true;
     }
   }
+  if (!t.chunked) t.reg_k = 0;
   if (t.chunked) {
     entries(t.ring, lo_ptr_, lrows, llev, lpos, true, lent, lblk, lsrc);
     entries(t.ring, up_ptr_, urows, ulev, upos, false, uent, ublk, usrc);
@@ -1157,6 +1169,22 @@ void Model::build_ilu_sweep_tables(const std::vector<int>& lrows, const std::vec
     };
     replay(lo_ptr_, lrows, lent, lblk, lsrc);
     replay(up_ptr_, urows, uent, ublk, usrc);
+    if (t.reg_k > 0) {
+      // row r's place q at r * K + q; padding has block -1
+      const int K = t.reg_k;
+      auto pad = [&](const std::vector<int>& ent, std::vector<int>& blk, std::vector<int>& src) {
+        std::vector<int> pb(static_cast<size_t>(ns) * K, -1), ps(static_cast<size_t>(ns) * K, -1);
+        for (long r = 0; r < ns; ++r)
+          for (int e = ent[r]; e < ent[r + 1]; ++e) {
+            pb[static_cast<size_t>(r) * K + (e - ent[r])] = blk[e];
+            ps[static_cast<size_t>(r) * K + (e - ent[r])] = src[e];
+          }
+        blk.swap(pb);
+        src.swap(ps);
+      };
+      pad(lent, lblk, lsrc);
+      pad(uent, ublk, usrc);
+    }
     lo_ent_.upload(lent, stream_);
     lo_blk_.upload(lblk.empty() ? std::vector<int>{0} : lblk, stream_);
     lo_src_.upload(lsrc.empty() ? std::vector<int>{0} : lsrc, stream_);

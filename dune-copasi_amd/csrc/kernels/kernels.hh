@@ -254,12 +254,14 @@ void ilu_factor(int B, int nrows, const int* rows, const int* rp, const int* col
 //   with the tables frow (row, first block, blocks, lower blocks), fub, fm
 //   and fpush (see ilu_factor_inbox_k); factor_threads threads.
 constexpr int kIluRegLower = 4, kIluRegPairs = 16, kIluRegRow = 9, kIluRegPush = 24;
+constexpr int kIluThreads = 256;  // workgroup of the chunked apply
 constexpr int kIluFubStride = 8;  // ints per row of fub (kIluRegLower + 1, padded to 16-byte loads)
 struct IluSweep {
   int nlo = 0, nup = 0;
   const int *lo_lev = nullptr, *lo_rows = nullptr, *up_lev = nullptr, *up_rows = nullptr;
   bool chunked = false;
   int nlo_chunk = 0, nup_chunk = 0, stage_ent = 0, stage_rows = 0, stage_levels = 0, ring = 0, width = 0;
+  int reg_k = 0;  // rows of at most reg_k blocks, levels of at most kIluThreads rows: the register path
   const int *lo_chunk = nullptr, *lo_ent = nullptr, *lo_blk = nullptr, *lo_src = nullptr;
   const int *up_chunk = nullptr, *up_ent = nullptr, *up_blk = nullptr, *up_src = nullptr;
   int factor_threads = 0;

@@ -817,19 +817,26 @@ __global__ void __launch_bounds__(256) ilu_apply_sweep_k(int nlo, const int* __r
 // right-hand sides and (backward sweep) diagonal inverses are first gathered
 // into LDS with every load in flight at once; its levels then run from LDS
 // alone, each row's result going to the ring slot (l mod ring) * width + r -
-// lev[l]; at the chunk's end the results are written to `res`.  The sums
-// run in the blocks' column order, as ilu_lower_row / ilu_upper_row sum them.
+// lev[l] (value a of slot q at ring[a * ring * width + q]); at the chunk's
+// end the results are written to `res`.  The sums run in the blocks' column
+// order, as ilu_lower_row / ilu_upper_row sum them.
 // Byte offsets of the chunk stage in LDS (kernel and launcher): blocks,
 // diagonal inverses, right-hand sides (doubles), then source slots, entry
 // offsets, row ids and level bounds (ints), then the ring (8-byte aligned).
+// With reg_k = K > 0 every row has K entry places (padding has source -1)
+// and the rows' values lie component-major: block entry a of place k of
+// chunk row rr at [(k * B*B + a) * stage_rows + rr], source [k * stage_rows
+// + rr], right-hand side and diagonal inverse [a * stage_rows + rr], so the
+// rows of one level (consecutive rr, one per lane) read consecutive words.
 struct IluChunkLayout {
   size_t d, in, src, ent, row, lev, ring, total;
   __host__ __device__ IluChunkLayout(const IluSweep& t, int B, int vbytes) {
     const size_t BB = (size_t)B * B;
-    d = 8 * (size_t)t.stage_ent * BB;
+    const size_t nent = t.reg_k > 0 ? (size_t)t.reg_k * t.stage_rows : (size_t)t.stage_ent;
+    d = 8 * nent * BB;
     in = d + 8 * (size_t)t.stage_rows * BB;
     src = in + 8 * (size_t)t.stage_rows * B;
-    ent = src + 4 * (size_t)t.stage_ent;
+    ent = src + 4 * nent;
     row = ent + 4 * ((size_t)t.stage_rows + 1);
     lev = row + 4 * (size_t)t.stage_rows;
     ring = (lev + 4 * ((size_t)t.stage_levels + 1) + 7) & ~(size_t)7;
@@ -837,7 +844,15 @@ struct IluChunkLayout {
   }
 };
 
-template <int B, typename V, bool UPPER>
+// A thread's row of one level, read from the chunk stage (K > 0 below).
+template <int B, int K, bool UPPER>
+struct IluStagedRow {
+  int rr = -1, o = 0;
+  int src[K];
+  double L[K][B * B], in[B], d[UPPER ? B * B : 1];
+};
+
+template <int B, typename V, bool UPPER, int K = 0>
 __device__ __forceinline__ void ilu_chunk_sweep(const IluSweep& t, int nchunk, const int* __restrict__ chunk,
                                                 const int* __restrict__ lev, const int* __restrict__ rows,
                                                 const int* __restrict__ ent, const int* __restrict__ blk,
@@ -853,7 +868,8 @@ __device__ __forceinline__ void ilu_chunk_sweep(const IluSweep& t, int nchunk, c
   int* sEnt = reinterpret_cast<int*>(lds + lay.ent);            // stage_rows + 1
   int* sRow = reinterpret_cast<int*>(lds + lay.row);            // stage_rows
   int* sLev = reinterpret_cast<int*>(lds + lay.lev);            // stage_levels + 1
-  V* ring = reinterpret_cast<V*>(lds + lay.ring);               // ring * width * B
+  V* ring = reinterpret_cast<V*>(lds + lay.ring);               // B * ring * width, component-major
+  const int RS = t.ring * t.width;                              // ring slots
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int c = 0; c < nchunk; ++c) {
     const int L0 = chunk[c], L1 = chunk[c + 1];
@@ -863,7 +879,9 @@ __device__ __forceinline__ void ilu_chunk_sweep(const IluSweep& t, int nchunk, c
     // index loads, then its value loads, issued together before any store
     constexpr int KE = BB <= 4 ? 8 : BB <= 16 ? 4 : 1, KR = 4;
     for (int l = L0 + tid; l <= L1; l += nt) sLev[l - L0] = lev[l];
-    for (int r = r0 + tid; r <= r1; r += nt) sEnt[r - r0] = ent[r] - e0;
+    if constexpr (K == 0)
+      for (int r = r0 + tid; r <= r1; r += nt) sEnt[r - r0] = ent[r] - e0;
+    const int SR = t.stage_rows;
     for (int rb = r0; rb < r1; rb += nt * KR) {
       int ri[KR];
 #pragma unroll
@@ -887,12 +905,44 @@ __device__ __forceinline__ void ilu_chunk_sweep(const IluSweep& t, int nchunk, c
           const int rr = rb + tid + k * nt - r0;
           sRow[rr] = ri[k];
 #pragma unroll
-          for (int a = 0; a < B; ++a) sIn[rr * B + a] = vin[k][a];
+          for (int a = 0; a < B; ++a) sIn[K > 0 ? a * SR + rr : rr * B + a] = vin[k][a];
           if constexpr (UPPER)
 #pragma unroll
-            for (int a = 0; a < BB; ++a) sD[rr * BB + a] = vd[k][a];
+            for (int a = 0; a < BB; ++a) sD[K > 0 ? a * SR + rr : rr * BB + a] = vd[k][a];
         }
     }
+    if constexpr (K > 0) {
+      // the padded places (row-major in blk / src: row r's place k at
+      // r * K + k), taken rr-fastest so that the LDS stores are consecutive
+      const int nr = r1 - r0, nq = nr * K;
+      for (int qb = 0; qb < nq; qb += nt * KE) {
+        int bi[KE], si[KE], rq[KE], kq[KE];
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+          const int q = qb + tid + k * nt;
+          kq[k] = q / (nr > 0 ? nr : 1);
+          rq[k] = q - kq[k] * nr;
+          const bool ok = q < nq;
+          bi[k] = ok ? blk[(size_t)(r0 + rq[k]) * K + kq[k]] : -1;
+          si[k] = ok ? src[(size_t)(r0 + rq[k]) * K + kq[k]] : -1;
+          if (!ok) kq[k] = -1;
+        }
+        double v[KE][BB];
+#pragma unroll
+        for (int k = 0; k < KE; ++k)
+          if (bi[k] >= 0)
+#pragma unroll
+            for (int a = 0; a < BB; ++a) v[k][a] = lu[(size_t)bi[k] * BB + a];
+#pragma unroll
+        for (int k = 0; k < KE; ++k)
+          if (kq[k] >= 0) {
+            sSrc[kq[k] * SR + rq[k]] = bi[k] >= 0 ? si[k] : -1;
+            if (bi[k] >= 0)
+#pragma unroll
+              for (int a = 0; a < BB; ++a) sL[(size_t)(kq[k] * BB + a) * SR + rq[k]] = v[k][a];
+          }
+      }
+    } else
     for (int eb = e0; eb < e1; eb += nt * KE) {
       int bi[KE], si[KE];
 #pragma unroll
@@ -917,6 +967,73 @@ __device__ __forceinline__ void ilu_chunk_sweep(const IluSweep& t, int nchunk, c
         }
     }
     __syncthreads();
+    if constexpr (K > 0) {
+      // at most K blocks per row and one row per thread per level: the
+      // thread's row of level l + 1 is read from the stage while level l
+      // runs, so a level's critical path is its ring reads, its sums and
+      // one barrier (the sums in the same order as below)
+      using Row = IluStagedRow<B, K, UPPER>;
+      auto fetch = [&](int l, Row& R) {
+        R.rr = -1;
+        if (l >= L1) return;
+        const int b0 = sLev[l - L0], r = b0 + tid;
+        if (r >= sLev[l + 1 - L0]) return;
+        const int rr = r - r0;
+        R.rr = rr;
+        R.o = (l % t.ring) * t.width + (r - b0);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          R.src[k] = sSrc[k * SR + rr];
+#pragma unroll
+          for (int a = 0; a < BB; ++a) R.L[k][a] = sL[(size_t)(k * BB + a) * SR + rr];
+        }
+#pragma unroll
+        for (int a = 0; a < B; ++a) R.in[a] = sIn[a * SR + rr];
+        if constexpr (UPPER)
+#pragma unroll
+          for (int a = 0; a < BB; ++a) R.d[a] = sD[(size_t)a * SR + rr];
+      };
+      Row cur, nxt;
+      fetch(L0, cur);
+      for (int l = L0; l < L1; ++l) {
+        double yv[K][B];
+        if (cur.rr >= 0) {
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+            if (cur.src[k] >= 0)
+#pragma unroll
+              for (int cc = 0; cc < B; ++cc) yv[k][cc] = (double)ring[(size_t)cc * RS + cur.src[k]];
+        }
+        fetch(l + 1, nxt);
+        if (cur.rr >= 0) {
+          double sum[B];
+#pragma unroll
+          for (int a = 0; a < B; ++a) sum[a] = cur.in[a];
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+            if (cur.src[k] >= 0)
+#pragma unroll
+              for (int a = 0; a < B; ++a)
+#pragma unroll
+                for (int cc = 0; cc < B; ++cc) sum[a] -= cur.L[k][a * B + cc] * yv[k][cc];
+          V* o = ring + cur.o;
+          if constexpr (UPPER) {
+#pragma unroll
+            for (int a = 0; a < B; ++a) {
+              double v = 0.0;
+#pragma unroll
+              for (int cc = 0; cc < B; ++cc) v += cur.d[a * B + cc] * sum[cc];
+              o[(size_t)a * RS] = (V)v;
+            }
+          } else {
+#pragma unroll
+            for (int a = 0; a < B; ++a) o[(size_t)a * RS] = (V)sum[a];
+          }
+        }
+        __syncthreads();
+        cur = nxt;
+      }
+    } else
     for (int l = L0; l < L1; ++l) {
       const int b0 = sLev[l - L0], b1 = sLev[l + 1 - L0];
       const int base = (l % t.ring) * t.width - b0;
@@ -928,13 +1045,13 @@ __device__ __forceinline__ void ilu_chunk_sweep(const IluSweep& t, int nchunk, c
         const int ee = sEnt[rr + 1];
         for (int e = sEnt[rr]; e < ee; ++e) {
           const double* L = sL + (size_t)e * BB;
-          const V* yk = ring + (size_t)sSrc[e] * B;
+          const V* yk = ring + sSrc[e];
 #pragma unroll
           for (int a = 0; a < B; ++a)
 #pragma unroll
-            for (int cc = 0; cc < B; ++cc) sum[a] -= L[a * B + cc] * (double)yk[cc];
+            for (int cc = 0; cc < B; ++cc) sum[a] -= L[a * B + cc] * (double)yk[(size_t)cc * RS];
         }
-        V* o = ring + (size_t)(base + r) * B;
+        V* o = ring + (base + r);
         if constexpr (UPPER) {
           const double* d = sD + (size_t)rr * BB;
 #pragma unroll
@@ -942,11 +1059,11 @@ __device__ __forceinline__ void ilu_chunk_sweep(const IluSweep& t, int nchunk, c
             double v = 0.0;
 #pragma unroll
             for (int cc = 0; cc < B; ++cc) v += d[a * B + cc] * sum[cc];
-            o[a] = (V)v;
+            o[(size_t)a * RS] = (V)v;
           }
         } else {
 #pragma unroll
-          for (int a = 0; a < B; ++a) o[a] = (V)sum[a];
+          for (int a = 0; a < B; ++a) o[(size_t)a * RS] = (V)sum[a];
         }
       }
       __syncthreads();
@@ -955,9 +1072,9 @@ __device__ __forceinline__ void ilu_chunk_sweep(const IluSweep& t, int nchunk, c
       const int b0 = sLev[l - L0], b1 = sLev[l + 1 - L0];
       const int base = (l % t.ring) * t.width - b0;
       for (int r = b0 + tid; r < b1; r += nt) {
-        const V* o = ring + (size_t)(base + r) * B;
+        const V* o = ring + (base + r);
 #pragma unroll
-        for (int a = 0; a < B; ++a) res[(size_t)sRow[r - r0] * B + a] = o[a];
+        for (int a = 0; a < B; ++a) res[(size_t)sRow[r - r0] * B + a] = o[(size_t)a * RS];
       }
     }
     __syncthreads();
@@ -966,17 +1083,20 @@ __device__ __forceinline__ void ilu_chunk_sweep(const IluSweep& t, int nchunk, c
 
 // Both triangular solves of ILU0 in one workgroup, the levels in chunks
 // staged through LDS (ilu_chunk_sweep): out = U^-1 L^-1 in, tmp = L^-1 in.
+// K > 0: the register-pipelined level loop (rows of at most K blocks, levels
+// of at most kIluThreads rows; K * B * B <= 32 keeps two rows in registers).
 extern __shared__ char ilu_chunk_lds[];
-template <int B, typename V>
-__global__ void __launch_bounds__(256) ilu_apply_chunk_k(IluSweep t, const double* __restrict__ lu,
-                                                         const double* __restrict__ dinv, const V* __restrict__ in,
-                                                         V* __restrict__ tmp, V* __restrict__ out,
-                                                         const BicgScalars* __restrict__ sc) {
+template <int B, typename V, int K>
+__global__ void __launch_bounds__(kIluThreads) ilu_apply_chunk_k(IluSweep t, const double* __restrict__ lu,
+                                                                 const double* __restrict__ dinv,
+                                                                 const V* __restrict__ in, V* __restrict__ tmp,
+                                                                 V* __restrict__ out,
+                                                                 const BicgScalars* __restrict__ sc) {
   if (sc && finished(sc)) return;
-  ilu_chunk_sweep<B, V, false>(t, t.nlo_chunk, t.lo_chunk, t.lo_lev, t.lo_rows, t.lo_ent, t.lo_blk, t.lo_src, lu,
-                               nullptr, in, tmp, ilu_chunk_lds);
-  ilu_chunk_sweep<B, V, true>(t, t.nup_chunk, t.up_chunk, t.up_lev, t.up_rows, t.up_ent, t.up_blk, t.up_src, lu,
-                              dinv, tmp, out, ilu_chunk_lds);
+  ilu_chunk_sweep<B, V, false, K>(t, t.nlo_chunk, t.lo_chunk, t.lo_lev, t.lo_rows, t.lo_ent, t.lo_blk, t.lo_src,
+                                  lu, nullptr, in, tmp, ilu_chunk_lds);
+  ilu_chunk_sweep<B, V, true, K>(t, t.nup_chunk, t.up_chunk, t.up_lev, t.up_rows, t.up_ent, t.up_blk, t.up_src, lu,
+                                 dinv, tmp, out, ilu_chunk_lds);
 }
 
 // ----------------------------------------------------------------- state I/O helpers
@@ -1230,18 +1350,30 @@ void ilu_factor_sweep(int B, const IluSweep& t, const int* rp, const int* col, c
 
 // the chunked apply with `lds` bytes of dynamic LDS; above 64 KiB the
 // kernel's limit is raised first (once per instantiation and size)
-template <int BB, typename V>
-void launch_ilu_chunk(size_t lds, const IluSweep& t, const double* lu, const double* dinv, const V* in, V* tmp, V* out,
-                      const BicgScalars* sc, hipStream_t s) {
+template <int BB, typename V, int K>
+void launch_ilu_chunk_k(size_t lds, const IluSweep& t, const double* lu, const double* dinv, const V* in, V* tmp,
+                        V* out, const BicgScalars* sc, hipStream_t s) {
   static std::atomic<size_t> raised{64 * 1024};
   if (lds > raised.load()) {
-    DCG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&ilu_apply_chunk_k<BB, V>),
+    DCG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&ilu_apply_chunk_k<BB, V, K>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     size_t cur = raised.load();
     while (lds > cur && !raised.compare_exchange_weak(cur, lds)) {
     }
   }
-  ilu_apply_chunk_k<BB, V><<<1, 256, lds, s>>>(t, lu, dinv, in, tmp, out, sc);
+  ilu_apply_chunk_k<BB, V, K><<<1, kIluThreads, lds, s>>>(t, lu, dinv, in, tmp, out, sc);
+}
+template <int BB, typename V>
+void launch_ilu_chunk(size_t lds, const IluSweep& t, const double* lu, const double* dinv, const V* in, V* tmp, V* out,
+                      const BicgScalars* sc, hipStream_t s) {
+  // the register path where the host planned one (K * B * B <= 32)
+  if constexpr (BB <= 2) {
+    if (t.reg_k == 4) return launch_ilu_chunk_k<BB, V, 4>(lds, t, lu, dinv, in, tmp, out, sc, s);
+    if (t.reg_k == 8) return launch_ilu_chunk_k<BB, V, 8>(lds, t, lu, dinv, in, tmp, out, sc, s);
+  }
+  if constexpr (BB == 1)
+    if (t.reg_k == 16) return launch_ilu_chunk_k<BB, V, 16>(lds, t, lu, dinv, in, tmp, out, sc, s);
+  launch_ilu_chunk_k<BB, V, 0>(lds, t, lu, dinv, in, tmp, out, sc, s);
 }
 
 template <typename V>
