@@ -1237,11 +1237,13 @@ struct ApplyTile {
 // are the identity, from v); with part, the workgroup's partial dots (y.w1,
 // y.w2) go to part[0] and part[1].  LDS scratch: accv (zeroed by the
 // caller) when SCATTER, else out.
-template <typename TL, typename T, typename CF>
+// VP, YP: pointer types of the vectors read and of y (global-memory pointers
+// from the point apply, plain ones from the matrix-free apply).
+template <typename TL, typename T, typename CF, typename VP, typename YP>
 __device__ __forceinline__ void cells_apply(CF coef, const T* vs, const int* sl,
-                                            const unsigned char* cms, T* out, T* accv, const T* __restrict__ v,
-                                            T* __restrict__ y, double ca, const double (&dv)[NDV > 0 ? NDV : 1],
-                                            const T* __restrict__ w1, const T* __restrict__ w2,
+                                            const unsigned char* cms, T* out, T* accv, VP __restrict__ v,
+                                            YP __restrict__ y, double ca, const double (&dv)[NDV > 0 ? NDV : 1],
+                                            VP __restrict__ w1, VP __restrict__ w2,
                                             double* __restrict__ part) {
   constexpr int TX = TL::TX, TY = TL::TY, XB = TL::XB, YB = TL::YB, CXB = TL::CXB, CYB = TL::CYB;
   constexpr int NCB = TL::NCB, NROW = TL::NROW;
@@ -1599,7 +1601,7 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
       if constexpr (CELL_APPLY) {
         cells_apply<ApplyTile<TX, TY, TZ>, double>(
             [&](int cl, int wd) { return ((const double*)work)[(size_t)wd * NCB + cl]; }, xs, sl, cms, nullptr, acc4,
-            base, r, ca, dv, nullptr, nullptr, nullptr);
+            base, r, ca, dv, (const double*)nullptr, (const double*)nullptr, nullptr);
         return;
       }
     }
@@ -1953,12 +1955,19 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
 // compartment's slots are the grid's vertices in order.  With part, the
 // workgroup also writes its rows' partial dots (y.w1, y.w2) to part[0] and
 // part[1] (w2 may be null), as the BiCGSTAB finaliser reads.
+// (pointers typed in the global address space: the batched entry takes them
+// from member descriptors, where the compiler could not tell their space
+// and would use flat accesses)
+template <typename X>
+using dcg_gp = __attribute__((address_space(1))) X*;
 template <typename TL, typename T>
-__device__ __forceinline__ void pc_apply(int tile, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,
-                                         const T* __restrict__ pc, const T* __restrict__ v, T* __restrict__ y,
-                                         const double* __restrict__ tca, int n0, int n1, int n2,
-                                         const double* __restrict__ prm, const T* __restrict__ w1,
-                                         const T* __restrict__ w2, double* __restrict__ part) {
+__device__ __forceinline__ void pc_apply(int tile, dcg_gp<const int> __restrict__ g2l,
+                                         dcg_gp<const unsigned char> __restrict__ cmask,
+                                         dcg_gp<const T> __restrict__ pc, dcg_gp<const T> __restrict__ v,
+                                         dcg_gp<T> __restrict__ y, dcg_gp<const double> __restrict__ tca, int n0,
+                                         int n1, int n2, const double* __restrict__ prm,
+                                         dcg_gp<const T> __restrict__ w1, dcg_gp<const T> __restrict__ w2,
+                                         double* __restrict__ part) {
   constexpr int TX = TL::TX, TY = TL::TY, TZ = TL::TZ, XB = TL::XB, YB = TL::YB, CXB = TL::CXB, CYB = TL::CYB;
   constexpr int NVB = TL::NVB, NCB = TL::NCB;
   const double t = tca[0], ca = tca[1];
@@ -2061,8 +2070,10 @@ extern "C" __global__ void __launch_bounds__(256)
     if (stop && *stop) return;                                                                                    \
     const int c = (int)(blockIdx.x % 8u) * (int)(gridDim.x / 8u) + (int)(blockIdx.x / 8u);                       \
     if (c >= ntile) return;                                                                                       \
-    %C%::pc_apply<TL, T>(tiles[c], g2l, cmask, pc, v, y, tca, n0, n1, n2, prm, w1, w2,                            \
-                         part ? part + 2 * (size_t)c : nullptr);                                                  \
+    %C%::pc_apply<TL, T>(tiles[c], (%C%::dcg_gp<const int>)g2l, (%C%::dcg_gp<const unsigned char>)cmask,          \
+                         (%C%::dcg_gp<const T>)pc, (%C%::dcg_gp<const T>)v, (%C%::dcg_gp<T>)y,                    \
+                         (%C%::dcg_gp<const double>)tca, n0, n1, n2, prm, (%C%::dcg_gp<const T>)w1,               \
+                         (%C%::dcg_gp<const T>)w2, part ? part + 2 * (size_t)c : nullptr);                        \
   }
 %POINT_ENTRIES%
 #undef DCG_POINT_ENTRY
@@ -2102,10 +2113,12 @@ extern "C" __global__ void __launch_bounds__(256)
     const dcg_pc_member& M = mem[w.x];                                                                            \
     if (skip == 0 && *M.done) return;                                                                             \
     if (skip == 1 && *M.irdone) return;                                                                           \
-    %C%::pc_apply<TL, T>(w.y, M.g2l, M.cmask, M.PC, M.VEC[xsel], M.VEC[ysel], M.tca, M.n[0], M.n[1], M.n[2],      \
-                         M.prm,                                                                                   \
-                     w1sel >= 0 ? M.VEC[w1sel] : nullptr, w2sel >= 0 ? M.VEC[w2sel] : nullptr,                    \
-                     part ? part + 2 * (size_t)(part_off + c) : nullptr);                                         \
+    using GT = %C%::dcg_gp<const T>;                                                                              \
+    %C%::pc_apply<TL, T>(w.y, (%C%::dcg_gp<const int>)M.g2l, (%C%::dcg_gp<const unsigned char>)M.cmask,          \
+                         (GT)M.PC, (GT)M.VEC[xsel], (%C%::dcg_gp<T>)M.VEC[ysel],                                  \
+                         (%C%::dcg_gp<const double>)M.tca, M.n[0], M.n[1], M.n[2], M.prm,                         \
+                         w1sel >= 0 ? (GT)M.VEC[w1sel] : nullptr, w2sel >= 0 ? (GT)M.VEC[w2sel] : nullptr,        \
+                         part ? part + 2 * (size_t)(part_off + c) : nullptr);                                     \
   }
 %PBATCH_ENTRIES%
 #undef DCG_PBATCH_ENTRY

@@ -22,6 +22,18 @@ __device__ __forceinline__ bool skipped(const Skip& sk, int m) {
   return sk.p && ((sk.p[(size_t)m * sk.stride] != 0) == (sk.when != 0));
 }
 
+// A member's pointers come from its descriptor in memory, so the compiler
+// cannot tell their address space and would access them with flat
+// instructions; they all point into device global memory, and passing them
+// through that address space lets it use global loads and stores (and pair
+// adjacent ones into wide accesses).
+template <typename T>
+using global_ptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ global_ptr<T> in_global(T* p) {
+  return (global_ptr<T>)p;
+}
+
 // vector selectors: FP64 vectors by BatchVec index, FP32 Krylov vectors at
 // the same index as their FP64 twins (kVKry + i), the FP32 x at kVKry + 9
 template <typename V>
@@ -241,22 +253,30 @@ __global__ void __launch_bounds__(kThreads) bp_k(const int2* __restrict__ work, 
   const MemberView& m = mv[w.x];
   const int s = w.y * kThreads + threadIdx.x;
   if (s >= m.nslot) return;
-  const V* r = vsel<V>(m, kVKry + 0);
-  V* p = vsel<V>(m, kVKry + 2);
-  const V* v = vsel<V>(m, kVKry + 3);
-  V* ph = vsel<V>(m, kVKry + 4);
+  const auto r = in_global(vsel<V>(m, kVKry + 0));
+  const auto p = in_global(vsel<V>(m, kVKry + 2));
+  const auto v = in_global(vsel<V>(m, kVKry + 3));
+  const auto ph = in_global(vsel<V>(m, kVKry + 4));
   const double beta = S.beta, omega = S.omega;
-  V pv[B];
+  // each vector's B values read together (adjacent accesses pair up)
+  const size_t k0 = (size_t)s * B;
+  V rv[B], pold[B], vv[B], pv[B], o[B];
 #pragma unroll
-  for (int i = 0; i < B; ++i) {
-    const size_t k = (size_t)s * B + i;
-    pv[i] = (V)((double)r[k] + beta * ((double)p[k] - omega * (double)v[k]));
-    p[k] = pv[i];
-  }
-  if (jacobi) dinv_apply<B>(m.dinv, s, pv, ph + (size_t)s * B, jacobi == 2);
+  for (int i = 0; i < B; ++i) rv[i] = r[k0 + i];
+#pragma unroll
+  for (int i = 0; i < B; ++i) pold[i] = p[k0 + i];
+#pragma unroll
+  for (int i = 0; i < B; ++i) vv[i] = v[k0 + i];
+#pragma unroll
+  for (int i = 0; i < B; ++i) pv[i] = (V)((double)rv[i] + beta * ((double)pold[i] - omega * (double)vv[i]));
+#pragma unroll
+  for (int i = 0; i < B; ++i) p[k0 + i] = pv[i];
+  if (jacobi) dinv_apply<B>((const double*)in_global(m.dinv), s, pv, o, jacobi == 2);
   else
 #pragma unroll
-    for (int i = 0; i < B; ++i) ph[(size_t)s * B + i] = pv[i];
+    for (int i = 0; i < B; ++i) o[i] = pv[i];
+#pragma unroll
+  for (int i = 0; i < B; ++i) ph[k0 + i] = o[i];
 }
 
 // s = r - alpha v; sh = M^-1 s; partial ||s||^2
@@ -272,23 +292,30 @@ __global__ void __launch_bounds__(kThreads) bs_k(const int2* __restrict__ work, 
   const int s = w.y * kThreads + threadIdx.x;
   double acc[1] = {0.0};
   if (s < m.nslot) {
-    const V* r = vsel<V>(m, kVKry + 0);
-    const V* v = vsel<V>(m, kVKry + 3);
-    V* sv = vsel<V>(m, kVKry + 5);
-    V* sh = vsel<V>(m, kVKry + 6);
+    const auto r = in_global(vsel<V>(m, kVKry + 0));
+    const auto v = in_global(vsel<V>(m, kVKry + 3));
+    const auto sv = in_global(vsel<V>(m, kVKry + 5));
+    const auto sh = in_global(vsel<V>(m, kVKry + 6));
     const double alpha = S.alpha;
-    V t[B];
+    const size_t k0 = (size_t)s * B;
+    V rv[B], vv[B], t[B], o[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i) rv[i] = r[k0 + i];
+#pragma unroll
+    for (int i = 0; i < B; ++i) vv[i] = v[k0 + i];
 #pragma unroll
     for (int i = 0; i < B; ++i) {
-      const size_t k = (size_t)s * B + i;
-      t[i] = (V)((double)r[k] - alpha * (double)v[k]);
-      sv[k] = t[i];
+      t[i] = (V)((double)rv[i] - alpha * (double)vv[i]);
       acc[0] += (double)t[i] * (double)t[i];
     }
-    if (jacobi) dinv_apply<B>(m.dinv, s, t, sh + (size_t)s * B, jacobi == 2);
+#pragma unroll
+    for (int i = 0; i < B; ++i) sv[k0 + i] = t[i];
+    if (jacobi) dinv_apply<B>((const double*)in_global(m.dinv), s, t, o, jacobi == 2);
     else
 #pragma unroll
-      for (int i = 0; i < B; ++i) sh[(size_t)s * B + i] = t[i];
+      for (int i = 0; i < B; ++i) o[i] = t[i];
+#pragma unroll
+    for (int i = 0; i < B; ++i) sh[k0 + i] = o[i];
   }
   block_sum(acc, red);
   if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
@@ -308,26 +335,44 @@ __global__ void __launch_bounds__(kThreads) bxr_k(const int2* __restrict__ work,
   double acc[2] = {0.0, 0.0};
   const bool half = S.half != 0;
   if (s < m.nslot) {
-    V* xv = vsel<V>(m, x);
-    V* r = vsel<V>(m, kVKry + 0);
-    const V* rh = vsel<V>(m, kVKry + 1);
-    const V* ph = vsel<V>(m, kVKry + 4);
-    const V* sv = vsel<V>(m, kVKry + 5);
-    const V* sh = vsel<V>(m, kVKry + 6);
-    const V* t = vsel<V>(m, kVKry + 7);
+    const auto xv = in_global(vsel<V>(m, x));
+    const auto r = in_global(vsel<V>(m, kVKry + 0));
+    const auto rh = in_global(vsel<V>(m, kVKry + 1));
+    const auto ph = in_global(vsel<V>(m, kVKry + 4));
+    const auto sv = in_global(vsel<V>(m, kVKry + 5));
+    const auto sh = in_global(vsel<V>(m, kVKry + 6));
+    const auto t = in_global(vsel<V>(m, kVKry + 7));
     const double alpha = S.alpha, omega = S.omega;
+    const size_t k0 = (size_t)s * B;
+    V x0[B], pv[B];
 #pragma unroll
-    for (int i = 0; i < B; ++i) {
-      const size_t k = (size_t)s * B + i;
-      if (half) {
-        xv[k] = (V)((double)xv[k] + alpha * (double)ph[k]);
-      } else {
-        xv[k] = (V)((double)xv[k] + alpha * (double)ph[k] + omega * (double)sh[k]);
-        const V rv = (V)((double)sv[k] - omega * (double)t[k]);
-        r[k] = rv;
-        acc[0] += (double)rh[k] * (double)rv;
-        acc[1] += (double)rv * (double)rv;
+    for (int i = 0; i < B; ++i) x0[i] = xv[k0 + i];
+#pragma unroll
+    for (int i = 0; i < B; ++i) pv[i] = ph[k0 + i];
+    if (half) {
+#pragma unroll
+      for (int i = 0; i < B; ++i) xv[k0 + i] = (V)((double)x0[i] + alpha * (double)pv[i]);
+    } else {
+      V hv[B], sw[B], tv[B], r0[B], xn[B], rn[B];
+#pragma unroll
+      for (int i = 0; i < B; ++i) hv[i] = sh[k0 + i];
+#pragma unroll
+      for (int i = 0; i < B; ++i) sw[i] = sv[k0 + i];
+#pragma unroll
+      for (int i = 0; i < B; ++i) tv[i] = t[k0 + i];
+#pragma unroll
+      for (int i = 0; i < B; ++i) r0[i] = rh[k0 + i];
+#pragma unroll
+      for (int i = 0; i < B; ++i) {
+        xn[i] = (V)((double)x0[i] + alpha * (double)pv[i] + omega * (double)hv[i]);
+        rn[i] = (V)((double)sw[i] - omega * (double)tv[i]);
+        acc[0] += (double)r0[i] * (double)rn[i];
+        acc[1] += (double)rn[i] * (double)rn[i];
       }
+#pragma unroll
+      for (int i = 0; i < B; ++i) xv[k0 + i] = xn[i];
+#pragma unroll
+      for (int i = 0; i < B; ++i) r[k0 + i] = rn[i];
     }
   }
   block_sum(acc, red);
@@ -353,12 +398,16 @@ __global__ void __launch_bounds__(kThreads) bspmv_k(int nwork, const int4* __res
   const MemberView& m = mv[w.x];
   double acc[2] = {0.0, 0.0};
   const int r0 = w.y, r1 = w.y + (w.w >> 16), k0 = w.z, k1 = w.z + (w.w & 0xffff);
+  const auto rp = in_global(m.rp), col = in_global(m.col);
+  const auto val = in_global(matrix<T>(m));
+  const auto xv = in_global((const V*)vsel<V>(m, x));
+  const auto yv = in_global(vsel<V>(m, y));
+  const auto w1v = in_global((const V*)vsel<V>(m, w1));
+  const auto w2v = in_global((const V*)vsel<V>(m, w2));
   if (BLOCKS)
-    spmv_blocks_chunk<B, T, V, StageT<T>>(r0, r1, k0, k1, m.rp, m.col, matrix<T>(m), vsel<V>(m, x), vsel<V>(m, y),
-                               vsel<V>(m, w1), vsel<V>(m, w2), acc, staged);
+    spmv_blocks_chunk<B, T, V, StageT<T>>(r0, r1, k0, k1, rp, col, val, xv, yv, w1v, w2v, acc, staged);
   else
-    spmv_entries_chunk<B, T, V, StageT<T>>(r0, r1, k0, k1, m.rp, m.col, matrix<T>(m), vsel<V>(m, x), vsel<V>(m, y),
-                                vsel<V>(m, w1), vsel<V>(m, w2), acc, staged);
+    spmv_entries_chunk<B, T, V, StageT<T>>(r0, r1, k0, k1, rp, col, val, xv, yv, w1v, w2v, acc, staged);
   if (part) {
     block_sum(acc, red);
     if (threadIdx.x == 0) {

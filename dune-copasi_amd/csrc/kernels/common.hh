@@ -235,12 +235,13 @@ using StageT = typename std::conditional<std::is_same<T, float>::value, float, d
 // fit the LDS stage.  Phase 1 streams the chunk's blocks
 // with consecutive threads on consecutive (block, row-of-block) pairs, so
 // value loads are fully coalesced; products go to LDS.  Phase 2 sums each row
-// from LDS and adds y.w1, y.w2 to acc.
-template <int B, typename T, typename V, typename S>
-__device__ __forceinline__ void spmv_entries_chunk(int r0, int r1, int k0, int k1, const int* __restrict__ rp,
-                                                   const int* __restrict__ col, const T* __restrict__ val,
-                                                   const V* __restrict__ x, V* __restrict__ y,
-                                                   const V* __restrict__ w1, const V* __restrict__ w2,
+// from LDS and adds y.w1, y.w2 to acc.  IP, MP, XP, YP: the pointer types of
+// the pattern, the values, the vectors read and y (plain pointers, or
+// global-memory ones where they come from a batch member's descriptor).
+template <int B, typename T, typename V, typename S, typename IP, typename MP, typename XP, typename YP>
+__device__ __forceinline__ void spmv_entries_chunk(int r0, int r1, int k0, int k1, IP __restrict__ rp,
+                                                   IP __restrict__ col, MP __restrict__ val, XP __restrict__ x,
+                                                   YP __restrict__ y, XP __restrict__ w1, XP __restrict__ w2,
                                                    double (&acc)[2], S* __restrict__ staged) {
   const int ne = (k1 - k0) * B;
   constexpr int U = 4;  // independent entries per thread in flight
@@ -251,7 +252,7 @@ __device__ __forceinline__ void spmv_entries_chunk(int r0, int r1, int k0, int k
     for (int u = 0; u < U; ++u) {
       const int e = e0 + u * kThreads;
       const int kb = k0 + (e < ne ? e : 0) / B, i = (e < ne ? e : 0) % B;
-      const T* a = val + (size_t)kb * (B * B) + i * B;
+      const auto a = val + (size_t)kb * (B * B) + i * B;
 #pragma unroll
       for (int j = 0; j < B; ++j) av[u][j] = __builtin_nontemporal_load(a + j);
       cv[u] = col[kb];
@@ -260,7 +261,7 @@ __device__ __forceinline__ void spmv_entries_chunk(int r0, int r1, int k0, int k
     for (int u = 0; u < U; ++u) {
       const int e = e0 + u * kThreads;
       if (e >= ne) break;
-      const V* xv = x + (size_t)cv[u] * B;
+      const auto xv = x + (size_t)cv[u] * B;
       double s = 0.0;
 #pragma unroll
       for (int j = 0; j < B; ++j) s += (double)av[u][j] * (double)xv[j];
@@ -284,11 +285,10 @@ __device__ __forceinline__ void spmv_entries_chunk(int r0, int r1, int k0, int k
 // Block form: one thread per whole B x B block (B*B contiguous values, one
 // column index, one B-vector of x), writing B products to the stage; halves
 // the per-entry index and gather work for B = 2 and makes FP32 loads 16 bytes.
-template <int B, typename T, typename V, typename S>
-__device__ __forceinline__ void spmv_blocks_chunk(int r0, int r1, int k0, int k1, const int* __restrict__ rp,
-                                                  const int* __restrict__ col, const T* __restrict__ val,
-                                                  const V* __restrict__ x, V* __restrict__ y,
-                                                  const V* __restrict__ w1, const V* __restrict__ w2,
+template <int B, typename T, typename V, typename S, typename IP, typename MP, typename XP, typename YP>
+__device__ __forceinline__ void spmv_blocks_chunk(int r0, int r1, int k0, int k1, IP __restrict__ rp,
+                                                  IP __restrict__ col, MP __restrict__ val, XP __restrict__ x,
+                                                  YP __restrict__ y, XP __restrict__ w1, XP __restrict__ w2,
                                                   double (&acc)[2], S* __restrict__ staged) {
   const int nb = k1 - k0;
   constexpr int U = 2;
@@ -299,7 +299,7 @@ __device__ __forceinline__ void spmv_blocks_chunk(int r0, int r1, int k0, int k1
     for (int u = 0; u < U; ++u) {
       const int b = b0 + u * kThreads;
       const int kb = k0 + (b < nb ? b : 0);
-      const T* a = val + (size_t)kb * (B * B);
+      const auto a = val + (size_t)kb * (B * B);
 #pragma unroll
       for (int j = 0; j < B * B; ++j) av[u][j] = __builtin_nontemporal_load(a + j);
       cv[u] = col[kb];
@@ -308,7 +308,7 @@ __device__ __forceinline__ void spmv_blocks_chunk(int r0, int r1, int k0, int k1
     for (int u = 0; u < U; ++u) {
       const int b = b0 + u * kThreads;
       if (b >= nb) break;
-      const V* xv = x + (size_t)cv[u] * B;
+      const auto xv = x + (size_t)cv[u] * B;
       double xl[B];
 #pragma unroll
       for (int j = 0; j < B; ++j) xl[j] = (double)xv[j];

@@ -372,21 +372,54 @@ __global__ void __launch_bounds__(kThreads) bicg_xr_k(int nslot, V* __restrict__
   double acc[2] = {0.0, 0.0};
   const bool half = sc->half != 0;
   const double alpha = sc->alpha, omega = sc->omega;
+  // each vector's B values of a slot are read together (adjacent accesses
+  // pair into 16-byte loads and stores)
+  if (half) {
 #pragma unroll
-  for (int u = 0; u < kXrSlots; ++u) {
-    const int s = (blockIdx.x * kXrSlots + u) * blockDim.x + threadIdx.x;
-    if (s >= nslot) break;
+    for (int u = 0; u < kXrSlots; ++u) {
+      const int s = (blockIdx.x * kXrSlots + u) * blockDim.x + threadIdx.x;
+      if (s < nslot) {
+        const size_t k0 = (size_t)s * B;
+        V xv[B], pv[B];
 #pragma unroll
-    for (int i = 0; i < B; ++i) {
-      const size_t k = (size_t)s * B + i;
-      if (half) {
-        x[k] = (V)((double)x[k] + alpha * (double)ph[k]);
-      } else {
-        x[k] = (V)((double)x[k] + alpha * (double)ph[k] + omega * (double)sh[k]);
-        const V rv = (V)((double)sv[k] - omega * (double)t[k]);
-        r[k] = rv;
-        acc[0] += (double)rh[k] * (double)rv;
-        acc[1] += (double)rv * (double)rv;
+        for (int i = 0; i < B; ++i) xv[i] = x[k0 + i];
+#pragma unroll
+        for (int i = 0; i < B; ++i) pv[i] = ph[k0 + i];
+#pragma unroll
+        for (int i = 0; i < B; ++i) x[k0 + i] = (V)((double)xv[i] + alpha * (double)pv[i]);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < kXrSlots; ++u) {
+      const int s = (blockIdx.x * kXrSlots + u) * blockDim.x + threadIdx.x;
+      if (s < nslot) {
+        const size_t k0 = (size_t)s * B;
+        V xv[B], pv[B], hv[B], sw[B], tv[B], rv0[B];
+#pragma unroll
+        for (int i = 0; i < B; ++i) xv[i] = x[k0 + i];
+#pragma unroll
+        for (int i = 0; i < B; ++i) pv[i] = ph[k0 + i];
+#pragma unroll
+        for (int i = 0; i < B; ++i) hv[i] = sh[k0 + i];
+#pragma unroll
+        for (int i = 0; i < B; ++i) sw[i] = sv[k0 + i];
+#pragma unroll
+        for (int i = 0; i < B; ++i) tv[i] = t[k0 + i];
+#pragma unroll
+        for (int i = 0; i < B; ++i) rv0[i] = rh[k0 + i];
+        V xn[B], rn[B];
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+          xn[i] = (V)((double)xv[i] + alpha * (double)pv[i] + omega * (double)hv[i]);
+          rn[i] = (V)((double)sw[i] - omega * (double)tv[i]);
+          acc[0] += (double)rv0[i] * (double)rn[i];
+          acc[1] += (double)rn[i] * (double)rn[i];
+        }
+#pragma unroll
+        for (int i = 0; i < B; ++i) x[k0 + i] = xn[i];
+#pragma unroll
+        for (int i = 0; i < B; ++i) r[k0 + i] = rn[i];
       }
     }
   }
