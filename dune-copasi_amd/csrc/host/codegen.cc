@@ -1466,13 +1466,15 @@ __device__ __forceinline__ void cells_apply(CF coef, const T* vs, const int* sl,
 // one tile of vertices: the residual rows and/or the BCSR block rows it owns.
 // MODE fixes the work at compile time (1 residual, 2 Jacobian, 4 apply, 8
 // point coefficients), so each entry point gets registers and LDS for its own
-// path only; MODE 0 takes it from mode_rt (the batched entry point).
-template <int MODE>
-__device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l, const unsigned char* __restrict__ cmask,
-         const double* __restrict__ x, const double* __restrict__ base, double t, double cm, double ca,
-         double* __restrict__ r, double* __restrict__ val, const int* __restrict__ own_start,
-         const unsigned* __restrict__ own_mask, int n0, int n1, int n2, double ox, double oy, double oz, double hx,
-         double hy, double hz, int mode_rt, const double* __restrict__ prm) {
+// path only; MODE 0 takes it from mode_rt (the batched entry point).  GI,
+// GB, GX, GR, GU: the pointer types of the tables, the vectors read and the
+// outputs (plain pointers, or global-memory ones where they come from a
+// batch member's descriptor).
+template <int MODE, typename GI, typename GB, typename GX, typename GR, typename GU>
+__device__ __forceinline__ void tile_body(int tile, GI __restrict__ g2l, GB __restrict__ cmask, GX __restrict__ x,
+         GX __restrict__ base, double t, double cm, double ca, GR __restrict__ r, GR __restrict__ val,
+         GI __restrict__ own_start, GU __restrict__ own_mask, int n0, int n1, int n2, double ox, double oy,
+         double oz, double hx, double hy, double hz, int mode_rt, const double* __restrict__ prm) {
   const int mode = MODE ? MODE : mode_rt;
   __shared__ double xs[NVB * BC];
   __shared__ int sl[NVB];
@@ -1578,7 +1580,7 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
         if (s < 0) continue;
         double acc[AI][BC];
         block_rows<AI>(0, 0, 0, li, lj, lk, i0, cms, work, dv, ca, acc);
-        double* blk = r + (size_t)s * (DCG_B * DCG_B);
+        const auto blk = r + (size_t)s * (DCG_B * DCG_B);
 #pragma unroll
         for (int i = 0; i < AI; ++i)
 #pragma unroll
@@ -1601,7 +1603,7 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
       if constexpr (CELL_APPLY) {
         cells_apply<ApplyTile<TX, TY, TZ>, double>(
             [&](int cl, int wd) { return ((const double*)work)[(size_t)wd * NCB + cl]; }, xs, sl, cms, nullptr, acc4,
-            base, r, ca, dv, (const double*)nullptr, (const double*)nullptr, nullptr);
+            base, r, ca, dv, (GX) nullptr, (GX) nullptr, nullptr);
         return;
       }
     }
@@ -1736,7 +1738,7 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
             if (!slot_offset(a0, a1, a2, half, dx, dy, dz)) continue;
             const int o = (dx + 1) + (DIM > 1 ? 3 * (dy + 1) : 0) + (DIM > 2 ? 9 * (dz + 1) : 0);
             if (!((mk >> o) & 1u)) continue;
-            double* blk = val + (size_t)(st + __popc(mk & ((1u << o) - 1u))) * (DCG_B * DCG_B);
+            const auto blk = val + (size_t)(st + __popc(mk & ((1u << o) - 1u))) * (DCG_B * DCG_B);
             const bool diag = o == NST / 2;
 #pragma unroll
             for (int i = 0; i < DCG_B; ++i)
@@ -1874,7 +1876,7 @@ __device__ __forceinline__ void tile_body(int tile, const int* __restrict__ g2l,
         double acc[AI][BC];
         const int i0 = BC > 4 ? ib : 0;  // first block row of this lane
         block_rows<AI>(dx, dy, dz, li, lj, lk, i0, cms, work, dv, ca, acc);
-        double* blk = val + (size_t)pos * (DCG_B * DCG_B);
+        const auto blk = val + (size_t)pos * (DCG_B * DCG_B);
 #pragma unroll
         for (int i = 0; i < (BC > 4 ? 1 : DCG_B); ++i) {
           const int bi = i0 + i;
@@ -2099,9 +2101,13 @@ extern "C" __global__ void __launch_bounds__(256)
     M.tca[0] = cf[0];
     M.tca[1] = cf[2];
   }
-  %C%::tile_body<8>(w.y, M.g2l[%COMP%], M.cmask[%COMP%], M.vec[xsel], nullptr, cf[0], cf[1], cf[2], M.pdiag,
-                    M.pc[%COMP%], M.own_start, M.own_mask, M.n[0], M.n[1], M.n[2], M.origin[0], M.origin[1],
-                    M.origin[2], M.h[0], M.h[1], M.h[2], 8, M.prm);
+  // (the member's pointers in the global address space: global, not flat, accesses)
+  using GX = %C%::dcg_gp<const double>;
+  using GR = %C%::dcg_gp<double>;
+  %C%::tile_body<8>(w.y, (%C%::dcg_gp<const int>)M.g2l[%COMP%], (%C%::dcg_gp<const unsigned char>)M.cmask[%COMP%],
+                    (GX)M.vec[xsel], (GX) nullptr, cf[0], cf[1], cf[2], (GR)M.pdiag, (GR)M.pc[%COMP%],
+                    (%C%::dcg_gp<const int>)M.own_start, (%C%::dcg_gp<const unsigned>)M.own_mask, M.n[0], M.n[1],
+                    M.n[2], M.origin[0], M.origin[1], M.origin[2], M.h[0], M.h[1], M.h[2], 8, M.prm);
 }
 #define DCG_PBATCH_ENTRY(NAME, T, VEC, PC, TL)                                                                    \
   extern "C" __global__ void __launch_bounds__(TL::NT, TL::MINW)                                                  \
@@ -2137,9 +2143,14 @@ extern "C" __global__ void __launch_bounds__(256)
   const double* cf = coef + 4 * (size_t)w.x;
   if (cf[3] == 0.0) return;
   const dcg_asm_member& M = mem[w.x];
-  %C%::tile_body<0>(w.y, M.g2l[%COMP%], M.cmask[%COMP%], M.vec[xsel], bsel >= 0 ? M.vec[bsel] : nullptr, cf[0], cf[1],
-                 cf[2], rsel >= 0 ? M.vec[rsel] : nullptr, (mode & 2) ? M.val : nullptr, M.own_start, M.own_mask,
-                 M.n[0], M.n[1], M.n[2], M.origin[0], M.origin[1], M.origin[2], M.h[0], M.h[1], M.h[2], mode, M.prm);
+  // (the member's pointers in the global address space: global, not flat, accesses)
+  using GX = %C%::dcg_gp<const double>;
+  using GR = %C%::dcg_gp<double>;
+  %C%::tile_body<0>(w.y, (%C%::dcg_gp<const int>)M.g2l[%COMP%], (%C%::dcg_gp<const unsigned char>)M.cmask[%COMP%],
+                    (GX)M.vec[xsel], bsel >= 0 ? (GX)M.vec[bsel] : (GX) nullptr, cf[0], cf[1], cf[2],
+                    rsel >= 0 ? (GR)M.vec[rsel] : (GR) nullptr, (mode & 2) ? (GR)M.val : (GR) nullptr,
+                    (%C%::dcg_gp<const int>)M.own_start, (%C%::dcg_gp<const unsigned>)M.own_mask, M.n[0], M.n[1],
+                    M.n[2], M.origin[0], M.origin[1], M.origin[2], M.h[0], M.h[1], M.h[2], mode, M.prm);
 }
 )";
 
