@@ -65,20 +65,23 @@ VARIANT_KERNELS = {"lds_blocks": "spmv_blk_k", "lds_entries": "spmv_k", "rows_g8
 
 
 def committed_traffic(workload, precision, chosen_kernel):
-    """HBM bytes per launch of the SpMV from the committed counter summary
-    (rocprofv3 FETCH_SIZE/WRITE_SIZE passes calibrated on a plain copy), when
-    it covers the kernel variant this run uses; None otherwise."""
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    summary (rocprofv3 FETCH_SIZE/WRITE_SIZE passes calibrated on a plain
+    copy; one `kernel =` line, then its `traffic_bytes =`, per kernel
+    variant measured), when it covers the variant this run uses; None
+    otherwise."""
     name = TRAFFIC_FILES.get((workload, precision))
     if not name or not (ROOT / "profiles" / name).exists():
         return None
-    kv = {}
+    kernel, found = None, {}
     for line in (ROOT / "profiles" / name).read_text().splitlines():
         if "=" in line and not line.startswith("#"):
-            k, v = line.split("=", 1)
-            kv[k.strip()] = v.strip()
-    if kv.get("kernel", "") != chosen_kernel:
-        return None
-    return float(kv["traffic_bytes"])
+            k, v = (x.strip() for x in line.split("=", 1))
+            if k == "kernel":
+                kernel = v
+            elif k == "traffic_bytes" and kernel is not None:
+                found[kernel] = float(v)
+    return found.get(chosen_kernel)
 
 
 def _host_threads():
