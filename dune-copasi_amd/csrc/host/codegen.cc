@@ -1889,34 +1889,72 @@ __device__ __forceinline__ void tile_body(int tile, GI __restrict__ g2l, GB __re
     }
     __syncthreads();
   }
+  // box cells with blocks of up to 4 species: a cell's corner sums stay in
+  // registers and go to LDS once, component-major (corner c, species i of
+  // box cell cl at work[(c * BC + i) * NCB + cl]), so the lanes of a wave
+  // (consecutive cells) store and the rows read consecutive words; larger
+  // blocks and simplex classes sum into a per-cell array
+  // (work[cl * NCORNER * BC + c * BC + i])
+  constexpr bool RES_CM = BC <= 4 && NCLASS == 1;
   if (mode & 1) {
     // phase 1: element residual per cell, folded onto the cell's corners
     for (int cl = threadIdx.x; cl < NCB; cl += blockDim.x) {
-      double* dst = work + (size_t)cl * (NCORNER * BC);
-      for (int k = 0; k < NCORNER * BC; ++k) dst[k] = 0.0;
       const unsigned m = cms[cl];
-      if (!m) continue;
       const int ci = cl % CXB, cj = (cl / CXB) % CYB, ck = cl / (CXB * CYB);
       const double cx = ox + (vi0 + ci) * hx, cy = oy + (vj0 + cj) * hy, cz = oz + (vk0 + ck) * hz;
-      for (int p = 0; p < NCLASS; ++p) {
-        if (!((m >> p) & 1u)) continue;
-        if constexpr (BC > 4) {
-          int bidx[NV];
-          for (int a = 0; a < NV; ++a) {
-            const int c = CORNER[p][a];
-            bidx[a] = (ci + (c & 1)) + XB * ((cj + ((c >> 1) & 1)) + YB * (ck + ((c >> 2) & 1)));
+      if constexpr (RES_CM) {
+        double oc[NCORNER][BC];
+#pragma unroll
+        for (int c = 0; c < NCORNER; ++c)
+#pragma unroll
+          for (int i = 0; i < BC; ++i) oc[c][i] = 0.0;
+        if (m) {
+#pragma unroll
+          for (int p = 0; p < NCLASS; ++p) {
+            if (!((m >> p) & 1u)) continue;
+            double xl[NV][BC], rl[NV][BC];
+#pragma unroll
+            for (int a = 0; a < NV; ++a) {
+              const int c = CORNER[p][a];
+              const int b = (ci + (c & 1)) + XB * ((cj + ((c >> 1) & 1)) + YB * (ck + ((c >> 2) & 1)));
+#pragma unroll
+              for (int i = 0; i < BC; ++i) xl[a][i] = xs[b * BC + i];
+            }
+            residual_rt(p, xl, cx, cy, cz, t, cm, ca, rl, prm);
+#pragma unroll
+            for (int a = 0; a < NV; ++a)
+#pragma unroll
+              for (int i = 0; i < BC; ++i) oc[CORNER[p][a]][i] += rl[a][i];
           }
-          residual_lds(p, xs, bidx, cx, cy, cz, t, cm, ca, dst, prm);
-        } else {
-          double xl[NV][BC], rl[NV][BC];
-          for (int a = 0; a < NV; ++a) {
-            const int c = CORNER[p][a];
-            const int b = (ci + (c & 1)) + XB * ((cj + ((c >> 1) & 1)) + YB * (ck + ((c >> 2) & 1)));
-            for (int i = 0; i < BC; ++i) xl[a][i] = xs[b * BC + i];
+        }
+#pragma unroll
+        for (int c = 0; c < NCORNER; ++c)
+#pragma unroll
+          for (int i = 0; i < BC; ++i) work[(size_t)(c * BC + i) * NCB + cl] = oc[c][i];
+      } else {
+        double* dst = work + (size_t)cl * (NCORNER * BC);
+        for (int k = 0; k < NCORNER * BC; ++k) dst[k] = 0.0;
+        if (!m) continue;
+        for (int p = 0; p < NCLASS; ++p) {
+          if (!((m >> p) & 1u)) continue;
+          if constexpr (BC > 4) {
+            int bidx[NV];
+            for (int a = 0; a < NV; ++a) {
+              const int c = CORNER[p][a];
+              bidx[a] = (ci + (c & 1)) + XB * ((cj + ((c >> 1) & 1)) + YB * (ck + ((c >> 2) & 1)));
+            }
+            residual_lds(p, xs, bidx, cx, cy, cz, t, cm, ca, dst, prm);
+          } else {
+            double xl[NV][BC], rl[NV][BC];
+            for (int a = 0; a < NV; ++a) {
+              const int c = CORNER[p][a];
+              const int b = (ci + (c & 1)) + XB * ((cj + ((c >> 1) & 1)) + YB * (ck + ((c >> 2) & 1)));
+              for (int i = 0; i < BC; ++i) xl[a][i] = xs[b * BC + i];
+            }
+            residual_rt(p, xl, cx, cy, cz, t, cm, ca, rl, prm);
+            for (int a = 0; a < NV; ++a)
+              for (int i = 0; i < BC; ++i) dst[CORNER[p][a] * BC + i] += rl[a][i];
           }
-          residual_rt(p, xl, cx, cy, cz, t, cm, ca, rl, prm);
-          for (int a = 0; a < NV; ++a)
-            for (int i = 0; i < BC; ++i) dst[CORNER[p][a] * BC + i] += rl[a][i];
         }
       }
     }
@@ -1936,7 +1974,7 @@ __device__ __forceinline__ void tile_body(int tile, GI __restrict__ g2l, GB __re
               const int cl = (li + ex) + CXB * ((lj + ey) + CYB * (lk + ez));
               if (!cms[cl]) continue;
               const int cv = (1 - ex) | (DIM > 1 ? (1 - ey) << 1 : 0) | (DIM > 2 ? (1 - ez) << 2 : 0);
-              v += work[(size_t)cl * (NCORNER * BC) + cv * BC + i];
+              v += RES_CM ? work[(size_t)(cv * BC + i) * NCB + cl] : work[(size_t)cl * (NCORNER * BC) + cv * BC + i];
             }
       }
       r[(size_t)s * DCG_B + i] = v;
