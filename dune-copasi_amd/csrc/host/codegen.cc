@@ -1076,7 +1076,10 @@ constexpr int MC_WORDS = NCB * NCLASS * NQ * (NMC > 0 ? NMC : 1);
 // cell's stored point coefficients live once, in its owner's block
 constexpr int NOWN = TX * (DIM > 1 ? TY : 1) * (DIM > 2 ? TZ : 1);
 constexpr int OWN_WORDS = NCLASS * NQ * (NMC > 0 ? NMC : 1) * NOWN;
-constexpr int RL_WORDS = NCB * NCORNER * BC;
+// residual corner sums: component-major for box cells with blocks of up to
+// 4 species, else per cell with an odd stride (RL_CELL doubles)
+constexpr int RL_CELL = NCORNER * BC + 1;
+constexpr int RL_WORDS = NCB * RL_CELL;
 // the Jacobian paths also pass row sums between the two lanes of a row
 constexpr int XCH_WORDS = NROW * ((NZB > BC ? NZB : BC) > 0 ? (NZB > BC ? NZB : BC) : 1);
 constexpr int JW_WORDS = MC_WORDS > XCH_WORDS ? MC_WORDS : XCH_WORDS;
@@ -1889,20 +1892,21 @@ __device__ __forceinline__ void tile_body(int tile, GI __restrict__ g2l, GB __re
     }
     __syncthreads();
   }
-  // box cells with blocks of up to 4 species: a cell's corner sums stay in
-  // registers and go to LDS once, component-major (corner c, species i of
+  // Box cells with blocks of up to 4 species keep a cell's corner sums in
+  // registers and store them once, component-major (corner c, species i of
   // box cell cl at work[(c * BC + i) * NCB + cl]), so the lanes of a wave
   // (consecutive cells) store and the rows read consecutive words; larger
-  // blocks and simplex classes sum into a per-cell array
-  // (work[cl * NCORNER * BC + c * BC + i])
-  constexpr bool RES_CM = BC <= 4 && NCLASS == 1;
+  // blocks and simplex classes add into a per-cell array whose odd stride
+  // (RL_CELL doubles) spreads the lanes over the LDS banks
+  // (work[cl * RL_CELL + c * BC + i]).
+  constexpr bool RES_REG = BC <= 4 && NCLASS == 1;
   if (mode & 1) {
     // phase 1: element residual per cell, folded onto the cell's corners
     for (int cl = threadIdx.x; cl < NCB; cl += blockDim.x) {
       const unsigned m = cms[cl];
       const int ci = cl % CXB, cj = (cl / CXB) % CYB, ck = cl / (CXB * CYB);
       const double cx = ox + (vi0 + ci) * hx, cy = oy + (vj0 + cj) * hy, cz = oz + (vk0 + ck) * hz;
-      if constexpr (RES_CM) {
+      if constexpr (RES_REG) {
         double oc[NCORNER][BC];
 #pragma unroll
         for (int c = 0; c < NCORNER; ++c)
@@ -1932,7 +1936,7 @@ __device__ __forceinline__ void tile_body(int tile, GI __restrict__ g2l, GB __re
 #pragma unroll
           for (int i = 0; i < BC; ++i) work[(size_t)(c * BC + i) * NCB + cl] = oc[c][i];
       } else {
-        double* dst = work + (size_t)cl * (NCORNER * BC);
+        double* dst = work + (size_t)cl * RL_CELL;
         for (int k = 0; k < NCORNER * BC; ++k) dst[k] = 0.0;
         if (!m) continue;
         for (int p = 0; p < NCLASS; ++p) {
@@ -1974,7 +1978,7 @@ __device__ __forceinline__ void tile_body(int tile, GI __restrict__ g2l, GB __re
               const int cl = (li + ex) + CXB * ((lj + ey) + CYB * (lk + ez));
               if (!cms[cl]) continue;
               const int cv = (1 - ex) | (DIM > 1 ? (1 - ey) << 1 : 0) | (DIM > 2 ? (1 - ez) << 2 : 0);
-              v += RES_CM ? work[(size_t)(cv * BC + i) * NCB + cl] : work[(size_t)cl * (NCORNER * BC) + cv * BC + i];
+              v += RES_REG ? work[(size_t)(cv * BC + i) * NCB + cl] : work[(size_t)cl * RL_CELL + cv * BC + i];
             }
       }
       r[(size_t)s * DCG_B + i] = v;
@@ -2255,7 +2259,7 @@ std::array<int, 3> choose_tile(const Grid& g, int bc, int nmc) {
     long xb = c[0] + 2, yb = g.dim > 1 ? c[1] + 2 : 1, zb = g.dim > 2 ? c[2] + 2 : 1;
     long cxb = c[0] + 1, cyb = g.dim > 1 ? c[1] + 1 : 1, czb = g.dim > 2 ? c[2] + 1 : 1;
     long ncb = cxb * cyb * czb;
-    long mc = ncb * g.nclass * nq * std::max(1, nmc), rl = ncb * g.ncorner * bc;
+    long mc = ncb * g.nclass * nq * std::max(1, nmc), rl = ncb * (g.ncorner * bc + 1);
     long bytes = 8 * std::max(mc, rl) + 8 * xb * yb * zb * bc + 4 * xb * yb * zb + ncb;
     if (bytes <= 64 * 1024) return c;
   }
