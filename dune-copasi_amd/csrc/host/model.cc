@@ -1318,7 +1318,7 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
       reinterpret_cast<const int*>(reinterpret_cast<const char*>(sc) + offsetof(k::BicgScalars, done));
   // the partial dots' finaliser; stage 3 first finalises the s-update's
   // partial norms (stage 2, deferred past the second operator launch)
-  const int nblk_s = (ns + 255) / 256;
+  const int nblk_s = k::bicg_s_parts(ns);
   auto finalise = [&](int nparts, int stage) {
     if (stage == 3) k::bicg_fin23(nblk_s, w.spart, nparts, part, sc, stream_);
     else k::bicg_fin(nparts, part, 2, stage, sc, stream_);
@@ -1418,7 +1418,7 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
   const int* done_flag =
       reinterpret_cast<const int*>(reinterpret_cast<const char*>(sc_.get()) + offsetof(k::BicgScalars, done));
   double* spart = part_.get() + spart_off_;
-  const int nblk_s = (ns + 255) / 256;
+  const int nblk_s = k::bicg_s_parts(ns);
   auto finalise = [&](int nparts, int stage) {
     if (stage == 3) k::bicg_fin23(nblk_s, spart, nparts, part_.get(), sc_.get(), stream_);
     else k::bicg_fin(nparts, part_.get(), 2, stage, sc_.get(), stream_);

@@ -225,6 +225,8 @@ void bicg_p(int B, int nslot, const V* r, V* p, const V* v, V* ph, const double*
 template <typename V>
 void bicg_s(int B, int nslot, const V* r, const V* v, V* sv, V* sh, const double* dinv, bool diag, const BicgScalars* sc,
             double* part, hipStream_t s);
+// workgroup partials bicg_s writes for nslot slots
+int bicg_s_parts(int nslot);
 // returns the number of partial pairs it writes (the finaliser's count)
 template <typename V>
 int bicg_xr(int B, int nslot, V* x, const V* ph, const V* sh, const V* sv, const V* t, V* r, const V* rh,

@@ -284,27 +284,43 @@ __global__ void block_inverse_k(int nslot, const double* __restrict__ val, const
 // the solve has finished, so the host enqueues iterations without waiting.
 __device__ __forceinline__ bool finished(const BicgScalars* sc) { return sc->done != 0; }
 
+// The p- and s-updates take kVecSlots slots per thread (kThreads apart, so
+// every pass of the workgroup is contiguous): more bytes in flight per
+// thread, and a quarter of the s-update's workgroup partials.
+constexpr int kVecSlots = 4;
+
 // p = r + beta (p - omega v); ph = M^-1 p (Jacobi fused, else copy)
 template <int B, typename V>
-__global__ void bicg_p_k(int nslot, const V* __restrict__ r, V* __restrict__ p, const V* __restrict__ v,
-                         V* __restrict__ ph, const double* __restrict__ dinv, int diag,
-                         const BicgScalars* __restrict__ sc) {
+__global__ void __launch_bounds__(kThreads) bicg_p_k(int nslot, const V* __restrict__ r, V* __restrict__ p,
+                                                     const V* __restrict__ v, V* __restrict__ ph,
+                                                     const double* __restrict__ dinv, int diag,
+                                                     const BicgScalars* __restrict__ sc) {
   if (finished(sc)) return;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nslot) return;
   const double beta = sc->beta, omega = sc->omega;
-  V pv[B];
 #pragma unroll
-  for (int i = 0; i < B; ++i) {
-    const size_t k = (size_t)s * B + i;
-    const double nv = (double)r[k] + beta * ((double)p[k] - omega * (double)v[k]);
-    pv[i] = (V)nv;
-    p[k] = pv[i];
+  for (int u = 0; u < kVecSlots; ++u) {
+    const int s = (blockIdx.x * kVecSlots + u) * blockDim.x + threadIdx.x;
+    if (s < nslot) {
+      const size_t k0 = (size_t)s * B;
+      V rv[B], po[B], vv[B], pv[B], o[B];
+#pragma unroll
+      for (int i = 0; i < B; ++i) rv[i] = r[k0 + i];
+#pragma unroll
+      for (int i = 0; i < B; ++i) po[i] = p[k0 + i];
+#pragma unroll
+      for (int i = 0; i < B; ++i) vv[i] = v[k0 + i];
+#pragma unroll
+      for (int i = 0; i < B; ++i) pv[i] = (V)((double)rv[i] + beta * ((double)po[i] - omega * (double)vv[i]));
+#pragma unroll
+      for (int i = 0; i < B; ++i) p[k0 + i] = pv[i];
+      if (dinv) dinv_apply<B>(dinv, s, pv, o, diag != 0);
+      else
+#pragma unroll
+        for (int i = 0; i < B; ++i) o[i] = pv[i];
+#pragma unroll
+      for (int i = 0; i < B; ++i) ph[k0 + i] = o[i];
+    }
   }
-  if (dinv) dinv_apply<B>(dinv, s, pv, ph + (size_t)s * B, diag != 0);
-  else
-#pragma unroll
-    for (int i = 0; i < B; ++i) ph[(size_t)s * B + i] = pv[i];
 }
 
 // s = r - alpha v; sh = M^-1 s; partial ||s||^2
@@ -315,22 +331,32 @@ __global__ void __launch_bounds__(kThreads) bicg_s_k(int nslot, const V* __restr
                                                      const BicgScalars* __restrict__ sc, double* __restrict__ part) {
   if (finished(sc)) return;
   __shared__ double red[16];
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[1] = {0.0};
-  if (s < nslot) {
-    const double alpha = sc->alpha;
-    V t[B];
+  const double alpha = sc->alpha;
 #pragma unroll
-    for (int i = 0; i < B; ++i) {
-      const size_t k = (size_t)s * B + i;
-      t[i] = (V)((double)r[k] - alpha * (double)v[k]);
-      sv[k] = t[i];
-      acc[0] += (double)t[i] * (double)t[i];
+  for (int u = 0; u < kVecSlots; ++u) {
+    const int s = (blockIdx.x * kVecSlots + u) * blockDim.x + threadIdx.x;
+    if (s < nslot) {
+      const size_t k0 = (size_t)s * B;
+      V rv[B], vv[B], t[B], o[B];
+#pragma unroll
+      for (int i = 0; i < B; ++i) rv[i] = r[k0 + i];
+#pragma unroll
+      for (int i = 0; i < B; ++i) vv[i] = v[k0 + i];
+#pragma unroll
+      for (int i = 0; i < B; ++i) {
+        t[i] = (V)((double)rv[i] - alpha * (double)vv[i]);
+        acc[0] += (double)t[i] * (double)t[i];
+      }
+#pragma unroll
+      for (int i = 0; i < B; ++i) sv[k0 + i] = t[i];
+      if (dinv) dinv_apply<B>(dinv, s, t, o, diag != 0);
+      else
+#pragma unroll
+        for (int i = 0; i < B; ++i) o[i] = t[i];
+#pragma unroll
+      for (int i = 0; i < B; ++i) sh[k0 + i] = o[i];
     }
-    if (dinv) dinv_apply<B>(dinv, s, t, sh + (size_t)s * B, diag != 0);
-    else
-#pragma unroll
-      for (int i = 0; i < B; ++i) sh[(size_t)s * B + i] = t[i];
   }
   block_sum(acc, red);
   if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
@@ -1317,7 +1343,9 @@ template void bicg_init<float>(long, const double*, float*, float*, float*, floa
 template <typename V>
 void bicg_p(int B, int nslot, const V* r, V* p, const V* v, V* ph, const double* dinv, bool diag, const BicgScalars* sc,
             hipStream_t s) {
-  DCG_DISPATCH_B(B, (bicg_p_k<BB, V><<<grid_for(nslot), kThreads, 0, s>>>(nslot, r, p, v, ph, dinv, diag ? 1 : 0, sc)));
+  const int grid = (nslot + kThreads * kVecSlots - 1) / (kThreads * kVecSlots);
+  if (grid > 0)
+    DCG_DISPATCH_B(B, (bicg_p_k<BB, V><<<grid, kThreads, 0, s>>>(nslot, r, p, v, ph, dinv, diag ? 1 : 0, sc)));
 }
 template void bicg_p<double>(int, int, const double*, double*, const double*, double*, const double*, bool,
                              const BicgScalars*, hipStream_t);
@@ -1327,8 +1355,11 @@ template void bicg_p<float>(int, int, const float*, float*, const float*, float*
 template <typename V>
 void bicg_s(int B, int nslot, const V* r, const V* v, V* sv, V* sh, const double* dinv, bool diag, const BicgScalars* sc,
             double* part, hipStream_t s) {
-  DCG_DISPATCH_B(B, (bicg_s_k<BB, V><<<grid_for(nslot), kThreads, 0, s>>>(nslot, r, v, sv, sh, dinv, diag ? 1 : 0, sc, part)));
+  const int grid = bicg_s_parts(nslot);
+  if (grid > 0)
+    DCG_DISPATCH_B(B, (bicg_s_k<BB, V><<<grid, kThreads, 0, s>>>(nslot, r, v, sv, sh, dinv, diag ? 1 : 0, sc, part)));
 }
+int bicg_s_parts(int nslot) { return (nslot + kThreads * kVecSlots - 1) / (kThreads * kVecSlots); }
 template void bicg_s<double>(int, int, const double*, const double*, double*, double*, const double*, bool,
                              const BicgScalars*, double*, hipStream_t);
 template void bicg_s<float>(int, int, const float*, const float*, float*, float*, const double*, bool,
