@@ -196,7 +196,7 @@ void Batch::rebuild() {
   for (auto& g : sgroups_) {
     const int M = static_cast<int>(g.mem.size());
     std::vector<k::MemberView> mv(M);
-    std::vector<int2> vwork, vrange(M), crange(M);
+    std::vector<int2> vwork, vrange(M), vwork4, vrange4(M), crange(M);
     std::vector<int4> cwork;
     for (int j = 0; j < M; ++j) {
       const Model& m = *members_[g.mem[j]].m;
@@ -205,6 +205,10 @@ void Batch::rebuild() {
       vrange[j].x = static_cast<int>(vwork.size());
       for (int b = 0; b < nblk; ++b) vwork.push_back(make_int2(j, b));
       vrange[j].y = static_cast<int>(vwork.size());
+      const int nblk4 = (mv[j].nslot + 256 * k::kBVecSlots - 1) / (256 * k::kBVecSlots);
+      vrange4[j].x = static_cast<int>(vwork4.size());
+      for (int b = 0; b < nblk4; ++b) vwork4.push_back(make_int2(j, b));
+      vrange4[j].y = static_cast<int>(vwork4.size());
       crange[j].x = static_cast<int>(cwork.size());
       const auto& ch = m.chunk_host_;
       const auto& rp = m.L_.row_ptr;
@@ -221,6 +225,8 @@ void Batch::rebuild() {
     g.mv.upload(mv, stream_);
     g.vwork.upload(vwork, stream_);
     g.vrange.upload(vrange, stream_);
+    g.vwork4.upload(vwork4, stream_);
+    g.vrange4.upload(vrange4, stream_);
     g.cwork.upload(cwork, stream_);
     g.crange.upload(crange, stream_);
     g.sc.resize(M);
@@ -232,6 +238,9 @@ void Batch::rebuild() {
     g.w.vwork = g.vwork.get();
     g.w.nvwork = static_cast<int>(vwork.size());
     g.w.vrange = g.vrange.get();
+    g.w.vwork4 = g.vwork4.get();
+    g.w.nvwork4 = static_cast<int>(vwork4.size());
+    g.w.vrange4 = g.vrange4.get();
     g.w.cwork = g.cwork.get();
     g.w.ncwork = static_cast<int>(cwork.size());
     g.w.crange = g.crange.get();

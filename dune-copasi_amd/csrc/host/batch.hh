@@ -106,7 +106,7 @@ class Batch {
     bool points = false;
     std::vector<int> mem;  // indices into members_
     DeviceArray<k::MemberView> mv;
-    DeviceArray<int2> vwork, vrange, crange;
+    DeviceArray<int2> vwork, vrange, vwork4, vrange4, crange;
     DeviceArray<int4> cwork;
     k::BatchWork w;
     DeviceArray<k::BicgScalars> sc;

@@ -243,6 +243,10 @@ __global__ void __launch_bounds__(kThreads) binit_k(const int2* __restrict__ wor
   if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
 }
 
+// The BiCGSTAB updates go over vwork4 items (member, block of kBVecSlots *
+// kThreads slots): each thread takes kBVecSlots slots kThreads apart, so
+// every pass of the workgroup is contiguous and more bytes are in flight.
+
 // p = r + beta (p - omega v); ph = M^-1 p
 template <int B, typename V>
 __global__ void __launch_bounds__(kThreads) bp_k(const int2* __restrict__ work, const MemberView* __restrict__ mv,
@@ -251,32 +255,35 @@ __global__ void __launch_bounds__(kThreads) bp_k(const int2* __restrict__ work, 
   const BicgScalars& S = sc[w.x];
   if (S.done) return;
   const MemberView& m = mv[w.x];
-  const int s = w.y * kThreads + threadIdx.x;
-  if (s >= m.nslot) return;
   const auto r = in_global(vsel<V>(m, kVKry + 0));
   const auto p = in_global(vsel<V>(m, kVKry + 2));
   const auto v = in_global(vsel<V>(m, kVKry + 3));
   const auto ph = in_global(vsel<V>(m, kVKry + 4));
   const double beta = S.beta, omega = S.omega;
-  // each vector's B values read together (adjacent accesses pair up)
-  const size_t k0 = (size_t)s * B;
-  V rv[B], pold[B], vv[B], pv[B], o[B];
 #pragma unroll
-  for (int i = 0; i < B; ++i) rv[i] = r[k0 + i];
+  for (int u = 0; u < kBVecSlots; ++u) {
+    const int s = (w.y * kBVecSlots + u) * kThreads + threadIdx.x;
+    if (s >= m.nslot) break;
+    // each vector's B values read together (adjacent accesses pair up)
+    const size_t k0 = (size_t)s * B;
+    V rv[B], pold[B], vv[B], pv[B], o[B];
 #pragma unroll
-  for (int i = 0; i < B; ++i) pold[i] = p[k0 + i];
+    for (int i = 0; i < B; ++i) rv[i] = r[k0 + i];
 #pragma unroll
-  for (int i = 0; i < B; ++i) vv[i] = v[k0 + i];
+    for (int i = 0; i < B; ++i) pold[i] = p[k0 + i];
 #pragma unroll
-  for (int i = 0; i < B; ++i) pv[i] = (V)((double)rv[i] + beta * ((double)pold[i] - omega * (double)vv[i]));
+    for (int i = 0; i < B; ++i) vv[i] = v[k0 + i];
 #pragma unroll
-  for (int i = 0; i < B; ++i) p[k0 + i] = pv[i];
-  if (jacobi) dinv_apply<B>((const double*)in_global(m.dinv), s, pv, o, jacobi == 2);
-  else
+    for (int i = 0; i < B; ++i) pv[i] = (V)((double)rv[i] + beta * ((double)pold[i] - omega * (double)vv[i]));
 #pragma unroll
-    for (int i = 0; i < B; ++i) o[i] = pv[i];
+    for (int i = 0; i < B; ++i) p[k0 + i] = pv[i];
+    if (jacobi) dinv_apply<B>((const double*)in_global(m.dinv), s, pv, o, jacobi == 2);
+    else
 #pragma unroll
-  for (int i = 0; i < B; ++i) ph[k0 + i] = o[i];
+      for (int i = 0; i < B; ++i) o[i] = pv[i];
+#pragma unroll
+    for (int i = 0; i < B; ++i) ph[k0 + i] = o[i];
+  }
 }
 
 // s = r - alpha v; sh = M^-1 s; partial ||s||^2
@@ -289,14 +296,16 @@ __global__ void __launch_bounds__(kThreads) bs_k(const int2* __restrict__ work, 
   const BicgScalars& S = sc[w.x];
   if (S.done) return;  // uniform over the workgroup
   const MemberView& m = mv[w.x];
-  const int s = w.y * kThreads + threadIdx.x;
   double acc[1] = {0.0};
-  if (s < m.nslot) {
-    const auto r = in_global(vsel<V>(m, kVKry + 0));
-    const auto v = in_global(vsel<V>(m, kVKry + 3));
-    const auto sv = in_global(vsel<V>(m, kVKry + 5));
-    const auto sh = in_global(vsel<V>(m, kVKry + 6));
-    const double alpha = S.alpha;
+  const auto r = in_global(vsel<V>(m, kVKry + 0));
+  const auto v = in_global(vsel<V>(m, kVKry + 3));
+  const auto sv = in_global(vsel<V>(m, kVKry + 5));
+  const auto sh = in_global(vsel<V>(m, kVKry + 6));
+  const double alpha = S.alpha;
+#pragma unroll
+  for (int u = 0; u < kBVecSlots; ++u) {
+    const int s = (w.y * kBVecSlots + u) * kThreads + threadIdx.x;
+    if (s >= m.nslot) break;
     const size_t k0 = (size_t)s * B;
     V rv[B], vv[B], t[B], o[B];
 #pragma unroll
@@ -331,18 +340,20 @@ __global__ void __launch_bounds__(kThreads) bxr_k(const int2* __restrict__ work,
   const BicgScalars& S = sc[w.x];
   if (S.done) return;
   const MemberView& m = mv[w.x];
-  const int s = w.y * kThreads + threadIdx.x;
   double acc[2] = {0.0, 0.0};
   const bool half = S.half != 0;
-  if (s < m.nslot) {
-    const auto xv = in_global(vsel<V>(m, x));
-    const auto r = in_global(vsel<V>(m, kVKry + 0));
-    const auto rh = in_global(vsel<V>(m, kVKry + 1));
-    const auto ph = in_global(vsel<V>(m, kVKry + 4));
-    const auto sv = in_global(vsel<V>(m, kVKry + 5));
-    const auto sh = in_global(vsel<V>(m, kVKry + 6));
-    const auto t = in_global(vsel<V>(m, kVKry + 7));
-    const double alpha = S.alpha, omega = S.omega;
+  const auto xv = in_global(vsel<V>(m, x));
+  const auto r = in_global(vsel<V>(m, kVKry + 0));
+  const auto rh = in_global(vsel<V>(m, kVKry + 1));
+  const auto ph = in_global(vsel<V>(m, kVKry + 4));
+  const auto sv = in_global(vsel<V>(m, kVKry + 5));
+  const auto sh = in_global(vsel<V>(m, kVKry + 6));
+  const auto t = in_global(vsel<V>(m, kVKry + 7));
+  const double alpha = S.alpha, omega = S.omega;
+#pragma unroll
+  for (int u = 0; u < kBVecSlots; ++u) {
+    const int s = (w.y * kBVecSlots + u) * kThreads + threadIdx.x;
+    if (s >= m.nslot) break;
     const size_t k0 = (size_t)s * B;
     V x0[B], pv[B];
 #pragma unroll
@@ -600,19 +611,19 @@ void b_bicg_iteration(int B, int variant, const BatchWork& w, const MemberView* 
     DCG_BATCH_B(B, (bspmv_launch<BB, T, V>(variant, w, mv, none, sc, xs, ys, w1, w2, part, st)));
   };
   DCG_BATCH_B(B, {
-    bp_k<BB, V><<<w.nvwork, kThreads, 0, st>>>(w.vwork, mv, jac, sc);
+    bp_k<BB, V><<<w.nvwork4, kThreads, 0, st>>>(w.vwork4, mv, jac, sc);
     mark(0);
     matvec(kVKry + 4, kVKry + 3, kVKry + 1, -1);
     mark(1);
     bfin_k<<<w.members, kThreads, 0, st>>>(orange, part, 2, 1, sc);
-    bs_k<BB, V><<<w.nvwork, kThreads, 0, st>>>(w.vwork, mv, jac, sc, part);
-    bfin_k<<<w.members, kThreads, 0, st>>>(w.vrange, part, 1, 2, sc);
+    bs_k<BB, V><<<w.nvwork4, kThreads, 0, st>>>(w.vwork4, mv, jac, sc, part);
+    bfin_k<<<w.members, kThreads, 0, st>>>(w.vrange4, part, 1, 2, sc);
     mark(2);
     matvec(kVKry + 6, kVKry + 7, kVKry + 5, kVKry + 7);
     mark(3);
     bfin_k<<<w.members, kThreads, 0, st>>>(orange, part, 2, 3, sc);
-    bxr_k<BB, V><<<w.nvwork, kThreads, 0, st>>>(w.vwork, mv, x, sc, part);
-    bfin_k<<<w.members, kThreads, 0, st>>>(w.vrange, part, 2, 4, sc);
+    bxr_k<BB, V><<<w.nvwork4, kThreads, 0, st>>>(w.vwork4, mv, x, sc, part);
+    bfin_k<<<w.members, kThreads, 0, st>>>(w.vrange4, part, 2, 4, sc);
   });
 }
 template void b_bicg_iteration<double>(int, int, const BatchWork&, const MemberView*, int, int, double*,

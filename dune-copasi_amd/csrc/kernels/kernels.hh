@@ -112,14 +112,19 @@ struct Skip {
   int when = 0;
 };
 
-// work lists: (member, 256-slot block) for vectors; for SpMV one item per
-// chunk of rows, {member, first row, first block, rows << 16 | blocks}, so a
-// workgroup starts streaming after a single dependent load; range[m] =
-// [first, last) work items of member m
+// work lists: (member, 256-slot block) for vectors, (member, block of
+// kBVecSlots * 256 slots) for the BiCGSTAB updates (vwork4, vrange4); for
+// SpMV one item per chunk of rows, {member, first row, first block, rows <<
+// 16 | blocks}, so a workgroup starts streaming after a single dependent
+// load; range[m] = [first, last) work items of member m
+constexpr int kBVecSlots = 4;
 struct BatchWork {
   const int2* vwork = nullptr;
   int nvwork = 0;
   const int2* vrange = nullptr;
+  const int2* vwork4 = nullptr;
+  int nvwork4 = 0;
+  const int2* vrange4 = nullptr;
   const int4* cwork = nullptr;
   int ncwork = 0;
   const int2* crange = nullptr;
