@@ -223,7 +223,7 @@ def test_time_steps_match_oracle(dca, wl, name, make, steps):
         err = max(rel_l2_per_species(gpu.compartments(), ug, uo))
         assert err <= 1e-8, (k, err)
         assert abs(tg - t) < 1e-12
-    # the survey's accuracy measure: per species, relative, in the mass-matrix norm
+    # the parity bar also in the mass-matrix norm: per species, relative
     errm = max(rel_mass_l2_per_species(orc, ug, uo, t))
     assert errm <= 1e-8, errm
 

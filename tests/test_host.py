@@ -2,7 +2,7 @@
 
 * libdcopasi.so loads and exports every function include/dcopasi.h declares;
 * the host-only model (device < 0) builds the grid, compartments and BCSR
-  pattern: block counts equal the survey's counts for the benchmark meshes,
+  pattern: block counts equal the stencil counts of the benchmark meshes,
   compartment node sets and the sparsity equal the oracle's (which is built by
   independent code), and bad input fails with the reference's messages;
 * the expression compiler, exercised through `initial.expression`, agrees
