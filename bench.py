@@ -525,8 +525,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true",
                     help="single-model FP64 runs: skip the mixed-precision run of the same workload")
-    ap.add_argument("--copy-probe-mib", type=int, default=0,
-                    help="also run a plain device copy of this size (counter calibration under rocprofv3)")
+    ap.add_argument("--copy-probe-mib", type=int, default=1024,
+                    help="after the timed steps, a plain device copy of this size each way: the box's copy"
+                         " bandwidth beside the roofline, and the counter calibration under rocprofv3 (0: none)")
     ap.add_argument("--ir-inner", type=float, default=1e-3,
                     help="mixed precision: relative tolerance of each FP32 Krylov solve inside the FP64 refinement")
     ap.add_argument("--members", type=int, default=256, help="batch workload: number of models")
