@@ -722,7 +722,7 @@ void Batch::iterate(SolverGroup& g, bool fp32) {
         if (sc[j].done) continue;
         all = false;
         const Model& mj = *members_[g.mem[j]].m;
-        bytes += mj.point_storage() ? mj.point_apply_bytes(fp32 ? 4 : 8) : mj.spmv_bytes(fp32 ? 4 : 8, fp32 ? 4 : 8, 1);
+        bytes += mj.point_storage() ? mj.point_apply_bytes(fp32 ? 4 : 8, 1) : mj.spmv_bytes(fp32 ? 4 : 8, fp32 ? 4 : 8, 1);
       }
       if (all || done_its > opt_.lin_maxit + 4) break;
       if (fp32)

@@ -1335,7 +1335,7 @@ int Model::bicgstab64(const double* b, double* x, double rtol, double abstol, in
     if (use_points_) {
       prof_begin(kPointApply64);
       const int np = launch_points_apply(in, out, done_flag, false, w1, w2, part);
-      prof_end(kPointApply64, point_apply_bytes(8));
+      prof_end(kPointApply64, point_apply_bytes(8, 1));
       if (!np) k::dot_parts(n, out, w1, w2, part, nred, sc, stream_);
       finalise(np ? np : nred, stage);
       return;
@@ -1427,7 +1427,7 @@ int Model::bicgstab32(const double* b, float* x, double rtol, int maxit, int* it
     if (use_points_) {
       prof_begin(kPointApply32);
       const int np = launch_points_apply(in, out, done_flag, true, w1, w2, part_.get());
-      prof_end(kPointApply32, point_apply_bytes(4));
+      prof_end(kPointApply32, point_apply_bytes(4, 1));
       if (!np) k::dot_parts(n, out, w1, w2, part_.get(), nred, sc_.get(), stream_);
       finalise(np ? np : nred, stage);
       return;
@@ -1824,11 +1824,13 @@ double Model::apply_bytes() const {
 }
 
 // Algorithmic bytes of one point-coefficient apply: the coefficients of every
-// cell of the compartments once, v read and y written once, the cell class
-// masks and vertex -> slot maps the tiles read.
-double Model::point_apply_bytes(int value_bytes) const {
+// cell of the compartments once, v read and y written once, extra_vectors
+// more vectors read (the BiCGSTAB applies read one dot operand; their other
+// operand is absent or y itself), the cell class masks and vertex -> slot
+// maps the tiles read.
+double Model::point_apply_bytes(int value_bytes, int extra_vectors) const {
   const double N = static_cast<double>(L_.nslot), B = L_.B;
-  double b = 2.0 * value_bytes * B * N;
+  double b = (2.0 + extra_vectors) * value_bytes * B * N;
   for (size_t c = 0; c < L_.comps.size(); ++c) {
     if (!gen_.point_words[c]) continue;
     const double cells = static_cast<double>(L_.comps[c].cells.size());

@@ -277,7 +277,7 @@ class Model {
 
  public:
   // algorithmic bytes of one point-coefficient apply / set-up
-  double point_apply_bytes(int value_bytes) const;
+  double point_apply_bytes(int value_bytes, int extra_vectors = 0) const;
   double point_setup_bytes() const;
   // the Krylov operator is the point-coefficient one
   bool point_storage() const { return cfg_.points; }

@@ -448,3 +448,28 @@ def test_gmsh_errors(dca, gmsh_files, tmp_path):
                     "$Elements\n1\n1 3 2 1 1 1 2 3 4\n$EndElements\n")
     with pytest.raises(dca.DcgError, match="simplices only"):
         dca.Model(unstructured_models.gray_scott(quad), "", -1)
+
+
+def test_committed_traffic_covers_the_apply_variants():
+    """The committed counter summary of the 96^3 FP64 workload gives each
+    measured point-apply variant its calibrated HBM bytes per launch; the
+    figure is at least the algorithmic bytes of one apply (every cell's 8
+    points x 3 coefficients and class mask, then the direction, the result
+    and one dot operand over 2 x 97^3 values, 8 bytes each, and a 4-byte
+    slot per vertex) and not far above them."""
+    import importlib.util as iu
+    spec = iu.spec_from_file_location("bench_for_test", str(__import__("pathlib").Path(__file__).parents[1] / "bench.py"))
+    bench = iu.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    algorithmic = 96 ** 3 * (8.0 * 24 + 1) + 3 * 8.0 * 2 * 97 ** 3 + 4.0 * 97 ** 3
+    assert round(algorithmic) == 218213044
+    seen = 0
+    for k in range(9):
+        t = bench.committed_traffic("ms96", "fp64", f"dcg_tile_0_p{k}")
+        if t is None:
+            continue
+        seen += 1
+        assert algorithmic <= t <= 1.5 * algorithmic, (k, t, algorithmic)
+    assert seen >= 2
+    assert bench.committed_traffic("ms96", "fp64", "dcg_tile_0_pf4") is None
+    assert bench.committed_traffic("gs128", "fp64", "dcg_tile_0_p4") is None
