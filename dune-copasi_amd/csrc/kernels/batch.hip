@@ -246,6 +246,8 @@ __global__ void __launch_bounds__(kThreads) binit_k(const int2* __restrict__ wor
 // The BiCGSTAB updates go over vwork4 items (member, block of kBVecSlots *
 // kThreads slots): each thread takes kBVecSlots slots kThreads apart, so
 // every pass of the workgroup is contiguous and more bytes are in flight.
+// (Batch::rebuild sizes both work lists in blocks of 256 slots.)
+static_assert(kThreads == 256, "batch work lists are built for 256-thread workgroups");
 
 // p = r + beta (p - omega v); ph = M^-1 p
 template <int B, typename V>
